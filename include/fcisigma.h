@@ -1,0 +1,162 @@
+/*
+ * fcisigma -- MI355X (gfx950) direct-CI sigma engine, C ABI.
+ *
+ * Plain C types only.  Device arrays are HIP device pointers; `stream` is a
+ * hipStream_t passed as void* (NULL = default stream).  Every entry point
+ * returns an int status:
+ *     FCIS_SUCCESS 0, FCIS_ERR_INVALID -1, FCIS_ERR_MEMORY -2,
+ *     FCIS_ERR_DEVICE -3
+ * mirroring gas_status_t of the reference (pyscf/lib/mcscf/fci_gas.h:38-42)
+ * plus a device-error code; fcis_last_error() describes the last failure.
+ *
+ * CI vectors are (nstr_alpha, nstr_beta) row-major float64 on the device,
+ * the layout of pyscf.fci.direct_spin1 CAS vectors.  Integrals are row-major
+ * host arrays: h1e (norb,norb), two-electron tensors as (npair,npair) 4-fold
+ * pair matrices, npair = norb(norb+1)/2, pair index max*(max+1)/2+min
+ * (fci_gas.h:171-176 gas_link_pair_index).
+ */
+#ifndef FCISIGMA_H
+#define FCISIGMA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+        FCIS_SUCCESS = 0,
+        FCIS_ERR_INVALID = -1,
+        FCIS_ERR_MEMORY = -2,
+        FCIS_ERR_DEVICE = -3
+};
+
+/* Arithmetic of the t1 contraction (sigma plan). */
+enum {
+        FCIS_PREC_FP64 = 0,     /* FP64 MFMA                                  */
+        FCIS_PREC_FP32 = 1,     /* FP32 MFMA, FP32 t1/vt1, FP64 gather/scatter */
+        FCIS_PREC_BF16X3 = 2    /* 3-term bf16 split on MFMA, FP32 accumulate  */
+};
+
+typedef struct fcis_space fcis_space_t;
+typedef struct fcis_sigma_plan fcis_sigma_plan_t;
+
+/* Text of the last error on the calling thread. */
+const char *fcis_last_error(void);
+
+/* Library / device version info: writes "gfx950 ..." into buf. */
+int fcis_device_info(char *buf, int buflen);
+
+/*
+ * CAS string space (norb, na, nb): alpha/beta string addressing and link
+ * tables, built ON THE DEVICE.  Replaces the string space and link-table
+ * construction that pyscf.fci.direct_spin1 obtains from
+ * cistring.gen_linkstr_index (called for the reference's solvers through
+ * direct_spin1.contract_2e, pyscf/csf_fci/csf.py:449) and that libfci_gas
+ * builds in gas_space_from_blocks / fill_link_table
+ * (pyscf/lib/mcscf/fci_string_gas.c:1683, :1304-1427; ABI fci_gas.h:220).
+ * norb <= 32, 0 <= na, nb <= norb.
+ */
+int fcis_space_create(fcis_space_t **out, int norb, int na, int nb, void *stream);
+void fcis_space_free(fcis_space_t *space);
+int fcis_space_info(const fcis_space_t *space, int64_t *nstr_alpha,
+                    int64_t *nstr_beta, int *nlink_alpha, int *nlink_beta);
+
+/*
+ * Export a link table in pyscf's cistring.gen_linkstr_index layout:
+ * int32 [nstr, nlink, 4] rows (cre, des, addr, sign), or with trilidx != 0
+ * (pair, 0, addr, sign) as gen_linkstr_index_trilidx.  Row order: the
+ * diagonal (i,i) entries over occupied i ascending, then occupied i
+ * ascending x virtual a ascending.  spin 0 = alpha, 1 = beta.  d_out is a
+ * device array.  (Replaces cistring.gen_linkstr_index; libfci_gas analogue
+ * fill_link_table, fci_string_gas.c:1304.)
+ */
+int fcis_gen_linkstr_index(const fcis_space_t *space, int spin, int trilidx,
+                           int32_t *d_out, void *stream);
+
+/* String <-> address (colex rank, cistring.str2addr/addr2str;
+ * fci_string_gas.c:263-298).  Host-side helpers. */
+int64_t fcis_str2addr(int norb, int nelec, uint64_t string);
+uint64_t fcis_addr2str(int norb, int nelec, int64_t addr);
+
+/*
+ * absorb_h1e: h2e = fac * [eri + delta_kl f_ij + delta_ij f_kl],
+ * f = (h1e - 1/2 sum_j (ij|jk)) / (nelec + 1e-100).  Host arrays.
+ * Replaces direct_spin1.absorb_h1e as wrapped at
+ * pyscf/mcscf/fci_gas.py:132-138 (called by the reference at
+ * fci_gas.py:1166 and csf.py:445 with fac = 0.5).
+ */
+int fcis_absorb_h1e(int norb, int nelec, double fac, const double *h1e,
+                    const double *eri, double *h2e_out);
+
+/*
+ * Sigma plan: sigma = H c for an absorbed pair matrix h2e (host array,
+ * copied).  Replaces fci_contract_gas_plan_create / _execute / _free
+ * (fci_gas.h:242-249; fci_contract_gas.c:2388-2454) and the
+ * direct_spin1.contract_2e call at csf.py:449 / fci_gas.py:919-939.
+ * workspace_bytes bounds the t1/vt1 device workspace (0 = library default);
+ * the plan tiles alpha strings so that the workspace fits.
+ * precision: FCIS_PREC_*.  The plan must not be executed concurrently.
+ */
+int fcis_sigma_plan_create(fcis_sigma_plan_t **out, const fcis_space_t *space,
+                           const double *h2e, int precision,
+                           uint64_t workspace_bytes);
+/* d_ci0 and d_ci1 are device (nstr_a, nstr_b) float64 arrays, must differ
+ * (the reference rejects aliasing, fci_contract_gas.c:2426).  d_ci1 is fully
+ * overwritten. */
+int fcis_sigma_plan_execute(fcis_sigma_plan_t *plan, const double *d_ci0,
+                            double *d_ci1, void *stream);
+void fcis_sigma_plan_free(fcis_sigma_plan_t *plan);
+/* Tiles per execute, alpha strings per tile, workspace bytes. */
+int fcis_sigma_plan_info(const fcis_sigma_plan_t *plan, int *ntiles,
+                         int64_t *tile_alpha, uint64_t *workspace_bytes);
+/* Replace the Hamiltonian of an existing plan (same space). */
+int fcis_sigma_plan_set_h2e(fcis_sigma_plan_t *plan, const double *h2e);
+
+/*
+ * Hamiltonian diagonal from the un-absorbed integrals (host h1e, eri pair
+ * matrix) into device d_hdiag (nstr_a*nstr_b).  Replaces fci_make_hdiag_gas
+ * (fci_gas.h:205; fci_hdiag_gas.c:678) / direct_spin1.make_hdiag.
+ */
+int fcis_make_hdiag(const fcis_space_t *space, const double *h1e,
+                    const double *eri, double *d_hdiag, void *stream);
+
+/*
+ * Spin-resolved transition 1- and 2-RDMs (pyscf conventions:
+ * dm1[p,q] = <bra|q^+ p|ket>, dm2[p,q,r,s] = <bra|p^+ r^+ s q|ket>,
+ * dm2ab alpha on (p,q), beta on (r,s)).  Device bra/ket, HOST outputs
+ * (norb^2 and norb^4 float64).  Replaces fci_rdm_gas_plan_make_rdm12s
+ * (fci_gas.h:287; fci_rdm_gas.c:2207) / direct_spin1.trans_rdm12s.
+ * Any output pointer may be NULL to skip it (dm2 NULLs skip the 2-RDM).
+ */
+int fcis_trans_rdm12s(const fcis_space_t *space, const double *d_bra,
+                      const double *d_ket, double *dm1a, double *dm1b,
+                      double *dm2aa, double *dm2ab, double *dm2bb,
+                      void *stream);
+
+/* ---- Davidson primitives on device vectors (pyscf.lib.davidson1 work,
+ *      called at fci_gas.py:1344-1350; vectors stay in HBM) ----------- */
+
+/* out[i] = <X_i, v>, i < m; X rows of length n with row stride ldx.
+ * Deterministic two-stage reduction.  out is a HOST array. */
+int fcis_dots(int m, int64_t n, const double *d_x, int64_t ldx,
+              const double *d_v, double *out, void *stream);
+/* y = beta*y + sum_i coef[i] X_i  (coef: host array of m <= 64 values). */
+int fcis_lincomb(int m, int64_t n, const double *d_x, int64_t ldx,
+                 const double *coef, double beta, double *d_y, void *stream);
+/* out = r / (hdiag - e), |hdiag - e| < level_shift -> level_shift
+ * (precond, fci_gas.py:1337-1342).  out may alias r. */
+int fcis_precond(int64_t n, const double *d_hdiag, double e,
+                 double level_shift, const double *d_r, double *d_out,
+                 void *stream);
+/* Initial guess of root `root`: unit vector at `addr` plus the reproducible
+ * splitmix64 sign noise of total norm noise_norm over all n entries
+ * (get_init_guess / _add_distributed_noise, fci_gas.py:61-88, :881-902). */
+int fcis_init_guess(int64_t n, int64_t addr, int root, double noise_norm,
+                    double *d_out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FCISIGMA_H */

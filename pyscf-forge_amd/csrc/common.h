@@ -1,0 +1,87 @@
+// Internal definitions shared by the fcisigma HIP sources (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "../../include/fcisigma.h"
+
+#define FCIS_MAXORB 32
+
+namespace fcis {
+
+void set_error(const char *fmt, ...);
+
+// Check a HIP call; on failure record the error and return FCIS_ERR_DEVICE.
+#define FCIS_HIP(call)                                                        \
+        do {                                                                  \
+                hipError_t e_ = (call);                                       \
+                if (e_ != hipSuccess) {                                       \
+                        ::fcis::set_error("%s:%d %s: %s", __FILE__, __LINE__, \
+                                          #call, hipGetErrorString(e_));      \
+                        return FCIS_ERR_DEVICE;                               \
+                }                                                             \
+        } while (0)
+
+#define FCIS_CHECK_LAUNCH()                                                   \
+        do {                                                                  \
+                hipError_t e_ = hipGetLastError();                            \
+                if (e_ != hipSuccess) {                                       \
+                        ::fcis::set_error("%s:%d launch: %s", __FILE__,       \
+                                          __LINE__, hipGetErrorString(e_));   \
+                        return FCIS_ERR_DEVICE;                               \
+                }                                                             \
+        } while (0)
+
+static inline int64_t binom_host(int n, int k)
+{
+        if (k < 0 || k > n) return 0;
+        if (k > n - k) k = n - k;
+        int64_t r = 1;
+        for (int i = 1; i <= k; i++) r = r * (n - k + i) / i;
+        return r;
+}
+
+__host__ __device__ static inline int pair_index(int p, int q)
+{
+        int hi = p > q ? p : q, lo = p > q ? q : p;
+        return hi * (hi + 1) / 2 + lo;
+}
+
+}  // namespace fcis
+
+// A compact link: target string address and a packed descriptor
+//   info = neg | pair << 1 | des << 12 | cre << 18
+// for a_cre^+ a_des |str> = (-1)^neg |addr>, pair = pair_index(cre, des).
+// Stored transposed, link[k * nstr + str], so that a wavefront reading the
+// k-th link of 64 consecutive strings issues one coalesced load.
+struct fcis_link {
+        int32_t addr;
+        int32_t info;
+};
+
+__host__ __device__ static inline int link_neg(int32_t info) { return info & 1; }
+__host__ __device__ static inline int link_pair(int32_t info) { return (info >> 1) & 0x7ff; }
+__host__ __device__ static inline int link_des(int32_t info) { return (info >> 12) & 0x3f; }
+__host__ __device__ static inline int link_cre(int32_t info) { return (info >> 18) & 0x3f; }
+__host__ __device__ static inline int32_t link_pack(int cre, int des, int pair, int neg)
+{
+        return neg | (pair << 1) | (des << 12) | (cre << 18);
+}
+
+struct fcis_space {
+        int norb, na, nb;
+        int npair;
+        int64_t nstra, nstrb;
+        int nlinka, nlinkb;
+        fcis_link *d_linka;     // [nlinka][nstra]
+        fcis_link *d_linkb;     // [nlinkb][nstrb]
+        uint64_t *d_stra;       // strings in address order
+        uint64_t *d_strb;
+        int device;
+};
+
+namespace fcis {
+int build_links(fcis_space *s, hipStream_t stream);
+}

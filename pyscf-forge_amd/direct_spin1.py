@@ -1,0 +1,502 @@
+"""Drop-in direct-CI solver on MI355X with pyscf's ``direct_spin1`` API.
+
+Module functions and ``FCISolver`` keep the signatures of
+pyscf.fci.direct_spin1 as the reference's solvers call them (SURVEYed
+boundary: ``absorb_h1e(h1e, eri, norb, nelec, fac=1)`` csf.py:518,
+``contract_2e(eri, fcivec, norb, nelec, link_index=None)`` fci_gas.py:919 /
+csf.py:525, ``make_hdiag`` fci_gas.py:904, ``kernel`` fci_gas.py:1150,
+``make_rdm1(s)/make_rdm12(s)/trans_rdm12(s)`` fci_gas.py:1401-1483,
+``energy`` fci_gas.py:946).  CAS CI vectors are (nstr_alpha, nstr_beta)
+float64 arrays.  Host numpy inputs are returned as numpy; device torch
+tensors stay on the device.
+
+Every call runs the HIP library (libfcisigma.so); there is no CPU path.
+"""
+
+import math
+
+import numpy
+
+from . import _lib
+from . import davidson as _davidson
+
+PSPACE_MATVEC_MAX = 64   # exact small-space diagonalisation bound (fci_gas.py:38)
+INIT_NOISE_NORM = 1e-3   # fci_gas.py:40
+
+
+def _unpack_nelec(nelec, spin=None):
+    if isinstance(nelec, (int, numpy.integer)):
+        if spin is None:
+            nb = int(nelec) // 2
+        else:
+            nb = (int(nelec) - int(spin)) // 2
+        na = int(nelec) - nb
+        return na, nb
+    return int(nelec[0]), int(nelec[1])
+
+
+def pair_index(p, q):
+    hi, lo = max(p, q), min(p, q)
+    return hi * (hi + 1) // 2 + lo
+
+
+def pair_matrix(eri, norb):
+    """(npair, npair) float64 4-fold pair matrix from a pair matrix, a
+    4-index tensor or an 8-fold packed vector (the layouts
+    _as_pair_matrix accepts, pyscf/mcscf/fci_gas.py:101-124)."""
+    norb = int(norb)
+    npair = norb * (norb + 1) // 2
+    arr = numpy.asarray(eri)
+    if numpy.iscomplexobj(arr):
+        raise TypeError("the sigma kernels require real-valued integrals")
+    arr = numpy.asarray(arr, dtype=numpy.float64)
+    if arr.shape == (npair, npair):
+        return numpy.ascontiguousarray(arr)
+    if arr.shape == (norb,) * 4:
+        i, j = numpy.tril_indices(norb)
+        return numpy.ascontiguousarray(arr[i, j][:, i, j])
+    if arr.ndim == 1 and arr.size == npair * (npair + 1) // 2:
+        out = numpy.empty((npair, npair))
+        iu = numpy.tril_indices(npair)
+        out[iu] = arr
+        out.T[iu] = arr
+        return out
+    raise ValueError("could not convert ERI of shape %s to pair layout" % (arr.shape,))
+
+
+def absorb_h1e(h1e, eri, norb, nelec, fac=1):
+    """Absorbed two-electron Hamiltonian (pair layout); the reference's
+    solvers call it with fac=0.5 (fci_gas.py:1166, csf.py:445)."""
+    lib = _lib.load()
+    norb = int(norb)
+    ne = sum(_unpack_nelec(nelec))
+    h1e = numpy.ascontiguousarray(h1e, dtype=numpy.float64).reshape(norb, norb)
+    eri = pair_matrix(eri, norb)
+    out = numpy.empty_like(eri)
+    _lib.check(lib.fcis_absorb_h1e(norb, ne, float(fac), _lib.dptr(h1e),
+                                   _lib.dptr(eri), _lib.dptr(out)), "fcis_absorb_h1e")
+    return out
+
+
+# ---------------------------------------------------------------- spaces
+
+class Space:
+    """Device string space (link tables built on the GPU)."""
+
+    def __init__(self, norb, nelec):
+        import ctypes
+        self.lib = _lib.load()
+        self.norb = int(norb)
+        self.nelec = _unpack_nelec(nelec)
+        h = ctypes.c_void_p()
+        _lib.check(self.lib.fcis_space_create(ctypes.byref(h), self.norb, self.nelec[0],
+                                              self.nelec[1], _lib.stream_ptr()),
+                   "fcis_space_create")
+        self.handle = h
+        na, nb = ctypes.c_int64(), ctypes.c_int64()
+        la, lb = ctypes.c_int(), ctypes.c_int()
+        _lib.check(self.lib.fcis_space_info(h, ctypes.byref(na), ctypes.byref(nb),
+                                            ctypes.byref(la), ctypes.byref(lb)),
+                   "fcis_space_info")
+        self.nstra, self.nstrb = na.value, nb.value
+        self.nlinka, self.nlinkb = la.value, lb.value
+        self.ndet = self.nstra * self.nstrb
+        self.shape = (self.nstra, self.nstrb)
+
+    def link_index(self, spin=0, tril=False):
+        """pyscf gen_linkstr_index layout [nstr, nlink, 4] (device tensor)."""
+        import torch
+        nstr = self.nstra if spin == 0 else self.nstrb
+        nlink = self.nlinka if spin == 0 else self.nlinkb
+        out = torch.zeros((nstr, nlink, 4), dtype=torch.int32, device="cuda")
+        _lib.check(self.lib.fcis_gen_linkstr_index(self.handle, int(spin), 1 if tril else 0,
+                                                   _lib.vptr(out), _lib.stream_ptr()),
+                   "fcis_gen_linkstr_index")
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None) is not None:
+            self.lib.fcis_space_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_SPACES = {}
+
+
+def get_space(norb, nelec):
+    key = (int(norb),) + _unpack_nelec(nelec)
+    sp = _SPACES.get(key)
+    if sp is None:
+        sp = _SPACES[key] = Space(norb, key[1:])
+    return sp
+
+
+def gen_linkstr_index(orb_list, nocc, tril=False):
+    """cistring.gen_linkstr_index for range(norb), built on the device,
+    returned as a host int32 array [nstr, nlink, 4]."""
+    norb = len(orb_list) if hasattr(orb_list, "__len__") else int(orb_list)
+    sp = get_space(norb, (nocc, nocc))
+    return sp.link_index(0, tril).cpu().numpy()
+
+
+class SigmaPlan:
+    """Reusable sigma plan (fcis_sigma_plan_*), the analogue of the
+    reference's GasContractPlan (pyscf/mcscf/fci_gas.py:570-650)."""
+
+    def __init__(self, space, h2e, precision="fp64", workspace_bytes=0):
+        import ctypes
+        self.lib = _lib.load()
+        self.space = space
+        prec = _lib.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        self.precision = precision
+        self.h2e = pair_matrix(h2e, space.norb)
+        h = ctypes.c_void_p()
+        _lib.check(self.lib.fcis_sigma_plan_create(ctypes.byref(h), space.handle,
+                                                   _lib.dptr(self.h2e), prec,
+                                                   int(workspace_bytes)),
+                   "fcis_sigma_plan_create")
+        self.handle = h
+        nt, ta, ws = ctypes.c_int(), ctypes.c_int64(), ctypes.c_uint64()
+        self.lib.fcis_sigma_plan_info(h, ctypes.byref(nt), ctypes.byref(ta), ctypes.byref(ws))
+        self.ntiles, self.tile_alpha, self.workspace_bytes = nt.value, ta.value, ws.value
+
+    def __call__(self, c, out):
+        """out <- H c (device float64 tensors with space.ndet elements)."""
+        if c.data_ptr() == out.data_ptr():
+            raise ValueError("sigma input and output must not alias")
+        _lib.check(self.lib.fcis_sigma_plan_execute(self.handle, _lib.vptr(c), _lib.vptr(out),
+                                                    _lib.stream_ptr()),
+                   "fcis_sigma_plan_execute")
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None) is not None:
+            self.lib.fcis_sigma_plan_free(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _to_device(x, ndet):
+    import torch
+    if isinstance(x, torch.Tensor):
+        t = x.detach().to(device="cuda", dtype=torch.float64).contiguous().reshape(-1)
+        host = False
+    else:
+        a = numpy.ascontiguousarray(x, dtype=numpy.float64).reshape(-1)
+        t = torch.from_numpy(a).to("cuda")
+        host = True
+    if t.numel() != ndet:
+        raise ValueError("CI vector size %d does not match determinant count %d"
+                         % (t.numel(), ndet))
+    return t, host
+
+
+def _default_workspace():
+    import torch
+    free, _ = torch.cuda.mem_get_info()
+    return int(free * 0.45)
+
+
+def contract_2e(eri, fcivec, norb, nelec, link_index=None, precision="fp64", **kwargs):
+    """sigma = H c for the absorbed pair matrix ``eri`` (direct_spin1.contract_2e)."""
+    sp = get_space(norb, nelec)
+    c, host = _to_device(fcivec, sp.ndet)
+    import torch
+    out = torch.empty_like(c)
+    with SigmaPlan(sp, eri, precision, kwargs.get("workspace_bytes", 0) or _default_workspace()) as plan:
+        plan(c, out)
+    shape = numpy.shape(fcivec) if host else tuple(fcivec.shape)
+    if host:
+        return out.cpu().numpy().reshape(shape)
+    return out.reshape(shape)
+
+
+def make_hdiag(h1e, eri, norb, nelec, compress=False):
+    """Hamiltonian diagonal (un-absorbed integrals), device tensor if
+    ``device=True`` else numpy (direct_spin1.make_hdiag)."""
+    return _make_hdiag_dev(h1e, eri, norb, nelec).cpu().numpy()
+
+
+def _make_hdiag_dev(h1e, eri, norb, nelec):
+    import torch
+    sp = get_space(norb, nelec)
+    h1e = numpy.ascontiguousarray(h1e, dtype=numpy.float64).reshape(sp.norb, sp.norb)
+    eri = pair_matrix(eri, sp.norb)
+    out = torch.empty(sp.ndet, dtype=torch.float64, device="cuda")
+    _lib.check(sp.lib.fcis_make_hdiag(sp.handle, _lib.dptr(h1e), _lib.dptr(eri),
+                                      _lib.vptr(out), _lib.stream_ptr()), "fcis_make_hdiag")
+    return out
+
+
+def trans_rdm12s(cibra, ciket, norb, nelec, link_index=None, reorder=True):
+    """((dm1a, dm1b), (dm2aa, dm2ab, dm2ba, dm2bb)) as direct_spin1.trans_rdm12s."""
+    (dm1a, dm1b), (dm2aa, dm2ab, dm2bb) = _rdm12s(cibra, ciket, norb, nelec)
+    _, (_, dm2ba_ji, _) = _rdm12s(ciket, cibra, norb, nelec)
+    return (dm1a, dm1b), (dm2aa, dm2ab, dm2ba_ji.transpose(3, 2, 1, 0), dm2bb)
+
+
+def _rdm12s(cibra, ciket, norb, nelec, with_dm2=True):
+    sp = get_space(norb, nelec)
+    bra, _ = _to_device(cibra, sp.ndet)
+    ket, _ = _to_device(ciket, sp.ndet)
+    n = sp.norb
+    d1a, d1b = numpy.empty((n, n)), numpy.empty((n, n))
+    if with_dm2:
+        d2 = [numpy.empty((n,) * 4) for _ in range(3)]
+        ptrs = [_lib.dptr(x) for x in d2]
+    else:
+        d2 = [None] * 3
+        ptrs = [None] * 3
+    _lib.check(sp.lib.fcis_trans_rdm12s(sp.handle, _lib.vptr(bra), _lib.vptr(ket),
+                                        _lib.dptr(d1a), _lib.dptr(d1b), *ptrs,
+                                        _lib.stream_ptr()), "fcis_trans_rdm12s")
+    return (d1a, d1b), tuple(d2)
+
+
+def make_rdm12s(fcivec, norb, nelec, link_index=None, reorder=True):
+    """((dm1a, dm1b), (dm2aa, dm2ab, dm2bb)) (direct_spin1.make_rdm12s)."""
+    return _rdm12s(fcivec, fcivec, norb, nelec)
+
+
+def make_rdm12(fcivec, norb, nelec, link_index=None, reorder=True):
+    """Spin-traced (dm1, dm2), dm2 = aa + bb + ab + ab^T(2,3,0,1)
+    (assembly of pyscf/mcscf/fci_gas.py:724-733)."""
+    (dm1a, dm1b), (dm2aa, dm2ab, dm2bb) = make_rdm12s(fcivec, norb, nelec)
+    dm2 = dm2aa + dm2bb + dm2ab + dm2ab.transpose(2, 3, 0, 1)
+    return dm1a + dm1b, dm2
+
+
+def trans_rdm12(cibra, ciket, norb, nelec, link_index=None, reorder=True):
+    (dm1a, dm1b), (dm2aa, dm2ab, dm2bb) = _rdm12s(cibra, ciket, norb, nelec)
+    _, (_, dm2ba_ji, _) = _rdm12s(ciket, cibra, norb, nelec)
+    dm2ba = dm2ba_ji.transpose(3, 2, 1, 0)
+    return dm1a + dm1b, dm2aa + dm2ab + dm2ba + dm2bb
+
+
+def make_rdm1s(fcivec, norb, nelec, link_index=None):
+    (dm1a, dm1b), _ = _rdm12s(fcivec, fcivec, norb, nelec, with_dm2=False)
+    return dm1a, dm1b
+
+
+def make_rdm1(fcivec, norb, nelec, link_index=None):
+    dm1a, dm1b = make_rdm1s(fcivec, norb, nelec)
+    return dm1a + dm1b
+
+
+def trans_rdm1s(cibra, ciket, norb, nelec, link_index=None):
+    (dm1a, dm1b), _ = _rdm12s(cibra, ciket, norb, nelec, with_dm2=False)
+    return dm1a, dm1b
+
+
+def trans_rdm1(cibra, ciket, norb, nelec, link_index=None):
+    dm1a, dm1b = trans_rdm1s(cibra, ciket, norb, nelec)
+    return dm1a + dm1b
+
+
+def energy(h1e, eri, fcivec, norb, nelec, link_index=None):
+    """<c|H|c> for a normalised c (direct_spin1.energy / fci_gas.py:946-952)."""
+    import torch
+    sp = get_space(norb, nelec)
+    h2e = absorb_h1e(h1e, eri, norb, nelec, 0.5)
+    c, _ = _to_device(fcivec, sp.ndet)
+    out = torch.empty_like(c)
+    with SigmaPlan(sp, h2e, "fp64", _default_workspace()) as plan:
+        plan(c, out)
+    return float(torch.dot(c, out))
+
+
+def spin_square_from_rdm12s(dm1s, dm2s, nelec):
+    """<S^2> from spin-resolved RDMs (pyscf/mcscf/fci_gas.py:1486-1499)."""
+    na, nb = _unpack_nelec(nelec)
+    dm2ab = dm2s[1]
+    sz = 0.5 * (na - nb)
+    ss = sz * (sz + 1.0) + nb - float(numpy.einsum("pqqp->", dm2ab))
+    return ss, max(0.0, 4.0 * ss + 1.0) ** 0.5
+
+
+# ---------------------------------------------------------------- solver
+
+class FCISolver:
+    """CAS FCI solver with pyscf's fcisolver protocol (kernel, contract_2e,
+    absorb_h1e, make_hdiag, make_rdm1(s)/make_rdm12(s)/trans_rdm*, energy,
+    spin_square), running every O(N_det) step on the GPU.
+
+    Extra attribute ``precision`` selects the sigma arithmetic used during
+    the Davidson iterations ("fp64", "fp32", "bf16x3"); non-FP64 solves end
+    with FP64 sigma refinement (see ``mixed``)."""
+
+    def __init__(self, mol=None):
+        self.mol = mol
+        self.verbose = 0
+        self.max_memory = 4000
+        self.nroots = 1
+        self.spin = None
+        self.conv_tol = 1e-10
+        self.conv_tol_residual = None
+        self.max_cycle = 50
+        self.max_space = 12
+        self.lindep = 1e-14
+        self.level_shift = 1e-3
+        self.davidson_only = False
+        self.pspace_size = 400
+        self.precision = "fp64"
+        self.workspace_bytes = 0
+        self.converged = False
+        self.eci = None
+        self.ci = None
+        self.norb = None
+        self.nelec = None
+        self.stats = {}
+
+    # -- pyscf protocol -------------------------------------------------
+    def absorb_h1e(self, h1e, eri, norb, nelec, fac=1):
+        return absorb_h1e(h1e, eri, norb, nelec, fac)
+
+    def contract_2e(self, eri, fcivec, norb, nelec, link_index=None, **kwargs):
+        return contract_2e(eri, fcivec, norb, nelec, link_index,
+                           precision=kwargs.pop("precision", "fp64"))
+
+    def make_hdiag(self, h1e, eri, norb, nelec, *args, **kwargs):
+        return make_hdiag(h1e, eri, norb, nelec)
+
+    def energy(self, h1e, eri, fcivec, norb, nelec, link_index=None):
+        return energy(h1e, eri, fcivec, norb, nelec)
+
+    def make_rdm1s(self, ci, norb, nelec, link_index=None):
+        return make_rdm1s(ci, norb, self._nelec(nelec))
+
+    def make_rdm1(self, ci, norb, nelec, link_index=None):
+        return make_rdm1(ci, norb, self._nelec(nelec))
+
+    def make_rdm12s(self, ci, norb, nelec, link_index=None, reorder=True):
+        return make_rdm12s(ci, norb, self._nelec(nelec))
+
+    def make_rdm12(self, ci, norb, nelec, link_index=None, reorder=True):
+        return make_rdm12(ci, norb, self._nelec(nelec))
+
+    def trans_rdm1s(self, cibra, ciket, norb, nelec, link_index=None):
+        return trans_rdm1s(cibra, ciket, norb, self._nelec(nelec))
+
+    def trans_rdm1(self, cibra, ciket, norb, nelec, link_index=None):
+        return trans_rdm1(cibra, ciket, norb, self._nelec(nelec))
+
+    def trans_rdm12s(self, cibra, ciket, norb, nelec, link_index=None, reorder=True):
+        return trans_rdm12s(cibra, ciket, norb, self._nelec(nelec))
+
+    def trans_rdm12(self, cibra, ciket, norb, nelec, link_index=None, reorder=True):
+        return trans_rdm12(cibra, ciket, norb, self._nelec(nelec))
+
+    def spin_square(self, ci, norb, nelec, *args, **kwargs):
+        nelec = self._nelec(nelec)
+        dm1s, dm2s = make_rdm12s(ci, norb, nelec)
+        return spin_square_from_rdm12s(dm1s, dm2s, nelec)
+
+    def _nelec(self, nelec):
+        return _unpack_nelec(nelec, self.spin)
+
+    def get_init_guess(self, norb, nelec, nroots, hdiag):
+        """Lowest-diagonal unit vectors plus reproducible global noise
+        (fci_gas.py:881-902), as host arrays."""
+        return [g.cpu().numpy() for g in self._init_guess_dev(hdiag, nroots)]
+
+    def _init_guess_dev(self, hdiag, nroots):
+        import torch
+        hd = hdiag if isinstance(hdiag, torch.Tensor) else torch.as_tensor(
+            numpy.asarray(hdiag, dtype=numpy.float64)).cuda()
+        n = hd.numel()
+        k = min(nroots, n)
+        addrs = torch.topk(hd, k, largest=False).indices.cpu().numpy()
+        lib = _lib.load()
+        out = []
+        for root, a in enumerate(addrs):
+            g = torch.empty(n, dtype=torch.float64, device=hd.device)
+            _lib.check(lib.fcis_init_guess(n, int(a), root, INIT_NOISE_NORM, _lib.vptr(g),
+                                           _lib.stream_ptr()), "fcis_init_guess")
+            out.append(g)
+        return out
+
+    def kernel(self, h1e, eri, norb, nelec, ci0=None, ecore=0, nroots=None, **kwargs):
+        """Davidson solve; returns (e + ecore, ci) like direct_spin1.kernel."""
+        import torch
+        nroots = int(nroots if nroots is not None else kwargs.pop("nroots", self.nroots))
+        if nroots < 1:
+            raise ValueError("nroots must be positive")
+
+        def option(name, default):
+            v = kwargs.pop(name, None)
+            return default if v is None else v
+
+        tol = option("tol", self.conv_tol)
+        tol_residual = option("tol_residual", self.conv_tol_residual)
+        max_cycle = option("max_cycle", self.max_cycle)
+        max_space = option("max_space", self.max_space)
+        lindep = option("lindep", self.lindep)
+        level_shift = option("level_shift", self.level_shift)
+        davidson_only = bool(option("davidson_only", self.davidson_only))
+        precision = option("precision", self.precision)
+        verbose = option("verbose", self.verbose)
+        nelec = self._nelec(nelec)
+        self.norb, self.nelec = int(norb), nelec
+        sp = get_space(norb, nelec)
+        if nroots > sp.ndet:
+            raise ValueError("nroots exceeds the determinant count")
+        h1e = numpy.ascontiguousarray(h1e, dtype=numpy.float64).reshape(sp.norb, sp.norb)
+        eri = pair_matrix(eri, sp.norb)
+        h2e = absorb_h1e(h1e, eri, norb, nelec, 0.5)
+        hdiag = _make_hdiag_dev(h1e, eri, norb, nelec)
+
+        if (ci0 is None and not davidson_only and sp.ndet <= min(PSPACE_MATVEC_MAX,
+                                                                 self.pspace_size)):
+            e, c = self._exact_small(sp, h2e, nroots)
+            conv = numpy.ones(nroots, dtype=bool)
+        else:
+            from .mixed import solve
+            e, c, conv, stats = solve(self, sp, h2e, hdiag, nroots, ci0, precision,
+                                      tol=tol, tol_residual=tol_residual,
+                                      max_cycle=max_cycle, max_space=max_space,
+                                      lindep=lindep, level_shift=level_shift,
+                                      verbose=verbose)
+            self.stats = stats
+        e = numpy.asarray(e, dtype=numpy.float64).reshape(-1) + float(ecore)
+        cis = [x.reshape(sp.shape) for x in c]
+        if nroots == 1:
+            self.converged = bool(conv[0])
+            self.eci = float(e[0])
+            self.ci = cis[0].cpu().numpy()
+        else:
+            self.converged = numpy.asarray(conv, dtype=bool)
+            self.eci = e
+            self.ci = [x.cpu().numpy() for x in cis]
+        return self.eci, self.ci
+
+    def _exact_small(self, sp, h2e, nroots):
+        """H from sigma on unit vectors, then eigh (fci_gas.py:1226-1257)."""
+        import torch
+        n = sp.ndet
+        eye = torch.eye(n, dtype=torch.float64, device="cuda")
+        H = torch.empty_like(eye)
+        with SigmaPlan(sp, h2e, "fp64", _default_workspace()) as plan:
+            for j in range(n):
+                plan(eye[j], H[j])
+        Hh = H.cpu().numpy()
+        Hh = 0.5 * (Hh + Hh.T)
+        w, v = numpy.linalg.eigh(Hh)
+        vecs = [torch.from_numpy(numpy.ascontiguousarray(v[:, k])).cuda() for k in range(nroots)]
+        return w[:nroots], vecs

@@ -1,0 +1,95 @@
+"""Davidson drivers: plain FP64, and mixed precision with FP64 refinement.
+
+Mixed precision (precision "fp32" or "bf16x3"): the Davidson iterations use
+a sigma whose t1 contraction runs in FP32 MFMA or in a 3-term bf16 split
+(FP32 accumulation), while residuals, orthogonalisation and the subspace
+eigenproblem stay FP64.  When the low-precision residual reaches its noise
+floor the solve is continued from the current Ritz vectors with the FP64
+sigma (iterative refinement) until the FP64 criteria of the reference's
+Davidson call (tol on the energy change, sqrt(tol) on the residual,
+fci_gas.py:1344-1350) hold, so the returned energy and vectors satisfy the
+same convergence test as a plain FP64 solve.
+"""
+
+import time
+
+import numpy
+
+from . import _lib
+from . import davidson as _dv
+
+# low-precision stage: residual floor and energy-change tolerance
+LOW_STAGE = {
+    "fp32": dict(tol=1e-7, tol_residual=3e-4),
+    "bf16x3": dict(tol=1e-6, tol_residual=1e-3),
+}
+
+
+def _davidson_bytes(ndet, nroots, max_space):
+    cap = min(int(max_space) + (nroots - 1) * 3, 64)
+    cap = max(cap, 2 * nroots)
+    return (2 * cap + 2 * nroots + 2 + nroots) * ndet * 8
+
+
+def solve(solver, sp, h2e, hdiag, nroots, ci0, precision, tol=1e-10, tol_residual=None,
+          max_cycle=50, max_space=12, lindep=1e-14, level_shift=1e-3, verbose=0):
+    import torch
+    from .direct_spin1 import SigmaPlan
+
+    lib = _lib.load()
+    n = sp.ndet
+    stream = _lib.stream_ptr()
+
+    def precond(r, e, out):
+        _lib.check(lib.fcis_precond(n, _lib.vptr(hdiag), float(e), float(level_shift),
+                                    _lib.vptr(r), _lib.vptr(out), stream), "fcis_precond")
+
+    if ci0 is None:
+        guesses = solver._init_guess_dev(hdiag, nroots)
+    else:
+        if isinstance(ci0, (list, tuple)):
+            vecs = list(ci0)
+        else:
+            arr = ci0
+            vecs = [arr] if (getattr(arr, "ndim", 1) <= 2 and nroots == 1) else list(arr)
+        guesses = []
+        for v in vecs:
+            t = torch.as_tensor(numpy.asarray(v) if not isinstance(v, torch.Tensor) else v)
+            t = t.to(device="cuda", dtype=torch.float64).reshape(-1).contiguous()
+            if t.numel() != n:
+                raise ValueError("initial CI vector size does not match determinant count")
+            guesses.append(t)
+        if len(guesses) < nroots:
+            guesses += solver._init_guess_dev(hdiag, nroots)[len(guesses):]
+
+    free = torch.cuda.mem_get_info()[0]
+    need = _davidson_bytes(n, nroots, max_space)
+    ws = int(solver.workspace_bytes) or max(int((free - need) * 0.85), 1 << 26)
+    stats = {"precision": precision}
+    verbose = bool(verbose and verbose > 4)
+
+    def run(prec, guesses, **kw):
+        t0 = time.perf_counter()
+        with SigmaPlan(sp, h2e, prec, ws) as plan:
+            res = _dv.davidson(plan, precond, guesses, nroots=nroots, lindep=lindep,
+                               max_space=max_space, verbose=verbose, **kw)
+        torch.cuda.synchronize()
+        return res, time.perf_counter() - t0, plan
+
+    if precision == "fp64":
+        res, dt, plan = run("fp64", guesses, tol=tol, tol_residual=tol_residual,
+                            max_cycle=max_cycle)
+        stats.update(time=dt, nhop=res.nhop, cycles=res.cycles, ntiles=plan.ntiles)
+        return res.e, list(res.x), res.converged, stats
+
+    if precision not in LOW_STAGE:
+        raise ValueError("unknown precision %r" % (precision,))
+    lo = LOW_STAGE[precision]
+    res_lo, dt_lo, plan = run(precision, guesses, max_cycle=max_cycle, **lo)
+    stats.update(time_low=dt_lo, nhop_low=res_lo.nhop, cycles_low=res_lo.cycles,
+                 e_low=res_lo.e.tolist(), rnorm_low=res_lo.rnorm.tolist())
+    res, dt, plan = run("fp64", list(res_lo.x), tol=tol, tol_residual=tol_residual,
+                        max_cycle=max_cycle)
+    stats.update(time_refine=dt, nhop_refine=res.nhop, cycles_refine=res.cycles,
+                 time=dt_lo + dt, ntiles=plan.ntiles)
+    return res.e, list(res.x), res.converged, stats

@@ -1,0 +1,147 @@
+"""GPU parity of the HIP path against the oracle and the reference's golden
+vectors (libfci_gas), through the C ABI.  FP64 tolerances: sigma within
+1e-13 of max|sigma| (the benchmark's "1e-13 relative"), hdiag 1e-12,
+RDMs 1e-12 (the reference's own GAS-vs-CAS tolerances, test_gasci.py:272-293,
+:765-786), energies 1e-10."""
+
+import math
+
+import numpy
+import pytest
+
+import oracle
+from conftest import golden_cases, has_gpu, load_golden, package
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not has_gpu(), reason="needs a HIP device")]
+
+pkg = package()
+
+
+def _dev(x):
+    import torch
+    return torch.from_numpy(numpy.ascontiguousarray(x, dtype=numpy.float64).reshape(-1)).cuda()
+
+
+@pytest.mark.parametrize("n,k", [(1, 0), (1, 1), (4, 2), (6, 1), (8, 4), (9, 5), (12, 6), (16, 8), (16, 1)])
+def test_link_tables_bit_exact(n, k):
+    sp = pkg.get_space(n, (k, k))
+    for tril in (False, True):
+        mine = sp.link_index(0, tril).cpu().numpy()
+        ref = oracle.gen_linkstr_index(n, k, tril)
+        assert mine.shape == ref.shape
+        assert numpy.array_equal(mine, ref)
+
+
+@pytest.mark.parametrize("name", golden_cases())
+def test_sigma_hdiag_rdm_vs_golden(name):
+    g = load_golden(name)
+    n, ne = g["norb"], g["nelec"]
+    h2e = pkg.absorb_h1e(g["h1e"], g["eri"], n, ne, 0.5)
+    numpy.testing.assert_allclose(h2e, g["h2e"], rtol=0, atol=1e-14)
+    sig = pkg.contract_2e(h2e, g["ci"], n, ne)
+    scale = numpy.abs(g["sigma"]).max()
+    numpy.testing.assert_allclose(sig, g["sigma"], rtol=0, atol=1e-13 * scale)
+    hd = pkg.make_hdiag(g["h1e"], g["eri"], n, ne).reshape(g["hdiag"].shape)
+    numpy.testing.assert_allclose(hd, g["hdiag"], rtol=0, atol=1e-12)
+    (d1a, d1b), (aa, ab, bb) = pkg.direct_spin1._rdm12s(g["ci"], g["ket"], n, ne)
+    for mine, key in ((d1a, "dm1a"), (d1b, "dm1b"), (aa, "dm2aa"), (ab, "dm2ab"), (bb, "dm2bb")):
+        numpy.testing.assert_allclose(mine, g[key], rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("n,ne,seed", [(10, (5, 5), 1), (11, (6, 4), 2), (12, (6, 6), 0), (9, (9, 0), 3), (7, (0, 3), 4)])
+def test_sigma_vs_oracle_medium(n, ne, seed):
+    h1e, eri, c = pkg.synthetic.make_problem(n, ne, seed=seed)
+    h2e = oracle.absorb_h1e(h1e, eri, n, ne, 0.5)
+    ref = oracle.contract_2e(h2e, c, n, ne)
+    sig = pkg.contract_2e(h2e, c, n, ne)
+    numpy.testing.assert_allclose(sig, ref, rtol=0, atol=1e-13 * numpy.abs(ref).max())
+
+
+def test_sigma_tiling_and_determinism():
+    import torch
+    n, ne = 10, (5, 4)
+    h1e, eri, c = pkg.synthetic.make_problem(n, ne, seed=5)
+    h2e = pkg.absorb_h1e(h1e, eri, n, ne, 0.5)
+    sp = pkg.get_space(n, ne)
+    cd = _dev(c)
+    outs = []
+    npair = n * (n + 1) // 2
+    for ws in (1 << 30, 3 * 2 * npair * sp.nstrb * 8, 1):
+        with pkg.SigmaPlan(sp, h2e, "fp64", ws) as plan:
+            o1 = torch.empty_like(cd)
+            o2 = torch.empty_like(cd)
+            plan(cd, o1)
+            plan(cd, o2)
+            assert torch.equal(o1, o2), "sigma is not bitwise reproducible"
+            outs.append((plan.ntiles, o1.cpu().numpy()))
+    assert outs[0][0] == 1 and outs[-1][0] == sp.nstra
+    for nt, o in outs[1:]:
+        numpy.testing.assert_allclose(o, outs[0][1], rtol=0, atol=1e-13 * numpy.abs(outs[0][1]).max())
+
+
+def test_sigma_full_size_properties_c1():
+    """(14o,14e): symmetry <x|Hy> = <y|Hx>, linearity and diagonal
+    consistency at the full benchmark size (size-independent checks)."""
+    import torch
+    n, ne = 14, (7, 7)
+    h1e, eri = pkg.synthetic.make_problem(n, ne, seed=0, with_ci=False)
+    h2e = pkg.absorb_h1e(h1e, eri, n, ne, 0.5)
+    sp = pkg.get_space(n, ne)
+    g = torch.Generator(device="cuda").manual_seed(0)
+    x = torch.randn(sp.ndet, dtype=torch.float64, device="cuda", generator=g)
+    y = torch.randn(sp.ndet, dtype=torch.float64, device="cuda", generator=g)
+    hx, hy, hz = torch.empty_like(x), torch.empty_like(x), torch.empty_like(x)
+    with pkg.SigmaPlan(sp, h2e, "fp64", 8 << 30) as plan:
+        plan(x, hx)
+        plan(y, hy)
+        plan(2.0 * x - 3.0 * y, hz)
+        a, b = float(torch.dot(x, hy)), float(torch.dot(y, hx))
+        assert abs(a - b) < 1e-10 * max(abs(a), 1.0)
+        lin = (hz - (2.0 * hx - 3.0 * hy)).abs().max().item()
+        assert lin < 1e-11 * hx.abs().max().item()
+        hd = pkg.direct_spin1._make_hdiag_dev(h1e, eri, n, ne)
+        for i in (0, 12345, sp.ndet // 2, sp.ndet - 1):
+            e = torch.zeros_like(x)
+            e[i] = 1.0
+            plan(e, hz)
+            assert abs(hz[i].item() - hd[i].item()) < 1e-12
+
+
+def test_davidson_c0_vs_oracle():
+    n, ne = 8, (4, 4)
+    h1e, eri = pkg.synthetic.make_problem(n, ne, seed=0, with_ci=False)
+    solver = pkg.FCISolver()
+    e, ci = solver.kernel(h1e, eri, n, ne)
+    w, v = oracle.lowest_eigenvalues(h1e, eri, n, ne, nroots=3)
+    assert solver.converged
+    assert abs(e - w[0]) < 1e-10
+    assert abs(numpy.linalg.norm(ci) - 1.0) < 1e-12
+    assert abs(abs(float(ci.reshape(-1) @ v[0])) - 1.0) < 1e-8
+    assert abs(oracle.energy(h1e, eri, ci, n, ne) - e) < 1e-10
+    es, cis = solver.kernel(h1e, eri, n, ne, nroots=3)
+    numpy.testing.assert_allclose(es, w[:3], atol=1e-9)
+
+
+def test_davidson_small_space_exact():
+    n, ne = 4, (2, 1)
+    h1e, eri = pkg.synthetic.make_problem(n, ne, seed=2, with_ci=False)
+    from oracle import brute
+    w = numpy.linalg.eigvalsh(brute.hamiltonian(h1e, eri, n, ne))
+    solver = pkg.FCISolver()
+    e, ci = solver.kernel(h1e, eri, n, ne, nroots=2)
+    numpy.testing.assert_allclose(e, w[:2], atol=1e-12)
+    e1, _ = solver.kernel(h1e, eri, n, ne, davidson_only=True)
+    assert abs(e1 - w[0]) < 1e-10
+
+
+def test_rdm_identities_c4_size():
+    n, ne = 12, (6, 6)
+    h1e, eri, c = pkg.synthetic.make_problem(n, ne, seed=0)
+    dm1, dm2 = pkg.make_rdm12(c, n, ne)
+    N = sum(ne)
+    assert abs(numpy.trace(dm1) - N) < 1e-10
+    numpy.testing.assert_allclose(numpy.einsum("pqrr->pq", dm2), (N - 1) * dm1, atol=1e-10)
+    e_rdm = numpy.einsum("pq,pq", h1e, dm1) + 0.5 * numpy.einsum(
+        "pqrs,pqrs", oracle.unpack_pair_matrix(eri, n), dm2)
+    assert abs(e_rdm - pkg.energy(h1e, eri, c, n, ne)) < 1e-10
