@@ -86,7 +86,7 @@ __global__ __launch_bounds__(256) void k_rdm_gemm(int64_t ndet, int wp, int nblk
         }
         double *out = part + (size_t)blockIdx.x * wp * wp;
         for (int r = 0; r < 4; r++) {
-                int u = ub * 16 + 4 * lk + r, v = vb * 16 + li;
+                int u = ub * 16 + lk + 4 * r, v = vb * 16 + li;  // f64 MFMA D layout
                 out[(size_t)u * wp + v] = acc[r];
         }
 }

@@ -75,8 +75,9 @@ __global__ __launch_bounds__(256) void k_gather(int npair, int64_t nb, int64_t a
 // hfrag holds h pre-arranged per (k-step, 16-row tile, lane) so that every
 // A-fragment load is one contiguous 512-byte read:
 //   hfrag[((ks * mtiles + mt) * 64) + lane] = h[mt*16 + lane%16][4*ks + lane/16]
-// Lane layout of the 16x16x4 f64 MFMA: A[i][k] at lane i + 16k, B[k][j] at
-// lane j + 16k, D[i][j] in lane j + 16*(i/4), element i%4.
+// Lane layout of the 16x16x4 f64 MFMA (measured on gfx950): A[i][k] at lane
+// i + 16k, B[k][j] at lane j + 16k, D[i][j] in lane j + 16*(i%4), element
+// i/4 (the f32/bf16 forms use lane j + 16*(i/4), element i%4).
 __global__ __launch_bounds__(256) void k_contract_f64(int npair, int mtiles, int ksteps,
                                                       int64_t ndet, int64_t ld,
                                                       const double *__restrict__ hfrag,
@@ -112,6 +113,140 @@ __global__ __launch_bounds__(256) void k_contract_f64(int npair, int mtiles, int
                                 for (int j = 0; j < GEMM_NSUB; j++)
                                         acc[m][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
                                                 a, bfr[j], acc[m][j], 0, 0, 0);
+                        }
+                }
+        }
+#pragma unroll
+        for (int m = 0; m < GEMM_MT; m++) {
+                if (mt0 + m >= mtiles) continue;
+#pragma unroll
+                for (int j = 0; j < GEMM_NSUB; j++) {
+                        int64_t col = col0 + j * 16 + li;
+                        if (col >= ndet) continue;
+#pragma unroll
+                        for (int r = 0; r < 4; r++) {
+                                int row = (mt0 + m) * 16 + lk + 4 * r;
+                                if (row < npair) vt1[(int64_t)row * ld + col] = acc[m][j][r];
+                        }
+                }
+        }
+}
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// FP32 MFMA contraction (v_mfma_f32_16x16x4_f32, same lane layout as the
+// f64 form); t1 and vt1 are float, h fragments are float.
+__global__ __launch_bounds__(256) void k_contract_f32(int npair, int mtiles, int ksteps,
+                                                      int64_t ndet, int64_t ld,
+                                                      const float *__restrict__ hfrag,
+                                                      const float *__restrict__ t1,
+                                                      float *__restrict__ vt1)
+{
+        const int lane = threadIdx.x & 63;
+        const int wave = threadIdx.x >> 6;
+        const int mt0 = blockIdx.y * GEMM_MT;
+        const int64_t col0 = (int64_t)blockIdx.x * GEMM_NCOL + wave * (GEMM_NSUB * 16);
+        const int li = lane & 15, lk = lane >> 4;
+
+        v4f acc[GEMM_MT][GEMM_NSUB];
+#pragma unroll
+        for (int m = 0; m < GEMM_MT; m++)
+#pragma unroll
+                for (int j = 0; j < GEMM_NSUB; j++) acc[m][j] = v4f{0, 0, 0, 0};
+
+        for (int ks = 0; ks < ksteps; ks++) {
+                int k = 4 * ks + lk;
+                float bfr[GEMM_NSUB];
+#pragma unroll
+                for (int j = 0; j < GEMM_NSUB; j++) {
+                        int64_t col = col0 + j * 16 + li;
+                        bfr[j] = (k < npair && col < ndet) ? t1[(int64_t)k * ld + col] : 0.0f;
+                }
+                const float *hf = hfrag + ((int64_t)ks * mtiles + mt0) * 64 + lane;
+#pragma unroll
+                for (int m = 0; m < GEMM_MT; m++) {
+                        if (mt0 + m < mtiles) {
+                                float a = hf[m * 64];
+#pragma unroll
+                                for (int j = 0; j < GEMM_NSUB; j++)
+                                        acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                                                a, bfr[j], acc[m][j], 0, 0, 0);
+                        }
+                }
+        }
+#pragma unroll
+        for (int m = 0; m < GEMM_MT; m++) {
+                if (mt0 + m >= mtiles) continue;
+#pragma unroll
+                for (int j = 0; j < GEMM_NSUB; j++) {
+                        int64_t col = col0 + j * 16 + li;
+                        if (col >= ndet) continue;
+#pragma unroll
+                        for (int r = 0; r < 4; r++) {
+                                int row = (mt0 + m) * 16 + 4 * lk + r;
+                                if (row < npair) vt1[(int64_t)row * ld + col] = acc[m][j][r];
+                        }
+                }
+        }
+}
+
+// 3-term bf16 split contraction with FP32 accumulation:
+//   h = h_hi + h_lo, t = t_hi + t_lo (each term rounded to bf16),
+//   h t ~= h_hi t_hi + h_hi t_lo + h_lo t_hi
+// on v_mfma_f32_16x16x32_bf16 (gfx950).  Lane layout: A[i][k] for
+// k = 8*(lane/16) + e (e = 0..7) at lane i + 16*(lane/16), B[k][j] likewise,
+// D as in the 16x16x4 forms.  t1 is read as float and split in registers;
+// h fragments are pre-split on the host:
+//   hhi/hlo[((ks * mtiles + mt) * 64) + lane][e] = h[mt*16 + lane%16][32*ks + 8*(lane/16) + e]
+__global__ __launch_bounds__(256) void k_contract_bf16x3(int npair, int mtiles, int ksteps,
+                                                         int64_t ndet, int64_t ld,
+                                                         const bf16x8 *__restrict__ hhi,
+                                                         const bf16x8 *__restrict__ hlo,
+                                                         const float *__restrict__ t1,
+                                                         float *__restrict__ vt1)
+{
+        const int lane = threadIdx.x & 63;
+        const int wave = threadIdx.x >> 6;
+        const int mt0 = blockIdx.y * GEMM_MT;
+        const int64_t col0 = (int64_t)blockIdx.x * GEMM_NCOL + wave * (GEMM_NSUB * 16);
+        const int li = lane & 15, lk = lane >> 4;
+
+        v4f acc[GEMM_MT][GEMM_NSUB];
+#pragma unroll
+        for (int m = 0; m < GEMM_MT; m++)
+#pragma unroll
+                for (int j = 0; j < GEMM_NSUB; j++) acc[m][j] = v4f{0, 0, 0, 0};
+
+        for (int ks = 0; ks < ksteps; ks++) {
+                bf16x8 bh[GEMM_NSUB], bl[GEMM_NSUB];
+#pragma unroll
+                for (int j = 0; j < GEMM_NSUB; j++) {
+                        int64_t col = col0 + j * 16 + li;
+#pragma unroll
+                        for (int e = 0; e < 8; e++) {
+                                int k = 32 * ks + 8 * lk + e;
+                                float x = (k < npair && col < ndet) ? t1[(int64_t)k * ld + col] : 0.0f;
+                                __bf16 hi = (__bf16)x;
+                                bh[j][e] = hi;
+                                bl[j][e] = (__bf16)(x - (float)hi);
+                        }
+                }
+                const int64_t fo = ((int64_t)ks * mtiles + mt0) * 64 + lane;
+#pragma unroll
+                for (int m = 0; m < GEMM_MT; m++) {
+                        if (mt0 + m < mtiles) {
+                                bf16x8 ah = hhi[fo + m * 64];
+                                bf16x8 al = hlo[fo + m * 64];
+#pragma unroll
+                                for (int j = 0; j < GEMM_NSUB; j++) {
+                                        acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                                al, bh[j], acc[m][j], 0, 0, 0);
+                                        acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                                ah, bl[j], acc[m][j], 0, 0, 0);
+                                        acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                                ah, bh[j], acc[m][j], 0, 0, 0);
+                                }
                         }
                 }
         }
@@ -180,8 +315,9 @@ __global__ void k_build_pmap(uint32_t nstr, int nlink, const fcis_link *__restri
 struct fcis_sigma_plan {
         const fcis_space *space;
         int precision;
-        int npair, mtiles, ksteps;
-        double *d_hfrag;
+        int npair, mtiles, ksteps;   // ksteps: MFMA k-steps (4 for f64/f32, 32 for bf16)
+        void *d_hfrag;               // FP64 / FP32 fragments, or bf16 hi fragments
+        void *d_hfrag_lo;            // bf16 lo fragments (BF16X3 only)
         int32_t *d_pmapa, *d_pmapb;
         int64_t tile_a;
         int ntiles;
@@ -192,11 +328,56 @@ struct fcis_sigma_plan {
 
 namespace {
 
+// float -> bf16 bits, round to nearest even (NaN kept quiet).
+uint16_t bf16_bits(float x)
+{
+        uint32_t u;
+        memcpy(&u, &x, 4);
+        if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
+        u += 0x7fffu + ((u >> 16) & 1u);
+        return (uint16_t)(u >> 16);
+}
+
+float bf16_value(uint16_t b)
+{
+        uint32_t u = (uint32_t)b << 16;
+        float x;
+        memcpy(&x, &u, 4);
+        return x;
+}
+
+size_t frag_count(const fcis_sigma_plan *p)
+{
+        return (size_t)p->ksteps * p->mtiles * 64;
+}
+
 int upload_h(fcis_sigma_plan *p, const double *h2e)
 {
-        int np = p->npair;
+        const int np = p->npair;
         p->h.assign(h2e, h2e + (size_t)np * np);
-        std::vector<double> frag((size_t)p->ksteps * p->mtiles * 64, 0.0);
+        const size_t nf = frag_count(p);
+        if (p->precision == FCIS_PREC_BF16X3) {
+                // 8 consecutive k per lane; k-step of 32
+                std::vector<uint16_t> hi(nf * 8), lo(nf * 8);
+                for (int ks = 0; ks < p->ksteps; ks++)
+                        for (int mt = 0; mt < p->mtiles; mt++)
+                                for (int l = 0; l < 64; l++)
+                                        for (int e = 0; e < 8; e++) {
+                                                int i = mt * 16 + (l & 15);
+                                                int k = 32 * ks + 8 * (l >> 4) + e;
+                                                float x = (i < np && k < np)
+                                                                  ? (float)h2e[(size_t)i * np + k]
+                                                                  : 0.0f;
+                                                uint16_t xh = bf16_bits(x);
+                                                size_t o = (((size_t)ks * p->mtiles + mt) * 64 + l) * 8 + e;
+                                                hi[o] = xh;
+                                                lo[o] = bf16_bits(x - bf16_value(xh));
+                                        }
+                FCIS_HIP(hipMemcpy(p->d_hfrag, hi.data(), hi.size() * 2, hipMemcpyHostToDevice));
+                FCIS_HIP(hipMemcpy(p->d_hfrag_lo, lo.data(), lo.size() * 2, hipMemcpyHostToDevice));
+                return FCIS_SUCCESS;
+        }
+        std::vector<double> frag(nf, 0.0);
         for (int ks = 0; ks < p->ksteps; ks++)
                 for (int mt = 0; mt < p->mtiles; mt++)
                         for (int l = 0; l < 64; l++) {
@@ -205,8 +386,12 @@ int upload_h(fcis_sigma_plan *p, const double *h2e)
                                         frag[((size_t)ks * p->mtiles + mt) * 64 + l] =
                                                 h2e[(size_t)i * np + k];
                         }
-        FCIS_HIP(hipMemcpy(p->d_hfrag, frag.data(), frag.size() * sizeof(double),
-                           hipMemcpyHostToDevice));
+        if (p->precision == FCIS_PREC_FP64) {
+                FCIS_HIP(hipMemcpy(p->d_hfrag, frag.data(), nf * 8, hipMemcpyHostToDevice));
+        } else {
+                std::vector<float> f(frag.begin(), frag.end());
+                FCIS_HIP(hipMemcpy(p->d_hfrag, f.data(), nf * 4, hipMemcpyHostToDevice));
+        }
         return FCIS_SUCCESS;
 }
 
@@ -221,8 +406,9 @@ int fcis_sigma_plan_create(fcis_sigma_plan_t **out, const fcis_space_t *space,
                 fcis::set_error("fcis_sigma_plan_create: null argument");
                 return FCIS_ERR_INVALID;
         }
-        if (precision != FCIS_PREC_FP64) {
-                fcis::set_error("fcis_sigma_plan_create: precision %d not available", precision);
+        if (precision != FCIS_PREC_FP64 && precision != FCIS_PREC_FP32 &&
+            precision != FCIS_PREC_BF16X3) {
+                fcis::set_error("fcis_sigma_plan_create: unknown precision %d", precision);
                 return FCIS_ERR_INVALID;
         }
         *out = nullptr;
@@ -232,9 +418,10 @@ int fcis_sigma_plan_create(fcis_sigma_plan_t **out, const fcis_space_t *space,
         p->precision = precision;
         p->npair = space->npair;
         p->mtiles = (space->npair + 15) / 16;
-        p->ksteps = (space->npair + 3) / 4;
-        size_t elem = sizeof(double);
-        int64_t nb = space->nstrb;
+        const int kstep = precision == FCIS_PREC_BF16X3 ? 32 : 4;
+        p->ksteps = (space->npair + kstep - 1) / kstep;
+        const size_t elem = precision == FCIS_PREC_FP64 ? 8 : 4;
+        const int64_t nb = space->nstrb;
         if (workspace_bytes == 0) workspace_bytes = (uint64_t)16 << 30;
         // two buffers (t1, vt1) of npair x tile_a x nb elements
         uint64_t per_alpha = 2ull * p->npair * (uint64_t)nb * elem;
@@ -245,16 +432,18 @@ int fcis_sigma_plan_create(fcis_sigma_plan_t **out, const fcis_space_t *space,
         p->ntiles = (int)((space->nstra + ta - 1) / ta);
         size_t buf = (size_t)p->npair * ta * nb * elem;
         p->ws_bytes = 2 * buf;
-        size_t hbytes = (size_t)p->ksteps * p->mtiles * 64 * sizeof(double);
+        const size_t nf = frag_count(p);
+        size_t hbytes = precision == FCIS_PREC_BF16X3 ? nf * 16 : nf * elem;
         size_t pa = (size_t)p->npair * space->nstra * 4, pb = (size_t)p->npair * nb * 4;
         if (hipMalloc(&p->d_hfrag, hbytes) != hipSuccess ||
+            hipMalloc(&p->d_hfrag_lo, precision == FCIS_PREC_BF16X3 ? hbytes : 16) != hipSuccess ||
             hipMalloc(&p->d_pmapa, pa ? pa : 4) != hipSuccess ||
             hipMalloc(&p->d_pmapb, pb ? pb : 4) != hipSuccess ||
             hipMalloc(&p->d_t1, buf ? buf : 8) != hipSuccess ||
             hipMalloc(&p->d_vt1, buf ? buf : 8) != hipSuccess) {
                 fcis_sigma_plan_free(p);
                 fcis::set_error("fcis_sigma_plan_create: device allocation of %zu bytes failed",
-                                2 * buf + hbytes + pa + pb);
+                                2 * buf + 2 * hbytes + pa + pb);
                 return FCIS_ERR_MEMORY;
         }
         int st = upload_h(p, h2e);
@@ -309,26 +498,42 @@ int fcis_sigma_plan_execute(fcis_sigma_plan_t *p, const double *ci0, double *ci1
         }
         hipStream_t st = (hipStream_t)stream;
         const fcis_space *s = p->space;
-        int64_t nb = s->nstrb, na = s->nstra;
+        const int64_t nb = s->nstrb, na = s->nstra;
         if (na * nb == 0) return FCIS_SUCCESS;
+        const dim3 gblk(64 * GEMM_WAVES);
         for (int t = 0; t < p->ntiles; t++) {
-                int64_t a0 = (int64_t)t * p->tile_a;
-                int64_t ta = na - a0 < p->tile_a ? na - a0 : p->tile_a;
-                int64_t ndet = ta * nb;
-                int64_t ld = ndet;
-                double *t1 = (double *)p->d_t1, *vt1 = (double *)p->d_vt1;
-                hipLaunchKernelGGL(k_gather<double>, dim3(ta * ((nb + 255) / 256)), dim3(256), 0, st,
-                                   p->npair, nb, a0, na, ld, p->d_pmapa, p->d_pmapb, ci0, t1);
-                FCIS_CHECK_LAUNCH();
-                hipLaunchKernelGGL(k_contract_f64,
-                                   dim3((ndet + GEMM_NCOL - 1) / GEMM_NCOL,
-                                        (p->mtiles + GEMM_MT - 1) / GEMM_MT),
-                                   dim3(64 * GEMM_WAVES), 0, st, p->npair, p->mtiles, p->ksteps,
-                                   ndet, ld, p->d_hfrag, t1, vt1);
-                FCIS_CHECK_LAUNCH();
-                hipLaunchKernelGGL(k_scatter<double>, dim3(na * ((nb + 255) / 256)), dim3(256), 0, st,
-                                   nb, na, a0, ta, ld, s->nlinka, s->nlinkb, t > 0 ? 1 : 0,
-                                   s->d_linka, s->d_linkb, vt1, ci1);
+                const int64_t a0 = (int64_t)t * p->tile_a;
+                const int64_t ta = na - a0 < p->tile_a ? na - a0 : p->tile_a;
+                const int64_t ndet = ta * nb;
+                const int64_t ld = ndet;
+                const dim3 ggrid((ndet + GEMM_NCOL - 1) / GEMM_NCOL, (p->mtiles + GEMM_MT - 1) / GEMM_MT);
+                const dim3 rgrid(ta * ((nb + 255) / 256)), sgrid(na * ((nb + 255) / 256));
+                if (p->precision == FCIS_PREC_FP64) {
+                        double *t1 = (double *)p->d_t1, *vt1 = (double *)p->d_vt1;
+                        hipLaunchKernelGGL(k_gather<double>, rgrid, dim3(256), 0, st, p->npair, nb, a0,
+                                           na, ld, p->d_pmapa, p->d_pmapb, ci0, t1);
+                        hipLaunchKernelGGL(k_contract_f64, ggrid, gblk, 0, st, p->npair, p->mtiles,
+                                           p->ksteps, ndet, ld, (const double *)p->d_hfrag, t1, vt1);
+                        hipLaunchKernelGGL(k_scatter<double>, sgrid, dim3(256), 0, st, nb, na, a0, ta,
+                                           ld, s->nlinka, s->nlinkb, t > 0 ? 1 : 0, s->d_linka,
+                                           s->d_linkb, vt1, ci1);
+                } else {
+                        float *t1 = (float *)p->d_t1, *vt1 = (float *)p->d_vt1;
+                        hipLaunchKernelGGL(k_gather<float>, rgrid, dim3(256), 0, st, p->npair, nb, a0,
+                                           na, ld, p->d_pmapa, p->d_pmapb, ci0, t1);
+                        if (p->precision == FCIS_PREC_FP32)
+                                hipLaunchKernelGGL(k_contract_f32, ggrid, gblk, 0, st, p->npair,
+                                                   p->mtiles, p->ksteps, ndet, ld,
+                                                   (const float *)p->d_hfrag, t1, vt1);
+                        else
+                                hipLaunchKernelGGL(k_contract_bf16x3, ggrid, gblk, 0, st, p->npair,
+                                                   p->mtiles, p->ksteps, ndet, ld,
+                                                   (const bf16x8 *)p->d_hfrag,
+                                                   (const bf16x8 *)p->d_hfrag_lo, t1, vt1);
+                        hipLaunchKernelGGL(k_scatter<float>, sgrid, dim3(256), 0, st, nb, na, a0, ta,
+                                           ld, s->nlinka, s->nlinkb, t > 0 ? 1 : 0, s->d_linka,
+                                           s->d_linkb, vt1, ci1);
+                }
                 FCIS_CHECK_LAUNCH();
         }
         return FCIS_SUCCESS;
@@ -338,6 +543,7 @@ void fcis_sigma_plan_free(fcis_sigma_plan_t *p)
 {
         if (p == nullptr) return;
         (void)hipFree(p->d_hfrag);
+        (void)hipFree(p->d_hfrag_lo);
         (void)hipFree(p->d_pmapa);
         (void)hipFree(p->d_pmapb);
         (void)hipFree(p->d_t1);

@@ -292,14 +292,18 @@ int fcis_space_info(const fcis_space_t *s, int64_t *nstra, int64_t *nstrb,
 int fcis_gen_linkstr_index(const fcis_space_t *s, int spin, int trilidx,
                            int32_t *d_out, void *stream)
 {
-        if (s == nullptr || d_out == nullptr || (spin != 0 && spin != 1)) {
+        if (s == nullptr || (spin != 0 && spin != 1)) {
                 fcis::set_error("fcis_gen_linkstr_index: invalid argument");
                 return FCIS_ERR_INVALID;
         }
         int k = spin ? s->nb : s->na;
         uint32_t nstr = (uint32_t)(spin ? s->nstrb : s->nstra);
         int nlink = spin ? s->nlinkb : s->nlinka;
-        if (nlink == 0) return FCIS_SUCCESS;
+        if (nlink == 0) return FCIS_SUCCESS;  // empty table: nothing to write
+        if (d_out == nullptr) {
+                fcis::set_error("fcis_gen_linkstr_index: null output");
+                return FCIS_ERR_INVALID;
+        }
         hipLaunchKernelGGL(k_export_links, dim3((nstr + 255) / 256, nlink), dim3(256), 0,
                            (hipStream_t)stream, s->norb, k, nstr, nlink, trilidx ? 1 : 0,
                            spin ? s->d_strb : s->d_stra, d_out);

@@ -31,44 +31,60 @@ class DavidsonResult:
         self.rnorm = rnorm
 
 
-def _dots(lib, m, n, X, v, stream):
-    out = numpy.empty(max(m, 1))
-    if m:
-        _lib.check(lib.fcis_dots(m, n, _lib.vptr(X), n, _lib.vptr(v),
-                                 _lib.dptr(out), stream), "fcis_dots")
-    return out[:m]
+class DeviceOps:
+    """Vector algebra of the solver on device float64 vectors, through the
+    library's HIP kernels (fcis_dots / fcis_lincomb)."""
 
+    def __init__(self):
+        self.lib = _lib.load()
+        self.stream = _lib.stream_ptr()
 
-def _lincomb(lib, m, n, X, coef, beta, y, stream):
-    coef = numpy.ascontiguousarray(coef, dtype=numpy.float64)
-    _lib.check(lib.fcis_lincomb(m, n, _lib.vptr(X), n, _lib.dptr(coef),
-                                float(beta), _lib.vptr(y), stream), "fcis_lincomb")
+    def empty(self, shape, like):
+        import torch
+        return torch.empty(shape, dtype=torch.float64, device=like.device)
+
+    def clone(self, v):
+        return v.detach().clone()
+
+    def copy(self, dst, src):
+        dst.copy_(src)
+
+    def dots(self, m, n, X, v):
+        out = numpy.empty(max(m, 1))
+        if m:
+            _lib.check(self.lib.fcis_dots(m, n, _lib.vptr(X), n, _lib.vptr(v),
+                                          _lib.dptr(out), self.stream), "fcis_dots")
+        return out[:m]
+
+    def lincomb(self, m, n, X, coef, beta, y):
+        coef = numpy.ascontiguousarray(coef, dtype=numpy.float64)
+        _lib.check(self.lib.fcis_lincomb(m, n, _lib.vptr(X), n, _lib.dptr(coef), float(beta),
+                                         _lib.vptr(y), self.stream), "fcis_lincomb")
 
 
 def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
              max_cycle=50, max_space=12, lindep=1e-14, verbose=False,
-             callback=None):
+             callback=None, ops=None):
     """Lowest ``nroots`` eigenpairs.  x0: list of device vectors (guesses).
 
     Returns DavidsonResult with e (numpy), x / ax (device tensors of shape
     (nroots, n)), per-root convergence flags, cycle and H-application counts.
     """
-    import torch
-
-    lib = _lib.load()
-    stream = _lib.stream_ptr()
-    n = int(x0[0].numel())
-    dev = x0[0].device
+    ops = ops if ops is not None else DeviceOps()
+    n = int(x0[0].reshape(-1).shape[0])
     if tol_residual is None:
         tol_residual = math.sqrt(tol)
     cap = int(max_space) + (nroots - 1) * 3
     cap = max(cap, nroots + len(x0))
     cap = min(cap, 64, n)
-    X = torch.empty((cap, n), dtype=torch.float64, device=dev)
-    AX = torch.empty((cap, n), dtype=torch.float64, device=dev)
-    xr = torch.empty((nroots, n), dtype=torch.float64, device=dev)
-    axr = torch.empty((nroots, n), dtype=torch.float64, device=dev)
-    r = torch.empty(n, dtype=torch.float64, device=dev)
+    X = ops.empty((cap, n), x0[0])
+    AX = ops.empty((cap, n), x0[0])
+    xr = ops.empty((nroots, n), x0[0])
+    axr = ops.empty((nroots, n), x0[0])
+    xp = ops.empty((nroots, n), x0[0])      # previous Ritz directions (restart)
+    axp = ops.empty((nroots, n), x0[0])
+    r = ops.empty((n,), x0[0])
+    v_prev = None
     heff = numpy.zeros((cap, cap))
     space = 0
     nhop = 0
@@ -77,27 +93,27 @@ def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
         """Orthonormalise t against X[:space] (Gram-Schmidt, twice) and
         append it with its image.  Returns False if t is linearly dependent."""
         nonlocal space, nhop
-        norm0 = math.sqrt(_dots(lib, 1, n, t, t, stream)[0])
+        norm0 = math.sqrt(ops.dots(1, n, t, t)[0])
         if norm0 == 0.0:
             return False
         for _ in range(2):
             if space:
-                c = _dots(lib, space, n, X, t, stream)
-                _lincomb(lib, space, n, X, -c, 1.0, t, stream)
-        norm = math.sqrt(max(_dots(lib, 1, n, t, t, stream)[0], 0.0))
+                c = ops.dots(space, n, X, t)
+                ops.lincomb(space, n, X, -c, 1.0, t)
+        norm = math.sqrt(max(ops.dots(1, n, t, t)[0], 0.0))
         if norm * norm < lindep or norm < 1e-7 * norm0:
             return False
-        _lincomb(lib, 1, n, t, numpy.array([1.0 / norm]), 0.0, X[space], stream)
+        ops.lincomb(1, n, t, numpy.array([1.0 / norm]), 0.0, X[space])
         hop(X[space], AX[space])
         nhop += 1
-        col = _dots(lib, space + 1, n, X, AX[space], stream)
+        col = ops.dots(space + 1, n, X, AX[space])
         heff[:space + 1, space] = col
         heff[space, :space + 1] = col
         space += 1
         return True
 
     for g in x0:
-        t = g.detach().clone().to(torch.float64)
+        t = ops.clone(g)
         if space < cap:
             add_vector(t)
     if space < nroots:
@@ -114,17 +130,17 @@ def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
         e = w[:nroots].copy()
         v = v[:, :nroots]
         for k in range(nroots):
-            _lincomb(lib, space, n, X, v[:, k], 0.0, xr[k], stream)
-            _lincomb(lib, space, n, AX, v[:, k], 0.0, axr[k], stream)
+            ops.lincomb(space, n, X, v[:, k], 0.0, xr[k])
+            ops.lincomb(space, n, AX, v[:, k], 0.0, axr[k])
         de = e - e_old
         new_vectors = []
         for k in range(nroots):
-            _lincomb(lib, 1, n, axr[k], numpy.array([1.0]), 0.0, r, stream)
-            _lincomb(lib, 1, n, xr[k], numpy.array([-e[k]]), 1.0, r, stream)
-            rnorm[k] = math.sqrt(max(_dots(lib, 1, n, r, r, stream)[0], 0.0))
+            ops.lincomb(1, n, axr[k], numpy.array([1.0]), 0.0, r)
+            ops.lincomb(1, n, xr[k], numpy.array([-e[k]]), 1.0, r)
+            rnorm[k] = math.sqrt(max(ops.dots(1, n, r, r)[0], 0.0))
             conv[k] = abs(de[k]) < tol and rnorm[k] < tol_residual
             if not conv[k]:
-                t = torch.empty(n, dtype=torch.float64, device=dev)
+                t = ops.empty((n,), x0[0])
                 precond(r, e[k], t)
                 new_vectors.append(t)
         if verbose:
@@ -136,12 +152,35 @@ def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
         if conv.all():
             break
         if space + len(new_vectors) > cap:
-            # restart from the current Ritz vectors, keeping their images
-            X[:nroots].copy_(xr)
-            AX[:nroots].copy_(axr)
-            space = nroots
+            # Thick restart: keep the current Ritz vectors and the part of the
+            # previous iteration's Ritz vectors orthogonal to them (the
+            # locally optimal direction), formed in subspace coordinates so
+            # their images come from AX without new H applications.
+            keep = v.copy()
+            if v_prev is not None:
+                prev = numpy.zeros((space, v_prev.shape[1]))
+                prev[:v_prev.shape[0]] = v_prev
+                for k in range(prev.shape[1]):
+                    p = prev[:, k] - keep @ (keep.T @ prev[:, k])
+                    p -= keep @ (keep.T @ p)
+                    nrm = numpy.linalg.norm(p)
+                    if nrm > 1e-3 and keep.shape[1] < min(2 * nroots, cap - len(new_vectors)):
+                        keep = numpy.hstack([keep, (p / nrm)[:, None]])
+            nk = keep.shape[1]
+            for k in range(nk - nroots):
+                ops.lincomb(space, n, X, keep[:, nroots + k], 0.0, xp[k])
+                ops.lincomb(space, n, AX, keep[:, nroots + k], 0.0, axp[k])
+            hk = keep.T @ h @ keep
+            ops.copy(X[:nroots], xr)
+            ops.copy(AX[:nroots], axr)
+            for k in range(nk - nroots):
+                ops.copy(X[nroots + k], xp[k])
+                ops.copy(AX[nroots + k], axp[k])
+            space = nk
             heff[:] = 0.0
-            heff[:nroots, :nroots] = numpy.diag(e)
+            heff[:nk, :nk] = 0.5 * (hk + hk.T)
+            v = numpy.eye(nk)[:, :nroots]
+        v_prev = v
         added = 0
         for t in new_vectors:
             if space < cap and add_vector(t):

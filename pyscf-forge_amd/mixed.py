@@ -20,8 +20,8 @@ from . import davidson as _dv
 
 # low-precision stage: residual floor and energy-change tolerance
 LOW_STAGE = {
-    "fp32": dict(tol=1e-7, tol_residual=3e-4),
-    "bf16x3": dict(tol=1e-6, tol_residual=1e-3),
+    "fp32": dict(tol=1e-8, tol_residual=2e-5),
+    "bf16x3": dict(tol=1e-6, tol_residual=5e-4),
 }
 
 

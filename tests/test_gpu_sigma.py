@@ -145,3 +145,35 @@ def test_rdm_identities_c4_size():
     e_rdm = numpy.einsum("pq,pq", h1e, dm1) + 0.5 * numpy.einsum(
         "pqrs,pqrs", oracle.unpack_pair_matrix(eri, n), dm2)
     assert abs(e_rdm - pkg.energy(h1e, eri, c, n, ne)) < 1e-10
+
+
+# Low-precision sigma: measured against the FP64 oracle with tolerances
+# written here (relative to max|sigma|): FP32 t1 contraction 2e-6,
+# bf16 3-term split 2e-4.
+@pytest.mark.parametrize("precision,tol", [("fp32", 2e-6), ("bf16x3", 2e-4)])
+@pytest.mark.parametrize("n,ne,seed", [(8, (4, 4), 0), (10, (5, 4), 1), (12, (6, 6), 2)])
+def test_sigma_low_precision(precision, tol, n, ne, seed):
+    h1e, eri, c = pkg.synthetic.make_problem(n, ne, seed=seed)
+    h2e = oracle.absorb_h1e(h1e, eri, n, ne, 0.5)
+    ref = oracle.contract_2e(h2e, c, n, ne)
+    sig = pkg.contract_2e(h2e, c, n, ne, precision=precision)
+    err = numpy.abs(sig - ref).max() / numpy.abs(ref).max()
+    print("%s n=%d rel err %.3e" % (precision, n, err))
+    assert err < tol
+    assert err > 1e-12  # it really ran in reduced precision
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_mixed_precision_davidson_matches_fp64(precision):
+    n, ne = 10, (5, 5)
+    h1e, eri = pkg.synthetic.make_problem(n, ne, seed=0, with_ci=False)
+    s64 = pkg.FCISolver()
+    e64, c64 = s64.kernel(h1e, eri, n, ne)
+    sm = pkg.FCISolver()
+    sm.precision = precision
+    em, cm = sm.kernel(h1e, eri, n, ne)
+    w, _ = oracle.lowest_eigenvalues(h1e, eri, n, ne)
+    assert sm.converged and s64.converged
+    assert abs(e64 - w[0]) < 1e-9
+    assert abs(em - w[0]) < 1e-8
+    assert sm.stats["nhop_low"] > 0 and sm.stats["nhop_refine"] > 0
