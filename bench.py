@@ -1,0 +1,241 @@
+#!/usr/bin/env python
+"""sigma = H c throughput and Davidson time at (16o,16e) on one MI355X.
+
+Workload: CAS (16o, 16e), nelec = (8, 8): 12870 x 12870 = 165,636,900
+determinants, npair = 136.  Integrals from the synthetic recipe
+(pyscf-forge_amd/synthetic.py, seed 0); the sigma input vector is a seeded
+normal vector generated on the device (the recipe's host draw of 165.6M
+normals is slow and irrelevant to the timing).
+
+One "step" is one full FP64 sigma = H c with c and sigma resident in HBM.
+`value` = FP64 sigma/s.  The same line reports the FP32 and bf16x3
+sigma rates with their error against the FP64 sigma, the FP64 and
+mixed-precision Davidson times to the ground state with their energy
+difference, the roofline of the dominant kernel (the contraction, timed with
+HIP events on its own stream inside this run) and a CPU baseline.
+
+    python bench.py [--steps K] [--warmup W] [--no-davidson] [--no-cpu]
+"""
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}   # dense MFMA TFLOP/s
+HBM_PEAK = 8000.0                                          # GB/s
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def kernel_roofline(plan, sp, prec, calls, sync):
+    """Per-kernel averages from the plan's HIP-event timers."""
+    plan.set_timing(True)
+    out = plan._bench_out
+    for _ in range(calls):
+        plan(plan._bench_in, out)
+    sync()
+    ms, nl = plan.timing()
+    plan.set_timing(False)
+    npair = sp.norb * (sp.norb + 1) // 2
+    s = 8 if prec == "fp64" else 4
+    ndet = sp.ndet
+    per_sigma_launch = nl / calls
+    avg = ms / nl                      # ms per launch
+    dets_per_launch = ndet / per_sigma_launch[1]
+    flop = 2.0 * npair * npair * dets_per_launch
+    gather_bytes = (8.0 + s * npair) * dets_per_launch
+    scatter_bytes = (s * npair + 8.0) * dets_per_launch
+    return {
+        "gather": {"ms": avg[0], "GB/s": gather_bytes / avg[0] / 1e6,
+                   "frac_hbm": gather_bytes / avg[0] / 1e6 / HBM_PEAK},
+        "contract": {"ms": avg[1], "TFLOP/s": flop / avg[1] / 1e9,
+                     "frac_mfma": flop / avg[1] / 1e9 / PEAK[prec],
+                     "flop_per_launch": flop},
+        "scatter": {"ms": avg[2], "GB/s": scatter_bytes / avg[2] / 1e6,
+                    "frac_hbm": scatter_bytes / avg[2] / 1e6 / HBM_PEAK},
+        "launches_per_sigma": per_sigma_launch.tolist(),
+    }
+
+
+def cpu_baseline(pkg, budget_s=12.0):
+    """The reference's own libfci_gas (oracle/_ref, built from the reference
+    sources) on the host cores: full FP64 sigma of the (14o,14e) problem,
+    repeated for ~budget_s, scaled to (16o,16e) sigma/s by the algorithmic
+    work ratio F_sigma(14o)/F_sigma(16o)."""
+    import numpy
+    import oracle
+    if not oracle.ref_available():
+        return None
+    ref = oracle.RefLib()
+    n, ne = 14, (7, 7)
+    h1e, eri = pkg.synthetic.make_problem(n, ne, seed=0, with_ci=False)
+    h2e = oracle.absorb_h1e(h1e, eri, n, ne, 0.5)
+    nstr = math.comb(n, 7)
+    c = numpy.random.default_rng(0).standard_normal(nstr * nstr)
+    c /= numpy.linalg.norm(c)
+    out = numpy.empty_like(c)
+    plan = ref.plan(h2e, n, ne)
+    plan(c, out)
+    t0 = time.perf_counter()
+    reps = 0
+    while True:
+        plan(c, out)
+        reps += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = (time.perf_counter() - t0) / reps
+    plan.close()
+    f1 = 2.0 * (n * (n + 1) // 2) ** 2 * nstr * nstr
+    f2 = 2.0 * 136 ** 2 * math.comb(16, 8) ** 2
+    cores = int(os.environ.get("OMP_NUM_THREADS", 0)) or len(os.sched_getaffinity(0))
+    return {"value": (1.0 / dt) * f1 / f2, "unit": "sigma/s", "cores": cores,
+            "kind": "reference",
+            "sample": "libfci_gas (reference C, OpenMP, -O3 -march=x86-64-v3) FP64 sigma of "
+                      "(14o,14e) [11.8M dets], %d runs of %.2f s, scaled to (16o,16e) by "
+                      "F_sigma ratio %.4f" % (reps, dt, f1 / f2),
+            "sigma_s_14o": dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--norb", type=int, default=16)
+    ap.add_argument("--no-davidson", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precisions", default="fp64,fp32,bf16x3")
+    ap.add_argument("--max-cycle", type=int, default=100)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X: this build has no multi-GPU path "
+              "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import importlib
+    import numpy
+    import torch
+    pkg = importlib.import_module("pyscf-forge_amd")
+    torch.cuda.set_device(0)
+    sync = torch.cuda.synchronize
+
+    n = args.norb
+    ne = (n // 2, n // 2)
+    h1e, eri = pkg.synthetic.make_problem(n, ne, seed=0, with_ci=False)
+    h2e = pkg.absorb_h1e(h1e, eri, n, ne, 0.5)
+    sp = pkg.get_space(n, ne)
+    log("space (%do,%de): %d dets" % (n, sum(ne), sp.ndet))
+    g = torch.Generator(device="cuda").manual_seed(0)
+    c = torch.randn(sp.ndet, dtype=torch.float64, device="cuda", generator=g)
+    c /= torch.linalg.vector_norm(c)
+    out = torch.empty_like(c)
+    npair = n * (n + 1) // 2
+    F = 2.0 * npair * npair * sp.ndet
+
+    result = {}
+    kern = {}
+    sigma64 = None
+    precs = [p for p in args.precisions.split(",") if p]
+    for prec in precs:
+        free = torch.cuda.mem_get_info()[0]
+        plan = pkg.SigmaPlan(sp, h2e, prec, int(free * 0.85))
+        plan._bench_in, plan._bench_out = c, out
+        for _ in range(args.warmup):
+            plan(c, out)
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan(c, out)
+        sync()
+        dt = (time.perf_counter() - t0) / args.steps
+        kern[prec] = kernel_roofline(plan, sp, prec, 2, sync)
+        kern[prec]["ntiles"] = plan.ntiles
+        if prec == "fp64":
+            sigma64 = out.clone()
+            err = 0.0
+        else:
+            err = float(torch.linalg.vector_norm(out - sigma64) / torch.linalg.vector_norm(sigma64)) \
+                if sigma64 is not None else None
+        result[prec] = {"ms": dt * 1e3, "sigma_per_s": 1.0 / dt,
+                        "TFLOP/s": F / dt / 1e12, "rel_err_vs_fp64": err}
+        log("%s sigma %.2f ms (%.2f TFLOP/s eff.), contraction %.2f TFLOP/s, err %s" % (
+            prec, dt * 1e3, F / dt / 1e12, kern[prec]["contract"]["TFLOP/s"], err))
+        plan.close()
+        del plan
+    del out, c, sigma64
+    torch.cuda.empty_cache()
+
+    dav = None
+    if not args.no_davidson:
+        dav = {}
+        e64 = None
+        for prec in ["fp64"] + [p for p in precs if p != "fp64"]:
+            solver = pkg.FCISolver()
+            solver.precision = prec
+            solver.max_cycle = args.max_cycle
+            sync()
+            t0 = time.perf_counter()
+            e, ci = solver.kernel(h1e, eri, n, ne)
+            sync()
+            dt = time.perf_counter() - t0
+            if prec == "fp64":
+                e64 = e
+            dav[prec] = {"seconds": dt, "energy": e, "converged": bool(solver.converged),
+                         "dE_vs_fp64": None if e64 is None else abs(e - e64),
+                         "stats": {k: v for k, v in solver.stats.items()
+                                   if isinstance(v, (int, float, str))}}
+            log("Davidson %s: %.2f s, E = %.12f, dE = %s, %s" % (
+                prec, dt, e, dav[prec]["dE_vs_fp64"], dav[prec]["stats"]))
+            del ci, solver
+            torch.cuda.empty_cache()
+
+    cpu = None
+    if not args.no_cpu:
+        log("CPU baseline (reference libfci_gas on host cores)")
+        cpu = cpu_baseline(pkg)
+
+    main_k = kern["fp64"]["contract"] if "fp64" in kern else None
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_contract_f64_16o16e.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    fp64 = result.get("fp64", {})
+    line = {
+        "metric": "sigma=H*c per second (FP64) at (16o,16e); Davidson time FP64 vs mixed",
+        "value": fp64.get("sigma_per_s"),
+        "unit": "sigma/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": fp64.get("ms"),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "CAS(16o,16e) sigma=H*c, nelec=(8,8), 165636900 dets, "
+                               "npair=136, synthetic integrals seed 0",
+                   "norb": n, "nelec": list(ne), "ndet": sp.ndet},
+        "roofline": None if main_k is None else {
+            "bound": "mfma", "kernel": "k_contract_f64",
+            "achieved": main_k["TFLOP/s"], "peak": PEAK["fp64"], "unit": "TFLOP/s",
+            "frac": main_k["TFLOP/s"] / PEAK["fp64"], "traffic": traffic},
+        "cpu_baseline": cpu,
+        "precisions": result,
+        "kernels": kern,
+        "davidson": dav,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -112,6 +112,14 @@ int fcis_sigma_plan_info(const fcis_sigma_plan_t *plan, int *ntiles,
                          int64_t *tile_alpha, uint64_t *workspace_bytes);
 /* Replace the Hamiltonian of an existing plan (same space). */
 int fcis_sigma_plan_set_h2e(fcis_sigma_plan_t *plan, const double *h2e);
+/* Per-kernel timing with HIP events recorded on the execute stream
+ * (enable != 0 turns it on and clears the counters; execute then waits for
+ * its last event).  ms[0..2] = accumulated gather / contraction / scatter
+ * milliseconds, launches[0..2] = launch counts.  (Measurement hook, no
+ * counterpart in the reference.) */
+int fcis_sigma_plan_set_timing(fcis_sigma_plan_t *plan, int enable);
+int fcis_sigma_plan_get_timing(const fcis_sigma_plan_t *plan, double *ms,
+                               int64_t *launches);
 
 /*
  * Hamiltonian diagonal from the un-absorbed integrals (host h1e, eri pair

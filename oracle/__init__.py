@@ -51,6 +51,8 @@ def _load():
         lib.orc_absorb_h1e.argtypes = [ctypes.c_int, ctypes.c_int,
                                        ctypes.c_double, _dp, _dp, _dp]
         lib.orc_contract_2e.argtypes = [ctypes.c_int] * 3 + [_dp, _dp, _dp]
+        lib.orc_contract_2e_sample.argtypes = [ctypes.c_int] * 3 + [_dp, _dp, _dp,
+                                                                     ctypes.c_int64]
         lib.orc_make_hdiag.argtypes = [ctypes.c_int] * 3 + [_dp, _dp, _dp]
         lib.orc_trans_rdm12s.argtypes = [ctypes.c_int] * 3 + [_dp] * 7
         _lib = lib
@@ -141,6 +143,17 @@ def contract_2e(h2e, fcivec, norb, nelec):
     out = numpy.empty_like(c)
     _load().orc_contract_2e(norb, na, nb, _d(h2e), _d(c), _d(out))
     return out.reshape(shape)
+
+
+def contract_2e_sample(h2e, fcivec, norb, nelec, ncols):
+    """Time-sampling variant of contract_2e: only the first ``ncols`` beta
+    columns' blocks are processed (a partial sigma, for CPU timing)."""
+    na, nb = _unpack_nelec(nelec)
+    h2e = pair_matrix(h2e, norb)
+    c = _c64(fcivec).reshape(-1)
+    out = numpy.empty_like(c)
+    _load().orc_contract_2e_sample(norb, na, nb, _d(h2e), _d(c), _d(out), int(ncols))
+    return out
 
 
 def make_hdiag(h1e, eri, norb, nelec):

@@ -255,8 +255,26 @@ static tlink_t *make_tlinks(int n, int k, int *nlink_out)
  *   ci1[Ia,Jb] += s vt1[pq,b]  (beta links of b)
  *   ci1[Ja,b]  += s vt1[pq,b]  (alpha links of Ia; thread-private buffer)
  * c and ci1 are (nstra, nstrb) row-major. */
+static int contract_2e_cols(int n, int na, int nb, const double *h2e,
+                            const double *ci0, double *ci1, int64_t ncols);
+
 int orc_contract_2e(int n, int na, int nb, const double *h2e,
                     const double *ci0, double *ci1)
+{
+        return contract_2e_cols(n, na, nb, h2e, ci0, ci1, -1);
+}
+
+/* Timing sample: only the first `ncols` beta columns' t1 blocks (whole
+ * beta blocks of 64) are processed, so the cost is that fraction of a full
+ * sigma; ci1 holds a partial result.  Used for the CPU baseline only. */
+int orc_contract_2e_sample(int n, int na, int nb, const double *h2e,
+                           const double *ci0, double *ci1, int64_t ncols)
+{
+        return contract_2e_cols(n, na, nb, h2e, ci0, ci1, ncols);
+}
+
+static int contract_2e_cols(int n, int na, int nb, const double *h2e,
+                            const double *ci0, double *ci1, int64_t ncols)
 {
         int np = n * (n + 1) / 2;
         int64_t nsa = binom64(n, na), nsb = binom64(n, nb);
@@ -270,7 +288,8 @@ int orc_contract_2e(int n, int na, int nb, const double *h2e,
         double *bufs = malloc(sizeof(double) * (size_t)nthr * nsa * STRB_BLK + 8);
         memset(ci1, 0, sizeof(double) * nsa * nsb);
 
-        for (int64_t b0 = 0; b0 < nsb; b0 += STRB_BLK) {
+        int64_t bend = (ncols >= 0 && ncols < nsb) ? ncols : nsb;
+        for (int64_t b0 = 0; b0 < bend; b0 += STRB_BLK) {
                 int bl = (int)(nsb - b0 < STRB_BLK ? nsb - b0 : STRB_BLK);
 #pragma omp parallel
                 {

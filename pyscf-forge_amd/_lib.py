@@ -44,6 +44,8 @@ SIGNATURES = {
     "fcis_sigma_plan_free": (None, [_vp]),
     "fcis_sigma_plan_info": (ctypes.c_int, [_vp, _ip, _i64p, _u64p]),
     "fcis_sigma_plan_set_h2e": (ctypes.c_int, [_vp, _dp]),
+    "fcis_sigma_plan_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "fcis_sigma_plan_get_timing": (ctypes.c_int, [_vp, _dp, _i64p]),
     "fcis_make_hdiag": (ctypes.c_int, [_vp, _dp, _dp, _vp, _vp]),
     "fcis_trans_rdm12s": (ctypes.c_int, [_vp, _vp, _vp, _dp, _dp, _dp, _dp, _dp, _vp]),
     "fcis_dots": (ctypes.c_int, [ctypes.c_int, _i64, _vp, _i64, _vp, _dp, _vp]),

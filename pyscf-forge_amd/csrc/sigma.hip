@@ -324,6 +324,10 @@ struct fcis_sigma_plan {
         void *d_t1, *d_vt1;
         uint64_t ws_bytes;
         std::vector<double> h;
+        int timing;
+        hipEvent_t ev[4];
+        double ms[3];
+        int64_t launches[3];
 };
 
 namespace {
@@ -508,12 +512,15 @@ int fcis_sigma_plan_execute(fcis_sigma_plan_t *p, const double *ci0, double *ci1
                 const int64_t ld = ndet;
                 const dim3 ggrid((ndet + GEMM_NCOL - 1) / GEMM_NCOL, (p->mtiles + GEMM_MT - 1) / GEMM_MT);
                 const dim3 rgrid(ta * ((nb + 255) / 256)), sgrid(na * ((nb + 255) / 256));
+                if (p->timing) FCIS_HIP(hipEventRecord(p->ev[0], st));
                 if (p->precision == FCIS_PREC_FP64) {
                         double *t1 = (double *)p->d_t1, *vt1 = (double *)p->d_vt1;
                         hipLaunchKernelGGL(k_gather<double>, rgrid, dim3(256), 0, st, p->npair, nb, a0,
                                            na, ld, p->d_pmapa, p->d_pmapb, ci0, t1);
+                        if (p->timing) FCIS_HIP(hipEventRecord(p->ev[1], st));
                         hipLaunchKernelGGL(k_contract_f64, ggrid, gblk, 0, st, p->npair, p->mtiles,
                                            p->ksteps, ndet, ld, (const double *)p->d_hfrag, t1, vt1);
+                        if (p->timing) FCIS_HIP(hipEventRecord(p->ev[2], st));
                         hipLaunchKernelGGL(k_scatter<double>, sgrid, dim3(256), 0, st, nb, na, a0, ta,
                                            ld, s->nlinka, s->nlinkb, t > 0 ? 1 : 0, s->d_linka,
                                            s->d_linkb, vt1, ci1);
@@ -521,6 +528,7 @@ int fcis_sigma_plan_execute(fcis_sigma_plan_t *p, const double *ci0, double *ci1
                         float *t1 = (float *)p->d_t1, *vt1 = (float *)p->d_vt1;
                         hipLaunchKernelGGL(k_gather<float>, rgrid, dim3(256), 0, st, p->npair, nb, a0,
                                            na, ld, p->d_pmapa, p->d_pmapb, ci0, t1);
+                        if (p->timing) FCIS_HIP(hipEventRecord(p->ev[1], st));
                         if (p->precision == FCIS_PREC_FP32)
                                 hipLaunchKernelGGL(k_contract_f32, ggrid, gblk, 0, st, p->npair,
                                                    p->mtiles, p->ksteps, ndet, ld,
@@ -530,11 +538,49 @@ int fcis_sigma_plan_execute(fcis_sigma_plan_t *p, const double *ci0, double *ci1
                                                    p->mtiles, p->ksteps, ndet, ld,
                                                    (const bf16x8 *)p->d_hfrag,
                                                    (const bf16x8 *)p->d_hfrag_lo, t1, vt1);
+                        if (p->timing) FCIS_HIP(hipEventRecord(p->ev[2], st));
                         hipLaunchKernelGGL(k_scatter<float>, sgrid, dim3(256), 0, st, nb, na, a0, ta,
                                            ld, s->nlinka, s->nlinkb, t > 0 ? 1 : 0, s->d_linka,
                                            s->d_linkb, vt1, ci1);
                 }
                 FCIS_CHECK_LAUNCH();
+                if (p->timing) {
+                        FCIS_HIP(hipEventRecord(p->ev[3], st));
+                        FCIS_HIP(hipEventSynchronize(p->ev[3]));
+                        for (int k = 0; k < 3; k++) {
+                                float ms = 0.0f;
+                                FCIS_HIP(hipEventElapsedTime(&ms, p->ev[k], p->ev[k + 1]));
+                                p->ms[k] += ms;
+                                p->launches[k] += 1;
+                        }
+                }
+        }
+        return FCIS_SUCCESS;
+}
+
+int fcis_sigma_plan_set_timing(fcis_sigma_plan_t *p, int enable)
+{
+        if (p == nullptr) return FCIS_ERR_INVALID;
+        if (enable && !p->timing) {
+                for (int k = 0; k < 4; k++) FCIS_HIP(hipEventCreate(&p->ev[k]));
+        }
+        if (!enable && p->timing) {
+                for (int k = 0; k < 4; k++) (void)hipEventDestroy(p->ev[k]);
+        }
+        p->timing = enable ? 1 : 0;
+        for (int k = 0; k < 3; k++) {
+                p->ms[k] = 0.0;
+                p->launches[k] = 0;
+        }
+        return FCIS_SUCCESS;
+}
+
+int fcis_sigma_plan_get_timing(const fcis_sigma_plan_t *p, double *ms, int64_t *launches)
+{
+        if (p == nullptr) return FCIS_ERR_INVALID;
+        for (int k = 0; k < 3; k++) {
+                if (ms) ms[k] = p->ms[k];
+                if (launches) launches[k] = p->launches[k];
         }
         return FCIS_SUCCESS;
 }
@@ -542,6 +588,7 @@ int fcis_sigma_plan_execute(fcis_sigma_plan_t *p, const double *ci0, double *ci1
 void fcis_sigma_plan_free(fcis_sigma_plan_t *p)
 {
         if (p == nullptr) return;
+        if (p->timing) fcis_sigma_plan_set_timing(p, 0);
         (void)hipFree(p->d_hfrag);
         (void)hipFree(p->d_hfrag_lo);
         (void)hipFree(p->d_pmapa);

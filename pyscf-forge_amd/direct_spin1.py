@@ -13,6 +13,7 @@ tensors stay on the device.
 Every call runs the HIP library (libfcisigma.so); there is no CPU path.
 """
 
+import ctypes
 import math
 
 import numpy
@@ -174,6 +175,19 @@ class SigmaPlan:
                                                     _lib.stream_ptr()),
                    "fcis_sigma_plan_execute")
         return out
+
+    def set_timing(self, enable=True):
+        """Per-kernel HIP-event timing of gather / contraction / scatter."""
+        _lib.check(self.lib.fcis_sigma_plan_set_timing(self.handle, 1 if enable else 0),
+                   "fcis_sigma_plan_set_timing")
+
+    def timing(self):
+        """(ms[3], launches[3]) accumulated since set_timing(True)."""
+        ms = numpy.zeros(3)
+        n = (ctypes.c_int64 * 3)()
+        _lib.check(self.lib.fcis_sigma_plan_get_timing(self.handle, _lib.dptr(ms), n),
+                   "fcis_sigma_plan_get_timing")
+        return ms, numpy.array(list(n))
 
     def close(self):
         if getattr(self, "handle", None) is not None:
