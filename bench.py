@@ -114,6 +114,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precisions", default="fp64,fp32,bf16x3")
     ap.add_argument("--max-cycle", type=int, default=100)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a 4 GiB device copy first (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path "
@@ -126,6 +128,16 @@ def main():
     pkg = importlib.import_module("pyscf-forge_amd")
     torch.cuda.set_device(0)
     sync = torch.cuda.synchronize
+
+    if args.copy_calibration:
+        x = torch.ones(1 << 29, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(2):
+            y.copy_(x)
+        sync()
+        log("copy calibration: 2 x 4 GiB device copies")
+        del x, y
+        torch.cuda.empty_cache()
 
     n = args.norb
     ne = (n // 2, n // 2)
