@@ -53,16 +53,22 @@ def kernel_roofline(plan, sp, prec, calls, sync):
     flop = 2.0 * npair * npair * dets_per_launch
     gather_bytes = (8.0 + s * npair) * dets_per_launch
     scatter_bytes = (s * npair + 8.0) * dets_per_launch
-    return {
-        "gather": {"ms": avg[0], "GB/s": gather_bytes / avg[0] / 1e6,
-                   "frac_hbm": gather_bytes / avg[0] / 1e6 / HBM_PEAK},
+    out = {
         "contract": {"ms": avg[1], "TFLOP/s": flop / avg[1] / 1e9,
                      "frac_mfma": flop / avg[1] / 1e9 / PEAK[prec],
-                     "flop_per_launch": flop},
+                     "flop_per_launch": flop, "dets_per_launch": dets_per_launch},
         "scatter": {"ms": avg[2], "GB/s": scatter_bytes / avg[2] / 1e6,
-                    "frac_hbm": scatter_bytes / avg[2] / 1e6 / HBM_PEAK},
+                    "frac_hbm": scatter_bytes / avg[2] / 1e6 / HBM_PEAK,
+                    "bytes_per_launch": scatter_bytes},
         "launches_per_sigma": per_sigma_launch.tolist(),
     }
+    if avg[0] > 1e-3:   # separate gather kernel (unfused path)
+        out["gather"] = {"ms": avg[0], "GB/s": gather_bytes / avg[0] / 1e6,
+                         "frac_hbm": gather_bytes / avg[0] / 1e6 / HBM_PEAK,
+                         "bytes_per_launch": gather_bytes}
+    else:               # gather fused into the contraction kernel
+        out["contract"]["fused_gather"] = True
+    return out
 
 
 def cpu_baseline(pkg, budget_s=12.0):
@@ -238,7 +244,7 @@ def main():
                                "npair=136, synthetic integrals seed 0",
                    "norb": n, "nelec": list(ne), "ndet": sp.ndet},
         "roofline": None if main_k is None else {
-            "bound": "mfma", "kernel": "k_contract_f64",
+            "bound": "mfma", "kernel": "k_fused<fp64>" if main_k.get("fused_gather") else "k_contract_f64",
             "achieved": main_k["TFLOP/s"], "peak": PEAK["fp64"], "unit": "TFLOP/s",
             "frac": main_k["TFLOP/s"] / PEAK["fp64"], "traffic": traffic},
         "cpu_baseline": cpu,

@@ -52,8 +52,10 @@ __host__ __device__ static inline int pair_index(int p, int q)
 }  // namespace fcis
 
 // A compact link: target string address and a packed descriptor
-//   info = neg | pair << 1 | des << 12 | cre << 18
-// for a_cre^+ a_des |str> = (-1)^neg |addr>, pair = pair_index(cre, des).
+//   info = neg | pair << 1 | des << 12 | cre << 18 | rslot << 24
+// for a_cre^+ a_des |str> = (-1)^neg |addr>, pair = pair_index(cre, des);
+// rslot is the slot of the reverse link (a_des^+ a_cre |addr> = same sign
+// |str>) in addr's own list (valid when nlink <= 255).
 // Stored transposed, link[k * nstr + str], so that a wavefront reading the
 // k-th link of 64 consecutive strings issues one coalesced load.
 struct fcis_link {
@@ -65,9 +67,10 @@ __host__ __device__ static inline int link_neg(int32_t info) { return info & 1; 
 __host__ __device__ static inline int link_pair(int32_t info) { return (info >> 1) & 0x7ff; }
 __host__ __device__ static inline int link_des(int32_t info) { return (info >> 12) & 0x3f; }
 __host__ __device__ static inline int link_cre(int32_t info) { return (info >> 18) & 0x3f; }
-__host__ __device__ static inline int32_t link_pack(int cre, int des, int pair, int neg)
+__host__ __device__ static inline int link_rslot(int32_t info) { return (info >> 24) & 0xff; }
+__host__ __device__ static inline int32_t link_pack(int cre, int des, int pair, int neg, int rslot)
 {
-        return neg | (pair << 1) | (des << 12) | (cre << 18);
+        return neg | (pair << 1) | (des << 12) | (cre << 18) | ((rslot & 0xff) << 24);
 }
 
 struct fcis_space {

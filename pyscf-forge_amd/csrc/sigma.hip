@@ -135,8 +135,8 @@ __global__ __launch_bounds__(256) void k_contract_f64(int npair, int mtiles, int
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
-// FP32 MFMA contraction (v_mfma_f32_16x16x4_f32, same lane layout as the
-// f64 form); t1 and vt1 are float, h fragments are float.
+// FP32 MFMA contraction (v_mfma_f32_16x16x4_f32: A and B as in the f64
+// form, D[i][j] in lane j + 16*(i/4), element i%4); t1, vt1, h are float.
 __global__ __launch_bounds__(256) void k_contract_f32(int npair, int mtiles, int ksteps,
                                                       int64_t ndet, int64_t ld,
                                                       const float *__restrict__ hfrag,
@@ -266,6 +266,389 @@ __global__ __launch_bounds__(256) void k_contract_bf16x3(int npair, int mtiles, 
         }
 }
 
+// ---------------------------------------------------------------------------
+// Fused row kernel: gather + MFMA contraction + beta scatter for whole
+// alpha strings.
+//
+// A workgroup owns one alpha string I at a time (persistent over rows) and
+// walks its nb determinants in chunks of FUSED_NB.  It has one wavefront per
+// 16-row tile of h (mtiles <= FUSED_MAXW); each wavefront keeps its tile's A
+// fragments for every k-step in registers, so the MFMA loop reads only B
+// fragments, from LDS.  The B tile (npair x FUSED_NB) is gathered from c
+// straight into LDS,
+//   t1[pq][b] = sa c[Ja, b] + sb c[I, Jb]   (pmapa[pq][I] -> (Ja, sa),
+//                                             pmapb[pq][b] -> (Jb, sb)),
+// double-buffered and software-pipelined (pair maps two chunks ahead, c one
+// chunk ahead of the MFMAs), so t1 never touches HBM.  Of the product
+// vt1 = h t1:
+//   * the beta part of the scatter stays on chip: sigma[I, Jb] +=
+//     sb vt1[pq][b] for the beta link (pq -> Jb) of b, accumulated into the
+//     LDS copy of sigma row I with LDS float atomics (the only
+//     non-deterministic summation order in the engine; see the
+//     deterministic three-kernel path);
+//   * the alpha part is written to HBM in alpha-link-slot layout,
+//     walpha[k][I][b] for the k-th alpha link of I (72 of the 136 pair
+//     rows at norb = 16), and pulled by k_alpha_pull.
+// Independent accumulators per k-step phase (FP64/FP32) or per split term
+// (bf16x3) keep several MFMA chains in flight per wavefront.
+constexpr int FUSED_NB = 16;      // determinants per chunk (one MFMA column block)
+constexpr int FUSED_MAXW = 9;     // wavefronts per workgroup (norb <= 16)
+constexpr int FUSED_KS = 34;      // k-steps held in registers (4-wide MFMA): npair <= 136
+constexpr int FUSED_KS_BF16 = 5;  // k-steps held in registers (32-wide bf16 MFMA)
+constexpr int FUSED_NACC = 2;     // independent accumulator chains (FP64/FP32)
+constexpr int FUSED_MAXNB = 16384;  // sigma row kept in LDS; pmapb fits int16
+
+template <int PREC>
+struct FusedTypes;
+template <>
+struct FusedTypes<FCIS_PREC_FP64> {
+        typedef double B;
+        typedef double A;
+        typedef v4d Acc;
+        typedef double Out;
+        static constexpr int KS = FUSED_KS;
+        static constexpr int KW = 4;
+        static constexpr int NBLD = 3;    // 9 + 3 wavefronts: 3 per SIMD at <= 168 VGPRs
+};
+template <>
+struct FusedTypes<FCIS_PREC_FP32> {
+        typedef float B;
+        typedef float A;
+        typedef v4f Acc;
+        typedef float Out;
+        static constexpr int KS = FUSED_KS;
+        static constexpr int KW = 4;
+        static constexpr int NBLD = 7;    // 9 + 7 wavefronts: 4 per SIMD at <= 128 VGPRs
+};
+template <>
+struct FusedTypes<FCIS_PREC_BF16X3> {
+        typedef float B;
+        typedef bf16x8 A;
+        typedef v4f Acc;
+        typedef float Out;
+        static constexpr int KS = FUSED_KS_BF16;
+        static constexpr int KW = 32;
+        static constexpr int NBLD = 7;
+};
+
+template <int PREC>
+constexpr size_t fused_lds_bytes(int64_t nb)
+{
+        typedef typename FusedTypes<PREC>::B BT;
+        return (size_t)((nb + FUSED_NB - 1) / FUSED_NB * FUSED_NB) * sizeof(double) +
+               2 * sizeof(BT) * FusedTypes<PREC>::KS * FusedTypes<PREC>::KW * FUSED_NB +
+               2 * sizeof(int) * 16 * FUSED_MAXW + 3 * sizeof(int16_t) * 16 * FUSED_MAXW * FUSED_NB;
+}
+
+// Builder wavefronts: gather the B tile of chunk j+1 (pair maps, then c)
+// while the MFMA wavefronts consume chunk j.
+// Builder state of one chunk: the B-tile element values in flight and the
+// next-but-one chunk's beta pair-map slice in flight.
+template <int NBLD>
+struct FusedBuild {
+        static constexpr int EB = (16 * FUSED_MAXW * FUSED_NB + 64 * NBLD - 1) / (64 * NBLD);
+        static constexpr int EM = (2 * 16 * FUSED_MAXW + 64 * NBLD - 1) / (64 * NBLD);
+        int pa[EB], pb[EB];
+        double va[EB], vb[EB];
+        int4 mv[EM];          // 8 int16 map entries per 16-byte load
+};
+
+// Issue the c loads of chunk j (element (k, col) of the tile is
+//   sa c[Ja, b0+col] + sb c[I, Jb],  arow[k] -> (Ja, sa), mb[k][col] -> (Jb, sb)),
+// addressed with 32-bit byte offsets (the fused path needs c below 4 GiB).
+template <int NBLD>
+__device__ __forceinline__ void fb_issue_c(FusedBuild<NBLD> &st, int npair, int64_t nb, int64_t I,
+                                           int64_t j, int bt, const int *__restrict__ arow,
+                                           const int16_t *__restrict__ mb,
+                                           const double *__restrict__ c)
+{
+        constexpr int NB = FUSED_NB, EB = FusedBuild<NBLD>::EB;
+        const int nbt = NBLD * 64, nelem = npair * NB;
+        const uint32_t b0 = (uint32_t)(j * NB);
+        const char *cb = reinterpret_cast<const char *>(c);
+        const uint32_t rowI = (uint32_t)(I * nb);
+#pragma unroll
+        for (int s = 0; s < EB; s++) {
+                int e = bt + s * nbt;
+                st.pa[s] = -1;
+                st.pb[s] = -1;
+                if (e < nelem) {
+                        int k = e / NB;
+                        if (b0 + (uint32_t)(e - k * NB) < (uint32_t)nb) {
+                                st.pa[s] = arow[k];
+                                st.pb[s] = mb[e];   // -1 where b has no link with pair k
+                        }
+                }
+        }
+#pragma unroll
+        for (int s = 0; s < EB; s++) {
+                int col = (bt + s * nbt) % NB;
+                uint32_t oa = ((uint32_t)(st.pa[s] >> 1) * (uint32_t)nb + b0 + col) * 8u;
+                uint32_t ob = (rowI + (uint32_t)(st.pb[s] >> 1)) * 8u;
+                st.va[s] = st.pa[s] >= 0 ? *reinterpret_cast<const double *>(cb + oa) : 0.0;
+                st.vb[s] = st.pb[s] >= 0 ? *reinterpret_cast<const double *>(cb + ob) : 0.0;
+        }
+}
+
+template <int NBLD, typename BT>
+__device__ __forceinline__ void fb_commit_c(FusedBuild<NBLD> &st, int npair, int bt, BT *bb)
+{
+        constexpr int EB = FusedBuild<NBLD>::EB;
+        const int nbt = NBLD * 64, nelem = npair * FUSED_NB;
+#pragma unroll
+        for (int s = 0; s < EB; s++) {
+                int e = bt + s * nbt;
+                if (e < nelem) {
+                        double v = ((st.pa[s] & 1) ? -st.va[s] : st.va[s]) +
+                                   ((st.pb[s] & 1) ? -st.vb[s] : st.vb[s]);
+                        bb[e] = (BT)v;
+                }
+        }
+}
+
+// Beta pair-map slice of chunk j, pmapb[k][b0 .. b0+NB) for k < npair (16
+// int16 = two 16-byte loads per pair; pmapb rows are padded to a multiple of
+// FUSED_NB entries with -1, so no load crosses a row).
+template <int NBLD>
+__device__ __forceinline__ void fb_issue_maps(FusedBuild<NBLD> &st, int npair, int64_t nbp,
+                                              int64_t j, int bt, const int16_t *__restrict__ pmapb)
+{
+        constexpr int EM = FusedBuild<NBLD>::EM;
+        const int64_t b0 = j * FUSED_NB;
+#pragma unroll
+        for (int s = 0; s < EM; s++) {
+                int t = bt + s * NBLD * 64;
+                if (t < 2 * npair) {
+                        int k = t >> 1, h = (t & 1) * 8;
+                        st.mv[s] = *reinterpret_cast<const int4 *>(pmapb + (int64_t)k * nbp + b0 + h);
+                }
+        }
+}
+
+template <int NBLD>
+__device__ __forceinline__ void fb_commit_maps(FusedBuild<NBLD> &st, int npair, int bt,
+                                               int16_t *__restrict__ mb)
+{
+        constexpr int EM = FusedBuild<NBLD>::EM;
+#pragma unroll
+        for (int s = 0; s < EM; s++) {
+                int t = bt + s * NBLD * 64;
+                if (t < 2 * npair) {
+                        int k = t >> 1, h = (t & 1) * 8;
+                        *reinterpret_cast<int4 *>(mb + k * FUSED_NB + h) = st.mv[s];
+                }
+        }
+}
+
+template <int PREC>
+__global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_fused_row(
+        int npair, int mtiles, int ksteps, int nlinka, int64_t nb, int64_t nstra, int64_t a0,
+        int64_t ta, int64_t ld, int accumulate, const fcis_link *__restrict__ linka,
+        const int32_t *__restrict__ pmapa, const int16_t *__restrict__ pmapb, int64_t nbp,
+        const double *__restrict__ c, const void *__restrict__ hfrag_v,
+        const void *__restrict__ hfrag_lo_v, typename FusedTypes<PREC>::Out *__restrict__ walpha,
+        double *__restrict__ sigma)
+{
+        typedef FusedTypes<PREC> Ty;
+        typedef typename Ty::B BT;
+        constexpr int KS = Ty::KS, KW = Ty::KW, NB = FUSED_NB;
+        constexpr int KPMAX = KS * KW;
+        constexpr int MAXP = 16 * FUSED_MAXW;
+        extern __shared__ __align__(16) char smem[];
+        double *srow = reinterpret_cast<double *>(smem);   // nbp doubles: 16-byte aligned end
+        BT *buf0 = reinterpret_cast<BT *>(smem + (size_t)nbp * sizeof(double));
+        BT *buf1 = buf0 + KPMAX * NB;
+        int *arow = reinterpret_cast<int *>(buf1 + KPMAX * NB);   // pmapa[pq][I]
+        int *aslot = arow + MAXP;                                  // alpha slot of pq, or -1
+        int16_t *maps = reinterpret_cast<int16_t *>(aslot + MAXP); // 3 x [MAXP][NB] beta maps
+
+        const int tid = threadIdx.x, nthr = blockDim.x;
+        const int wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
+        const bool mfma_wave = wave < mtiles;
+        const int kp = ksteps * KW;
+        const int64_t nch = (nb + NB - 1) / NB;
+
+        for (int e = npair * NB + tid; e < kp * NB; e += nthr) {
+                buf0[e] = (BT)0;
+                buf1[e] = (BT)0;
+        }
+        for (int64_t i = tid; i < nb; i += nthr) srow[i] = 0.0;
+
+        // Barrier schedule per row (both roles): (1) row start, (2) arow,
+        // (3) aslot + maps of chunks 0,1, (4) chunk 0 built, then one per chunk.
+        if (!mfma_wave) {
+                // ---- builder wavefronts
+                const int bt = tid - mtiles * 64;
+                FusedBuild<Ty::NBLD> st;
+                for (int64_t r = blockIdx.x; r < ta; r += gridDim.x) {
+                        const int64_t I = a0 + r;
+                        __syncthreads();   // (1)
+                        for (int q = bt; q < npair; q += Ty::NBLD * 64) {
+                                arow[q] = pmapa[(int64_t)q * nstra + I];
+                                aslot[q] = -1;
+                        }
+                        __syncthreads();   // (2)
+                        for (int k = bt; k < nlinka; k += Ty::NBLD * 64)
+                                aslot[link_pair(linka[(int64_t)k * nstra + I].info)] = k;
+                        fb_issue_maps(st, npair, nbp, 0, bt, pmapb);
+                        fb_commit_maps(st, npair, bt, maps);
+                        if (nch > 1) {
+                                fb_issue_maps(st, npair, nbp, 1, bt, pmapb);
+                                fb_commit_maps(st, npair, bt, maps + MAXP * NB);
+                        }
+                        __syncthreads();   // (3)
+                        fb_issue_c(st, npair, nb, I, 0, bt, arow, maps, c);
+                        fb_commit_c(st, npair, bt, buf0);
+                        __syncthreads();   // (4)
+                        for (int64_t j = 0; j < nch; j++) {
+                                // both round trips in flight together: c for chunk j+1,
+                                // pair maps for chunk j+2
+                                if (j + 1 < nch)
+                                        fb_issue_c(st, npair, nb, I, j + 1, bt, arow,
+                                                   maps + ((j + 1) % 3) * MAXP * NB, c);
+                                if (j + 2 < nch) fb_issue_maps(st, npair, nbp, j + 2, bt, pmapb);
+                                if (j + 1 < nch) fb_commit_c(st, npair, bt, ((j + 1) & 1) ? buf1 : buf0);
+                                if (j + 2 < nch)
+                                        fb_commit_maps(st, npair, bt, maps + ((j + 2) % 3) * MAXP * NB);
+                                __syncthreads();
+                        }
+                        double *dst = sigma + I * nb;
+                        for (int64_t i = tid; i < nb; i += nthr) {
+                                double v = srow[i];
+                                dst[i] = accumulate ? dst[i] + v : v;
+                                srow[i] = 0.0;
+                        }
+                }
+                return;
+        }
+
+        // ---- MFMA wavefronts: this wavefront's 16-row tile of h in registers
+        typename Ty::A ah[KS];
+        typename Ty::A al[PREC == FCIS_PREC_BF16X3 ? KS : 1];
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+                if (ks < ksteps) {
+                        size_t o = ((size_t)ks * mtiles + wave) * 64 + lane;
+                        ah[ks] = ((const typename Ty::A *)hfrag_v)[o];
+                        if constexpr (PREC == FCIS_PREC_BF16X3)
+                                al[ks] = ((const typename Ty::A *)hfrag_lo_v)[o];
+                }
+        }
+        for (int64_t r = blockIdx.x; r < ta; r += gridDim.x) {
+                const int64_t I = a0 + r;
+                __syncthreads();   // (1)
+                __syncthreads();   // (2)
+                __syncthreads();   // (3)
+                __syncthreads();   // (4)
+                for (int64_t j = 0; j < nch; j++) {
+                        const BT *bb = (j & 1) ? buf1 : buf0;
+                        const int16_t *mb = maps + (j % 3) * MAXP * NB;
+                        typename Ty::Acc acc;
+                        if constexpr (PREC == FCIS_PREC_BF16X3) {
+                                typename Ty::Acc a0c = {0, 0, 0, 0}, a1c = a0c, a2c = a0c;
+#pragma unroll
+                                for (int ks = 0; ks < KS; ks++) {
+                                        if (ks < ksteps) {
+                                                bf16x8 bh, bl;
+#pragma unroll
+                                                for (int e = 0; e < 8; e++) {
+                                                        float xv = bb[(32 * ks + 8 * lk + e) * NB + li];
+                                                        __bf16 hi = (__bf16)xv;
+                                                        bh[e] = hi;
+                                                        bl[e] = (__bf16)(xv - (float)hi);
+                                                }
+                                                a0c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[ks], bh, a0c, 0, 0, 0);
+                                                a1c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ks], bl, a1c, 0, 0, 0);
+                                                a2c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ks], bh, a2c, 0, 0, 0);
+                                        }
+                                }
+                                acc = (a0c + a1c) + a2c;
+                        } else {
+                                typename Ty::Acc ac[FUSED_NACC];
+#pragma unroll
+                                for (int z = 0; z < FUSED_NACC; z++) ac[z] = typename Ty::Acc{0, 0, 0, 0};
+#pragma unroll
+                                for (int ks = 0; ks < KS; ks++) {
+                                        if (ks < ksteps) {
+                                                BT bv = bb[(4 * ks + lk) * NB + li];
+                                                if constexpr (PREC == FCIS_PREC_FP64)
+                                                        ac[ks % FUSED_NACC] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                                                                ah[ks], bv, ac[ks % FUSED_NACC], 0, 0, 0);
+                                                else
+                                                        ac[ks % FUSED_NACC] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                                                                ah[ks], bv, ac[ks % FUSED_NACC], 0, 0, 0);
+                                        }
+                                }
+                                acc = ac[0] + ac[1];
+                        }
+                        const int64_t b = j * NB + li;
+                        if (b < nb) {
+#pragma unroll
+                                for (int q = 0; q < 4; q++) {
+                                        // f64 MFMA: D row lk + 4q; f32/bf16 MFMA: D row 4lk + q
+                                        int pq = wave * 16 + (PREC == FCIS_PREC_FP64 ? lk + 4 * q : 4 * lk + q);
+                                        if (pq < npair) {
+                                                double v = (double)acc[q];
+                                                int e = mb[pq * NB + li];
+                                                if (e >= 0) atomicAdd(&srow[e >> 1], (e & 1) ? -v : v);
+                                                int k = aslot[pq];
+                                                if (k >= 0) walpha[(int64_t)k * ld + r * nb + b] = acc[q];
+                                        }
+                                }
+                        }
+                        __syncthreads();
+                }
+                double *dst = sigma + I * nb;
+                for (int64_t i = tid; i < nb; i += nthr) {
+                        double v = srow[i];
+                        dst[i] = accumulate ? dst[i] + v : v;
+                        srow[i] = 0.0;
+                }
+        }
+}
+
+// Alpha pull for one alpha tile [a0, a0+ta): one thread per destination
+// determinant (J, Jb), adding sum_k s_k walpha[rslot_k][I_k - a0][Jb] over
+// the alpha links k of J whose target I_k lies in the tile; the reverse
+// slot rslot_k is the slot of the link I_k -> J in I_k's list.
+template <typename T>
+__global__ __launch_bounds__(256) void k_alpha_pull(int64_t nb, int64_t nstra, int64_t a0,
+                                                    int64_t ta, int64_t ld, int nlinka,
+                                                    const fcis_link *__restrict__ linka,
+                                                    const T *__restrict__ walpha,
+                                                    double *__restrict__ sigma)
+{
+        const int64_t nchunk = (nb + 255) / 256;
+        const int64_t J = blockIdx.x / nchunk;
+        const int64_t Jb = (blockIdx.x - J * nchunk) * 256 + threadIdx.x;
+        if (Jb >= nb) return;
+        double acc = 0.0;
+        for (int k = 0; k < nlinka; k++) {
+                fcis_link L = linka[(int64_t)k * nstra + J];
+                int64_t I = L.addr;
+                if (I < a0 || I >= a0 + ta) continue;
+                double v = (double)walpha[(int64_t)link_rslot(L.info) * ld + (I - a0) * nb + Jb];
+                acc += link_neg(L.info) ? -v : v;
+        }
+        sigma[J * nb + Jb] += acc;
+}
+
+__global__ void k_zero(int64_t n, double *__restrict__ x)
+{
+        int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (i < n) x[i] = 0.0;
+}
+
+// int16 copy of pmapb with rows padded to nbp entries (-1 padding).
+__global__ void k_pmap16(int npair, int64_t nb, int64_t nbp, const int32_t *__restrict__ in,
+                         int16_t *__restrict__ out)
+{
+        int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= (int64_t)npair * nbp) return;
+        int64_t k = i / nbp, b = i - k * nbp;
+        out[i] = b < nb ? (int16_t)in[k * nb + b] : (int16_t)-1;
+}
+
 // Pull scatter for one alpha tile [a0, a0+ta): one thread per destination
 // determinant (J, Jb).  accumulate = 0 on the first tile (sigma overwritten).
 template <typename T>
@@ -324,6 +707,12 @@ struct fcis_sigma_plan {
         void *d_t1, *d_vt1;
         uint64_t ws_bytes;
         std::vector<double> h;
+        int fused;                   // fused row kernel + alpha pull
+        int fused_grid;              // persistent workgroups of the fused kernel
+        size_t fused_lds;            // its dynamic LDS bytes
+        int16_t *d_pmapb16;          // beta pair map as int16, rows padded to nbp (fused path)
+        int64_t nbp;
+        int nbld;                    // builder wavefronts of the fused kernel
         int timing;
         hipEvent_t ev[4];
         double ms[3];
@@ -427,15 +816,46 @@ int fcis_sigma_plan_create(fcis_sigma_plan_t **out, const fcis_space_t *space,
         const size_t elem = precision == FCIS_PREC_FP64 ? 8 : 4;
         const int64_t nb = space->nstrb;
         if (workspace_bytes == 0) workspace_bytes = (uint64_t)16 << 30;
-        // two buffers (t1, vt1) of npair x tile_a x nb elements
-        uint64_t per_alpha = 2ull * p->npair * (uint64_t)nb * elem;
+        // The fused row path keeps h in registers (npair <= 136), sigma row I
+        // in LDS (nb <= 16384, int16 beta pair map) and needs reverse link
+        // slots (nlink <= 255).  FCIS_DETERMINISTIC selects the bitwise
+        // reproducible three-kernel path instead.
+        const size_t lds = precision == FCIS_PREC_FP64 ? fused_lds_bytes<FCIS_PREC_FP64>(nb)
+                         : precision == FCIS_PREC_FP32 ? fused_lds_bytes<FCIS_PREC_FP32>(nb)
+                                                        : fused_lds_bytes<FCIS_PREC_BF16X3>(nb);
+        p->fused = p->mtiles <= FUSED_MAXW &&
+                   p->ksteps <= (precision == FCIS_PREC_BF16X3 ? FUSED_KS_BF16 : FUSED_KS) &&
+                   nb < FUSED_MAXNB && space->nlinka <= 255 && lds <= 160 * 1024 &&
+                   (uint64_t)space->nstra * (uint64_t)nb * 8ull < (1ull << 32) &&
+                   !getenv("FCIS_DETERMINISTIC");
+        if (p->fused) {
+                int dev = 0, ncu = 0, occ = 0;
+                FCIS_HIP(hipGetDevice(&dev));
+                FCIS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+                const void *fk = precision == FCIS_PREC_FP64 ? (const void *)k_fused_row<FCIS_PREC_FP64>
+                               : precision == FCIS_PREC_FP32 ? (const void *)k_fused_row<FCIS_PREC_FP32>
+                                                              : (const void *)k_fused_row<FCIS_PREC_BF16X3>;
+                FCIS_HIP(hipFuncSetAttribute(fk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                p->nbld = precision == FCIS_PREC_FP64 ? FusedTypes<FCIS_PREC_FP64>::NBLD
+                        : precision == FCIS_PREC_FP32 ? FusedTypes<FCIS_PREC_FP32>::NBLD
+                                                       : FusedTypes<FCIS_PREC_BF16X3>::NBLD;
+                p->nbp = (nb + FUSED_NB - 1) / FUSED_NB * FUSED_NB;
+                FCIS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fk, 64 * (p->mtiles + p->nbld), lds));
+                if (occ < 1) occ = 1;
+                p->fused_grid = ncu * occ;
+                p->fused_lds = lds;
+        }
+        // workspace per alpha string: walpha (nlinka rows, fused path) or
+        // t1 + vt1 (npair rows each)
+        const uint64_t rows_per_alpha = p->fused ? (uint64_t)space->nlinka : 2ull * p->npair;
+        uint64_t per_alpha = rows_per_alpha * (uint64_t)nb * elem;
         int64_t ta = per_alpha ? (int64_t)(workspace_bytes / per_alpha) : space->nstra;
         if (ta < 1) ta = 1;
         if (ta > space->nstra) ta = space->nstra;
         p->tile_a = ta;
         p->ntiles = (int)((space->nstra + ta - 1) / ta);
-        size_t buf = (size_t)p->npair * ta * nb * elem;
-        p->ws_bytes = 2 * buf;
+        size_t buf = (size_t)(p->fused ? space->nlinka : p->npair) * ta * nb * elem;
+        p->ws_bytes = p->fused ? buf : 2 * buf;
         const size_t nf = frag_count(p);
         size_t hbytes = precision == FCIS_PREC_BF16X3 ? nf * 16 : nf * elem;
         size_t pa = (size_t)p->npair * space->nstra * 4, pb = (size_t)p->npair * nb * 4;
@@ -443,7 +863,8 @@ int fcis_sigma_plan_create(fcis_sigma_plan_t **out, const fcis_space_t *space,
             hipMalloc(&p->d_hfrag_lo, precision == FCIS_PREC_BF16X3 ? hbytes : 16) != hipSuccess ||
             hipMalloc(&p->d_pmapa, pa ? pa : 4) != hipSuccess ||
             hipMalloc(&p->d_pmapb, pb ? pb : 4) != hipSuccess ||
-            hipMalloc(&p->d_t1, buf ? buf : 8) != hipSuccess ||
+            hipMalloc(&p->d_pmapb16, p->fused ? (size_t)p->npair * p->nbp * 2 + 16 : 16) != hipSuccess ||
+            hipMalloc(&p->d_t1, (buf && !p->fused) ? buf : 8) != hipSuccess ||
             hipMalloc(&p->d_vt1, buf ? buf : 8) != hipSuccess) {
                 fcis_sigma_plan_free(p);
                 fcis::set_error("fcis_sigma_plan_create: device allocation of %zu bytes failed",
@@ -467,6 +888,12 @@ int fcis_sigma_plan_create(fcis_sigma_plan_t **out, const fcis_space_t *space,
                 hipLaunchKernelGGL(k_build_pmap, dim3((nb + 255) / 256, space->nlinkb),
                                    dim3(256), 0, 0, (uint32_t)nb, space->nlinkb,
                                    space->d_linkb, p->d_pmapb);
+                FCIS_CHECK_LAUNCH();
+        }
+        if (p->fused && pb > 0) {
+                int64_t n16 = (int64_t)p->npair * p->nbp;
+                hipLaunchKernelGGL(k_pmap16, dim3((n16 + 255) / 256), dim3(256), 0, 0, p->npair, nb,
+                                   p->nbp, p->d_pmapb, p->d_pmapb16);
                 FCIS_CHECK_LAUNCH();
         }
         FCIS_HIP(hipDeviceSynchronize());
@@ -513,7 +940,43 @@ int fcis_sigma_plan_execute(fcis_sigma_plan_t *p, const double *ci0, double *ci1
                 const dim3 ggrid((ndet + GEMM_NCOL - 1) / GEMM_NCOL, (p->mtiles + GEMM_MT - 1) / GEMM_MT);
                 const dim3 rgrid(ta * ((nb + 255) / 256)), sgrid(na * ((nb + 255) / 256));
                 if (p->timing) FCIS_HIP(hipEventRecord(p->ev[0], st));
-                if (p->precision == FCIS_PREC_FP64) {
+                if (p->fused) {
+                        const dim3 fgrid(p->fused_grid), fblk(64 * (p->mtiles + p->nbld));
+                        const dim3 pgrid(na * ((nb + 255) / 256));
+                        const int acc = p->ntiles > 1 ? 1 : 0;
+                        const int64_t lda = ta * nb;   // walpha row stride
+                        if (t == 0 && acc)
+                                hipLaunchKernelGGL(k_zero, dim3((na * nb + 255) / 256), dim3(256), 0, st,
+                                                   na * nb, ci1);
+                        if (p->timing) FCIS_HIP(hipEventRecord(p->ev[1], st));
+                        if (p->precision == FCIS_PREC_FP64) {
+                                double *wa = (double *)p->d_vt1;
+                                hipLaunchKernelGGL(k_fused_row<FCIS_PREC_FP64>, fgrid, fblk, p->fused_lds, st,
+                                                   p->npair, p->mtiles, p->ksteps, s->nlinka, nb, na, a0,
+                                                   ta, lda, acc, s->d_linka, p->d_pmapa, p->d_pmapb16,
+                                                   p->nbp, ci0, p->d_hfrag, p->d_hfrag_lo, wa, ci1);
+                                if (p->timing) FCIS_HIP(hipEventRecord(p->ev[2], st));
+                                hipLaunchKernelGGL(k_alpha_pull<double>, pgrid, dim3(256), 0, st, nb, na, a0,
+                                                   ta, lda, s->nlinka, s->d_linka, wa, ci1);
+                        } else {
+                                float *wa = (float *)p->d_vt1;
+                                if (p->precision == FCIS_PREC_FP32)
+                                        hipLaunchKernelGGL(k_fused_row<FCIS_PREC_FP32>, fgrid, fblk,
+                                                           p->fused_lds, st, p->npair, p->mtiles, p->ksteps,
+                                                           s->nlinka, nb, na, a0, ta, lda, acc, s->d_linka,
+                                                           p->d_pmapa, p->d_pmapb16, p->nbp, ci0, p->d_hfrag,
+                                                           p->d_hfrag_lo, wa, ci1);
+                                else
+                                        hipLaunchKernelGGL(k_fused_row<FCIS_PREC_BF16X3>, fgrid, fblk,
+                                                           p->fused_lds, st, p->npair, p->mtiles, p->ksteps,
+                                                           s->nlinka, nb, na, a0, ta, lda, acc, s->d_linka,
+                                                           p->d_pmapa, p->d_pmapb16, p->nbp, ci0, p->d_hfrag,
+                                                           p->d_hfrag_lo, wa, ci1);
+                                if (p->timing) FCIS_HIP(hipEventRecord(p->ev[2], st));
+                                hipLaunchKernelGGL(k_alpha_pull<float>, pgrid, dim3(256), 0, st, nb, na, a0,
+                                                   ta, lda, s->nlinka, s->d_linka, wa, ci1);
+                        }
+                } else if (p->precision == FCIS_PREC_FP64) {
                         double *t1 = (double *)p->d_t1, *vt1 = (double *)p->d_vt1;
                         hipLaunchKernelGGL(k_gather<double>, rgrid, dim3(256), 0, st, p->npair, nb, a0,
                                            na, ld, p->d_pmapa, p->d_pmapb, ci0, t1);
@@ -593,6 +1056,7 @@ void fcis_sigma_plan_free(fcis_sigma_plan_t *p)
         (void)hipFree(p->d_hfrag_lo);
         (void)hipFree(p->d_pmapa);
         (void)hipFree(p->d_pmapb);
+        (void)hipFree(p->d_pmapb16);
         (void)hipFree(p->d_t1);
         (void)hipFree(p->d_vt1);
         delete p;

@@ -81,6 +81,7 @@ struct LinkEntry {
         int cre, des;
         uint32_t addr;
         int sign;
+        int rslot;      // slot of the reverse excitation in addr's list
 };
 
 __device__ inline LinkEntry link_entry(int n, int k, uint64_t s, uint32_t str, int slot)
@@ -91,6 +92,7 @@ __device__ inline LinkEntry link_entry(int n, int k, uint64_t s, uint32_t str, i
                 e.cre = e.des = p;
                 e.addr = str;
                 e.sign = 1;
+                e.rslot = slot;
                 return e;
         }
         int nvir = n - k;
@@ -106,6 +108,11 @@ __device__ inline LinkEntry link_entry(int n, int k, uint64_t s, uint32_t str, i
         e.des = q;
         e.addr = rank(s1);
         e.sign = (__popcll(s & mask) & 1) ? -1 : 1;
+        // reverse excitation a_q^+ a_p |s1>: p is occupied in s1 (occupied
+        // index = #occupied below p), q is empty (virtual index = #empty below q)
+        int io = __popcll(s1 & ((1ull << p) - 1));
+        int iv = __popcll(~s1 & full & ((1ull << q) - 1));
+        e.rslot = k + io * nvir + iv;
         return e;
 }
 
@@ -126,7 +133,8 @@ __global__ void k_build_links(int n, int k, uint32_t nstr, int nlink,
         LinkEntry e = link_entry(n, k, strs[str], str, slot);
         fcis_link out;
         out.addr = (int32_t)e.addr;
-        out.info = link_pack(e.cre, e.des, fcis::pair_index(e.cre, e.des), e.sign < 0 ? 1 : 0);
+        out.info = link_pack(e.cre, e.des, fcis::pair_index(e.cre, e.des), e.sign < 0 ? 1 : 0,
+                             nlink <= 255 ? e.rslot : 0);
         link[(size_t)slot * nstr + str] = out;
 }
 

@@ -58,8 +58,14 @@ def test_sigma_vs_oracle_medium(n, ne, seed):
     numpy.testing.assert_allclose(sig, ref, rtol=0, atol=1e-13 * numpy.abs(ref).max())
 
 
-def test_sigma_tiling_and_determinism():
+@pytest.mark.parametrize("deterministic", [False, True])
+def test_sigma_tiling_and_determinism(deterministic, monkeypatch):
+    """Alpha tiling (1, 84 and nstra tiles) gives the same sigma; the
+    FCIS_DETERMINISTIC three-kernel path is bitwise reproducible, the fused
+    path (LDS atomics in its beta scatter) reproducible to rounding."""
     import torch
+    if deterministic:
+        monkeypatch.setenv("FCIS_DETERMINISTIC", "1")
     n, ne = 10, (5, 4)
     h1e, eri, c = pkg.synthetic.make_problem(n, ne, seed=5)
     h2e = pkg.absorb_h1e(h1e, eri, n, ne, 0.5)
@@ -73,7 +79,10 @@ def test_sigma_tiling_and_determinism():
             o2 = torch.empty_like(cd)
             plan(cd, o1)
             plan(cd, o2)
-            assert torch.equal(o1, o2), "sigma is not bitwise reproducible"
+            if deterministic:
+                assert torch.equal(o1, o2), "sigma is not bitwise reproducible"
+            else:
+                assert (o1 - o2).abs().max().item() < 1e-14 * o1.abs().max().item()
             outs.append((plan.ntiles, o1.cpu().numpy()))
     assert outs[0][0] == 1 and outs[-1][0] == sp.nstra
     for nt, o in outs[1:]:
@@ -177,3 +186,25 @@ def test_mixed_precision_davidson_matches_fp64(precision):
     assert abs(e64 - w[0]) < 1e-9
     assert abs(em - w[0]) < 1e-8
     assert sm.stats["nhop_low"] > 0 and sm.stats["nhop_refine"] > 0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_fused_and_unfused_paths_agree(precision, monkeypatch):
+    """The fused row path (norb <= 16) and the deterministic three-kernel
+    path give the same sigma to rounding."""
+    import torch
+    n, ne = 9, (5, 4)
+    h1e, eri, c = pkg.synthetic.make_problem(n, ne, seed=8)
+    h2e = pkg.absorb_h1e(h1e, eri, n, ne, 0.5)
+    sp = pkg.get_space(n, ne)
+    cd = _dev(c)
+    with pkg.SigmaPlan(sp, h2e, precision, 1 << 28) as plan:
+        a = torch.empty_like(cd)
+        plan(cd, a)
+    monkeypatch.setenv("FCIS_DETERMINISTIC", "1")
+    with pkg.SigmaPlan(sp, h2e, precision, 1 << 28) as plan:
+        b = torch.empty_like(cd)
+        plan(cd, b)
+    tol = 1e-13 if precision == "fp64" else 2e-6
+    scale = a.abs().max().item()
+    assert (a - b).abs().max().item() < tol * scale
