@@ -36,7 +36,14 @@ def log(msg):
 
 
 def kernel_roofline(plan, sp, prec, calls, sync):
-    """Per-kernel averages from the plan's HIP-event timers."""
+    """Per-kernel averages from the plan's HIP-event timers, with the
+    algorithmic work per launch:
+      contraction  F = 2 npair^2 dets                          (MFMA bound)
+      fused path   reads c (8 B/det), writes walpha (s nlinka B/det) and
+                   sigma (8 B/det); alpha pull reads walpha and sigma and
+                   writes sigma: (s nlinka + 16) B/det          (HBM bound)
+      3-kernel     gather  8 + s npair B/det, scatter s npair + 8 B/det
+    with s = 8 (FP64) or 4 (FP32 / bf16x3 intermediates)."""
     plan.set_timing(True)
     out = plan._bench_out
     for _ in range(calls):
@@ -46,28 +53,36 @@ def kernel_roofline(plan, sp, prec, calls, sync):
     plan.set_timing(False)
     npair = sp.norb * (sp.norb + 1) // 2
     s = 8 if prec == "fp64" else 4
-    ndet = sp.ndet
     per_sigma_launch = nl / calls
     avg = ms / nl                      # ms per launch
-    dets_per_launch = ndet / per_sigma_launch[1]
-    flop = 2.0 * npair * npair * dets_per_launch
-    gather_bytes = (8.0 + s * npair) * dets_per_launch
-    scatter_bytes = (s * npair + 8.0) * dets_per_launch
-    out = {
-        "contract": {"ms": avg[1], "TFLOP/s": flop / avg[1] / 1e9,
-                     "frac_mfma": flop / avg[1] / 1e9 / PEAK[prec],
-                     "flop_per_launch": flop, "dets_per_launch": dets_per_launch},
-        "scatter": {"ms": avg[2], "GB/s": scatter_bytes / avg[2] / 1e6,
-                    "frac_hbm": scatter_bytes / avg[2] / 1e6 / HBM_PEAK,
-                    "bytes_per_launch": scatter_bytes},
-        "launches_per_sigma": per_sigma_launch.tolist(),
-    }
-    if avg[0] > 1e-3:   # separate gather kernel (unfused path)
-        out["gather"] = {"ms": avg[0], "GB/s": gather_bytes / avg[0] / 1e6,
-                         "frac_hbm": gather_bytes / avg[0] / 1e6 / HBM_PEAK,
-                         "bytes_per_launch": gather_bytes}
-    else:               # gather fused into the contraction kernel
-        out["contract"]["fused_gather"] = True
+    dets = sp.ndet / per_sigma_launch[1]
+    flop = 2.0 * npair * npair * dets
+    out = {"launches_per_sigma": per_sigma_launch.tolist(), "fused": bool(plan.fused)}
+    if plan.fused:
+        fused_bytes = (16.0 + s * sp.nlinka) * dets
+        pull_bytes = (s * sp.nlinka + 16.0) * dets
+        out["k_fused_row"] = {"ms": avg[1], "TFLOP/s": flop / avg[1] / 1e9,
+                              "frac_mfma": flop / avg[1] / 1e9 / PEAK[prec],
+                              "flop_per_launch": flop, "dets_per_launch": dets,
+                              "alg_bytes_per_launch": fused_bytes,
+                              "GB/s": fused_bytes / avg[1] / 1e6}
+        out["k_alpha_pull"] = {"ms": avg[2], "GB/s": pull_bytes / avg[2] / 1e6,
+                               "frac_hbm": pull_bytes / avg[2] / 1e6 / HBM_PEAK,
+                               "alg_bytes_per_launch": pull_bytes}
+        out["contract"] = out["k_fused_row"]
+    else:
+        gather_bytes = (8.0 + s * npair) * dets
+        scatter_bytes = (s * npair + 8.0) * dets
+        out["k_gather"] = {"ms": avg[0], "GB/s": gather_bytes / avg[0] / 1e6,
+                           "frac_hbm": gather_bytes / avg[0] / 1e6 / HBM_PEAK,
+                           "alg_bytes_per_launch": gather_bytes}
+        out["k_contract"] = {"ms": avg[1], "TFLOP/s": flop / avg[1] / 1e9,
+                             "frac_mfma": flop / avg[1] / 1e9 / PEAK[prec],
+                             "flop_per_launch": flop, "dets_per_launch": dets}
+        out["k_scatter"] = {"ms": avg[2], "GB/s": scatter_bytes / avg[2] / 1e6,
+                            "frac_hbm": scatter_bytes / avg[2] / 1e6 / HBM_PEAK,
+                            "alg_bytes_per_launch": scatter_bytes}
+        out["contract"] = out["k_contract"]
     return out
 
 
@@ -221,11 +236,16 @@ def main():
         cpu = cpu_baseline(pkg)
 
     main_k = kern["fp64"]["contract"] if "fp64" in kern else None
+    main_name = None
+    if main_k is not None:
+        main_name = "k_fused_row<fp64>" if kern["fp64"]["fused"] else "k_contract_f64"
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_contract_f64_16o16e.json")
+    tpath = os.path.join(ROOT, "profiles", "traffic_sigma_f64_16o16e.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            tj = json.load(f)
+        if tj.get("kernel") == main_name:
+            traffic = tj.get("hbm_bytes_per_launch")
     fp64 = result.get("fp64", {})
     line = {
         "metric": "sigma=H*c per second (FP64) at (16o,16e); Davidson time FP64 vs mixed",

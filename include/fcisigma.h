@@ -110,6 +110,13 @@ void fcis_sigma_plan_free(fcis_sigma_plan_t *plan);
 /* Tiles per execute, alpha strings per tile, workspace bytes. */
 int fcis_sigma_plan_info(const fcis_sigma_plan_t *plan, int *ntiles,
                          int64_t *tile_alpha, uint64_t *workspace_bytes);
+/* 1 if the plan runs the fused row path (norb <= 16: gather, MFMA
+ * contraction and beta scatter in one kernel, then an alpha pull; its beta
+ * scatter accumulates with LDS atomics, so results are reproducible to
+ * rounding only), 0 for the bitwise-reproducible three-kernel path
+ * (gather, contraction, pull scatter; selected for larger spaces or with
+ * the environment variable FCIS_DETERMINISTIC set at plan creation). */
+int fcis_sigma_plan_path(const fcis_sigma_plan_t *plan);
 /* Replace the Hamiltonian of an existing plan (same space). */
 int fcis_sigma_plan_set_h2e(fcis_sigma_plan_t *plan, const double *h2e);
 /* Per-kernel timing with HIP events recorded on the execute stream

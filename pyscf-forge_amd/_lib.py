@@ -44,6 +44,7 @@ SIGNATURES = {
     "fcis_sigma_plan_free": (None, [_vp]),
     "fcis_sigma_plan_info": (ctypes.c_int, [_vp, _ip, _i64p, _u64p]),
     "fcis_sigma_plan_set_h2e": (ctypes.c_int, [_vp, _dp]),
+    "fcis_sigma_plan_path": (ctypes.c_int, [_vp]),
     "fcis_sigma_plan_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fcis_sigma_plan_get_timing": (ctypes.c_int, [_vp, _dp, _i64p]),
     "fcis_make_hdiag": (ctypes.c_int, [_vp, _dp, _dp, _vp, _vp]),

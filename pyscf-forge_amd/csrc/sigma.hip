@@ -901,6 +901,12 @@ int fcis_sigma_plan_create(fcis_sigma_plan_t **out, const fcis_space_t *space,
         return FCIS_SUCCESS;
 }
 
+int fcis_sigma_plan_path(const fcis_sigma_plan_t *p)
+{
+        if (p == nullptr) return FCIS_ERR_INVALID;
+        return p->fused ? 1 : 0;
+}
+
 int fcis_sigma_plan_set_h2e(fcis_sigma_plan_t *p, const double *h2e)
 {
         if (p == nullptr || h2e == nullptr) return FCIS_ERR_INVALID;

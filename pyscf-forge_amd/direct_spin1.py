@@ -166,6 +166,7 @@ class SigmaPlan:
         nt, ta, ws = ctypes.c_int(), ctypes.c_int64(), ctypes.c_uint64()
         self.lib.fcis_sigma_plan_info(h, ctypes.byref(nt), ctypes.byref(ta), ctypes.byref(ws))
         self.ntiles, self.tile_alpha, self.workspace_bytes = nt.value, ta.value, ws.value
+        self.fused = self.lib.fcis_sigma_plan_path(h) == 1
 
     def __call__(self, c, out):
         """out <- H c (device float64 tensors with space.ndet elements)."""
