@@ -19,3 +19,4 @@ from .direct_spin1 import (  # noqa: F401
 # drop-in alias: code written against pyscf.fci.direct_spin1 can use
 # ``from <package> import direct_spin1`` unchanged.
 from . import direct_spin1  # noqa: F401,E402
+from . import autograd  # noqa: F401,E402
