@@ -49,6 +49,7 @@ def test_autograd_c4_finite_differences():
     n, ne = 12, (6, 6)
     h1e, eri = pkg.synthetic.make_problem(n, ne, seed=0, with_ci=False)
     solver = pkg.FCISolver()
+    solver.max_cycle = 200   # the synthetic (12o,12e) problem needs ~60 Davidson cycles
     e0, c = solver.kernel(h1e, eri, n, ne)
     assert solver.converged
     E, gh, ge, _ = _grads(h1e, eri, c, n, ne)
