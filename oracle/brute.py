@@ -73,3 +73,41 @@ def hamiltonian(h1e, eri, n, nelec):
     K = numpy.einsum("pqqs->ps", eri)
     H -= 0.5 * numpy.einsum("ps,psij->ij", K, E)
     return H
+
+
+def spin_square_matrix(n, nelec):
+    """S^2 in the (alpha address, beta address) determinant basis:
+    S^2 = S_- S_+ + S_z (S_z + 1), S_+ = sum_p a^+_{p alpha} a_{p beta},
+    with alpha spin-orbitals ordered before beta ones (the sign convention of
+    the alpha-string x beta-string product basis)."""
+    na, nb = nelec
+    sa, sb = _strings(n, na), _strings(n, nb)
+    ia = {s: i for i, s in enumerate(sa)}
+    N = len(sa) * len(sb)
+    S2 = numpy.zeros((N, N))
+    sz = 0.5 * (na - nb)
+    for i, a in enumerate(sa):
+        for j, b in enumerate(sb):
+            col = i * len(sb) + j
+            S2[col, col] += sz * (sz + 1)
+            # S_- S_+ |a b> = sum_{p,q} a+_{q beta} a_{q alpha} a+_{p alpha} a_{p beta} |a b>
+            for p in range(n):
+                if not (b >> p) & 1 or (a >> p) & 1:
+                    continue
+                # a_{p beta}: sign = (-1)^(na + #beta below p) in the combined ordering
+                s1 = (-1) ** (na + bin(b & ((1 << p) - 1)).count("1"))
+                b1 = b ^ (1 << p)
+                s2 = (-1) ** bin(a & ((1 << p) - 1)).count("1")
+                a1 = a | (1 << p)
+                for q in range(n):
+                    if not (a1 >> q) & 1 or (b1 >> q) & 1:
+                        continue
+                    s3 = (-1) ** bin(a1 & ((1 << q) - 1)).count("1")
+                    a2 = a1 ^ (1 << q)
+                    s4 = (-1) ** (na + bin(b1 & ((1 << q) - 1)).count("1"))
+                    b2 = b1 | (1 << q)
+                    if a2 not in ia:
+                        continue
+                    row = ia[a2] * len(sb) + _strings(n, nb).index(b2)
+                    S2[row, col] += s1 * s2 * s3 * s4
+    return S2
