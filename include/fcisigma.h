@@ -149,6 +149,42 @@ int fcis_trans_rdm12s(const fcis_space_t *space, const double *d_bra,
                       double *dm2aa, double *dm2ab, double *dm2bb,
                       void *stream);
 
+/* ---- Configuration state functions (spin-adapted basis of the CSF-FCI
+ *      solver, pyscf/csf_fci) ----------------------------------------- */
+
+/*
+ * CSF transformer for spin multiplicity smult = 2S+1 on a CAS space.
+ * Built on the device: the CSD -> determinant address map (make_csd_mask,
+ * csdstring.py:28-47) and the per-block spin eigenvectors
+ * sgn * umat (get_spin_evecs csfstring.py:913-960 -> FCICSFmakecsf
+ * csfstring.c:86-141; phase csfstring.py:623-627).  CSF order is the
+ * reference's: npair blocks, pair configuration, unpaired configuration,
+ * coupling string (get_csfvec_shape csfstring.py:858-911).  The space must
+ * outlive the transformer.  Fails with FCIS_ERR_INVALID if no state of that
+ * spin exists for (na, nb).
+ */
+typedef struct fcis_csf fcis_csf_t;
+int fcis_csf_create(fcis_csf_t **out, const fcis_space_t *space, int smult,
+                    void *stream);
+void fcis_csf_free(fcis_csf_t *csf);
+int fcis_csf_info(const fcis_csf_t *csf, int64_t *ncsf, int64_t *ndet);
+/* nvec determinant vectors (rows of ndet) -> nvec CSF vectors (rows of
+ * ncsf), no normalisation (_transform_det2csf csfstring.py:569-674 /
+ * vec_det2csf(normalize=False) :45-52). */
+int fcis_csf_det2csf(const fcis_csf_t *csf, const double *d_det, double *d_csf,
+                     int nvec, void *stream);
+/* nvec CSF vectors -> determinant vectors; determinants of spin blocks with
+ * no CSF of this multiplicity are set to zero (transform_civec_csf2det
+ * csfstring.py:416-487 / vec_csf2det(normalize=False) :54-60). */
+int fcis_csf_csf2det(const fcis_csf_t *csf, const double *d_csf, double *d_det,
+                     int nvec, void *stream);
+/* Diagonal of H in the CSF basis (make_hdiag_csf csf.py:70-144 ->
+ * FCICSFhdiag csfstring.c:295-361): diag(U^T H_conf U) per configuration,
+ * from the determinant diagonal d_hdiag_det (fcis_make_hdiag) and the
+ * exchange integrals of the HOST (npair, npair) pair matrix eri. */
+int fcis_csf_hdiag(fcis_csf_t *csf, const double *eri, const double *d_hdiag_det,
+                   double *d_hdiag_csf, void *stream);
+
 /* ---- Davidson primitives on device vectors (pyscf.lib.davidson1 work,
  *      called at fci_gas.py:1344-1350; vectors stay in HBM) ----------- */
 

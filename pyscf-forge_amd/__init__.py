@@ -20,3 +20,4 @@ from .direct_spin1 import (  # noqa: F401
 # ``from <package> import direct_spin1`` unchanged.
 from . import direct_spin1  # noqa: F401,E402
 from . import autograd  # noqa: F401,E402
+from . import csf_fci  # noqa: F401,E402

@@ -49,6 +49,12 @@ SIGNATURES = {
     "fcis_sigma_plan_get_timing": (ctypes.c_int, [_vp, _dp, _i64p]),
     "fcis_make_hdiag": (ctypes.c_int, [_vp, _dp, _dp, _vp, _vp]),
     "fcis_trans_rdm12s": (ctypes.c_int, [_vp, _vp, _vp, _dp, _dp, _dp, _dp, _dp, _vp]),
+    "fcis_csf_create": (ctypes.c_int, [ctypes.POINTER(_vp), _vp, ctypes.c_int, _vp]),
+    "fcis_csf_free": (None, [_vp]),
+    "fcis_csf_info": (ctypes.c_int, [_vp, _i64p, _i64p]),
+    "fcis_csf_det2csf": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp]),
+    "fcis_csf_csf2det": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp]),
+    "fcis_csf_hdiag": (ctypes.c_int, [_vp, _dp, _vp, _vp, _vp]),
     "fcis_dots": (ctypes.c_int, [ctypes.c_int, _i64, _vp, _i64, _vp, _dp, _vp]),
     "fcis_lincomb": (ctypes.c_int, [ctypes.c_int, _i64, _vp, _i64, _dp,
                                     ctypes.c_double, _vp, _vp]),

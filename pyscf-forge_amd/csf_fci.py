@@ -1,0 +1,383 @@
+"""Spin-adapted (CSF-basis) CAS FCI on the GPU.
+
+Mirrors pyscf-forge's CSF-FCI solver (pyscf/csf_fci/csf.py) and its
+transformer (pyscf/csf_fci/csfstring.py:33-120):
+
+* ``CSFTransformer(norb, neleca, nelecb, smult)`` with ``vec_det2csf`` /
+  ``vec_csf2det`` (csfstring.py:45-60; same argument meaning, normalisation
+  and return_norm behaviour), ``ncsf``, ``ndet``.  The transform runs in
+  HIP kernels (fcis_csf_det2csf / fcis_csf_csf2det); host numpy vectors are
+  copied to HBM and back, device torch tensors stay on the device.
+* ``make_hdiag_csf`` (csf.py:70-144) on the device.
+* ``FCISolver`` (csf.py:499-560): attributes ``smult`` and ``transformer``;
+  ``kernel`` follows csf.kernel (csf.py:368-497): Davidson in the CSF basis
+  with hop = det2csf . sigma . csf2det (csf.py:447-450), preconditioned by
+  the CSF diagonal, started from the lowest-diagonal CSFs
+  (get_init_guess csf.py:51-64), exact diagonalisation when the CSF space
+  fits the pspace (csf.py:416-429); ``ci0`` and the returned ``ci`` are
+  determinant-basis (na, nb) arrays.
+
+Point-group symmetry (orbsym / wfnsym) is not part of this path.
+"""
+
+import math
+
+import numpy
+
+from . import _lib
+from . import direct_spin1 as _ds
+
+
+class ImpossibleSpinError(RuntimeError):
+    """No CSF of the requested multiplicity (csfstring.py:27-31)."""
+
+
+class CSFTransformer:
+    """Determinant <-> CSF transform for (norb, neleca, nelecb, 2S+1)."""
+
+    def __init__(self, norb, neleca, nelecb, smult, orbsym=None, wfnsym=None, max_memory=None):
+        if orbsym is not None or wfnsym is not None:
+            raise NotImplementedError("point-group symmetry is not supported by this transformer")
+        self.wfnsym = None
+        self.max_memory = max_memory
+        self._h = None
+        self._update_spin_cache(norb, neleca, nelecb, smult)
+
+    # -- cache ---------------------------------------------------------
+    def _update_spin_cache(self, norb, neleca, nelecb, smult):
+        key = (int(norb), int(neleca), int(nelecb), int(smult))
+        if self._h is not None and key == self._key:
+            return
+        self.close()
+        import ctypes
+        self._space = _ds.get_space(norb, (neleca, nelecb))
+        lib = _lib.load()
+        h = ctypes.c_void_p()
+        st = lib.fcis_csf_create(ctypes.byref(h), self._space.handle, key[3], _lib.stream_ptr())
+        if st != _lib.SUCCESS:
+            msg = lib.fcis_last_error().decode()
+            if st == _lib.ERR_INVALID:
+                raise ImpossibleSpinError(msg)
+            _lib.check(st, "fcis_csf_create")
+        self._h = h
+        self._key = key
+        ncsf = ctypes.c_int64()
+        ndet = ctypes.c_int64()
+        _lib.check(lib.fcis_csf_info(h, ctypes.byref(ncsf), ctypes.byref(ndet)), "fcis_csf_info")
+        self.ncsf = int(ncsf.value)
+        self.ndet = int(ndet.value)
+
+    @property
+    def norb(self):
+        return self._key[0]
+
+    @property
+    def neleca(self):
+        return self._key[1]
+
+    @property
+    def nelecb(self):
+        return self._key[2]
+
+    @property
+    def smult(self):
+        return self._key[3]
+
+    @property
+    def ndeta(self):
+        return self._space.shape[0]
+
+    @property
+    def ndetb(self):
+        return self._space.shape[1]
+
+    @property
+    def n(self):
+        return self.ncsf
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            _lib.load().fcis_csf_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- device transforms (no normalisation) --------------------------
+    def det2csf_dev(self, det, out, nvec=1):
+        """out[nvec, ncsf] <- det[nvec, ndet] (device float64, contiguous)."""
+        _lib.check(_lib.load().fcis_csf_det2csf(self._h, _lib.vptr(det), _lib.vptr(out),
+                                                int(nvec), _lib.stream_ptr()),
+                   "fcis_csf_det2csf")
+
+    def csf2det_dev(self, csf, out, nvec=1):
+        """out[nvec, ndet] <- csf[nvec, ncsf] (device float64, contiguous)."""
+        _lib.check(_lib.load().fcis_csf_csf2det(self._h, _lib.vptr(csf), _lib.vptr(out),
+                                                int(nvec), _lib.stream_ptr()),
+                   "fcis_csf_csf2det")
+
+    def hop(self, plan):
+        """x (CSF) -> H x (CSF) through the determinant-space sigma plan."""
+        import torch
+        det = torch.empty(self.ndet, dtype=torch.float64, device="cuda")
+        sig = torch.empty_like(det)
+
+        def apply(x, out):
+            self.csf2det_dev(x, det)
+            plan(det, sig)
+            self.det2csf_dev(sig, out)
+        return apply
+
+    # -- pyscf-facing transforms ----------------------------------------
+    def _apply(self, civec, nin, nout, fn, order, normalize, return_norm):
+        import torch
+        on_dev = isinstance(civec, torch.Tensor)
+        x = civec if on_dev else torch.as_tensor(numpy.asarray(civec, dtype=numpy.float64))
+        x = x.to(device="cuda", dtype=torch.float64)
+        vec_on_cols = order.upper() == "F"
+        if x.numel() == nin:
+            x = x.reshape(1, nin)
+            single = True
+        else:
+            x = x.reshape(nin, -1).T if vec_on_cols else x.reshape(-1, nin)
+            single = False
+        x = x.contiguous()
+        nvec = x.shape[0]
+        out = torch.empty((nvec, nout), dtype=torch.float64, device="cuda")
+        if nout:
+            fn(x, out, nvec)
+        norm = torch.linalg.norm(out, dim=1)
+        if normalize:
+            out = out / norm[:, None]
+        if vec_on_cols and not single:
+            out = out.T.contiguous()
+        if single:
+            out = out.reshape(nout)
+            norm = norm[0]
+        if not on_dev:
+            out = out.cpu().numpy()
+            norm = norm.cpu().numpy() if not single else float(norm)
+        return (out, norm) if return_norm else out
+
+    def vec_det2csf(self, civec, order="C", normalize=True, return_norm=False):
+        """Determinant CI vector(s) -> CSF vector(s) (csfstring.py:45-52)."""
+        return self._apply(civec, self.ndet, self.ncsf, self.det2csf_dev, order, normalize,
+                           return_norm)
+
+    def vec_csf2det(self, civec, order="C", normalize=True, return_norm=False):
+        """CSF vector(s) -> flat determinant vector(s) (csfstring.py:54-60)."""
+        return self._apply(civec, self.ncsf, self.ndet, self.csf2det_dev, order, normalize,
+                           return_norm)
+
+    def pack_csf(self, csfvec, order="C"):
+        return csfvec
+
+    def unpack_csf(self, csfvec, order="C"):
+        return csfvec
+
+
+def count_csfs(nspin, smult):
+    """CSFs of nspin singly occupied orbitals coupled to 2S+1 = smult:
+    C(nspin, nspin/2 - S) - C(nspin, nspin/2 - S - 1) (branching rule)."""
+    twos = int(smult) - 1
+    if twos < 0 or twos > nspin or (nspin - twos) % 2:
+        return 0
+    k = (nspin - twos) // 2
+    return math.comb(nspin, k) - (math.comb(nspin, k - 1) if k > 0 else 0)
+
+
+def count_all_csfs(norb, neleca, nelecb, smult):
+    """Number of CSFs of multiplicity smult (csfstring.count_all_csfs)."""
+    if abs(neleca - nelecb) > smult - 1:
+        return 0
+    total = 0
+    for npair in range(max(0, neleca + nelecb - norb), min(neleca, nelecb) + 1):
+        nspin = neleca + nelecb - 2 * npair
+        total += (math.comb(norb, npair) * math.comb(norb - npair, nspin)
+                  * count_csfs(nspin, smult))
+    return total
+
+
+def make_hdiag_csf(h1e, eri, norb, nelec, transformer, hdiag_det=None, max_memory=None):
+    """CSF-basis diagonal of H (csf.py:70-144) as a device tensor."""
+    import torch
+    norb = int(norb)
+    nelec = _ds._unpack_nelec(nelec)
+    h1e = numpy.ascontiguousarray(h1e, dtype=numpy.float64).reshape(norb, norb)
+    eri = numpy.ascontiguousarray(_ds.pair_matrix(eri, norb))
+    if hdiag_det is None:
+        hdiag_det = _ds._make_hdiag_dev(h1e, eri, norb, nelec)
+    elif not isinstance(hdiag_det, torch.Tensor):
+        hdiag_det = torch.as_tensor(numpy.asarray(hdiag_det, dtype=numpy.float64)).cuda()
+    hdiag_det = hdiag_det.reshape(-1).contiguous()
+    out = torch.empty(transformer.ncsf, dtype=torch.float64, device="cuda")
+    _lib.check(_lib.load().fcis_csf_hdiag(transformer._h, _lib.dptr(eri), _lib.vptr(hdiag_det),
+                                          _lib.vptr(out), _lib.stream_ptr()), "fcis_csf_hdiag")
+    return out
+
+
+class FCISolver(_ds.FCISolver):
+    """CSF-basis CAS FCI solver (csf_fci.csf_solver for CAS, csf.py:499-560).
+
+    ``smult`` = 2S+1 (default: the lowest allowed by nelec and ``spin``).
+    ``ci0``/``ci`` are determinant-basis arrays; Davidson runs on CSF
+    vectors of length ``transformer.ncsf`` kept in HBM."""
+
+    def __init__(self, mol=None, smult=None):
+        super().__init__(mol)
+        self.smult = smult
+        self.transformer = None
+        self.pspace_size = 200
+
+    def check_transformer_cache(self, norb=None, nelec=None, smult=None):
+        norb = self.norb if norb is None else norb
+        nelec = self._nelec(self.nelec if nelec is None else nelec)
+        if smult is not None:
+            self.smult = smult
+        if self.smult is None:
+            self.smult = abs(nelec[0] - nelec[1]) + 1
+        if self.transformer is None:
+            self.transformer = CSFTransformer(norb, nelec[0], nelec[1], self.smult)
+        else:
+            self.transformer._update_spin_cache(norb, nelec[0], nelec[1], self.smult)
+        return self.transformer
+
+    def make_hdiag_csf(self, h1e, eri, norb, nelec, hdiag_det=None, smult=None, max_memory=None):
+        self.norb, self.nelec = int(norb), self._nelec(nelec)
+        t = self.check_transformer_cache(smult=smult)
+        return make_hdiag_csf(h1e, eri, norb, nelec, t, hdiag_det=hdiag_det)
+
+    def _init_guess_dev(self, hdiag, nroots):
+        """Unit CSF vectors at the lowest diagonal elements, the first root
+        with +-1e-5 at its ends (direct_spin1._get_init_guess applied to
+        the CSF diagonal, as csf.get_init_guess does, csf.py:51-64)."""
+        import torch
+        n = hdiag.numel()
+        k = min(nroots, n)
+        addrs = torch.topk(hdiag, k, largest=False).indices
+        out = []
+        for a in addrs.tolist():
+            g = torch.zeros(n, dtype=torch.float64, device=hdiag.device)
+            g[a] = 1.0
+            out.append(g)
+        if n > 1:
+            out[0][0] += 1e-5
+            out[0][-1] -= 1e-5
+        return out
+
+    def get_init_guess(self, norb, nelec, nroots, hdiag_csf):
+        """Determinant-basis initial guesses (csf.get_init_guess)."""
+        import torch
+        hd = hdiag_csf if isinstance(hdiag_csf, torch.Tensor) else torch.as_tensor(
+            numpy.asarray(hdiag_csf, dtype=numpy.float64)).cuda()
+        self.norb, self.nelec = int(norb), self._nelec(nelec)
+        t = self.check_transformer_cache()
+        return [t.vec_csf2det(g).cpu().numpy() for g in self._init_guess_dev(hd, nroots)]
+
+    def pspace(self, h1e, eri, norb, nelec, hdiag_det=None, hdiag_csf=None, npsp=200, **kwargs):
+        """(csf_addr, h0): H in the npsp CSFs of lowest diagonal, built from
+        H applied to those unit CSF vectors (csf.pspace, csf.py:246-366)."""
+        import torch
+        self.norb, self.nelec = int(norb), self._nelec(nelec)
+        t = self.check_transformer_cache(smult=kwargs.get("smult"))
+        sp = _ds.get_space(norb, self.nelec)
+        h1e = numpy.ascontiguousarray(h1e, dtype=numpy.float64).reshape(sp.norb, sp.norb)
+        eri = _ds.pair_matrix(eri, sp.norb)
+        if hdiag_csf is None:
+            hdiag_csf = make_hdiag_csf(h1e, eri, norb, self.nelec, t, hdiag_det=hdiag_det)
+        elif not isinstance(hdiag_csf, torch.Tensor):
+            hdiag_csf = torch.as_tensor(numpy.asarray(hdiag_csf, dtype=numpy.float64)).cuda()
+        npsp = min(int(npsp), t.ncsf)
+        addr = torch.sort(torch.topk(hdiag_csf, npsp, largest=False).indices).values
+        h2e = _ds.absorb_h1e(h1e, eri, norb, self.nelec, 0.5)
+        return addr.cpu().numpy(), self._h_block(sp, t, h2e, addr)
+
+    def _h_block(self, sp, t, h2e, addr):
+        import torch
+        m = addr.numel()
+        x = torch.zeros(t.ncsf, dtype=torch.float64, device="cuda")
+        hx = torch.empty_like(x)
+        H = numpy.empty((m, m))
+        with _ds.SigmaPlan(sp, h2e, "fp64", _ds._default_workspace()) as plan:
+            hop = t.hop(plan)
+            for j, a in enumerate(addr.tolist()):
+                x.zero_()
+                x[a] = 1.0
+                hop(x, hx)
+                H[:, j] = hx[addr].cpu().numpy()
+        return 0.5 * (H + H.T)
+
+    def kernel(self, h1e, eri, norb, nelec, ci0=None, ecore=0, nroots=None, smult=None,
+               **kwargs):
+        """(e + ecore, ci) with ci in the determinant basis (csf.kernel)."""
+        import torch
+        nroots = int(nroots if nroots is not None else kwargs.pop("nroots", self.nroots))
+        if nroots < 1:
+            raise ValueError("nroots must be positive")
+
+        def option(name, default):
+            v = kwargs.pop(name, None)
+            return default if v is None else v
+
+        tol = option("tol", self.conv_tol)
+        tol_residual = option("tol_residual", self.conv_tol_residual)
+        max_cycle = option("max_cycle", self.max_cycle)
+        max_space = option("max_space", self.max_space)
+        lindep = option("lindep", self.lindep)
+        level_shift = option("level_shift", self.level_shift)
+        davidson_only = bool(option("davidson_only", self.davidson_only))
+        pspace_size = int(option("pspace_size", self.pspace_size))
+        precision = option("precision", self.precision)
+        verbose = option("verbose", self.verbose)
+        nelec = self._nelec(nelec)
+        self.norb, self.nelec = int(norb), nelec
+        t = self.check_transformer_cache(smult=smult)
+        sp = _ds.get_space(norb, nelec)
+        nroots = min(nroots, t.ncsf)
+        if t.ncsf == 0:
+            raise ImpossibleSpinError("no CSF with 2S+1 = %d" % self.smult)
+        h1e = numpy.ascontiguousarray(h1e, dtype=numpy.float64).reshape(sp.norb, sp.norb)
+        eri = _ds.pair_matrix(eri, sp.norb)
+        h2e = _ds.absorb_h1e(h1e, eri, norb, nelec, 0.5)
+        hdiag_det = _ds._make_hdiag_dev(h1e, eri, norb, nelec)
+        hdiag = make_hdiag_csf(h1e, eri, norb, nelec, t, hdiag_det=hdiag_det)
+
+        if ci0 is None and not davidson_only and pspace_size >= t.ncsf:
+            addr = torch.arange(t.ncsf, device="cuda")
+            H = self._h_block(sp, t, h2e, addr)
+            w, v = numpy.linalg.eigh(H)
+            e = w[:nroots]
+            c = [torch.from_numpy(numpy.ascontiguousarray(v[:, k])).cuda() for k in range(nroots)]
+            conv = numpy.ones(nroots, dtype=bool)
+        else:
+            if ci0 is not None:
+                vecs = list(ci0) if isinstance(ci0, (list, tuple)) else (
+                    [ci0] if numpy.asarray(ci0).size == t.ndet else list(ci0))
+                ci0 = [t.vec_det2csf(torch.as_tensor(numpy.asarray(v, dtype=numpy.float64))
+                                     .cuda().reshape(-1)) for v in vecs]
+            from .mixed import solve
+            e, c, conv, stats = solve(self, sp, h2e, hdiag, nroots, ci0, precision,
+                                      tol=tol, tol_residual=tol_residual,
+                                      max_cycle=max_cycle, max_space=max_space,
+                                      lindep=lindep, level_shift=level_shift,
+                                      verbose=verbose, basis=t)
+            self.stats = stats
+        e = numpy.asarray(e, dtype=numpy.float64).reshape(-1) + float(ecore)
+        cis = [t.vec_csf2det(x.reshape(-1), normalize=False).reshape(sp.shape) for x in c]
+        if nroots == 1:
+            self.converged = bool(conv[0])
+            self.eci = float(e[0])
+            self.ci = cis[0].cpu().numpy()
+        else:
+            self.converged = numpy.asarray(conv, dtype=bool)
+            self.eci = e
+            self.ci = [x.cpu().numpy() for x in cis]
+        return self.eci, self.ci
+
+
+def csf_solver(mol=None, smult=None):
+    """CAS CSF solver factory (pyscf.csf_fci.csf_solver for CAS)."""
+    return FCISolver(mol, smult=smult)

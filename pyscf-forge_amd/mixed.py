@@ -32,12 +32,16 @@ def _davidson_bytes(ndet, nroots, max_space):
 
 
 def solve(solver, sp, h2e, hdiag, nroots, ci0, precision, tol=1e-10, tol_residual=None,
-          max_cycle=50, max_space=12, lindep=1e-14, level_shift=1e-3, verbose=0):
+          max_cycle=50, max_space=12, lindep=1e-14, level_shift=1e-3, verbose=0,
+          basis=None):
+    """Davidson solve in the determinant basis, or, with ``basis`` (an object
+    with ``n`` and ``hop(plan)``, e.g. a CSF transformer), in that basis with
+    H x = basis.hop(plan)(x, out); hdiag and ci0 are then in that basis."""
     import torch
     from .direct_spin1 import SigmaPlan
 
     lib = _lib.load()
-    n = sp.ndet
+    n = sp.ndet if basis is None else int(basis.n)
     stream = _lib.stream_ptr()
 
     def precond(r, e, out):
@@ -57,13 +61,16 @@ def solve(solver, sp, h2e, hdiag, nroots, ci0, precision, tol=1e-10, tol_residua
             t = torch.as_tensor(numpy.asarray(v) if not isinstance(v, torch.Tensor) else v)
             t = t.to(device="cuda", dtype=torch.float64).reshape(-1).contiguous()
             if t.numel() != n:
-                raise ValueError("initial CI vector size does not match determinant count")
+                raise ValueError("initial vector size %d does not match the basis size %d"
+                                 % (t.numel(), n))
             guesses.append(t)
         if len(guesses) < nroots:
             guesses += solver._init_guess_dev(hdiag, nroots)[len(guesses):]
 
     free = torch.cuda.mem_get_info()[0]
     need = _davidson_bytes(n, nroots, max_space)
+    if basis is not None:
+        need += 2 * sp.ndet * 8
     ws = int(solver.workspace_bytes) or max(int((free - need) * 0.85), 1 << 26)
     stats = {"precision": precision}
     verbose = bool(verbose and verbose > 4)
@@ -71,7 +78,8 @@ def solve(solver, sp, h2e, hdiag, nroots, ci0, precision, tol=1e-10, tol_residua
     def run(prec, guesses, **kw):
         t0 = time.perf_counter()
         with SigmaPlan(sp, h2e, prec, ws) as plan:
-            res = _dv.davidson(plan, precond, guesses, nroots=nroots, lindep=lindep,
+            hop = plan if basis is None else basis.hop(plan)
+            res = _dv.davidson(hop, precond, guesses, nroots=nroots, lindep=lindep,
                                max_space=max_space, verbose=verbose, **kw)
         torch.cuda.synchronize()
         return res, time.perf_counter() - t0, plan

@@ -75,6 +75,17 @@ def test_argument_validation_without_device():
                             ctypes.c_void_p(1), None) == -1
     assert lib.fcis_sigma_plan_execute(None, None, None, None) == -1
     assert lib.fcis_init_guess(10, 10, 0, 1e-3, ctypes.c_void_p(1), None) == -1
+    assert lib.fcis_csf_create(ctypes.byref(h), None, 1, None) == -1
+    assert lib.fcis_csf_det2csf(None, None, None, 1, None) == -1
+    assert lib.fcis_csf_csf2det(None, None, None, 1, None) == -1
+
+
+def test_csf_counts_match_oracle():
+    import oracle.csf as oc
+    for n, na, nb, sm in [(6, 3, 3, 1), (6, 3, 3, 3), (7, 4, 3, 2), (8, 4, 4, 5), (10, 5, 5, 1),
+                          (16, 8, 8, 1), (5, 3, 2, 4), (6, 4, 2, 1)]:
+        ref = oc.csf_shape(n, na, nb, sm)[1] if abs(na - nb) <= sm - 1 else 0
+        assert pkg.csf_fci.count_all_csfs(n, na, nb, sm) == ref
 
 
 def test_batch_manifest():
