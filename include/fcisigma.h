@@ -185,6 +185,70 @@ int fcis_csf_csf2det(const fcis_csf_t *csf, const double *d_csf, double *d_det,
 int fcis_csf_hdiag(fcis_csf_t *csf, const double *eri, const double *d_hdiag_det,
                    double *d_hdiag_csf, void *stream);
 
+/* ---- Many CAS problems per launch ---------------------------------------
+ *
+ * A batch of nprob independent CAS problems (spaces[p], absorbed pair
+ * matrices h2e[p] as in fcis_sigma_plan_create), each with nvec[p] <= 64
+ * vector slots (Davidson roots).  Vectors of all problems are packed into
+ * one flat device array: slot v of problem p starts at offsets[p] +
+ * v * ndet_p (fcis_batch_info).  fcis_batch_sigma computes, for every
+ * active slot, d_out = H_p d_in in one persistent kernel launch (replayed
+ * from a hipGraph captured on the first call with a given buffer pair) and
+ * zeroes inactive slots.  norb <= 16 per problem.  The spaces must outlive
+ * the batch.  The reference's solvers contract one problem per call
+ * (direct_spin1.contract_2e, pyscf/csf_fci/csf.py:449); this entry point
+ * has no single counterpart: it is the contract_2e of many fcisolver.kernel
+ * calls at once.
+ */
+typedef struct fcis_batch fcis_batch_t;
+int fcis_batch_create(fcis_batch_t **out, int nprob, const fcis_space_t *const *spaces,
+                      const double *const *h2e, const int *nvec, int precision,
+                      void *stream);
+void fcis_batch_free(fcis_batch_t *batch);
+/* total packed length; offsets (host array of nprob, may be NULL); the
+ * current schedule (alpha strings per work item, persistent workgroups). */
+int fcis_batch_info(const fcis_batch_t *batch, int64_t *total, int64_t *offsets,
+                    int *rows_per_item, int *grid);
+/* Host mask of sum(nvec) bytes, problem-major: nonzero = compute the slot. */
+int fcis_batch_set_active(fcis_batch_t *batch, const uint8_t *mask, void *stream);
+/* enable = 0 launches eagerly instead of through the captured graph. */
+int fcis_batch_set_graph(fcis_batch_t *batch, int enable);
+int fcis_batch_sigma(fcis_batch_t *batch, const double *d_in, double *d_out,
+                     void *stream);
+/* Autotuner: times candidate schedules (alpha strings per work item x grid
+ * size) on these buffers (d_out is overwritten), keeps the fastest and
+ * returns its time in *best_ms.  (Tile scheduling of the reference's
+ * OpenMP tasks is a cost model instead, fci_contract_gas.c:1845-2036.) */
+int fcis_batch_tune(fcis_batch_t *batch, const double *d_in, double *d_out, int reps,
+                    double *best_ms, void *stream);
+
+/* Vector algebra on many device vectors of different length, one launch
+ * per call (the Davidson work of many solves at once; pyscf.lib.davidson1's
+ * dot / axpy / precond, fci_gas.py:1337-1350). */
+typedef struct {
+        const double *x, *y;   /* device vectors of length n */
+        int64_t n;
+} fcis_dot_entry;
+typedef struct {
+        double *y;             /* y = beta*y + sum_{j<m} coef[off+j] xs[off+j] */
+        int64_t n;
+        double beta;           /* beta == 0: y is not read */
+        int32_t m, off;
+} fcis_lincomb_entry;
+typedef struct {
+        const double *hdiag, *r;
+        double *out;           /* out = r / (hdiag - e), |.| < level_shift -> level_shift */
+        int64_t n;
+        double e;
+} fcis_precond_entry;
+/* out: host array of nent dot products (fixed summation order). */
+int fcis_dots_list(int nent, const fcis_dot_entry *ent, double *out, void *stream);
+/* xs: host array of nterm device pointers; coef: host array of nterm. */
+int fcis_lincomb_list(int nent, const fcis_lincomb_entry *ent, int nterm,
+                      const double *const *xs, const double *coef, void *stream);
+int fcis_precond_list(int nent, const fcis_precond_entry *ent, double level_shift,
+                      void *stream);
+
 /* ---- Davidson primitives on device vectors (pyscf.lib.davidson1 work,
  *      called at fci_gas.py:1344-1350; vectors stay in HBM) ----------- */
 

@@ -21,3 +21,4 @@ from .direct_spin1 import (  # noqa: F401
 from . import direct_spin1  # noqa: F401,E402
 from . import autograd  # noqa: F401,E402
 from . import csf_fci  # noqa: F401,E402
+from . import batch  # noqa: F401,E402

@@ -55,6 +55,17 @@ SIGNATURES = {
     "fcis_csf_det2csf": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp]),
     "fcis_csf_csf2det": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp]),
     "fcis_csf_hdiag": (ctypes.c_int, [_vp, _dp, _vp, _vp, _vp]),
+    "fcis_batch_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _vp, _vp, _vp,
+                                         ctypes.c_int, _vp]),
+    "fcis_batch_free": (None, [_vp]),
+    "fcis_batch_info": (ctypes.c_int, [_vp, _i64p, _vp, _ip, _ip]),
+    "fcis_batch_set_active": (ctypes.c_int, [_vp, _vp, _vp]),
+    "fcis_batch_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "fcis_batch_sigma": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "fcis_batch_tune": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _dp, _vp]),
+    "fcis_dots_list": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp]),
+    "fcis_lincomb_list": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp]),
+    "fcis_precond_list": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_double, _vp]),
     "fcis_dots": (ctypes.c_int, [ctypes.c_int, _i64, _vp, _i64, _vp, _dp, _vp]),
     "fcis_lincomb": (ctypes.c_int, [ctypes.c_int, _i64, _vp, _i64, _dp,
                                     ctypes.c_double, _vp, _vp]),

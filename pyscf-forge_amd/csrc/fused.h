@@ -1,0 +1,252 @@
+// Building blocks of the fused row kernels (single problem: sigma.hip,
+// batched problems: batch.hip).  gfx950 only.
+#pragma once
+
+#include <vector>
+
+#include "common.h"
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int FUSED_NB = 16;      // determinants per chunk (one MFMA column block)
+constexpr int FUSED_MAXW = 9;     // wavefronts per workgroup (norb <= 16)
+constexpr int FUSED_KS = 34;      // k-steps held in registers (4-wide MFMA): npair <= 136
+constexpr int FUSED_KS_BF16 = 5;  // k-steps held in registers (32-wide bf16 MFMA)
+constexpr int FUSED_NACC = 2;     // independent accumulator chains (FP64/FP32)
+constexpr int FUSED_MAXNB = 16384;  // sigma row kept in LDS; pmapb fits int16
+
+template <int PREC>
+struct FusedTypes;
+template <>
+struct FusedTypes<FCIS_PREC_FP64> {
+        typedef double B;
+        typedef double A;
+        typedef v4d Acc;
+        typedef double Out;
+        static constexpr int KS = FUSED_KS;
+        static constexpr int KW = 4;
+        static constexpr int NBLD = 3;    // 9 + 3 wavefronts: 3 per SIMD at <= 168 VGPRs
+};
+template <>
+struct FusedTypes<FCIS_PREC_FP32> {
+        typedef float B;
+        typedef float A;
+        typedef v4f Acc;
+        typedef float Out;
+        static constexpr int KS = FUSED_KS;
+        static constexpr int KW = 4;
+        static constexpr int NBLD = 7;    // 9 + 7 wavefronts: 4 per SIMD at <= 128 VGPRs
+};
+template <>
+struct FusedTypes<FCIS_PREC_BF16X3> {
+        typedef float B;
+        typedef bf16x8 A;
+        typedef v4f Acc;
+        typedef float Out;
+        static constexpr int KS = FUSED_KS_BF16;
+        static constexpr int KW = 32;
+        static constexpr int NBLD = 7;
+};
+
+template <int PREC>
+constexpr size_t fused_lds_bytes(int64_t nb)
+{
+        typedef typename FusedTypes<PREC>::B BT;
+        return (size_t)((nb + FUSED_NB - 1) / FUSED_NB * FUSED_NB) * sizeof(double) +
+               2 * sizeof(BT) * FusedTypes<PREC>::KS * FusedTypes<PREC>::KW * FUSED_NB +
+               2 * sizeof(int) * 16 * FUSED_MAXW + 3 * sizeof(int16_t) * 16 * FUSED_MAXW * FUSED_NB;
+}
+
+// Builder wavefronts: gather the B tile of chunk j+1 (pair maps, then c)
+// while the MFMA wavefronts consume chunk j.
+// Builder state of one chunk: the B-tile element values in flight and the
+// next-but-one chunk's beta pair-map slice in flight.
+template <int NBLD, int MAXW = FUSED_MAXW>
+struct FusedBuild {
+        static constexpr int EB = (16 * MAXW * FUSED_NB + 64 * NBLD - 1) / (64 * NBLD);
+        static constexpr int EM = (2 * 16 * MAXW + 64 * NBLD - 1) / (64 * NBLD);
+        int pa[EB], pb[EB];
+        double va[EB], vb[EB];
+        v4i mv[EM];           // 8 int16 map entries per 16-byte load
+};
+
+// Issue the c loads of chunk j (element (k, col) of the tile is
+//   sa c[Ja, b0+col] + sb c[I, Jb],  arow[k] -> (Ja, sa), mb[k][col] -> (Jb, sb)),
+// addressed with 32-bit byte offsets (the fused path needs c below 4 GiB).
+template <int NBLD, int MAXW>
+__device__ __forceinline__ void fb_issue_c(FusedBuild<NBLD, MAXW> &st, int npair, int64_t nb, int64_t I,
+                                           int64_t j, int bt, const int *__restrict__ arow,
+                                           const int16_t *__restrict__ mb,
+                                           const double *__restrict__ c)
+{
+        constexpr int NB = FUSED_NB, EB = FusedBuild<NBLD, MAXW>::EB;
+        const int nbt = NBLD * 64, nelem = npair * NB;
+        const uint32_t b0 = (uint32_t)(j * NB);
+        const char *cb = reinterpret_cast<const char *>(c);
+        const uint32_t rowI = (uint32_t)(I * nb);
+#pragma unroll
+        for (int s = 0; s < EB; s++) {
+                int e = bt + s * nbt;
+                st.pa[s] = -1;
+                st.pb[s] = -1;
+                if (e < nelem) {
+                        int k = e / NB;
+                        if (b0 + (uint32_t)(e - k * NB) < (uint32_t)nb) {
+                                st.pa[s] = arow[k];
+                                st.pb[s] = mb[e];   // -1 where b has no link with pair k
+                        }
+                }
+        }
+#pragma unroll
+        for (int s = 0; s < EB; s++) {
+                int col = (bt + s * nbt) % NB;
+                uint32_t oa = ((uint32_t)(st.pa[s] >> 1) * (uint32_t)nb + b0 + col) * 8u;
+                uint32_t ob = (rowI + (uint32_t)(st.pb[s] >> 1)) * 8u;
+                st.va[s] = st.pa[s] >= 0 ? *reinterpret_cast<const double *>(cb + oa) : 0.0;
+                st.vb[s] = st.pb[s] >= 0 ? *reinterpret_cast<const double *>(cb + ob) : 0.0;
+        }
+}
+
+template <int NBLD, int MAXW, typename BT>
+__device__ __forceinline__ void fb_commit_c(FusedBuild<NBLD, MAXW> &st, int npair, int bt, BT *bb)
+{
+        constexpr int EB = FusedBuild<NBLD, MAXW>::EB;
+        const int nbt = NBLD * 64, nelem = npair * FUSED_NB;
+#pragma unroll
+        for (int s = 0; s < EB; s++) {
+                int e = bt + s * nbt;
+                if (e < nelem) {
+                        double v = ((st.pa[s] & 1) ? -st.va[s] : st.va[s]) +
+                                   ((st.pb[s] & 1) ? -st.vb[s] : st.vb[s]);
+                        bb[e] = (BT)v;
+                }
+        }
+}
+
+// Beta pair-map slice of chunk j, pmapb[k][b0 .. b0+NB) for k < npair (16
+// int16 = two 16-byte loads per pair; pmapb rows are padded to a multiple of
+// FUSED_NB entries with -1, so no load crosses a row).
+template <int NBLD, int MAXW>
+__device__ __forceinline__ void fb_issue_maps(FusedBuild<NBLD, MAXW> &st, int npair, int64_t nbp,
+                                              int64_t j, int bt, const int16_t *__restrict__ pmapb)
+{
+        constexpr int EM = FusedBuild<NBLD, MAXW>::EM;
+        const int64_t b0 = j * FUSED_NB;
+#pragma unroll
+        for (int s = 0; s < EM; s++) {
+                int t = bt + s * NBLD * 64;
+                if (t < 2 * npair) {
+                        int k = t >> 1, h = (t & 1) * 8;
+                        st.mv[s] = *reinterpret_cast<const v4i *>(pmapb + (int64_t)k * nbp + b0 + h);
+                }
+        }
+}
+
+template <int NBLD, int MAXW>
+__device__ __forceinline__ void fb_commit_maps(FusedBuild<NBLD, MAXW> &st, int npair, int bt,
+                                               int16_t *__restrict__ mb)
+{
+        constexpr int EM = FusedBuild<NBLD, MAXW>::EM;
+#pragma unroll
+        for (int s = 0; s < EM; s++) {
+                int t = bt + s * NBLD * 64;
+                if (t < 2 * npair) {
+                        int k = t >> 1, h = (t & 1) * 8;
+                        *reinterpret_cast<v4i *>(mb + k * FUSED_NB + h) = st.mv[s];
+                }
+        }
+}
+
+// One chunk of the contraction on MFMA: the 16 x 16 block
+//   acc[i][j] = sum_k h[16 w + i][k] t1[k][j],  k < 4 ksteps (32 for bf16x3),
+// for this wavefront's A fragments ah (and the bf16 low halves al) held in
+// registers and the B tile bb in LDS ([k][FUSED_NB] layout).
+template <int PREC, int KS>
+__device__ __forceinline__ typename FusedTypes<PREC>::Acc fused_mma(
+        const typename FusedTypes<PREC>::A (&ah)[KS],
+        const typename FusedTypes<PREC>::A (&al)[PREC == FCIS_PREC_BF16X3 ? KS : 1],
+        const typename FusedTypes<PREC>::B *bb, int ksteps, int lk, int li)
+{
+        typedef FusedTypes<PREC> Ty;
+        typedef typename Ty::B BT;
+        constexpr int NB = FUSED_NB;
+        typename Ty::Acc acc;
+        if constexpr (PREC == FCIS_PREC_BF16X3) {
+                typename Ty::Acc a0c = {0, 0, 0, 0}, a1c = a0c, a2c = a0c;
+#pragma unroll
+                for (int ks = 0; ks < KS; ks++) {
+                        if (ks < ksteps) {
+                                bf16x8 bh, bl;
+#pragma unroll
+                                for (int e = 0; e < 8; e++) {
+                                        float xv = bb[(32 * ks + 8 * lk + e) * NB + li];
+                                        __bf16 hi = (__bf16)xv;
+                                        bh[e] = hi;
+                                        bl[e] = (__bf16)(xv - (float)hi);
+                                }
+                                a0c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[ks], bh, a0c, 0, 0, 0);
+                                a1c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ks], bl, a1c, 0, 0, 0);
+                                a2c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ks], bh, a2c, 0, 0, 0);
+                        }
+                }
+                acc = (a0c + a1c) + a2c;
+        } else {
+                typename Ty::Acc ac[FUSED_NACC];
+#pragma unroll
+                for (int z = 0; z < FUSED_NACC; z++) ac[z] = typename Ty::Acc{0, 0, 0, 0};
+#pragma unroll
+                for (int ks = 0; ks < KS; ks++) {
+                        if (ks < ksteps) {
+                                BT bv = bb[(4 * ks + lk) * NB + li];
+                                if constexpr (PREC == FCIS_PREC_FP64)
+                                        ac[ks % FUSED_NACC] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                                                ah[ks], bv, ac[ks % FUSED_NACC], 0, 0, 0);
+                                else
+                                        ac[ks % FUSED_NACC] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                                                ah[ks], bv, ac[ks % FUSED_NACC], 0, 0, 0);
+                        }
+                }
+                acc = ac[0] + ac[1];
+        }
+        return acc;
+}
+
+}  // namespace
+
+// State behind fcis_sigma_plan_t (sigma.hip); the batched kernels of
+// batch.hip reuse its h fragments and pair maps.
+struct fcis_sigma_plan {
+        const fcis_space *space;
+        int precision;
+        int npair, mtiles, ksteps;   // ksteps: MFMA k-steps (4 for f64/f32, 32 for bf16)
+        void *d_hfrag;               // FP64 / FP32 fragments, or bf16 hi fragments
+        void *d_hfrag_lo;            // bf16 lo fragments (BF16X3 only)
+        int32_t *d_pmapa, *d_pmapb;
+        int64_t tile_a;
+        int ntiles;
+        void *d_t1, *d_vt1;
+        uint64_t ws_bytes;
+        std::vector<double> h;
+        int fused;                   // fused row kernel + alpha pull
+        int fused_grid;              // persistent workgroups of the fused kernel
+        size_t fused_lds;            // its dynamic LDS bytes
+        int16_t *d_pmapb16;          // beta pair map as int16, rows padded to nbp (fused path)
+        int64_t nbp;
+        int nbld;                    // builder wavefronts of the fused kernel
+        int timing;
+        hipEvent_t ev[4];
+        double ms[3];
+        int64_t launches[3];
+};
+
+namespace fcis {
+// fcis_sigma_plan_create; honour_env = false ignores FCIS_DETERMINISTIC
+// (the batch always needs the fused tables).
+int sigma_plan_create(fcis_sigma_plan **out, const fcis_space *space, const double *h2e,
+                      int precision, uint64_t workspace_bytes, bool honour_env);
+}  // namespace fcis
