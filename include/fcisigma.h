@@ -222,30 +222,34 @@ int fcis_batch_sigma(fcis_batch_t *batch, const double *d_in, double *d_out,
 int fcis_batch_tune(fcis_batch_t *batch, const double *d_in, double *d_out, int reps,
                     double *best_ms, void *stream);
 
-/* Vector algebra on many device vectors of different length, one launch
- * per call (the Davidson work of many solves at once; pyscf.lib.davidson1's
- * dot / axpy / precond, fci_gas.py:1337-1350). */
+/* Block vector algebra on many device vectors of different length, one
+ * launch per call (the Davidson work of many solves at once:
+ * pyscf.lib.davidson1's projections, axpy and precond, called at
+ * fci_gas.py:1337-1350).  An entry names nx vectors X = xs[xoff ..
+ * xoff+nx) and ny vectors Y = ys[yoff .. yoff+ny), all of length n
+ * (xs, ys: HOST arrays of device pointers). */
 typedef struct {
-        const double *x, *y;   /* device vectors of length n */
         int64_t n;
-} fcis_dot_entry;
-typedef struct {
-        double *y;             /* y = beta*y + sum_{j<m} coef[off+j] xs[off+j] */
-        int64_t n;
-        double beta;           /* beta == 0: y is not read */
-        int32_t m, off;
-} fcis_lincomb_entry;
+        int32_t nx, ny;        /* gemm: nx <= 128 */
+        int32_t xoff, yoff;
+        int32_t ooff;          /* gram: first output; gemm: first coefficient */
+        int32_t pad;
+        double beta;           /* gemm only; beta == 0: Y is not read */
+} fcis_block_entry;
 typedef struct {
         const double *hdiag, *r;
         double *out;           /* out = r / (hdiag - e), |.| < level_shift -> level_shift */
         int64_t n;
         double e;
 } fcis_precond_entry;
-/* out: host array of nent dot products (fixed summation order). */
-int fcis_dots_list(int nent, const fcis_dot_entry *ent, double *out, void *stream);
-/* xs: host array of nterm device pointers; coef: host array of nterm. */
-int fcis_lincomb_list(int nent, const fcis_lincomb_entry *ent, int nterm,
-                      const double *const *xs, const double *coef, void *stream);
+/* out[ooff + i*ny + j] = <X_i, Y_j> (out: HOST array of nout; partial sums
+ * added in a fixed order). */
+int fcis_gram_list(int nent, const fcis_block_entry *ent, int nxs, const double *const *xs,
+                   int nys, const double *const *ys, int nout, double *out, void *stream);
+/* Y_j = beta Y_j + sum_i coef[ooff + i*ny + j] X_i (coef: HOST array of
+ * ncoef).  Y may alias X (in-place block transforms) when ny <= 16. */
+int fcis_gemm_list(int nent, const fcis_block_entry *ent, int nxs, const double *const *xs,
+                   int nys, double *const *ys, int ncoef, const double *coef, void *stream);
 int fcis_precond_list(int nent, const fcis_precond_entry *ent, double level_shift,
                       void *stream);
 

@@ -63,8 +63,10 @@ SIGNATURES = {
     "fcis_batch_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fcis_batch_sigma": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "fcis_batch_tune": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _dp, _vp]),
-    "fcis_dots_list": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp]),
-    "fcis_lincomb_list": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp]),
+    "fcis_gram_list": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp,
+                                      ctypes.c_int, _vp, _vp]),
+    "fcis_gemm_list": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp,
+                                      ctypes.c_int, _vp, _vp]),
     "fcis_precond_list": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_double, _vp]),
     "fcis_dots": (ctypes.c_int, [ctypes.c_int, _i64, _vp, _i64, _vp, _dp, _vp]),
     "fcis_lincomb": (ctypes.c_int, [ctypes.c_int, _i64, _vp, _i64, _dp,

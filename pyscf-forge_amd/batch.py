@@ -6,8 +6,8 @@ hipGraph.  ``solve_batch`` runs the Davidson iterations of all problems in
 lockstep (each problem keeps its own subspace, Ritz pairs and convergence
 flags, as in one fcisolver.kernel call per problem, fci_gas.py:1150-1399),
 so that one iteration of the whole batch costs one batched sigma launch
-plus a handful of list-form vector kernels (fcis_dots_list,
-fcis_lincomb_list, fcis_precond_list) whatever the number of problems.
+plus a handful of block vector kernels (fcis_gram_list, fcis_gemm_list,
+fcis_precond_list) whatever the number of problems.
 """
 
 import ctypes
@@ -19,52 +19,61 @@ import numpy
 from . import _lib
 from . import direct_spin1 as _ds
 
-DOT_DTYPE = numpy.dtype([("x", "<u8"), ("y", "<u8"), ("n", "<i8")])
-LIN_DTYPE = numpy.dtype([("y", "<u8"), ("n", "<i8"), ("beta", "<f8"), ("m", "<i4"), ("off", "<i4")])
+BLOCK_DTYPE = numpy.dtype([("n", "<i8"), ("nx", "<i4"), ("ny", "<i4"), ("xoff", "<i4"),
+                         ("yoff", "<i4"), ("ooff", "<i4"), ("pad", "<i4"), ("beta", "<f8")])
 PRE_DTYPE = numpy.dtype([("hdiag", "<u8"), ("r", "<u8"), ("out", "<u8"), ("n", "<i8"),
                          ("e", "<f8")])
 
 
-class ListOps:
-    """Vector algebra over many device vectors of different lengths, one
-    kernel launch per call.  Vectors are given by device address and length."""
+class BlockOps:
+    """Block vector algebra over many device vectors of different lengths,
+    one kernel launch per call (fcis_gram_list / fcis_gemm_list /
+    fcis_precond_list).  A block is an array of device addresses."""
 
     def __init__(self):
         self.lib = _lib.load()
 
-    def dots(self, x, y, n):
-        """[<x_i, y_i>] for address arrays x, y and lengths n."""
-        m = len(n)
-        if m == 0:
-            return numpy.zeros(0)
-        ent = numpy.empty(m, DOT_DTYPE)
-        ent["x"] = x
-        ent["y"] = y
-        ent["n"] = n
-        out = numpy.empty(m)
-        _lib.check(self.lib.fcis_dots_list(m, ent.ctypes.data, out.ctypes.data,
-                                           _lib.stream_ptr()), "fcis_dots_list")
-        return out
+    @staticmethod
+    def _entries(blocks, nout_fn):
+        ent = numpy.zeros(len(blocks), BLOCK_DTYPE)
+        xo = yo = oo = 0
+        for i, b in enumerate(blocks):
+            nx, ny = len(b[0]), len(b[1])
+            ent[i] = (b[2], nx, ny, xo, yo, oo, 0, b[4] if len(b) > 4 else 0.0)
+            xo += nx
+            yo += ny
+            oo += nout_fn(nx, ny)
+        xs = numpy.concatenate([numpy.asarray(b[0], dtype=numpy.uint64) for b in blocks])
+        ys = numpy.concatenate([numpy.asarray(b[1], dtype=numpy.uint64) for b in blocks])
+        return ent, xs, ys, oo
 
-    def lincomb(self, entries):
-        """entries: list of (y_addr, n, beta, [(x_addr, coef), ...]):
-        y = beta*y + sum coef*x."""
-        if not entries:
+    def gram(self, blocks):
+        """blocks: list of (x_addrs, y_addrs, n) -> list of (nx, ny) X^T Y."""
+        if not blocks:
+            return []
+        ent, xs, ys, nout = self._entries(blocks, lambda nx, ny: nx * ny)
+        out = numpy.empty(max(nout, 1))
+        _lib.check(self.lib.fcis_gram_list(len(blocks), ent.ctypes.data, len(xs), xs.ctypes.data,
+                                           len(ys), ys.ctypes.data, nout, out.ctypes.data,
+                                           _lib.stream_ptr()), "fcis_gram_list")
+        res, o = [], 0
+        for b in blocks:
+            nx, ny = len(b[0]), len(b[1])
+            res.append(out[o:o + nx * ny].reshape(nx, ny))
+            o += nx * ny
+        return res
+
+    def gemm(self, blocks):
+        """blocks: list of (x_addrs, y_addrs, n, coef (nx, ny), beta):
+        Y = beta Y + X coef."""
+        if not blocks:
             return
-        nterm = sum(len(e[3]) for e in entries)
-        ent = numpy.empty(len(entries), LIN_DTYPE)
-        xs = numpy.empty(max(nterm, 1), numpy.uint64)
-        cf = numpy.empty(max(nterm, 1))
-        off = 0
-        for i, (y, n, beta, terms) in enumerate(entries):
-            ent[i] = (y, n, beta, len(terms), off)
-            for x, c in terms:
-                xs[off] = x
-                cf[off] = c
-                off += 1
-        _lib.check(self.lib.fcis_lincomb_list(len(entries), ent.ctypes.data, nterm,
-                                              xs.ctypes.data, cf.ctypes.data, _lib.stream_ptr()),
-                   "fcis_lincomb_list")
+        ent, xs, ys, ncoef = self._entries(blocks, lambda nx, ny: nx * ny)
+        cf = numpy.concatenate([numpy.ascontiguousarray(b[3], dtype=numpy.float64).reshape(-1)
+                                for b in blocks] + [numpy.zeros(1)])
+        _lib.check(self.lib.fcis_gemm_list(len(blocks), ent.ctypes.data, len(xs), xs.ctypes.data,
+                                           len(ys), ys.ctypes.data, ncoef, cf.ctypes.data,
+                                           _lib.stream_ptr()), "fcis_gemm_list")
 
     def precond(self, entries, level_shift):
         """entries: list of (hdiag_addr, r_addr, out_addr, n, e)."""
@@ -191,30 +200,34 @@ def _prepare(problems):
 
 
 def solve_batch(problems, nroots=4, tol=1e-10, tol_residual=None, max_cycle=100, max_space=12,
-                lindep=1e-14, level_shift=1e-3, tune=True, use_graph=True, verbose=False):
+                lindep=1e-14, level_shift=1e-3, tune=True, use_graph=True, verbose=False,
+                nguard=0):
     """Lowest ``nroots`` states of every problem (h1e, eri, norb, nelec).
 
     Davidson per problem as direct_spin1.FCISolver.kernel (initial guess of
     fci_gas.py:881-902, preconditioner :1337-1342, convergence |dE| < tol
     and |r| < tol_residual = sqrt(tol), thick restart), all problems advancing
-    together.  Returns a BatchResult (energies without ecore)."""
+    together; nguard extra roots are iterated but not required to converge.
+    Returns a BatchResult (energies without ecore)."""
     import torch
     t_setup = time.perf_counter()
     lib = _lib.load()
-    ops = ListOps()
+    ops = BlockOps()
     stream = _lib.stream_ptr()
     if tol_residual is None:
         tol_residual = math.sqrt(tol)
     spaces, h2es, ints = _prepare(problems)
     P = len(spaces)
     nd = numpy.array([s.ndet for s in spaces], dtype=numpy.int64)
-    kk = numpy.minimum(int(nroots), nd).astype(int)
+    kreq = numpy.minimum(int(nroots), nd).astype(int)
+    kk = numpy.minimum(int(nroots) + int(nguard), nd).astype(int)
+    if kk.max() > 8:
+        raise ValueError("at most 8 roots (with guards) per problem")
     cap = numpy.array([min(max(int(max_space) + (k - 1) * 3, 2 * k), 64, n)
                        for k, n in zip(kk, nd)], dtype=int)
     bs = BatchSigma(spaces, h2es, kk)
     tin, tout = bs.empty(), bs.empty()
 
-    # packed device storage
     def packed(counts):
         off = numpy.concatenate([[0], numpy.cumsum(counts * nd)[:-1]]).astype(numpy.int64)
         buf = torch.empty(int((counts * nd).sum()), dtype=torch.float64, device="cuda")
@@ -222,20 +235,16 @@ def solve_batch(problems, nroots=4, tol=1e-10, tol_residual=None, max_cycle=100,
 
     X, xoff = packed(cap)
     AX, _ = packed(cap)
-    XR, roff = packed(kk)
-    AXR, _ = packed(kk)
-    XP, _ = packed(kk)
-    AXP, _ = packed(kk)
     HD, hoff = packed(numpy.ones(P, dtype=int))
-    b8 = 8
+    stride = 8 * nd
+    xbase = X.data_ptr() + 8 * xoff
+    axbase = AX.data_ptr() + 8 * xoff
+    tbase = tin.data_ptr() + 8 * bs.offsets
+    obase = tout.data_ptr() + 8 * bs.offsets
 
-    def addr(buf, off, p, i):
-        return buf.data_ptr() + b8 * int(off[p] + i * nd[p])
+    def vecs(base, p, i0, i1):
+        return base[p] + stride[p] * numpy.arange(i0, i1, dtype=numpy.int64)
 
-    tin_addr = lambda p, i: tin.data_ptr() + b8 * int(bs.offsets[p] + i * nd[p])   # noqa: E731
-    tout_addr = lambda p, i: tout.data_ptr() + b8 * int(bs.offsets[p] + i * nd[p])  # noqa: E731
-
-    # hdiag and initial guesses (lowest diagonal + splitmix64 noise)
     for p, (sp, (h1, er)) in enumerate(zip(spaces, ints)):
         hd = HD[int(hoff[p]):int(hoff[p] + nd[p])]
         _lib.check(lib.fcis_make_hdiag(sp.handle, _lib.dptr(h1), _lib.dptr(er), _lib.vptr(hd),
@@ -243,7 +252,7 @@ def solve_batch(problems, nroots=4, tol=1e-10, tol_residual=None, max_cycle=100,
         addrs = torch.topk(hd, int(kk[p]), largest=False).indices.cpu().numpy()
         for r, a in enumerate(addrs):
             _lib.check(lib.fcis_init_guess(int(nd[p]), int(a), r, _ds.INIT_NOISE_NORM,
-                                           ctypes.c_void_p(tin_addr(p, r)), stream),
+                                           ctypes.c_void_p(int(tbase[p] + stride[p] * r)), stream),
                        "fcis_init_guess")
     torch.cuda.synchronize()
     if tune:
@@ -256,106 +265,84 @@ def solve_batch(problems, nroots=4, tol=1e-10, tol_residual=None, max_cycle=100,
     e_old = [numpy.full(k, numpy.inf) for k in kk]
     conv = [numpy.zeros(k, dtype=bool) for k in kk]
     rnorm = [numpy.full(k, numpy.inf) for k in kk]
+    vfin = [None] * P
     done = numpy.zeros(P, dtype=bool)
     cycles = numpy.zeros(P, dtype=int)
     nhop = numpy.zeros(P, dtype=int)
     v_prev = [None] * P
-    nsigma = 0
-    t_sigma = 0.0
+    tstat = dict(nsigma=0, t_sigma=0.0)
 
-    def orth_and_apply(newcnt):
-        """The first newcnt[p] vectors of tin (problem p) are candidates:
-        project out X[p][:space], orthonormalise within the block (dropping
-        dependent directions), append to X, apply H, append to AX, update
-        heff.  Returns the number of vectors added per problem."""
-        nonlocal nsigma, t_sigma
-        act = [p for p in range(P) if newcnt[p] > 0]
-        norm0 = {}
+    def project_out(act, cnt):
+        """T[p][0..cnt) -= X[p] (X[p]^T T[p]) (one classical Gram-Schmidt pass)."""
+        pr = [p for p in act if space[p] > 0 and cnt[p] > 0]
+        if not pr:
+            return
+        G = ops.gram([(vecs(xbase, p, 0, space[p]), vecs(tbase, p, 0, cnt[p]), nd[p]) for p in pr])
+        ops.gemm([(vecs(xbase, p, 0, space[p]), vecs(tbase, p, 0, cnt[p]), nd[p], -g, 1.0)
+                  for p, g in zip(pr, G)])
+
+    def orthonormalise(act, cnt, thresh):
+        """T[p][0..cnt) -> orthonormal T[p][0..a) spanning the directions of
+        the block Gram matrix above thresh[p] (in place); returns a."""
+        pr = [p for p in act if cnt[p] > 0]
+        G = ops.gram([(vecs(tbase, p, 0, cnt[p]), vecs(tbase, p, 0, cnt[p]), nd[p]) for p in pr])
+        got = numpy.zeros(P, dtype=int)
+        blocks = []
+        for p, g in zip(pr, G):
+            w, u = numpy.linalg.eigh(0.5 * (g + g.T))
+            keep = numpy.nonzero(w > thresh[p])[0][::-1]
+            keep = keep[:cap[p] - space[p]]
+            if len(keep):
+                blocks.append((vecs(tbase, p, 0, cnt[p]), vecs(tbase, p, 0, len(keep)), nd[p],
+                               u[:, keep] / numpy.sqrt(w[keep]), 0.0))
+            got[p] = len(keep)
+        ops.gemm(blocks)
+        return got
+
+    def orth_and_apply(cnt):
+        """Candidates T[p][0..cnt): CGS2 against X[p], two-pass block
+        orthonormalisation, append to X, apply H, append to AX, extend heff."""
+        act = [p for p in range(P) if cnt[p] > 0]
+        if not act:
+            return numpy.zeros(P, dtype=int)
+        pr = [p for p in act if space[p] > 0]
+        # norms of the raw candidates, for the relative dependence test
+        G0 = ops.gram([(vecs(tbase, p, 0, cnt[p]), vecs(tbase, p, 0, cnt[p]), nd[p]) for p in act])
+        n0 = {p: float(numpy.diag(g).max()) for p, g in zip(act, G0)}
+        project_out(pr, cnt)
+        project_out(pr, cnt)
+        th1 = {p: max(lindep, 1e-14 * n0[p]) for p in act}
+        a1 = orthonormalise(act, cnt, th1)
+        project_out(pr, a1)
+        a2 = orthonormalise(act, a1, {p: 1e-8 for p in act})
+        act = [p for p in act if a2[p] > 0]
+        # append to X (copy; T keeps the vectors as the sigma input)
+        ops.gemm([(vecs(tbase, p, 0, a2[p]), vecs(xbase, p, space[p], space[p] + a2[p]), nd[p],
+                   numpy.eye(a2[p]), 0.0) for p in act])
         if act:
-            d = ops.dots([tin_addr(p, j) for p in act for j in range(newcnt[p])],
-                         [tin_addr(p, j) for p in act for j in range(newcnt[p])],
-                         [nd[p] for p in act for j in range(newcnt[p])])
-            i = 0
+            mask = numpy.zeros(bs.nslots, dtype=numpy.uint8)
             for p in act:
-                norm0[p] = numpy.sqrt(numpy.maximum(d[i:i + newcnt[p]], 0.0))
-                i += newcnt[p]
-        for _ in range(2):
-            pr = [p for p in act if space[p] > 0]
-            if not pr:
-                break
-            xa, ya, ln, idx = [], [], [], []
-            for p in pr:
-                for j in range(newcnt[p]):
-                    for i in range(space[p]):
-                        xa.append(addr(X, xoff, p, i))
-                        ya.append(tin_addr(p, j))
-                        ln.append(nd[p])
-                        idx.append((p, j, i))
-            c = ops.dots(xa, ya, ln)
-            coef = {}
-            for (p, j, i), v in zip(idx, c):
-                coef.setdefault((p, j), []).append((addr(X, xoff, p, i), -v))
-            ops.lincomb([(tin_addr(p, j), nd[p], 1.0, coef[(p, j)]) for p in pr
-                         for j in range(newcnt[p])])
-        # Gram matrices of the blocks
-        xa, ya, ln, idx = [], [], [], []
-        for p in act:
-            for j in range(newcnt[p]):
-                for l in range(j + 1):
-                    xa.append(tin_addr(p, j))
-                    ya.append(tin_addr(p, l))
-                    ln.append(nd[p])
-                    idx.append((p, j, l))
-        g = ops.dots(xa, ya, ln)
-        G = {p: numpy.zeros((newcnt[p], newcnt[p])) for p in act}
-        for (p, j, l), v in zip(idx, g):
-            G[p][j, l] = G[p][l, j] = v
-        added = numpy.zeros(P, dtype=int)
-        ent = []
-        for p in act:
-            w, u = numpy.linalg.eigh(G[p])
-            keep = [q for q in range(len(w)) if w[q] > lindep and
-                    math.sqrt(max(w[q], 0.0)) > 1e-7 * float(norm0[p].max())]
-            keep = keep[::-1][:cap[p] - space[p]]      # largest first
-            for qi, q in enumerate(keep):
-                terms = [(tin_addr(p, j), u[j, q] / math.sqrt(w[q])) for j in range(newcnt[p])]
-                ent.append((addr(X, xoff, p, space[p] + qi), nd[p], 0.0, terms))
-            added[p] = len(keep)
-        ops.lincomb(ent)
-        # stage the new vectors for sigma
-        ops.lincomb([(tin_addr(p, q), nd[p], 0.0, [(addr(X, xoff, p, space[p] + q), 1.0)])
-                     for p in act for q in range(added[p])])
-        mask = numpy.zeros(bs.nslots, dtype=numpy.uint8)
-        for p in act:
-            mask[bs.slot_base[p]:bs.slot_base[p] + added[p]] = 1
-        if mask.any():
+                mask[bs.slot_base[p]:bs.slot_base[p] + a2[p]] = 1
             bs.set_active(mask)
             t0 = time.perf_counter()
             bs(tin, tout)
             torch.cuda.synchronize()
-            t_sigma += time.perf_counter() - t0
-            nsigma += 1
-            ops.lincomb([(addr(AX, xoff, p, space[p] + q), nd[p], 0.0, [(tout_addr(p, q), 1.0)])
-                         for p in act for q in range(added[p])])
-            nhop[:] += added
-            # heff columns
-            xa, ya, ln, idx = [], [], [], []
-            for p in act:
-                for q in range(added[p]):
-                    col = space[p] + q
-                    for i in range(col + 1):
-                        xa.append(addr(X, xoff, p, i))
-                        ya.append(addr(AX, xoff, p, col))
-                        ln.append(nd[p])
-                        idx.append((p, i, col))
-            h = ops.dots(xa, ya, ln)
-            for (p, i, col), v in zip(idx, h):
-                heff[p][i, col] = heff[p][col, i] = v
-        space[:] += added
-        return added
+            tstat["t_sigma"] += time.perf_counter() - t0
+            tstat["nsigma"] += 1
+            ops.gemm([(vecs(obase, p, 0, a2[p]), vecs(axbase, p, space[p], space[p] + a2[p]), nd[p],
+                       numpy.eye(a2[p]), 0.0) for p in act])
+            Hc = ops.gram([(vecs(xbase, p, 0, space[p] + a2[p]),
+                            vecs(axbase, p, space[p], space[p] + a2[p]), nd[p]) for p in act])
+            for p, hcol in zip(act, Hc):
+                s0, s1 = space[p], space[p] + a2[p]
+                heff[p][:s1, s0:s1] = hcol
+                heff[p][s0:s1, :s1] = hcol.T
+                nhop[p] += a2[p]
+                space[p] = s1
+        return a2
 
     t_iter = time.perf_counter()
-    added = orth_and_apply(kk.copy())
+    orth_and_apply(kk.copy())
     if (space < kk).any():
         raise RuntimeError("Davidson: initial guesses are linearly dependent")
 
@@ -364,62 +351,41 @@ def solve_batch(problems, nroots=4, tol=1e-10, tol_residual=None, max_cycle=100,
         if not act:
             break
         V = {}
-        ritz = []
         for p in act:
-            s = space[p]
-            h = 0.5 * (heff[p][:s, :s] + heff[p][:s, :s].T)
+            sp_ = space[p]
+            h = 0.5 * (heff[p][:sp_, :sp_] + heff[p][:sp_, :sp_].T)
             w, v = numpy.linalg.eigh(h)
             V[p] = (w[:kk[p]].copy(), v[:, :kk[p]].copy(), h)
             cycles[p] = icyc + 1
-        # Ritz vectors and their images
-        ent = []
-        for p in act:
-            e, v, _ = V[p]
-            for k in range(kk[p]):
-                ent.append((addr(XR, roff, p, k), nd[p], 0.0,
-                            [(addr(X, xoff, p, i), v[i, k]) for i in range(space[p])]))
-                ent.append((addr(AXR, roff, p, k), nd[p], 0.0,
-                            [(addr(AX, xoff, p, i), v[i, k]) for i in range(space[p])]))
-        ops.lincomb(ent)
-        # residuals r = A x - e x into tin
-        ops.lincomb([(tin_addr(p, k), nd[p], 0.0, [(addr(AXR, roff, p, k), 1.0),
-                                                   (addr(XR, roff, p, k), -V[p][0][k])])
-                     for p in act for k in range(kk[p])])
-        rr = ops.dots([tin_addr(p, k) for p in act for k in range(kk[p])],
-                      [tin_addr(p, k) for p in act for k in range(kk[p])],
-                      [nd[p] for p in act for k in range(kk[p])])
-        i = 0
+        # residuals r_j = sum_i V_ij (AX_i - e_j X_i) into tout
+        ops.gemm([(numpy.concatenate([vecs(xbase, p, 0, space[p]), vecs(axbase, p, 0, space[p])]),
+                   vecs(obase, p, 0, kk[p]), nd[p],
+                   numpy.vstack([-V[p][1] * V[p][0][None, :], V[p][1]]), 0.0) for p in act])
+        R = ops.gram([(vecs(obase, p, 0, kk[p]), vecs(obase, p, 0, kk[p]), nd[p]) for p in act])
         newcnt = numpy.zeros(P, dtype=int)
         pre = []
-        for p in act:
+        for p, rr in zip(act, R):
             e = V[p][0]
             de = e - e_old[p]
-            rn = numpy.sqrt(numpy.maximum(rr[i:i + kk[p]], 0.0))
-            i += kk[p]
-            rnorm[p] = rn
-            conv[p] = (numpy.abs(de) < tol) & (rn < tol_residual)
+            rnorm[p] = numpy.sqrt(numpy.maximum(numpy.diag(rr), 0.0))
+            conv[p] = (numpy.abs(de) < tol) & (rnorm[p] < tol_residual)
             e_old[p] = e
-            if conv[p].all():
+            if conv[p][:kreq[p]].all():
                 done[p] = True
+                vfin[p] = V[p][1][:, :kreq[p]].copy()
                 continue
             for k in numpy.nonzero(~conv[p])[0]:
-                pre.append((HD.data_ptr() + b8 * int(hoff[p]), tin_addr(p, k),
-                            tin_addr(p, newcnt[p]), nd[p], float(e[k])))
+                pre.append((int(HD.data_ptr() + 8 * hoff[p]), int(obase[p] + stride[p] * k),
+                            int(tbase[p] + stride[p] * newcnt[p]), int(nd[p]), float(e[k])))
                 newcnt[p] += 1
         if verbose:
             print("batch davidson %d: %d active, max |r| %.2e" % (
                 icyc, len(act), max(float(rnorm[p].max()) for p in act)))
-        # preconditioned residuals land in tout at their root's slot, then
-        # move to the compacted candidate slots of tin (no entry reads a
-        # slot another entry writes)
-        if pre:
-            shift = tout.data_ptr() - tin.data_ptr()
-            ops.precond([(hd_a, r_a, r_a + shift, n, e) for (hd_a, r_a, _, n, e) in pre],
-                        level_shift)
-            ops.lincomb([(o_a, n, 0.0, [(r_a + shift, 1.0)]) for (_, r_a, o_a, n, _) in pre])
+        ops.precond(pre, level_shift)
         if done.all():
             break
-        # thick restart where the new block does not fit
+        # thick restart where the new block does not fit: X <- X keep in place
+        rs = []
         for p in act:
             if done[p] or space[p] + newcnt[p] <= cap[p]:
                 continue
@@ -434,27 +400,18 @@ def solve_batch(problems, nroots=4, tol=1e-10, tol_residual=None, max_cycle=100,
                     nrm = numpy.linalg.norm(q)
                     if nrm > 1e-3 and keep.shape[1] < min(2 * kk[p], cap[p] - newcnt[p]):
                         keep = numpy.hstack([keep, (q / nrm)[:, None]])
-            nk = keep.shape[1]
-            ent = []
-            for k in range(nk - kk[p]):
-                ent.append((addr(XP, roff, p, k), nd[p], 0.0,
-                            [(addr(X, xoff, p, i), keep[i, kk[p] + k]) for i in range(space[p])]))
-                ent.append((addr(AXP, roff, p, k), nd[p], 0.0,
-                            [(addr(AX, xoff, p, i), keep[i, kk[p] + k]) for i in range(space[p])]))
-            ops.lincomb(ent)
-            ent = []
-            for k in range(kk[p]):
-                ent.append((addr(X, xoff, p, k), nd[p], 0.0, [(addr(XR, roff, p, k), 1.0)]))
-                ent.append((addr(AX, xoff, p, k), nd[p], 0.0, [(addr(AXR, roff, p, k), 1.0)]))
-            for k in range(nk - kk[p]):
-                ent.append((addr(X, xoff, p, kk[p] + k), nd[p], 0.0, [(addr(XP, roff, p, k), 1.0)]))
-                ent.append((addr(AX, xoff, p, kk[p] + k), nd[p], 0.0, [(addr(AXP, roff, p, k), 1.0)]))
-            ops.lincomb(ent)
-            hk = keep.T @ h @ keep
-            heff[p][:] = 0.0
-            heff[p][:nk, :nk] = 0.5 * (hk + hk.T)
-            space[p] = nk
-            V[p] = (e, numpy.eye(nk)[:, :kk[p]], None)
+            rs.append((p, keep))
+        if rs:
+            ops.gemm([b for p, keep in rs for b in (
+                (vecs(xbase, p, 0, space[p]), vecs(xbase, p, 0, keep.shape[1]), nd[p], keep, 0.0),
+                (vecs(axbase, p, 0, space[p]), vecs(axbase, p, 0, keep.shape[1]), nd[p], keep, 0.0))])
+            for p, keep in rs:
+                nk = keep.shape[1]
+                hk = keep.T @ V[p][2] @ keep
+                heff[p][:] = 0.0
+                heff[p][:nk, :nk] = 0.5 * (hk + hk.T)
+                space[p] = nk
+                V[p] = (V[p][0], numpy.eye(nk)[:, :kk[p]], None)
         for p in act:
             if not done[p]:
                 v_prev[p] = V[p][1]
@@ -462,15 +419,28 @@ def solve_batch(problems, nroots=4, tol=1e-10, tol_residual=None, max_cycle=100,
         for p in act:
             if not done[p] and newcnt[p] > 0 and added[p] == 0:
                 done[p] = True      # no new direction: stop this problem
+                vfin[p] = V[p][1][:, :kreq[p]].copy()
+    # problems still running at max_cycle: Ritz vectors of their last step
+    for p in range(P):
+        if vfin[p] is None:
+            sp_ = space[p]
+            h = 0.5 * (heff[p][:sp_, :sp_] + heff[p][:sp_, :sp_].T)
+            w, v = numpy.linalg.eigh(h)
+            vfin[p] = v[:, :kreq[p]]
+    XR, roff = packed(kreq)
+    rbase = XR.data_ptr() + 8 * roff
+    ops.gemm([(vecs(xbase, p, 0, vfin[p].shape[0]), vecs(rbase, p, 0, kreq[p]), nd[p], vfin[p], 0.0)
+              for p in range(P)])
     torch.cuda.synchronize()
     t_iter = time.perf_counter() - t_iter
-    ci = [XR[int(roff[p]):int(roff[p] + kk[p] * nd[p])].view(int(kk[p]), int(nd[p]))
+    ci = [XR[int(roff[p]):int(roff[p] + kreq[p] * nd[p])].view(int(kreq[p]), int(nd[p]))
           for p in range(P)]
     rows, grid = bs.schedule()
-    stats = dict(time_setup=t_setup, time_iterations=t_iter, sigma_launches=nsigma,
-                 time_sigma=t_sigma, rows_per_item=rows, grid=grid, tuned_ms=bs.tuned_ms,
+    stats = dict(time_setup=t_setup, time_iterations=t_iter, sigma_launches=tstat["nsigma"],
+                 time_sigma=tstat["t_sigma"], rows_per_item=rows, grid=grid, tuned_ms=bs.tuned_ms,
                  ndet_total=int(nd.sum()))
-    res = BatchResult([e.copy() for e in e_old], [c.copy() for c in conv], cycles.copy(),
-                      nhop.copy(), [r.copy() for r in rnorm], ci, stats)
-    res._keep = (bs, X, AX, XR, AXR, XP, AXP, HD, tin, tout)
+    res = BatchResult([e[:k].copy() for e, k in zip(e_old, kreq)],
+                      [c[:k].copy() for c, k in zip(conv, kreq)], cycles.copy(), nhop.copy(),
+                      [r[:k].copy() for r, k in zip(rnorm, kreq)], ci, stats)
+    res._keep = (bs, X, AX, HD, tin, tout)
     return res

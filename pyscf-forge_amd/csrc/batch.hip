@@ -27,12 +27,7 @@
 // own alpha destination ranges the same way, fci_contract_gas.c:1934-2036);
 // the batching itself has no counterpart in the reference, whose solvers
 // handle one problem per call.
-//
-// The list kernels at the end apply the Davidson vector algebra (dot
-// products, linear combinations, diagonal preconditioning) to many vectors
-// of different length in one launch each.
 #include <algorithm>
-#include <mutex>
 #include <new>
 #include <stdlib.h>
 #include <string.h>
@@ -259,103 +254,6 @@ __global__ void k_batch_reset(int64_t n, double *__restrict__ out, int *__restri
         int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
         if (i == 0) *counter = 0;
         if (i < n) out[i] = 0.0;
-}
-
-// ---------------------------------------------------------------- list ops
-
-constexpr int LIST_THREADS = 256;
-constexpr int64_t LIST_CHUNK = 65536;   // elements per workgroup task
-
-struct ListTask {
-        int32_t entry, pad;
-        int64_t start, len;
-};
-
-__global__ __launch_bounds__(LIST_THREADS) void k_dots_list(const ListTask *__restrict__ tasks,
-                                                            const fcis_dot_entry *__restrict__ ent,
-                                                            double *__restrict__ part)
-{
-        __shared__ double red[LIST_THREADS];
-        const ListTask t = tasks[blockIdx.x];
-        const fcis_dot_entry E = ent[t.entry];
-        const double *x = E.x + t.start, *y = E.y + t.start;
-        double s = 0.0;
-        for (int64_t k = threadIdx.x; k < t.len; k += LIST_THREADS) s += x[k] * y[k];
-        red[threadIdx.x] = s;
-        __syncthreads();
-        for (int w = LIST_THREADS / 2; w > 0; w >>= 1) {
-                if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-                __syncthreads();
-        }
-        if (threadIdx.x == 0) part[blockIdx.x] = red[0];
-}
-
-__global__ __launch_bounds__(LIST_THREADS) void k_lincomb_list(const ListTask *__restrict__ tasks,
-                                                               const fcis_lincomb_entry *__restrict__ ent,
-                                                               const double *const *__restrict__ xs,
-                                                               const double *__restrict__ coef)
-{
-        const ListTask t = tasks[blockIdx.x];
-        const fcis_lincomb_entry E = ent[t.entry];
-        for (int64_t k = threadIdx.x; k < t.len; k += LIST_THREADS) {
-                const int64_t i = t.start + k;
-                double acc = E.beta == 0.0 ? 0.0 : E.beta * E.y[i];
-                for (int j = 0; j < E.m; j++) acc += coef[E.off + j] * xs[E.off + j][i];
-                E.y[i] = acc;
-        }
-}
-
-__global__ __launch_bounds__(LIST_THREADS) void k_precond_list(const ListTask *__restrict__ tasks,
-                                                               const fcis_precond_entry *__restrict__ ent,
-                                                               double shift)
-{
-        const ListTask t = tasks[blockIdx.x];
-        const fcis_precond_entry E = ent[t.entry];
-        for (int64_t k = threadIdx.x; k < t.len; k += LIST_THREADS) {
-                const int64_t i = t.start + k;
-                double d = E.hdiag[i] - E.e;
-                if (fabs(d) < shift) d = shift;
-                E.out[i] = E.r[i] / d;
-        }
-}
-
-// Pinned host staging + device table buffer shared by the list calls
-// (grown on demand; one call at a time per process).
-struct ListStage {
-        char *host = nullptr, *dev = nullptr;
-        size_t cap = 0;
-};
-ListStage g_stage[16];
-std::mutex g_stage_mu;
-
-int stage_reserve(ListStage &s, size_t bytes)
-{
-        if (bytes <= s.cap) return FCIS_SUCCESS;
-        size_t cap = std::max(bytes, (size_t)1 << 20);
-        if (s.host) (void)hipHostFree(s.host);
-        if (s.dev) (void)hipFree(s.dev);
-        s.host = s.dev = nullptr;
-        s.cap = 0;
-        if (hipHostMalloc((void **)&s.host, cap) != hipSuccess ||
-            hipMalloc((void **)&s.dev, cap) != hipSuccess) {
-                fcis::set_error("list ops: staging allocation of %zu bytes failed", cap);
-                return FCIS_ERR_MEMORY;
-        }
-        s.cap = cap;
-        return FCIS_SUCCESS;
-}
-
-size_t align_up(size_t x)
-{
-        return (x + 255) & ~(size_t)255;
-}
-
-void make_tasks(std::vector<ListTask> &tasks, int nent, const int64_t *lens)
-{
-        tasks.clear();
-        for (int e = 0; e < nent; e++)
-                for (int64_t s = 0; s < lens[e]; s += LIST_CHUNK)
-                        tasks.push_back(ListTask{e, 0, s, std::min(LIST_CHUNK, lens[e] - s)});
 }
 
 }  // namespace
@@ -803,144 +701,6 @@ int fcis_batch_tune(fcis_batch_t *b, const double *d_in, double *d_out, int reps
         b->tuned_ms = best;
         if (best_ms) *best_ms = best;
         return build_items(b, brow);
-}
-
-// ---- list ops
-
-int fcis_dots_list(int nent, const fcis_dot_entry *ent, double *out, void *stream)
-{
-        if (nent < 0 || (nent > 0 && (ent == nullptr || out == nullptr))) {
-                fcis::set_error("fcis_dots_list: invalid argument");
-                return FCIS_ERR_INVALID;
-        }
-        if (nent == 0) return FCIS_SUCCESS;
-        std::vector<int64_t> lens(nent);
-        for (int e = 0; e < nent; e++) {
-                if (ent[e].n < 0 || (ent[e].n > 0 && (ent[e].x == nullptr || ent[e].y == nullptr))) {
-                        fcis::set_error("fcis_dots_list: entry %d invalid", e);
-                        return FCIS_ERR_INVALID;
-                }
-                lens[e] = ent[e].n;
-        }
-        std::vector<ListTask> tasks;
-        make_tasks(tasks, nent, lens.data());
-        for (int e = 0; e < nent; e++) out[e] = 0.0;
-        if (tasks.empty()) return FCIS_SUCCESS;
-        int dev = 0;
-        FCIS_HIP(hipGetDevice(&dev));
-        std::lock_guard<std::mutex> lock(g_stage_mu);
-        ListStage &S = g_stage[dev & 15];
-        const size_t o_ent = align_up(tasks.size() * sizeof(ListTask));
-        const size_t o_part = o_ent + align_up(nent * sizeof(fcis_dot_entry));
-        const size_t need = o_part + tasks.size() * sizeof(double);
-        int st = stage_reserve(S, need);
-        if (st != FCIS_SUCCESS) return st;
-        hipStream_t s = (hipStream_t)stream;
-        memcpy(S.host, tasks.data(), tasks.size() * sizeof(ListTask));
-        memcpy(S.host + o_ent, ent, nent * sizeof(fcis_dot_entry));
-        FCIS_HIP(hipMemcpyAsync(S.dev, S.host, o_part, hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_dots_list, dim3(tasks.size()), dim3(LIST_THREADS), 0, s,
-                           (const ListTask *)S.dev, (const fcis_dot_entry *)(S.dev + o_ent),
-                           (double *)(S.dev + o_part));
-        FCIS_CHECK_LAUNCH();
-        FCIS_HIP(hipMemcpyAsync(S.host + o_part, S.dev + o_part, tasks.size() * sizeof(double),
-                                hipMemcpyDeviceToHost, s));
-        FCIS_HIP(hipStreamSynchronize(s));
-        const double *part = (const double *)(S.host + o_part);
-        for (size_t t = 0; t < tasks.size(); t++) out[tasks[t].entry] += part[t];
-        return FCIS_SUCCESS;
-}
-
-int fcis_lincomb_list(int nent, const fcis_lincomb_entry *ent, int nterm, const double *const *xs,
-                      const double *coef, void *stream)
-{
-        if (nent < 0 || nterm < 0 || (nent > 0 && ent == nullptr) ||
-            (nterm > 0 && (xs == nullptr || coef == nullptr))) {
-                fcis::set_error("fcis_lincomb_list: invalid argument");
-                return FCIS_ERR_INVALID;
-        }
-        if (nent == 0) return FCIS_SUCCESS;
-        std::vector<int64_t> lens(nent);
-        for (int e = 0; e < nent; e++) {
-                const fcis_lincomb_entry &E = ent[e];
-                if (E.n < 0 || E.m < 0 || E.off < 0 || E.off + E.m > nterm ||
-                    (E.n > 0 && E.y == nullptr)) {
-                        fcis::set_error("fcis_lincomb_list: entry %d invalid", e);
-                        return FCIS_ERR_INVALID;
-                }
-                for (int j = 0; j < E.m; j++)
-                        if (E.n > 0 && xs[E.off + j] == nullptr) {
-                                fcis::set_error("fcis_lincomb_list: entry %d term %d is NULL", e, j);
-                                return FCIS_ERR_INVALID;
-                        }
-                lens[e] = E.n;
-        }
-        std::vector<ListTask> tasks;
-        make_tasks(tasks, nent, lens.data());
-        if (tasks.empty()) return FCIS_SUCCESS;
-        int dev = 0;
-        FCIS_HIP(hipGetDevice(&dev));
-        std::lock_guard<std::mutex> lock(g_stage_mu);
-        ListStage &S = g_stage[dev & 15];
-        const size_t o_ent = align_up(tasks.size() * sizeof(ListTask));
-        const size_t o_xs = o_ent + align_up(nent * sizeof(fcis_lincomb_entry));
-        const size_t o_cf = o_xs + align_up((size_t)nterm * sizeof(double *));
-        const size_t need = o_cf + (size_t)nterm * sizeof(double) + 8;
-        int st = stage_reserve(S, need);
-        if (st != FCIS_SUCCESS) return st;
-        hipStream_t s = (hipStream_t)stream;
-        memcpy(S.host, tasks.data(), tasks.size() * sizeof(ListTask));
-        memcpy(S.host + o_ent, ent, nent * sizeof(fcis_lincomb_entry));
-        if (nterm) {
-                memcpy(S.host + o_xs, xs, nterm * sizeof(double *));
-                memcpy(S.host + o_cf, coef, nterm * sizeof(double));
-        }
-        FCIS_HIP(hipMemcpyAsync(S.dev, S.host, need, hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_lincomb_list, dim3(tasks.size()), dim3(LIST_THREADS), 0, s,
-                           (const ListTask *)S.dev, (const fcis_lincomb_entry *)(S.dev + o_ent),
-                           (const double *const *)(S.dev + o_xs), (const double *)(S.dev + o_cf));
-        FCIS_CHECK_LAUNCH();
-        FCIS_HIP(hipStreamSynchronize(s));
-        return FCIS_SUCCESS;
-}
-
-int fcis_precond_list(int nent, const fcis_precond_entry *ent, double level_shift, void *stream)
-{
-        if (nent < 0 || (nent > 0 && ent == nullptr)) {
-                fcis::set_error("fcis_precond_list: invalid argument");
-                return FCIS_ERR_INVALID;
-        }
-        if (nent == 0) return FCIS_SUCCESS;
-        std::vector<int64_t> lens(nent);
-        for (int e = 0; e < nent; e++) {
-                const fcis_precond_entry &E = ent[e];
-                if (E.n < 0 || (E.n > 0 && (E.hdiag == nullptr || E.r == nullptr || E.out == nullptr))) {
-                        fcis::set_error("fcis_precond_list: entry %d invalid", e);
-                        return FCIS_ERR_INVALID;
-                }
-                lens[e] = E.n;
-        }
-        std::vector<ListTask> tasks;
-        make_tasks(tasks, nent, lens.data());
-        if (tasks.empty()) return FCIS_SUCCESS;
-        int dev = 0;
-        FCIS_HIP(hipGetDevice(&dev));
-        std::lock_guard<std::mutex> lock(g_stage_mu);
-        ListStage &S = g_stage[dev & 15];
-        const size_t o_ent = align_up(tasks.size() * sizeof(ListTask));
-        const size_t need = o_ent + nent * sizeof(fcis_precond_entry);
-        int st = stage_reserve(S, need);
-        if (st != FCIS_SUCCESS) return st;
-        hipStream_t s = (hipStream_t)stream;
-        memcpy(S.host, tasks.data(), tasks.size() * sizeof(ListTask));
-        memcpy(S.host + o_ent, ent, nent * sizeof(fcis_precond_entry));
-        FCIS_HIP(hipMemcpyAsync(S.dev, S.host, need, hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_precond_list, dim3(tasks.size()), dim3(LIST_THREADS), 0, s,
-                           (const ListTask *)S.dev, (const fcis_precond_entry *)(S.dev + o_ent),
-                           level_shift);
-        FCIS_CHECK_LAUNCH();
-        FCIS_HIP(hipStreamSynchronize(s));
-        return FCIS_SUCCESS;
 }
 
 }  // extern "C"

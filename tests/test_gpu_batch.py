@@ -1,6 +1,6 @@
 """GPU parity of the batched path: one persistent launch computing sigma for
 many CAS problems and vector slots (against the single-problem plan and
-the oracle), its hipGraph replays, the autotuner, the list-form vector
+the oracle), its hipGraph replays, the autotuner, the block vector
 kernels and the lockstep batched Davidson (against dense / ARPACK spectra
 from the oracle and against the single-problem solver).  FP64 sigma
 tolerance 1e-13 of max|sigma| (the batch accumulates with atomics, so
@@ -140,32 +140,44 @@ def test_batch_low_precision(precision, tol):
             assert 1e-12 < err < tol, (p, v, err)
 
 
-def test_list_ops_match_torch():
+def test_block_ops_match_torch():
     import torch
-    ops = batch.ListOps()
+    ops = batch.BlockOps()
     g = torch.Generator(device="cuda").manual_seed(4)
     lens = [1, 300, 65536, 65537, 200000]
-    xs = [torch.randn(n, dtype=torch.float64, device="cuda", generator=g) for n in lens]
-    ys = [torch.randn(n, dtype=torch.float64, device="cuda", generator=g) for n in lens]
-    d = ops.dots([x.data_ptr() for x in xs], [y.data_ptr() for y in ys], lens)
-    for k, (x, y) in enumerate(zip(xs, ys)):
-        assert abs(d[k] - float(torch.dot(x, y))) < 1e-12 * max(1.0, abs(d[k]))
-    # y = 0.5 y + 2 x - 3 y  (y among the terms)
-    ref = [0.5 * y + 2.0 * x - 3.0 * y for x, y in zip(xs, ys)]
-    ops.lincomb([(y.data_ptr(), y.numel(), 0.5, [(x.data_ptr(), 2.0), (y.data_ptr(), -3.0)])
-                 for x, y in zip(xs, ys)])
-    for y, r in zip(ys, ref):
-        assert float((y - r).abs().max()) < 1e-13 * float(r.abs().max())
-    # beta = 0 never reads y (NaN there must not leak)
-    z = torch.full((lens[2],), float("nan"), dtype=torch.float64, device="cuda")
-    ops.lincomb([(z.data_ptr(), z.numel(), 0.0, [(xs[2].data_ptr(), 1.0)])])
-    assert torch.equal(z, xs[2])
+    nx, ny = [3, 9, 1, 12, 5], [2, 4, 5, 1, 3]
+    X = [torch.randn(a, n, dtype=torch.float64, device="cuda", generator=g) for a, n in zip(nx, lens)]
+    Y = [torch.randn(b, n, dtype=torch.float64, device="cuda", generator=g) for b, n in zip(ny, lens)]
+
+    def rows(t):
+        return [t[i].data_ptr() for i in range(t.shape[0])]
+
+    G = ops.gram([(rows(x), rows(y), n) for x, y, n in zip(X, Y, lens)])
+    for x, y, gm in zip(X, Y, G):
+        ref = (x @ y.T).cpu().numpy()
+        assert numpy.abs(gm - ref).max() < 1e-12 * max(1.0, numpy.abs(ref).max())
+    # Y = 0.5 Y + X C
+    C = [numpy.random.default_rng(k).standard_normal((a, b)) for k, (a, b) in enumerate(zip(nx, ny))]
+    ref = [0.5 * y + torch.from_numpy(c).cuda().T @ x for x, y, c in zip(X, Y, C)]
+    ops.gemm([(rows(x), rows(y), n, c, 0.5) for x, y, n, c in zip(X, Y, lens, C)])
+    for y, r in zip(Y, ref):
+        assert float((y - r).abs().max()) < 1e-12 * float(r.abs().max())
+    # in-place block transform X[:m] <- X C (Y aliases X), beta = 0 ignores NaN in Y
+    x = X[3]
+    c = numpy.random.default_rng(9).standard_normal((nx[3], 6))
+    r = torch.from_numpy(c).cuda().T @ x
+    ops.gemm([(rows(x), rows(x)[:6], lens[3], c, 0.0)])
+    assert float((x[:6] - r).abs().max()) < 1e-12 * float(r.abs().max())
+    z = torch.full((1, lens[2]), float("nan"), dtype=torch.float64, device="cuda")
+    ops.gemm([(rows(X[2]), rows(z), lens[2], numpy.ones((1, 1)), 0.0)])
+    assert torch.equal(z, X[2])
     hd = torch.linspace(-1.0, 1.0, lens[3], dtype=torch.float64, device="cuda")
+    rv = torch.randn(lens[3], dtype=torch.float64, device="cuda", generator=g)
     out = torch.empty_like(hd)
-    ops.precond([(hd.data_ptr(), xs[3].data_ptr(), out.data_ptr(), lens[3], 0.25)], 1e-3)
+    ops.precond([(hd.data_ptr(), rv.data_ptr(), out.data_ptr(), lens[3], 0.25)], 1e-3)
     den = hd - 0.25
     den = torch.where(den.abs() < 1e-3, torch.full_like(den, 1e-3), den)
-    assert float((out - xs[3] / den).abs().max()) < 1e-12 * float(out.abs().max())
+    assert float((out - rv / den).abs().max()) < 1e-12 * float(out.abs().max())
 
 
 def test_solve_batch_small_vs_dense_spectra():
