@@ -12,9 +12,11 @@ One "step" is one full FP64 sigma = H c with c and sigma resident in HBM.
 sigma rates with their error against the FP64 sigma, the FP64 and
 mixed-precision Davidson times to the ground state with their energy
 difference, the roofline of the dominant kernel (the contraction, timed with
-HIP events on its own stream inside this run) and a CPU baseline.
+HIP events on its own stream inside this run), a CPU baseline, and the
+batched workload: 64 uneven CAS problems (norb 6..12, 4 roots each) solved
+together with one batched sigma launch per Davidson iteration.
 
-    python bench.py [--steps K] [--warmup W] [--no-davidson] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--no-davidson] [--no-cpu] [--no-batch]
 """
 
 import argparse
@@ -125,6 +127,45 @@ def cpu_baseline(pkg, budget_s=12.0):
             "sigma_s_14o": dt}
 
 
+def batch_solve(pkg, sync):
+    """64 uneven CAS problems (norb 6..12, nroots = 4, integrals from seed i)
+    solved together: one batched sigma launch per Davidson iteration.
+    problems/s counts the whole solve (string spaces, diagonals, guesses,
+    schedule tuning and iterations)."""
+    import numpy
+    man = pkg.synthetic.batch_manifest()
+    probs = []
+    F = 0.0
+    for n, ne, s in man:
+        h1e, eri = pkg.synthetic.make_problem(n, ne, seed=s, with_ci=False)
+        probs.append((h1e, eri, n, ne))
+        nd = math.comb(n, ne[0]) * math.comb(n, ne[1])
+        F += 4 * 2.0 * (n * (n + 1) // 2) ** 2 * nd
+    sync()
+    t0 = time.perf_counter()
+    res = pkg.batch.solve_batch(probs, nroots=4, max_cycle=200)
+    sync()
+    dt = time.perf_counter() - t0
+    st = res.stats
+    nconv = int(sum(bool(c.all()) for c in res.converged))
+    tf = F / (st["tuned_ms"] * 1e-3) / 1e12 if st.get("tuned_ms") else None
+    out = {
+        "workload": "64 CAS problems, norb 6..12, nelec (n//2+d, n//2), d in {0,1}, "
+                    "nroots 4, %d dets in total" % st["ndet_total"],
+        "problems_per_s": len(probs) / dt, "seconds": dt,
+        "seconds_setup": st["time_setup"], "seconds_iterations": st["time_iterations"],
+        "seconds_sigma": st["time_sigma"], "sigma_launches": st["sigma_launches"],
+        "converged": nconv, "max_cycles": int(numpy.max(res.cycles)),
+        "schedule": {"rows_per_item": st["rows_per_item"], "grid": st["grid"]},
+        "sigma_all_slots_ms": st["tuned_ms"],
+        "sigma_all_slots_TFLOP/s": tf,
+        "sigma_frac_of_fp64_peak": None if tf is None else tf / PEAK["fp64"],
+    }
+    log("batch: %d problems in %.2f s (%.2f problems/s), %d converged, sigma(all slots) %.2f ms = "
+        "%.1f TFLOP/s" % (len(probs), dt, len(probs) / dt, nconv, st["tuned_ms"], tf or 0.0))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +176,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precisions", default="fp64,fp32,bf16x3")
     ap.add_argument("--max-cycle", type=int, default=100)
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the 64-problem batched solve (nroots=4)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 4 GiB device copy first (HBM counter calibration)")
     args = ap.parse_args()
@@ -230,6 +273,10 @@ def main():
             del ci, solver
             torch.cuda.empty_cache()
 
+    batch = None
+    if not args.no_batch:
+        batch = batch_solve(pkg, sync)
+
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (reference libfci_gas on host cores)")
@@ -264,13 +311,14 @@ def main():
                                "npair=136, synthetic integrals seed 0",
                    "norb": n, "nelec": list(ne), "ndet": sp.ndet},
         "roofline": None if main_k is None else {
-            "bound": "mfma", "kernel": "k_fused<fp64>" if main_k.get("fused_gather") else "k_contract_f64",
+            "bound": "mfma", "kernel": main_name,
             "achieved": main_k["TFLOP/s"], "peak": PEAK["fp64"], "unit": "TFLOP/s",
             "frac": main_k["TFLOP/s"] / PEAK["fp64"], "traffic": traffic},
         "cpu_baseline": cpu,
         "precisions": result,
         "kernels": kern,
         "davidson": dav,
+        "batch": batch,
     }
     print(json.dumps(line), flush=True)
 

@@ -177,14 +177,14 @@ __global__ __launch_bounds__(64 * (MAXW + FusedTypes<PREC>::NBLD)) void k_batch_
                 if (W.prob != cur) {
                         cur = W.prob;
                         if (mma) {
+                                // fragments past the problem's k-steps are zero, so
+                                // that mma_steps may run past them
 #pragma unroll
                                 for (int ks = 0; ks < KS; ks++) {
-                                        if (ks < P.ksteps) {
-                                                size_t o = ((size_t)ks * P.mtiles + wave) * 64 + lane;
-                                                ah[ks] = hfrag_all[P.hf + o];
-                                                if constexpr (PREC == FCIS_PREC_BF16X3)
-                                                        al[ks] = hfrag_lo_all[P.hf + o];
-                                        }
+                                        const size_t o = ((size_t)ks * P.mtiles + wave) * 64 + lane;
+                                        ah[ks] = ks < P.ksteps ? hfrag_all[P.hf + o] : typename Ty::A{};
+                                        if constexpr (PREC == FCIS_PREC_BF16X3)
+                                                al[ks] = ks < P.ksteps ? hfrag_lo_all[P.hf + o] : typename Ty::A{};
                                 }
                         }
                 }
@@ -205,8 +205,11 @@ __global__ __launch_bounds__(64 * (MAXW + FusedTypes<PREC>::NBLD)) void k_batch_
                                         if (mma) {
                                                 const BT *bb = (j & 1) ? buf1 : buf0;
                                                 const int16_t *mb = maps + (j % 3) * MAXP * NB;
+                                                // two unconditional depths: half or all of KS
                                                 const typename Ty::Acc acc =
-                                                        fused_mma<PREC, KS>(ah, al, bb, P.ksteps, lk, li);
+                                                        P.ksteps <= KS / 2
+                                                                ? mma_steps<PREC, KS / 2, KS>(ah, al, bb, lk, li)
+                                                                : mma_steps<PREC, KS, KS>(ah, al, bb, lk, li);
                                                 const int64_t b = j * NB + li;
                                                 if (b < P.nb) {
 #pragma unroll

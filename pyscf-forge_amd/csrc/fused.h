@@ -163,57 +163,53 @@ __device__ __forceinline__ void fb_commit_maps(FusedBuild<NBLD, MAXW> &st, int n
 }
 
 // One chunk of the contraction on MFMA: the 16 x 16 block
-//   acc[i][j] = sum_k h[16 w + i][k] t1[k][j],  k < 4 ksteps (32 for bf16x3),
-// for this wavefront's A fragments ah (and the bf16 low halves al) held in
-// registers and the B tile bb in LDS ([k][FUSED_NB] layout).
-template <int PREC, int KS>
-__device__ __forceinline__ typename FusedTypes<PREC>::Acc fused_mma(
+//   acc[i][j] = sum_k h[16 w + i][k] t1[k][j]
+// over exactly KSP k-steps (4 k per step; 32 for bf16x3), without
+// conditions: A fragments past the problem's k-steps must be zero in
+// registers and B rows past npair zero in LDS.  ah/al: this wavefront's A
+// fragments (al: bf16 low halves), bb: the B tile in LDS ([k][FUSED_NB]).
+template <int PREC, int KSP, int KS>
+__device__ __forceinline__ typename FusedTypes<PREC>::Acc mma_steps(
         const typename FusedTypes<PREC>::A (&ah)[KS],
         const typename FusedTypes<PREC>::A (&al)[PREC == FCIS_PREC_BF16X3 ? KS : 1],
-        const typename FusedTypes<PREC>::B *bb, int ksteps, int lk, int li)
+        const typename FusedTypes<PREC>::B *bb, int lk, int li)
 {
+        static_assert(KSP <= KS, "more k-steps than A fragments");
         typedef FusedTypes<PREC> Ty;
-        typedef typename Ty::B BT;
         constexpr int NB = FUSED_NB;
-        typename Ty::Acc acc;
         if constexpr (PREC == FCIS_PREC_BF16X3) {
                 typename Ty::Acc a0c = {0, 0, 0, 0}, a1c = a0c, a2c = a0c;
 #pragma unroll
-                for (int ks = 0; ks < KS; ks++) {
-                        if (ks < ksteps) {
-                                bf16x8 bh, bl;
+                for (int ks = 0; ks < KSP; ks++) {
+                        bf16x8 bh, bl;
 #pragma unroll
-                                for (int e = 0; e < 8; e++) {
-                                        float xv = bb[(32 * ks + 8 * lk + e) * NB + li];
-                                        __bf16 hi = (__bf16)xv;
-                                        bh[e] = hi;
-                                        bl[e] = (__bf16)(xv - (float)hi);
-                                }
-                                a0c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[ks], bh, a0c, 0, 0, 0);
-                                a1c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ks], bl, a1c, 0, 0, 0);
-                                a2c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ks], bh, a2c, 0, 0, 0);
+                        for (int e = 0; e < 8; e++) {
+                                float xv = bb[(32 * ks + 8 * lk + e) * NB + li];
+                                __bf16 hi = (__bf16)xv;
+                                bh[e] = hi;
+                                bl[e] = (__bf16)(xv - (float)hi);
                         }
+                        a0c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[ks], bh, a0c, 0, 0, 0);
+                        a1c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ks], bl, a1c, 0, 0, 0);
+                        a2c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ks], bh, a2c, 0, 0, 0);
                 }
-                acc = (a0c + a1c) + a2c;
+                return (a0c + a1c) + a2c;
         } else {
                 typename Ty::Acc ac[FUSED_NACC];
 #pragma unroll
                 for (int z = 0; z < FUSED_NACC; z++) ac[z] = typename Ty::Acc{0, 0, 0, 0};
 #pragma unroll
-                for (int ks = 0; ks < KS; ks++) {
-                        if (ks < ksteps) {
-                                BT bv = bb[(4 * ks + lk) * NB + li];
-                                if constexpr (PREC == FCIS_PREC_FP64)
-                                        ac[ks % FUSED_NACC] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-                                                ah[ks], bv, ac[ks % FUSED_NACC], 0, 0, 0);
-                                else
-                                        ac[ks % FUSED_NACC] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-                                                ah[ks], bv, ac[ks % FUSED_NACC], 0, 0, 0);
-                        }
+                for (int ks = 0; ks < KSP; ks++) {
+                        typename Ty::B bv = bb[(4 * ks + lk) * NB + li];
+                        if constexpr (PREC == FCIS_PREC_FP64)
+                                ac[ks % FUSED_NACC] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                                        ah[ks], bv, ac[ks % FUSED_NACC], 0, 0, 0);
+                        else
+                                ac[ks % FUSED_NACC] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                                        ah[ks], bv, ac[ks % FUSED_NACC], 0, 0, 0);
                 }
-                acc = ac[0] + ac[1];
+                return ac[0] + ac[1];
         }
-        return acc;
 }
 
 }  // namespace
