@@ -143,7 +143,7 @@ def batch_solve(pkg, sync):
         F += 4 * 2.0 * (n * (n + 1) // 2) ** 2 * nd
     sync()
     t0 = time.perf_counter()
-    res = pkg.batch.solve_batch(probs, nroots=4, max_cycle=200)
+    res = pkg.batch.solve_batch(probs, nroots=4, max_cycle=300)
     sync()
     dt = time.perf_counter() - t0
     st = res.stats
@@ -153,7 +153,8 @@ def batch_solve(pkg, sync):
         "workload": "64 CAS problems, norb 6..12, nelec (n//2+d, n//2), d in {0,1}, "
                     "nroots 4, %d dets in total" % st["ndet_total"],
         "problems_per_s": len(probs) / dt, "seconds": dt,
-        "seconds_setup": st["time_setup"], "seconds_iterations": st["time_iterations"],
+        "seconds_setup": st["time_setup"], "seconds_batch_create": st["time_create"],
+        "seconds_tune": st["time_tune"], "seconds_iterations": st["time_iterations"],
         "seconds_sigma": st["time_sigma"], "sigma_launches": st["sigma_launches"],
         "converged": nconv, "max_cycles": int(numpy.max(res.cycles)),
         "schedule": {"rows_per_item": st["rows_per_item"], "grid": st["grid"]},

@@ -656,44 +656,42 @@ int fcis_batch_tune(fcis_batch_t *b, const double *d_in, double *d_out, int reps
         if (reps < 1) reps = 1;
         hipStream_t st = (hipStream_t)stream;
         drop_graph(b);
-        const int rows_cand[] = {1, 2, 4, 8, 16};
-        std::vector<int> grid_cand = {b->ncu, 2 * b->ncu, b->occ * b->ncu};
-        std::sort(grid_cand.begin(), grid_cand.end());
-        grid_cand.erase(std::unique(grid_cand.begin(), grid_cand.end()), grid_cand.end());
         hipEvent_t e0, e1;
         FCIS_HIP(hipEventCreate(&e0));
         FCIS_HIP(hipEventCreate(&e1));
         double best = 1e300;
-        int brow = 1, bgrid = b->grid;
+        int brow = 1, bgrid = b->occ * b->ncu;
         int status = FCIS_SUCCESS;
-        for (int rows : rows_cand) {
+        // time one schedule: min over reps
+        auto trial = [&](int rows, int grid) {
+                if (status != FCIS_SUCCESS) return;
                 status = build_items(b, rows);
-                if (status != FCIS_SUCCESS) break;
-                for (int g : grid_cand) {
-                        if (g > b->occ * b->ncu) continue;
-                        b->grid = g;
-                        status = enqueue(b, d_in, d_out, st);   // warm-up
-                        if (status != FCIS_SUCCESS) break;
-                        float ms = 0.0f;
-                        double tmin = 1e300;
-                        for (int r = 0; r < reps && status == FCIS_SUCCESS; r++) {
-                                if (hipEventRecord(e0, st) != hipSuccess) status = FCIS_ERR_DEVICE;
-                                if (status == FCIS_SUCCESS) status = enqueue(b, d_in, d_out, st);
-                                if (status == FCIS_SUCCESS && (hipEventRecord(e1, st) != hipSuccess ||
-                                                               hipEventSynchronize(e1) != hipSuccess ||
-                                                               hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-                                        status = FCIS_ERR_DEVICE;
-                                tmin = std::min(tmin, (double)ms);
-                        }
-                        if (status != FCIS_SUCCESS) break;
-                        if (tmin < best) {
-                                best = tmin;
-                                brow = rows;
-                                bgrid = g;
-                        }
+                b->grid = grid;
+                float ms = 0.0f;
+                double tmin = 1e300;
+                for (int r = 0; r < reps && status == FCIS_SUCCESS; r++) {
+                        if (hipEventRecord(e0, st) != hipSuccess) status = FCIS_ERR_DEVICE;
+                        if (status == FCIS_SUCCESS) status = enqueue(b, d_in, d_out, st);
+                        if (status == FCIS_SUCCESS && (hipEventRecord(e1, st) != hipSuccess ||
+                                                       hipEventSynchronize(e1) != hipSuccess ||
+                                                       hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                                status = FCIS_ERR_DEVICE;
+                        tmin = std::min(tmin, (double)ms);
                 }
-                if (status != FCIS_SUCCESS) break;
-        }
+                if (status == FCIS_SUCCESS && tmin < best) {
+                        best = tmin;
+                        brow = rows;
+                        bgrid = grid;
+                }
+        };
+        // warm-up, then alpha strings per item at full residency, then the
+        // grid size at the best item size
+        status = enqueue(b, d_in, d_out, st);
+        if (status == FCIS_SUCCESS && hipStreamSynchronize(st) != hipSuccess) status = FCIS_ERR_DEVICE;
+        for (int rows : {1, 2, 4, 8}) trial(rows, b->occ * b->ncu);
+        const int rbest = brow;
+        for (int g : {b->ncu, 2 * b->ncu})
+                if (g < b->occ * b->ncu) trial(rbest, g);
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         if (status != FCIS_SUCCESS) {
