@@ -86,9 +86,53 @@ __global__ __launch_bounds__(LIST_THREADS) void k_gram_list(const BlockTask *__r
         }
 }
 
+// One output tile of the block product, NJ outputs wide; every thread
+// handles UNR elements per pass so that UNR independent loads of each X_i
+// are in flight.
+template <int NJ, int UNR>
+__device__ __forceinline__ void gemm_tile(const BlockTask &T, const fcis_block_entry &E, int nj,
+                                          const double *const *xb, double *const *yb,
+                                          const double *cs)
+{
+        double *yp[NJ];
+#pragma unroll
+        for (int b = 0; b < NJ; b++) yp[b] = b < nj ? yb[b] + T.start : nullptr;
+        const int nx = E.nx;
+        for (int64_t k0 = threadIdx.x; k0 < T.len; k0 += (int64_t)LIST_THREADS * UNR) {
+                bool in[UNR];
+#pragma unroll
+                for (int u = 0; u < UNR; u++) in[u] = k0 + u * LIST_THREADS < T.len;
+                double acc[UNR][NJ];
+#pragma unroll
+                for (int u = 0; u < UNR; u++)
+#pragma unroll
+                        for (int b = 0; b < NJ; b++)
+                                acc[u][b] = (b < nj && in[u] && E.beta != 0.0)
+                                                    ? E.beta * yp[b][k0 + u * LIST_THREADS] : 0.0;
+#pragma unroll 2
+                for (int i = 0; i < nx; i++) {
+                        const double *x = xb[i] + T.start + k0;
+                        double xv[UNR];
+#pragma unroll
+                        for (int u = 0; u < UNR; u++) xv[u] = in[u] ? x[u * LIST_THREADS] : 0.0;
+#pragma unroll
+                        for (int b = 0; b < NJ; b++) {
+                                const double c = cs[i * GEMM_TJ + b];
+#pragma unroll
+                                for (int u = 0; u < UNR; u++) acc[u][b] += c * xv[u];
+                        }
+                }
+#pragma unroll
+                for (int u = 0; u < UNR; u++)
+#pragma unroll
+                        for (int b = 0; b < NJ; b++)
+                                if (b < nj && in[u]) yp[b][k0 + u * LIST_THREADS] = acc[u][b];
+        }
+}
+
 // Y[j0+jj][t] = beta Y[j0+jj][t] + sum_i C[i][j0+jj] X[i][t] for the
-// chunk; every thread reads all X[i][t] before writing its Y[.][t], so Y
-// may alias X when ny <= GEMM_TJ (one output tile per chunk).
+// chunk; every thread reads all X[i][t] of its elements before writing its
+// Y[.][t], so Y may alias X when ny <= GEMM_TJ (one output tile per chunk).
 __global__ __launch_bounds__(LIST_THREADS) void k_gemm_list(const BlockTask *__restrict__ tasks,
                                                             const fcis_block_entry *__restrict__ ent,
                                                             const double *const *__restrict__ xs,
@@ -104,23 +148,18 @@ __global__ __launch_bounds__(LIST_THREADS) void k_gemm_list(const BlockTask *__r
                 cs[e] = jj < nj ? coef[E.ooff + (int64_t)i * E.ny + T.j0 + jj] : 0.0;
         }
         __syncthreads();
-        double *yp[GEMM_TJ];
-#pragma unroll
-        for (int b = 0; b < GEMM_TJ; b++) yp[b] = b < nj ? ys[E.yoff + T.j0 + b] + T.start : nullptr;
         const double *const *xb = xs + E.xoff;
-        for (int64_t k = threadIdx.x; k < T.len; k += LIST_THREADS) {
-                double acc[GEMM_TJ];
-#pragma unroll
-                for (int b = 0; b < GEMM_TJ; b++) acc[b] = (b < nj && E.beta != 0.0) ? E.beta * yp[b][k] : 0.0;
-                for (int i = 0; i < nx; i++) {
-                        const double x = xb[i][T.start + k];
-#pragma unroll
-                        for (int b = 0; b < GEMM_TJ; b++) acc[b] += cs[i * GEMM_TJ + b] * x;
-                }
-#pragma unroll
-                for (int b = 0; b < GEMM_TJ; b++)
-                        if (b < nj) yp[b][k] = acc[b];
-        }
+        double *const *yb = ys + E.yoff + T.j0;
+        if (nj <= 1)
+                gemm_tile<1, 8>(T, E, nj, xb, yb, cs);
+        else if (nj <= 2)
+                gemm_tile<2, 8>(T, E, nj, xb, yb, cs);
+        else if (nj <= 4)
+                gemm_tile<4, 4>(T, E, nj, xb, yb, cs);
+        else if (nj <= 8)
+                gemm_tile<8, 4>(T, E, nj, xb, yb, cs);
+        else
+                gemm_tile<GEMM_TJ, 2>(T, E, nj, xb, yb, cs);
 }
 
 __global__ __launch_bounds__(LIST_THREADS) void k_precond_list(const BlockTask *__restrict__ tasks,
