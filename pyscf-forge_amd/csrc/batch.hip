@@ -57,6 +57,14 @@ struct BatchItem {
         int prob, r0, r1, pad;
 };
 
+// Builder wavefronts of the batched kernel (FP64: 3, enough at <= 12
+// orbitals, which leaves room for two workgroups per CU; FP32/bf16: 7).
+template <int PREC>
+constexpr int batch_nbld()
+{
+        return PREC == FCIS_PREC_FP64 ? 3 : 7;
+}
+
 template <int PREC, int KS, int MAXW>
 constexpr size_t batch_lds_bytes(int64_t nbpmax)
 {
@@ -72,7 +80,7 @@ __device__ __forceinline__ void atomic_add_f64(double *p, double v)
 }
 
 template <int PREC, int KS, int MAXW>
-__global__ __launch_bounds__(64 * (MAXW + FusedTypes<PREC>::NBLD)) void k_batch_row(
+__global__ __launch_bounds__(64 * (MAXW + batch_nbld<PREC>())) void k_batch_row(
         const BatchProb *__restrict__ probs, const BatchItem *__restrict__ items, int nitems,
         int *__restrict__ counter, const uint8_t *__restrict__ active, int64_t nbpmax,
         const int32_t *__restrict__ pmapa_all, const int16_t *__restrict__ pmapb_all,
@@ -82,7 +90,7 @@ __global__ __launch_bounds__(64 * (MAXW + FusedTypes<PREC>::NBLD)) void k_batch_
 {
         typedef FusedTypes<PREC> Ty;
         typedef typename Ty::B BT;
-        constexpr int KW = Ty::KW, NB = FUSED_NB, NBLD = Ty::NBLD;
+        constexpr int KW = Ty::KW, NB = FUSED_NB, NBLD = batch_nbld<PREC>();
         constexpr int KPMAX = KS * KW;
         constexpr int MAXP = 16 * MAXW;
         extern __shared__ __align__(16) char smem[];
@@ -299,7 +307,7 @@ int launch_row(fcis_batch *b, const double *cin, double *cout, hipStream_t st)
 {
         typedef typename FusedTypes<PREC>::A A;
         hipLaunchKernelGGL((k_batch_row<PREC, KS, MAXW>), dim3(b->grid),
-                           dim3(64 * (MAXW + FusedTypes<PREC>::NBLD)), b->lds, st, b->d_probs,
+                           dim3(64 * (MAXW + batch_nbld<PREC>())), b->lds, st, b->d_probs,
                            b->d_items, b->nitems, b->d_counter, b->d_active, b->nbpmax, b->d_pmapa,
                            b->d_pmapb, (const A *)b->d_hfrag, (const A *)b->d_hfrag_lo, cin, cout);
         FCIS_CHECK_LAUNCH();
@@ -339,9 +347,9 @@ size_t batch_lds(int precision, int variant, int64_t nbpmax)
 int batch_threads(int precision, int variant)
 {
         const int maxw = variant ? FUSED_MAXW : BATCH_SMALL_MAXW;
-        const int nbld = precision == FCIS_PREC_FP64 ? FusedTypes<FCIS_PREC_FP64>::NBLD
-                       : precision == FCIS_PREC_FP32 ? FusedTypes<FCIS_PREC_FP32>::NBLD
-                                                      : FusedTypes<FCIS_PREC_BF16X3>::NBLD;
+        const int nbld = precision == FCIS_PREC_FP64 ? batch_nbld<FCIS_PREC_FP64>()
+                       : precision == FCIS_PREC_FP32 ? batch_nbld<FCIS_PREC_FP32>()
+                                                      : batch_nbld<FCIS_PREC_BF16X3>();
         return 64 * (maxw + nbld);
 }
 

@@ -30,7 +30,7 @@ struct FusedTypes<FCIS_PREC_FP64> {
         typedef double Out;
         static constexpr int KS = FUSED_KS;
         static constexpr int KW = 4;
-        static constexpr int NBLD = 3;    // 9 + 3 wavefronts: 3 per SIMD at <= 168 VGPRs
+        static constexpr int NBLD = 7;    // 9 + 7 wavefronts: 4 per SIMD at <= 128 VGPRs
 };
 template <>
 struct FusedTypes<FCIS_PREC_FP32> {

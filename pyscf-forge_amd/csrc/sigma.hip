@@ -315,7 +315,7 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
         const int kp = ksteps * KW;
         const int64_t nch = (nb + NB - 1) / NB;
 
-        for (int e = npair * NB + tid; e < kp * NB; e += nthr) {
+        for (int e = npair * NB + tid; e < KPMAX * NB; e += nthr) {
                 buf0[e] = (BT)0;
                 buf1[e] = (BT)0;
         }
@@ -374,12 +374,11 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
         typename Ty::A al[PREC == FCIS_PREC_BF16X3 ? KS : 1];
 #pragma unroll
         for (int ks = 0; ks < KS; ks++) {
-                if (ks < ksteps) {
-                        size_t o = ((size_t)ks * mtiles + wave) * 64 + lane;
-                        ah[ks] = ((const typename Ty::A *)hfrag_v)[o];
-                        if constexpr (PREC == FCIS_PREC_BF16X3)
-                                al[ks] = ((const typename Ty::A *)hfrag_lo_v)[o];
-                }
+                // zero past the problem's k-steps: mma_steps runs all KS
+                const size_t o = ((size_t)ks * mtiles + wave) * 64 + lane;
+                ah[ks] = ks < ksteps ? ((const typename Ty::A *)hfrag_v)[o] : typename Ty::A{};
+                if constexpr (PREC == FCIS_PREC_BF16X3)
+                        al[ks] = ks < ksteps ? ((const typename Ty::A *)hfrag_lo_v)[o] : typename Ty::A{};
         }
         for (int64_t r = blockIdx.x; r < ta; r += gridDim.x) {
                 const int64_t I = a0 + r;
@@ -390,44 +389,7 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                 for (int64_t j = 0; j < nch; j++) {
                         const BT *bb = (j & 1) ? buf1 : buf0;
                         const int16_t *mb = maps + (j % 3) * MAXP * NB;
-                        typename Ty::Acc acc;
-                        if constexpr (PREC == FCIS_PREC_BF16X3) {
-                                typename Ty::Acc a0c = {0, 0, 0, 0}, a1c = a0c, a2c = a0c;
-#pragma unroll
-                                for (int ks = 0; ks < KS; ks++) {
-                                        if (ks < ksteps) {
-                                                bf16x8 bh, bl;
-#pragma unroll
-                                                for (int e = 0; e < 8; e++) {
-                                                        float xv = bb[(32 * ks + 8 * lk + e) * NB + li];
-                                                        __bf16 hi = (__bf16)xv;
-                                                        bh[e] = hi;
-                                                        bl[e] = (__bf16)(xv - (float)hi);
-                                                }
-                                                a0c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[ks], bh, a0c, 0, 0, 0);
-                                                a1c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ks], bl, a1c, 0, 0, 0);
-                                                a2c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ks], bh, a2c, 0, 0, 0);
-                                        }
-                                }
-                                acc = (a0c + a1c) + a2c;
-                        } else {
-                                typename Ty::Acc ac[FUSED_NACC];
-#pragma unroll
-                                for (int z = 0; z < FUSED_NACC; z++) ac[z] = typename Ty::Acc{0, 0, 0, 0};
-#pragma unroll
-                                for (int ks = 0; ks < KS; ks++) {
-                                        if (ks < ksteps) {
-                                                BT bv = bb[(4 * ks + lk) * NB + li];
-                                                if constexpr (PREC == FCIS_PREC_FP64)
-                                                        ac[ks % FUSED_NACC] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-                                                                ah[ks], bv, ac[ks % FUSED_NACC], 0, 0, 0);
-                                                else
-                                                        ac[ks % FUSED_NACC] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-                                                                ah[ks], bv, ac[ks % FUSED_NACC], 0, 0, 0);
-                                        }
-                                }
-                                acc = ac[0] + ac[1];
-                        }
+                        const typename Ty::Acc acc = mma_steps<PREC, KS, KS>(ah, al, bb, lk, li);
                         const int64_t b = j * NB + li;
                         if (b < nb) {
 #pragma unroll
