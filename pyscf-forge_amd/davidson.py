@@ -7,7 +7,8 @@ sqrt(tol)): the trial vectors X and their images AX = H X live in HBM; only
 the small subspace matrix (at most max_space + 3*(nroots-1) on a side) is
 brought to the host for its FP64 eigendecomposition.  All O(N) work (dot
 products, Ritz combinations, residuals, preconditioning, Gram-Schmidt) runs
-in the library's HIP kernels; PyTorch only allocates the device buffers.
+in the library's HIP kernels (block gram / gemm, preconditioner); PyTorch
+only allocates the device buffers.
 
 ``hop(x, out)`` writes H x into ``out`` (device float64 vectors of length
 ``n``); ``precond(r, e, out)`` writes the preconditioned residual.
@@ -33,11 +34,12 @@ class DavidsonResult:
 
 class DeviceOps:
     """Vector algebra of the solver on device float64 vectors, through the
-    library's HIP kernels (fcis_dots / fcis_lincomb)."""
+    library's block kernels: X^T v as one gram launch and beta y + X c as
+    one gemm launch, each streaming every vector of X once."""
 
     def __init__(self):
-        self.lib = _lib.load()
-        self.stream = _lib.stream_ptr()
+        from .blockops import BlockOps
+        self.ops = BlockOps()
 
     def empty(self, shape, like):
         import torch
@@ -49,17 +51,21 @@ class DeviceOps:
     def copy(self, dst, src):
         dst.copy_(src)
 
+    @staticmethod
+    def _rows(X, m, n):
+        if not X.is_contiguous():
+            raise ValueError("vector blocks must be contiguous")
+        return X.data_ptr() + 8 * int(n) * numpy.arange(m, dtype=numpy.int64)
+
     def dots(self, m, n, X, v):
-        out = numpy.empty(max(m, 1))
-        if m:
-            _lib.check(self.lib.fcis_dots(m, n, _lib.vptr(X), n, _lib.vptr(v),
-                                          _lib.dptr(out), self.stream), "fcis_dots")
-        return out[:m]
+        if m == 0:
+            return numpy.zeros(0)
+        g = self.ops.gram([(self._rows(X, m, n), [v.data_ptr()], int(n))])[0]
+        return g[:, 0].copy()
 
     def lincomb(self, m, n, X, coef, beta, y):
-        coef = numpy.ascontiguousarray(coef, dtype=numpy.float64)
-        _lib.check(self.lib.fcis_lincomb(m, n, _lib.vptr(X), n, _lib.dptr(coef), float(beta),
-                                         _lib.vptr(y), self.stream), "fcis_lincomb")
+        coef = numpy.ascontiguousarray(coef, dtype=numpy.float64).reshape(m, 1)
+        self.ops.gemm([(self._rows(X, m, n), [y.data_ptr()], int(n), coef, float(beta))])
 
 
 def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
