@@ -43,6 +43,24 @@ static inline int64_t binom_host(int n, int k)
         return r;
 }
 
+// Binomial coefficients C(n, k), n <= FCIS_MAXORB, for a __constant__
+// table in each source that needs them on the device.
+struct BinomTable {
+        uint32_t v[FCIS_MAXORB + 1][FCIS_MAXORB + 1];
+};
+
+constexpr BinomTable make_binom_table()
+{
+        BinomTable t{};
+        for (int n = 0; n <= FCIS_MAXORB; n++) {
+                t.v[n][0] = 1;
+                for (int k = 1; k <= n; k++) {
+                        t.v[n][k] = t.v[n - 1][k - 1] + (k <= n - 1 ? t.v[n - 1][k] : 0u);
+                }
+        }
+        return t;
+}
+
 __host__ __device__ static inline int pair_index(int p, int q)
 {
         int hi = p > q ? p : q, lo = p > q ? q : p;

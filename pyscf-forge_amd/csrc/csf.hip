@@ -44,12 +44,11 @@ struct CsfTable {
         CsfBlock b[CSF_MAXBLK];
 };
 
+__constant__ fcis::BinomTable c_binom = fcis::make_binom_table();
+
 __device__ inline uint32_t cbinom(int n, int k)
 {
-        if (k < 0 || k > n) return 0u;
-        uint64_t r = 1;
-        for (int i = 1; i <= k; i++) r = r * (uint64_t)(n - k + i) / (uint64_t)i;
-        return (uint32_t)r;
+        return (k < 0 || k > n) ? 0u : c_binom.v[n][k];
 }
 
 __device__ inline uint64_t cunrank(int n, int k, uint64_t addr)

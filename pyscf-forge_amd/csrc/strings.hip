@@ -19,23 +19,7 @@
 
 namespace {
 
-struct BinomTable {
-        uint32_t v[FCIS_MAXORB + 1][FCIS_MAXORB + 1];
-};
-
-constexpr BinomTable make_binom_table()
-{
-        BinomTable t{};
-        for (int n = 0; n <= FCIS_MAXORB; n++) {
-                t.v[n][0] = 1;
-                for (int k = 1; k <= n; k++) {
-                        t.v[n][k] = t.v[n - 1][k - 1] + (k <= n - 1 ? t.v[n - 1][k] : 0u);
-                }
-        }
-        return t;
-}
-
-__constant__ BinomTable c_binom = make_binom_table();
+__constant__ fcis::BinomTable c_binom = fcis::make_binom_table();
 
 __device__ inline uint32_t dbinom(int n, int k)
 {
