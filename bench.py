@@ -12,11 +12,12 @@ One "step" is one full FP64 sigma = H c with c and sigma resident in HBM.
 sigma rates with their error against the FP64 sigma, the FP64 and
 mixed-precision Davidson times to the ground state with their energy
 difference, the roofline of the dominant kernel (the contraction, timed with
-HIP events on its own stream inside this run), a CPU baseline, and the
+HIP events on its own stream inside this run), a CPU baseline, the
 batched workload: 64 uneven CAS problems (norb 6..12, 4 roots each) solved
-together with one batched sigma launch per Davidson iteration.
+together with one batched sigma launch per Davidson iteration, and the
+singlet ground state in the CSF basis (csf2det, sigma, det2csf per step).
 
-    python bench.py [--steps K] [--warmup W] [--no-davidson] [--no-cpu] [--no-batch]
+    python bench.py [--steps K] [--warmup W] [--no-davidson] [--no-cpu] [--no-batch] [--no-csf]
 """
 
 import argparse
@@ -167,6 +168,49 @@ def batch_solve(pkg, sync):
     return out
 
 
+def csf_solve(pkg, sync, h1e, eri, n, ne, max_cycle):
+    """Singlet ground state in the CSF basis (the reference's csf_fci
+    solver): Davidson on CSF vectors, each H application = csf2det, sigma,
+    det2csf on the device; FP64 and bf16x3-mixed."""
+    import torch
+    out = {}
+    t0 = time.perf_counter()
+    t = pkg.csf_fci.CSFTransformer(n, ne[0], ne[1], 1)
+    g = torch.Generator(device="cuda").manual_seed(1)
+    d = torch.randn(t.ndet, dtype=torch.float64, device="cuda", generator=g)
+    x = torch.empty(t.ncsf, dtype=torch.float64, device="cuda")
+    sync()
+    out["transformer_s"] = time.perf_counter() - t0
+    out["ncsf"], out["ndet"] = t.ncsf, t.ndet
+    for name, fn in [("det2csf_ms", lambda: t.det2csf_dev(d, x)), ("csf2det_ms", lambda: t.csf2det_dev(x, d))]:
+        fn()
+        sync()
+        t1 = time.perf_counter()
+        for _ in range(3):
+            fn()
+        sync()
+        out[name] = (time.perf_counter() - t1) / 3 * 1e3
+    del d, x
+    e64 = None
+    for prec in ["fp64", "bf16x3"]:
+        s = pkg.csf_fci.FCISolver(smult=1)
+        s.precision = prec
+        s.max_cycle = max_cycle
+        sync()
+        t1 = time.perf_counter()
+        e, ci = s.kernel(h1e, eri, n, ne)
+        sync()
+        dt = time.perf_counter() - t1
+        e64 = e if prec == "fp64" else e64
+        out[prec] = {"seconds": dt, "energy": e, "converged": bool(s.converged),
+                     "dE_vs_fp64": abs(e - e64),
+                     "stats": {k: v for k, v in s.stats.items() if isinstance(v, (int, float, str))}}
+        log("CSF Davidson %s: %.2f s, E = %.12f, %s" % (prec, dt, e, out[prec]["stats"]))
+        del ci, s
+        torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,6 +223,8 @@ def main():
     ap.add_argument("--max-cycle", type=int, default=100)
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the 64-problem batched solve (nroots=4)")
+    ap.add_argument("--no-csf", action="store_true",
+                    help="skip the CSF-basis (singlet) Davidson at the same size")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 4 GiB device copy first (HBM counter calibration)")
     args = ap.parse_args()
@@ -278,6 +324,10 @@ def main():
     if not args.no_batch:
         batch = batch_solve(pkg, sync)
 
+    csf = None
+    if not args.no_csf and not args.no_davidson:
+        csf = csf_solve(pkg, sync, h1e, eri, n, ne, args.max_cycle)
+
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (reference libfci_gas on host cores)")
@@ -320,6 +370,7 @@ def main():
         "kernels": kern,
         "davidson": dav,
         "batch": batch,
+        "csf": csf,
     }
     print(json.dumps(line), flush=True)
 
