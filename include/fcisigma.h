@@ -274,6 +274,64 @@ int fcis_precond(int64_t n, const double *d_hdiag, double e,
 int fcis_init_guess(int64_t n, int64_t addr, int root, double noise_norm,
                     double *d_out, void *stream);
 
+/* ---- Restricted (GAS) determinant spaces (pyscf/mcscf/fci_gas.py,
+ *      libfci_gas pyscf/lib/mcscf/fci_gas.h) ----------------------------
+ *
+ * GAS vectors are flat float64 device arrays in the reference's canonical
+ * block order: blocks in the order of the block set D, block b an
+ * alpha-major (nstr(sa), nstr(sb)) rectangle at offset[b] (fci_gas.h:52-60).
+ * The restricted Hamiltonian is P H P with P the projection on the GAS
+ * determinants; the device path embeds into the CAS array of the borrowed
+ * CAS space, applies the CAS kernels and projects back. */
+typedef struct fcis_gas_space fcis_gas_space_t;
+typedef struct fcis_gas_sigma_plan fcis_gas_sigma_plan_t;
+
+/* Space of the block set D (nblock rows of 2*ngas ints: alpha occupation
+ * per subspace, then beta), unique and lexicographically sorted, on the CAS
+ * space `cas` (norb = sum(norb[]), na, nb), which must outlive it.
+ * ngas <= 15, 1 <= norb[k] <= 31; the CAS space must hold fewer than 2^32
+ * determinants.  Sector ids, block offsets and sector string addresses
+ * equal the reference's.  Replaces gas_space_from_blocks (fci_gas.h:220,
+ * fci_string_gas.c:1683). */
+int fcis_gas_space_create(fcis_gas_space_t **out, const fcis_space_t *cas, int ngas,
+                          const int *norb, int nblock, const int *block_occ, void *stream);
+void fcis_gas_space_free(fcis_gas_space_t *gas);       /* gas_space_free */
+/* Determinants, blocks and sectors (gas_space_t ndet/nblock/nsector). */
+int fcis_gas_space_info(const fcis_gas_space_t *gas, int64_t *ndet, int *nblock,
+                        int *nsector);
+/* Per block (host arrays of nblock): offset, alpha/beta sector ids, string
+ * counts (GasSpace.block_descriptors, fci_gas.py:533-542). */
+int fcis_gas_block_info(const fcis_gas_space_t *gas, int64_t *offset, int *sa, int *sb,
+                        int64_t *na, int64_t *nb);
+/* Per sector: occupations (host [nsector][ngas]) and string counts. */
+int fcis_gas_sector_info(const fcis_gas_space_t *gas, int *occ, int64_t *nstr);
+/* gas_addr2str_sector / gas_str2addr_sector (fci_string_gas.c:462-489). */
+int fcis_gas_addr2str(const fcis_gas_space_t *gas, int sector, int64_t addr, uint64_t *str);
+int fcis_gas_str2addr(const fcis_gas_space_t *gas, int sector, uint64_t str, int64_t *addr);
+/* d_cas[v] = gas2fci(d_gas[v]) and d_gas[v] = fci2gas(d_cas[v]) for nvec
+ * vectors (fci_gas.py:141-208); d_cas is fully overwritten. */
+int fcis_gas_embed(const fcis_gas_space_t *gas, const double *d_gas, double *d_cas, int nvec,
+                   void *stream);
+int fcis_gas_project(const fcis_gas_space_t *gas, const double *d_cas, double *d_gas,
+                     int nvec, void *stream);
+/* Diagonal of H in GAS order (fci_make_hdiag_gas, fci_gas.h:205). */
+int fcis_gas_make_hdiag(const fcis_gas_space_t *gas, const double *h1e, const double *eri,
+                        double *d_hdiag, void *stream);
+/* Spin-resolved transition RDMs of GAS vectors, conventions and host
+ * outputs as fcis_trans_rdm12s (fci_rdm_gas_plan_make_rdm12s, fci_gas.h:287). */
+int fcis_gas_trans_rdm12s(const fcis_gas_space_t *gas, const double *d_bra,
+                          const double *d_ket, double *dm1a, double *dm1b, double *dm2aa,
+                          double *dm2ab, double *dm2bb, void *stream);
+/* Reusable restricted sigma plan: d_ci1 = P H P d_ci0 with the absorbed
+ * pair matrix h2e, precision and workspace as fcis_sigma_plan_create.  The
+ * space must outlive the plan; d_ci0 != d_ci1.  Replaces
+ * fci_contract_gas_plan_create / _execute / _free (fci_gas.h:242-249). */
+int fcis_gas_sigma_plan_create(fcis_gas_sigma_plan_t **out, const fcis_gas_space_t *gas,
+                               const double *h2e, int precision, uint64_t workspace_bytes);
+int fcis_gas_sigma_plan_execute(fcis_gas_sigma_plan_t *plan, const double *d_ci0,
+                                double *d_ci1, void *stream);
+void fcis_gas_sigma_plan_free(fcis_gas_sigma_plan_t *plan);
+
 #ifdef __cplusplus
 }
 #endif

@@ -311,6 +311,107 @@ class RefLib:
 
         return _Plan()
 
+    def gas(self, gas_orbs, nelec, blocks):
+        """The reference's own restricted space (gas_space_from_blocks) for
+        canonical blocks D; see RefGas."""
+        return RefGas(self.lib, gas_orbs, nelec, blocks)
+
+
+class RefGas:
+    """ctypes view of one libfci_gas space built from blocks (ref_harness.c
+    refh_gas_*): layout, sector strings, sigma, hdiag and RDMs."""
+
+    def __init__(self, lib, gas_orbs, nelec, blocks):
+        _l = ctypes.c_long
+        _lp = ctypes.POINTER(ctypes.c_long)
+        lib.refh_gas_create.restype = ctypes.c_void_p
+        lib.refh_gas_create.argtypes = [ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int, _ip]
+        lib.refh_gas_free.argtypes = [ctypes.c_void_p]
+        lib.refh_gas_info.argtypes = [ctypes.c_void_p, _lp, _ip, _ip]
+        lib.refh_gas_blocks.argtypes = [ctypes.c_void_p, _lp, _ip, _ip, _lp, _lp]
+        lib.refh_gas_sectors.argtypes = [ctypes.c_void_p, _ip, _lp]
+        lib.refh_gas_sector_strings.argtypes = [ctypes.c_void_p, ctypes.c_int, _up]
+        lib.refh_gas_contract.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        lib.refh_gas_hdiag.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        lib.refh_gas_rdm12s.argtypes = [ctypes.c_void_p] + [_dp] * 7
+        self.lib = lib
+        self.norb = tuple(int(x) for x in gas_orbs)
+        self.ngas = len(self.norb)
+        self.norb_total = sum(self.norb)
+        self.nelec = _unpack_nelec(nelec)
+        norb_a = numpy.ascontiguousarray(self.norb, dtype=numpy.int32)
+        occ = numpy.ascontiguousarray(blocks, dtype=numpy.int32).reshape(-1)
+        nblock = occ.size // (2 * self.ngas)
+        self.handle = lib.refh_gas_create(self.ngas, norb_a.ctypes.data_as(_ip),
+                                          self.nelec[0], self.nelec[1], nblock,
+                                          occ.ctypes.data_as(_ip))
+        if not self.handle:
+            raise RuntimeError("gas_space_from_blocks failed")
+        nd, nbk, ns = _l(), ctypes.c_int32(), ctypes.c_int32()
+        lib.refh_gas_info(self.handle, ctypes.byref(nd), ctypes.byref(nbk), ctypes.byref(ns))
+        self.ndet, self.nblock, self.nsector = nd.value, nbk.value, ns.value
+        off = numpy.zeros(self.nblock, dtype=numpy.int64)
+        sa = numpy.zeros(self.nblock, dtype=numpy.int32)
+        sb = numpy.zeros(self.nblock, dtype=numpy.int32)
+        na = numpy.zeros(self.nblock, dtype=numpy.int64)
+        nb = numpy.zeros(self.nblock, dtype=numpy.int64)
+        lib.refh_gas_blocks(self.handle, off.ctypes.data_as(_lp), sa.ctypes.data_as(_ip),
+                            sb.ctypes.data_as(_ip), na.ctypes.data_as(_lp),
+                            nb.ctypes.data_as(_lp))
+        self.block_desc = [dict(offset=int(a), sa=int(b), sb=int(c), na=int(d), nb=int(e))
+                           for a, b, c, d, e in zip(off, sa, sb, na, nb)]
+        socc = numpy.zeros((self.nsector, self.ngas), dtype=numpy.int32)
+        snstr = numpy.zeros(self.nsector, dtype=numpy.int64)
+        lib.refh_gas_sectors(self.handle, socc.ctypes.data_as(_ip), snstr.ctypes.data_as(_lp))
+        self.sectors = [tuple(int(v) for v in r) for r in socc]
+        self.sector_nstr = [int(x) for x in snstr]
+
+    def sector_strings(self, sid):
+        out = numpy.zeros(self.sector_nstr[sid], dtype=numpy.uint64)
+        self.lib.refh_gas_sector_strings(self.handle, int(sid), out.ctypes.data_as(_up))
+        return out.astype(numpy.int64)
+
+    def contract_2e(self, h2e, ci):
+        h2e = pair_matrix(h2e, self.norb_total)
+        c = _c64(ci).reshape(-1)
+        out = numpy.zeros_like(c)
+        st = self.lib.refh_gas_contract(self.handle, _d(h2e), _d(c), _d(out))
+        if st != 0:
+            raise RuntimeError("libfci_gas contract failed with status %d" % st)
+        return out
+
+    def make_hdiag(self, h1e, eri):
+        h1e = _c64(h1e, (self.norb_total, self.norb_total))
+        eri = pair_matrix(eri, self.norb_total)
+        out = numpy.empty(self.ndet)
+        st = self.lib.refh_gas_hdiag(self.handle, _d(h1e), _d(eri), _d(out))
+        if st != 0:
+            raise RuntimeError("libfci_gas hdiag failed with status %d" % st)
+        return out
+
+    def trans_rdm12s(self, bra, ket):
+        n = self.norb_total
+        bra = _c64(bra).reshape(-1)
+        ket = _c64(ket).reshape(-1)
+        d = [numpy.zeros((n, n)), numpy.zeros((n, n))] + [numpy.zeros((n,) * 4)
+                                                          for _ in range(3)]
+        st = self.lib.refh_gas_rdm12s(self.handle, _d(bra), _d(ket), *[_d(x) for x in d])
+        if st != 0:
+            raise RuntimeError("libfci_gas rdm failed with status %d" % st)
+        return (d[0], d[1]), (d[2], d[3], d[4])
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.refh_gas_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
 
 def ref_available():
     return os.path.exists(LIBREF)

@@ -183,3 +183,170 @@ long refh_str2addr(int norb, int na, int nb, int spin, unsigned long long str)
         gas_space_free(&gas);
         return addr;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Restricted (multi-space) GAS spaces, gas_space_from_blocks with the      */
+/* caller's canonical block set D (fci_gas.h:220, fci_string_gas.c:1683).   */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+        int ngas, na, nb, nblock;
+        int norb[GAS_MAX_NGAS];
+        int *occ;                       /* [nblock][2*ngas] */
+        gas_space_t raw;                /* raw links: layout, hdiag, RDMs */
+} refh_gas_t;
+
+static int gas_build(const refh_gas_t *h, gas_space_t *gas, int compress)
+{
+        memset(gas, 0, sizeof(*gas));
+        if (gas_space_from_blocks(gas, h->ngas, h->norb, h->na, h->nb,
+                                  h->nblock, h->occ) != 0) {
+                return -1;
+        }
+        if (compress && gas_space_compress_links(gas) != 0) {
+                gas_space_free(gas);
+                return -1;
+        }
+        return 0;
+}
+
+void refh_gas_free(void *handle)
+{
+        refh_gas_t *h = handle;
+        if (h == 0) {
+                return;
+        }
+        gas_space_free(&h->raw);
+        free(h->occ);
+        free(h);
+}
+
+void *refh_gas_create(int ngas, const int *norb, int na, int nb,
+                      int nblock, const int *block_occ)
+{
+        if (ngas <= 0 || ngas > GAS_MAX_NGAS || nblock <= 0) {
+                return 0;
+        }
+        refh_gas_t *h = calloc(1, sizeof(*h));
+        if (h == 0) {
+                return 0;
+        }
+        h->ngas = ngas;
+        h->na = na;
+        h->nb = nb;
+        h->nblock = nblock;
+        memcpy(h->norb, norb, sizeof(int) * (size_t)ngas);
+        h->occ = malloc(sizeof(int) * (size_t)nblock * 2 * ngas);
+        if (h->occ == 0) {
+                free(h);
+                return 0;
+        }
+        memcpy(h->occ, block_occ, sizeof(int) * (size_t)nblock * 2 * ngas);
+        if (gas_build(h, &h->raw, 0) != 0) {
+                free(h->occ);
+                free(h);
+                return 0;
+        }
+        return h;
+}
+
+/* ndet, nblock, nsector of the built space. */
+void refh_gas_info(void *handle, long *ndet, int *nblock, int *nsector)
+{
+        refh_gas_t *h = handle;
+        *ndet = (long)h->raw.ndet;
+        *nblock = (int)h->raw.nblock;
+        *nsector = (int)h->raw.nsector;
+}
+
+/* Per block: offset, alpha/beta sector ids and string counts. */
+void refh_gas_blocks(void *handle, long *offset, int *sa, int *sb,
+                     long *na, long *nb)
+{
+        refh_gas_t *h = handle;
+        for (uint32_t b = 0; b < h->raw.nblock; b++) {
+                const gas_block_t *blk = h->raw.block + b;
+                offset[b] = (long)blk->offset;
+                sa[b] = blk->sa;
+                sb[b] = blk->sb;
+                na[b] = (long)h->raw.sector_nstr[blk->sa];
+                nb[b] = (long)h->raw.sector_nstr[blk->sb];
+        }
+}
+
+/* Per sector: occupations [nsector][ngas] and string count. */
+void refh_gas_sectors(void *handle, int *occ, long *nstr)
+{
+        refh_gas_t *h = handle;
+        for (uint32_t s = 0; s < h->raw.nsector; s++) {
+                for (int g = 0; g < h->ngas; g++) {
+                        occ[s * h->ngas + g] = h->raw.sector_occ[s * h->ngas + g];
+                }
+                nstr[s] = (long)h->raw.sector_nstr[s];
+        }
+}
+
+/* All strings of one sector in address order. */
+void refh_gas_sector_strings(void *handle, int sector, unsigned long long *out)
+{
+        refh_gas_t *h = handle;
+        for (uint32_t a = 0; a < h->raw.sector_nstr[sector]; a++) {
+                out[a] = gas_addr2str_sector(&h->raw, (gas_sid_t)sector, a);
+        }
+}
+
+/* H c with the absorbed pair matrix h2e; gos = h2e + h2e^T (fci_gas.py:588). */
+int refh_gas_contract(void *handle, const double *h2e, const double *ci0,
+                      double *ci1)
+{
+        refh_gas_t *h = handle;
+        gas_space_t gas;
+        gas_contract_plan_t *plan = 0;
+        size_t norb = 0, npair;
+        double *gos;
+        int ret;
+        for (int g = 0; g < h->ngas; g++) {
+                norb += (size_t)h->norb[g];
+        }
+        npair = norb * (norb + 1) / 2;
+        gos = malloc(sizeof(double) * npair * npair);
+        if (gos == 0 || gas_build(h, &gas, 1) != 0) {
+                free(gos);
+                return -1;
+        }
+        for (size_t i = 0; i < npair; i++) {
+                for (size_t j = 0; j < npair; j++) {
+                        gos[i * npair + j] = h2e[i * npair + j] + h2e[j * npair + i];
+                }
+        }
+        ret = fci_contract_gas_plan_create(&plan, &gas, h2e, gos);
+        if (ret == 0) {
+                ret = fci_contract_gas_plan_execute(plan, ci0, ci1);
+                fci_contract_gas_plan_free(plan);
+        }
+        gas_space_free(&gas);
+        free(gos);
+        return ret;
+}
+
+int refh_gas_hdiag(void *handle, const double *h1e, const double *eri,
+                   double *hdiag)
+{
+        refh_gas_t *h = handle;
+        return fci_make_hdiag_gas(&h->raw, h1e, eri, hdiag);
+}
+
+int refh_gas_rdm12s(void *handle, const double *bra, const double *ket,
+                    double *dm1a, double *dm1b, double *dm2aa, double *dm2ab,
+                    double *dm2bb)
+{
+        refh_gas_t *h = handle;
+        gas_rdm_plan_t *plan = 0;
+        int ret = fci_rdm_gas_plan_create(&plan, &h->raw);
+        if (ret == 0) {
+                ret = fci_rdm_gas_plan_make_rdm12s(plan, bra, ket, dm1a, dm1b,
+                                                   dm2aa, dm2ab, dm2bb);
+                fci_rdm_gas_plan_free(plan);
+        }
+        return ret;
+}

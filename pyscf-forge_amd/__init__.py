@@ -22,3 +22,5 @@ from . import direct_spin1  # noqa: F401,E402
 from . import autograd  # noqa: F401,E402
 from . import csf_fci  # noqa: F401,E402
 from . import batch  # noqa: F401,E402
+from . import addons_gas  # noqa: F401,E402
+from . import fci_gas  # noqa: F401,E402

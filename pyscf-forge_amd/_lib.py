@@ -75,6 +75,22 @@ SIGNATURES = {
                                     _vp, _vp, _vp]),
     "fcis_init_guess": (ctypes.c_int, [_i64, _i64, ctypes.c_int, ctypes.c_double,
                                        _vp, _vp]),
+    "fcis_gas_space_create": (ctypes.c_int, [ctypes.POINTER(_vp), _vp, ctypes.c_int, _ip,
+                                             ctypes.c_int, _ip, _vp]),
+    "fcis_gas_space_free": (None, [_vp]),
+    "fcis_gas_space_info": (ctypes.c_int, [_vp, _i64p, _ip, _ip]),
+    "fcis_gas_block_info": (ctypes.c_int, [_vp, _i64p, _ip, _ip, _i64p, _i64p]),
+    "fcis_gas_sector_info": (ctypes.c_int, [_vp, _ip, _i64p]),
+    "fcis_gas_addr2str": (ctypes.c_int, [_vp, ctypes.c_int, _i64, _u64p]),
+    "fcis_gas_str2addr": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint64, _i64p]),
+    "fcis_gas_embed": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp]),
+    "fcis_gas_project": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp]),
+    "fcis_gas_make_hdiag": (ctypes.c_int, [_vp, _dp, _dp, _vp, _vp]),
+    "fcis_gas_trans_rdm12s": (ctypes.c_int, [_vp, _vp, _vp, _dp, _dp, _dp, _dp, _dp, _vp]),
+    "fcis_gas_sigma_plan_create": (ctypes.c_int, [ctypes.POINTER(_vp), _vp, _dp, ctypes.c_int,
+                                                  ctypes.c_uint64]),
+    "fcis_gas_sigma_plan_execute": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "fcis_gas_sigma_plan_free": (None, [_vp]),
 }
 
 _LIB = None

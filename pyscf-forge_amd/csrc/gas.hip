@@ -1,0 +1,583 @@
+// Restricted (GAS) determinant spaces on the device
+// (pyscf-forge's GAS-CI, pyscf/mcscf/fci_gas.py, pyscf/lib/mcscf/).
+//
+// A GAS space splits the norb active orbitals into ngas consecutive
+// subspaces and keeps the determinants of a block set D: rows (alpha
+// occupation per subspace, beta occupation per subspace), unique and
+// lexicographically sorted.  Layout, as the reference's gas_space_t
+// (fci_gas.h:52-60, fci_string_gas.c:588-791):
+//   sectors  the alpha and beta occupation tuples of D and their single
+//            inter-subspace excitations, unique, sorted lexicographically;
+//            a sector's strings are addressed row-major over subspaces with
+//            the last subspace fastest, each subspace by its combinatorial
+//            address;
+//   blocks   in the order of D; block b is an alpha-major rectangle
+//            (nstr(sa) x nstr(sb)) at offset = running determinant count.
+// The restricted Hamiltonian is the projection P H P of the CAS one
+// (pyscf/mcscf/test/test_gasci.py:522-575 checks libfci_gas that way), so
+// the device path embeds a GAS vector in the CAS (nstr_a, nstr_b) array
+// (fci_gas.py gas2fci :141-182), runs the CAS sigma and projects back
+// (fci2gas :185-208).  d_map[d] is the CAS determinant index of GAS
+// determinant d; it is built on the device from per-sector CAS string
+// addresses.
+#include <stdlib.h>
+#include <string.h>
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "common.h"
+
+#define GAS_MAXNGAS 15
+
+struct fcis_gas_space {
+        const fcis_space *cas;
+        int ngas, na, nb;
+        int norb[GAS_MAXNGAS], start[GAS_MAXNGAS];
+        int nsector;
+        std::vector<uint8_t> occ;       // [nsector][ngas]
+        std::vector<int64_t> stride;    // [nsector][ngas]
+        std::vector<int64_t> nstr;      // [nsector]
+        int nblock;
+        std::vector<int64_t> boff;      // [nblock]
+        std::vector<int> bsa, bsb;      // [nblock]
+        int64_t ndet;
+        int64_t ncas;
+        uint32_t *d_map;                // [ndet] CAS determinant index
+};
+
+struct fcis_gas_sigma_plan {
+        const fcis_gas_space *gas;
+        fcis_sigma_plan_t *cas_plan;
+        double *d_a, *d_b;              // CAS-sized embed / sigma buffers
+};
+
+namespace {
+
+constexpr int GAS_THREADS = 256;
+
+uint64_t unrank_local(int n, int k, int64_t addr)
+{
+        uint64_t s = 0;
+        int e = k;
+        for (int p = n - 1; p >= 0 && e > 0; p--) {
+                int64_t c = fcis::binom_host(p, e);
+                if (c <= addr) {
+                        s |= 1ull << p;
+                        addr -= c;
+                        e--;
+                }
+        }
+        return s;
+}
+
+int64_t rank_colex(uint64_t s)
+{
+        int64_t a = 0;
+        int i = 1;
+        while (s) {
+                int p = __builtin_ctzll(s);
+                a += fcis::binom_host(p, i++);
+                s &= s - 1;
+        }
+        return a;
+}
+
+uint64_t sector_string(const fcis_gas_space *g, int s, int64_t addr)
+{
+        uint64_t str = 0;
+        const uint8_t *occ = &g->occ[(size_t)s * g->ngas];
+        const int64_t *st = &g->stride[(size_t)s * g->ngas];
+        for (int k = 0; k < g->ngas; k++) {
+                int64_t q = addr / st[k];
+                addr -= q * st[k];
+                str |= unrank_local(g->norb[k], occ[k], q) << g->start[k];
+        }
+        return str;
+}
+
+// Grid-stride: d_map[d] for every GAS determinant.  Block b of a
+// determinant is found by binary search over the block offsets.
+__global__ void k_gas_map(const int64_t *__restrict__ boff, const int64_t *__restrict__ bnb,
+                          const int64_t *__restrict__ abase, const int64_t *__restrict__ bbase,
+                          int nblock, const uint32_t *__restrict__ secaddr, int64_t nstrb,
+                          int64_t ndet, uint32_t *__restrict__ map)
+{
+        for (int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; d < ndet;
+             d += (int64_t)gridDim.x * blockDim.x) {
+                int lo = 0, hi = nblock - 1;
+                while (lo < hi) {
+                        int mid = (lo + hi + 1) >> 1;
+                        if (boff[mid] <= d) lo = mid;
+                        else hi = mid - 1;
+                }
+                int64_t r = d - boff[lo];
+                int64_t nb = bnb[lo];
+                int64_t ia = r / nb, ib = r - ia * nb;
+                map[d] = (uint32_t)((int64_t)secaddr[abase[lo] + ia] * nstrb +
+                                    secaddr[bbase[lo] + ib]);
+        }
+}
+
+__global__ void k_gas_zero(double *__restrict__ x, int64_t n)
+{
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+             i += (int64_t)gridDim.x * blockDim.x) {
+                x[i] = 0.0;
+        }
+}
+
+// cas[v][map[d]] = gas[v][d]
+__global__ void k_gas_scatter(const uint32_t *__restrict__ map, const double *__restrict__ gas,
+                              double *__restrict__ cas, int64_t ndet, int64_t ncas)
+{
+        const int v = blockIdx.y;
+        const double *src = gas + (int64_t)v * ndet;
+        double *dst = cas + (int64_t)v * ncas;
+        for (int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; d < ndet;
+             d += (int64_t)gridDim.x * blockDim.x) {
+                dst[map[d]] = src[d];
+        }
+}
+
+// gas[v][d] = cas[v][map[d]]
+__global__ void k_gas_gather(const uint32_t *__restrict__ map, const double *__restrict__ cas,
+                             double *__restrict__ gas, int64_t ndet, int64_t ncas)
+{
+        const int v = blockIdx.y;
+        const double *src = cas + (int64_t)v * ncas;
+        double *dst = gas + (int64_t)v * ndet;
+        for (int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; d < ndet;
+             d += (int64_t)gridDim.x * blockDim.x) {
+                dst[d] = src[map[d]];
+        }
+}
+
+unsigned grid_for(int64_t n)
+{
+        int64_t g = (n + GAS_THREADS - 1) / GAS_THREADS;
+        return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, 8192));
+}
+
+int embed(const fcis_gas_space *g, const double *in, double *out, int nvec, hipStream_t st)
+{
+        hipLaunchKernelGGL(k_gas_zero, dim3(grid_for(g->ncas * nvec)), dim3(GAS_THREADS), 0, st,
+                           out, g->ncas * nvec);
+        FCIS_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_gas_scatter, dim3(grid_for(g->ndet), nvec), dim3(GAS_THREADS), 0, st,
+                           g->d_map, in, out, g->ndet, g->ncas);
+        FCIS_CHECK_LAUNCH();
+        return FCIS_SUCCESS;
+}
+
+int project(const fcis_gas_space *g, const double *in, double *out, int nvec, hipStream_t st)
+{
+        hipLaunchKernelGGL(k_gas_gather, dim3(grid_for(g->ndet), nvec), dim3(GAS_THREADS), 0, st,
+                           g->d_map, in, out, g->ndet, g->ncas);
+        FCIS_CHECK_LAUNCH();
+        return FCIS_SUCCESS;
+}
+
+int lex_cmp(const uint8_t *a, const uint8_t *b, int n)
+{
+        for (int i = 0; i < n; i++) {
+                if (a[i] != b[i]) return a[i] < b[i] ? -1 : 1;
+        }
+        return 0;
+}
+
+// Device scratch of n doubles, freed after the stream drains.
+struct Scratch {
+        double *p = nullptr;
+        hipStream_t st;
+        explicit Scratch(hipStream_t s) : st(s) {}
+        ~Scratch()
+        {
+                if (p) {
+                        (void)hipStreamSynchronize(st);
+                        (void)hipFree(p);
+                }
+        }
+};
+
+}  // namespace
+
+extern "C" {
+
+void fcis_gas_space_free(fcis_gas_space_t *g)
+{
+        if (g == nullptr) return;
+        if (g->d_map) (void)hipFree(g->d_map);
+        delete g;
+}
+
+int fcis_gas_space_create(fcis_gas_space_t **out, const fcis_space_t *cas, int ngas,
+                          const int *norb, int nblock, const int *block_occ, void *stream)
+{
+        if (out == nullptr || cas == nullptr || norb == nullptr || block_occ == nullptr ||
+            ngas < 1 || ngas > GAS_MAXNGAS || nblock < 1) {
+                fcis::set_error("fcis_gas_space_create: invalid argument");
+                return FCIS_ERR_INVALID;
+        }
+        *out = nullptr;
+        hipStream_t st = (hipStream_t)stream;
+        fcis_gas_space *g = new (std::nothrow) fcis_gas_space();
+        if (g == nullptr) return FCIS_ERR_MEMORY;
+        g->cas = cas;
+        g->ngas = ngas;
+        g->na = cas->na;
+        g->nb = cas->nb;
+        g->d_map = nullptr;
+        int tot = 0;
+        for (int k = 0; k < ngas; k++) {
+                if (norb[k] < 1 || norb[k] > 31) {
+                        fcis_gas_space_free(g);
+                        fcis::set_error("fcis_gas_space_create: subspace %d has %d orbitals", k,
+                                        norb[k]);
+                        return FCIS_ERR_INVALID;
+                }
+                g->norb[k] = norb[k];
+                g->start[k] = tot;
+                tot += norb[k];
+        }
+        if (tot != cas->norb) {
+                fcis_gas_space_free(g);
+                fcis::set_error("fcis_gas_space_create: subspaces hold %d orbitals, the CAS "
+                                "space %d", tot, cas->norb);
+                return FCIS_ERR_INVALID;
+        }
+        g->ncas = cas->nstra * cas->nstrb;
+        if (g->ncas >= ((int64_t)1 << 32)) {
+                fcis_gas_space_free(g);
+                fcis::set_error("fcis_gas_space_create: CAS space of 2^32 or more determinants");
+                return FCIS_ERR_INVALID;
+        }
+        // blocks: valid occupations, electron counts, strictly increasing
+        const int w = 2 * ngas;
+        std::vector<uint8_t> rows((size_t)nblock * w);
+        for (int b = 0; b < nblock; b++) {
+                int sa = 0, sb = 0;
+                for (int k = 0; k < w; k++) {
+                        int v = block_occ[(size_t)b * w + k];
+                        if (v < 0 || v > norb[k % ngas]) {
+                                fcis_gas_space_free(g);
+                                fcis::set_error("fcis_gas_space_create: block %d occupation "
+                                                "out of range", b);
+                                return FCIS_ERR_INVALID;
+                        }
+                        rows[(size_t)b * w + k] = (uint8_t)v;
+                        (k < ngas ? sa : sb) += v;
+                }
+                if (sa != g->na || sb != g->nb) {
+                        fcis_gas_space_free(g);
+                        fcis::set_error("fcis_gas_space_create: block %d holds (%d, %d) "
+                                        "electrons, the space (%d, %d)", b, sa, sb, g->na, g->nb);
+                        return FCIS_ERR_INVALID;
+                }
+                if (b > 0 && lex_cmp(&rows[(size_t)(b - 1) * w], &rows[(size_t)b * w], w) >= 0) {
+                        fcis_gas_space_free(g);
+                        fcis::set_error("fcis_gas_space_create: blocks must be unique and "
+                                        "sorted (block %d)", b);
+                        return FCIS_ERR_INVALID;
+                }
+        }
+        // sector pool: block sectors and their single inter-subspace moves
+        std::vector<std::vector<uint8_t>> pool;
+        std::vector<uint8_t> t(ngas);
+        for (int b = 0; b < nblock; b++) {
+                for (int spin = 0; spin < 2; spin++) {
+                        const uint8_t *o = &rows[(size_t)b * w + spin * ngas];
+                        pool.emplace_back(o, o + ngas);
+                        for (int r = 0; r < ngas; r++) {
+                                for (int s = 0; s < ngas; s++) {
+                                        if (r == s || o[r] == 0 || o[s] >= norb[s]) continue;
+                                        std::copy(o, o + ngas, t.begin());
+                                        t[r]--;
+                                        t[s]++;
+                                        pool.push_back(t);
+                                }
+                        }
+                }
+        }
+        std::sort(pool.begin(), pool.end());
+        pool.erase(std::unique(pool.begin(), pool.end()), pool.end());
+        if (pool.size() > 65534) {
+                fcis_gas_space_free(g);
+                fcis::set_error("fcis_gas_space_create: more than 65534 sectors");
+                return FCIS_ERR_INVALID;
+        }
+        g->nsector = (int)pool.size();
+        g->occ.resize((size_t)g->nsector * ngas);
+        g->stride.resize((size_t)g->nsector * ngas);
+        g->nstr.resize(g->nsector);
+        for (int s = 0; s < g->nsector; s++) {
+                int64_t acc = 1;
+                for (int k = ngas - 1; k >= 0; k--) {
+                        g->occ[(size_t)s * ngas + k] = pool[s][k];
+                        g->stride[(size_t)s * ngas + k] = acc;
+                        acc *= fcis::binom_host(norb[k], pool[s][k]);
+                }
+                g->nstr[s] = acc;
+        }
+        auto find = [&](const uint8_t *o) {
+                std::vector<uint8_t> key(o, o + ngas);
+                return (int)(std::lower_bound(pool.begin(), pool.end(), key) - pool.begin());
+        };
+        g->nblock = nblock;
+        g->boff.resize(nblock);
+        g->bsa.resize(nblock);
+        g->bsb.resize(nblock);
+        int64_t off = 0;
+        for (int b = 0; b < nblock; b++) {
+                g->bsa[b] = find(&rows[(size_t)b * w]);
+                g->bsb[b] = find(&rows[(size_t)b * w + ngas]);
+                g->boff[b] = off;
+                off += g->nstr[g->bsa[b]] * g->nstr[g->bsb[b]];
+        }
+        g->ndet = off;
+        if (g->ndet >= ((int64_t)1 << 32)) {
+                fcis_gas_space_free(g);
+                fcis::set_error("fcis_gas_space_create: 2^32 or more determinants");
+                return FCIS_ERR_INVALID;
+        }
+        // CAS string addresses of every sector used by a block
+        std::vector<int64_t> base(g->nsector, -1);
+        std::vector<uint32_t> secaddr;
+        for (int b = 0; b < nblock; b++) {
+                for (int s : {g->bsa[b], g->bsb[b]}) {
+                        if (base[s] >= 0) continue;
+                        base[s] = (int64_t)secaddr.size();
+                        for (int64_t a = 0; a < g->nstr[s]; a++) {
+                                secaddr.push_back((uint32_t)rank_colex(sector_string(g, s, a)));
+                        }
+                }
+        }
+        std::vector<int64_t> tab(4 * (size_t)nblock);
+        for (int b = 0; b < nblock; b++) {
+                tab[b] = g->boff[b];
+                tab[nblock + b] = g->nstr[g->bsb[b]];
+                tab[2 * nblock + b] = base[g->bsa[b]];
+                tab[3 * nblock + b] = base[g->bsb[b]];
+        }
+        int64_t *d_tab = nullptr;
+        uint32_t *d_sec = nullptr;
+        int st_ok = FCIS_SUCCESS;
+        if (hipMalloc(&g->d_map, 4 * (size_t)g->ndet + 4) != hipSuccess ||
+            hipMalloc(&d_tab, 8 * tab.size()) != hipSuccess ||
+            hipMalloc(&d_sec, 4 * secaddr.size() + 4) != hipSuccess) {
+                fcis::set_error("fcis_gas_space_create: device allocation failed");
+                st_ok = FCIS_ERR_MEMORY;
+        } else if (hipMemcpyAsync(d_tab, tab.data(), 8 * tab.size(), hipMemcpyHostToDevice, st) !=
+                           hipSuccess ||
+                   hipMemcpyAsync(d_sec, secaddr.data(), 4 * secaddr.size(),
+                                  hipMemcpyHostToDevice, st) != hipSuccess) {
+                fcis::set_error("fcis_gas_space_create: upload failed");
+                st_ok = FCIS_ERR_DEVICE;
+        } else {
+                hipLaunchKernelGGL(k_gas_map, dim3(grid_for(g->ndet)), dim3(GAS_THREADS), 0, st,
+                                   d_tab, d_tab + nblock, d_tab + 2 * nblock, d_tab + 3 * nblock,
+                                   nblock, d_sec, cas->nstrb, g->ndet, g->d_map);
+                hipError_t e = hipGetLastError();
+                if (e == hipSuccess) e = hipStreamSynchronize(st);
+                if (e != hipSuccess) {
+                        fcis::set_error("fcis_gas_space_create: %s", hipGetErrorString(e));
+                        st_ok = FCIS_ERR_DEVICE;
+                }
+        }
+        if (d_tab) (void)hipFree(d_tab);
+        if (d_sec) (void)hipFree(d_sec);
+        if (st_ok != FCIS_SUCCESS) {
+                fcis_gas_space_free(g);
+                return st_ok;
+        }
+        *out = g;
+        return FCIS_SUCCESS;
+}
+
+int fcis_gas_space_info(const fcis_gas_space_t *g, int64_t *ndet, int *nblock, int *nsector)
+{
+        if (g == nullptr) {
+                fcis::set_error("fcis_gas_space_info: null space");
+                return FCIS_ERR_INVALID;
+        }
+        if (ndet) *ndet = g->ndet;
+        if (nblock) *nblock = g->nblock;
+        if (nsector) *nsector = g->nsector;
+        return FCIS_SUCCESS;
+}
+
+int fcis_gas_block_info(const fcis_gas_space_t *g, int64_t *offset, int *sa, int *sb,
+                        int64_t *na, int64_t *nb)
+{
+        if (g == nullptr) {
+                fcis::set_error("fcis_gas_block_info: null space");
+                return FCIS_ERR_INVALID;
+        }
+        for (int b = 0; b < g->nblock; b++) {
+                if (offset) offset[b] = g->boff[b];
+                if (sa) sa[b] = g->bsa[b];
+                if (sb) sb[b] = g->bsb[b];
+                if (na) na[b] = g->nstr[g->bsa[b]];
+                if (nb) nb[b] = g->nstr[g->bsb[b]];
+        }
+        return FCIS_SUCCESS;
+}
+
+int fcis_gas_sector_info(const fcis_gas_space_t *g, int *occ, int64_t *nstr)
+{
+        if (g == nullptr) {
+                fcis::set_error("fcis_gas_sector_info: null space");
+                return FCIS_ERR_INVALID;
+        }
+        for (int s = 0; s < g->nsector; s++) {
+                if (occ) {
+                        for (int k = 0; k < g->ngas; k++) {
+                                occ[(size_t)s * g->ngas + k] = g->occ[(size_t)s * g->ngas + k];
+                        }
+                }
+                if (nstr) nstr[s] = g->nstr[s];
+        }
+        return FCIS_SUCCESS;
+}
+
+int fcis_gas_addr2str(const fcis_gas_space_t *g, int sector, int64_t addr, uint64_t *str)
+{
+        if (g == nullptr || str == nullptr || sector < 0 || sector >= g->nsector || addr < 0 ||
+            addr >= g->nstr[sector]) {
+                fcis::set_error("fcis_gas_addr2str: invalid sector or address");
+                return FCIS_ERR_INVALID;
+        }
+        *str = sector_string(g, sector, addr);
+        return FCIS_SUCCESS;
+}
+
+int fcis_gas_str2addr(const fcis_gas_space_t *g, int sector, uint64_t str, int64_t *addr)
+{
+        if (g == nullptr || addr == nullptr || sector < 0 || sector >= g->nsector) {
+                fcis::set_error("fcis_gas_str2addr: invalid sector");
+                return FCIS_ERR_INVALID;
+        }
+        int64_t a = 0;
+        for (int k = 0; k < g->ngas; k++) {
+                uint64_t sub = (str >> g->start[k]) & ((1ull << g->norb[k]) - 1);
+                if (__builtin_popcountll(sub) != g->occ[(size_t)sector * g->ngas + k]) {
+                        fcis::set_error("fcis_gas_str2addr: string not in sector %d", sector);
+                        return FCIS_ERR_INVALID;
+                }
+                a += rank_colex(sub) * g->stride[(size_t)sector * g->ngas + k];
+        }
+        if (str >> (g->start[g->ngas - 1] + g->norb[g->ngas - 1])) {
+                fcis::set_error("fcis_gas_str2addr: string has orbitals outside the space");
+                return FCIS_ERR_INVALID;
+        }
+        *addr = a;
+        return FCIS_SUCCESS;
+}
+
+int fcis_gas_embed(const fcis_gas_space_t *g, const double *d_gas, double *d_cas, int nvec,
+                   void *stream)
+{
+        if (g == nullptr || d_gas == nullptr || d_cas == nullptr || nvec < 1 || nvec > 65535) {
+                fcis::set_error("fcis_gas_embed: invalid argument");
+                return FCIS_ERR_INVALID;
+        }
+        return embed(g, d_gas, d_cas, nvec, (hipStream_t)stream);
+}
+
+int fcis_gas_project(const fcis_gas_space_t *g, const double *d_cas, double *d_gas, int nvec,
+                     void *stream)
+{
+        if (g == nullptr || d_gas == nullptr || d_cas == nullptr || nvec < 1 || nvec > 65535) {
+                fcis::set_error("fcis_gas_project: invalid argument");
+                return FCIS_ERR_INVALID;
+        }
+        return project(g, d_cas, d_gas, nvec, (hipStream_t)stream);
+}
+
+int fcis_gas_make_hdiag(const fcis_gas_space_t *g, const double *h1e, const double *eri,
+                        double *d_hdiag, void *stream)
+{
+        if (g == nullptr || h1e == nullptr || eri == nullptr || d_hdiag == nullptr) {
+                fcis::set_error("fcis_gas_make_hdiag: invalid argument");
+                return FCIS_ERR_INVALID;
+        }
+        hipStream_t st = (hipStream_t)stream;
+        Scratch tmp(st);
+        FCIS_HIP(hipMalloc(&tmp.p, 8 * (size_t)g->ncas));
+        int r = fcis_make_hdiag(g->cas, h1e, eri, tmp.p, stream);
+        if (r != FCIS_SUCCESS) return r;
+        return project(g, tmp.p, d_hdiag, 1, st);
+}
+
+int fcis_gas_trans_rdm12s(const fcis_gas_space_t *g, const double *d_bra, const double *d_ket,
+                          double *dm1a, double *dm1b, double *dm2aa, double *dm2ab,
+                          double *dm2bb, void *stream)
+{
+        if (g == nullptr || d_bra == nullptr || d_ket == nullptr) {
+                fcis::set_error("fcis_gas_trans_rdm12s: invalid argument");
+                return FCIS_ERR_INVALID;
+        }
+        hipStream_t st = (hipStream_t)stream;
+        Scratch tmp(st);
+        FCIS_HIP(hipMalloc(&tmp.p, 16 * (size_t)g->ncas));
+        int r = embed(g, d_bra, tmp.p, 1, st);
+        if (r == FCIS_SUCCESS) r = embed(g, d_ket, tmp.p + g->ncas, 1, st);
+        if (r == FCIS_SUCCESS) {
+                r = fcis_trans_rdm12s(g->cas, tmp.p, tmp.p + g->ncas, dm1a, dm1b, dm2aa, dm2ab,
+                                      dm2bb, stream);
+        }
+        return r;
+}
+
+void fcis_gas_sigma_plan_free(fcis_gas_sigma_plan_t *p)
+{
+        if (p == nullptr) return;
+        if (p->cas_plan) fcis_sigma_plan_free(p->cas_plan);
+        if (p->d_a) (void)hipFree(p->d_a);
+        delete p;
+}
+
+int fcis_gas_sigma_plan_create(fcis_gas_sigma_plan_t **out, const fcis_gas_space_t *g,
+                               const double *h2e, int precision, uint64_t workspace_bytes)
+{
+        if (out == nullptr || g == nullptr || h2e == nullptr) {
+                fcis::set_error("fcis_gas_sigma_plan_create: invalid argument");
+                return FCIS_ERR_INVALID;
+        }
+        *out = nullptr;
+        fcis_gas_sigma_plan *p = new (std::nothrow) fcis_gas_sigma_plan();
+        if (p == nullptr) return FCIS_ERR_MEMORY;
+        p->gas = g;
+        p->cas_plan = nullptr;
+        p->d_a = p->d_b = nullptr;
+        if (hipMalloc(&p->d_a, 16 * (size_t)g->ncas) != hipSuccess) {
+                fcis_gas_sigma_plan_free(p);
+                fcis::set_error("fcis_gas_sigma_plan_create: device allocation failed");
+                return FCIS_ERR_MEMORY;
+        }
+        p->d_b = p->d_a + g->ncas;
+        int r = fcis_sigma_plan_create(&p->cas_plan, g->cas, h2e, precision, workspace_bytes);
+        if (r != FCIS_SUCCESS) {
+                fcis_gas_sigma_plan_free(p);
+                return r;
+        }
+        *out = p;
+        return FCIS_SUCCESS;
+}
+
+int fcis_gas_sigma_plan_execute(fcis_gas_sigma_plan_t *p, const double *d_ci0, double *d_ci1,
+                                void *stream)
+{
+        if (p == nullptr || d_ci0 == nullptr || d_ci1 == nullptr || d_ci0 == d_ci1) {
+                fcis::set_error("fcis_gas_sigma_plan_execute: invalid argument (input and "
+                                "output must be distinct device arrays)");
+                return FCIS_ERR_INVALID;
+        }
+        hipStream_t st = (hipStream_t)stream;
+        int r = embed(p->gas, d_ci0, p->d_a, 1, st);
+        if (r == FCIS_SUCCESS) r = fcis_sigma_plan_execute(p->cas_plan, p->d_a, p->d_b, stream);
+        if (r == FCIS_SUCCESS) r = project(p->gas, p->d_b, d_ci1, 1, st);
+        return r;
+}
+
+}  // extern "C"

@@ -1,0 +1,703 @@
+"""Restricted (GAS) CI on the device, with the interface of pyscf-forge's
+GAS solver (pyscf/mcscf/fci_gas.py).
+
+CI vectors are one-dimensional arrays in the reference's canonical GAS block
+order.  The restricted Hamiltonian is P H P, P the projection on the GAS
+determinants (the identity the reference's test checks libfci_gas against,
+pyscf/mcscf/test/test_gasci.py:522-575): every sigma embeds the GAS vector
+in the CAS array on the device (gas2fci), runs the CAS sigma kernels and
+projects back (fci2gas); the diagonal and the RDMs come from the CAS kernels
+the same way.  Cost per sigma is therefore that of the CAS sigma of
+sum(gas_orbs) orbitals, and the space is limited to CAS spaces of fewer
+than 2^32 determinants that fit in device memory.
+"""
+
+import ctypes
+
+import numpy
+
+from . import _lib
+from . import addons_gas
+from . import direct_spin1 as _ds
+
+GAS_PSPACE_MATVEC_MAX = 64      # fci_gas.py:38
+GAS_INIT_NOISE_NORM = 1e-3      # fci_gas.py:40
+
+
+def _pair(eri, norb):
+    return _ds.pair_matrix(eri, norb)
+
+
+def absorb_h1e(h1e, eri, norb, nelec, fac=0.5):
+    """Absorbed two-electron Hamiltonian as an (npair, npair) pair matrix
+    (fci_gas.py:132-138)."""
+    h1e = numpy.ascontiguousarray(h1e, dtype=numpy.float64).reshape(int(norb), int(norb))
+    return _ds.absorb_h1e(h1e, _pair(eri, norb), norb, nelec, fac)
+
+
+class GasSpace:
+    """Device GAS space for subspace sizes ``norb`` and block set ``blocks``
+    (GasSpace, fci_gas.py:420-567).  Usable as a context manager; close()
+    releases the device map.  The CAS string space of sum(norb) orbitals
+    is shared with the CAS solver."""
+
+    def __init__(self, norb, nelec, blocks=None, lib=None, compress_links=False):
+        self.norb = tuple(int(x) for x in norb)
+        self.nelec = _ds._unpack_nelec(nelec)
+        if blocks is None:
+            if len(self.norb) != 1:
+                raise ValueError("blocks are required for multi-space GAS")
+            blocks = numpy.asarray([self.nelec], dtype=numpy.int32)
+        self.blocks = addons_gas.normalize_blocks(blocks, len(self.norb))
+        self.limits = addons_gas.check_kernel_limits(self.norb, self.nelec, self.blocks)
+        self.lib = lib or _lib.load()
+        self.cas = _ds.get_space(sum(self.norb), self.nelec)
+        self._compressed = bool(compress_links)
+        norb_a = numpy.ascontiguousarray(self.norb, dtype=numpy.int32)
+        occ = numpy.ascontiguousarray(self.blocks.reshape(-1), dtype=numpy.int32)
+        h = ctypes.c_void_p()
+        _lib.check(self.lib.fcis_gas_space_create(
+            ctypes.byref(h), self.cas.handle, len(self.norb),
+            norb_a.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), int(self.blocks.shape[0]),
+            occ.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _lib.stream_ptr()),
+            "fcis_gas_space_create")
+        self.handle = h
+        nd, nb, ns = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()
+        _lib.check(self.lib.fcis_gas_space_info(h, ctypes.byref(nd), ctypes.byref(nb),
+                                                ctypes.byref(ns)), "fcis_gas_space_info")
+        self._ndet, self._nblock, self._nsector = nd.value, nb.value, ns.value
+        self._blocks_desc = None
+
+    # -- descriptors (fci_gas.py:459-542) -------------------------------
+    def _live(self):
+        if self.handle is None:
+            raise RuntimeError("GAS space is closed")
+        return self.handle
+
+    @property
+    def ndet(self):
+        return self._ndet
+
+    @property
+    def n(self):
+        """Vector length of this basis (the Davidson driver's basis size)."""
+        return self._ndet
+
+    @property
+    def nsector(self):
+        return self._nsector
+
+    @property
+    def nblock(self):
+        return self._nblock
+
+    @property
+    def norb_total(self):
+        return sum(self.norb)
+
+    def core_info(self):
+        self._live()
+        start = tuple(int(x) for x in numpy.cumsum((0,) + self.norb)[:-1])
+        return {"ngas": len(self.norb), "norb": self.norb, "start": start,
+                "norb_total": self.norb_total, "nelec": self.nelec, "ndet": self.ndet,
+                "nsector": self.nsector, "nblock": self.nblock}
+
+    def compress_links(self):
+        """The device link tables serve sigma, diagonal and RDMs alike;
+        kept for interface parity (gas_space_compress_links)."""
+        self._compressed = True
+
+    def links_are_compressed(self):
+        return self._compressed
+
+    def addr2str(self, sector, addr):
+        s = ctypes.c_uint64()
+        _lib.check(self.lib.fcis_gas_addr2str(self._live(), int(sector), int(addr),
+                                              ctypes.byref(s)), "fcis_gas_addr2str")
+        return int(s.value)
+
+    def str2addr(self, sector, string):
+        a = ctypes.c_int64()
+        _lib.check(self.lib.fcis_gas_str2addr(self._live(), int(sector), int(string),
+                                              ctypes.byref(a)), "fcis_gas_str2addr")
+        return int(a.value)
+
+    def sector_occupations(self):
+        occ = numpy.zeros((self.nsector, len(self.norb)), dtype=numpy.int32)
+        nstr = numpy.zeros(self.nsector, dtype=numpy.int64)
+        _lib.check(self.lib.fcis_gas_sector_info(
+            self._live(), occ.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+            nstr.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))), "fcis_gas_sector_info")
+        return occ, nstr
+
+    def block_descriptors(self):
+        if self._blocks_desc is None:
+            n = self.nblock
+            off, na, nb = (numpy.zeros(n, dtype=numpy.int64) for _ in range(3))
+            sa, sb = numpy.zeros(n, dtype=numpy.int32), numpy.zeros(n, dtype=numpy.int32)
+            ip, lp = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int64)
+            _lib.check(self.lib.fcis_gas_block_info(
+                self._live(), off.ctypes.data_as(lp), sa.ctypes.data_as(ip),
+                sb.ctypes.data_as(ip), na.ctypes.data_as(lp), nb.ctypes.data_as(lp)),
+                "fcis_gas_block_info")
+            self._blocks_desc = [dict(offset=int(a), sa=int(b), sb=int(c), na=int(d), nb=int(e))
+                                 for a, b, c, d, e in zip(off, sa, sb, na, nb)]
+        for d in self._blocks_desc:
+            yield dict(d)
+
+    def block_ndet(self, block_id):
+        d = list(self.block_descriptors())[int(block_id)]
+        return d["na"] * d["nb"]
+
+    # -- device transforms ----------------------------------------------
+    def embed_dev(self, gas_vec, cas_out, nvec=1):
+        """cas_out[v] <- gas2fci(gas_vec[v]) (device float64, contiguous)."""
+        _lib.check(self.lib.fcis_gas_embed(self._live(), _lib.vptr(gas_vec), _lib.vptr(cas_out),
+                                           int(nvec), _lib.stream_ptr()), "fcis_gas_embed")
+
+    def project_dev(self, cas_vec, gas_out, nvec=1):
+        """gas_out[v] <- fci2gas(cas_vec[v])."""
+        _lib.check(self.lib.fcis_gas_project(self._live(), _lib.vptr(cas_vec),
+                                             _lib.vptr(gas_out), int(nvec), _lib.stream_ptr()),
+                   "fcis_gas_project")
+
+    def hop(self, plan):
+        """x (GAS) -> P H P x through a CAS sigma plan on ``self.cas``."""
+        import torch
+        a = torch.empty(self.cas.ndet, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+
+        def apply(x, out):
+            self.embed_dev(x, a)
+            plan(a, b)
+            self.project_dev(b, out)
+        return apply
+
+    def close(self):
+        if getattr(self, "handle", None) is not None:
+            self.lib.fcis_gas_space_free(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_GAS_SPACES = {}
+
+
+def _cached_space(norb, nelec, blocks):
+    key = (tuple(norb), tuple(nelec), numpy.asarray(blocks, dtype=numpy.int32).tobytes())
+    g = _GAS_SPACES.get(key)
+    if g is None:
+        g = _GAS_SPACES[key] = GasSpace(norb, nelec, blocks)
+    return g
+
+
+def _vector_in(x, n):
+    """(device tensor of n float64, was-host flag, original shape)."""
+    import torch
+    if isinstance(x, torch.Tensor):
+        t = x.detach().to(device="cuda", dtype=torch.float64).contiguous().reshape(-1)
+        host, shape = False, tuple(x.shape)
+    else:
+        a = numpy.ascontiguousarray(x, dtype=numpy.float64)
+        shape = a.shape
+        t = torch.from_numpy(a.reshape(-1)).to("cuda")
+        host = True
+    if t.numel() != n:
+        raise ValueError("CI vector size does not match GAS determinant count")
+    return t, host, shape
+
+
+def gas2fci(fcivec, gas):
+    """GAS vector -> CAS (nstr_alpha, nstr_beta) array, zeros outside the
+    GAS space (fci_gas.py:141-182).  numpy in, numpy out; device tensors
+    stay on the device."""
+    import torch
+    c, host, _ = _vector_in(fcivec, gas.ndet)
+    out = torch.empty(gas.cas.shape, dtype=torch.float64, device="cuda")
+    gas.embed_dev(c, out)
+    return out.cpu().numpy() if host else out
+
+
+def fci2gas(fcivec, gas):
+    """CAS array -> GAS vector in canonical block order (fci_gas.py:185-208)."""
+    import torch
+    if isinstance(fcivec, torch.Tensor):
+        c, host = fcivec.detach().to(device="cuda", dtype=torch.float64).contiguous(), False
+    else:
+        c, host = torch.from_numpy(numpy.ascontiguousarray(fcivec, dtype=numpy.float64)).cuda(), True
+    if c.numel() != gas.cas.ndet:
+        raise ValueError("CI vector size does not match full FCI determinant count")
+    out = torch.empty(gas.ndet, dtype=torch.float64, device="cuda")
+    gas.project_dev(c.reshape(-1), out)
+    return out.cpu().numpy() if host else out
+
+
+class GasContractPlan:
+    """Reusable restricted sigma plan (GasContractPlan, fci_gas.py:570-650),
+    through fcis_gas_sigma_plan_*.  The space must outlive the plan."""
+
+    def __init__(self, gas, eri, precision="fp64", workspace_bytes=0):
+        if gas.handle is None:
+            raise RuntimeError("GAS space is closed")
+        self.lib = gas.lib
+        self.gas = gas
+        self.eri = _pair(eri, gas.norb_total)
+        prec = _lib.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        h = ctypes.c_void_p()
+        _lib.check(self.lib.fcis_gas_sigma_plan_create(
+            ctypes.byref(h), gas.handle, _lib.dptr(self.eri), prec,
+            int(workspace_bytes) or _ds._default_workspace()), "fcis_gas_sigma_plan_create")
+        self._plan = h
+
+    @property
+    def ndet(self):
+        return self.gas.ndet
+
+    @property
+    def norb(self):
+        return self.gas.norb_total
+
+    @property
+    def nelec(self):
+        return self.gas.nelec
+
+    def execute(self, c, out):
+        """out <- P H P c for device tensors."""
+        if self._plan is None or self.gas is None:
+            raise RuntimeError("Hamiltonian contraction plan is closed")
+        if c.data_ptr() == out.data_ptr():
+            raise ValueError("sigma input and output must not alias")
+        _lib.check(self.lib.fcis_gas_sigma_plan_execute(self._plan, _lib.vptr(c),
+                                                        _lib.vptr(out), _lib.stream_ptr()),
+                   "fcis_gas_sigma_plan_execute")
+        return out
+
+    def contract(self, fcivec):
+        """P H P c; same shape and residence (host/device) as the input."""
+        import torch
+        if self._plan is None or self.gas is None:
+            raise RuntimeError("Hamiltonian contraction plan is closed")
+        c, host, shape = _vector_in(fcivec, self.ndet)
+        out = torch.empty_like(c)
+        self.execute(c, out)
+        return out.cpu().numpy().reshape(shape) if host else out.reshape(shape)
+
+    def close(self):
+        if getattr(self, "_plan", None) is not None:
+            self.lib.fcis_gas_sigma_plan_free(self._plan)
+            self._plan = None
+        self.gas = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GasRDMPlan:
+    """RDMs of GAS vectors (GasRDMPlan, fci_gas.py:653-757)."""
+
+    def __init__(self, solver, norb, nelec, gas=None):
+        self.lib = _lib.load()
+        self._owned = gas is None
+        self.gas = solver.make_space(norb, nelec) if gas is None else gas
+
+    @property
+    def ndet(self):
+        return self.gas.ndet
+
+    @property
+    def norb(self):
+        return self.gas.norb_total
+
+    def _rdm(self, bra, ket, with_dm2):
+        if self.gas is None or self.gas.handle is None:
+            raise RuntimeError("RDM plan is closed")
+        b, _, _ = _vector_in(bra, self.ndet)
+        k, _, _ = _vector_in(ket, self.ndet)
+        n = self.norb
+        d1a, d1b = numpy.zeros((n, n)), numpy.zeros((n, n))
+        d2 = [numpy.zeros((n,) * 4) for _ in range(3)] if with_dm2 else [None] * 3
+        _lib.check(self.lib.fcis_gas_trans_rdm12s(
+            self.gas.handle, _lib.vptr(b), _lib.vptr(k), _lib.dptr(d1a), _lib.dptr(d1b),
+            *[None if x is None else _lib.dptr(x) for x in d2], _lib.stream_ptr()),
+            "fcis_gas_trans_rdm12s")
+        return (d1a, d1b), tuple(d2)
+
+    def make_rdm1s(self, cibra, ciket):
+        return self._rdm(cibra, ciket, False)[0]
+
+    def make_rdm1(self, cibra, ciket):
+        a, b = self.make_rdm1s(cibra, ciket)
+        return a + b
+
+    def make_rdm12s(self, cibra, ciket):
+        return self._rdm(cibra, ciket, True)
+
+    def make_rdm12(self, cibra, ciket):
+        (a, b), (aa, ab, bb) = self.make_rdm12s(cibra, ciket)
+        return a + b, aa + bb + ab + ab.transpose(2, 3, 0, 1)
+
+    def close(self):
+        """Release the plan (and its space when the plan created it)."""
+        if getattr(self, "gas", None) is not None:
+            if self._owned:
+                self.gas.close()
+            self.gas = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class FCISolver(_ds.FCISolver):
+    """Determinant-space GAS solver (fci_gas.FCISolver, fci_gas.py:760-1483)
+    with every O(ndet) step on the GPU.  Restrictions come from
+    ``gas_orbs``, ``gas_restr`` and ``gas_restr_type`` (the four forms of
+    addons_gas.normalize_gas_spec); CI vectors are 1-D in GAS block order.
+    ``precision`` selects the Davidson sigma arithmetic as in the CAS
+    solver (FP64 refinement for "fp32" / "bf16x3")."""
+
+    def __init__(self, mol=None, gas_orbs=None, gas_restr=None,
+                 gas_restr_type=addons_gas.GAS_RESTR_SPIN_SUPERGROUP, lib=None):
+        super().__init__(mol)
+        self.gas_orbs = tuple(gas_orbs) if gas_orbs is not None else None
+        self.gas_restr = gas_restr
+        self.gas_restr_type = gas_restr_type
+        self.lib = lib or _lib.load()
+        self.e_spin_penalty = None
+        self.e_physical = None
+        self.spin_penalty_method = None
+
+    # -- spaces ----------------------------------------------------------
+    def _space_spec(self, norb, nelec):
+        nelec = _ds._unpack_nelec(nelec, self.spin)
+        if self.gas_orbs is None:
+            orbs, D = addons_gas.normalize_gas_spec((int(norb),), nelec)
+        else:
+            if sum(self.gas_orbs) != int(norb):
+                raise ValueError("sum(gas_orbs) must equal norb")
+            orbs, D = addons_gas.normalize_gas_spec(self.gas_orbs, nelec, self.gas_restr,
+                                                    self.gas_restr_type)
+        return orbs, nelec, D
+
+    def space_info(self, norb, nelec):
+        orbs, nelec, D = self._space_spec(norb, nelec)
+        return addons_gas.check_kernel_limits(orbs, nelec, D)
+
+    def make_space(self, norb, nelec, compress_links=False):
+        """An owned GasSpace for this solver's restriction."""
+        orbs, nelec, D = self._space_spec(norb, nelec)
+        return GasSpace(orbs, nelec, D, lib=self.lib, compress_links=compress_links)
+
+    def _space(self, norb, nelec):
+        orbs, nelec, D = self._space_spec(norb, nelec)
+        return _cached_space(orbs, nelec, D)
+
+    def make_rdm_plan(self, norb, nelec):
+        return GasRDMPlan(self, norb, nelec)
+
+    def make_contract_plan(self, eri, norb, nelec, precision="fp64"):
+        return GasContractPlan(self._space(norb, nelec), eri, precision)
+
+    def _check_plan(self, plan, norb, nelec, kind, cls):
+        if not isinstance(plan, cls):
+            raise TypeError("plan must be a %s" % cls.__name__)
+        if plan.gas is None or plan.gas.handle is None:
+            raise RuntimeError(kind + " plan is closed")
+        orbs, nelec, D = self._space_spec(norb, nelec)
+        if (tuple(orbs) != plan.gas.norb or tuple(nelec) != tuple(plan.gas.nelec)
+                or not numpy.array_equal(D, plan.gas.blocks)):
+            raise ValueError(kind + " plan does not match the GAS space")
+
+    # -- pyscf protocol ----------------------------------------------------
+    def get_init_guess(self, norb, nelec, nroots, hdiag, gas=None):
+        """Lowest-diagonal unit vectors plus global noise, GAS layout
+        (fci_gas.py:881-902)."""
+        n = (self.space_info(norb, nelec)["ndet_estimate"] if gas is None else gas.ndet)
+        hd = numpy.asarray(hdiag).reshape(-1) if not hasattr(hdiag, "device") else hdiag
+        if hd.shape[0] != n:
+            raise ValueError("hdiag size does not match GAS determinant count")
+        if not 1 <= int(nroots) <= n:
+            raise ValueError("nroots must be between 1 and the GAS dimension")
+        return [g.cpu().numpy() for g in self._init_guess_dev(hd, int(nroots))]
+
+    def make_hdiag(self, h1e, eri, norb, nelec, *args, **kwargs):
+        return self._hdiag_dev(h1e, eri, norb, nelec).cpu().numpy()
+
+    def _hdiag_dev(self, h1e, eri, norb, nelec, gas=None):
+        import torch
+        gas = gas or self._space(norb, nelec)
+        n = gas.norb_total
+        h1e = numpy.ascontiguousarray(h1e, dtype=numpy.float64).reshape(n, n)
+        eri = _pair(eri, n)
+        out = torch.empty(gas.ndet, dtype=torch.float64, device="cuda")
+        _lib.check(self.lib.fcis_gas_make_hdiag(gas.handle, _lib.dptr(h1e), _lib.dptr(eri),
+                                                _lib.vptr(out), _lib.stream_ptr()),
+                   "fcis_gas_make_hdiag")
+        return out
+
+    def contract_2e(self, eri, fcivec, norb, nelec, link_index=None, *args, **kwargs):
+        """P H P c for the absorbed pair matrix ``eri``; with ``plan=`` (a
+        GasContractPlan) the plan's Hamiltonian is used and ``eri`` ignored."""
+        plan = kwargs.pop("plan", None)
+        if plan is not None:
+            self._check_plan(plan, norb, nelec, "contraction", GasContractPlan)
+            return plan.contract(fcivec)
+        with GasContractPlan(self._space(norb, nelec), eri,
+                             kwargs.pop("precision", "fp64")) as p:
+            return p.contract(fcivec)
+
+    def contract_1e(self, f1e, fcivec, norb, nelec, link_index=None, **kwargs):
+        raise NotImplementedError(
+            "GASCI uses the absorbed one-electron Hamiltonian and contract_2e")
+
+    def energy(self, h1e, eri, fcivec, norb, nelec, link_index=None):
+        import torch
+        h2e = absorb_h1e(h1e, eri, norb, _ds._unpack_nelec(nelec, self.spin), 0.5)
+        gas = self._space(norb, nelec)
+        c, _, _ = _vector_in(fcivec, gas.ndet)
+        with GasContractPlan(gas, h2e) as p:
+            hc = p.execute(c, torch.empty_like(c))
+        return float(torch.dot(c, hc))
+
+    def pspace(self, h1e, eri, norb, nelec, hdiag=None, np=400):
+        """(addresses, H block) over the lowest-diagonal determinants,
+        at most GAS_PSPACE_MATVEC_MAX of them (fci_gas.py:955-988)."""
+        import torch
+        if int(np) < 1:
+            raise ValueError("np must be positive")
+        gas = self._space(norb, nelec)
+        nelec = _ds._unpack_nelec(nelec, self.spin)
+        hd = self._hdiag_dev(h1e, eri, norb, nelec, gas) if hdiag is None else \
+            torch.as_tensor(numpy.asarray(hdiag, dtype=numpy.float64).reshape(-1)).cuda()
+        if hd.numel() != gas.ndet:
+            raise ValueError("hdiag size does not match GAS determinant count")
+        m = min(int(np), gas.ndet, GAS_PSPACE_MATVEC_MAX)
+        addr = torch.topk(hd, m, largest=False).indices
+        h2e = absorb_h1e(h1e, eri, norb, nelec, 0.5)
+        return addr.cpu().numpy(), self._h_block(gas, h2e, addr)
+
+    def _h_block(self, gas, h2e, addr):
+        import torch
+        x = torch.zeros(gas.ndet, dtype=torch.float64, device="cuda")
+        hx = torch.empty_like(x)
+        m = addr.numel()
+        H = numpy.empty((m, m))
+        with GasContractPlan(gas, h2e) as p:
+            for j, a in enumerate(addr.tolist()):
+                x.zero_()
+                x[a] = 1.0
+                p.execute(x, hx)
+                H[:, j] = hx[addr].cpu().numpy()
+        return 0.5 * (H + H.T)
+
+    def gen_linkstr(self, norb, nelec, tril=True, spin=None):
+        raise NotImplementedError(
+            "GAS link tables are owned by GasSpace and are not CAS linkstr arrays")
+
+    def large_ci(self, fcivec, norb, nelec, tol=0.1, return_strs=True):
+        """(coefficient, alpha string, beta string) of the large GAS
+        coefficients (fci_gas.py:1093-1128)."""
+        gas = self._space(norb, nelec)
+        ci = numpy.asarray(fcivec.cpu() if hasattr(fcivec, "cpu") else fcivec,
+                           dtype=numpy.float64).reshape(-1)
+        if ci.size != gas.ndet:
+            raise ValueError("CI vector size does not match GAS determinant count")
+        sel = numpy.flatnonzero(numpy.abs(ci) > float(tol))
+        if sel.size == 0:
+            sel = numpy.asarray([int(numpy.argmax(numpy.abs(ci)))])
+        blocks = list(gas.block_descriptors())
+        offs = numpy.asarray([b["offset"] for b in blocks])
+        out = []
+        for a in sel.tolist():
+            d = blocks[int(numpy.searchsorted(offs, a, side="right")) - 1]
+            ia, ib = divmod(a - d["offset"], d["nb"])
+            sa, sb = gas.addr2str(d["sa"], ia), gas.addr2str(d["sb"], ib)
+            if return_strs:
+                out.append((ci[a], bin(sa), bin(sb)))
+            else:
+                occ = lambda s: numpy.asarray([p for p in range(int(norb)) if (s >> p) & 1],
+                                              dtype=numpy.int32)
+                out.append((ci[a], occ(sa), occ(sb)))
+        return out
+
+    def kernel(self, h1e, eri, norb, nelec, ci0=None, ecore=0, nroots=None, **kwargs):
+        """(e + ecore, ci) of the restricted problem (fci_gas.py:1150-1399):
+        exact small-space diagonalisation up to GAS_PSPACE_MATVEC_MAX
+        determinants, otherwise the device Davidson on P H P."""
+        import torch
+        nroots = int(nroots if nroots is not None else kwargs.pop("nroots", self.nroots))
+        if nroots < 1:
+            raise ValueError("nroots must be positive")
+        if getattr(self, "ss_penalty", None) is not None:
+            raise NotImplementedError("the spin penalty (fix_spin_) is not available on the "
+                                      "device GAS solver")
+        if kwargs.get("orbsym") is not None or kwargs.get("wfnsym") is not None:
+            raise NotImplementedError(
+                "orbital and wavefunction symmetry filtering is not implemented")
+
+        def option(name, default):
+            v = kwargs.pop(name, None)
+            return default if v is None else v
+
+        tol = option("tol", self.conv_tol)
+        tol_residual = option("tol_residual", self.conv_tol_residual)
+        max_cycle = option("max_cycle", self.max_cycle)
+        max_space = option("max_space", self.max_space)
+        lindep = option("lindep", self.lindep)
+        level_shift = option("level_shift", self.level_shift)
+        davidson_only = bool(option("davidson_only", self.davidson_only))
+        pspace_size = int(option("pspace_size", self.pspace_size))
+        precision = option("precision", self.precision)
+        verbose = option("verbose", self.verbose)
+        if pspace_size < 0:
+            raise ValueError("pspace_size must be nonnegative")
+        nelec = _ds._unpack_nelec(nelec, self.spin)
+        self.norb, self.nelec = int(norb), nelec
+        gas = self._space(norb, nelec)
+        if nroots > gas.ndet:
+            raise ValueError("nroots exceeds GAS determinant count")
+        n = gas.norb_total
+        h1e = numpy.ascontiguousarray(h1e, dtype=numpy.float64).reshape(n, n)
+        eri = _pair(eri, n)
+        h2e = absorb_h1e(h1e, eri, norb, nelec, 0.5)
+        hdiag = self._hdiag_dev(h1e, eri, norb, nelec, gas)
+
+        if (ci0 is None and not davidson_only and gas.ndet <= pspace_size
+                and gas.ndet <= GAS_PSPACE_MATVEC_MAX):
+            addr = torch.arange(gas.ndet, device="cuda")
+            w, v = numpy.linalg.eigh(self._h_block(gas, h2e, addr))
+            e = w[:nroots]
+            c = [torch.from_numpy(numpy.ascontiguousarray(v[:, k])).cuda() for k in range(nroots)]
+            conv = numpy.ones(nroots, dtype=bool)
+        else:
+            if ci0 is not None:
+                ci0 = self._guess_list(ci0, nroots, gas.ndet)
+            from .mixed import solve
+            e, c, conv, stats = solve(self, gas.cas, h2e, hdiag, nroots, ci0, precision,
+                                      tol=tol, tol_residual=tol_residual,
+                                      max_cycle=max_cycle, max_space=max_space,
+                                      lindep=lindep, level_shift=level_shift,
+                                      verbose=verbose, basis=gas)
+            self.stats = stats
+        e = numpy.asarray(e, dtype=numpy.float64).reshape(-1) + float(ecore)
+        cis = [x.reshape(-1).cpu().numpy() for x in c]
+        if nroots == 1:
+            self.converged = bool(numpy.asarray(conv).reshape(-1)[0])
+            self.eci = float(e[0])
+            self.ci = cis[0]
+        else:
+            self.converged = numpy.asarray(conv, dtype=bool)[:nroots]
+            self.eci = e[:nroots]
+            self.ci = cis[:nroots]
+        return self.eci, self.ci
+
+    @staticmethod
+    def _guess_list(ci0, nroots, ndet):
+        """Normalised initial vectors from a list, an (nroots, ndet) or
+        (ndet, nroots) array, or one vector (fci_gas.py:1275-1305)."""
+        if isinstance(ci0, (list, tuple)):
+            vecs = [numpy.asarray(x, dtype=numpy.float64).reshape(-1) for x in ci0]
+        else:
+            arr = numpy.asarray(ci0, dtype=numpy.float64)
+            if arr.ndim == 2 and arr.shape[0] == nroots and arr.shape[1] == ndet:
+                vecs = list(arr)
+            elif arr.ndim == 2 and arr.shape[1] == nroots and arr.shape[0] == ndet:
+                vecs = list(arr.T)
+            else:
+                vecs = [arr.reshape(-1)]
+        out = []
+        for v in vecs:
+            if v.size != ndet:
+                raise ValueError("initial CI vector size does not match GAS determinant count")
+            nrm = numpy.linalg.norm(v)
+            if nrm == 0:
+                raise ValueError("initial CI vector must be nonzero")
+            out.append(v / nrm)
+        return out
+
+    # -- densities -----------------------------------------------------------
+    def _rdm_plan(self, norb, nelec, plan):
+        if plan is None:
+            return GasRDMPlan(self, norb, nelec, gas=self._space(norb, nelec))
+        self._check_plan(plan, norb, nelec, "RDM", GasRDMPlan)
+        return plan
+
+    def make_rdm1s(self, ci, norb, nelec, link_index=None, *, plan=None):
+        return self._rdm_plan(norb, nelec, plan).make_rdm1s(ci, ci)
+
+    def make_rdm1(self, ci, norb, nelec, link_index=None, *, plan=None):
+        return self._rdm_plan(norb, nelec, plan).make_rdm1(ci, ci)
+
+    def make_rdm12s(self, ci, norb, nelec, link_index=None, reorder=True, *, plan=None):
+        if not reorder:
+            raise NotImplementedError("reorder=False is not supported")
+        return self._rdm_plan(norb, nelec, plan).make_rdm12s(ci, ci)
+
+    def make_rdm12(self, ci, norb, nelec, link_index=None, reorder=True, *, plan=None):
+        if not reorder:
+            raise NotImplementedError("reorder=False is not supported")
+        return self._rdm_plan(norb, nelec, plan).make_rdm12(ci, ci)
+
+    def make_rdm2(self, ci, norb, nelec, link_index=None, reorder=True, *, plan=None):
+        return self.make_rdm12(ci, norb, nelec, link_index, reorder, plan=plan)[1]
+
+    def make_rdm123(self, *args, **kwargs):
+        raise NotImplementedError("third- and fourth-order density matrices are not "
+                                  "implemented for the GAS solver")
+
+    make_rdm123s = make_rdm1234 = make_rdm1234s = make_rdm123
+
+    def trans_rdm1s(self, cibra, ciket, norb, nelec, link_index=None, *, plan=None):
+        return self._rdm_plan(norb, nelec, plan).make_rdm1s(cibra, ciket)
+
+    def trans_rdm1(self, cibra, ciket, norb, nelec, link_index=None, *, plan=None):
+        return self._rdm_plan(norb, nelec, plan).make_rdm1(cibra, ciket)
+
+    def trans_rdm12s(self, cibra, ciket, norb, nelec, link_index=None, reorder=True, *,
+                     plan=None):
+        if not reorder:
+            raise NotImplementedError("reorder=False is not supported")
+        p = self._rdm_plan(norb, nelec, plan)
+        dm1s, (aa, ab, bb) = p.make_rdm12s(cibra, ciket)
+        _, (_, ba_ji, _) = p.make_rdm12s(ciket, cibra)
+        return dm1s, (aa, ab, ba_ji.transpose(3, 2, 1, 0), bb)
+
+    def trans_rdm12(self, cibra, ciket, norb, nelec, link_index=None, reorder=True, *,
+                    plan=None):
+        if not reorder:
+            raise NotImplementedError("reorder=False is not supported")
+        return self._rdm_plan(norb, nelec, plan).make_rdm12(cibra, ciket)
+
+    def spin_square(self, ci, norb, nelec, *args, **kwargs):
+        nelec = _ds._unpack_nelec(nelec, self.spin)
+        dm1s, dm2s = self.make_rdm12s(ci, norb, nelec)
+        return _ds.spin_square_from_rdm12s(dm1s, dm2s, nelec)
+
+    def to_gpu(self, *args, **kwargs):
+        """Already on the GPU."""
+        return self
+
+
+spin_square_from_rdm12s = _ds.spin_square_from_rdm12s

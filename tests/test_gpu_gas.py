@@ -1,0 +1,218 @@
+"""GPU parity of the restricted-space (GAS) path against the reference
+libfci_gas fixtures (tests/golden/gas/, gen_gas_golden.py) and the oracle.
+
+Bars: layout (sector ids, string counts, block offsets, sector strings,
+addresses) bit-exact; FP64 sigma within 1e-13 of max|sigma| (the fused CAS
+kernel accumulates its beta part with atomics); diagonal 1e-12; RDMs 1e-12;
+energies 1e-9 (Davidson tolerance 1e-10 on the energy change)."""
+
+import glob
+import json
+import os
+
+import numpy
+import pytest
+
+import oracle
+from oracle import gas as og
+from conftest import GOLDEN, has_gpu, package
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not has_gpu(), reason="needs a HIP device")]
+
+pkg = package()
+fg = pkg.fci_gas
+
+CASES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "gas", "*.npz")))
+
+
+def load(name):
+    with numpy.load(os.path.join(GOLDEN, "gas", name + ".npz")) as f:
+        d = {k: f[k] for k in f.files}
+    d["gas_orbs"] = tuple(int(x) for x in d["gas_orbs"])
+    d["nelec"] = tuple(int(x) for x in d["nelec"])
+    d["restr_type"] = str(d["restr_type"])
+    d["restr"] = json.loads(str(d["restr"]))
+    return d
+
+
+def solver_for(d):
+    return fg.FCISolver(gas_orbs=d["gas_orbs"], gas_restr=d["restr"],
+                        gas_restr_type=d["restr_type"])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_device_layout_matches_libfci_gas(name):
+    d = load(name)
+    print("case", name, flush=True)
+    with fg.GasSpace(d["gas_orbs"], d["nelec"], d["blocks"]) as gas:
+        assert gas.ndet == d["ci"].size and gas.nsector == len(d["sector_nstr"])
+        desc = list(gas.block_descriptors())
+        for k in ("offset", "sa", "sb", "na", "nb"):
+            assert [b[k] for b in desc] == d["block_" + k].tolist(), k
+        occ, nstr = gas.sector_occupations()
+        numpy.testing.assert_array_equal(occ, d["sector_occ"])
+        numpy.testing.assert_array_equal(nstr, d["sector_nstr"])
+        strings = [gas.addr2str(s, a) for s in range(gas.nsector) for a in range(int(nstr[s]))]
+        numpy.testing.assert_array_equal(numpy.asarray(strings), d["sector_strings"])
+        pos = 0
+        for s in range(gas.nsector):
+            for a in range(int(nstr[s])):
+                assert gas.str2addr(s, strings[pos]) == a
+                pos += 1
+        with pytest.raises(RuntimeError):
+            gas.str2addr(0, (1 << gas.norb_total) - 1)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_sigma_hdiag_rdm_match_libfci_gas(name):
+    d = load(name)
+    print("case", name, flush=True)
+    sol = solver_for(d)
+    n = sum(d["gas_orbs"])
+    sig = sol.contract_2e(d["h2e"], d["ci"], n, d["nelec"])
+    scale = max(1.0, numpy.abs(d["sigma"]).max())
+    assert numpy.abs(sig - d["sigma"]).max() < 1e-13 * scale
+    numpy.testing.assert_allclose(sol.make_hdiag(d["h1e"], d["eri"], n, d["nelec"]), d["hdiag"],
+                                  rtol=0, atol=1e-12)
+    (a, b), (aa, ab, bb) = sol.make_rdm_plan(n, d["nelec"]).make_rdm12s(d["ci"], d["ket"])
+    for got, key in zip((a, b, aa, ab, bb), ("dm1a", "dm1b", "dm2aa", "dm2ab", "dm2bb")):
+        numpy.testing.assert_allclose(got, d[key], rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 2e-6), ("bf16x3", 2e-4)])
+def test_low_precision_sigma(precision, tol):
+    d = load("gas_ras_n10")
+    sol = solver_for(d)
+    sig = sol.contract_2e(d["h2e"], d["ci"], 10, d["nelec"], precision=precision)
+    err = numpy.abs(sig - d["sigma"]).max() / numpy.abs(d["sigma"]).max()
+    assert 1e-13 < err < tol
+
+
+def test_embed_project_and_plan_reuse():
+    import torch
+    d = load("gas_cum4_n10")
+    sol = solver_for(d)
+    L = og.GasLayout(d["gas_orbs"], d["nelec"], d["blocks"])
+    gas = sol.make_space(10, d["nelec"])
+    full = fg.gas2fci(d["ci"], gas)
+    numpy.testing.assert_array_equal(full, og.gas2fci(d["ci"], L))
+    numpy.testing.assert_array_equal(fg.fci2gas(full, gas), d["ci"])
+    rng = numpy.random.default_rng(2)
+    x = rng.normal(size=L.cas_shape)
+    numpy.testing.assert_array_equal(fg.fci2gas(x, gas), og.fci2gas(x, L))
+    # device tensors stay on the device; two vectors at once
+    xs = torch.from_numpy(rng.normal(size=(2, gas.ndet))).cuda()
+    cas = torch.empty((2,) + gas.cas.shape, dtype=torch.float64, device="cuda")
+    gas.embed_dev(xs, cas, nvec=2)
+    back = torch.empty_like(xs)
+    gas.project_dev(cas, back, nvec=2)
+    assert torch.equal(back, xs)
+    # one plan, several vectors, against the fixture and the oracle
+    with sol.make_contract_plan(d["h2e"], 10, d["nelec"]) as plan:
+        s1 = sol.contract_2e(None, d["ci"], 10, d["nelec"], plan=plan)
+        s2 = plan.contract(d["ket"])
+        assert numpy.abs(s1 - d["sigma"]).max() < 1e-13 * numpy.abs(d["sigma"]).max()
+        ref2 = og.gas_contract_2e(d["h2e"], d["ket"], L)
+        assert numpy.abs(s2 - ref2).max() < 1e-13 * numpy.abs(ref2).max()
+        with pytest.raises(ValueError):
+            plan.contract(d["ci"][:-1])
+        other = fg.FCISolver(gas_orbs=(5, 5), gas_restr=[[5, 5], [6, 4]],
+                             gas_restr_type="supergroup")
+        with pytest.raises(ValueError):
+            other.contract_2e(None, d["ci"], 10, d["nelec"], plan=plan)
+    with pytest.raises(RuntimeError):
+        plan.contract(d["ci"])
+    gas.close()
+    with pytest.raises(RuntimeError):
+        gas.addr2str(0, 0)
+
+
+@pytest.mark.parametrize("name", [c for c in CASES if c.endswith(("n3", "n4"))])
+def test_small_space_exact_path(name):
+    d = load(name)
+    sol = solver_for(d)
+    n = sum(d["gas_orbs"])
+    nroots = min(4, d["ci"].size)
+    e, c = sol.kernel(d["h1e"], d["eri"], n, d["nelec"], nroots=nroots, ecore=0.25)
+    numpy.testing.assert_allclose(numpy.asarray(e) - 0.25, d["evals"][:nroots], rtol=0, atol=1e-10)
+    e0 = sol.energy(d["h1e"], d["eri"], c[0], n, d["nelec"])
+    assert abs(e0 - d["evals"][0]) < 1e-10
+
+
+def test_davidson_matches_dense_spectrum():
+    d = load("gas_sg_n8")
+    sol = solver_for(d)
+    e, c = sol.kernel(d["h1e"], d["eri"], 8, d["nelec"], nroots=4, max_cycle=200)
+    assert numpy.all(sol.converged)
+    numpy.testing.assert_allclose(e, d["evals"], rtol=0, atol=1e-9)
+    C = numpy.asarray(c)
+    numpy.testing.assert_allclose(C @ C.T, numpy.eye(4), atol=1e-8)
+    L = og.GasLayout(d["gas_orbs"], d["nelec"], d["blocks"])
+    r = og.gas_contract_2e(d["h2e"], c[0], L) - e[0] * c[0]
+    assert numpy.linalg.norm(r) < 1e-4
+    # spin complete supergroup space: eigenstates have integer S(S+1)
+    ss, mult = sol.spin_square(c[0], 8, d["nelec"])
+    assert abs(ss - round(ss)) < 1e-6 or abs(ss - 0.75) < 1e-6 or abs(ss - 3.75) < 1e-6
+    # RDM trace and energy from RDMs
+    dm1, dm2 = sol.make_rdm12(c[0], 8, d["nelec"])
+    assert abs(numpy.trace(dm1) - sum(d["nelec"])) < 1e-10
+    eri4 = oracle.unpack_pair_matrix(d["eri"], 8)
+    e_rdm = numpy.einsum("pq,pq", d["h1e"], dm1) + 0.5 * numpy.einsum("pqrs,pqrs", eri4, dm2)
+    assert abs(e_rdm - e[0]) < 1e-9
+
+
+def test_davidson_ras_n10_fp64_and_mixed():
+    d = load("gas_ras_n10")
+    L = og.GasLayout(d["gas_orbs"], d["nelec"], d["blocks"])
+    energies = {}
+    for precision in ("fp64", "bf16x3"):
+        print("precision", precision, flush=True)
+        sol = solver_for(d)
+        sol.precision = precision
+        e, c = sol.kernel(d["h1e"], d["eri"], 10, d["nelec"], nroots=2, max_cycle=300)
+        assert numpy.all(sol.converged)
+        for k in range(2):
+            r = og.gas_contract_2e(d["h2e"], c[k], L) - e[k] * c[k]
+            assert numpy.linalg.norm(r) < 1e-4, (precision, k, numpy.linalg.norm(r))
+        energies[precision] = numpy.asarray(e)
+    numpy.testing.assert_allclose(energies["bf16x3"], energies["fp64"], rtol=0, atol=1e-9)
+
+
+def test_gas_as_cas_equals_cas_solver():
+    """One subspace, one block: the GAS solver is the CAS solver
+    (test_gasci.py:1295-1313)."""
+    n, ne = 8, (4, 3)
+    h1e, eri = pkg.synthetic.make_problem(n, ne, seed=5, with_ci=False)
+    e1, c1 = fg.FCISolver().kernel(h1e, eri, n, ne, nroots=2)
+    e2, c2 = pkg.FCISolver().kernel(h1e, eri, n, ne, nroots=2)
+    numpy.testing.assert_allclose(e1, e2, rtol=0, atol=1e-9)
+    for a, b in zip(c1, c2):
+        assert abs(abs(numpy.dot(a.reshape(-1), b.reshape(-1))) - 1) < 1e-8
+
+
+def test_ras_16_orbitals_large_space():
+    """RAS(4,8,4) with (8,8) electrons, 2 holes / 2 particles: a GAS space
+    inside the (16o,16e) CAS.  Sigma against the oracle's projection of the
+    device CAS sigma on sampled determinants would repeat the path, so the
+    check is size-independent: symmetry <x|Hy> = <y|Hx>, linearity, and the
+    layout's determinant count."""
+    import torch
+    orbs, ne = (4, 8, 4), (8, 8)
+    sol = fg.FCISolver(gas_orbs=orbs, gas_restr={"max_holes": 2, "max_particles": 2},
+                       gas_restr_type="ras")
+    info = sol.space_info(16, ne)
+    h1e, eri = pkg.synthetic.make_problem(16, ne, seed=0, with_ci=False)
+    h2e = fg.absorb_h1e(h1e, eri, 16, ne)
+    with sol.make_contract_plan(h2e, 16, ne) as plan:
+        assert plan.ndet == info["ndet_estimate"]
+        g = torch.Generator(device="cuda").manual_seed(1)
+        x = torch.randn(plan.ndet, dtype=torch.float64, device="cuda", generator=g)
+        y = torch.randn(plan.ndet, dtype=torch.float64, device="cuda", generator=g)
+        hx, hy, hz = (torch.empty_like(x) for _ in range(3))
+        plan.execute(x, hx)
+        plan.execute(y, hy)
+        plan.execute(2.0 * x - y, hz)
+        a, b = float(torch.dot(x, hy)), float(torch.dot(y, hx))
+        assert abs(a - b) < 1e-10 * max(abs(a), 1.0)
+        assert float((hz - (2.0 * hx - hy)).abs().max()) < 1e-11 * float(hx.abs().max())
