@@ -331,6 +331,19 @@ int fcis_gas_sigma_plan_create(fcis_gas_sigma_plan_t **out, const fcis_gas_space
 int fcis_gas_sigma_plan_execute(fcis_gas_sigma_plan_t *plan, const double *d_ci0,
                                 double *d_ci1, void *stream);
 void fcis_gas_sigma_plan_free(fcis_gas_sigma_plan_t *plan);
+/* d_ci1 = P H P d_ci0 with a caller's sigma plan on the space's CAS space
+ * (any precision; the Davidson driver's plans), d_work a device array of
+ * 2 * nstr_a * nstr_b doubles.  With a fused plan (norb <= 16) the fused
+ * kernel runs only over the alpha strings and 16-determinant beta chunks
+ * within one excitation of the GAS space (fcis_gas_rows_info), where t1
+ * and vt1 can be nonzero; otherwise the full CAS sigma is projected.  The
+ * work of fci_contract_gas_plan_execute (fci_gas.h:245). */
+int fcis_gas_sigma_execute(const fcis_gas_space_t *gas, fcis_sigma_plan_t *cas_plan,
+                           const double *d_ci0, double *d_ci1, double *d_work, void *stream);
+/* Restricted work of the fused path: alpha rows, beta chunks over those
+ * rows, and the chunk count of the full CAS (nstr_a * ceil(nstr_b / 16)). */
+int fcis_gas_rows_info(const fcis_gas_space_t *gas, int64_t *nrows, int64_t *nchunks,
+                       int64_t *nchunks_cas);
 
 #ifdef __cplusplus
 }

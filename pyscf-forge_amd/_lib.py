@@ -91,6 +91,8 @@ SIGNATURES = {
                                                   ctypes.c_uint64]),
     "fcis_gas_sigma_plan_execute": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "fcis_gas_sigma_plan_free": (None, [_vp]),
+    "fcis_gas_sigma_execute": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "fcis_gas_rows_info": (ctypes.c_int, [_vp, _i64p, _i64p, _i64p]),
 }
 
 _LIB = None

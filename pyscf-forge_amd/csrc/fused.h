@@ -240,9 +240,30 @@ struct fcis_sigma_plan {
         int64_t launches[3];
 };
 
+// Restricted rows of the fused kernel (restricted determinant spaces): the
+// alpha strings to process, in processing order, each with its list of
+// FUSED_NB-wide beta chunks, and the determinants whose sigma is wanted.
+// Exact when every determinant reachable by one excitation from a wanted
+// determinant or from the support of c lies in a listed chunk of a listed row
+// (t1 and vt1 vanish elsewhere).
+struct fcis_rowset {
+        int64_t nrows;
+        const int32_t *d_rows;     // [nrows] alpha string of each position
+        const int2 *d_rowcl;       // [nrows] (offset into d_clist, chunk count >= 1)
+        const uint16_t *d_clist;   // chunk indices, ascending per row
+        const int32_t *d_rpos;     // [nstra] position of each alpha string, -1 if absent
+        int64_t ndets;
+        const uint32_t *d_dets;    // [ndets] wanted determinants (I * nstrb + Ib)
+};
+
 namespace fcis {
 // fcis_sigma_plan_create; honour_env = false ignores FCIS_DETERMINISTIC
 // (the batch always needs the fused tables).
 int sigma_plan_create(fcis_sigma_plan **out, const fcis_space *space, const double *h2e,
                       int precision, uint64_t workspace_bytes, bool honour_env);
+// sigma on the wanted determinants of a row set (fused plans only):
+// out[d] = (H ci0)[dets[d]]; ci0 is a full (nstra, nstrb) array, sigma_ws a
+// full-size work array.
+int sigma_plan_execute_rows(fcis_sigma_plan *p, const fcis_rowset *rs, const double *ci0,
+                            double *sigma_ws, double *out, hipStream_t stream);
 }  // namespace fcis

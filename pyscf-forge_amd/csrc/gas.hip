@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "common.h"
+#include "fused.h"
 
 #define GAS_MAXNGAS 15
 
@@ -44,6 +45,18 @@ struct fcis_gas_space {
         int64_t ndet;
         int64_t ncas;
         uint32_t *d_map;                // [ndet] CAS determinant index
+        // Restricted rows of the fused CAS sigma (fcis_rowset): every alpha
+        // string whose sector pairs with some beta sector in S1, the GAS
+        // blocks and their single-excitation neighbours (sa', sb), (sa, sb')
+        // with sa' (sb') one inter-subspace move from sa (sb) or equal, each
+        // with the FUSED_NB-wide beta chunks holding strings of those beta
+        // sectors.  t1 and vt1 vanish outside S1 when c lives on the GAS
+        // determinants, so sigma on them is exact.
+        int32_t *d_rows, *d_rpos;
+        int2 *d_rowcl;
+        uint16_t *d_clist;
+        fcis_rowset rs;
+        int64_t nchunk_rows;            // sum of chunk counts over rows
 };
 
 struct fcis_gas_sigma_plan {
@@ -200,6 +213,104 @@ struct Scratch {
         }
 };
 
+int sector_of(const fcis_gas_space *g, const std::vector<std::vector<uint8_t>> &pool,
+              uint64_t str)
+{
+        std::vector<uint8_t> o(g->ngas);
+        for (int k = 0; k < g->ngas; k++)
+                o[k] = (uint8_t)__builtin_popcountll((str >> g->start[k]) & ((1ull << g->norb[k]) - 1));
+        auto it = std::lower_bound(pool.begin(), pool.end(), o);
+        return (it != pool.end() && *it == o) ? (int)(it - pool.begin()) : -1;
+}
+
+// Tables of fcis_rowset for the fused CAS sigma restricted to S1 (see the
+// struct); rows sorted by chunk count, longest first, for the persistent
+// grid's round-robin.
+int build_rowset(fcis_gas_space *g, const std::vector<std::vector<uint8_t>> &pool,
+                 const std::vector<uint8_t> &blk, hipStream_t st)
+{
+        const int ngas = g->ngas, ns = g->nsector;
+        const int64_t na = g->cas->nstra, nb = g->cas->nstrb;
+        const int64_t nch = (nb + FUSED_NB - 1) / FUSED_NB;
+        const int ntot = g->cas->norb;
+        // S1 sector pairs
+        std::vector<std::vector<char>> allow(ns, std::vector<char>(ns, 0));
+        auto moves = [&](int s) {
+                std::vector<int> out{s};
+                for (int r = 0; r < ngas; r++) {
+                        for (int t = 0; t < ngas; t++) {
+                                if (r == t || pool[s][r] == 0 || pool[s][t] >= g->norb[t]) continue;
+                                std::vector<uint8_t> o = pool[s];
+                                o[r]--;
+                                o[t]++;
+                                auto it = std::lower_bound(pool.begin(), pool.end(), o);
+                                if (it != pool.end() && *it == o) out.push_back((int)(it - pool.begin()));
+                        }
+                }
+                return out;
+        };
+        for (int b = 0; b < g->nblock; b++) {
+                (void)blk;
+                const int sa = g->bsa[b], sb = g->bsb[b];
+                for (int x : moves(sa)) allow[x][sb] = 1;
+                for (int y : moves(sb)) allow[sa][y] = 1;
+        }
+        // sector of every CAS beta string; chunk lists per alpha sector
+        std::vector<int> bsec(nb);
+        for (int64_t j = 0; j < nb; j++) bsec[j] = sector_of(g, pool, unrank_local(ntot, g->nb, j));
+        std::vector<int64_t> soff(ns, 0), scnt(ns, 0);
+        std::vector<uint16_t> clist;
+        std::vector<char> mark(nch);
+        for (int s = 0; s < ns; s++) {
+                std::fill(mark.begin(), mark.end(), 0);
+                for (int64_t j = 0; j < nb; j++)
+                        if (bsec[j] >= 0 && allow[s][bsec[j]]) mark[j / FUSED_NB] = 1;
+                soff[s] = (int64_t)clist.size();
+                for (int64_t c = 0; c < nch; c++)
+                        if (mark[c]) clist.push_back((uint16_t)c);
+                scnt[s] = (int64_t)clist.size() - soff[s];
+        }
+        std::vector<std::pair<int64_t, int32_t>> order;   // (-count, alpha string)
+        std::vector<int> asec(na);
+        for (int64_t i = 0; i < na; i++) {
+                asec[i] = sector_of(g, pool, unrank_local(ntot, g->na, i));
+                if (asec[i] >= 0 && scnt[asec[i]] > 0) order.emplace_back(-scnt[asec[i]], (int32_t)i);
+        }
+        std::sort(order.begin(), order.end());
+        const int64_t nrows = (int64_t)order.size();
+        std::vector<int32_t> rows(nrows), rpos(na, -1);
+        std::vector<int2> rowcl(nrows);
+        int64_t tot = 0;
+        for (int64_t r = 0; r < nrows; r++) {
+                const int32_t I = order[r].second;
+                rows[r] = I;
+                rpos[I] = (int32_t)r;
+                rowcl[r] = make_int2((int)soff[asec[I]], (int)scnt[asec[I]]);
+                tot += scnt[asec[I]];
+        }
+        g->nchunk_rows = tot;
+        if (hipMalloc(&g->d_rows, 4 * (size_t)nrows + 4) != hipSuccess ||
+            hipMalloc(&g->d_rpos, 4 * (size_t)na + 4) != hipSuccess ||
+            hipMalloc(&g->d_rowcl, 8 * (size_t)nrows + 8) != hipSuccess ||
+            hipMalloc(&g->d_clist, 2 * clist.size() + 2) != hipSuccess) {
+                fcis::set_error("fcis_gas_space_create: device allocation failed");
+                return FCIS_ERR_MEMORY;
+        }
+        FCIS_HIP(hipMemcpyAsync(g->d_rows, rows.data(), 4 * (size_t)nrows, hipMemcpyHostToDevice, st));
+        FCIS_HIP(hipMemcpyAsync(g->d_rpos, rpos.data(), 4 * (size_t)na, hipMemcpyHostToDevice, st));
+        FCIS_HIP(hipMemcpyAsync(g->d_rowcl, rowcl.data(), 8 * (size_t)nrows, hipMemcpyHostToDevice, st));
+        FCIS_HIP(hipMemcpyAsync(g->d_clist, clist.data(), 2 * clist.size(), hipMemcpyHostToDevice, st));
+        FCIS_HIP(hipStreamSynchronize(st));
+        g->rs.nrows = nrows;
+        g->rs.d_rows = g->d_rows;
+        g->rs.d_rowcl = g->d_rowcl;
+        g->rs.d_clist = g->d_clist;
+        g->rs.d_rpos = g->d_rpos;
+        g->rs.ndets = g->ndet;
+        g->rs.d_dets = g->d_map;
+        return FCIS_SUCCESS;
+}
+
 }  // namespace
 
 extern "C" {
@@ -208,6 +319,10 @@ void fcis_gas_space_free(fcis_gas_space_t *g)
 {
         if (g == nullptr) return;
         if (g->d_map) (void)hipFree(g->d_map);
+        if (g->d_rows) (void)hipFree(g->d_rows);
+        if (g->d_rpos) (void)hipFree(g->d_rpos);
+        if (g->d_rowcl) (void)hipFree(g->d_rowcl);
+        if (g->d_clist) (void)hipFree(g->d_clist);
         delete g;
 }
 
@@ -228,6 +343,9 @@ int fcis_gas_space_create(fcis_gas_space_t **out, const fcis_space_t *cas, int n
         g->na = cas->na;
         g->nb = cas->nb;
         g->d_map = nullptr;
+        g->d_rows = g->d_rpos = nullptr;
+        g->d_rowcl = nullptr;
+        g->d_clist = nullptr;
         int tot = 0;
         for (int k = 0; k < ngas; k++) {
                 if (norb[k] < 1 || norb[k] > 31) {
@@ -386,6 +504,7 @@ int fcis_gas_space_create(fcis_gas_space_t **out, const fcis_space_t *cas, int n
         }
         if (d_tab) (void)hipFree(d_tab);
         if (d_sec) (void)hipFree(d_sec);
+        if (st_ok == FCIS_SUCCESS) st_ok = build_rowset(g, pool, rows, st);
         if (st_ok != FCIS_SUCCESS) {
                 fcis_gas_space_free(g);
                 return st_ok;
@@ -565,6 +684,38 @@ int fcis_gas_sigma_plan_create(fcis_gas_sigma_plan_t **out, const fcis_gas_space
         return FCIS_SUCCESS;
 }
 
+int fcis_gas_sigma_execute(const fcis_gas_space_t *g, fcis_sigma_plan_t *cas_plan,
+                           const double *d_ci0, double *d_ci1, double *d_work, void *stream)
+{
+        if (g == nullptr || cas_plan == nullptr || d_ci0 == nullptr || d_ci1 == nullptr ||
+            d_work == nullptr || d_ci0 == d_ci1 || cas_plan->space != g->cas) {
+                fcis::set_error("fcis_gas_sigma_execute: invalid argument (distinct device "
+                                "arrays and a plan on the space's CAS space are required)");
+                return FCIS_ERR_INVALID;
+        }
+        hipStream_t st = (hipStream_t)stream;
+        double *a = d_work, *b = d_work + g->ncas;
+        int r = embed(g, d_ci0, a, 1, st);
+        if (r != FCIS_SUCCESS) return r;
+        if (cas_plan->fused) return fcis::sigma_plan_execute_rows(cas_plan, &g->rs, a, b, d_ci1, st);
+        r = fcis_sigma_plan_execute(cas_plan, a, b, stream);
+        if (r == FCIS_SUCCESS) r = project(g, b, d_ci1, 1, st);
+        return r;
+}
+
+int fcis_gas_rows_info(const fcis_gas_space_t *g, int64_t *nrows, int64_t *nchunks,
+                       int64_t *nchunks_cas)
+{
+        if (g == nullptr) {
+                fcis::set_error("fcis_gas_rows_info: null space");
+                return FCIS_ERR_INVALID;
+        }
+        if (nrows) *nrows = g->rs.nrows;
+        if (nchunks) *nchunks = g->nchunk_rows;
+        if (nchunks_cas) *nchunks_cas = g->cas->nstra * ((g->cas->nstrb + FUSED_NB - 1) / FUSED_NB);
+        return FCIS_SUCCESS;
+}
+
 int fcis_gas_sigma_plan_execute(fcis_gas_sigma_plan_t *p, const double *d_ci0, double *d_ci1,
                                 void *stream)
 {
@@ -573,11 +724,7 @@ int fcis_gas_sigma_plan_execute(fcis_gas_sigma_plan_t *p, const double *d_ci0, d
                                 "output must be distinct device arrays)");
                 return FCIS_ERR_INVALID;
         }
-        hipStream_t st = (hipStream_t)stream;
-        int r = embed(p->gas, d_ci0, p->d_a, 1, st);
-        if (r == FCIS_SUCCESS) r = fcis_sigma_plan_execute(p->cas_plan, p->d_a, p->d_b, stream);
-        if (r == FCIS_SUCCESS) r = project(p->gas, p->d_b, d_ci1, 1, st);
-        return r;
+        return fcis_gas_sigma_execute(p->gas, p->cas_plan, d_ci0, d_ci1, p->d_a, stream);
 }
 
 }  // extern "C"

@@ -287,14 +287,19 @@ __global__ __launch_bounds__(256) void k_contract_bf16x3(int npair, int mtiles, 
 //     rows at norb = 16), and pulled by k_alpha_pull.
 // Independent accumulators per k-step phase (FP64/FP32) or per split term
 // (bf16x3) keep several MFMA chains in flight per wavefront.
-template <int PREC>
+//
+// ROWS: restricted rows (fcis_rowset).  Position a0 + r processes alpha
+// string rows[a0 + r] over its listed beta chunks only; walpha is indexed by
+// position.  Otherwise position a0 + r is alpha string a0 + r, all chunks.
+template <int PREC, bool ROWS>
 __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_fused_row(
         int npair, int mtiles, int ksteps, int nlinka, int64_t nb, int64_t nstra, int64_t a0,
         int64_t ta, int64_t ld, int accumulate, const fcis_link *__restrict__ linka,
         const int32_t *__restrict__ pmapa, const int16_t *__restrict__ pmapb, int64_t nbp,
         const double *__restrict__ c, const void *__restrict__ hfrag_v,
         const void *__restrict__ hfrag_lo_v, typename FusedTypes<PREC>::Out *__restrict__ walpha,
-        double *__restrict__ sigma)
+        double *__restrict__ sigma, const int32_t *__restrict__ rows,
+        const int2 *__restrict__ rowcl, const uint16_t *__restrict__ clist)
 {
         typedef FusedTypes<PREC> Ty;
         typedef typename Ty::B BT;
@@ -312,7 +317,6 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
         const int tid = threadIdx.x, nthr = blockDim.x;
         const int wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
         const bool mfma_wave = wave < mtiles;
-        const int kp = ksteps * KW;
         const int64_t nch = (nb + NB - 1) / NB;
 
         for (int e = npair * NB + tid; e < KPMAX * NB; e += nthr) {
@@ -328,7 +332,15 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                 const int bt = tid - mtiles * 64;
                 FusedBuild<Ty::NBLD> st;
                 for (int64_t r = blockIdx.x; r < ta; r += gridDim.x) {
-                        const int64_t I = a0 + r;
+                        const int64_t I = ROWS ? (int64_t)rows[a0 + r] : a0 + r;
+                        int64_t nl = nch;
+                        const uint16_t *cl = nullptr;
+                        if constexpr (ROWS) {
+                                const int2 e = rowcl[a0 + r];
+                                cl = clist + e.x;
+                                nl = e.y;
+                        }
+                        auto chunk = [&](int64_t jj) -> int64_t { return ROWS ? (int64_t)cl[jj] : jj; };
                         __syncthreads();   // (1)
                         for (int q = bt; q < npair; q += Ty::NBLD * 64) {
                                 arow[q] = pmapa[(int64_t)q * nstra + I];
@@ -337,25 +349,26 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                         __syncthreads();   // (2)
                         for (int k = bt; k < nlinka; k += Ty::NBLD * 64)
                                 aslot[link_pair(linka[(int64_t)k * nstra + I].info)] = k;
-                        fb_issue_maps(st, npair, nbp, 0, bt, pmapb);
+                        fb_issue_maps(st, npair, nbp, chunk(0), bt, pmapb);
                         fb_commit_maps(st, npair, bt, maps);
-                        if (nch > 1) {
-                                fb_issue_maps(st, npair, nbp, 1, bt, pmapb);
+                        if (nl > 1) {
+                                fb_issue_maps(st, npair, nbp, chunk(1), bt, pmapb);
                                 fb_commit_maps(st, npair, bt, maps + MAXP * NB);
                         }
                         __syncthreads();   // (3)
-                        fb_issue_c(st, npair, nb, I, 0, bt, arow, maps, c);
+                        fb_issue_c(st, npair, nb, I, chunk(0), bt, arow, maps, c);
                         fb_commit_c(st, npair, bt, buf0);
                         __syncthreads();   // (4)
-                        for (int64_t j = 0; j < nch; j++) {
+                        for (int64_t j = 0; j < nl; j++) {
                                 // both round trips in flight together: c for chunk j+1,
                                 // pair maps for chunk j+2
-                                if (j + 1 < nch)
-                                        fb_issue_c(st, npair, nb, I, j + 1, bt, arow,
+                                if (j + 1 < nl)
+                                        fb_issue_c(st, npair, nb, I, chunk(j + 1), bt, arow,
                                                    maps + ((j + 1) % 3) * MAXP * NB, c);
-                                if (j + 2 < nch) fb_issue_maps(st, npair, nbp, j + 2, bt, pmapb);
-                                if (j + 1 < nch) fb_commit_c(st, npair, bt, ((j + 1) & 1) ? buf1 : buf0);
-                                if (j + 2 < nch)
+                                if (j + 2 < nl)
+                                        fb_issue_maps(st, npair, nbp, chunk(j + 2), bt, pmapb);
+                                if (j + 1 < nl) fb_commit_c(st, npair, bt, ((j + 1) & 1) ? buf1 : buf0);
+                                if (j + 2 < nl)
                                         fb_commit_maps(st, npair, bt, maps + ((j + 2) % 3) * MAXP * NB);
                                 __syncthreads();
                         }
@@ -381,16 +394,23 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                         al[ks] = ks < ksteps ? ((const typename Ty::A *)hfrag_lo_v)[o] : typename Ty::A{};
         }
         for (int64_t r = blockIdx.x; r < ta; r += gridDim.x) {
-                const int64_t I = a0 + r;
+                const int64_t I = ROWS ? (int64_t)rows[a0 + r] : a0 + r;
+                int64_t nl = nch;
+                const uint16_t *cl = nullptr;
+                if constexpr (ROWS) {
+                        const int2 e = rowcl[a0 + r];
+                        cl = clist + e.x;
+                        nl = e.y;
+                }
                 __syncthreads();   // (1)
                 __syncthreads();   // (2)
                 __syncthreads();   // (3)
                 __syncthreads();   // (4)
-                for (int64_t j = 0; j < nch; j++) {
+                for (int64_t j = 0; j < nl; j++) {
                         const BT *bb = (j & 1) ? buf1 : buf0;
                         const int16_t *mb = maps + (j % 3) * MAXP * NB;
                         const typename Ty::Acc acc = mma_steps<PREC, KS, KS>(ah, al, bb, lk, li);
-                        const int64_t b = j * NB + li;
+                        const int64_t b = (ROWS ? (int64_t)cl[j] : j) * NB + li;
                         if (b < nb) {
 #pragma unroll
                                 for (int q = 0; q < 4; q++) {
@@ -576,6 +596,126 @@ int upload_h(fcis_sigma_plan *p, const double *h2e)
 
 }  // namespace
 
+namespace {
+
+// One launch of the fused row kernel over positions [a0, a0 + ta) (alpha
+// strings, or the row set's positions when ROWS), walpha in p->d_vt1.
+template <bool ROWS>
+void launch_fused(const fcis_sigma_plan *p, hipStream_t st, int64_t a0, int64_t ta, int64_t lda,
+                  int acc, const double *ci0, double *sig, const fcis_rowset *rs)
+{
+        const fcis_space *s = p->space;
+        const int64_t nb = s->nstrb, na = s->nstra;
+        const dim3 fgrid(p->fused_grid), fblk(64 * (p->mtiles + p->nbld));
+        const int32_t *rows = ROWS ? rs->d_rows : nullptr;
+        const int2 *rowcl = ROWS ? rs->d_rowcl : nullptr;
+        const uint16_t *clist = ROWS ? rs->d_clist : nullptr;
+        if (p->precision == FCIS_PREC_FP64)
+                hipLaunchKernelGGL((k_fused_row<FCIS_PREC_FP64, ROWS>), fgrid, fblk, p->fused_lds, st,
+                                   p->npair, p->mtiles, p->ksteps, s->nlinka, nb, na, a0, ta, lda, acc,
+                                   s->d_linka, p->d_pmapa, p->d_pmapb16, p->nbp, ci0, p->d_hfrag,
+                                   p->d_hfrag_lo, (double *)p->d_vt1, sig, rows, rowcl, clist);
+        else if (p->precision == FCIS_PREC_FP32)
+                hipLaunchKernelGGL((k_fused_row<FCIS_PREC_FP32, ROWS>), fgrid, fblk, p->fused_lds, st,
+                                   p->npair, p->mtiles, p->ksteps, s->nlinka, nb, na, a0, ta, lda, acc,
+                                   s->d_linka, p->d_pmapa, p->d_pmapb16, p->nbp, ci0, p->d_hfrag,
+                                   p->d_hfrag_lo, (float *)p->d_vt1, sig, rows, rowcl, clist);
+        else
+                hipLaunchKernelGGL((k_fused_row<FCIS_PREC_BF16X3, ROWS>), fgrid, fblk, p->fused_lds, st,
+                                   p->npair, p->mtiles, p->ksteps, s->nlinka, nb, na, a0, ta, lda, acc,
+                                   s->d_linka, p->d_pmapa, p->d_pmapb16, p->nbp, ci0, p->d_hfrag,
+                                   p->d_hfrag_lo, (float *)p->d_vt1, sig, rows, rowcl, clist);
+}
+
+// Alpha pull onto the wanted determinants of a row set, for the positions
+// [r0, r0 + tr) of one launch: one thread per wanted determinant m =
+// dets[d] = (J, Jb) adds sum_k s_k walpha[rslot_k][rpos[I_k] - r0][Jb] over
+// the alpha links k of J whose source I_k has its position in the range.
+// The last launch (out != nullptr) writes out[d] = sigma[m] + that sum.
+template <typename T>
+__global__ __launch_bounds__(256) void k_alpha_pull_rows(
+        int64_t ndets, const uint32_t *__restrict__ dets, int64_t nb, int64_t nstra, int64_t r0,
+        int64_t tr, int64_t ld, int nlinka, const fcis_link *__restrict__ linka,
+        const int32_t *__restrict__ rpos, const T *__restrict__ walpha, double *__restrict__ sigma,
+        double *__restrict__ out)
+{
+        for (int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; d < ndets;
+             d += (int64_t)gridDim.x * blockDim.x) {
+                const int64_t m = dets[d];
+                const int64_t J = m / nb, Jb = m - J * nb;
+                double acc = 0.0;
+                for (int k = 0; k < nlinka; k++) {
+                        const fcis_link L = linka[(int64_t)k * nstra + J];
+                        const int64_t pos = rpos[L.addr];
+                        if (pos < r0 || pos >= r0 + tr) continue;
+                        const double v = (double)walpha[(int64_t)link_rslot(L.info) * ld +
+                                                        (pos - r0) * nb + Jb];
+                        acc += link_neg(L.info) ? -v : v;
+                }
+                if (out)
+                        out[d] = sigma[m] + acc;
+                else
+                        sigma[m] += acc;
+        }
+}
+
+}  // namespace
+
+int fcis::sigma_plan_execute_rows(fcis_sigma_plan *p, const fcis_rowset *rs, const double *ci0,
+                                  double *sig, double *out, hipStream_t st)
+{
+        if (p == nullptr || rs == nullptr || ci0 == nullptr || sig == nullptr || out == nullptr ||
+            ci0 == sig || !p->fused) {
+                fcis::set_error("sigma_plan_execute_rows: invalid argument or unfused plan");
+                return FCIS_ERR_INVALID;
+        }
+        const fcis_space *s = p->space;
+        const int64_t nb = s->nstrb, na = s->nstra;
+        if (rs->nrows == 0 || rs->ndets == 0) {
+                if (rs->ndets > 0)
+                        hipLaunchKernelGGL(k_zero, dim3((rs->ndets + 255) / 256), dim3(256), 0, st,
+                                           rs->ndets, out);
+                FCIS_CHECK_LAUNCH();
+                return FCIS_SUCCESS;
+        }
+        const int64_t tile = p->tile_a;
+        const int ntiles = (int)((rs->nrows + tile - 1) / tile);
+        const int acc = ntiles > 1 ? 1 : 0;
+        if (acc)
+                hipLaunchKernelGGL(k_zero, dim3((na * nb + 255) / 256), dim3(256), 0, st, na * nb, sig);
+        const int64_t pgrid = std::min<int64_t>((rs->ndets + 255) / 256, 16384);
+        for (int t = 0; t < ntiles; t++) {
+                const int64_t r0 = (int64_t)t * tile;
+                const int64_t tr = std::min<int64_t>(tile, rs->nrows - r0);
+                const int64_t lda = tr * nb;
+                double *o = t == ntiles - 1 ? out : nullptr;
+                if (p->timing) FCIS_HIP(hipEventRecord(p->ev[0], st));
+                if (p->timing) FCIS_HIP(hipEventRecord(p->ev[1], st));
+                launch_fused<true>(p, st, r0, tr, lda, acc, ci0, sig, rs);
+                if (p->timing) FCIS_HIP(hipEventRecord(p->ev[2], st));
+                if (p->precision == FCIS_PREC_FP64)
+                        hipLaunchKernelGGL(k_alpha_pull_rows<double>, dim3(pgrid), dim3(256), 0, st,
+                                           rs->ndets, rs->d_dets, nb, na, r0, tr, lda, s->nlinka,
+                                           s->d_linka, rs->d_rpos, (const double *)p->d_vt1, sig, o);
+                else
+                        hipLaunchKernelGGL(k_alpha_pull_rows<float>, dim3(pgrid), dim3(256), 0, st,
+                                           rs->ndets, rs->d_dets, nb, na, r0, tr, lda, s->nlinka,
+                                           s->d_linka, rs->d_rpos, (const float *)p->d_vt1, sig, o);
+                FCIS_CHECK_LAUNCH();
+                if (p->timing) {
+                        FCIS_HIP(hipEventRecord(p->ev[3], st));
+                        FCIS_HIP(hipEventSynchronize(p->ev[3]));
+                        for (int k = 0; k < 3; k++) {
+                                float ms = 0.0f;
+                                FCIS_HIP(hipEventElapsedTime(&ms, p->ev[k], p->ev[k + 1]));
+                                p->ms[k] += ms;
+                                p->launches[k] += 1;
+                        }
+                }
+        }
+        return FCIS_SUCCESS;
+}
+
 int fcis::sigma_plan_create(fcis_sigma_plan **out, const fcis_space *space, const double *h2e,
                             int precision, uint64_t workspace_bytes, bool honour_env)
 {
@@ -616,10 +756,14 @@ int fcis::sigma_plan_create(fcis_sigma_plan **out, const fcis_space *space, cons
                 int dev = 0, ncu = 0, occ = 0;
                 FCIS_HIP(hipGetDevice(&dev));
                 FCIS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-                const void *fk = precision == FCIS_PREC_FP64 ? (const void *)k_fused_row<FCIS_PREC_FP64>
-                               : precision == FCIS_PREC_FP32 ? (const void *)k_fused_row<FCIS_PREC_FP32>
-                                                              : (const void *)k_fused_row<FCIS_PREC_BF16X3>;
+                const void *fk = precision == FCIS_PREC_FP64 ? (const void *)k_fused_row<FCIS_PREC_FP64, false>
+                               : precision == FCIS_PREC_FP32 ? (const void *)k_fused_row<FCIS_PREC_FP32, false>
+                                                              : (const void *)k_fused_row<FCIS_PREC_BF16X3, false>;
+                const void *fkr = precision == FCIS_PREC_FP64 ? (const void *)k_fused_row<FCIS_PREC_FP64, true>
+                                : precision == FCIS_PREC_FP32 ? (const void *)k_fused_row<FCIS_PREC_FP32, true>
+                                                               : (const void *)k_fused_row<FCIS_PREC_BF16X3, true>;
                 FCIS_HIP(hipFuncSetAttribute(fk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                FCIS_HIP(hipFuncSetAttribute(fkr, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
                 p->nbld = precision == FCIS_PREC_FP64 ? FusedTypes<FCIS_PREC_FP64>::NBLD
                         : precision == FCIS_PREC_FP32 ? FusedTypes<FCIS_PREC_FP32>::NBLD
                                                        : FusedTypes<FCIS_PREC_BF16X3>::NBLD;
@@ -739,41 +883,23 @@ int fcis_sigma_plan_execute(fcis_sigma_plan_t *p, const double *ci0, double *ci1
                 const dim3 rgrid(ta * ((nb + 255) / 256)), sgrid(na * ((nb + 255) / 256));
                 if (p->timing) FCIS_HIP(hipEventRecord(p->ev[0], st));
                 if (p->fused) {
-                        const dim3 fgrid(p->fused_grid), fblk(64 * (p->mtiles + p->nbld));
-                        const dim3 pgrid(na * ((nb + 255) / 256));
                         const int acc = p->ntiles > 1 ? 1 : 0;
                         const int64_t lda = ta * nb;   // walpha row stride
                         if (t == 0 && acc)
                                 hipLaunchKernelGGL(k_zero, dim3((na * nb + 255) / 256), dim3(256), 0, st,
                                                    na * nb, ci1);
                         if (p->timing) FCIS_HIP(hipEventRecord(p->ev[1], st));
-                        if (p->precision == FCIS_PREC_FP64) {
-                                double *wa = (double *)p->d_vt1;
-                                hipLaunchKernelGGL(k_fused_row<FCIS_PREC_FP64>, fgrid, fblk, p->fused_lds, st,
-                                                   p->npair, p->mtiles, p->ksteps, s->nlinka, nb, na, a0,
-                                                   ta, lda, acc, s->d_linka, p->d_pmapa, p->d_pmapb16,
-                                                   p->nbp, ci0, p->d_hfrag, p->d_hfrag_lo, wa, ci1);
-                                if (p->timing) FCIS_HIP(hipEventRecord(p->ev[2], st));
+                        launch_fused<false>(p, st, a0, ta, lda, acc, ci0, ci1, nullptr);
+                        if (p->timing) FCIS_HIP(hipEventRecord(p->ev[2], st));
+                        const dim3 pgrid(na * ((nb + 255) / 256));
+                        if (p->precision == FCIS_PREC_FP64)
                                 hipLaunchKernelGGL(k_alpha_pull<double>, pgrid, dim3(256), 0, st, nb, na, a0,
-                                                   ta, lda, s->nlinka, s->d_linka, wa, ci1);
-                        } else {
-                                float *wa = (float *)p->d_vt1;
-                                if (p->precision == FCIS_PREC_FP32)
-                                        hipLaunchKernelGGL(k_fused_row<FCIS_PREC_FP32>, fgrid, fblk,
-                                                           p->fused_lds, st, p->npair, p->mtiles, p->ksteps,
-                                                           s->nlinka, nb, na, a0, ta, lda, acc, s->d_linka,
-                                                           p->d_pmapa, p->d_pmapb16, p->nbp, ci0, p->d_hfrag,
-                                                           p->d_hfrag_lo, wa, ci1);
-                                else
-                                        hipLaunchKernelGGL(k_fused_row<FCIS_PREC_BF16X3>, fgrid, fblk,
-                                                           p->fused_lds, st, p->npair, p->mtiles, p->ksteps,
-                                                           s->nlinka, nb, na, a0, ta, lda, acc, s->d_linka,
-                                                           p->d_pmapa, p->d_pmapb16, p->nbp, ci0, p->d_hfrag,
-                                                           p->d_hfrag_lo, wa, ci1);
-                                if (p->timing) FCIS_HIP(hipEventRecord(p->ev[2], st));
+                                                   ta, lda, s->nlinka, s->d_linka,
+                                                   (const double *)p->d_vt1, ci1);
+                        else
                                 hipLaunchKernelGGL(k_alpha_pull<float>, pgrid, dim3(256), 0, st, nb, na, a0,
-                                                   ta, lda, s->nlinka, s->d_linka, wa, ci1);
-                        }
+                                                   ta, lda, s->nlinka, s->d_linka,
+                                                   (const float *)p->d_vt1, ci1);
                 } else if (p->precision == FCIS_PREC_FP64) {
                         double *t1 = (double *)p->d_t1, *vt1 = (double *)p->d_vt1;
                         hipLaunchKernelGGL(k_gather<double>, rgrid, dim3(256), 0, st, p->npair, nb, a0,

@@ -161,17 +161,32 @@ class GasSpace:
                                              _lib.vptr(gas_out), int(nvec), _lib.stream_ptr()),
                    "fcis_gas_project")
 
+    def sigma_dev(self, plan, x, out, work):
+        """out <- P H P x with a CAS SigmaPlan on ``self.cas`` (any
+        precision); work: 2 * cas.ndet float64 device scratch."""
+        if x.data_ptr() == out.data_ptr():
+            raise ValueError("sigma input and output must not alias")
+        _lib.check(self.lib.fcis_gas_sigma_execute(self._live(), plan.handle, _lib.vptr(x),
+                                                   _lib.vptr(out), _lib.vptr(work),
+                                                   _lib.stream_ptr()), "fcis_gas_sigma_execute")
+        return out
+
     def hop(self, plan):
         """x (GAS) -> P H P x through a CAS sigma plan on ``self.cas``."""
         import torch
-        a = torch.empty(self.cas.ndet, dtype=torch.float64, device="cuda")
-        b = torch.empty_like(a)
+        work = torch.empty(2 * self.cas.ndet, dtype=torch.float64, device="cuda")
 
         def apply(x, out):
-            self.embed_dev(x, a)
-            plan(a, b)
-            self.project_dev(b, out)
+            self.sigma_dev(plan, x, out, work)
         return apply
+
+    def rows_info(self):
+        """(alpha rows, beta chunks, CAS chunks) of the restricted fused
+        sigma: the fraction of the CAS sigma's work it does."""
+        a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(self.lib.fcis_gas_rows_info(self._live(), ctypes.byref(a), ctypes.byref(b),
+                                               ctypes.byref(c)), "fcis_gas_rows_info")
+        return a.value, b.value, c.value
 
     def close(self):
         if getattr(self, "handle", None) is not None:

@@ -14,7 +14,7 @@ front end against it.  Each .npz holds:
   inputs h1e, eri (un-absorbed pair matrix), h2e (absorbed, fac=0.5),
   ci, ket (GAS vectors) and the reference outputs
   sigma = H ci, hdiag, dm1a, dm1b, dm2aa, dm2ab, dm2bb = trans_rdm12s(ci, ket),
-  and, for small spaces, evals: the 4 lowest eigenvalues of the dense
+  and, up to 1500 determinants, evals: the 4 lowest eigenvalues of the dense
   restricted H assembled from libfci_gas sigma columns.
 The integrals of the four-orbital cases follow make_integrals of the
 reference's tests (test_gasci.py:41-58); the others are synthetic.
@@ -101,7 +101,7 @@ def main():
         hdiag = R.make_hdiag(h1e, eri)
         (dm1a, dm1b), (dm2aa, dm2ab, dm2bb) = R.trans_rdm12s(ci, ket)
         extra = {}
-        if R.ndet <= 600:
+        if R.ndet <= 1500:
             H = numpy.empty((R.ndet, R.ndet))
             e = numpy.zeros(R.ndet)
             for j in range(R.ndet):
