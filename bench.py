@@ -14,10 +14,13 @@ mixed-precision Davidson times to the ground state with their energy
 difference, the roofline of the dominant kernel (the contraction, timed with
 HIP events on its own stream inside this run), a CPU baseline, the
 batched workload: 64 uneven CAS problems (norb 6..12, 4 roots each) solved
-together with one batched sigma launch per Davidson iteration, and the
-singlet ground state in the CSF basis (csf2det, sigma, det2csf per step).
+together with one batched sigma launch per Davidson iteration, the
+singlet ground state in the CSF basis (csf2det, sigma, det2csf per step),
+and a restricted RAS(4,8,4) space inside the same CAS (restricted sigma in
+three precisions, the reference libfci_gas sigma on it, Davidson).
 
-    python bench.py [--steps K] [--warmup W] [--no-davidson] [--no-cpu] [--no-batch] [--no-csf]
+    python bench.py [--steps K] [--warmup W] [--no-davidson] [--no-cpu] [--no-batch]
+                    [--no-csf] [--no-gas]
 """
 
 import argparse
@@ -211,6 +214,90 @@ def csf_solve(pkg, sync, h1e, eri, n, ne, max_cycle):
     return out
 
 
+def gas_solve(pkg, sync, h1e, eri, steps, with_cpu, max_cycle, davidson=True):
+    """Restricted (GAS) CI inside the (16o,16e) CAS: RAS(4,8,4) with (8,8)
+    electrons, at most 2 holes in RAS1 and 2 electrons in RAS3 (4,870,404
+    determinants, 2.9% of the CAS).  FP64 / FP32 / bf16x3 restricted sigma
+    (fused kernel over the alpha rows and beta chunks within one excitation
+    of the space), the reference's libfci_gas sigma on the same space on the
+    host cores, and the FP64 and bf16x3-mixed Davidson ground states."""
+    import numpy
+    import torch
+    fg = pkg.fci_gas
+    orbs, ne, restr = (4, 8, 4), (8, 8), {"max_holes": 2, "max_particles": 2}
+    sol = fg.FCISolver(gas_orbs=orbs, gas_restr=restr, gas_restr_type="ras")
+    t0 = time.perf_counter()
+    gas = sol._space(16, ne)
+    sync()
+    out = {"workload": "RAS(4,8,4) (8,8) max_holes 2 max_particles 2 in CAS(16o,16e), "
+                       "synthetic integrals seed 0",
+           "ndet": gas.ndet, "space_s": time.perf_counter() - t0}
+    nrows, nch, ncas = gas.rows_info()
+    out["rows"], out["chunks"], out["chunk_frac_of_cas"] = nrows, nch, nch / ncas
+    h2e = fg.absorb_h1e(h1e, eri, 16, ne)
+    g = torch.Generator(device="cuda").manual_seed(3)
+    x = torch.randn(gas.ndet, dtype=torch.float64, device="cuda", generator=g)
+    ref = None
+    for prec in ("fp64", "fp32", "bf16x3"):
+        with fg.GasContractPlan(gas, h2e, prec) as plan:
+            y = torch.empty_like(x)
+            plan.execute(x, y)
+            sync()
+            t1 = time.perf_counter()
+            for _ in range(steps):
+                plan.execute(x, y)
+            sync()
+            dt = (time.perf_counter() - t1) / steps
+        rec = {"ms": dt * 1e3, "sigma_per_s": 1.0 / dt}
+        if prec == "fp64":
+            ref = y.clone()
+        else:
+            rec["rel_err_vs_fp64"] = float((y - ref).norm() / ref.norm())
+        out[prec] = rec
+        log("GAS sigma %s: %.2f ms" % (prec, dt * 1e3))
+    if with_cpu:
+        import oracle
+        if oracle.ref_available():
+            D = gas.blocks
+            R = oracle.RefLib().gas(orbs, ne, D)
+            P = R.plan(h2e)
+            xc = x.cpu().numpy()
+            yc = numpy.empty_like(xc)
+            P(xc, yc)
+            t1 = time.perf_counter()
+            reps = 0
+            while reps < 2 or time.perf_counter() - t1 < 8.0:
+                P(xc, yc)
+                reps += 1
+            dt = (time.perf_counter() - t1) / reps
+            P.close()
+            cores = int(os.environ.get("OMP_NUM_THREADS", 0)) or len(os.sched_getaffinity(0))
+            out["cpu_reference"] = {"sigma_s": dt, "sigma_per_s": 1.0 / dt, "cores": cores,
+                                    "runs": reps, "kind": "reference libfci_gas",
+                                    "max_abs_diff_vs_gpu_fp64":
+                                        float(numpy.abs(yc - ref.cpu().numpy()).max())}
+            log("GAS sigma libfci_gas on %d cores: %.3f s" % (cores, dt))
+    del x, ref
+    e64 = None
+    for prec in ("fp64", "bf16x3") if davidson else ():
+        s = fg.FCISolver(gas_orbs=orbs, gas_restr=restr, gas_restr_type="ras")
+        s.precision = prec
+        s.max_cycle = max_cycle
+        sync()
+        t1 = time.perf_counter()
+        e, _ = s.kernel(h1e, eri, 16, ne)
+        sync()
+        dt = time.perf_counter() - t1
+        e64 = e if prec == "fp64" else e64
+        out["davidson_" + prec] = {"seconds": dt, "energy": e, "converged": bool(s.converged),
+                                   "dE_vs_fp64": abs(e - e64),
+                                   "stats": {k: v for k, v in s.stats.items()
+                                             if isinstance(v, (int, float, str))}}
+        log("GAS Davidson %s: %.2f s, E = %.12f" % (prec, dt, e))
+        torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -225,6 +312,8 @@ def main():
                     help="skip the 64-problem batched solve (nroots=4)")
     ap.add_argument("--no-csf", action="store_true",
                     help="skip the CSF-basis (singlet) Davidson at the same size")
+    ap.add_argument("--no-gas", action="store_true",
+                    help="skip the restricted (RAS) space inside the same CAS")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 4 GiB device copy first (HBM counter calibration)")
     args = ap.parse_args()
@@ -328,6 +417,11 @@ def main():
     if not args.no_csf and not args.no_davidson:
         csf = csf_solve(pkg, sync, h1e, eri, n, ne, args.max_cycle)
 
+    gas = None
+    if not args.no_gas and n == 16:
+        gas = gas_solve(pkg, sync, h1e, eri, args.steps, not args.no_cpu,
+                        max(args.max_cycle, 400), not args.no_davidson)
+
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (reference libfci_gas on host cores)")
@@ -371,6 +465,7 @@ def main():
         "davidson": dav,
         "batch": batch,
         "csf": csf,
+        "gas": gas,
     }
     print(json.dumps(line), flush=True)
 

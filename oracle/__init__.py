@@ -381,6 +381,31 @@ class RefGas:
             raise RuntimeError("libfci_gas contract failed with status %d" % st)
         return out
 
+    def plan(self, h2e):
+        """Reusable restricted sigma plan -> callable(ci, out) with close()
+        (used for CPU timing)."""
+        lib = self.lib
+        lib.refh_gas_plan_create.restype = ctypes.c_void_p
+        lib.refh_gas_plan_create.argtypes = [ctypes.c_void_p, _dp]
+        lib.refh_gas_plan_execute.argtypes = [ctypes.c_void_p, _dp, _dp]
+        lib.refh_gas_plan_free.argtypes = [ctypes.c_void_p]
+        h2e = pair_matrix(h2e, self.norb_total)
+        handle = lib.refh_gas_plan_create(self.handle, _d(h2e))
+        if not handle:
+            raise RuntimeError("libfci_gas restricted plan creation failed")
+
+        class _Plan:
+            def __call__(self, ci, out):
+                st = lib.refh_gas_plan_execute(handle, _d(ci), _d(out))
+                if st != 0:
+                    raise RuntimeError("plan execute failed %d" % st)
+                return out
+
+            def close(self):
+                lib.refh_gas_plan_free(handle)
+
+        return _Plan()
+
     def make_hdiag(self, h1e, eri):
         h1e = _c64(h1e, (self.norb_total, self.norb_total))
         eri = pair_matrix(eri, self.norb_total)

@@ -350,3 +350,64 @@ int refh_gas_rdm12s(void *handle, const double *bra, const double *ket,
         }
         return ret;
 }
+
+/* Reusable restricted sigma plan (compressed-link space + contract plan with
+ * its own h2e and gos copies), for timing the reference on a GAS space. */
+typedef struct {
+        gas_space_t gas;
+        gas_contract_plan_t *plan;
+        double *h2e, *gos;
+} refh_gas_plan_t;
+
+void refh_gas_plan_free(void *handle)
+{
+        refh_gas_plan_t *p = handle;
+        if (p == 0) {
+                return;
+        }
+        fci_contract_gas_plan_free(p->plan);
+        gas_space_free(&p->gas);
+        free(p->h2e);
+        free(p->gos);
+        free(p);
+}
+
+void *refh_gas_plan_create(void *space, const double *h2e)
+{
+        refh_gas_t *h = space;
+        size_t norb = 0, npair;
+        for (int g = 0; g < h->ngas; g++) {
+                norb += (size_t)h->norb[g];
+        }
+        npair = norb * (norb + 1) / 2;
+        refh_gas_plan_t *p = calloc(1, sizeof(*p));
+        if (p == 0) {
+                return 0;
+        }
+        p->h2e = malloc(sizeof(double) * npair * npair);
+        p->gos = malloc(sizeof(double) * npair * npair);
+        if (p->h2e == 0 || p->gos == 0 || gas_build(h, &p->gas, 1) != 0) {
+                free(p->h2e);
+                free(p->gos);
+                free(p);
+                return 0;
+        }
+        memcpy(p->h2e, h2e, sizeof(double) * npair * npair);
+        for (size_t i = 0; i < npair; i++) {
+                for (size_t j = 0; j < npair; j++) {
+                        p->gos[i * npair + j] = h2e[i * npair + j] + h2e[j * npair + i];
+                }
+        }
+        if (fci_contract_gas_plan_create(&p->plan, &p->gas, p->h2e, p->gos) != 0) {
+                p->plan = 0;
+                refh_gas_plan_free(p);
+                return 0;
+        }
+        return p;
+}
+
+int refh_gas_plan_execute(void *handle, const double *ci0, double *ci1)
+{
+        refh_gas_plan_t *p = handle;
+        return fci_contract_gas_plan_execute(p->plan, ci0, ci1);
+}
