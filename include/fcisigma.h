@@ -149,6 +149,17 @@ int fcis_trans_rdm12s(const fcis_space_t *space, const double *d_bra,
                       double *dm2aa, double *dm2ab, double *dm2bb,
                       void *stream);
 
+/*
+ * Hamiltonian block over m selected determinants (pspace): d_h0[i][j] =
+ * <D_i|H|D_j> (device, m x m, row-major) for the CAS addresses d_addr[i] =
+ * alpha_i * nstr_b + beta_i (device int64), by the Slater-Condon rules from
+ * the HOST un-absorbed h1e and (npair, npair) eri.  m <= 46340.  Replaces
+ * the sigma-per-column small-space builder (fci_gas.py:1130-1148) and the
+ * determinant pspace under csf.pspace (csf.py:246-366).
+ */
+int fcis_pspace(const fcis_space_t *space, const double *h1e, const double *eri, int m,
+                const int64_t *d_addr, double *d_h0, void *stream);
+
 /* ---- Configuration state functions (spin-adapted basis of the CSF-FCI
  *      solver, pyscf/csf_fci) ----------------------------------------- */
 
@@ -314,6 +325,11 @@ int fcis_gas_embed(const fcis_gas_space_t *gas, const double *d_gas, double *d_c
                    void *stream);
 int fcis_gas_project(const fcis_gas_space_t *gas, const double *d_cas, double *d_gas,
                      int nvec, void *stream);
+/* d_cas_addr[i] = CAS determinant index (alpha * nstr_b + beta) of GAS
+ * determinant d_addr[i] (device int64 arrays; -1 for an address outside
+ * the space): GAS pspaces through fcis_pspace. */
+int fcis_gas_det_index(const fcis_gas_space_t *gas, int64_t n, const int64_t *d_addr,
+                       int64_t *d_cas_addr, void *stream);
 /* Diagonal of H in GAS order (fci_make_hdiag_gas, fci_gas.h:205). */
 int fcis_gas_make_hdiag(const fcis_gas_space_t *gas, const double *h1e, const double *eri,
                         double *d_hdiag, void *stream);

@@ -49,6 +49,7 @@ SIGNATURES = {
     "fcis_sigma_plan_get_timing": (ctypes.c_int, [_vp, _dp, _i64p]),
     "fcis_make_hdiag": (ctypes.c_int, [_vp, _dp, _dp, _vp, _vp]),
     "fcis_trans_rdm12s": (ctypes.c_int, [_vp, _vp, _vp, _dp, _dp, _dp, _dp, _dp, _vp]),
+    "fcis_pspace": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int, _vp, _vp, _vp]),
     "fcis_csf_create": (ctypes.c_int, [ctypes.POINTER(_vp), _vp, ctypes.c_int, _vp]),
     "fcis_csf_free": (None, [_vp]),
     "fcis_csf_info": (ctypes.c_int, [_vp, _i64p, _i64p]),
@@ -93,6 +94,7 @@ SIGNATURES = {
     "fcis_gas_sigma_plan_free": (None, [_vp]),
     "fcis_gas_sigma_execute": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "fcis_gas_rows_info": (ctypes.c_int, [_vp, _i64p, _i64p, _i64p]),
+    "fcis_gas_det_index": (ctypes.c_int, [_vp, _i64, _vp, _vp, _vp]),
 }
 
 _LIB = None

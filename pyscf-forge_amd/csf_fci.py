@@ -278,8 +278,9 @@ class FCISolver(_ds.FCISolver):
         return [t.vec_csf2det(g).cpu().numpy() for g in self._init_guess_dev(hd, nroots)]
 
     def pspace(self, h1e, eri, norb, nelec, hdiag_det=None, hdiag_csf=None, npsp=200, **kwargs):
-        """(csf_addr, h0): H in the npsp CSFs of lowest diagonal, built from
-        H applied to those unit CSF vectors (csf.pspace, csf.py:246-366)."""
+        """(csf_addr, h0): H in the npsp CSFs of lowest diagonal (csf.pspace,
+        csf.py:246-366): the determinants of their configurations, the
+        Slater-Condon block over those on the GPU, transformed to the CSFs."""
         import torch
         self.norb, self.nelec = int(norb), self._nelec(nelec)
         t = self.check_transformer_cache(smult=kwargs.get("smult"))
@@ -290,24 +291,59 @@ class FCISolver(_ds.FCISolver):
             hdiag_csf = make_hdiag_csf(h1e, eri, norb, self.nelec, t, hdiag_det=hdiag_det)
         elif not isinstance(hdiag_csf, torch.Tensor):
             hdiag_csf = torch.as_tensor(numpy.asarray(hdiag_csf, dtype=numpy.float64)).cuda()
-        npsp = min(int(npsp), t.ncsf)
-        addr = torch.sort(torch.topk(hdiag_csf, npsp, largest=False).indices).values
-        h2e = _ds.absorb_h1e(h1e, eri, norb, self.nelec, 0.5)
-        return addr.cpu().numpy(), self._h_block(sp, t, h2e, addr)
+        addr = self._pspace_addr(t, hdiag_csf, npsp)
+        return addr.cpu().numpy(), self._h_block(sp, t, h1e, eri, addr)
 
-    def _h_block(self, sp, t, h2e, addr):
+    # determinants of the pspace configurations: at most this many (the
+    # reference bounds the same block by max_memory, csf.py:287-300)
+    PSPACE_MAX_DET = 20000
+
+    def _pspace_addr(self, t, hdiag_csf, npsp):
+        """Lowest-diagonal CSF addresses (ascending address order), trimmed
+        from the top of the diagonal until their configurations span at most
+        PSPACE_MAX_DET determinants."""
+        import torch
+        npsp = min(int(npsp), t.ncsf)
+        order = torch.topk(hdiag_csf, npsp, largest=False).indices
+        cnt = self._support_sizes(t, order)
+        keep = int(numpy.searchsorted(numpy.cumsum(cnt), self.PSPACE_MAX_DET, side="right"))
+        keep = max(1, keep)
+        return torch.sort(order[:keep]).values
+
+    @staticmethod
+    def _support(t, a, d, x):
+        """Determinant support of CSF a: (det addresses, coefficients)."""
+        import torch
+        x.zero_()
+        x[a] = 1.0
+        t.csf2det_dev(x, d)
+        nz = torch.nonzero(d).reshape(-1)
+        return nz, d[nz].clone()
+
+    def _support_sizes(self, t, addr):
+        import torch
+        d = torch.empty(t.ndet, dtype=torch.float64, device="cuda")
+        x = torch.zeros(t.ncsf, dtype=torch.float64, device="cuda")
+        return [int(self._support(t, a, d, x)[0].numel()) for a in addr.tolist()]
+
+    def _h_block(self, sp, t, h1e, eri, addr):
+        """H over the CSFs ``addr``: U^T H_det U with H_det the Slater-Condon
+        block over the union of their determinant supports (fcis_pspace)."""
         import torch
         m = addr.numel()
+        d = torch.empty(t.ndet, dtype=torch.float64, device="cuda")
         x = torch.zeros(t.ncsf, dtype=torch.float64, device="cuda")
-        hx = torch.empty_like(x)
-        H = numpy.empty((m, m))
-        with _ds.SigmaPlan(sp, h2e, "fp64", _ds._default_workspace()) as plan:
-            hop = t.hop(plan)
-            for j, a in enumerate(addr.tolist()):
-                x.zero_()
-                x[a] = 1.0
-                hop(x, hx)
-                H[:, j] = hx[addr].cpu().numpy()
+        sup = [self._support(t, a, d, x) for a in addr.tolist()]
+        del d
+        dets = torch.unique(torch.cat([s[0] for s in sup]))
+        if dets.numel() > max(self.PSPACE_MAX_DET, 1):
+            raise ValueError("pspace of %d CSFs spans %d determinants (limit %d)"
+                             % (m, dets.numel(), self.PSPACE_MAX_DET))
+        U = torch.zeros((dets.numel(), m), dtype=torch.float64, device="cuda")
+        for k, (nz, cf) in enumerate(sup):
+            U[torch.searchsorted(dets, nz), k] = cf
+        h0 = _ds._pspace_dev(sp, h1e, eri, dets)
+        H = (U.T @ (h0 @ U)).cpu().numpy()
         return 0.5 * (H + H.T)
 
     def kernel(self, h1e, eri, norb, nelec, ci0=None, ecore=0, nroots=None, smult=None,
@@ -345,12 +381,21 @@ class FCISolver(_ds.FCISolver):
         hdiag_det = _ds._make_hdiag_dev(h1e, eri, norb, nelec)
         hdiag = make_hdiag_csf(h1e, eri, norb, nelec, t, hdiag_det=hdiag_det)
 
+        # pspace of the lowest-diagonal CSFs (csf.py:414-420): the whole space
+        # when it is small enough, else the block of the preconditioner
+        npsp = max(pspace_size, nroots)
+        if pspace_size >= t.ncsf:
+            paddr = torch.arange(t.ncsf, device="cuda")
+        elif pspace_size > 0:
+            paddr = self._pspace_addr(t, hdiag, npsp)
+        else:
+            paddr = None
+        pw = pv = None
+        if paddr is not None:
+            pw, pv = numpy.linalg.eigh(self._h_block(sp, t, h1e, eri, paddr))
         if ci0 is None and not davidson_only and pspace_size >= t.ncsf:
-            addr = torch.arange(t.ncsf, device="cuda")
-            H = self._h_block(sp, t, h2e, addr)
-            w, v = numpy.linalg.eigh(H)
-            e = w[:nroots]
-            c = [torch.from_numpy(numpy.ascontiguousarray(v[:, k])).cuda() for k in range(nroots)]
+            e = pw[:nroots]
+            c = [torch.from_numpy(numpy.ascontiguousarray(pv[:, k])).cuda() for k in range(nroots)]
             conv = numpy.ones(nroots, dtype=bool)
         else:
             if ci0 is not None:
@@ -359,11 +404,14 @@ class FCISolver(_ds.FCISolver):
                 ci0 = [t.vec_det2csf(torch.as_tensor(numpy.asarray(v, dtype=numpy.float64))
                                      .cuda().reshape(-1)) for v in vecs]
             from .mixed import solve
+            from .davidson import make_pspace_precond
+            precond = None if pw is None else make_pspace_precond(hdiag, pw, pv, paddr,
+                                                                   level_shift)
             e, c, conv, stats = solve(self, sp, h2e, hdiag, nroots, ci0, precision,
                                       tol=tol, tol_residual=tol_residual,
                                       max_cycle=max_cycle, max_space=max_space,
                                       lindep=lindep, level_shift=level_shift,
-                                      verbose=verbose, basis=t)
+                                      verbose=verbose, basis=t, precond=precond)
             self.stats = stats
         e = numpy.asarray(e, dtype=numpy.float64).reshape(-1) + float(ecore)
         cis = [t.vec_csf2det(x.reshape(-1), normalize=False).reshape(sp.shape) for x in c]

@@ -166,6 +166,16 @@ __global__ void k_gas_gather(const uint32_t *__restrict__ map, const double *__r
         }
 }
 
+__global__ void k_gas_det_index(const uint32_t *__restrict__ map, int64_t ndet, int64_t n,
+                                const int64_t *__restrict__ in, int64_t *__restrict__ out)
+{
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+             i += (int64_t)gridDim.x * blockDim.x) {
+                const int64_t a = in[i];
+                out[i] = (a >= 0 && a < ndet) ? (int64_t)map[a] : -1;
+        }
+}
+
 unsigned grid_for(int64_t n)
 {
         int64_t g = (n + GAS_THREADS - 1) / GAS_THREADS;
@@ -590,6 +600,20 @@ int fcis_gas_str2addr(const fcis_gas_space_t *g, int sector, uint64_t str, int64
                 return FCIS_ERR_INVALID;
         }
         *addr = a;
+        return FCIS_SUCCESS;
+}
+
+int fcis_gas_det_index(const fcis_gas_space_t *g, int64_t n, const int64_t *d_addr,
+                       int64_t *d_cas_addr, void *stream)
+{
+        if (g == nullptr || n < 0 || (n > 0 && (d_addr == nullptr || d_cas_addr == nullptr))) {
+                fcis::set_error("fcis_gas_det_index: invalid argument");
+                return FCIS_ERR_INVALID;
+        }
+        if (n == 0) return FCIS_SUCCESS;
+        hipLaunchKernelGGL(k_gas_det_index, dim3(grid_for(n)), dim3(GAS_THREADS), 0,
+                           (hipStream_t)stream, g->d_map, g->ndet, n, d_addr, d_cas_addr);
+        FCIS_CHECK_LAUNCH();
         return FCIS_SUCCESS;
 }
 

@@ -11,7 +11,8 @@ in the library's HIP kernels (block gram / gemm, preconditioner); PyTorch
 only allocates the device buffers.
 
 ``hop(x, out)`` writes H x into ``out`` (device float64 vectors of length
-``n``); ``precond(r, e, out)`` writes the preconditioned residual.
+``n``); ``precond(r, e, out, x0)`` writes the preconditioned residual of
+the Ritz pair (e, x0).
 """
 
 import math
@@ -147,7 +148,7 @@ def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
             conv[k] = abs(de[k]) < tol and rnorm[k] < tol_residual
             if not conv[k]:
                 t = ops.empty((n,), x0[0])
-                precond(r, e[k], t)
+                precond(r, e[k], t, xr[k])
                 new_vectors.append(t)
         if verbose:
             print("davidson %d  space %d  e %s  de %s  |r| %s" % (
@@ -194,3 +195,40 @@ def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
         if added == 0:
             break
     return DavidsonResult(conv.copy(), e_old, xr, axr, cycles, nhop, rnorm.copy())
+
+
+def make_pspace_precond(hdiag, pw, pv, addr, level_shift=1e-3, ops=None):
+    """Preconditioner with an exact pspace block and Olsen's correction, the
+    one the reference's CSF solver builds from its pspace eigenpairs through
+    fci.make_precond(hdiag, pw, pv, addr) (pyscf/csf_fci/csf.py:431-440;
+    J. Olsen, P. Jorgensen, J. Simons, Chem. Phys. Lett. 169, 463 (1990)).
+    With e' = e - level_shift, D = hdiag - e' (|D| < 1e-8 -> 1e-8) and
+    M^-1 = 1/D off the pspace, (H0 - e')^-1 = pv (pw - e')^-1 pv^T on it:
+        e1 = <x0|M^-1 r> / <x0|M^-1 x0>,   t = (r - e1 x0) / D.
+    hdiag: device vector; pw, pv: eigenpairs of the pspace block (host);
+    addr: pspace addresses (device int64)."""
+    import torch
+    lib = _lib.load()
+    ops = ops if ops is not None else DeviceOps()
+    n = int(hdiag.numel())
+    pw = numpy.asarray(pw, dtype=numpy.float64)
+    pv = numpy.asarray(pv, dtype=numpy.float64)
+    addr = addr.to(device=hdiag.device, dtype=torch.int64)
+    d1 = torch.empty_like(hdiag)
+
+    def diag_solve(v, e, out):
+        _lib.check(lib.fcis_precond(n, _lib.vptr(hdiag), float(e), 1e-8, _lib.vptr(v),
+                                    _lib.vptr(out), _lib.stream_ptr()), "fcis_precond")
+
+    def precond(r, e, out, x0):
+        es = float(e) - level_shift
+        diag_solve(x0, es, d1)
+        diag_solve(r, es, out)
+        g = torch.stack([x0[addr], r[addr], d1[addr], out[addr]]).cpu().numpy()
+        xa, ra, d1a, d2a = g
+        inv = (pv / (pw - es)) @ pv.T
+        x0_h0r = ops.dots(1, n, x0, out)[0] - xa @ d2a + xa @ (inv @ ra)
+        x0_h0x0 = ops.dots(1, n, x0, d1)[0] - xa @ d1a + xa @ (inv @ xa)
+        e1 = x0_h0r / x0_h0x0
+        ops.lincomb(1, n, d1, numpy.array([-e1]), 1.0, out)
+    return precond

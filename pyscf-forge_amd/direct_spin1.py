@@ -260,6 +260,37 @@ def _make_hdiag_dev(h1e, eri, norb, nelec):
     return out
 
 
+def _pspace_dev(sp, h1e, eri, addr):
+    """H block over the CAS determinants ``addr`` (device int64 tensor) by
+    the Slater-Condon rules (fcis_pspace), as a device (m, m) tensor."""
+    import torch
+    h1e = numpy.ascontiguousarray(h1e, dtype=numpy.float64).reshape(sp.norb, sp.norb)
+    eri = pair_matrix(eri, sp.norb)
+    addr = addr.to(device="cuda", dtype=torch.int64).contiguous()
+    m = addr.numel()
+    h0 = torch.empty((m, m), dtype=torch.float64, device="cuda")
+    _lib.check(sp.lib.fcis_pspace(sp.handle, _lib.dptr(h1e), _lib.dptr(eri), int(m),
+                                  _lib.vptr(addr), _lib.vptr(h0), _lib.stream_ptr()),
+               "fcis_pspace")
+    return h0
+
+
+def pspace(h1e, eri, norb, nelec, hdiag=None, np=400):
+    """(addr, h0): the ``np`` determinants of lowest diagonal energy
+    (ascending) and H over them, from the un-absorbed integrals
+    (direct_spin1.pspace; built on the GPU by the Slater-Condon rules)."""
+    import torch
+    sp = get_space(norb, nelec)
+    hd = _make_hdiag_dev(h1e, eri, norb, nelec) if hdiag is None else \
+        torch.as_tensor(numpy.asarray(hdiag, dtype=numpy.float64).reshape(-1)).cuda()
+    if hd.numel() != sp.ndet:
+        raise ValueError("hdiag size does not match the determinant count")
+    m = min(int(np), sp.ndet)
+    addr = torch.topk(hd, m, largest=False).indices
+    h0 = _pspace_dev(sp, h1e, eri, addr)
+    return addr.cpu().numpy(), h0.cpu().numpy()
+
+
 def trans_rdm12s(cibra, ciket, norb, nelec, link_index=None, reorder=True):
     """((dm1a, dm1b), (dm2aa, dm2ab, dm2ba, dm2bb)) as direct_spin1.trans_rdm12s."""
     (dm1a, dm1b), (dm2aa, dm2ab, dm2bb) = _rdm12s(cibra, ciket, norb, nelec)
@@ -426,6 +457,9 @@ class FCISolver:
     def _nelec(self, nelec):
         return _unpack_nelec(nelec, self.spin)
 
+    def pspace(self, h1e, eri, norb, nelec, hdiag=None, np=400):
+        return pspace(h1e, eri, norb, self._nelec(nelec), hdiag, np)
+
     def get_init_guess(self, norb, nelec, nroots, hdiag):
         """Lowest-diagonal unit vectors plus reproducible global noise
         (fci_gas.py:881-902), as host arrays."""
@@ -479,7 +513,7 @@ class FCISolver:
 
         if (ci0 is None and not davidson_only and sp.ndet <= min(PSPACE_MATVEC_MAX,
                                                                  self.pspace_size)):
-            e, c = self._exact_small(sp, h2e, nroots)
+            e, c = self._exact_small(sp, h1e, eri, nroots)
             conv = numpy.ones(nroots, dtype=bool)
         else:
             from .mixed import solve
@@ -501,17 +535,12 @@ class FCISolver:
             self.ci = [x.cpu().numpy() for x in cis]
         return self.eci, self.ci
 
-    def _exact_small(self, sp, h2e, nroots):
-        """H from sigma on unit vectors, then eigh (fci_gas.py:1226-1257)."""
+    def _exact_small(self, sp, h1e, eri, nroots):
+        """Dense H of the whole space (Slater-Condon block on the GPU), then
+        eigh (the reference's small-space path, fci_gas.py:1226-1257)."""
         import torch
-        n = sp.ndet
-        eye = torch.eye(n, dtype=torch.float64, device="cuda")
-        H = torch.empty_like(eye)
-        with SigmaPlan(sp, h2e, "fp64", _default_workspace()) as plan:
-            for j in range(n):
-                plan(eye[j], H[j])
-        Hh = H.cpu().numpy()
-        Hh = 0.5 * (Hh + Hh.T)
+        addr = torch.arange(sp.ndet, dtype=torch.int64, device="cuda")
+        Hh = _pspace_dev(sp, h1e, eri, addr).cpu().numpy()
         w, v = numpy.linalg.eigh(Hh)
         vecs = [torch.from_numpy(numpy.ascontiguousarray(v[:, k])).cuda() for k in range(nroots)]
         return w[:nroots], vecs

@@ -510,22 +510,18 @@ class FCISolver(_ds.FCISolver):
             raise ValueError("hdiag size does not match GAS determinant count")
         m = min(int(np), gas.ndet, GAS_PSPACE_MATVEC_MAX)
         addr = torch.topk(hd, m, largest=False).indices
-        h2e = absorb_h1e(h1e, eri, norb, nelec, 0.5)
-        return addr.cpu().numpy(), self._h_block(gas, h2e, addr)
+        return addr.cpu().numpy(), self._h_block(gas, h1e, eri, addr)
 
-    def _h_block(self, gas, h2e, addr):
+    def _h_block(self, gas, h1e, eri, addr):
+        """H over the GAS determinants ``addr`` (device int64): their CAS
+        indices, then the Slater-Condon block (fcis_pspace)."""
         import torch
-        x = torch.zeros(gas.ndet, dtype=torch.float64, device="cuda")
-        hx = torch.empty_like(x)
-        m = addr.numel()
-        H = numpy.empty((m, m))
-        with GasContractPlan(gas, h2e) as p:
-            for j, a in enumerate(addr.tolist()):
-                x.zero_()
-                x[a] = 1.0
-                p.execute(x, hx)
-                H[:, j] = hx[addr].cpu().numpy()
-        return 0.5 * (H + H.T)
+        addr = addr.to(device="cuda", dtype=torch.int64).contiguous()
+        cas_addr = torch.empty_like(addr)
+        _lib.check(self.lib.fcis_gas_det_index(gas._live(), int(addr.numel()), _lib.vptr(addr),
+                                               _lib.vptr(cas_addr), _lib.stream_ptr()),
+                   "fcis_gas_det_index")
+        return _ds._pspace_dev(gas.cas, h1e, eri, cas_addr).cpu().numpy()
 
     def gen_linkstr(self, norb, nelec, tril=True, spin=None):
         raise NotImplementedError(
@@ -602,7 +598,7 @@ class FCISolver(_ds.FCISolver):
         if (ci0 is None and not davidson_only and gas.ndet <= pspace_size
                 and gas.ndet <= GAS_PSPACE_MATVEC_MAX):
             addr = torch.arange(gas.ndet, device="cuda")
-            w, v = numpy.linalg.eigh(self._h_block(gas, h2e, addr))
+            w, v = numpy.linalg.eigh(self._h_block(gas, h1e, eri, addr))
             e = w[:nroots]
             c = [torch.from_numpy(numpy.ascontiguousarray(v[:, k])).cuda() for k in range(nroots)]
             conv = numpy.ones(nroots, dtype=bool)

@@ -33,10 +33,12 @@ def _davidson_bytes(ndet, nroots, max_space):
 
 def solve(solver, sp, h2e, hdiag, nroots, ci0, precision, tol=1e-10, tol_residual=None,
           max_cycle=50, max_space=12, lindep=1e-14, level_shift=1e-3, verbose=0,
-          basis=None):
+          basis=None, precond=None):
     """Davidson solve in the determinant basis, or, with ``basis`` (an object
     with ``n`` and ``hop(plan)``, e.g. a CSF transformer), in that basis with
-    H x = basis.hop(plan)(x, out); hdiag and ci0 are then in that basis."""
+    H x = basis.hop(plan)(x, out); hdiag and ci0 are then in that basis.
+    ``precond(r, e, out, x0)`` replaces the diagonal preconditioner
+    r / (hdiag - e) (|hdiag - e| < level_shift -> level_shift)."""
     import torch
     from .direct_spin1 import SigmaPlan
 
@@ -44,9 +46,12 @@ def solve(solver, sp, h2e, hdiag, nroots, ci0, precision, tol=1e-10, tol_residua
     n = sp.ndet if basis is None else int(basis.n)
     stream = _lib.stream_ptr()
 
-    def precond(r, e, out):
+    def diag_precond(r, e, out, x0=None):
         _lib.check(lib.fcis_precond(n, _lib.vptr(hdiag), float(e), float(level_shift),
                                     _lib.vptr(r), _lib.vptr(out), stream), "fcis_precond")
+
+    if precond is None:
+        precond = diag_precond
 
     if ci0 is None:
         guesses = solver._init_guess_dev(hdiag, nroots)
