@@ -160,6 +160,21 @@ int fcis_trans_rdm12s(const fcis_space_t *space, const double *d_bra,
 int fcis_pspace(const fcis_space_t *space, const double *h1e, const double *eri, int m,
                 const int64_t *d_addr, double *d_h0, void *stream);
 
+/*
+ * Total spin S^2 on CAS determinant vectors: a plan holding the beta pair
+ * map (4 * npair * nstr_b bytes) on the space, which must outlive it.
+ * contract: d_ci1 = S^2 d_ci0 (distinct device arrays); diag: the diagonal
+ * of S^2.  Replaces the S^2 plan of the reference's GAS solver
+ * (_GasSpinPlan.contract / diagonal_vector, fci_gas.py:266-419) and
+ * pyscf.fci.spin_op.contract_ss for CAS vectors.
+ */
+typedef struct fcis_spin_plan fcis_spin_plan_t;
+int fcis_spin_plan_create(fcis_spin_plan_t **out, const fcis_space_t *space, void *stream);
+int fcis_spin_plan_contract(const fcis_spin_plan_t *plan, const double *d_ci0, double *d_ci1,
+                            void *stream);
+int fcis_spin_plan_diag(const fcis_spin_plan_t *plan, double *d_out, void *stream);
+void fcis_spin_plan_free(fcis_spin_plan_t *plan);
+
 /* ---- Configuration state functions (spin-adapted basis of the CSF-FCI
  *      solver, pyscf/csf_fci) ----------------------------------------- */
 
@@ -356,6 +371,11 @@ void fcis_gas_sigma_plan_free(fcis_gas_sigma_plan_t *plan);
  * work of fci_contract_gas_plan_execute (fci_gas.h:245). */
 int fcis_gas_sigma_execute(const fcis_gas_space_t *gas, fcis_sigma_plan_t *cas_plan,
                            const double *d_ci0, double *d_ci1, double *d_work, void *stream);
+/* d_ci1 = P S^2 P d_ci0 on GAS vectors with a spin plan on the space's CAS
+ * space (exact S^2 when D is spin complete, as the reference requires,
+ * fci_gas.py:993-1001), d_work 2 * nstr_a * nstr_b doubles. */
+int fcis_gas_contract_ss(const fcis_gas_space_t *gas, const fcis_spin_plan_t *spin,
+                         const double *d_ci0, double *d_ci1, double *d_work, void *stream);
 /* Restricted work of the fused path: alpha rows, beta chunks over those
  * rows, and the chunk count of the full CAS (nstr_a * ceil(nstr_b / 16)). */
 int fcis_gas_rows_info(const fcis_gas_space_t *gas, int64_t *nrows, int64_t *nchunks,

@@ -83,6 +83,7 @@ def spin_square_matrix(n, nelec):
     na, nb = nelec
     sa, sb = _strings(n, na), _strings(n, nb)
     ia = {s: i for i, s in enumerate(sa)}
+    ib = {s: j for j, s in enumerate(sb)}
     N = len(sa) * len(sb)
     S2 = numpy.zeros((N, N))
     sz = 0.5 * (na - nb)
@@ -108,6 +109,6 @@ def spin_square_matrix(n, nelec):
                     b2 = b1 | (1 << q)
                     if a2 not in ia:
                         continue
-                    row = ia[a2] * len(sb) + _strings(n, nb).index(b2)
+                    row = ia[a2] * len(sb) + ib[b2]
                     S2[row, col] += s1 * s2 * s3 * s4
     return S2

@@ -11,7 +11,8 @@ solver interface the reference's CSF/GAS solvers and CASCI/CASSCF use.
 from . import synthetic  # noqa: F401
 from . import _lib  # noqa: F401
 from .direct_spin1 import (  # noqa: F401
-    FCISolver, SigmaPlan, Space, absorb_h1e, contract_2e, energy, gen_linkstr_index,
+    FCISolver, SigmaPlan, Space, SpinPlan, absorb_h1e, contract_2e, contract_ss, energy,
+    gen_linkstr_index,
     get_space, make_hdiag, make_rdm1, make_rdm1s, make_rdm12, make_rdm12s,
     pair_matrix, pspace, spin_square_from_rdm12s, trans_rdm1, trans_rdm1s, trans_rdm12,
     trans_rdm12s)

@@ -95,6 +95,11 @@ SIGNATURES = {
     "fcis_gas_sigma_execute": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "fcis_gas_rows_info": (ctypes.c_int, [_vp, _i64p, _i64p, _i64p]),
     "fcis_gas_det_index": (ctypes.c_int, [_vp, _i64, _vp, _vp, _vp]),
+    "fcis_gas_contract_ss": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "fcis_spin_plan_create": (ctypes.c_int, [ctypes.POINTER(_vp), _vp, _vp]),
+    "fcis_spin_plan_contract": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "fcis_spin_plan_diag": (ctypes.c_int, [_vp, _vp, _vp]),
+    "fcis_spin_plan_free": (None, [_vp]),
 }
 
 _LIB = None
@@ -110,6 +115,10 @@ def load(path=None):
         raise RuntimeError(
             "libfcisigma.so not found at %s: build it with "
             "`python -c 'import __graft_entry__ as g; g.build()'`" % p)
+    # torch ships its own libamdhip64.so.7; loading torch first makes the
+    # library bind to that runtime (same soname), so that device memory,
+    # streams and kernels share one HIP runtime in the process.
+    import torch  # noqa: F401
     lib = ctypes.CDLL(p)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

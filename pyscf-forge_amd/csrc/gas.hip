@@ -727,6 +727,22 @@ int fcis_gas_sigma_execute(const fcis_gas_space_t *g, fcis_sigma_plan_t *cas_pla
         return r;
 }
 
+int fcis_gas_contract_ss(const fcis_gas_space_t *g, const fcis_spin_plan_t *spin,
+                         const double *d_ci0, double *d_ci1, double *d_work, void *stream)
+{
+        if (g == nullptr || spin == nullptr || d_ci0 == nullptr || d_ci1 == nullptr ||
+            d_work == nullptr || d_ci0 == d_ci1) {
+                fcis::set_error("fcis_gas_contract_ss: invalid argument");
+                return FCIS_ERR_INVALID;
+        }
+        hipStream_t st = (hipStream_t)stream;
+        double *a = d_work, *b = d_work + g->ncas;
+        int r = embed(g, d_ci0, a, 1, st);
+        if (r == FCIS_SUCCESS) r = fcis_spin_plan_contract(spin, a, b, stream);
+        if (r == FCIS_SUCCESS) r = project(g, b, d_ci1, 1, st);
+        return r;
+}
+
 int fcis_gas_rows_info(const fcis_gas_space_t *g, int64_t *nrows, int64_t *nchunks,
                        int64_t *nchunks_cas)
 {

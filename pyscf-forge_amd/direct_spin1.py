@@ -291,6 +291,62 @@ def pspace(h1e, eri, norb, nelec, hdiag=None, np=400):
     return addr.cpu().numpy(), h0.cpu().numpy()
 
 
+class SpinPlan:
+    """S^2 on device CAS vectors (fcis_spin_plan_*): contract(c, out) and
+    diag(out).  The space must outlive the plan."""
+
+    def __init__(self, space):
+        import ctypes
+        self.lib = _lib.load()
+        self.space = space
+        h = ctypes.c_void_p()
+        _lib.check(self.lib.fcis_spin_plan_create(ctypes.byref(h), space.handle,
+                                                  _lib.stream_ptr()), "fcis_spin_plan_create")
+        self.handle = h
+
+    def contract(self, c, out):
+        if c.data_ptr() == out.data_ptr():
+            raise ValueError("S^2 input and output must not alias")
+        _lib.check(self.lib.fcis_spin_plan_contract(self.handle, _lib.vptr(c), _lib.vptr(out),
+                                                    _lib.stream_ptr()), "fcis_spin_plan_contract")
+        return out
+
+    def diag(self, out):
+        _lib.check(self.lib.fcis_spin_plan_diag(self.handle, _lib.vptr(out), _lib.stream_ptr()),
+                   "fcis_spin_plan_diag")
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None) is not None:
+            self.lib.fcis_spin_plan_free(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def contract_ss(fcivec, norb, nelec):
+    """S^2 c (pyscf.fci.spin_op.contract_ss; the reference's S^2 contraction,
+    pyscf/mcscf/fci_gas.py:346-372), numpy in / numpy out or device tensors."""
+    import torch
+    sp = get_space(norb, nelec)
+    c, host = _to_device(fcivec, sp.ndet)
+    out = torch.empty_like(c)
+    with SpinPlan(sp) as plan:
+        plan.contract(c, out)
+    shape = numpy.shape(fcivec) if host else tuple(fcivec.shape)
+    return out.cpu().numpy().reshape(shape) if host else out.reshape(shape)
+
+
 def trans_rdm12s(cibra, ciket, norb, nelec, link_index=None, reorder=True):
     """((dm1a, dm1b), (dm2aa, dm2ab, dm2ba, dm2bb)) as direct_spin1.trans_rdm12s."""
     (dm1a, dm1b), (dm2aa, dm2ab, dm2bb) = _rdm12s(cibra, ciket, norb, nelec)
@@ -453,6 +509,9 @@ class FCISolver:
         nelec = self._nelec(nelec)
         dm1s, dm2s = make_rdm12s(ci, norb, nelec)
         return spin_square_from_rdm12s(dm1s, dm2s, nelec)
+
+    def contract_ss(self, fcivec, norb, nelec):
+        return contract_ss(fcivec, norb, self._nelec(nelec))
 
     def _nelec(self, nelec):
         return _unpack_nelec(nelec, self.spin)

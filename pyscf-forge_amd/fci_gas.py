@@ -21,6 +21,7 @@ from . import addons_gas
 from . import direct_spin1 as _ds
 
 GAS_PSPACE_MATVEC_MAX = 64      # fci_gas.py:38
+GAS_SPIN_PSPACE_MATVEC_MAX = 400    # fci_gas.py:39
 GAS_INIT_NOISE_NORM = 1e-3      # fci_gas.py:40
 
 
@@ -327,6 +328,151 @@ class GasContractPlan:
             pass
 
 
+class GasSpinPlan:
+    """S^2 on GAS vectors (_GasSpinPlan, fci_gas.py:266-419): the CAS S^2
+    kernel between embedding and projection, exact when the block set is
+    spin complete (checked by the callers, as in the reference).  Holds the
+    space, which must stay open."""
+
+    def __init__(self, gas):
+        import torch
+        self.gas = gas
+        self.ndet = gas.ndet
+        na, nb = gas.nelec
+        sz = 0.5 * (na - nb)
+        self.diagonal = sz * (sz + 1.0) + nb
+        self._plan = _ds.SpinPlan(gas.cas)
+        self._work = torch.empty(2 * gas.cas.ndet, dtype=torch.float64, device="cuda")
+
+    def contract_dev(self, x, out):
+        """out <- S^2 x for device GAS vectors."""
+        if x.data_ptr() == out.data_ptr():
+            raise ValueError("S^2 input and output must not alias")
+        _lib.check(self.gas.lib.fcis_gas_contract_ss(self.gas._live(), self._plan.handle,
+                                                     _lib.vptr(x), _lib.vptr(out),
+                                                     _lib.vptr(self._work), _lib.stream_ptr()),
+                   "fcis_gas_contract_ss")
+        return out
+
+    def contract(self, fcivec):
+        import torch
+        c, host, shape = _vector_in(fcivec, self.ndet)
+        out = self.contract_dev(c, torch.empty_like(c))
+        return out.cpu().numpy().reshape(shape) if host else out.reshape(shape)
+
+    def diagonal_dev(self):
+        import torch
+        full = torch.empty(self.gas.cas.ndet, dtype=torch.float64, device="cuda")
+        self._plan.diag(full)
+        out = torch.empty(self.ndet, dtype=torch.float64, device="cuda")
+        self.gas.project_dev(full, out)
+        return out
+
+    def diagonal_vector(self):
+        """Exact determinant-basis diagonal of S^2 (fci_gas.py:374-398)."""
+        return self.diagonal_dev().cpu().numpy()
+
+    def matrix(self):
+        """Dense S^2 of a bounded space (fci_gas.py:400-409)."""
+        import torch
+        x = torch.zeros(self.ndet, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        M = numpy.empty((self.ndet, self.ndet))
+        for j in range(self.ndet):
+            x.zero_()
+            x[j] = 1.0
+            M[:, j] = self.contract_dev(x, y).cpu().numpy()
+        return 0.5 * (M + M.T)
+
+    def project(self, vector, target, eigenvalues):
+        """Projection on the S(S+1) = target eigenspace (fci_gas.py:411-419):
+        prod over the other allowed eigenvalues l of (S^2 - l) / (target - l)."""
+        import torch
+        from .davidson import DeviceOps
+        ops = DeviceOps()
+        x, host, shape = _vector_in(vector, self.ndet)
+        x = x.clone()
+        t = torch.empty_like(x)
+        for lam in eigenvalues:
+            if abs(lam - target) < 1e-10:
+                continue
+            self.contract_dev(x, t)
+            ops.lincomb(1, self.ndet, x, numpy.array([-lam]), 1.0, t)
+            ops.lincomb(1, self.ndet, t, numpy.array([1.0 / (target - lam)]), 0.0, x)
+        return x.cpu().numpy().reshape(shape) if host else x.reshape(shape)
+
+
+def _spin_penalty_parameters(solver, norb, nelec):
+    """(shift, target, minimum, allowed S(S+1)) of the solver's spin
+    penalty, or None (fci_gas.py:211-239)."""
+    if getattr(solver, "ss_penalty", None) is None:
+        return None
+    shift = float(solver.ss_penalty)
+    if not numpy.isfinite(shift) or shift < 0.0:
+        raise ValueError("spin penalty shift must be finite and nonnegative")
+    na, nb = _ds._unpack_nelec(nelec)
+    sz = 0.5 * abs(na - nb)
+    minimum = sz * (sz + 1.0)
+    tv = getattr(solver, "ss_value", None)
+    target = minimum if tv is None else float(tv)
+    if not numpy.isfinite(target):
+        raise ValueError("target S^2 must be finite")
+    if target < minimum - 1e-10:
+        raise ValueError("target S^2 is below the minimum allowed by Nalpha-Nbeta")
+    ne = na + nb
+    smax = 0.5 * min(ne, 2 * int(norb) - ne)
+    spins = numpy.arange(sz, smax + 0.5, 1.0)
+    eig = spins * (spins + 1.0)
+    if not numpy.any(numpy.abs(eig - target) < 1e-10):
+        raise ValueError("target S^2 is not an allowed S(S+1) value for this space")
+    return shift, target, minimum, tuple(float(x) for x in eig)
+
+
+def fix_spin_(fciobj, shift=0.2, ss=None):
+    """Attach a spin penalty shift * (S^2 - ss) (or its square when ss is
+    above the minimum) to a GAS solver; ss None = the lowest S(S+1) allowed
+    by Nalpha - Nbeta (the attributes the reference's GAS kernel reads,
+    fci_gas.py:211-239)."""
+    fciobj.ss_penalty = shift
+    fciobj.ss_value = ss
+    return fciobj
+
+
+class _PenaltyBasis:
+    """GAS basis whose H application adds shift * (S^2 - target)[^2]
+    (_spin_penalty_action, fci_gas.py:242-252)."""
+
+    def __init__(self, gas, spin, params):
+        self.gas, self.spin, self.params = gas, spin, params
+        self.n = gas.ndet
+
+    def penalty(self, x, out, t1, t2, ops):
+        """out += shift * delta(x), using two scratch vectors."""
+        shift, target, minimum, _ = self.params
+        n = self.n
+        self.spin.contract_dev(x, t1)
+        ops.lincomb(1, n, x, numpy.array([-target]), 1.0, t1)
+        d = t1
+        if target >= minimum + 0.1:
+            self.spin.contract_dev(t1, t2)
+            ops.lincomb(1, n, t1, numpy.array([-target]), 1.0, t2)
+            d = t2
+        ops.lincomb(1, n, d, numpy.array([shift]), 1.0, out)
+
+    def hop(self, plan):
+        import torch
+        from .davidson import DeviceOps
+        base = self.gas.hop(plan)
+        ops = DeviceOps()
+        t1 = torch.empty(self.n, dtype=torch.float64, device="cuda")
+        t2 = torch.empty_like(t1)
+
+        def apply(x, out):
+            base(x, out)
+            self.penalty(x, out, t1, t2, ops)
+        return apply
+
+
 class GasRDMPlan:
     """RDMs of GAS vectors (GasRDMPlan, fci_gas.py:653-757)."""
 
@@ -434,6 +580,21 @@ class FCISolver(_ds.FCISolver):
 
     def make_contract_plan(self, eri, norb, nelec, precision="fp64"):
         return GasContractPlan(self._space(norb, nelec), eri, precision)
+
+    def make_spin_plan(self, norb, nelec):
+        """S^2 plan on this restriction; requires a spin-complete block set
+        (fci_gas.py:853-869)."""
+        orbs, nelec, D = self._space_spec(norb, nelec)
+        if not addons_gas.is_spin_complete(orbs, nelec, D):
+            raise ValueError("make_spin_plan requires a spin-complete GAS restriction")
+        return GasSpinPlan(self._space(norb, nelec))
+
+    def contract_ss(self, fcivec, norb, nelec):
+        """S^2 c on a GAS vector (fci_gas.py:993-1001)."""
+        orbs, nelec, D = self._space_spec(norb, nelec)
+        if not addons_gas.is_spin_complete(orbs, nelec, D):
+            raise ValueError("contract_ss requires a spin-complete GAS restriction")
+        return GasSpinPlan(self._space(norb, nelec)).contract(fcivec)
 
     def _check_plan(self, plan, norb, nelec, kind, cls):
         if not isinstance(plan, cls):
@@ -561,9 +722,6 @@ class FCISolver(_ds.FCISolver):
         nroots = int(nroots if nroots is not None else kwargs.pop("nroots", self.nroots))
         if nroots < 1:
             raise ValueError("nroots must be positive")
-        if getattr(self, "ss_penalty", None) is not None:
-            raise NotImplementedError("the spin penalty (fix_spin_) is not available on the "
-                                      "device GAS solver")
         if kwargs.get("orbsym") is not None or kwargs.get("wfnsym") is not None:
             raise NotImplementedError(
                 "orbital and wavefunction symmetry filtering is not implemented")
@@ -586,33 +744,77 @@ class FCISolver(_ds.FCISolver):
             raise ValueError("pspace_size must be nonnegative")
         nelec = _ds._unpack_nelec(nelec, self.spin)
         self.norb, self.nelec = int(norb), nelec
+        penalty = _spin_penalty_parameters(self, norb, nelec)
+        self.e_spin_penalty = self.e_physical = self.spin_penalty_method = None
         gas = self._space(norb, nelec)
         if nroots > gas.ndet:
             raise ValueError("nroots exceeds GAS determinant count")
+        spin = None
+        if penalty is not None:
+            if not addons_gas.is_spin_complete(gas.norb, nelec, gas.blocks):
+                raise ValueError("fix_spin_ requires a spin-complete GAS restriction")
+            spin = GasSpinPlan(gas)
         n = gas.norb_total
         h1e = numpy.ascontiguousarray(h1e, dtype=numpy.float64).reshape(n, n)
         eri = _pair(eri, n)
         h2e = absorb_h1e(h1e, eri, norb, nelec, 0.5)
         hdiag = self._hdiag_dev(h1e, eri, norb, nelec, gas)
 
-        if (ci0 is None and not davidson_only and gas.ndet <= pspace_size
-                and gas.ndet <= GAS_PSPACE_MATVEC_MAX):
+        full_pspace = (penalty is None and ci0 is None and not davidson_only
+                       and gas.ndet <= pspace_size and gas.ndet <= GAS_PSPACE_MATVEC_MAX)
+        full_spin_pspace = (penalty is not None and ci0 is None and pspace_size > 0
+                            and gas.ndet <= pspace_size
+                            and gas.ndet <= GAS_SPIN_PSPACE_MATVEC_MAX)
+        if full_pspace or full_spin_pspace:
             addr = torch.arange(gas.ndet, device="cuda")
-            w, v = numpy.linalg.eigh(self._h_block(gas, h1e, eri, addr))
+            H = self._h_block(gas, h1e, eri, addr)
+            if full_spin_pspace:
+                shift, target, minimum, _ = penalty
+                d = spin.matrix() - target * numpy.eye(gas.ndet)
+                if target >= minimum + 0.1:
+                    d = spin.matrix() @ d - target * d
+                H = H + shift * d
+                self.spin_penalty_method = "exact-small-space"
+            w, v = numpy.linalg.eigh(H)
             e = w[:nroots]
             c = [torch.from_numpy(numpy.ascontiguousarray(v[:, k])).cuda() for k in range(nroots)]
             conv = numpy.ones(nroots, dtype=bool)
         else:
-            if ci0 is not None:
+            basis, pre_diag = gas, hdiag
+            if penalty is not None:
+                ci0 = self._penalty_guesses(gas, spin, penalty, hdiag, ci0, nroots, lindep)
+                basis = _PenaltyBasis(gas, spin, penalty)
+                shift, target, minimum, _ = penalty
+                sd = spin.diagonal_dev() - target
+                if target >= minimum + 0.1:
+                    sd = sd * sd
+                pre_diag = hdiag + shift * sd
+                self.spin_penalty_method = "projected-plus-global-davidson"
+            elif ci0 is not None:
                 ci0 = self._guess_list(ci0, nroots, gas.ndet)
             from .mixed import solve
-            e, c, conv, stats = solve(self, gas.cas, h2e, hdiag, nroots, ci0, precision,
+            e, c, conv, stats = solve(self, gas.cas, h2e, pre_diag, nroots, ci0, precision,
                                       tol=tol, tol_residual=tol_residual,
                                       max_cycle=max_cycle, max_space=max_space,
                                       lindep=lindep, level_shift=level_shift,
-                                      verbose=verbose, basis=gas)
+                                      verbose=verbose, basis=basis)
             self.stats = stats
         e = numpy.asarray(e, dtype=numpy.float64).reshape(-1) + float(ecore)
+        if penalty is not None:
+            shift, target, minimum, _ = penalty
+            pen = []
+            for x in c[:nroots]:
+                x = x.reshape(-1)
+                sx = spin.contract_dev(x, torch.empty_like(x))
+                if target < minimum + 0.1:
+                    pen.append(shift * (float(torch.dot(x, sx)) - target))
+                else:
+                    dlt = sx - target * x
+                    pen.append(shift * float(torch.dot(dlt, dlt)))
+            pen = numpy.asarray(pen)
+            phys = e[:nroots] - pen
+            self.e_spin_penalty = float(pen[0]) if nroots == 1 else pen
+            self.e_physical = float(phys[0]) if nroots == 1 else phys
         cis = [x.reshape(-1).cpu().numpy() for x in c]
         if nroots == 1:
             self.converged = bool(numpy.asarray(conv).reshape(-1)[0])
@@ -623,6 +825,38 @@ class FCISolver(_ds.FCISolver):
             self.eci = e[:nroots]
             self.ci = cis[:nroots]
         return self.eci, self.ci
+
+    def _penalty_guesses(self, gas, spin, penalty, hdiag, ci0, nroots, lindep):
+        """Initial vectors of a penalised solve (fci_gas.py:1269-1338): the
+        default or given guesses projected on the target spin and
+        orthonormalised (nroots of them), then the unprojected guesses of
+        the first roots as global probes."""
+        import torch
+        shift, target, minimum, eig = penalty
+        count = min(gas.ndet, max(4 * nroots, nroots + 4))
+        default = self._init_guess_dev(hdiag, count)
+        if ci0 is None:
+            guess = default
+        else:
+            guess = [torch.from_numpy(v).cuda() for v in self._guess_list(ci0, nroots, gas.ndet)]
+            if len(guess) < count:
+                guess = guess + default[len(guess):count]
+        projected = []
+        for cand in guess:
+            v = spin.project(cand, target, eig)
+            for prev in projected:
+                v = v - torch.dot(prev, v) * prev
+            nrm = float(torch.linalg.norm(v))
+            if nrm > max(100.0 * lindep, 1e-12):
+                projected.append(v / nrm)
+            if len(projected) == nroots:
+                break
+        if len(projected) < nroots:
+            raise RuntimeError("could not construct enough independent target-spin "
+                               "initial guesses")
+        probes = [g / torch.linalg.norm(g) for g in guess[:max(1, nroots)]
+                  if float(torch.linalg.norm(g)) > 0.0]
+        return projected + probes
 
     @staticmethod
     def _guess_list(ci0, nroots, ndet):
