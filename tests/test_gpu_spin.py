@@ -134,7 +134,9 @@ def test_gas_spin_penalty_davidson_targets_triplet():
         print("target", ss, flush=True)
         sol = fg.fix_spin_(fg.FCISolver(gas_orbs=d["gas_orbs"], gas_restr=d["restr"],
                                         gas_restr_type=d["restr_type"]), 1.0, ss)
-        e, c = sol.kernel(d["h1e"], d["eri"], n, d["nelec"], max_cycle=300)
+        # the quadratic penalty (S^2 - 2)^2 with its diagonal-only
+        # preconditioner is stiff on these random integrals: 24-vector space
+        e, c = sol.kernel(d["h1e"], d["eri"], n, d["nelec"], max_cycle=600, max_space=24)
         assert sol.converged
         assert sol.spin_penalty_method == "projected-plus-global-davidson"
         assert abs(sol.e_physical - ref) < 1e-8, (ss, sol.e_physical, ref)
