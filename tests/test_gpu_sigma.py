@@ -208,3 +208,36 @@ def test_fused_and_unfused_paths_agree(precision, monkeypatch):
     tol = 1e-13 if precision == "fp64" else 2e-6
     scale = a.abs().max().item()
     assert (a - b).abs().max().item() < tol * scale
+
+
+@pytest.mark.parametrize("ne", [(2, 2), (3, 2), (1, 0)])
+def test_sigma_16_orbitals_vs_oracle(ne):
+    """norb = 16 (npair = 136): the FP64 fused kernel contracts the last 8
+    pair rows on the VALU of its 9th wavefront; sigma against the oracle."""
+    n = 16
+    h1e, eri, c = pkg.synthetic.make_problem(n, ne, seed=3)
+    h2e = pkg.absorb_h1e(h1e, eri, n, ne, 0.5)
+    ref = oracle.contract_2e(oracle.absorb_h1e(h1e, eri, n, ne, 0.5), c, n, ne).reshape(-1)
+    for prec, tol in (("fp64", 1e-13), ("fp32", 2e-6)):
+        got = pkg.contract_2e(h2e, c, n, ne, precision=prec).reshape(-1)
+        assert numpy.abs(got - ref).max() < tol * numpy.abs(ref).max(), prec
+
+
+def test_sigma_16_orbitals_fused_vs_three_kernel(monkeypatch):
+    import torch
+    n, ne = 16, (5, 4)
+    h1e, eri = pkg.synthetic.make_problem(n, ne, seed=5, with_ci=False)
+    h2e = pkg.absorb_h1e(h1e, eri, n, ne, 0.5)
+    sp = pkg.get_space(n, ne)
+    g = torch.Generator(device="cuda").manual_seed(7)
+    c = torch.randn(sp.ndet, dtype=torch.float64, device="cuda", generator=g)
+    with pkg.SigmaPlan(sp, h2e, "fp64", 1 << 33) as plan:
+        assert plan.fused
+        a = torch.empty_like(c)
+        plan(c, a)
+    monkeypatch.setenv("FCIS_DETERMINISTIC", "1")
+    with pkg.SigmaPlan(sp, h2e, "fp64", 1 << 33) as plan:
+        assert not plan.fused
+        b = torch.empty_like(c)
+        plan(c, b)
+    assert float((a - b).abs().max()) < 1e-13 * float(b.abs().max())
