@@ -58,6 +58,15 @@ int fcis_device_info(char *buf, int buflen);
  * norb <= 32, 0 <= na, nb <= norb.
  */
 int fcis_space_create(fcis_space_t **out, int norb, int na, int nb, void *stream);
+/* The same space with the caller's address order: stra / strb (host, nstr
+ * each) list every alpha / beta string once, string a at address a.  Link
+ * tables, sigma, diagonal, RDM, pspace and S^2 kernels work in that order
+ * (the CSF transform needs the colex order).  Used for restricted spaces,
+ * whose sectors are contiguous address ranges in sector-major order
+ * (fcis_gas_string_order).  No counterpart in the reference (its restricted
+ * spaces address strings per sector, fci_string_gas.c:462-489). */
+int fcis_space_create_ordered(fcis_space_t **out, int norb, int na, int nb,
+                              const uint64_t *stra, const uint64_t *strb, void *stream);
 void fcis_space_free(fcis_space_t *space);
 int fcis_space_info(const fcis_space_t *space, int64_t *nstr_alpha,
                     int64_t *nstr_beta, int *nlink_alpha, int *nlink_beta);
@@ -322,6 +331,11 @@ typedef struct fcis_gas_sigma_plan fcis_gas_sigma_plan_t;
 int fcis_gas_space_create(fcis_gas_space_t **out, const fcis_space_t *cas, int ngas,
                           const int *norb, int nblock, const int *block_occ, void *stream);
 void fcis_gas_space_free(fcis_gas_space_t *gas);       /* gas_space_free */
+/* All strings of nelec electrons in sum(norb[]) orbitals, sorted by their
+ * occupation tuple over the ngas subspaces (the reference's sector order),
+ * colex within a tuple: the sector-major address order for
+ * fcis_space_create_ordered.  out: host, C(sum(norb), nelec) entries. */
+int fcis_gas_string_order(int ngas, const int *norb, int nelec, uint64_t *out);
 /* Determinants, blocks and sectors (gas_space_t ndet/nblock/nsector). */
 int fcis_gas_space_info(const fcis_gas_space_t *gas, int64_t *ndet, int *nblock,
                         int *nsector);

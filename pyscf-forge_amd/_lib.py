@@ -31,6 +31,9 @@ SIGNATURES = {
     "fcis_space_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int,
                                          ctypes.c_int, ctypes.c_int, _vp]),
     "fcis_space_free": (None, [_vp]),
+    "fcis_space_create_ordered": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int,
+                                                 ctypes.c_int, ctypes.c_int, _u64p, _u64p, _vp]),
+    "fcis_gas_string_order": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int, _u64p]),
     "fcis_space_info": (ctypes.c_int, [_vp, _i64p, _i64p, _ip, _ip]),
     "fcis_gen_linkstr_index": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int,
                                               _vp, _vp]),

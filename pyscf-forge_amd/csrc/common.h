@@ -101,8 +101,18 @@ struct fcis_space {
         uint64_t *d_stra;       // strings in address order
         uint64_t *d_strb;
         int device;
+        // Spaces with a caller-given string order (fcis_space_create_ordered):
+        // host copies of the strings in address order and the address of
+        // each string by colex rank (host and device); NULL for colex spaces.
+        uint64_t *h_stra, *h_strb;
+        uint32_t *h_perma, *h_permb;
+        uint32_t *d_perma, *d_permb;
 };
 
 namespace fcis {
 int build_links(fcis_space *s, hipStream_t stream);
+
+// Host views of a space's addressing (colex or caller-ordered).
+uint64_t space_string(const fcis_space *s, int spin, int64_t addr);
+int64_t space_addr(const fcis_space *s, int spin, uint64_t str);
 }

@@ -354,6 +354,10 @@ int fcis_csf_create(fcis_csf_t **out, const fcis_space_t *s, int smult, void *st
                 fcis::set_error("fcis_csf_create: more than 2^32 determinants");
                 return FCIS_ERR_INVALID;
         }
+        if (s->h_stra != nullptr) {
+                fcis::set_error("fcis_csf_create: the CSF transform needs a colex-ordered space");
+                return FCIS_ERR_INVALID;
+        }
         fcis_csf *c = new (std::nothrow) fcis_csf();
         if (c == nullptr) return FCIS_ERR_MEMORY;
         c->space = s;

@@ -242,7 +242,6 @@ int build_rowset(fcis_gas_space *g, const std::vector<std::vector<uint8_t>> &poo
         const int ngas = g->ngas, ns = g->nsector;
         const int64_t na = g->cas->nstra, nb = g->cas->nstrb;
         const int64_t nch = (nb + FUSED_NB - 1) / FUSED_NB;
-        const int ntot = g->cas->norb;
         // S1 sector pairs
         std::vector<std::vector<char>> allow(ns, std::vector<char>(ns, 0));
         auto moves = [&](int s) {
@@ -267,7 +266,7 @@ int build_rowset(fcis_gas_space *g, const std::vector<std::vector<uint8_t>> &poo
         }
         // sector of every CAS beta string; chunk lists per alpha sector
         std::vector<int> bsec(nb);
-        for (int64_t j = 0; j < nb; j++) bsec[j] = sector_of(g, pool, unrank_local(ntot, g->nb, j));
+        for (int64_t j = 0; j < nb; j++) bsec[j] = sector_of(g, pool, fcis::space_string(g->cas, 1, j));
         std::vector<int64_t> soff(ns, 0), scnt(ns, 0);
         std::vector<uint16_t> clist;
         std::vector<char> mark(nch);
@@ -283,7 +282,7 @@ int build_rowset(fcis_gas_space *g, const std::vector<std::vector<uint8_t>> &poo
         std::vector<std::pair<int64_t, int32_t>> order;   // (-count, alpha string)
         std::vector<int> asec(na);
         for (int64_t i = 0; i < na; i++) {
-                asec[i] = sector_of(g, pool, unrank_local(ntot, g->na, i));
+                asec[i] = sector_of(g, pool, fcis::space_string(g->cas, 0, i));
                 if (asec[i] >= 0 && scnt[asec[i]] > 0) order.emplace_back(-scnt[asec[i]], (int32_t)i);
         }
         std::sort(order.begin(), order.end());
@@ -468,15 +467,19 @@ int fcis_gas_space_create(fcis_gas_space_t **out, const fcis_space_t *cas, int n
                 fcis::set_error("fcis_gas_space_create: 2^32 or more determinants");
                 return FCIS_ERR_INVALID;
         }
-        // CAS string addresses of every sector used by a block
-        std::vector<int64_t> base(g->nsector, -1);
+        // CAS string addresses (in the CAS space's own order) of every
+        // sector used by a block, per spin
+        std::vector<int64_t> base[2] = {std::vector<int64_t>(g->nsector, -1),
+                                        std::vector<int64_t>(g->nsector, -1)};
         std::vector<uint32_t> secaddr;
         for (int b = 0; b < nblock; b++) {
-                for (int s : {g->bsa[b], g->bsb[b]}) {
-                        if (base[s] >= 0) continue;
-                        base[s] = (int64_t)secaddr.size();
+                for (int spin = 0; spin < 2; spin++) {
+                        const int s = spin ? g->bsb[b] : g->bsa[b];
+                        if (base[spin][s] >= 0) continue;
+                        base[spin][s] = (int64_t)secaddr.size();
                         for (int64_t a = 0; a < g->nstr[s]; a++) {
-                                secaddr.push_back((uint32_t)rank_colex(sector_string(g, s, a)));
+                                secaddr.push_back(
+                                        (uint32_t)fcis::space_addr(cas, spin, sector_string(g, s, a)));
                         }
                 }
         }
@@ -484,8 +487,8 @@ int fcis_gas_space_create(fcis_gas_space_t **out, const fcis_space_t *cas, int n
         for (int b = 0; b < nblock; b++) {
                 tab[b] = g->boff[b];
                 tab[nblock + b] = g->nstr[g->bsb[b]];
-                tab[2 * nblock + b] = base[g->bsa[b]];
-                tab[3 * nblock + b] = base[g->bsb[b]];
+                tab[2 * nblock + b] = base[0][g->bsa[b]];
+                tab[3 * nblock + b] = base[1][g->bsb[b]];
         }
         int64_t *d_tab = nullptr;
         uint32_t *d_sec = nullptr;
@@ -520,6 +523,44 @@ int fcis_gas_space_create(fcis_gas_space_t **out, const fcis_space_t *cas, int n
                 return st_ok;
         }
         *out = g;
+        return FCIS_SUCCESS;
+}
+
+int fcis_gas_string_order(int ngas, const int *norb, int nelec, uint64_t *out)
+{
+        if (ngas < 1 || ngas > GAS_MAXNGAS || norb == nullptr || out == nullptr || nelec < 0) {
+                fcis::set_error("fcis_gas_string_order: invalid argument");
+                return FCIS_ERR_INVALID;
+        }
+        int tot = 0, start[GAS_MAXNGAS];
+        for (int k = 0; k < ngas; k++) {
+                if (norb[k] < 1) {
+                        fcis::set_error("fcis_gas_string_order: empty subspace");
+                        return FCIS_ERR_INVALID;
+                }
+                start[k] = tot;
+                tot += norb[k];
+        }
+        if (tot > FCIS_MAXORB || nelec > tot) {
+                fcis::set_error("fcis_gas_string_order: invalid orbital or electron count");
+                return FCIS_ERR_INVALID;
+        }
+        const int64_t ns = fcis::binom_host(tot, nelec);
+        std::vector<std::pair<std::vector<uint8_t>, int64_t>> key(ns);
+        for (int64_t a = 0; a < ns; a++) {
+                const uint64_t x = unrank_local(tot, nelec, a);
+                key[a].first.resize(ngas);
+                for (int k = 0; k < ngas; k++)
+                        key[a].first[k] = (uint8_t)__builtin_popcountll(
+                                (x >> start[k]) & ((1ull << norb[k]) - 1));
+                key[a].second = a;
+        }
+        std::stable_sort(key.begin(), key.end(),
+                         [](const std::pair<std::vector<uint8_t>, int64_t> &u,
+                            const std::pair<std::vector<uint8_t>, int64_t> &v) {
+                                 return u.first < v.first;
+                         });
+        for (int64_t a = 0; a < ns; a++) out[a] = unrank_local(tot, nelec, key[a].second);
         return FCIS_SUCCESS;
 }
 

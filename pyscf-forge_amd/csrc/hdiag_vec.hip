@@ -148,11 +148,10 @@ int fcis_make_hdiag(const fcis_space_t *s, const double *h1e, const double *eri,
                 }
         std::vector<double> ea(s->nstra), eb(s->nstrb), jb((size_t)s->nstrb * n);
         for (int spin = 0; spin < 2; spin++) {
-                int k = spin ? s->nb : s->na;
                 int64_t ns = spin ? s->nstrb : s->nstra;
                 double *e = spin ? eb.data() : ea.data();
                 for (int64_t a = 0; a < ns; a++) {
-                        uint64_t str = fcis_addr2str(n, k, a);
+                        uint64_t str = fcis::space_string(s, spin, a);
                         double v = 0;
                         for (int i = 0; i < n; i++) {
                                 if (!((str >> i) & 1)) continue;

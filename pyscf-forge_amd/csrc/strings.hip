@@ -68,7 +68,9 @@ struct LinkEntry {
         int rslot;      // slot of the reverse excitation in addr's list
 };
 
-__device__ inline LinkEntry link_entry(int n, int k, uint64_t s, uint32_t str, int slot)
+// perm: address of each string by colex rank (ordered spaces) or NULL.
+__device__ inline LinkEntry link_entry(int n, int k, uint64_t s, uint32_t str, int slot,
+                                       const uint32_t *__restrict__ perm)
 {
         LinkEntry e;
         if (slot < k) {
@@ -90,7 +92,7 @@ __device__ inline LinkEntry link_entry(int n, int k, uint64_t s, uint32_t str, i
         uint64_t mask = (1ull << hi) - (1ull << (lo + 1));
         e.cre = p;
         e.des = q;
-        e.addr = rank(s1);
+        e.addr = perm ? perm[rank(s1)] : rank(s1);
         e.sign = (__popcll(s & mask) & 1) ? -1 : 1;
         // reverse excitation a_q^+ a_p |s1>: p is occupied in s1 (occupied
         // index = #occupied below p), q is empty (virtual index = #empty below q)
@@ -109,12 +111,13 @@ __global__ void k_make_strings(int n, int k, uint32_t nstr, uint64_t *out)
 // grid.x covers strings, grid.y covers link slots.
 __global__ void k_build_links(int n, int k, uint32_t nstr, int nlink,
                               const uint64_t *__restrict__ strs,
+                              const uint32_t *__restrict__ perm,
                               fcis_link *__restrict__ link)
 {
         uint32_t str = blockIdx.x * blockDim.x + threadIdx.x;
         int slot = blockIdx.y;
         if (str >= nstr || slot >= nlink) return;
-        LinkEntry e = link_entry(n, k, strs[str], str, slot);
+        LinkEntry e = link_entry(n, k, strs[str], str, slot, perm);
         fcis_link out;
         out.addr = (int32_t)e.addr;
         out.info = link_pack(e.cre, e.des, fcis::pair_index(e.cre, e.des), e.sign < 0 ? 1 : 0,
@@ -124,12 +127,13 @@ __global__ void k_build_links(int n, int k, uint32_t nstr, int nlink,
 
 __global__ void k_export_links(int n, int k, uint32_t nstr, int nlink, int tril,
                                const uint64_t *__restrict__ strs,
+                               const uint32_t *__restrict__ perm,
                                int32_t *__restrict__ out)
 {
         uint32_t str = blockIdx.x * blockDim.x + threadIdx.x;
         int slot = blockIdx.y;
         if (str >= nstr || slot >= nlink) return;
-        LinkEntry e = link_entry(n, k, strs[str], str, slot);
+        LinkEntry e = link_entry(n, k, strs[str], str, slot, perm);
         int4 v;
         if (tril) {
                 v.x = fcis::pair_index(e.cre, e.des);
@@ -157,6 +161,19 @@ void set_error(const char *fmt, ...)
         va_end(ap);
 }
 
+uint64_t space_string(const fcis_space *s, int spin, int64_t addr)
+{
+        const uint64_t *h = spin ? s->h_strb : s->h_stra;
+        return h ? h[addr] : fcis_addr2str(s->norb, spin ? s->nb : s->na, addr);
+}
+
+int64_t space_addr(const fcis_space *s, int spin, uint64_t str)
+{
+        const int64_t r = fcis_str2addr(s->norb, spin ? s->nb : s->na, str);
+        const uint32_t *p = spin ? s->h_permb : s->h_perma;
+        return p ? (int64_t)p[r] : r;
+}
+
 int build_links(fcis_space *s, hipStream_t stream)
 {
         for (int spin = 0; spin < 2; spin++) {
@@ -164,14 +181,17 @@ int build_links(fcis_space *s, hipStream_t stream)
                 uint32_t nstr = (uint32_t)(spin ? s->nstrb : s->nstra);
                 int nlink = spin ? s->nlinkb : s->nlinka;
                 uint64_t *strs = spin ? s->d_strb : s->d_stra;
+                const uint32_t *perm = spin ? s->d_permb : s->d_perma;
                 fcis_link *link = spin ? s->d_linkb : s->d_linka;
-                hipLaunchKernelGGL(k_make_strings, dim3((nstr + 255) / 256), dim3(256), 0,
-                                   stream, s->norb, k, nstr, strs);
-                FCIS_CHECK_LAUNCH();
+                if (perm == nullptr) {   // colex: strings from their addresses
+                        hipLaunchKernelGGL(k_make_strings, dim3((nstr + 255) / 256), dim3(256), 0,
+                                           stream, s->norb, k, nstr, strs);
+                        FCIS_CHECK_LAUNCH();
+                }
                 if (nlink > 0) {
                         hipLaunchKernelGGL(k_build_links, dim3((nstr + 255) / 256, nlink),
                                            dim3(256), 0, stream, s->norb, k, nstr, nlink,
-                                           strs, link);
+                                           strs, perm, link);
                         FCIS_CHECK_LAUNCH();
                 }
         }
@@ -267,7 +287,102 @@ void fcis_space_free(fcis_space_t *s)
         (void)hipFree(s->d_linkb);
         (void)hipFree(s->d_stra);
         (void)hipFree(s->d_strb);
+        (void)hipFree(s->d_perma);
+        (void)hipFree(s->d_permb);
+        free(s->h_stra);
+        free(s->h_strb);
+        free(s->h_perma);
+        free(s->h_permb);
         free(s);
+}
+
+int fcis_space_create_ordered(fcis_space_t **out, int norb, int na, int nb,
+                              const uint64_t *stra, const uint64_t *strb, void *stream)
+{
+        if (out == nullptr || stra == nullptr || strb == nullptr || norb < 1 ||
+            norb > FCIS_MAXORB || na < 0 || nb < 0 || na > norb || nb > norb) {
+                fcis::set_error("fcis_space_create_ordered: invalid argument");
+                return FCIS_ERR_INVALID;
+        }
+        *out = nullptr;
+        fcis_space *s = (fcis_space *)calloc(1, sizeof(fcis_space));
+        if (s == nullptr) return FCIS_ERR_MEMORY;
+        s->norb = norb;
+        s->na = na;
+        s->nb = nb;
+        s->npair = norb * (norb + 1) / 2;
+        s->nstra = fcis::binom_host(norb, na);
+        s->nstrb = fcis::binom_host(norb, nb);
+        s->nlinka = na + na * (norb - na);
+        s->nlinkb = nb + nb * (norb - nb);
+        if (s->nstra * s->nstrb > (int64_t)1 << 40 || s->nstra >= ((int64_t)1 << 32) ||
+            s->nstrb >= ((int64_t)1 << 32)) {
+                free(s);
+                fcis::set_error("fcis_space_create_ordered: determinant space too large");
+                return FCIS_ERR_INVALID;
+        }
+        (void)hipGetDevice(&s->device);
+        // strings: each a k-subset of norb orbitals, every one exactly once
+        for (int spin = 0; spin < 2; spin++) {
+                const int k = spin ? nb : na;
+                const int64_t ns = spin ? s->nstrb : s->nstra;
+                const uint64_t *in = spin ? strb : stra;
+                uint64_t *h = (uint64_t *)malloc(8 * (size_t)ns + 8);
+                uint32_t *perm = (uint32_t *)malloc(4 * (size_t)ns + 4);
+                (spin ? s->h_strb : s->h_stra) = h;
+                (spin ? s->h_permb : s->h_perma) = perm;
+                if (h == nullptr || perm == nullptr) {
+                        fcis_space_free(s);
+                        return FCIS_ERR_MEMORY;
+                }
+                for (int64_t r = 0; r < ns; r++) perm[r] = UINT32_MAX;
+                for (int64_t a = 0; a < ns; a++) {
+                        const uint64_t x = in[a];
+                        const int64_t r = fcis_str2addr(norb, k, x);
+                        if ((norb < 64 && (x >> norb)) || __builtin_popcountll(x) != k || r >= ns ||
+                            perm[r] != UINT32_MAX) {
+                                fcis_space_free(s);
+                                fcis::set_error("fcis_space_create_ordered: %s string %lld is "
+                                                "invalid or repeated", spin ? "beta" : "alpha",
+                                                (long long)a);
+                                return FCIS_ERR_INVALID;
+                        }
+                        h[a] = x;
+                        perm[r] = (uint32_t)a;
+                }
+        }
+        hipStream_t st = (hipStream_t)stream;
+        size_t la = sizeof(fcis_link) * (size_t)s->nstra * (s->nlinka ? s->nlinka : 1);
+        size_t lb = sizeof(fcis_link) * (size_t)s->nstrb * (s->nlinkb ? s->nlinkb : 1);
+        if (hipMalloc(&s->d_linka, la) != hipSuccess || hipMalloc(&s->d_linkb, lb) != hipSuccess ||
+            hipMalloc(&s->d_stra, 8 * (size_t)s->nstra) != hipSuccess ||
+            hipMalloc(&s->d_strb, 8 * (size_t)s->nstrb) != hipSuccess ||
+            hipMalloc(&s->d_perma, 4 * (size_t)s->nstra) != hipSuccess ||
+            hipMalloc(&s->d_permb, 4 * (size_t)s->nstrb) != hipSuccess) {
+                fcis_space_free(s);
+                fcis::set_error("fcis_space_create_ordered: device allocation failed");
+                return FCIS_ERR_MEMORY;
+        }
+        if (hipMemcpyAsync(s->d_stra, s->h_stra, 8 * (size_t)s->nstra, hipMemcpyHostToDevice, st) !=
+                    hipSuccess ||
+            hipMemcpyAsync(s->d_strb, s->h_strb, 8 * (size_t)s->nstrb, hipMemcpyHostToDevice, st) !=
+                    hipSuccess ||
+            hipMemcpyAsync(s->d_perma, s->h_perma, 4 * (size_t)s->nstra, hipMemcpyHostToDevice, st) !=
+                    hipSuccess ||
+            hipMemcpyAsync(s->d_permb, s->h_permb, 4 * (size_t)s->nstrb, hipMemcpyHostToDevice, st) !=
+                    hipSuccess) {
+                fcis_space_free(s);
+                fcis::set_error("fcis_space_create_ordered: upload failed");
+                return FCIS_ERR_DEVICE;
+        }
+        int r = fcis::build_links(s, st);
+        if (r == FCIS_SUCCESS && hipStreamSynchronize(st) != hipSuccess) r = FCIS_ERR_DEVICE;
+        if (r != FCIS_SUCCESS) {
+                fcis_space_free(s);
+                return r;
+        }
+        *out = s;
+        return FCIS_SUCCESS;
 }
 
 int fcis_space_info(const fcis_space_t *s, int64_t *nstra, int64_t *nstrb,
@@ -298,7 +413,7 @@ int fcis_gen_linkstr_index(const fcis_space_t *s, int spin, int trilidx,
         }
         hipLaunchKernelGGL(k_export_links, dim3((nstr + 255) / 256, nlink), dim3(256), 0,
                            (hipStream_t)stream, s->norb, k, nstr, nlink, trilidx ? 1 : 0,
-                           spin ? s->d_strb : s->d_stra, d_out);
+                           spin ? s->d_strb : s->d_stra, spin ? s->d_permb : s->d_perma, d_out);
         FCIS_CHECK_LAUNCH();
         return FCIS_SUCCESS;
 }

@@ -82,17 +82,28 @@ def absorb_h1e(h1e, eri, norb, nelec, fac=1):
 # ---------------------------------------------------------------- spaces
 
 class Space:
-    """Device string space (link tables built on the GPU)."""
+    """Device string space (link tables built on the GPU).  Addresses are
+    colex ranks (pyscf's order) unless ``strings`` = (alpha, beta) uint64
+    arrays give another address order (fcis_space_create_ordered)."""
 
-    def __init__(self, norb, nelec):
+    def __init__(self, norb, nelec, strings=None):
         import ctypes
         self.lib = _lib.load()
         self.norb = int(norb)
         self.nelec = _unpack_nelec(nelec)
+        self.ordered = strings is not None
         h = ctypes.c_void_p()
-        _lib.check(self.lib.fcis_space_create(ctypes.byref(h), self.norb, self.nelec[0],
-                                              self.nelec[1], _lib.stream_ptr()),
-                   "fcis_space_create")
+        if strings is None:
+            _lib.check(self.lib.fcis_space_create(ctypes.byref(h), self.norb, self.nelec[0],
+                                                  self.nelec[1], _lib.stream_ptr()),
+                       "fcis_space_create")
+        else:
+            sa, sb = (numpy.ascontiguousarray(x, dtype=numpy.uint64) for x in strings)
+            up = ctypes.POINTER(ctypes.c_uint64)
+            _lib.check(self.lib.fcis_space_create_ordered(
+                ctypes.byref(h), self.norb, self.nelec[0], self.nelec[1],
+                sa.ctypes.data_as(up), sb.ctypes.data_as(up), _lib.stream_ptr()),
+                "fcis_space_create_ordered")
         self.handle = h
         na, nb = ctypes.c_int64(), ctypes.c_int64()
         la, lb = ctypes.c_int(), ctypes.c_int()

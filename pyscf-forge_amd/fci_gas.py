@@ -13,6 +13,7 @@ than 2^32 determinants that fit in device memory.
 """
 
 import ctypes
+import math
 
 import numpy
 
@@ -36,11 +37,40 @@ def absorb_h1e(h1e, eri, norb, nelec, fac=0.5):
     return _ds.absorb_h1e(h1e, _pair(eri, norb), norb, nelec, fac)
 
 
+_ORDERED = {}
+
+
+def _sector_major_space(norb, nelec):
+    """The CAS string space of sum(norb) orbitals with addresses in
+    sector-major order (strings grouped by their occupation of the
+    subspaces, fcis_gas_string_order), shared by the GAS spaces of one
+    partition.  One subspace: the colex space of the CAS solver."""
+    key = (tuple(norb), tuple(nelec))
+    if len(norb) == 1:
+        return _ds.get_space(norb[0], nelec)
+    sp = _ORDERED.get(key)
+    if sp is None:
+        lib = _lib.load()
+        n = sum(norb)
+        na = numpy.ascontiguousarray(norb, dtype=numpy.int32)
+        strs = []
+        for k in nelec:
+            out = numpy.empty(math.comb(n, k), dtype=numpy.uint64)
+            _lib.check(lib.fcis_gas_string_order(len(norb), na.ctypes.data_as(
+                ctypes.POINTER(ctypes.c_int)), int(k), out.ctypes.data_as(
+                ctypes.POINTER(ctypes.c_uint64))), "fcis_gas_string_order")
+            strs.append(out)
+        sp = _ORDERED[key] = _ds.Space(n, nelec, strings=tuple(strs))
+    return sp
+
+
 class GasSpace:
     """Device GAS space for subspace sizes ``norb`` and block set ``blocks``
     (GasSpace, fci_gas.py:420-567).  Usable as a context manager; close()
-    releases the device map.  The CAS string space of sum(norb) orbitals
-    is shared with the CAS solver."""
+    releases the device maps.  The kernels run on a CAS string space of
+    sum(norb) orbitals in sector-major address order (``cas``), so that
+    each sector is one address range; gas2fci / fci2gas use a second map to
+    the colex (pyscf) CAS layout."""
 
     def __init__(self, norb, nelec, blocks=None, lib=None, compress_links=False):
         self.norb = tuple(int(x) for x in norb)
@@ -52,22 +82,46 @@ class GasSpace:
         self.blocks = addons_gas.normalize_blocks(blocks, len(self.norb))
         self.limits = addons_gas.check_kernel_limits(self.norb, self.nelec, self.blocks)
         self.lib = lib or _lib.load()
-        self.cas = _ds.get_space(sum(self.norb), self.nelec)
+        self.cas = _sector_major_space(self.norb, self.nelec)
         self._compressed = bool(compress_links)
+        self.handle = self._create(self.cas)
+        self._colex_cas = _ds.get_space(sum(self.norb), self.nelec)
+        self._colex = self.handle if self._colex_cas is self.cas else None
+        nd, nb, ns = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()
+        _lib.check(self.lib.fcis_gas_space_info(self.handle, ctypes.byref(nd), ctypes.byref(nb),
+                                                ctypes.byref(ns)), "fcis_gas_space_info")
+        self._ndet, self._nblock, self._nsector = nd.value, nb.value, ns.value
+        self._blocks_desc = None
+
+    def _create(self, cas):
         norb_a = numpy.ascontiguousarray(self.norb, dtype=numpy.int32)
         occ = numpy.ascontiguousarray(self.blocks.reshape(-1), dtype=numpy.int32)
         h = ctypes.c_void_p()
         _lib.check(self.lib.fcis_gas_space_create(
-            ctypes.byref(h), self.cas.handle, len(self.norb),
+            ctypes.byref(h), cas.handle, len(self.norb),
             norb_a.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), int(self.blocks.shape[0]),
             occ.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _lib.stream_ptr()),
             "fcis_gas_space_create")
-        self.handle = h
-        nd, nb, ns = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()
-        _lib.check(self.lib.fcis_gas_space_info(h, ctypes.byref(nd), ctypes.byref(nb),
-                                                ctypes.byref(ns)), "fcis_gas_space_info")
-        self._ndet, self._nblock, self._nsector = nd.value, nb.value, ns.value
-        self._blocks_desc = None
+        return h
+
+    def _colex_handle(self):
+        """GAS space on the colex CAS space (for gas2fci / fci2gas)."""
+        self._live()
+        if self._colex is None:
+            self._colex = self._create(self._colex_cas)
+        return self._colex
+
+    def embed_colex(self, gas_vec, cas_out, nvec=1):
+        """cas_out[v] <- gas2fci(gas_vec[v]) in pyscf's (colex) CAS layout."""
+        _lib.check(self.lib.fcis_gas_embed(self._colex_handle(), _lib.vptr(gas_vec),
+                                           _lib.vptr(cas_out), int(nvec), _lib.stream_ptr()),
+                   "fcis_gas_embed")
+
+    def project_colex(self, cas_vec, gas_out, nvec=1):
+        """gas_out[v] <- fci2gas(cas_vec[v]) from pyscf's (colex) CAS layout."""
+        _lib.check(self.lib.fcis_gas_project(self._colex_handle(), _lib.vptr(cas_vec),
+                                             _lib.vptr(gas_out), int(nvec), _lib.stream_ptr()),
+                   "fcis_gas_project")
 
     # -- descriptors (fci_gas.py:459-542) -------------------------------
     def _live(self):
@@ -152,12 +206,13 @@ class GasSpace:
 
     # -- device transforms ----------------------------------------------
     def embed_dev(self, gas_vec, cas_out, nvec=1):
-        """cas_out[v] <- gas2fci(gas_vec[v]) (device float64, contiguous)."""
+        """cas_out[v] <- gas_vec[v] embedded in the kernels' CAS layout
+        (``self.cas`` address order; device float64, contiguous)."""
         _lib.check(self.lib.fcis_gas_embed(self._live(), _lib.vptr(gas_vec), _lib.vptr(cas_out),
                                            int(nvec), _lib.stream_ptr()), "fcis_gas_embed")
 
     def project_dev(self, cas_vec, gas_out, nvec=1):
-        """gas_out[v] <- fci2gas(cas_vec[v])."""
+        """gas_out[v] <- cas_vec[v] (``self.cas`` layout) projected on the space."""
         _lib.check(self.lib.fcis_gas_project(self._live(), _lib.vptr(cas_vec),
                                              _lib.vptr(gas_out), int(nvec), _lib.stream_ptr()),
                    "fcis_gas_project")
@@ -191,6 +246,9 @@ class GasSpace:
 
     def close(self):
         if getattr(self, "handle", None) is not None:
+            if self._colex is not None and self._colex is not self.handle:
+                self.lib.fcis_gas_space_free(self._colex)
+            self._colex = None
             self.lib.fcis_gas_space_free(self.handle)
             self.handle = None
 
@@ -240,8 +298,8 @@ def gas2fci(fcivec, gas):
     stay on the device."""
     import torch
     c, host, _ = _vector_in(fcivec, gas.ndet)
-    out = torch.empty(gas.cas.shape, dtype=torch.float64, device="cuda")
-    gas.embed_dev(c, out)
+    out = torch.empty(gas._colex_cas.shape, dtype=torch.float64, device="cuda")
+    gas.embed_colex(c, out)
     return out.cpu().numpy() if host else out
 
 
@@ -252,10 +310,10 @@ def fci2gas(fcivec, gas):
         c, host = fcivec.detach().to(device="cuda", dtype=torch.float64).contiguous(), False
     else:
         c, host = torch.from_numpy(numpy.ascontiguousarray(fcivec, dtype=numpy.float64)).cuda(), True
-    if c.numel() != gas.cas.ndet:
+    if c.numel() != gas._colex_cas.ndet:
         raise ValueError("CI vector size does not match full FCI determinant count")
     out = torch.empty(gas.ndet, dtype=torch.float64, device="cuda")
-    gas.project_dev(c.reshape(-1), out)
+    gas.project_colex(c.reshape(-1), out)
     return out.cpu().numpy() if host else out
 
 

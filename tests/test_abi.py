@@ -3,6 +3,7 @@ declared in include/fcisigma.h, and its host-only entry points and argument
 validation behave (no GPU needed: nothing here launches a kernel)."""
 
 import ctypes
+import math
 import os
 import re
 
@@ -93,3 +94,23 @@ def test_batch_manifest():
     assert len(man) == 64
     for n, (na, nb), seed in man:
         assert 6 <= n <= 12 and nb == n // 2 and na - nb in (0, 1)
+
+
+def test_gas_string_order_is_sector_major_permutation():
+    """fcis_gas_string_order: every string once, grouped by the occupation
+    tuple of the subspaces in lexicographic order, colex within a group."""
+    lib = pkg._lib.load()
+    norb = numpy.asarray([4, 8, 4], dtype=numpy.int32)
+    for k in (3, 8):
+        n = math.comb(16, k)
+        out = numpy.zeros(n, dtype=numpy.uint64)
+        st = lib.fcis_gas_string_order(3, norb.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), k,
+                                       out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+        assert st == 0
+        colex = oracle.make_strings(16, k).astype(numpy.uint64)
+        assert sorted(out.tolist()) == sorted(colex.tolist())
+        occ = [(bin(s & 0xF).count("1"), bin((s >> 4) & 0xFF).count("1"), bin(s >> 12).count("1"))
+               for s in out.tolist()]
+        rank = [lib.fcis_str2addr(16, k, s) for s in out.tolist()]
+        keys = list(zip(occ, rank))
+        assert keys == sorted(keys)
