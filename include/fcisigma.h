@@ -220,6 +220,23 @@ int fcis_csf_csf2det(const fcis_csf_t *csf, const double *d_csf, double *d_det,
 int fcis_csf_hdiag(fcis_csf_t *csf, const double *eri, const double *d_hdiag_det,
                    double *d_hdiag_csf, void *stream);
 
+/*
+ * Point-group packing of CSF vectors (csf_symm / CSFTransformer.pack_csf,
+ * pyscf/csf_fci/csfstring.py:74-103): with orbital irreps orbsym[norb]
+ * (D2h-subgroup ids 0..7, products by XOR), keep the CSFs whose
+ * configuration irrep (XOR over its singly occupied orbitals) is wfnsym;
+ * *nsym receives their count (orbsym NULL clears the packing, nsym = ncsf).
+ * pack: d_packed[v] (nsym) = the kept entries of d_full[v] (ncsf); unpack:
+ * the reverse, zero elsewhere.  irrep: the irrep of CSF `csf`.
+ */
+int fcis_csf_symmetry(fcis_csf_t *csf, const int *orbsym, int wfnsym, int64_t *nsym,
+                      void *stream);
+int fcis_csf_pack(const fcis_csf_t *csf, const double *d_full, double *d_packed, int nvec,
+                  void *stream);
+int fcis_csf_unpack(const fcis_csf_t *csf, const double *d_packed, double *d_full, int nvec,
+                    void *stream);
+int fcis_csf_irrep(const fcis_csf_t *csf, const int *orbsym, int64_t csf_index, int *irrep);
+
 /* ---- Many CAS problems per launch ---------------------------------------
  *
  * A batch of nprob independent CAS problems (spaces[p], absorbed pair

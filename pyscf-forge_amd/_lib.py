@@ -59,6 +59,10 @@ SIGNATURES = {
     "fcis_csf_det2csf": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp]),
     "fcis_csf_csf2det": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp]),
     "fcis_csf_hdiag": (ctypes.c_int, [_vp, _dp, _vp, _vp, _vp]),
+    "fcis_csf_symmetry": (ctypes.c_int, [_vp, _ip, ctypes.c_int, _i64p, _vp]),
+    "fcis_csf_pack": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp]),
+    "fcis_csf_unpack": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp]),
+    "fcis_csf_irrep": (ctypes.c_int, [_vp, _ip, _i64, _ip]),
     "fcis_batch_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _vp, _vp, _vp,
                                          ctypes.c_int, _vp]),
     "fcis_batch_free": (None, [_vp]),

@@ -16,8 +16,10 @@ transformer (pyscf/csf_fci/csfstring.py:33-120):
   (get_init_guess csf.py:51-64), exact diagonalisation when the CSF space
   fits the pspace (csf.py:416-429); ``ci0`` and the returned ``ci`` are
   determinant-basis (na, nb) arrays.
-
-Point-group symmetry (orbsym / wfnsym) is not part of this path.
+* Point-group symmetry (csf_symm.py, csfstring.py:74-103): with ``orbsym``
+  (D2h-subgroup irrep ids, products by XOR) and ``wfnsym``, CSF vectors are
+  packed to the configurations of irrep ``wfnsym`` (fcis_csf_pack /
+  _unpack on the device) and the solver works in that subspace.
 """
 
 import math
@@ -36,12 +38,75 @@ class CSFTransformer:
     """Determinant <-> CSF transform for (norb, neleca, nelecb, 2S+1)."""
 
     def __init__(self, norb, neleca, nelecb, smult, orbsym=None, wfnsym=None, max_memory=None):
-        if orbsym is not None or wfnsym is not None:
-            raise NotImplementedError("point-group symmetry is not supported by this transformer")
-        self.wfnsym = None
         self.max_memory = max_memory
         self._h = None
+        self._orbsym = None
+        self._wfnsym = None
+        self.nsym = None
         self._update_spin_cache(norb, neleca, nelecb, smult)
+        self._update_symm_cache(orbsym, wfnsym)
+
+    # -- point group (csfstring.py:74-103) -------------------------------
+    @property
+    def orbsym(self):
+        return self._orbsym
+
+    @property
+    def wfnsym(self):
+        return self._wfnsym
+
+    @wfnsym.setter
+    def wfnsym(self, value):
+        self._update_symm_cache(self._orbsym, value)
+
+    def _update_symm_cache(self, orbsym, wfnsym=None):
+        """Pack to the CSFs whose configuration irrep is wfnsym (both
+        given), or clear the packing."""
+        import ctypes
+        orbsym = None if orbsym is None else tuple(int(x) for x in orbsym)
+        wfnsym = None if wfnsym is None else int(wfnsym)
+        if orbsym is not None and len(orbsym) != self.norb:
+            raise ValueError("orbsym needs one irrep per orbital")
+        self._orbsym, self._wfnsym = orbsym, wfnsym
+        lib = _lib.load()
+        n = ctypes.c_int64()
+        if orbsym is None or wfnsym is None:
+            _lib.check(lib.fcis_csf_symmetry(self._h, None, 0, ctypes.byref(n), _lib.stream_ptr()),
+                       "fcis_csf_symmetry")
+            self.nsym = None
+            return
+        arr = numpy.ascontiguousarray(orbsym, dtype=numpy.int32)
+        _lib.check(lib.fcis_csf_symmetry(self._h, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                         wfnsym, ctypes.byref(n), _lib.stream_ptr()),
+                   "fcis_csf_symmetry")
+        self.nsym = int(n.value)
+
+    def csf_irrep(self, csf_index, orbsym=None):
+        """Irrep of the configuration of (full-space) CSF ``csf_index``."""
+        import ctypes
+        arr = numpy.ascontiguousarray(orbsym if orbsym is not None else self._orbsym,
+                                      dtype=numpy.int32)
+        out = ctypes.c_int()
+        _lib.check(_lib.load().fcis_csf_irrep(self._h, arr.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_int)), int(csf_index), ctypes.byref(out)), "fcis_csf_irrep")
+        return int(out.value)
+
+    @property
+    def symmetric(self):
+        return self.nsym is not None
+
+    @property
+    def ncsf_sym(self):
+        """Length of the (packed) CSF vectors of this transformer."""
+        return self.nsym if self.symmetric else self.ncsf
+
+    def pack_dev(self, full, out, nvec=1):
+        _lib.check(_lib.load().fcis_csf_pack(self._h, _lib.vptr(full), _lib.vptr(out), int(nvec),
+                                             _lib.stream_ptr()), "fcis_csf_pack")
+
+    def unpack_dev(self, packed, out, nvec=1):
+        _lib.check(_lib.load().fcis_csf_unpack(self._h, _lib.vptr(packed), _lib.vptr(out),
+                                               int(nvec), _lib.stream_ptr()), "fcis_csf_unpack")
 
     # -- cache ---------------------------------------------------------
     def _update_spin_cache(self, norb, neleca, nelecb, smult):
@@ -61,6 +126,7 @@ class CSFTransformer:
             _lib.check(st, "fcis_csf_create")
         self._h = h
         self._key = key
+        self.nsym = None
         ncsf = ctypes.c_int64()
         ndet = ctypes.c_int64()
         _lib.check(lib.fcis_csf_info(h, ctypes.byref(ncsf), ctypes.byref(ndet)), "fcis_csf_info")
@@ -93,7 +159,8 @@ class CSFTransformer:
 
     @property
     def n(self):
-        return self.ncsf
+        """Davidson basis size: packed CSF count under symmetry."""
+        return self.ncsf_sym
 
     def close(self):
         if getattr(self, "_h", None) is not None:
@@ -120,16 +187,26 @@ class CSFTransformer:
                    "fcis_csf_csf2det")
 
     def hop(self, plan):
-        """x (CSF) -> H x (CSF) through the determinant-space sigma plan."""
+        """x (CSF, packed under symmetry) -> H x through the
+        determinant-space sigma plan."""
         import torch
         det = torch.empty(self.ndet, dtype=torch.float64, device="cuda")
         sig = torch.empty_like(det)
+        if not self.symmetric:
+            def apply(x, out):
+                self.csf2det_dev(x, det)
+                plan(det, sig)
+                self.det2csf_dev(sig, out)
+            return apply
+        full = torch.empty(self.ncsf, dtype=torch.float64, device="cuda")
 
-        def apply(x, out):
-            self.csf2det_dev(x, det)
+        def apply_sym(x, out):
+            self.unpack_dev(x, full)
+            self.csf2det_dev(full, det)
             plan(det, sig)
-            self.det2csf_dev(sig, out)
-        return apply
+            self.det2csf_dev(sig, full)
+            self.pack_dev(full, out)
+        return apply_sym
 
     # -- pyscf-facing transforms ----------------------------------------
     def _apply(self, civec, nin, nout, fn, order, normalize, return_norm):
@@ -162,21 +239,55 @@ class CSFTransformer:
             norm = norm.cpu().numpy() if not single else float(norm)
         return (out, norm) if return_norm else out
 
+    def _det2csf_packed(self, det, out, nvec):
+        import torch
+        full = torch.empty((nvec, self.ncsf), dtype=torch.float64, device="cuda")
+        self.det2csf_dev(det, full, nvec)
+        self.pack_dev(full, out, nvec)
+
+    def _csf2det_packed(self, csf, out, nvec):
+        import torch
+        full = torch.empty((nvec, self.ncsf), dtype=torch.float64, device="cuda")
+        self.unpack_dev(csf, full, nvec)
+        self.csf2det_dev(full, out, nvec)
+
     def vec_det2csf(self, civec, order="C", normalize=True, return_norm=False):
-        """Determinant CI vector(s) -> CSF vector(s) (csfstring.py:45-52)."""
-        return self._apply(civec, self.ndet, self.ncsf, self.det2csf_dev, order, normalize,
-                           return_norm)
+        """Determinant CI vector(s) -> CSF vector(s), packed to wfnsym under
+        symmetry (csfstring.py:45-52)."""
+        fn = self._det2csf_packed if self.symmetric else self.det2csf_dev
+        return self._apply(civec, self.ndet, self.ncsf_sym, fn, order, normalize, return_norm)
 
     def vec_csf2det(self, civec, order="C", normalize=True, return_norm=False):
-        """CSF vector(s) -> flat determinant vector(s) (csfstring.py:54-60)."""
-        return self._apply(civec, self.ncsf, self.ndet, self.csf2det_dev, order, normalize,
-                           return_norm)
+        """CSF vector(s) (packed under symmetry) -> flat determinant
+        vector(s) (csfstring.py:54-60)."""
+        fn = self._csf2det_packed if self.symmetric else self.csf2det_dev
+        return self._apply(civec, self.ncsf_sym, self.ndet, fn, order, normalize, return_norm)
+
+    def _pack(self, csfvec, order, nin, nout, fn):
+        import torch
+        on_dev = isinstance(csfvec, torch.Tensor)
+        x = csfvec if on_dev else torch.as_tensor(numpy.asarray(csfvec, dtype=numpy.float64))
+        x = x.to(device="cuda", dtype=torch.float64)
+        cols = order.upper() == "F"
+        single = x.numel() == nin
+        x = x.reshape(1, nin) if single else (x.reshape(nin, -1).T if cols else x.reshape(-1, nin))
+        x = x.contiguous()
+        out = torch.empty((x.shape[0], nout), dtype=torch.float64, device="cuda")
+        fn(x, out, x.shape[0])
+        out = out.reshape(nout) if single else (out.T.contiguous() if cols else out)
+        return out if on_dev else out.cpu().numpy()
 
     def pack_csf(self, csfvec, order="C"):
-        return csfvec
+        """Full CSF vector(s) -> the wfnsym subspace (csfstring.py:74-80)."""
+        if not self.symmetric:
+            return csfvec
+        return self._pack(csfvec, order, self.ncsf, self.nsym, self.pack_dev)
 
     def unpack_csf(self, csfvec, order="C"):
-        return csfvec
+        """wfnsym-packed CSF vector(s) -> full, zero elsewhere (:82-88)."""
+        if not self.symmetric:
+            return csfvec
+        return self._pack(csfvec, order, self.nsym, self.ncsf, self.unpack_dev)
 
 
 def count_csfs(nspin, smult):
@@ -223,12 +334,16 @@ class FCISolver(_ds.FCISolver):
     """CSF-basis CAS FCI solver (csf_fci.csf_solver for CAS, csf.py:499-560).
 
     ``smult`` = 2S+1 (default: the lowest allowed by nelec and ``spin``).
-    ``ci0``/``ci`` are determinant-basis arrays; Davidson runs on CSF
-    vectors of length ``transformer.ncsf`` kept in HBM."""
+    ``orbsym`` / ``wfnsym`` (csf_symm.FCISolver, csf_symm.py:25-113):
+    restrict the solve to CSFs of irrep ``wfnsym`` (default: the irrep of
+    the lowest-diagonal CSF).  ``ci0``/``ci`` are determinant-basis arrays;
+    Davidson runs on CSF vectors (packed to ``wfnsym``) kept in HBM."""
 
-    def __init__(self, mol=None, smult=None):
+    def __init__(self, mol=None, smult=None, orbsym=None, wfnsym=None):
         super().__init__(mol)
         self.smult = smult
+        self.orbsym = orbsym
+        self.wfnsym = wfnsym
         self.transformer = None
         self.pspace_size = 200
 
@@ -243,12 +358,23 @@ class FCISolver(_ds.FCISolver):
             self.transformer = CSFTransformer(norb, nelec[0], nelec[1], self.smult)
         else:
             self.transformer._update_spin_cache(norb, nelec[0], nelec[1], self.smult)
+        self.transformer._update_symm_cache(self.orbsym, self.wfnsym)
         return self.transformer
 
+    def _packed_hdiag(self, t, hdiag_full):
+        import torch
+        if not t.symmetric:
+            return hdiag_full
+        out = torch.empty(t.nsym, dtype=torch.float64, device="cuda")
+        t.pack_dev(hdiag_full, out)
+        return out
+
     def make_hdiag_csf(self, h1e, eri, norb, nelec, hdiag_det=None, smult=None, max_memory=None):
+        """CSF diagonal, packed to wfnsym under symmetry."""
         self.norb, self.nelec = int(norb), self._nelec(nelec)
         t = self.check_transformer_cache(smult=smult)
-        return make_hdiag_csf(h1e, eri, norb, nelec, t, hdiag_det=hdiag_det)
+        return self._packed_hdiag(t, make_hdiag_csf(h1e, eri, norb, nelec, t,
+                                                    hdiag_det=hdiag_det))
 
     def _init_guess_dev(self, hdiag, nroots):
         """Unit CSF vectors at the lowest diagonal elements, the first root
@@ -288,7 +414,8 @@ class FCISolver(_ds.FCISolver):
         h1e = numpy.ascontiguousarray(h1e, dtype=numpy.float64).reshape(sp.norb, sp.norb)
         eri = _ds.pair_matrix(eri, sp.norb)
         if hdiag_csf is None:
-            hdiag_csf = make_hdiag_csf(h1e, eri, norb, self.nelec, t, hdiag_det=hdiag_det)
+            hdiag_csf = self._packed_hdiag(t, make_hdiag_csf(h1e, eri, norb, self.nelec, t,
+                                                             hdiag_det=hdiag_det))
         elif not isinstance(hdiag_csf, torch.Tensor):
             hdiag_csf = torch.as_tensor(numpy.asarray(hdiag_csf, dtype=numpy.float64)).cuda()
         addr = self._pspace_addr(t, hdiag_csf, npsp)
@@ -303,7 +430,7 @@ class FCISolver(_ds.FCISolver):
         from the top of the diagonal until their configurations span at most
         PSPACE_MAX_DET determinants."""
         import torch
-        npsp = min(int(npsp), t.ncsf)
+        npsp = min(int(npsp), t.ncsf_sym)
         order = torch.topk(hdiag_csf, npsp, largest=False).indices
         cnt = self._support_sizes(t, order)
         keep = int(numpy.searchsorted(numpy.cumsum(cnt), self.PSPACE_MAX_DET, side="right"))
@@ -312,18 +439,24 @@ class FCISolver(_ds.FCISolver):
 
     @staticmethod
     def _support(t, a, d, x):
-        """Determinant support of CSF a: (det addresses, coefficients)."""
+        """Determinant support of CSF a (packed address under symmetry):
+        (det addresses, coefficients)."""
         import torch
         x.zero_()
         x[a] = 1.0
-        t.csf2det_dev(x, d)
+        if t.symmetric:
+            full = torch.empty(t.ncsf, dtype=torch.float64, device="cuda")
+            t.unpack_dev(x, full)
+            t.csf2det_dev(full, d)
+        else:
+            t.csf2det_dev(x, d)
         nz = torch.nonzero(d).reshape(-1)
         return nz, d[nz].clone()
 
     def _support_sizes(self, t, addr):
         import torch
         d = torch.empty(t.ndet, dtype=torch.float64, device="cuda")
-        x = torch.zeros(t.ncsf, dtype=torch.float64, device="cuda")
+        x = torch.zeros(t.ncsf_sym, dtype=torch.float64, device="cuda")
         return [int(self._support(t, a, d, x)[0].numel()) for a in addr.tolist()]
 
     def _h_block(self, sp, t, h1e, eri, addr):
@@ -332,7 +465,7 @@ class FCISolver(_ds.FCISolver):
         import torch
         m = addr.numel()
         d = torch.empty(t.ndet, dtype=torch.float64, device="cuda")
-        x = torch.zeros(t.ncsf, dtype=torch.float64, device="cuda")
+        x = torch.zeros(t.ncsf_sym, dtype=torch.float64, device="cuda")
         sup = [self._support(t, a, d, x) for a in addr.tolist()]
         del d
         dets = torch.unique(torch.cat([s[0] for s in sup]))
@@ -346,9 +479,60 @@ class FCISolver(_ds.FCISolver):
         H = (U.T @ (h0 @ U)).cpu().numpy()
         return 0.5 * (H + H.T)
 
+    def guess_wfnsym(self, norb, nelec, fcivec=None, orbsym=None, wfnsym=None):
+        """Target irrep id (direct_spin1_symm.guess_wfnsym, as csf_symm.py:66
+        uses it): ``wfnsym`` if set, else the irrep of the largest
+        determinant of ``fcivec``, else that of the aufbau determinant, the
+        XOR of orbsym over its singly occupied orbitals.  None without
+        orbsym."""
+        orbsym = self.orbsym if orbsym is None else orbsym
+        wfnsym = self.wfnsym if wfnsym is None else wfnsym
+        if orbsym is None:
+            return None
+        if wfnsym is not None:
+            if isinstance(wfnsym, str):
+                raise TypeError("wfnsym must be an irrep id (int); irrep names need a "
+                                "point-group table")
+            return int(wfnsym)
+        orbsym = [int(x) for x in orbsym]
+        na, nb = self._nelec(nelec)
+        if fcivec is None:
+            w = 0
+            for i in range(min(na, nb), max(na, nb)):
+                w ^= orbsym[i]
+            return w
+        c = fcivec[0] if isinstance(fcivec, (list, tuple)) else fcivec
+        sp = _ds.get_space(norb, (na, nb))
+        c = numpy.asarray(c, dtype=numpy.float64).reshape(sp.shape)
+        ia, ib = numpy.unravel_index(int(numpy.argmax(abs(c))), sp.shape)
+        lib = _lib.load()
+        occ = int(lib.fcis_addr2str(int(norb), na, int(ia)))
+        occb = int(lib.fcis_addr2str(int(norb), nb, int(ib)))
+        w = 0
+        for i in range(int(norb)):
+            if ((occ ^ occb) >> i) & 1:
+                w ^= orbsym[i]
+        return w
+
     def kernel(self, h1e, eri, norb, nelec, ci0=None, ecore=0, nroots=None, smult=None,
                **kwargs):
-        """(e + ecore, ci) with ci in the determinant basis (csf.kernel)."""
+        """(e + ecore, ci) with ci in the determinant basis (csf.kernel;
+        with ``orbsym``, csf_symm.FCISolver.kernel, csf_symm.py:33-77:
+        orbsym / wfnsym kwargs apply to this call only)."""
+        orbsym_back, wfnsym_back = self.orbsym, self.wfnsym
+        if kwargs.get("orbsym") is not None:
+            self.orbsym = kwargs["orbsym"]
+        if kwargs.get("wfnsym") is not None:
+            self.wfnsym = kwargs["wfnsym"]
+        kwargs.pop("orbsym", None)
+        kwargs.pop("wfnsym", None)
+        try:
+            self.wfnsym = self.guess_wfnsym(norb, nelec, ci0)
+            return self._kernel(h1e, eri, norb, nelec, ci0, ecore, nroots, smult, **kwargs)
+        finally:
+            self.orbsym, self.wfnsym = orbsym_back, wfnsym_back
+
+    def _kernel(self, h1e, eri, norb, nelec, ci0, ecore, nroots, smult, **kwargs):
         import torch
         nroots = int(nroots if nroots is not None else kwargs.pop("nroots", self.nroots))
         if nroots < 1:
@@ -372,20 +556,22 @@ class FCISolver(_ds.FCISolver):
         self.norb, self.nelec = int(norb), nelec
         t = self.check_transformer_cache(smult=smult)
         sp = _ds.get_space(norb, nelec)
-        nroots = min(nroots, t.ncsf)
-        if t.ncsf == 0:
-            raise ImpossibleSpinError("no CSF with 2S+1 = %d" % self.smult)
+        nroots = min(nroots, t.ncsf_sym)
+        if t.ncsf_sym == 0:
+            raise ImpossibleSpinError("no CSF with 2S+1 = %d%s" % (
+                self.smult, "" if not t.symmetric else " and irrep %d" % t.wfnsym))
         h1e = numpy.ascontiguousarray(h1e, dtype=numpy.float64).reshape(sp.norb, sp.norb)
         eri = _ds.pair_matrix(eri, sp.norb)
         h2e = _ds.absorb_h1e(h1e, eri, norb, nelec, 0.5)
         hdiag_det = _ds._make_hdiag_dev(h1e, eri, norb, nelec)
-        hdiag = make_hdiag_csf(h1e, eri, norb, nelec, t, hdiag_det=hdiag_det)
+        hdiag = self._packed_hdiag(t, make_hdiag_csf(h1e, eri, norb, nelec, t,
+                                                     hdiag_det=hdiag_det))
 
         # pspace of the lowest-diagonal CSFs (csf.py:414-420): the whole space
         # when it is small enough, else the block of the preconditioner
         npsp = max(pspace_size, nroots)
-        if pspace_size >= t.ncsf:
-            paddr = torch.arange(t.ncsf, device="cuda")
+        if pspace_size >= t.ncsf_sym:
+            paddr = torch.arange(t.ncsf_sym, device="cuda")
         elif pspace_size > 0:
             paddr = self._pspace_addr(t, hdiag, npsp)
         else:
@@ -393,7 +579,7 @@ class FCISolver(_ds.FCISolver):
         pw = pv = None
         if paddr is not None:
             pw, pv = numpy.linalg.eigh(self._h_block(sp, t, h1e, eri, paddr))
-        if ci0 is None and not davidson_only and pspace_size >= t.ncsf:
+        if ci0 is None and not davidson_only and pspace_size >= t.ncsf_sym:
             e = pw[:nroots]
             c = [torch.from_numpy(numpy.ascontiguousarray(pv[:, k])).cuda() for k in range(nroots)]
             conv = numpy.ones(nroots, dtype=bool)
@@ -426,6 +612,14 @@ class FCISolver(_ds.FCISolver):
         return self.eci, self.ci
 
 
-def csf_solver(mol=None, smult=None):
-    """CAS CSF solver factory (pyscf.csf_fci.csf_solver for CAS)."""
-    return FCISolver(mol, smult=smult)
+def csf_solver(mol=None, smult=None, symm=None):
+    """CAS CSF solver factory (pyscf.csf_fci.csf_solver, csf_fci/__init__.py:6-12).
+    One class serves both cases: point-group symmetry switches on when the
+    solver is given ``orbsym`` (attribute or kernel kwarg); ``symm`` is
+    accepted for signature parity and taken from ``mol.symmetry`` when
+    unset, as the reference does."""
+    if mol is not None and symm is None:
+        symm = getattr(mol, "symmetry", None)
+    sol = FCISolver(mol, smult=smult)
+    sol.symm = bool(symm)
+    return sol

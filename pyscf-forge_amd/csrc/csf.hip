@@ -17,6 +17,7 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <algorithm>
 #include <new>
 #include <vector>
 
@@ -322,7 +323,86 @@ struct fcis_csf {
         uint32_t *d_mask;
         double *d_U;
         double *d_kex;
+        // point-group packing: the CSFs whose configuration has irrep wfnsym
+        int64_t nsym;
+        uint32_t *d_sym;        // [nsym] CSF indices, ascending
 };
+
+namespace {
+
+// Packed CSF vectors: packed[v][i] = full[v][sym[i]] and back (zero
+// elsewhere in the full vector).
+__global__ void k_csf_pack(int64_t nsym, int64_t ncsf, const uint32_t *__restrict__ sym,
+                           const double *__restrict__ full, double *__restrict__ packed)
+{
+        const int v = blockIdx.y;
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nsym;
+             i += (int64_t)gridDim.x * blockDim.x)
+                packed[v * nsym + i] = full[v * ncsf + sym[i]];
+}
+
+__global__ void k_csf_zero(int64_t n, double *__restrict__ x)
+{
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+             i += (int64_t)gridDim.x * blockDim.x)
+                x[i] = 0.0;
+}
+
+__global__ void k_csf_unpack(int64_t nsym, int64_t ncsf, const uint32_t *__restrict__ sym,
+                             const double *__restrict__ packed, double *__restrict__ full)
+{
+        const int v = blockIdx.y;
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nsym;
+             i += (int64_t)gridDim.x * blockDim.x)
+                full[v * ncsf + sym[i]] = packed[v * nsym + i];
+}
+
+uint64_t host_unrank(int n, int k, int64_t addr)
+{
+        uint64_t s = 0;
+        int e = k;
+        for (int p = n - 1; p >= 0 && e > 0; p--) {
+                int64_t c = fcis::binom_host(p, e);
+                if (c <= addr) {
+                        s |= 1ull << p;
+                        addr -= c;
+                        e--;
+                }
+        }
+        return s;
+}
+
+uint64_t host_deposit(uint64_t bits, uint64_t slots)
+{
+        uint64_t out = 0;
+        while (slots) {
+                uint64_t low = slots & (~slots + 1);
+                if (bits & 1) out |= low;
+                bits >>= 1;
+                slots ^= low;
+        }
+        return out;
+}
+
+// Irrep of configuration (block B, pair configuration dconf, singles
+// configuration sconf): XOR of the irreps of the singly occupied orbitals
+// (pairs contribute the totally symmetric irrep; D2h-subgroup irrep ids
+// multiply by XOR).
+int conf_irrep(const CsfTable &t, const CsfBlock &B, int64_t dconf, int64_t sconf,
+               const int *orbsym)
+{
+        const uint64_t full = (t.norb == 64) ? ~0ull : ((1ull << t.norb) - 1);
+        const uint64_t dstr = host_unrank(t.norb, B.npair, dconf);
+        uint64_t sbits = host_deposit(host_unrank(t.norb - B.npair, B.nspin, sconf), ~dstr & full);
+        int ir = 0;
+        while (sbits) {
+                ir ^= orbsym[__builtin_ctzll(sbits)];
+                sbits &= sbits - 1;
+        }
+        return ir;
+}
+
+}  // namespace
 
 static int csf_upload(fcis_csf *c, const std::vector<double> &Uall, hipStream_t st)
 {
@@ -418,6 +498,8 @@ int fcis_csf_create(fcis_csf_t **out, const fcis_space_t *s, int smult, void *st
         c->d_mask = nullptr;
         c->d_U = nullptr;
         c->d_kex = nullptr;
+        c->d_sym = nullptr;
+        c->nsym = 0;
         if (hipMalloc(&c->d_mask, 4 * (size_t)off + 4) != hipSuccess ||
             hipMalloc(&c->d_U, 8 * (Uall.size() + 1)) != hipSuccess ||
             hipMalloc(&c->d_kex, 8 * (size_t)norb * norb + 8) != hipSuccess) {
@@ -440,7 +522,106 @@ void fcis_csf_free(fcis_csf_t *c)
         (void)hipFree(c->d_mask);
         (void)hipFree(c->d_U);
         (void)hipFree(c->d_kex);
+        if (c->d_sym) (void)hipFree(c->d_sym);
         delete c;
+}
+
+int fcis_csf_symmetry(fcis_csf_t *c, const int *orbsym, int wfnsym, int64_t *nsym, void *stream)
+{
+        if (c == nullptr) {
+                fcis::set_error("fcis_csf_symmetry: null transformer");
+                return FCIS_ERR_INVALID;
+        }
+        if (c->d_sym) (void)hipFree(c->d_sym);
+        c->d_sym = nullptr;
+        c->nsym = c->ncsf;
+        if (orbsym == nullptr) {
+                if (nsym) *nsym = c->ncsf;
+                return FCIS_SUCCESS;
+        }
+        for (int p = 0; p < c->t.norb; p++) {
+                if (orbsym[p] < 0 || orbsym[p] > 7) {
+                        fcis::set_error("fcis_csf_symmetry: orbsym[%d] = %d is not a D2h-subgroup "
+                                        "irrep id (0..7)", p, orbsym[p]);
+                        return FCIS_ERR_INVALID;
+                }
+        }
+        std::vector<uint32_t> idx;
+        for (int b = 0; b < c->t.nblk; b++) {
+                const CsfBlock &B = c->t.b[b];
+                if (B.ncsf == 0) continue;
+                for (int64_t conf = 0; conf < B.nconf; conf++) {
+                        const int64_t dconf = conf / B.sconf, sconf = conf - dconf * B.sconf;
+                        if (conf_irrep(c->t, B, dconf, sconf, orbsym) != wfnsym) continue;
+                        for (int k = 0; k < B.ncsf; k++)
+                                idx.push_back((uint32_t)(B.coff + conf * B.ncsf + k));
+                }
+        }
+        c->nsym = (int64_t)idx.size();
+        if (nsym) *nsym = c->nsym;
+        if (hipMalloc(&c->d_sym, 4 * idx.size() + 4) != hipSuccess) {
+                c->d_sym = nullptr;
+                c->nsym = c->ncsf;
+                fcis::set_error("fcis_csf_symmetry: device allocation failed");
+                return FCIS_ERR_MEMORY;
+        }
+        hipStream_t st = (hipStream_t)stream;
+        if (!idx.empty())
+                FCIS_HIP(hipMemcpyAsync(c->d_sym, idx.data(), 4 * idx.size(), hipMemcpyHostToDevice, st));
+        FCIS_HIP(hipStreamSynchronize(st));
+        return FCIS_SUCCESS;
+}
+
+int fcis_csf_irrep(const fcis_csf_t *c, const int *orbsym, int64_t csf, int *irrep)
+{
+        if (c == nullptr || orbsym == nullptr || irrep == nullptr || csf < 0 || csf >= c->ncsf) {
+                fcis::set_error("fcis_csf_irrep: invalid argument");
+                return FCIS_ERR_INVALID;
+        }
+        int b = 0;
+        while (b + 1 < c->t.nblk && c->t.b[b + 1].coff <= csf) b++;
+        const CsfBlock &B = c->t.b[b];
+        const int64_t conf = (csf - B.coff) / B.ncsf;
+        const int64_t dconf = conf / B.sconf, sconf = conf - dconf * B.sconf;
+        *irrep = conf_irrep(c->t, B, dconf, sconf, orbsym);
+        return FCIS_SUCCESS;
+}
+
+int fcis_csf_pack(const fcis_csf_t *c, const double *d_full, double *d_packed, int nvec,
+                  void *stream)
+{
+        if (c == nullptr || c->d_sym == nullptr || d_full == nullptr || d_packed == nullptr ||
+            nvec < 1 || nvec > 65535) {
+                fcis::set_error("fcis_csf_pack: invalid argument (no symmetry set?)");
+                return FCIS_ERR_INVALID;
+        }
+        if (c->nsym == 0) return FCIS_SUCCESS;
+        const unsigned g = (unsigned)std::min<int64_t>((c->nsym + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_csf_pack, dim3(g, nvec), dim3(256), 0, (hipStream_t)stream, c->nsym,
+                           c->ncsf, c->d_sym, d_full, d_packed);
+        FCIS_CHECK_LAUNCH();
+        return FCIS_SUCCESS;
+}
+
+int fcis_csf_unpack(const fcis_csf_t *c, const double *d_packed, double *d_full, int nvec,
+                    void *stream)
+{
+        if (c == nullptr || c->d_sym == nullptr || d_full == nullptr || d_packed == nullptr ||
+            nvec < 1 || nvec > 65535) {
+                fcis::set_error("fcis_csf_unpack: invalid argument (no symmetry set?)");
+                return FCIS_ERR_INVALID;
+        }
+        hipStream_t st = (hipStream_t)stream;
+        const int64_t n = c->ncsf * nvec;
+        hipLaunchKernelGGL(k_csf_zero, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 8192)),
+                           dim3(256), 0, st, n, d_full);
+        FCIS_CHECK_LAUNCH();
+        if (c->nsym == 0) return FCIS_SUCCESS;
+        const unsigned g = (unsigned)std::min<int64_t>((c->nsym + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_csf_unpack, dim3(g, nvec), dim3(256), 0, st, c->nsym, c->ncsf,
+                           c->d_sym, d_packed, d_full);
+        FCIS_CHECK_LAUNCH();
+        return FCIS_SUCCESS;
 }
 
 int fcis_csf_info(const fcis_csf_t *c, int64_t *ncsf, int64_t *ndet)
