@@ -25,3 +25,4 @@ from . import csf_fci  # noqa: F401,E402
 from . import batch  # noqa: F401,E402
 from . import addons_gas  # noqa: F401,E402
 from . import fci_gas  # noqa: F401,E402
+from . import gasscf  # noqa: F401,E402

@@ -418,6 +418,21 @@ class GasSpinPlan:
         out = self.contract_dev(c, torch.empty_like(c))
         return out.cpu().numpy().reshape(shape) if host else out.reshape(shape)
 
+    def penalty(self, vector, parameters):
+        """shift * (S^2 - target) x, or shift * (S^2 - target)^2 x for a
+        target above the minimum (_spin_penalty_action, fci_gas.py:242-252),
+        with both S^2 applications on the device.  numpy in, numpy out."""
+        import torch
+        shift, target, minimum, _ = parameters
+        x, host, shape = _vector_in(vector, self.ndet)
+        d = self.contract_dev(x, torch.empty_like(x))
+        d.sub_(x, alpha=target)
+        if target >= minimum + 0.1:
+            d2 = self.contract_dev(d, torch.empty_like(d))
+            d = d2.sub_(d, alpha=target)
+        d.mul_(shift)
+        return d.cpu().numpy().reshape(shape) if host else d.reshape(shape)
+
     def diagonal_dev(self):
         import torch
         full = torch.empty(self.gas.cas.ndet, dtype=torch.float64, device="cuda")
@@ -484,6 +499,28 @@ def _spin_penalty_parameters(solver, norb, nelec):
     if not numpy.any(numpy.abs(eig - target) < 1e-10):
         raise ValueError("target S^2 is not an allowed S(S+1) value for this space")
     return shift, target, minimum, tuple(float(x) for x in eig)
+
+
+def _spin_penalty_action(contract_ss, vector, parameters):
+    """The spin penalty without H (fci_gas.py:242-252): shift * (S^2 - t) x,
+    squared for a target above the minimum; ``contract_ss`` maps a vector
+    to S^2 times it."""
+    shift, target, minimum, _ = parameters
+    delta = numpy.asarray(contract_ss(vector)) - target * vector
+    if target >= minimum + 0.1:
+        delta = numpy.asarray(contract_ss(delta)) - target * delta
+    return shift * delta
+
+
+def _spin_penalty_diagonal(spin_diagonal, parameters):
+    """Preconditioner diagonal of the penalty (fci_gas.py:255-263):
+    shift * (diag(S^2) - t), squared elementwise for the quadratic form (an
+    approximation of diag((S^2 - t)^2), as in the reference)."""
+    shift, target, minimum, _ = parameters
+    delta = spin_diagonal - target
+    if target >= minimum + 0.1:
+        delta = delta * delta
+    return shift * delta
 
 
 def fix_spin_(fciobj, shift=0.2, ss=None):
