@@ -212,8 +212,8 @@ def test_fused_and_unfused_paths_agree(precision, monkeypatch):
 
 @pytest.mark.parametrize("ne", [(2, 2), (3, 2), (1, 0)])
 def test_sigma_16_orbitals_vs_oracle(ne):
-    """norb = 16 (npair = 136): the FP64 fused kernel contracts the last 8
-    pair rows on the VALU of its 9th wavefront; sigma against the oracle."""
+    """norb = 16 (npair = 136): the fused kernels run a half-empty 9th
+    tile (rows past npair zero); sigma against the oracle."""
     n = 16
     h1e, eri, c = pkg.synthetic.make_problem(n, ne, seed=3)
     h2e = pkg.absorb_h1e(h1e, eri, n, ne, 0.5)
@@ -223,7 +223,10 @@ def test_sigma_16_orbitals_vs_oracle(ne):
         assert numpy.abs(got - ref).max() < tol * numpy.abs(ref).max(), prec
 
 
-def test_sigma_16_orbitals_fused_vs_three_kernel(monkeypatch):
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("fp32", 2e-6)])
+def test_sigma_16_orbitals_fused_vs_three_kernel(precision, tol, monkeypatch):
+    """The fused kernel (one or many alpha tiles) and the three-kernel path
+    agree at norb = 16 (npair = 136, a half-empty 9th tile)."""
     import torch
     n, ne = 16, (5, 4)
     h1e, eri = pkg.synthetic.make_problem(n, ne, seed=5, with_ci=False)
@@ -231,13 +234,20 @@ def test_sigma_16_orbitals_fused_vs_three_kernel(monkeypatch):
     sp = pkg.get_space(n, ne)
     g = torch.Generator(device="cuda").manual_seed(7)
     c = torch.randn(sp.ndet, dtype=torch.float64, device="cuda", generator=g)
-    with pkg.SigmaPlan(sp, h2e, "fp64", 1 << 33) as plan:
-        assert plan.fused
-        a = torch.empty_like(c)
-        plan(c, a)
-    monkeypatch.setenv("FCIS_DETERMINISTIC", "1")
-    with pkg.SigmaPlan(sp, h2e, "fp64", 1 << 33) as plan:
-        assert not plan.fused
-        b = torch.empty_like(c)
-        plan(c, b)
-    assert float((a - b).abs().max()) < 1e-13 * float(b.abs().max())
+    out = {}
+    for name, ws, env in (("fused", 1 << 33, {}), ("fused_tiles", 1 << 28, {}),
+                          ("three", 1 << 33, {"FCIS_DETERMINISTIC": "1"})):
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        with pkg.SigmaPlan(sp, h2e, precision, ws) as plan:
+            assert plan.fused == (name != "three")
+            if name == "fused_tiles":
+                assert plan.ntiles > 1
+            out[name] = torch.empty_like(c)
+            plan(c, out[name])
+        for k in env:
+            monkeypatch.delenv(k)
+    ref = out["three"]
+    scale = float(ref.abs().max())
+    for name in ("fused", "fused_tiles"):
+        assert float((out[name] - ref).abs().max()) < tol * scale, name
