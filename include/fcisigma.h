@@ -184,6 +184,20 @@ int fcis_spin_plan_contract(const fcis_spin_plan_t *plan, const double *d_ci0, d
 int fcis_spin_plan_diag(const fcis_spin_plan_t *plan, double *d_out, void *stream);
 void fcis_spin_plan_free(fcis_spin_plan_t *plan);
 
+/*
+ * Spin ladder operators on CAS determinant vectors: op = -1 applies
+ * S- = sum_p a+_{p beta} a_{p alpha}, from the (na, nb) space src to the
+ * (na - 1, nb + 1) space dst; op = +1 applies S+ = sum_p a+_{p alpha}
+ * a_{p beta}, to (na + 1, nb - 1).  Both spaces colex-addressed, same norb;
+ * d_ci1 (dst-sized) is overwritten.  Signs follow pyscf's string
+ * convention (a single creation or annihilation at p counts the occupied
+ * orbitals above p), with (-1)^(na - 1) (S-) or (-1)^na (S+) for passing
+ * the alpha string.  Replaces spin_op.contract_sladder
+ * (pyscf/csf_fci/spin_op.py:22-71).
+ */
+int fcis_spin_ladder(const fcis_space_t *src, const fcis_space_t *dst, int op,
+                     const double *d_ci0, double *d_ci1, void *stream);
+
 /* ---- Configuration state functions (spin-adapted basis of the CSF-FCI
  *      solver, pyscf/csf_fci) ----------------------------------------- */
 

@@ -26,3 +26,4 @@ from . import batch  # noqa: F401,E402
 from . import addons_gas  # noqa: F401,E402
 from . import fci_gas  # noqa: F401,E402
 from . import gasscf  # noqa: F401,E402
+from . import spin_op  # noqa: F401,E402

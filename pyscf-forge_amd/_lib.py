@@ -107,6 +107,7 @@ SIGNATURES = {
     "fcis_spin_plan_contract": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "fcis_spin_plan_diag": (ctypes.c_int, [_vp, _vp, _vp]),
     "fcis_spin_plan_free": (None, [_vp]),
+    "fcis_spin_ladder": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp, _vp]),
 }
 
 _LIB = None

@@ -13,6 +13,8 @@
 // workgroup per alpha string, threads over beta strings: the alpha link is
 // wave-uniform, the beta pair map row and the c row Ja are read coalesced /
 // row-local.
+//
+// The spin ladder operators S+ / S- (fcis_spin_ladder) share the file.
 #include <algorithm>
 #include <new>
 
@@ -74,6 +76,49 @@ __global__ void k_ss_diag(int64_t nstra, int64_t nb, double d0, const uint64_t *
              i += (int64_t)gridDim.x * blockDim.x) {
                 const int64_t a = i / nb;
                 out[i] = d0 - (double)__popcll(stra[a] & strb[i - a * nb]);
+        }
+}
+
+__constant__ fcis::BinomTable c_binom_spin = fcis::make_binom_table();
+
+__device__ inline uint32_t colex_rank(uint64_t s)
+{
+        uint32_t addr = 0;
+        for (int i = 1; s; i++, s &= s - 1) {
+                const int p = __ffsll((unsigned long long)s) - 1;
+                addr += p >= i ? c_binom_spin.v[p][i] : 0u;
+        }
+        return addr;
+}
+
+// S- (op = -1) or S+ (op = +1) from the (na, nb) vector c (row length
+// nbs) onto destination determinant (I, jb) of the (na + op, nb - op)
+// space: one workgroup per destination alpha string, threads over beta.
+// The source of orbital p is (I ^ p, Ib ^ p) with p empty in I and
+// occupied in Ib (S-) or the reverse (S+); its sign counts the occupied
+// orbitals above p in both source strings, which equal those of I and Ib.
+__global__ __launch_bounds__(256) void k_sladder(int op, int64_t nstra, int64_t nb, int64_t nbs,
+                                                 double fac, const uint64_t *__restrict__ stra,
+                                                 const uint64_t *__restrict__ strb,
+                                                 const double *__restrict__ c,
+                                                 double *__restrict__ out)
+{
+        for (int64_t I = blockIdx.x; I < nstra; I += gridDim.x) {
+                const uint64_t sa = stra[I];
+                for (int64_t jb = threadIdx.x; jb < nb; jb += blockDim.x) {
+                        const uint64_t sb = strb[jb];
+                        uint64_t cand = op < 0 ? (~sa & sb) : (sa & ~sb);
+                        double acc = 0.0;
+                        while (cand) {
+                                const int p = __ffsll((unsigned long long)cand) - 1;
+                                cand &= cand - 1;
+                                const uint64_t bit = 1ull << p;
+                                const double v = c[(int64_t)colex_rank(sa ^ bit) * nbs + colex_rank(sb ^ bit)];
+                                const int par = (__popcll(sa >> (p + 1)) + __popcll(sb >> (p + 1))) & 1;
+                                acc += par ? -v : v;
+                        }
+                        out[I * nb + jb] = fac * acc;
+                }
         }
 }
 
@@ -159,6 +204,35 @@ int fcis_spin_plan_diag(const fcis_spin_plan_t *p, double *d_out, void *stream)
         hipLaunchKernelGGL(k_ss_diag, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 16384)),
                            dim3(256), 0, (hipStream_t)stream, s->nstra, s->nstrb, ss_base(s),
                            s->d_stra, s->d_strb, d_out);
+        FCIS_CHECK_LAUNCH();
+        return FCIS_SUCCESS;
+}
+
+int fcis_spin_ladder(const fcis_space_t *src, const fcis_space_t *dst, int op,
+                     const double *d_ci0, double *d_ci1, void *stream)
+{
+        if (src == nullptr || dst == nullptr || d_ci0 == nullptr || d_ci1 == nullptr ||
+            d_ci0 == d_ci1 || (op != -1 && op != 1)) {
+                fcis::set_error("fcis_spin_ladder: invalid argument (op = -1 or +1, distinct "
+                                "device arrays)");
+                return FCIS_ERR_INVALID;
+        }
+        if (dst->norb != src->norb || dst->na != src->na + op || dst->nb != src->nb - op) {
+                fcis::set_error("fcis_spin_ladder: destination space must be (na %+d, nb %+d) "
+                                "of the source in the same orbitals", op, -op);
+                return FCIS_ERR_INVALID;
+        }
+        if (src->d_perma || src->d_permb || dst->d_perma || dst->d_permb) {
+                fcis::set_error("fcis_spin_ladder: colex-addressed spaces only");
+                return FCIS_ERR_INVALID;
+        }
+        const int64_t n = dst->nstra * dst->nstrb;
+        if (n == 0) return FCIS_SUCCESS;
+        // passing the alpha string: na - 1 operators for S-, na for S+
+        const double fac = ((op < 0 ? src->na - 1 : src->na) & 1) ? -1.0 : 1.0;
+        const unsigned grid = (unsigned)std::min<int64_t>(dst->nstra, 65536);
+        hipLaunchKernelGGL(k_sladder, dim3(grid), dim3(256), 0, (hipStream_t)stream, op, dst->nstra,
+                           dst->nstrb, src->nstrb, fac, dst->d_stra, dst->d_strb, d_ci0, d_ci1);
         FCIS_CHECK_LAUNCH();
         return FCIS_SUCCESS;
 }
