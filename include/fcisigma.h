@@ -251,6 +251,16 @@ int fcis_csf_unpack(const fcis_csf_t *csf, const double *d_packed, double *d_ful
                     void *stream);
 int fcis_csf_irrep(const fcis_csf_t *csf, const int *orbsym, int64_t csf_index, int *irrep);
 
+/*
+ * Configuration and spin coupling of full-space CSF csf: the bit masks of
+ * its doubly and singly occupied orbitals and its genealogical coupling
+ * string (bit i = 1: the i-th singly occupied orbital, counted from the
+ * lowest, couples up).  Replaces csfaddrs2str (csfstring.py:1169-1181),
+ * behind printable_csfstring (:1183-1207).
+ */
+int fcis_csf_describe(const fcis_csf_t *t, int64_t csf, uint64_t *docc, uint64_t *socc,
+                      uint64_t *coupling);
+
 /* ---- Many CAS problems per launch ---------------------------------------
  *
  * A batch of nprob independent CAS problems (spaces[p], absorbed pair

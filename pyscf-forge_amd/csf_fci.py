@@ -22,6 +22,7 @@ transformer (pyscf/csf_fci/csfstring.py:33-120):
   _unpack on the device) and the solver works in that subspace.
 """
 
+import itertools
 import math
 
 import numpy
@@ -98,7 +99,13 @@ class CSFTransformer:
     @property
     def ncsf_sym(self):
         """Length of the (packed) CSF vectors of this transformer."""
-        return self.nsym if self.symmetric else self.ncsf
+        return self.nsym if self.symmetric else self.ncsf_full
+
+    @property
+    def ncsf(self):
+        """CSF count, of irrep wfnsym under symmetry (csfstring.py:210-213);
+        ncsf_full counts every irrep."""
+        return self.ncsf_sym
 
     def pack_dev(self, full, out, nvec=1):
         _lib.check(_lib.load().fcis_csf_pack(self._h, _lib.vptr(full), _lib.vptr(out), int(nvec),
@@ -130,7 +137,7 @@ class CSFTransformer:
         ncsf = ctypes.c_int64()
         ndet = ctypes.c_int64()
         _lib.check(lib.fcis_csf_info(h, ctypes.byref(ncsf), ctypes.byref(ndet)), "fcis_csf_info")
-        self.ncsf = int(ncsf.value)
+        self.ncsf_full = int(ncsf.value)
         self.ndet = int(ndet.value)
 
     @property
@@ -198,7 +205,7 @@ class CSFTransformer:
                 plan(det, sig)
                 self.det2csf_dev(sig, out)
             return apply
-        full = torch.empty(self.ncsf, dtype=torch.float64, device="cuda")
+        full = torch.empty(self.ncsf_full, dtype=torch.float64, device="cuda")
 
         def apply_sym(x, out):
             self.unpack_dev(x, full)
@@ -241,13 +248,13 @@ class CSFTransformer:
 
     def _det2csf_packed(self, det, out, nvec):
         import torch
-        full = torch.empty((nvec, self.ncsf), dtype=torch.float64, device="cuda")
+        full = torch.empty((nvec, self.ncsf_full), dtype=torch.float64, device="cuda")
         self.det2csf_dev(det, full, nvec)
         self.pack_dev(full, out, nvec)
 
     def _csf2det_packed(self, csf, out, nvec):
         import torch
-        full = torch.empty((nvec, self.ncsf), dtype=torch.float64, device="cuda")
+        full = torch.empty((nvec, self.ncsf_full), dtype=torch.float64, device="cuda")
         self.unpack_dev(csf, full, nvec)
         self.csf2det_dev(full, out, nvec)
 
@@ -281,13 +288,177 @@ class CSFTransformer:
         """Full CSF vector(s) -> the wfnsym subspace (csfstring.py:74-80)."""
         if not self.symmetric:
             return csfvec
-        return self._pack(csfvec, order, self.ncsf, self.nsym, self.pack_dev)
+        return self._pack(csfvec, order, self.ncsf_full, self.nsym, self.pack_dev)
 
     def unpack_csf(self, csfvec, order="C"):
         """wfnsym-packed CSF vector(s) -> full, zero elsewhere (:82-88)."""
         if not self.symmetric:
             return csfvec
-        return self._pack(csfvec, order, self.nsym, self.ncsf, self.unpack_dev)
+        return self._pack(csfvec, order, self.nsym, self.ncsf_full, self.unpack_dev)
+
+    # -- determinant packing, projections, printing (csfstring.py:42-220) --
+    @property
+    def ndeta(self):
+        return math.comb(self.norb, self.neleca)
+
+    @property
+    def ndetb(self):
+        return math.comb(self.norb, self.nelecb)
+
+    def _det_sym_index(self):
+        """Device addresses of the determinants whose configuration has
+        irrep wfnsym (XOR of orbsym over the singly occupied orbitals, the
+        same as over every occupied spin orbital), ascending."""
+        import torch
+        key = (self._orbsym, self._wfnsym, self._key)
+        if getattr(self, "_det_idx_key", None) != key:
+            def irreps(k):
+                occ = numpy.array(list(itertools.combinations(range(self.norb), k)), dtype=numpy.int64)
+                occ = occ.reshape(-1, k)
+                val = (numpy.int64(1) << occ).sum(axis=1) if k else numpy.zeros(1, numpy.int64)
+                occ = occ[numpy.argsort(val, kind="stable")]     # colex = string value order
+                sym = numpy.asarray(self._orbsym, dtype=numpy.int64)
+                out = numpy.zeros(occ.shape[0], dtype=numpy.int64)
+                for col in range(k):
+                    out ^= sym[occ[:, col]]
+                return out
+            ia, ib = irreps(self.neleca), irreps(self.nelecb)
+            mask = (ia[:, None] ^ ib[None, :]).reshape(-1) == self._wfnsym
+            self._det_idx = torch.from_numpy(numpy.nonzero(mask)[0]).cuda()
+            self._det_idx_key = key
+        return self._det_idx
+
+    def _rows_in(self, vec, order, nin):
+        import torch
+        on_dev = isinstance(vec, torch.Tensor)
+        x = vec if on_dev else torch.as_tensor(numpy.asarray(vec, dtype=numpy.float64))
+        x = x.to(device="cuda", dtype=torch.float64)
+        cols = order.upper() == "F"
+        single = x.numel() == nin
+        x = x.reshape(1, nin) if single else (x.reshape(nin, -1).T if cols else x.reshape(-1, nin))
+        return x, on_dev, cols, single
+
+    @staticmethod
+    def _rows_out(out, on_dev, cols, single):
+        out = out.reshape(-1) if single else (out.T.contiguous() if cols else out)
+        return out if on_dev else out.cpu().numpy()
+
+    def pack_det(self, detvec, order="C"):
+        """Determinant vector(s) -> the determinants of irrep wfnsym
+        (csfstring.py:88-96); unchanged without symmetry."""
+        if not self.symmetric:
+            return detvec
+        x, on_dev, cols, single = self._rows_in(detvec, order, self.ndet)
+        return self._rows_out(x[:, self._det_sym_index()].contiguous(), on_dev, cols, single)
+
+    def unpack_det(self, detvec, order="C"):
+        """Packed determinant vector(s) -> full, zero elsewhere
+        (csfstring.py:98-103)."""
+        import torch
+        if not self.symmetric:
+            return detvec
+        idx = self._det_sym_index()
+        x, on_dev, cols, single = self._rows_in(detvec, order, idx.numel())
+        out = torch.zeros((x.shape[0], self.ndet), dtype=torch.float64, device="cuda")
+        out[:, idx] = x
+        return self._rows_out(out, on_dev, cols, single)
+
+    def project_civec(self, detarr, order="C", normalize=True, return_norm=False):
+        """Projection of determinant vector(s) on this spin (and irrep):
+        vec_csf2det(vec_det2csf(x)).  The reference leaves this
+        unimplemented (csfstring.py:42-43); ``return_norm`` gives the norm
+        of the projection before normalisation."""
+        x, norm = self.vec_det2csf(detarr, order=order, normalize=False, return_norm=True)
+        out = self.vec_csf2det(x, order=order, normalize=normalize)
+        return (out, norm) if return_norm else out
+
+    def mat_det2csf(self, mat):
+        raise NotImplementedError
+
+    def mat_csf2det(self, mat):
+        raise NotImplementedError
+
+    def _full_index(self, idx):
+        """Full-space CSF indices of packed ones (identity without symmetry)."""
+        import torch
+        idx = numpy.atleast_1d(numpy.asarray(idx, dtype=numpy.int64))
+        if not self.symmetric:
+            return idx
+        pos = torch.arange(1, self.nsym + 1, dtype=torch.float64, device="cuda")
+        full = torch.empty(self.ncsf_full, dtype=torch.float64, device="cuda")
+        self.unpack_dev(pos, full)
+        order = torch.nonzero(full).reshape(-1).cpu().numpy()
+        return order[idx]
+
+    def printable_csfstring(self, idx):
+        """'2'/'0'/'u'/'d' string (highest orbital first) of CSF(s) idx,
+        packed indices under symmetry (csfstring.py:147-155)."""
+        isarr = hasattr(idx, "__len__")
+        ret = [_printable(self._h, self.norb, int(i)) for i in self._full_index(idx)]
+        return ret if isarr else ret[0]
+
+    def printable_largest_csf(self, csfvec, npr, order="C", isdet=False, normalize=True):
+        """(strings, coefficients) of the npr largest |coefficients| of each
+        vector (csfstring.py:122-145)."""
+        if isdet:
+            csfvec = self.vec_det2csf(csfvec, order=order, normalize=normalize)
+        csfvec = self.unpack_csf(csfvec, order=order)
+        arr = numpy.asarray(csfvec.cpu().numpy() if hasattr(csfvec, "cpu") else csfvec)
+        if arr.ndim == 1:
+            vecs = [arr]
+        elif order.upper() == "C":
+            vecs = [arr[i, :] for i in range(arr.shape[0])]
+        elif order.upper() == "F":
+            vecs = [arr[:, i] for i in range(arr.shape[1])]
+        else:
+            raise RuntimeError('order must be "C" or "F"')
+        npr = min(vecs[0].size, npr)
+        idx = [numpy.argsort(-numpy.abs(v))[:npr] for v in vecs]
+        coef = [v[i] for v, i in zip(vecs, idx)]
+        strs = [[_printable(self._h, self.norb, int(k)) for k in i] for i in idx]
+        axis = int(order.upper() == "F")
+        return numpy.stack(strs, axis=axis), numpy.stack(coef, axis=axis)
+
+    def print_config(self, printer=print):
+        printer("***** CSFTransformer configuration *****")
+        printer("norb = {}".format(self.norb))
+        printer("neleca, nelecb = {}, {}".format(self.neleca, self.nelecb))
+        printer("smult = {}".format(self.smult))
+        printer("orbsym = {}".format(self.orbsym))
+        printer("wfnsym = {}".format(self.wfnsym))
+        printer("ndeta, ndetb = {}, {}".format(self.ndeta, self.ndetb))
+        printer("ncsf = {}".format(self.ncsf))
+
+
+def _printable(handle, norb, csf):
+    """One CSF as a string over the orbitals, highest first: '2' doubly
+    occupied, '0' empty, 'u' / 'd' singly occupied coupled up / down
+    (_printable_csfstring, csfstring.py:1187-1207)."""
+    import ctypes
+    d, so, cp = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+    _lib.check(_lib.load().fcis_csf_describe(handle, int(csf), ctypes.byref(d), ctypes.byref(so),
+                                             ctypes.byref(cp)), "fcis_csf_describe")
+    d, so, cp = d.value, so.value, cp.value
+    chars = []
+    k = 0
+    for p in range(norb):
+        if (d >> p) & 1:
+            chars.append("2")
+        elif (so >> p) & 1:
+            chars.append("u" if (cp >> k) & 1 else "d")
+            k += 1
+        else:
+            chars.append("0")
+    return "".join(reversed(chars))
+
+
+def printable_csfstring(norb, neleca, nelecb, smult, addrs):
+    """Strings of full-space CSF addresses (csfstring.py:1183-1185)."""
+    t = CSFTransformer(norb, neleca, nelecb, smult)
+    try:
+        return [_printable(t._h, norb, int(a)) for a in numpy.atleast_1d(addrs)]
+    finally:
+        t.close()
 
 
 def count_csfs(nspin, smult):
@@ -324,7 +495,7 @@ def make_hdiag_csf(h1e, eri, norb, nelec, transformer, hdiag_det=None, max_memor
     elif not isinstance(hdiag_det, torch.Tensor):
         hdiag_det = torch.as_tensor(numpy.asarray(hdiag_det, dtype=numpy.float64)).cuda()
     hdiag_det = hdiag_det.reshape(-1).contiguous()
-    out = torch.empty(transformer.ncsf, dtype=torch.float64, device="cuda")
+    out = torch.empty(transformer.ncsf_full, dtype=torch.float64, device="cuda")
     _lib.check(_lib.load().fcis_csf_hdiag(transformer._h, _lib.dptr(eri), _lib.vptr(hdiag_det),
                                           _lib.vptr(out), _lib.stream_ptr()), "fcis_csf_hdiag")
     return out
@@ -445,7 +616,7 @@ class FCISolver(_ds.FCISolver):
         x.zero_()
         x[a] = 1.0
         if t.symmetric:
-            full = torch.empty(t.ncsf, dtype=torch.float64, device="cuda")
+            full = torch.empty(t.ncsf_full, dtype=torch.float64, device="cuda")
             t.unpack_dev(x, full)
             t.csf2det_dev(full, d)
         else:

@@ -587,6 +587,27 @@ int fcis_csf_irrep(const fcis_csf_t *c, const int *orbsym, int64_t csf, int *irr
         return FCIS_SUCCESS;
 }
 
+int fcis_csf_describe(const fcis_csf_t *c, int64_t csf, uint64_t *docc, uint64_t *socc,
+                      uint64_t *coupling)
+{
+        if (c == nullptr || docc == nullptr || socc == nullptr || coupling == nullptr || csf < 0 ||
+            csf >= c->ncsf) {
+                fcis::set_error("fcis_csf_describe: invalid argument");
+                return FCIS_ERR_INVALID;
+        }
+        int b = 0;
+        while (b + 1 < c->t.nblk && c->t.b[b + 1].coff <= csf) b++;
+        const CsfBlock &B = c->t.b[b];
+        const int64_t rel = csf - B.coff;
+        const int64_t conf = rel / B.ncsf, k = rel - conf * B.ncsf;
+        const int64_t dconf = conf / B.sconf, sconf = conf - dconf * B.sconf;
+        const uint64_t full = (c->t.norb == 64) ? ~0ull : ((1ull << c->t.norb) - 1);
+        *docc = host_unrank(c->t.norb, B.npair, dconf);
+        *socc = host_deposit(host_unrank(c->t.norb - B.npair, B.nspin, sconf), ~*docc & full);
+        *coupling = coupling_strings(B.nspin, c->smult)[k];
+        return FCIS_SUCCESS;
+}
+
 int fcis_csf_pack(const fcis_csf_t *c, const double *d_full, double *d_packed, int nvec,
                   void *stream)
 {

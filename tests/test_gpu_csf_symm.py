@@ -109,7 +109,8 @@ def test_pack_is_the_irrep_gather(n, na, nb, sm, orbsym):
         d = t.vec_csf2det(y, normalize=False)
         assert abs(d[:, det_irr != w]).max(initial=0.0) == 0.0
         numpy.testing.assert_allclose(t.vec_det2csf(d, normalize=False), y, rtol=0, atol=1e-13)
-    assert total == t.ncsf
+    assert total == t.ncsf_full
+    assert t.ncsf == t.nsym
     t.close()
 
 
@@ -200,3 +201,84 @@ def test_irrep_count_formula():
         t = csf.CSFTransformer(n, na, nb, sm, orbsym=orbsym, wfnsym=w)
         assert t.nsym == counts[w], (w, t.nsym, counts[w])
         t.close()
+
+
+def test_pack_det_and_ncsf_under_symmetry():
+    n, na, nb, sm, orbsym = 6, 3, 3, 1, ORBSYM6
+    det_irr = _det_irreps(n, na, nb, orbsym)
+    rng = numpy.random.default_rng(4)
+    x = rng.standard_normal((2, det_irr.size))
+    t = csf.CSFTransformer(n, na, nb, sm, orbsym=orbsym, wfnsym=2)
+    assert t.ndeta == t.ndetb == 20 and t.ncsf == t.nsym == 40 and t.ncsf_full == 175
+    p = t.pack_det(x)
+    numpy.testing.assert_array_equal(p, x[:, det_irr == 2])
+    numpy.testing.assert_array_equal(t.unpack_det(p), numpy.where(det_irr == 2, x, 0.0))
+    pf = t.pack_det(numpy.asfortranarray(x.T), order="F")
+    numpy.testing.assert_array_equal(pf, x[:, det_irr == 2].T)
+    t.close()
+
+
+@pytest.mark.parametrize("n,na,nb,sm", [(6, 3, 3, 1), (5, 3, 2, 2), (6, 4, 2, 3)])
+def test_project_civec_is_the_spin_projector(n, na, nb, sm):
+    S2 = brute.spin_square_matrix(n, (na, nb))
+    w, v = numpy.linalg.eigh(0.5 * (S2 + S2.T))
+    s = 0.5 * (sm - 1)
+    P = v[:, abs(w - s * (s + 1)) < 1e-8]
+    rng = numpy.random.default_rng(n + sm)
+    x = rng.standard_normal(S2.shape[0])
+    t = csf.CSFTransformer(n, na, nb, sm)
+    y, norm = t.project_civec(x, normalize=False, return_norm=True)
+    ref = P @ (P.T @ x)
+    numpy.testing.assert_allclose(y, ref, rtol=0, atol=1e-12)
+    assert abs(norm - numpy.linalg.norm(ref)) < 1e-12
+    yn = t.project_civec(x)
+    numpy.testing.assert_allclose(yn, ref / numpy.linalg.norm(ref), rtol=0, atol=1e-12)
+    t.close()
+
+
+@pytest.mark.parametrize("n,na,nb,sm", [(6, 3, 2, 2), (6, 3, 3, 3), (5, 3, 3, 1), (7, 4, 3, 4)])
+def test_printable_csfstrings(n, na, nb, sm):
+    """Each CSF's string against its determinant support (the '2's are
+    the orbitals doubly occupied in every determinant, 'u'/'d' the singly
+    occupied ones) and its coupling against the CSF oracle's genealogical
+    strings, in order within each configuration (lowest SOMO first)."""
+    import oracle
+    import oracle.csf as ocsf
+    t = csf.CSFTransformer(n, na, nb, sm)
+    strs = t.printable_csfstring(list(range(t.ncsf)))
+    assert t.printable_csfstring(0) == strs[0]
+    assert strs == csf.printable_csfstring(n, na, nb, sm, numpy.arange(t.ncsf))
+    sa, sb = oracle.make_strings(n, na), oracle.make_strings(n, nb)
+    pos = 0
+    while pos < t.ncsf:
+        x = numpy.zeros(t.ncsf)
+        x[pos] = 1.0
+        d = t.vec_csf2det(x, normalize=False).reshape(len(sa), len(sb))
+        ia, ib = numpy.nonzero(abs(d) > 1e-14)
+        docc = {int(sa[i]) & int(sb[j]) for i, j in zip(ia, ib)}
+        socc = {int(sa[i]) ^ int(sb[j]) for i, j in zip(ia, ib)}
+        assert len(docc) == 1 and len(socc) == 1
+        docc, socc = docc.pop(), socc.pop()
+        nspin = bin(socc).count("1")
+        coups = ocsf.coupling_strings(nspin, sm)
+        for k, cp in enumerate(coups):
+            s = strs[pos + k]
+            somo = [p for p in range(n) if (socc >> p) & 1]
+            for p in range(n):
+                c = s[n - 1 - p]
+                if (docc >> p) & 1:
+                    assert c == "2"
+                elif (socc >> p) & 1:
+                    bit = (int(cp) >> somo.index(p)) & 1
+                    assert c == ("u" if bit else "d"), (s, bin(int(cp)))
+                else:
+                    assert c == "0"
+        pos += len(coups)
+    # the largest components come first
+    rng = numpy.random.default_rng(1)
+    v = rng.standard_normal(t.ncsf)
+    ps, cs = t.printable_largest_csf(v, 3)
+    order = numpy.argsort(-abs(v))[:3]
+    numpy.testing.assert_array_equal(cs[0], v[order])
+    assert list(ps[0]) == [strs[i] for i in order]
+    t.close()
