@@ -168,6 +168,11 @@ int fcis_trans_rdm12s(const fcis_space_t *space, const double *d_bra,
  */
 int fcis_pspace(const fcis_space_t *space, const double *h1e, const double *eri, int m,
                 const int64_t *d_addr, double *d_h0, void *stream);
+/* The same block with a spin-dependent one-electron part: h1a for the alpha
+ * occupations and moves, h1b for the beta ones (host, norb x norb each).
+ * Replaces FCIpspace_h0tril_uhf under csf.pspace (csf.py:305-320). */
+int fcis_pspace_uhf(const fcis_space_t *space, const double *h1a, const double *h1b,
+                    const double *eri, int m, const int64_t *d_addr, double *d_h0, void *stream);
 
 /*
  * Total spin S^2 on CAS determinant vectors: a plan holding the beta pair
@@ -197,6 +202,16 @@ void fcis_spin_plan_free(fcis_spin_plan_t *plan);
  */
 int fcis_spin_ladder(const fcis_space_t *src, const fcis_space_t *dst, int op,
                      const double *d_ci0, double *d_ci1, void *stream);
+
+/*
+ * Spin-dependent one-electron contraction on CAS determinant vectors:
+ * d_ci1 (+)= sum_pq h1a[p][q] E^alpha_pq ci0 + h1b[p][q] E^beta_pq ci0, with
+ * h1a, h1b DEVICE norb x norb arrays; accumulate = 0 overwrites d_ci1.
+ * Replaces direct_uhf.contract_1e as the CSF solver adds the spin part of
+ * a two-component h1e to sigma (csf.py:518-529).
+ */
+int fcis_contract_1e_ab(const fcis_space_t *space, const double *d_h1a, const double *d_h1b,
+                        const double *d_ci0, double *d_ci1, int accumulate, void *stream);
 
 /* ---- Configuration state functions (spin-adapted basis of the CSF-FCI
  *      solver, pyscf/csf_fci) ----------------------------------------- */

@@ -53,6 +53,8 @@ SIGNATURES = {
     "fcis_make_hdiag": (ctypes.c_int, [_vp, _dp, _dp, _vp, _vp]),
     "fcis_trans_rdm12s": (ctypes.c_int, [_vp, _vp, _vp, _dp, _dp, _dp, _dp, _dp, _vp]),
     "fcis_pspace": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int, _vp, _vp, _vp]),
+    "fcis_pspace_uhf": (ctypes.c_int, [_vp, _dp, _dp, _dp, ctypes.c_int, _vp, _vp, _vp]),
+    "fcis_contract_1e_ab": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
     "fcis_csf_create": (ctypes.c_int, [ctypes.POINTER(_vp), _vp, ctypes.c_int, _vp]),
     "fcis_csf_free": (None, [_vp]),
     "fcis_csf_info": (ctypes.c_int, [_vp, _i64p, _i64p]),

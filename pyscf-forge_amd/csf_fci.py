@@ -483,15 +483,130 @@ def count_all_csfs(norb, neleca, nelecb, smult):
     return total
 
 
+# ---- spin-dependent one-electron Hamiltonians (csf.py:28-49, 66-68, 518-529)
+def unpack_h1e_ab(h1e):
+    """(h_alpha, h_beta) of a one- or two-component h1e (csf.py:28-32)."""
+    h = numpy.asarray(h1e)
+    if h.ndim == 3 and h.shape[0] == 2:
+        return h1e[0], h1e[1]
+    return h1e, h1e
+
+
+def unpack_1RDM_ab(dm1):
+    """(dm_alpha, dm_beta) of a spin-summed or two-component 1-RDM (csf.py:34-38)."""
+    dm1 = numpy.asarray(dm1)
+    if dm1.ndim == 3 and dm1.shape[0] == 2:
+        return dm1[0], dm1[1]
+    return dm1 / 2, dm1 / 2
+
+
+def unpack_h1e_cs(h1e):
+    """(charge, spin) parts (h_a + h_b)/2, (h_a - h_b)/2 (csf.py:40-44)."""
+    h1e_a, h1e_b = unpack_h1e_ab(h1e)
+    h1e_a = numpy.asarray(h1e_a, dtype=numpy.float64)
+    h1e_b = numpy.asarray(h1e_b, dtype=numpy.float64)
+    return (h1e_a + h1e_b) / 2.0, (h1e_a - h1e_b) / 2.0
+
+
+def unpack_1RDM_cs(dm):
+    """(dm_a + dm_b, dm_a - dm_b) (csf.py:46-48)."""
+    dma, dmb = unpack_1RDM_ab(dm)
+    return dma + dmb, dma - dmb
+
+
+class _SpinTaggedPair(numpy.ndarray):
+    """Absorbed pair matrix carrying the spin part h1e_s of a two-component
+    h1e (the tag_array of csf.py:518-523)."""
+
+    def __array_finalize__(self, obj):
+        self.h1e_s = getattr(obj, "h1e_s", None)
+
+
+def _spin_part(h1e, norb):
+    """h1e_s of h1e as a (norb, norb) array, or None when it vanishes."""
+    h1e_c, h1e_s = unpack_h1e_cs(h1e)
+    h1e_s = numpy.ascontiguousarray(h1e_s, dtype=numpy.float64).reshape(norb, norb)
+    return h1e_c, (h1e_s if numpy.any(h1e_s) else None)
+
+
+def _string_diag_sums(norb, nelec, d):
+    """sum of d[p] over the occupied orbitals of each string, colex order."""
+    occ = numpy.array(list(itertools.combinations(range(norb), nelec)), dtype=numpy.int64)
+    occ = occ.reshape(-1, nelec)
+    if nelec == 0:
+        return numpy.zeros(1)
+    val = (numpy.int64(1) << occ).sum(axis=1)
+    occ = occ[numpy.argsort(val, kind="stable")]
+    return d[occ].sum(axis=1)
+
+
+def _hdiag_det_dev(h1e, eri, norb, nelec):
+    """Determinant diagonal for a one- or two-component h1e
+    (direct_uhf.make_hdiag as csf.make_hdiag_det wraps it, csf.py:66-68):
+    the spin-free diagonal of (h_a + h_b)/2 plus sum_{p in a} s_pp -
+    sum_{p in b} s_pp, s = (h_a - h_b)/2."""
+    import torch
+    norb = int(norb)
+    na, nb = _ds._unpack_nelec(nelec)
+    h1e_c, h1e_s = _spin_part(h1e, norb)
+    hd = _ds._make_hdiag_dev(h1e_c, eri, norb, (na, nb))
+    if h1e_s is not None:
+        d = numpy.diag(h1e_s)
+        corr = (_string_diag_sums(norb, na, d)[:, None] - _string_diag_sums(norb, nb, d)[None, :])
+        hd += torch.from_numpy(corr.reshape(-1)).to(hd.device)
+    return hd
+
+
+def _h1e_array(h1e, norb):
+    """h1e as a (norb, norb) or (2, norb, norb) float64 array."""
+    h = numpy.asarray(h1e, dtype=numpy.float64)
+    if h.ndim == 3 and h.shape[0] == 2:
+        return numpy.ascontiguousarray(h).reshape(2, norb, norb)
+    return numpy.ascontiguousarray(h).reshape(norb, norb)
+
+
+def make_hdiag_det(fci, h1e, eri, norb, nelec):
+    """Determinant diagonal, one- or two-component h1e (csf.py:66-68)."""
+    return _hdiag_det_dev(h1e, eri, norb, nelec).cpu().numpy()
+
+
+class _SpinSigma:
+    """Sigma plan plus the spin part of a two-component h1e:
+    out = H_c x + sum_pq s_pq (E^a_pq - E^b_pq) x."""
+
+    def __init__(self, plan, sp, h1e_s):
+        import torch
+        self.plan, self.sp = plan, sp
+        self.hs = torch.from_numpy(numpy.ascontiguousarray(h1e_s)).cuda()
+        self.mhs = -self.hs
+
+    def __call__(self, x, out):
+        self.plan(x, out)
+        _ds.contract_1e_ab_dev(self.sp, self.hs, self.mhs, x, out, accumulate=True)
+        return out
+
+
+class _SpinBasis:
+    """A Davidson basis (CSF transformer) whose H application includes the
+    spin part of h1e."""
+
+    def __init__(self, basis, sp, h1e_s):
+        self.basis, self.sp, self.h1e_s = basis, sp, h1e_s
+        self.n = basis.n
+
+    def hop(self, plan):
+        return self.basis.hop(_SpinSigma(plan, self.sp, self.h1e_s))
+
+
 def make_hdiag_csf(h1e, eri, norb, nelec, transformer, hdiag_det=None, max_memory=None):
-    """CSF-basis diagonal of H (csf.py:70-144) as a device tensor."""
+    """CSF-basis diagonal of H (csf.py:70-144) as a device tensor; h1e may
+    be two-component (h_alpha, h_beta)."""
     import torch
     norb = int(norb)
     nelec = _ds._unpack_nelec(nelec)
-    h1e = numpy.ascontiguousarray(h1e, dtype=numpy.float64).reshape(norb, norb)
     eri = numpy.ascontiguousarray(_ds.pair_matrix(eri, norb))
     if hdiag_det is None:
-        hdiag_det = _ds._make_hdiag_dev(h1e, eri, norb, nelec)
+        hdiag_det = _hdiag_det_dev(h1e, eri, norb, nelec)
     elif not isinstance(hdiag_det, torch.Tensor):
         hdiag_det = torch.as_tensor(numpy.asarray(hdiag_det, dtype=numpy.float64)).cuda()
     hdiag_det = hdiag_det.reshape(-1).contiguous()
@@ -540,6 +655,38 @@ class FCISolver(_ds.FCISolver):
         t.pack_dev(hdiag_full, out)
         return out
 
+    def make_hdiag(self, h1e, eri, norb, nelec, *args, **kwargs):
+        """Determinant diagonal, one- or two-component h1e (csf.py:502)."""
+        return make_hdiag_det(self, h1e, eri, norb, nelec)
+
+    def absorb_h1e(self, h1e, eri, norb, nelec, fac=1):
+        """Pair matrix of the charge part (h_a + h_b)/2, tagged with the spin
+        part (h_a - h_b)/2 when it is nonzero (csf.py:518-523)."""
+        h1e_c, h1e_s = _spin_part(h1e, int(norb))
+        h2 = _ds.absorb_h1e(h1e_c, eri, norb, nelec, fac)
+        if h1e_s is not None:
+            h2 = h2.view(_SpinTaggedPair)
+            h2.h1e_s = h1e_s
+        return h2
+
+    def contract_2e(self, eri, fcivec, norb, nelec, link_index=None, **kwargs):
+        """Sigma, plus sum_pq s_pq (E^a_pq - E^b_pq) c for a spin-tagged
+        pair matrix (csf.py:525-529)."""
+        import torch
+        hc = _ds.contract_2e(eri, fcivec, norb, nelec, link_index,
+                             precision=kwargs.pop("precision", "fp64"))
+        h1e_s = getattr(eri, "h1e_s", None)
+        if h1e_s is not None:
+            sp = _ds.get_space(norb, self._nelec(nelec))
+            on_dev = isinstance(fcivec, torch.Tensor)
+            x = (fcivec if on_dev else torch.as_tensor(numpy.asarray(fcivec, dtype=numpy.float64)))
+            x = x.to(device="cuda", dtype=torch.float64).contiguous().reshape(-1)
+            out = (hc if isinstance(hc, torch.Tensor)
+                   else torch.from_numpy(numpy.ascontiguousarray(hc))).to("cuda").reshape(-1)
+            _ds.contract_1e_ab_dev(sp, h1e_s, -h1e_s, x, out, accumulate=True)
+            hc = out.reshape(numpy.shape(hc)) if on_dev else out.cpu().numpy().reshape(numpy.shape(hc))
+        return hc
+
     def make_hdiag_csf(self, h1e, eri, norb, nelec, hdiag_det=None, smult=None, max_memory=None):
         """CSF diagonal, packed to wfnsym under symmetry."""
         self.norb, self.nelec = int(norb), self._nelec(nelec)
@@ -582,7 +729,7 @@ class FCISolver(_ds.FCISolver):
         self.norb, self.nelec = int(norb), self._nelec(nelec)
         t = self.check_transformer_cache(smult=kwargs.get("smult"))
         sp = _ds.get_space(norb, self.nelec)
-        h1e = numpy.ascontiguousarray(h1e, dtype=numpy.float64).reshape(sp.norb, sp.norb)
+        h1e = _h1e_array(h1e, sp.norb)
         eri = _ds.pair_matrix(eri, sp.norb)
         if hdiag_csf is None:
             hdiag_csf = self._packed_hdiag(t, make_hdiag_csf(h1e, eri, norb, self.nelec, t,
@@ -731,10 +878,13 @@ class FCISolver(_ds.FCISolver):
         if t.ncsf_sym == 0:
             raise ImpossibleSpinError("no CSF with 2S+1 = %d%s" % (
                 self.smult, "" if not t.symmetric else " and irrep %d" % t.wfnsym))
-        h1e = numpy.ascontiguousarray(h1e, dtype=numpy.float64).reshape(sp.norb, sp.norb)
+        # a two-component h1e: the charge part is absorbed, the spin part
+        # enters sigma as a one-electron term (csf.py:518-529)
+        h1e = _h1e_array(h1e, sp.norb)
         eri = _ds.pair_matrix(eri, sp.norb)
-        h2e = _ds.absorb_h1e(h1e, eri, norb, nelec, 0.5)
-        hdiag_det = _ds._make_hdiag_dev(h1e, eri, norb, nelec)
+        h1e_c, h1e_s = _spin_part(h1e, sp.norb)
+        h2e = _ds.absorb_h1e(h1e_c, eri, norb, nelec, 0.5)
+        hdiag_det = _hdiag_det_dev(h1e, eri, norb, nelec)
         hdiag = self._packed_hdiag(t, make_hdiag_csf(h1e, eri, norb, nelec, t,
                                                      hdiag_det=hdiag_det))
 
@@ -768,7 +918,9 @@ class FCISolver(_ds.FCISolver):
                                       tol=tol, tol_residual=tol_residual,
                                       max_cycle=max_cycle, max_space=max_space,
                                       lindep=lindep, level_shift=level_shift,
-                                      verbose=verbose, basis=t, precond=precond)
+                                      verbose=verbose,
+                                      basis=t if h1e_s is None else _SpinBasis(t, sp, h1e_s),
+                                      precond=precond)
             self.stats = stats
         e = numpy.asarray(e, dtype=numpy.float64).reshape(-1) + float(ecore)
         cis = [t.vec_csf2det(x.reshape(-1), normalize=False).reshape(sp.shape) for x in c]

@@ -16,7 +16,9 @@
 // with the signs of the excitation operators a+_p a_q acting on the ket's
 // strings (for two same-spin moves: the first on the ket, the second on its
 // result).  h1e and the (npair, npair) pair matrix are read from global
-// memory (at most 150 KB at 16 orbitals, L2-resident).
+// memory (at most 150 KB at 16 orbitals, L2-resident).  The one-electron
+// part may differ between the spins (h_a for alpha moves and occupations,
+// h_b for beta; the UHF pspace of csf.py:305-320); fcis_pspace is h_a = h_b.
 #include <vector>
 
 #include "common.h"
@@ -73,7 +75,8 @@ __device__ double single_elem(int p, int q, uint64_t same, uint64_t other,
 __global__ __launch_bounds__(256) void k_pspace(int n, int np, int m, const int64_t *__restrict__ addr,
                                                 int64_t nstrb, const uint64_t *__restrict__ stra,
                                                 const uint64_t *__restrict__ strb,
-                                                const double *__restrict__ h1e,
+                                                const double *__restrict__ h1a,
+                                                const double *__restrict__ h1b,
                                                 const double *__restrict__ eri, double *__restrict__ h0)
 {
         const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -87,7 +90,7 @@ __global__ __launch_bounds__(256) void k_pspace(int n, int np, int m, const int6
         const int na = __popcll(da) >> 1, nb = __popcll(db) >> 1;
         double v = 0.0;
         if (na + nb == 0) {
-                v = same_spin_diag(Ia, h1e, eri, n, np) + same_spin_diag(Ib, h1e, eri, n, np);
+                v = same_spin_diag(Ia, h1a, eri, n, np) + same_spin_diag(Ib, h1b, eri, n, np);
                 for (uint64_t a = Ia; a; a &= a - 1) {
                         const int p = __ffsll((long long)a) - 1;
                         for (uint64_t b = Ib; b; b &= b - 1) {
@@ -100,7 +103,8 @@ __global__ __launch_bounds__(256) void k_pspace(int n, int np, int m, const int6
                 const uint64_t I = alpha ? Ia : Ib, J = alpha ? Ja : Jb, d = alpha ? da : db;
                 const int p = __ffsll((long long)(d & I)) - 1;   // in I only
                 const int q = __ffsll((long long)(d & J)) - 1;   // in J only
-                v = exc_sign(J, p, q) * single_elem(p, q, I & J, alpha ? Ib : Ia, h1e, eri, n, np);
+                v = exc_sign(J, p, q) *
+                    single_elem(p, q, I & J, alpha ? Ib : Ia, alpha ? h1a : h1b, eri, n, np);
         } else if (na + nb == 2 && (na == 2 || nb == 2)) {
                 const bool alpha = na == 2;
                 const uint64_t I = alpha ? Ia : Ib, J = alpha ? Ja : Jb, d = alpha ? da : db;
@@ -125,10 +129,10 @@ __global__ __launch_bounds__(256) void k_pspace(int n, int np, int m, const int6
 
 extern "C" {
 
-int fcis_pspace(const fcis_space_t *s, const double *h1e, const double *eri, int m,
-                const int64_t *d_addr, double *d_h0, void *stream)
+int fcis_pspace_uhf(const fcis_space_t *s, const double *h1a, const double *h1b, const double *eri,
+                    int m, const int64_t *d_addr, double *d_h0, void *stream)
 {
-        if (s == nullptr || h1e == nullptr || eri == nullptr || m < 0 ||
+        if (s == nullptr || h1a == nullptr || h1b == nullptr || eri == nullptr || m < 0 ||
             (m > 0 && (d_addr == nullptr || d_h0 == nullptr))) {
                 fcis::set_error("fcis_pspace: invalid argument");
                 return FCIS_ERR_INVALID;
@@ -142,17 +146,18 @@ int fcis_pspace(const fcis_space_t *s, const double *h1e, const double *eri, int
         const int n = s->norb, np = s->npair;
         double *d_h = nullptr;
         const size_t nh = (size_t)n * n, ne = (size_t)np * np;
-        FCIS_HIP(hipMalloc(&d_h, 8 * (nh + ne)));
+        FCIS_HIP(hipMalloc(&d_h, 8 * (2 * nh + ne)));
         int ret = FCIS_SUCCESS;
-        if (hipMemcpyAsync(d_h, h1e, 8 * nh, hipMemcpyHostToDevice, st) != hipSuccess ||
-            hipMemcpyAsync(d_h + nh, eri, 8 * ne, hipMemcpyHostToDevice, st) != hipSuccess) {
+        if (hipMemcpyAsync(d_h, h1a, 8 * nh, hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(d_h + nh, h1b, 8 * nh, hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(d_h + 2 * nh, eri, 8 * ne, hipMemcpyHostToDevice, st) != hipSuccess) {
                 fcis::set_error("fcis_pspace: upload failed");
                 ret = FCIS_ERR_DEVICE;
         } else {
                 const int64_t nt = (int64_t)m * m;
                 hipLaunchKernelGGL(k_pspace, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, n,
                                    np, m, d_addr, s->nstrb, s->d_stra, s->d_strb, d_h, d_h + nh,
-                                   d_h0);
+                                   d_h + 2 * nh, d_h0);
                 hipError_t e = hipGetLastError();
                 if (e == hipSuccess) e = hipStreamSynchronize(st);
                 if (e != hipSuccess) {
@@ -162,6 +167,12 @@ int fcis_pspace(const fcis_space_t *s, const double *h1e, const double *eri, int
         }
         (void)hipFree(d_h);
         return ret;
+}
+
+int fcis_pspace(const fcis_space_t *s, const double *h1e, const double *eri, int m,
+                const int64_t *d_addr, double *d_h0, void *stream)
+{
+        return fcis_pspace_uhf(s, h1e, h1e, eri, m, d_addr, d_h0, stream);
 }
 
 }  // extern "C"

@@ -122,6 +122,39 @@ __global__ __launch_bounds__(256) void k_sladder(int op, int64_t nstra, int64_t 
         }
 }
 
+// Spin-dependent one-electron term, destination-owned: one workgroup per
+// alpha string I, threads over beta strings.  The link (cre, des) of a
+// string S gives <S'|E_{cre,des}|S> = sign, i.e. <S|E_{des,cre}|S'> = sign,
+// so the destination's own links carry h[des][cre].
+__global__ __launch_bounds__(256) void k_contract_1e_ab(int n, int nlinka, int nlinkb, int64_t nstra,
+                                                        int64_t nb, int accumulate,
+                                                        const fcis_link *__restrict__ linka,
+                                                        const fcis_link *__restrict__ linkb,
+                                                        const double *__restrict__ ha,
+                                                        const double *__restrict__ hb,
+                                                        const double *__restrict__ c,
+                                                        double *__restrict__ out)
+{
+        for (int64_t I = blockIdx.x; I < nstra; I += gridDim.x) {
+                for (int64_t jb = threadIdx.x; jb < nb; jb += blockDim.x) {
+                        double acc = accumulate ? out[I * nb + jb] : 0.0;
+                        for (int k = 0; k < nlinka; k++) {
+                                const fcis_link L = linka[(int64_t)k * nstra + I];
+                                const double v = ha[link_des(L.info) * n + link_cre(L.info)] *
+                                                 c[(int64_t)L.addr * nb + jb];
+                                acc += link_neg(L.info) ? -v : v;
+                        }
+                        for (int k = 0; k < nlinkb; k++) {
+                                const fcis_link L = linkb[(int64_t)k * nb + jb];
+                                const double v = hb[link_des(L.info) * n + link_cre(L.info)] *
+                                                 c[I * nb + L.addr];
+                                acc += link_neg(L.info) ? -v : v;
+                        }
+                        out[I * nb + jb] = acc;
+                }
+        }
+}
+
 double ss_base(const fcis_space *s)
 {
         const double sz = 0.5 * (s->na - s->nb);
@@ -204,6 +237,23 @@ int fcis_spin_plan_diag(const fcis_spin_plan_t *p, double *d_out, void *stream)
         hipLaunchKernelGGL(k_ss_diag, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 16384)),
                            dim3(256), 0, (hipStream_t)stream, s->nstra, s->nstrb, ss_base(s),
                            s->d_stra, s->d_strb, d_out);
+        FCIS_CHECK_LAUNCH();
+        return FCIS_SUCCESS;
+}
+
+int fcis_contract_1e_ab(const fcis_space_t *s, const double *d_h1a, const double *d_h1b,
+                        const double *d_ci0, double *d_ci1, int accumulate, void *stream)
+{
+        if (s == nullptr || d_h1a == nullptr || d_h1b == nullptr || d_ci0 == nullptr ||
+            d_ci1 == nullptr || d_ci0 == d_ci1) {
+                fcis::set_error("fcis_contract_1e_ab: invalid argument (distinct device arrays)");
+                return FCIS_ERR_INVALID;
+        }
+        if (s->nstra * s->nstrb == 0) return FCIS_SUCCESS;
+        const unsigned grid = (unsigned)std::min<int64_t>(s->nstra, 65536);
+        hipLaunchKernelGGL(k_contract_1e_ab, dim3(grid), dim3(256), 0, (hipStream_t)stream, s->norb,
+                           s->nlinka, s->nlinkb, s->nstra, s->nstrb, accumulate ? 1 : 0, s->d_linka,
+                           s->d_linkb, d_h1a, d_h1b, d_ci0, d_ci1);
         FCIS_CHECK_LAUNCH();
         return FCIS_SUCCESS;
 }

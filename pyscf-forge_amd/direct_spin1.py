@@ -273,17 +273,44 @@ def _make_hdiag_dev(h1e, eri, norb, nelec):
 
 def _pspace_dev(sp, h1e, eri, addr):
     """H block over the CAS determinants ``addr`` (device int64 tensor) by
-    the Slater-Condon rules (fcis_pspace), as a device (m, m) tensor."""
+    the Slater-Condon rules (fcis_pspace), as a device (m, m) tensor.  A
+    two-component h1e (h_alpha, h_beta) gives the UHF block
+    (fcis_pspace_uhf)."""
     import torch
-    h1e = numpy.ascontiguousarray(h1e, dtype=numpy.float64).reshape(sp.norb, sp.norb)
+    h = numpy.asarray(h1e, dtype=numpy.float64)
     eri = pair_matrix(eri, sp.norb)
     addr = addr.to(device="cuda", dtype=torch.int64).contiguous()
     m = addr.numel()
     h0 = torch.empty((m, m), dtype=torch.float64, device="cuda")
-    _lib.check(sp.lib.fcis_pspace(sp.handle, _lib.dptr(h1e), _lib.dptr(eri), int(m),
+    if h.ndim == 3:
+        ha = numpy.ascontiguousarray(h[0]).reshape(sp.norb, sp.norb)
+        hb = numpy.ascontiguousarray(h[1]).reshape(sp.norb, sp.norb)
+        _lib.check(sp.lib.fcis_pspace_uhf(sp.handle, _lib.dptr(ha), _lib.dptr(hb), _lib.dptr(eri),
+                                          int(m), _lib.vptr(addr), _lib.vptr(h0),
+                                          _lib.stream_ptr()), "fcis_pspace_uhf")
+        return h0
+    h = numpy.ascontiguousarray(h).reshape(sp.norb, sp.norb)
+    _lib.check(sp.lib.fcis_pspace(sp.handle, _lib.dptr(h), _lib.dptr(eri), int(m),
                                   _lib.vptr(addr), _lib.vptr(h0), _lib.stream_ptr()),
                "fcis_pspace")
     return h0
+
+
+def contract_1e_ab_dev(sp, h1a, h1b, x, out, accumulate=False):
+    """out (+)= sum_pq h1a[p,q] E^a_pq x + h1b[p,q] E^b_pq x on device
+    vectors (fcis_contract_1e_ab); h1a / h1b host or device (norb, norb)."""
+    import torch
+
+    def dev(h):
+        return torch.as_tensor(numpy.asarray(h, dtype=numpy.float64) if not isinstance(
+            h, torch.Tensor) else h).to(device="cuda", dtype=torch.float64).contiguous()
+    ha, hb = dev(h1a), dev(h1b)
+    if ha.numel() != sp.norb ** 2 or hb.numel() != sp.norb ** 2:
+        raise ValueError("one-electron arrays must be norb x norb")
+    _lib.check(sp.lib.fcis_contract_1e_ab(sp.handle, _lib.vptr(ha), _lib.vptr(hb), _lib.vptr(x),
+                                          _lib.vptr(out), 1 if accumulate else 0,
+                                          _lib.stream_ptr()), "fcis_contract_1e_ab")
+    return out
 
 
 def pspace(h1e, eri, norb, nelec, hdiag=None, np=400):
