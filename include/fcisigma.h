@@ -392,6 +392,17 @@ void fcis_gas_space_free(fcis_gas_space_t *gas);       /* gas_space_free */
  * colex within a tuple: the sector-major address order for
  * fcis_space_create_ordered.  out: host, C(sum(norb), nelec) entries. */
 int fcis_gas_string_order(int ngas, const int *norb, int nelec, uint64_t *out);
+/* Sequences of two-element rotations on fibers of a device array, the string
+ * axes of GAS blocks under an orbital rotation within each subspace
+ * (transform_ci_within_gas, fci_gas.py:1003-1091): for every fiber f (the
+ * elements d_x[base[f] + i * stride[f]], i < extent <= 16384) and rotation
+ * r in order, each pair k in [roff[r], roff[r+1]) becomes
+ *   x[left[k]] <- c x[left[k]] + s x[right[k]],
+ *   x[right[k]] <- -s x[left[k]] + c x[right[k]],   (c, s) = cs[2r], cs[2r+1],
+ * then x[i] *= sign[i].  All arrays device-resident. */
+int fcis_rotate_fibers(double *d_x, int64_t nfiber, const int64_t *d_base, const int64_t *d_stride,
+                       int extent, int nrot, const int32_t *d_roff, const int32_t *d_left,
+                       const int32_t *d_right, const double *d_cs, const double *d_sign, void *stream);
 /* Determinants, blocks and sectors (gas_space_t ndet/nblock/nsector). */
 int fcis_gas_space_info(const fcis_gas_space_t *gas, int64_t *ndet, int *nblock,
                         int *nsector);

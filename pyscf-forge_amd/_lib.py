@@ -34,6 +34,8 @@ SIGNATURES = {
     "fcis_space_create_ordered": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int,
                                                  ctypes.c_int, ctypes.c_int, _u64p, _u64p, _vp]),
     "fcis_gas_string_order": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int, _u64p]),
+    "fcis_rotate_fibers": (ctypes.c_int, [_vp, _i64, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp,
+                                          _vp, _vp, _vp, _vp, _vp]),
     "fcis_space_info": (ctypes.c_int, [_vp, _i64p, _i64p, _ip, _ip]),
     "fcis_gen_linkstr_index": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int,
                                               _vp, _vp]),

@@ -320,9 +320,66 @@ int build_rowset(fcis_gas_space *g, const std::vector<std::vector<uint8_t>> &poo
         return FCIS_SUCCESS;
 }
 
+// Adjacent-orbital Givens rotations on the string axis of a GAS tensor
+// (transform_ci_within_gas): one workgroup per fiber (a line of `extent`
+// elements at base + i * stride), staged in LDS; rotation r mixes the
+// disjoint string pairs (left, right) of its list in parallel, rotations in
+// order with a barrier between them, then the reflection signs.
+__global__ __launch_bounds__(256) void k_rotate_fibers(double *__restrict__ x, int64_t nfiber,
+                                                       const int64_t *__restrict__ base,
+                                                       const int64_t *__restrict__ stride, int extent,
+                                                       int nrot, const int32_t *__restrict__ roff,
+                                                       const int32_t *__restrict__ left,
+                                                       const int32_t *__restrict__ right,
+                                                       const double *__restrict__ cs,
+                                                       const double *__restrict__ sign)
+{
+        extern __shared__ double fib[];
+        for (int64_t f = blockIdx.x; f < nfiber; f += gridDim.x) {
+                const int64_t b0 = base[f], st = stride[f];
+                for (int i = threadIdx.x; i < extent; i += blockDim.x) fib[i] = x[b0 + i * st];
+                __syncthreads();
+                for (int r = 0; r < nrot; r++) {
+                        const double c = cs[2 * r], s = cs[2 * r + 1];
+                        for (int k = roff[r] + threadIdx.x; k < roff[r + 1]; k += blockDim.x) {
+                                const int l = left[k], q = right[k];
+                                const double a = fib[l], b = fib[q];
+                                fib[l] = c * a + s * b;
+                                fib[q] = -s * a + c * b;
+                        }
+                        __syncthreads();
+                }
+                for (int i = threadIdx.x; i < extent; i += blockDim.x) x[b0 + i * st] = fib[i] * sign[i];
+                __syncthreads();
+        }
+}
+
 }  // namespace
 
 extern "C" {
+
+int fcis_rotate_fibers(double *d_x, int64_t nfiber, const int64_t *d_base, const int64_t *d_stride,
+                       int extent, int nrot, const int32_t *d_roff, const int32_t *d_left,
+                       const int32_t *d_right, const double *d_cs, const double *d_sign, void *stream)
+{
+        if (d_x == nullptr || nfiber < 0 || extent < 1 || nrot < 0 || extent > 16384 ||
+            (nfiber > 0 && (d_base == nullptr || d_stride == nullptr || d_roff == nullptr ||
+                            d_sign == nullptr || (nrot > 0 && (d_cs == nullptr || d_left == nullptr ||
+                                                               d_right == nullptr))))) {
+                fcis::set_error("fcis_rotate_fibers: invalid argument (extent <= 16384)");
+                return FCIS_ERR_INVALID;
+        }
+        if (nfiber == 0) return FCIS_SUCCESS;
+        const size_t lds = 8 * (size_t)extent;
+        if (lds > 64 * 1024)
+                FCIS_HIP(hipFuncSetAttribute((const void *)k_rotate_fibers,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        const unsigned grid = (unsigned)std::min<int64_t>(nfiber, 65536);
+        hipLaunchKernelGGL(k_rotate_fibers, dim3(grid), dim3(256), lds, (hipStream_t)stream, d_x, nfiber,
+                           d_base, d_stride, extent, nrot, d_roff, d_left, d_right, d_cs, d_sign);
+        FCIS_CHECK_LAUNCH();
+        return FCIS_SUCCESS;
+}
 
 void fcis_gas_space_free(fcis_gas_space_t *g)
 {

@@ -13,6 +13,7 @@ than 2^32 determinants that fit in device memory.
 """
 
 import ctypes
+import itertools
 import math
 
 import numpy
@@ -266,6 +267,27 @@ class GasSpace:
 
 
 _GAS_SPACES = {}
+
+
+def _givens_factors(w):
+    """Adjacent-row Givens rotations (p, p + 1, c, s) that take the square
+    orthogonal w to a diagonal D = +-1, column by column from the bottom
+    (rows p, q <- c row_p + s row_q, -s row_p + c row_q), in the order
+    made, and diag(D)."""
+    w = numpy.array(w, dtype=numpy.float64)
+    n = w.shape[0]
+    rots = []
+    for col in range(n - 1):
+        for q in range(n - 1, col, -1):
+            p = q - 1
+            a, b = w[p, col], w[q, col]
+            if b == 0.0:
+                continue
+            r = math.hypot(a, b)
+            c, s = a / r, b / r
+            w[[p, q]] = numpy.array([[c, s], [-s, c]]) @ w[[p, q]]
+            rots.append((p, q, c, s))
+    return rots, numpy.where(numpy.diag(w) < 0, -1.0, 1.0)
 
 
 def _cached_space(norb, nelec, blocks):
@@ -1034,6 +1056,96 @@ class FCISolver(_ds.FCISolver):
         nelec = _ds._unpack_nelec(nelec, self.spin)
         dm1s, dm2s = self.make_rdm12s(ci, norb, nelec)
         return _ds.spin_square_from_rdm12s(dm1s, dm2s, nelec)
+
+    def transform_ci_within_gas(self, fcivec, norb, nelec, u):
+        """CI coefficients for mo_new = mo_old @ u, u orthogonal and block
+        diagonal over the GAS subspaces (fci_gas.py:1003-1091): each block
+        u_g = G_1^T ... G_m^T D (adjacent-orbital Givens rotations, D = +-1)
+        acts on the string axes of subspace g of every block as the
+        rotations in order, then the signs of D; on the device
+        (fcis_rotate_fibers), one launch per (subspace, electron count,
+        spin) axis.  numpy in, numpy out; the input is not modified."""
+        import torch
+        orbs, ne, _ = self._space_spec(norb, nelec)
+        norb = int(norb)
+        u = numpy.asarray(u, dtype=numpy.float64)
+        if u.shape != (norb, norb) or not numpy.all(numpy.isfinite(u)) or not numpy.allclose(
+                u.T @ u, numpy.eye(norb), atol=1e-10, rtol=0):
+            raise ValueError("GAS orbital rotation must be finite and orthogonal")
+        bounds = numpy.cumsum((0,) + tuple(orbs))
+        inside = numpy.zeros((norb, norb), dtype=bool)
+        for g in range(len(orbs)):
+            inside[bounds[g]:bounds[g + 1], bounds[g]:bounds[g + 1]] = True
+        if numpy.any(numpy.abs(u[~inside]) > 1e-12):
+            raise ValueError("orbital rotation must not mix different GAS subspaces")
+        factors = [_givens_factors(u[bounds[g]:bounds[g + 1], bounds[g]:bounds[g + 1]])
+                   for g in range(len(orbs))]
+        gas = self._space(norb, nelec)
+        c, host, shape = _vector_in(fcivec, gas.ndet)
+        x = c.clone()
+        occ, _ = gas.sector_occupations()
+        G = len(orbs)
+        axes = {}
+        for blk in gas.block_descriptors():
+            occs = [int(v) for v in occ[blk["sa"]]] + [int(v) for v in occ[blk["sb"]]]
+            dims = [math.comb(orbs[a % G], occs[a]) for a in range(2 * G)]
+            for a in range(2 * G):
+                st = int(numpy.prod(dims[a + 1:], dtype=numpy.int64))
+                outer = int(numpy.prod(dims[:a], dtype=numpy.int64))
+                base = blk["offset"] + (numpy.arange(outer, dtype=numpy.int64)[:, None] * (dims[a] * st)
+                                        + numpy.arange(st, dtype=numpy.int64)[None, :]).reshape(-1)
+                axes.setdefault((a % G, occs[a], a // G), []).append((base, st))
+        lib = _lib.load()
+        for (g, k, _spin), fibers in axes.items():
+            rots, d = factors[g]
+            strings = numpy.array(sorted(sum(1 << p for p in o) for o in
+                                         itertools.combinations(range(orbs[g]), k)), dtype=numpy.int64)
+            sign = numpy.ones(strings.size)
+            for p in numpy.flatnonzero(d < 0):
+                sign *= 1 - 2 * ((strings >> p) & 1)
+            if not rots and numpy.all(sign > 0):
+                continue
+            roff, lefts, rights, cs = [0], [], [], []
+            for p, q, cc, ss in rots:
+                left = numpy.flatnonzero(((strings >> p) & 1) & (1 - ((strings >> q) & 1)))
+                lefts.append(left)
+                rights.append(numpy.searchsorted(strings, strings[left] ^ ((1 << p) | (1 << q))))
+                roff.append(roff[-1] + left.size)
+                cs += [cc, ss]
+            dev = lambda a, t: torch.as_tensor(numpy.ascontiguousarray(a, dtype=t)).cuda()  # noqa: E731
+            base = dev(numpy.concatenate([f[0] for f in fibers]), numpy.int64)
+            stride = dev(numpy.concatenate([numpy.full(f[0].size, f[1], dtype=numpy.int64)
+                                            for f in fibers]), numpy.int64)
+            # one trailing pad keeps the pair arrays non-empty (no pairs when the
+            # subspace is empty or full); roff bounds every rotation's range
+            cat = (lambda v: numpy.concatenate(v + [numpy.zeros(1, dtype=numpy.int64)]))
+            t_roff = dev(roff, numpy.int32)
+            t_left, t_right = dev(cat(lefts), numpy.int32), dev(cat(rights), numpy.int32)
+            t_cs, t_sign = dev(cs if cs else [0.0], numpy.float64), dev(sign, numpy.float64)
+            _lib.check(lib.fcis_rotate_fibers(_lib.vptr(x), int(base.numel()), _lib.vptr(base),
+                                              _lib.vptr(stride), int(strings.size), len(rots),
+                                              _lib.vptr(t_roff), _lib.vptr(t_left),
+                                              _lib.vptr(t_right), _lib.vptr(t_cs),
+                                              _lib.vptr(t_sign), _lib.stream_ptr()),
+                       "fcis_rotate_fibers")
+            torch.cuda.current_stream().synchronize()
+        return x.cpu().numpy().reshape(shape) if host else x.reshape(shape)
+
+    def transform_ci_for_orbital_rotation(self, fcivec, norb, nelec, u):
+        raise NotImplementedError("arbitrary active-orbital rotations do not preserve a GAS space")
+
+    def dump_flags(self, verbose=None):
+        """FCI settings and the spin-penalty ones (fci_gas.py:805-814)."""
+        log = print if (verbose if verbose is not None else self.verbose) else (lambda *a: None)
+        log("******** %s ********" % self.__class__)
+        log("gas_orbs = %s, gas_restr_type = %s" % (self.gas_orbs, self.gas_restr_type))
+        log("max. cycles = %d, conv_tol = %g, max_space = %d, precision = %s" % (
+            self.max_cycle, self.conv_tol, self.max_space, self.precision))
+        if getattr(self, "ss_penalty", None) is not None:
+            target = getattr(self, "ss_value", None)
+            log("spin penalty shift = %g" % self.ss_penalty)
+            log("target S^2 = %s" % ("minimum" if target is None else target))
+        return self
 
     def to_gpu(self, *args, **kwargs):
         """Already on the GPU."""
