@@ -596,6 +596,7 @@ class FCISolver:
         lindep = option("lindep", self.lindep)
         level_shift = option("level_shift", self.level_shift)
         davidson_only = bool(option("davidson_only", self.davidson_only))
+        pspace_size = int(option("pspace_size", self.pspace_size))
         precision = option("precision", self.precision)
         verbose = option("verbose", self.verbose)
         nelec = self._nelec(nelec)
@@ -609,16 +610,26 @@ class FCISolver:
         hdiag = _make_hdiag_dev(h1e, eri, norb, nelec)
 
         if (ci0 is None and not davidson_only and sp.ndet <= min(PSPACE_MATVEC_MAX,
-                                                                 self.pspace_size)):
+                                                                 pspace_size)):
             e, c = self._exact_small(sp, h1e, eri, nroots)
             conv = numpy.ones(nroots, dtype=bool)
         else:
             from .mixed import solve
+            from .davidson import make_pspace_precond
+            # pyscf's direct_spin1 kernel preconditions with the exact block
+            # of the pspace_size lowest-diagonal determinants (Olsen's
+            # correction); pspace_size = 0 gives the diagonal preconditioner
+            precond = None
+            npsp = min(int(pspace_size), sp.ndet)
+            if npsp > 0 and sp.ndet > PSPACE_MATVEC_MAX:
+                addr = torch.topk(hdiag, npsp, largest=False).indices
+                pw, pv = numpy.linalg.eigh(_pspace_dev(sp, h1e, eri, addr).cpu().numpy())
+                precond = make_pspace_precond(hdiag, pw, pv, addr, level_shift)
             e, c, conv, stats = solve(self, sp, h2e, hdiag, nroots, ci0, precision,
                                       tol=tol, tol_residual=tol_residual,
                                       max_cycle=max_cycle, max_space=max_space,
                                       lindep=lindep, level_shift=level_shift,
-                                      verbose=verbose)
+                                      verbose=verbose, precond=precond)
             self.stats = stats
         e = numpy.asarray(e, dtype=numpy.float64).reshape(-1) + float(ecore)
         cis = [x.reshape(sp.shape) for x in c]
