@@ -27,3 +27,4 @@ from . import addons_gas  # noqa: F401,E402
 from . import fci_gas  # noqa: F401,E402
 from . import gasscf  # noqa: F401,E402
 from . import spin_op  # noqa: F401,E402
+from . import csf_symm  # noqa: F401,E402

@@ -655,6 +655,19 @@ class FCISolver(_ds.FCISolver):
         t.pack_dev(hdiag_full, out)
         return out
 
+    def log_transformer_cache(self, tverbose=0, **kwargs):
+        """Print the transformer settings (csf.py:542-552); tverbose > 0
+        prints, 0 is silent."""
+        if kwargs:
+            self.__dict__.update(kwargs)
+            self.check_transformer_cache()
+        if self.transformer is None:
+            return
+        self.transformer.print_config(print if tverbose else (lambda *a, **k: None))
+
+    def print_transformer_cache(self, **kwargs):
+        return self.log_transformer_cache(10, **kwargs)
+
     def make_hdiag(self, h1e, eri, norb, nelec, *args, **kwargs):
         """Determinant diagonal, one- or two-component h1e (csf.py:502)."""
         return make_hdiag_det(self, h1e, eri, norb, nelec)
@@ -694,7 +707,8 @@ class FCISolver(_ds.FCISolver):
         return self._packed_hdiag(t, make_hdiag_csf(h1e, eri, norb, nelec, t,
                                                     hdiag_det=hdiag_det))
 
-    def _init_guess_dev(self, hdiag, nroots):
+    @staticmethod
+    def _init_guess_dev(hdiag, nroots):
         """Unit CSF vectors at the lowest diagonal elements, the first root
         with +-1e-5 at its ends (direct_spin1._get_init_guess applied to
         the CSF diagonal, as csf.get_init_guess does, csf.py:51-64)."""
@@ -933,6 +947,41 @@ class FCISolver(_ds.FCISolver):
             self.eci = e
             self.ci = [x.cpu().numpy() for x in cis]
         return self.eci, self.ci
+
+
+def get_init_guess(norb, nelec, nroots, hdiag_csf, transformer):
+    """Determinant-basis guesses from the (packed) CSF diagonal
+    (csf.py:51-64)."""
+    import torch
+    if transformer.ncsf < nroots:
+        raise AssertionError("Can't find {} roots among only {} CSFs of symmetry {}".format(
+            nroots, transformer.ncsf, transformer.wfnsym))
+    hd = hdiag_csf if isinstance(hdiag_csf, torch.Tensor) else torch.as_tensor(
+        numpy.asarray(hdiag_csf, dtype=numpy.float64)).cuda()
+    if hd.numel() != transformer.ncsf:
+        hd = transformer.pack_csf(hd)
+    guesses = FCISolver._init_guess_dev(hd, nroots)
+    return [transformer.vec_csf2det(g).cpu().numpy() for g in guesses]
+
+
+def pspace(fci, h1e, eri, norb, nelec, transformer, hdiag_det=None, hdiag_csf=None, npsp=200,
+           max_memory=None):
+    """(csf_addr, h0) of the npsp lowest-diagonal CSFs (csf.py:246-366)."""
+    fci.transformer = transformer
+    return fci.pspace(h1e, eri, norb, nelec, hdiag_det=hdiag_det, hdiag_csf=hdiag_csf, npsp=npsp,
+                      smult=transformer.smult)
+
+
+def kernel(fci, h1e, eri, norb, nelec, smult=None, idx_sym=None, ci0=None, transformer=None,
+           **kwargs):
+    """(e, ci) of ``fci`` in the CSF basis (csf.py:368-497); the point-group
+    subspace comes from the solver's orbsym / wfnsym (idx_sym, the
+    reference's boolean CSF mask, is accepted only as None)."""
+    if idx_sym is not None:
+        raise NotImplementedError("pass orbsym / wfnsym to the solver instead of idx_sym")
+    if transformer is not None:
+        fci.transformer = transformer
+    return fci.kernel(h1e, eri, norb, nelec, ci0=ci0, smult=smult, **kwargs)
 
 
 def csf_solver(mol=None, smult=None, symm=None):

@@ -282,3 +282,28 @@ def test_printable_csfstrings(n, na, nb, sm):
     numpy.testing.assert_array_equal(cs[0], v[order])
     assert list(ps[0]) == [strs[i] for i in order]
     t.close()
+
+
+def test_module_level_csf_api():
+    """csf.get_init_guess / pspace / kernel (csf.py:51-64, 246, 368) and
+    csf_symm.FCISolver as module functions over a solver."""
+    n, na, nb, sm, orbsym = 6, 3, 3, 1, ORBSYM6
+    h1, g = _symm_problem(orbsym, 21)
+    sol = pkg.csf_symm.FCISolver(smult=sm, orbsym=orbsym, wfnsym=0)
+    t = sol.check_transformer_cache(n, (na, nb))
+    hd = sol.make_hdiag_csf(h1, g, n, (na, nb))
+    guesses = csf.get_init_guess(n, (na, nb), 2, hd, t)
+    assert len(guesses) == 2
+    det_irr = _det_irreps(n, na, nb, orbsym)
+    for v in guesses:
+        assert abs(v.reshape(-1)[det_irr != 0]).max(initial=0.0) == 0.0
+    addr, h0 = csf.pspace(sol, h1, g, n, (na, nb), t, npsp=10)
+    assert h0.shape == (addr.size, addr.size) and addr.size <= 10
+    e, _ = csf.kernel(sol, h1, g, n, (na, nb), smult=sm, nroots=1)
+    ref, _ = _levels(h1, g, n, na, nb, sm, orbsym, 0)
+    assert abs(e - ref[0]) < 1e-9
+    with pytest.raises(NotImplementedError):
+        csf.kernel(sol, h1, g, n, (na, nb), idx_sym=numpy.ones(t.ncsf_full, dtype=bool))
+    lines = []
+    sol.transformer.print_config(lines.append)
+    assert any("ncsf = %d" % t.ncsf in x for x in lines)
