@@ -452,6 +452,47 @@ def _printable(handle, norb, csf):
     return "".join(reversed(chars))
 
 
+def _with_transformer(norb, neleca, nelecb, smult, fn):
+    t = CSFTransformer(norb, neleca, nelecb, smult)
+    try:
+        return fn(t)
+    finally:
+        t.close()
+
+
+def transform_civec_det2csf(detarr, norb, neleca, nelecb, smult, csd_mask=None,
+                            vec_on_cols=False, do_normalize=True, max_memory=None):
+    """(csfarr, norm) (csfstring.py:341-414), on the device."""
+    order = "F" if vec_on_cols else "C"
+    return _with_transformer(norb, neleca, nelecb, smult, lambda t: t.vec_det2csf(
+        detarr, order=order, normalize=do_normalize, return_norm=True))
+
+
+def transform_civec_csf2det(csfarr, norb, neleca, nelecb, smult, csd_mask=None,
+                            vec_on_cols=False, do_normalize=True, max_memory=None):
+    """(detarr, norm) (csfstring.py:416-487), on the device."""
+    order = "F" if vec_on_cols else "C"
+    return _with_transformer(norb, neleca, nelecb, smult, lambda t: t.vec_csf2det(
+        csfarr, order=order, normalize=do_normalize, return_norm=True))
+
+
+def check_spinstate_norm(detarr, norb, neleca, nelecb, smult, csd_mask=None):
+    """Norm of the projection of detarr on 2S+1 = smult (csfstring.py:297-299)."""
+    return transform_civec_det2csf(detarr, norb, neleca, nelecb, smult)[1]
+
+
+def project_civec_csf(detarr, norb, neleca, nelecb, smult, csd_mask=None, max_memory=None):
+    """(normalised projection on 2S+1 = smult in the determinant basis,
+    its norm) (csfstring.py:302-339); the input shape is kept."""
+    shape = numpy.shape(detarr)
+
+    def fn(t):
+        out, norm = t.project_civec(numpy.asarray(detarr).reshape(-1), normalize=True,
+                                    return_norm=True)
+        return out.reshape(shape), norm
+    return _with_transformer(norb, neleca, nelecb, smult, fn)
+
+
 def printable_csfstring(norb, neleca, nelecb, smult, addrs):
     """Strings of full-space CSF addresses (csfstring.py:1183-1185)."""
     t = CSFTransformer(norb, neleca, nelecb, smult)

@@ -307,3 +307,24 @@ def test_module_level_csf_api():
     lines = []
     sol.transformer.print_config(lines.append)
     assert any("ncsf = %d" % t.ncsf in x for x in lines)
+
+
+def test_module_level_transforms():
+    """transform_civec_det2csf / _csf2det, check_spinstate_norm and
+    project_civec_csf (csfstring.py:297-487) against the transformer and
+    the brute-force spin projector."""
+    n, na, nb, sm = 6, 3, 3, 3
+    rng = numpy.random.default_rng(8)
+    S2 = brute.spin_square_matrix(n, (na, nb))
+    w, v = numpy.linalg.eigh(0.5 * (S2 + S2.T))
+    P = v[:, abs(w - 2.0) < 1e-8]
+    x = rng.standard_normal(S2.shape[0])
+    ref = P @ (P.T @ x)
+    y, norm = csf.transform_civec_det2csf(x, n, na, nb, sm, do_normalize=False)
+    assert abs(norm - numpy.linalg.norm(ref)) < 1e-12
+    assert abs(csf.check_spinstate_norm(x, n, na, nb, sm) - norm) < 1e-12
+    d, _ = csf.transform_civec_csf2det(y, n, na, nb, sm, do_normalize=False)
+    numpy.testing.assert_allclose(d, ref, rtol=0, atol=1e-12)
+    p, pn = csf.project_civec_csf(x.reshape(20, 20), n, na, nb, sm)
+    assert p.shape == (20, 20) and abs(pn - numpy.linalg.norm(ref)) < 1e-12
+    numpy.testing.assert_allclose(p.reshape(-1), ref / numpy.linalg.norm(ref), rtol=0, atol=1e-12)
