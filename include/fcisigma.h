@@ -276,6 +276,19 @@ int fcis_csf_irrep(const fcis_csf_t *csf, const int *orbsym, int64_t csf_index, 
 int fcis_csf_describe(const fcis_csf_t *t, int64_t csf, uint64_t *docc, uint64_t *socc,
                       uint64_t *coupling);
 
+/*
+ * Determinant supports of m full-space CSFs (device d_csf[m]): row i of the
+ * device arrays d_addr / d_coef [m][maxdet] holds the determinant addresses
+ * and coefficients of CSF i (nonzero entries, CSD order), d_count[i] their
+ * number (-1: index out of range).  maxdet >= fcis_csf_max_support (the
+ * largest configuration's determinant count).  The configurations' rows of
+ * U that the reference gathers for its CSF pspace (csf.py:282-366,
+ * transform_opmat_det2csf_pspace csfstring.py:676-787).
+ */
+int fcis_csf_max_support(const fcis_csf_t *t, int *maxdet);
+int fcis_csf_support(const fcis_csf_t *t, int m, const int64_t *d_csf, int maxdet, int64_t *d_addr,
+                     double *d_coef, int *d_count, void *stream);
+
 /* ---- Many CAS problems per launch ---------------------------------------
  *
  * A batch of nprob independent CAS problems (spaces[p], absorbed pair

@@ -68,6 +68,8 @@ SIGNATURES = {
     "fcis_csf_unpack": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp]),
     "fcis_csf_irrep": (ctypes.c_int, [_vp, _ip, _i64, _ip]),
     "fcis_csf_describe": (ctypes.c_int, [_vp, _i64, _u64p, _u64p, _u64p]),
+    "fcis_csf_max_support": (ctypes.c_int, [_vp, _ip]),
+    "fcis_csf_support": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp, _vp]),
     "fcis_batch_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _vp, _vp, _vp,
                                          ctypes.c_int, _vp]),
     "fcis_batch_free": (None, [_vp]),

@@ -378,17 +378,49 @@ class CSFTransformer:
     def mat_csf2det(self, mat):
         raise NotImplementedError
 
-    def _full_index(self, idx):
-        """Full-space CSF indices of packed ones (identity without symmetry)."""
+    def full_index_dev(self, idx):
+        """Full-space CSF indices (device int64) of packed ones (identity
+        without symmetry)."""
         import torch
-        idx = numpy.atleast_1d(numpy.asarray(idx, dtype=numpy.int64))
+        idx = torch.as_tensor(idx).to(device="cuda", dtype=torch.int64).reshape(-1)
         if not self.symmetric:
             return idx
-        pos = torch.arange(1, self.nsym + 1, dtype=torch.float64, device="cuda")
-        full = torch.empty(self.ncsf_full, dtype=torch.float64, device="cuda")
-        self.unpack_dev(pos, full)
-        order = torch.nonzero(full).reshape(-1).cpu().numpy()
-        return order[idx]
+        key = (self._key, self._orbsym, self._wfnsym)
+        if getattr(self, "_sym_full_key", None) != key:
+            pos = torch.arange(1, self.nsym + 1, dtype=torch.float64, device="cuda")
+            full = torch.empty(self.ncsf_full, dtype=torch.float64, device="cuda")
+            self.unpack_dev(pos, full)
+            self._sym_full = torch.nonzero(full).reshape(-1)
+            self._sym_full_key = key
+        return self._sym_full[idx]
+
+    def _full_index(self, idx):
+        idx = numpy.atleast_1d(numpy.asarray(idx, dtype=numpy.int64))
+        return self.full_index_dev(idx).cpu().numpy()
+
+    @property
+    def max_support(self):
+        """Largest number of determinants in one configuration."""
+        import ctypes
+        v = ctypes.c_int()
+        _lib.check(_lib.load().fcis_csf_max_support(self._h, ctypes.byref(v)),
+                   "fcis_csf_max_support")
+        return v.value
+
+    def supports(self, idx):
+        """Determinant supports of CSFs ``idx`` (packed under symmetry):
+        device (addr [m, maxdet], coef [m, maxdet], count [m]), row i the
+        nonzero entries of CSF i's determinant expansion."""
+        import torch
+        full = self.full_index_dev(idx)
+        m, mx = int(full.numel()), self.max_support
+        A = torch.empty((m, mx), dtype=torch.int64, device="cuda")
+        C = torch.empty((m, mx), dtype=torch.float64, device="cuda")
+        n = torch.empty(m, dtype=torch.int32, device="cuda")
+        _lib.check(_lib.load().fcis_csf_support(self._h, m, _lib.vptr(full), mx, _lib.vptr(A),
+                                                _lib.vptr(C), _lib.vptr(n), _lib.stream_ptr()),
+                   "fcis_csf_support")
+        return A, C, n
 
     def printable_csfstring(self, idx):
         """'2'/'0'/'u'/'d' string (highest orbital first) of CSF(s) idx,
@@ -811,43 +843,23 @@ class FCISolver(_ds.FCISolver):
         return torch.sort(order[:keep]).values
 
     @staticmethod
-    def _support(t, a, d, x):
-        """Determinant support of CSF a (packed address under symmetry):
-        (det addresses, coefficients)."""
-        import torch
-        x.zero_()
-        x[a] = 1.0
-        if t.symmetric:
-            full = torch.empty(t.ncsf_full, dtype=torch.float64, device="cuda")
-            t.unpack_dev(x, full)
-            t.csf2det_dev(full, d)
-        else:
-            t.csf2det_dev(x, d)
-        nz = torch.nonzero(d).reshape(-1)
-        return nz, d[nz].clone()
-
-    def _support_sizes(self, t, addr):
-        import torch
-        d = torch.empty(t.ndet, dtype=torch.float64, device="cuda")
-        x = torch.zeros(t.ncsf_sym, dtype=torch.float64, device="cuda")
-        return [int(self._support(t, a, d, x)[0].numel()) for a in addr.tolist()]
+    def _support_sizes(t, addr):
+        return t.supports(addr)[2].cpu().numpy().astype(numpy.int64)
 
     def _h_block(self, sp, t, h1e, eri, addr):
         """H over the CSFs ``addr``: U^T H_det U with H_det the Slater-Condon
         block over the union of their determinant supports (fcis_pspace)."""
         import torch
         m = addr.numel()
-        d = torch.empty(t.ndet, dtype=torch.float64, device="cuda")
-        x = torch.zeros(t.ncsf_sym, dtype=torch.float64, device="cuda")
-        sup = [self._support(t, a, d, x) for a in addr.tolist()]
-        del d
-        dets = torch.unique(torch.cat([s[0] for s in sup]))
+        A, C, n = t.supports(addr)
+        valid = torch.arange(A.shape[1], device=A.device)[None, :] < n[:, None].to(torch.int64)
+        dets = torch.unique(A[valid])
         if dets.numel() > max(self.PSPACE_MAX_DET, 1):
             raise ValueError("pspace of %d CSFs spans %d determinants (limit %d)"
                              % (m, dets.numel(), self.PSPACE_MAX_DET))
         U = torch.zeros((dets.numel(), m), dtype=torch.float64, device="cuda")
-        for k, (nz, cf) in enumerate(sup):
-            U[torch.searchsorted(dets, nz), k] = cf
+        cols = torch.arange(m, device=A.device)[:, None].expand_as(A)
+        U[torch.searchsorted(dets, A[valid]), cols[valid]] = C[valid]
         h0 = _ds._pspace_dev(sp, h1e, eri, dets)
         H = (U.T @ (h0 @ U)).cpu().numpy()
         return 0.5 * (H + H.T)

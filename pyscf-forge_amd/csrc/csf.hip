@@ -170,6 +170,40 @@ __global__ void k_csf2det(CsfTable t, int nvec, int64_t ndet_all, int64_t ncsf_a
         det[(int64_t)v * ndet_all + mask[pos]] = acc;
 }
 
+// Determinant supports of m CSFs (full-space indices): for CSF i the
+// determinant addresses and coefficients of its nonzero U column entries,
+// in CSD order, compacted into row i of [m][maxdet]; count[i] entries, -1
+// for an index out of range.  One thread per CSF.
+__global__ void k_csf_support(CsfTable t, int m, int64_t ncsf_all, const int64_t *__restrict__ csf,
+                              const uint32_t *__restrict__ mask, const double *__restrict__ U,
+                              int maxdet, int64_t *__restrict__ addr, double *__restrict__ coef,
+                              int *__restrict__ count)
+{
+        const int i = blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= m) return;
+        const int64_t c = csf[i];
+        if (c < 0 || c >= ncsf_all) {
+                count[i] = -1;
+                return;
+        }
+        const CsfBlock &B = t.b[find_block_csf(t, c)];
+        const int64_t li = c - B.coff;
+        const int64_t conf = li / B.ncsf;
+        const int k = (int)(li - conf * B.ncsf);
+        const uint32_t *mp = mask + B.off + conf * B.ndet;
+        const double *u = U + B.uoff + k;
+        int n = 0;
+        for (int d = 0; d < B.ndet && n < maxdet; d++) {
+                const double v = u[(int64_t)d * B.ncsf];
+                if (v != 0.0) {
+                        addr[(int64_t)i * maxdet + n] = mp[d];
+                        coef[(int64_t)i * maxdet + n] = v;
+                        n++;
+                }
+        }
+        count[i] = n;
+}
+
 // diag(U^T H_conf U): H_conf diagonal = hdiag_det, off-diagonal between spin
 // dets that swap the spins of singles i (up in d) and j (down in d):
 // sgn * (o_i o_j | o_j o_i), sgn = -(-1)^(i+j) (csfstring.c:295-361, with the
@@ -584,6 +618,37 @@ int fcis_csf_irrep(const fcis_csf_t *c, const int *orbsym, int64_t csf, int *irr
         const int64_t conf = (csf - B.coff) / B.ncsf;
         const int64_t dconf = conf / B.sconf, sconf = conf - dconf * B.sconf;
         *irrep = conf_irrep(c->t, B, dconf, sconf, orbsym);
+        return FCIS_SUCCESS;
+}
+
+int fcis_csf_max_support(const fcis_csf_t *c, int *maxdet)
+{
+        if (c == nullptr || maxdet == nullptr) {
+                fcis::set_error("fcis_csf_max_support: invalid argument");
+                return FCIS_ERR_INVALID;
+        }
+        int mx = 1;
+        for (int b = 0; b < c->t.nblk; b++)
+                if (c->t.b[b].ncsf > 0 && c->t.b[b].ndet > mx) mx = c->t.b[b].ndet;
+        *maxdet = mx;
+        return FCIS_SUCCESS;
+}
+
+int fcis_csf_support(const fcis_csf_t *c, int m, const int64_t *d_csf, int maxdet, int64_t *d_addr,
+                     double *d_coef, int *d_count, void *stream)
+{
+        int need = 0;
+        if (c == nullptr || m < 0 || (m > 0 && (d_csf == nullptr || d_addr == nullptr ||
+                                                d_coef == nullptr || d_count == nullptr)) ||
+            fcis_csf_max_support(c, &need) != FCIS_SUCCESS || maxdet < need) {
+                fcis::set_error("fcis_csf_support: invalid argument (maxdet below the largest "
+                                "configuration's determinant count?)");
+                return FCIS_ERR_INVALID;
+        }
+        if (m == 0) return FCIS_SUCCESS;
+        hipLaunchKernelGGL(k_csf_support, dim3((m + 63) / 64), dim3(64), 0, (hipStream_t)stream, c->t, m,
+                           c->ncsf, d_csf, c->d_mask, c->d_U, maxdet, d_addr, d_coef, d_count);
+        FCIS_CHECK_LAUNCH();
         return FCIS_SUCCESS;
 }
 

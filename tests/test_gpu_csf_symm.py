@@ -328,3 +328,27 @@ def test_module_level_transforms():
     p, pn = csf.project_civec_csf(x.reshape(20, 20), n, na, nb, sm)
     assert p.shape == (20, 20) and abs(pn - numpy.linalg.norm(ref)) < 1e-12
     numpy.testing.assert_allclose(p.reshape(-1), ref / numpy.linalg.norm(ref), rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("sym", [False, True])
+def test_supports_match_unit_transforms(sym):
+    """fcis_csf_support rows equal the nonzero entries of csf2det of unit
+    CSF vectors (packed indices under symmetry)."""
+    import torch
+    n, na, nb, sm = 7, 4, 3, 2
+    t = csf.CSFTransformer(n, na, nb, sm, orbsym=ORBSYM7 if sym else None,
+                           wfnsym=3 if sym else None)
+    idx = torch.tensor([0, 1, 5, t.ncsf // 2, t.ncsf - 1], device="cuda")
+    A, C, cnt = t.supports(idx)
+    for row, i in enumerate(idx.tolist()):
+        x = numpy.zeros(t.ncsf)
+        x[i] = 1.0
+        d = t.vec_csf2det(x, normalize=False)
+        nz = numpy.nonzero(d)[0]
+        k = int(cnt[row])
+        got_a = A[row, :k].cpu().numpy()
+        got_c = C[row, :k].cpu().numpy()
+        order = numpy.argsort(got_a)
+        numpy.testing.assert_array_equal(got_a[order], nz)
+        numpy.testing.assert_allclose(got_c[order], d[nz], rtol=0, atol=1e-15)
+    t.close()
