@@ -2,6 +2,7 @@
 // batched problems: batch.hip).  gfx950 only.
 #pragma once
 
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -19,6 +20,7 @@ constexpr int FUSED_KS = 34;      // k-steps held in registers (4-wide MFMA): np
 constexpr int FUSED_KS_BF16 = 5;  // k-steps held in registers (32-wide bf16 MFMA)
 constexpr int FUSED_NACC = 2;     // independent accumulator chains (FP64/FP32)
 constexpr int FUSED_MAXNB = 16384;  // sigma row kept in LDS; pmapb fits int16
+constexpr int FUSED_NMAP = 4;       // beta pair-map slices in LDS (chunks j .. j+3)
 
 template <int PREC>
 struct FusedTypes;
@@ -59,7 +61,8 @@ constexpr size_t fused_lds_bytes(int64_t nb)
         typedef typename FusedTypes<PREC>::B BT;
         return (size_t)((nb + FUSED_NB - 1) / FUSED_NB * FUSED_NB) * sizeof(double) +
                2 * sizeof(BT) * FusedTypes<PREC>::KS * FusedTypes<PREC>::KW * FUSED_NB +
-               2 * sizeof(int) * 16 * FUSED_MAXW + 3 * sizeof(int16_t) * 16 * FUSED_MAXW * FUSED_NB;
+               2 * sizeof(int) * 16 * FUSED_MAXW +
+               FUSED_NMAP * sizeof(int16_t) * 16 * FUSED_MAXW * FUSED_NB;
 }
 
 // Builder wavefronts: gather the B tile of chunk j+1 (pair maps, then c)
@@ -70,7 +73,9 @@ template <int NBLD, int MAXW = FUSED_MAXW>
 struct FusedBuild {
         static constexpr int EB = (16 * MAXW * FUSED_NB + 64 * NBLD - 1) / (64 * NBLD);
         static constexpr int EM = (2 * 16 * MAXW + 64 * NBLD - 1) / (64 * NBLD);
-        int pa[EB], pb[EB];
+        static_assert(EB <= 16, "element flags hold 16 elements");
+        typedef typename std::conditional<(EB <= 8), uint32_t, uint64_t>::type Flags;
+        Flags fl;             // element s: bit 4s alpha term, 4s+1 its sign, 4s+2 beta term, 4s+3 sign
         double va[EB], vb[EB];
         v4i mv[EM];           // 8 int16 map entries per 16-byte load
 };
@@ -89,26 +94,32 @@ __device__ __forceinline__ void fb_issue_c(FusedBuild<NBLD, MAXW> &st, int npair
         const uint32_t b0 = (uint32_t)(j * NB);
         const char *cb = reinterpret_cast<const char *>(c);
         const uint32_t rowI = (uint32_t)(I * nb);
+        // No branches around the loads: every lane loads (invalid elements
+        // from c[0], discarded at commit), so that the number of loads in
+        // flight is static and a commit waits only for its own chunk's loads
+        // while the next chunk's are outstanding.
+        typedef typename FusedBuild<NBLD, MAXW>::Flags Flags;
+        Flags fl = 0;
+        uint32_t oa[EB], ob[EB];
 #pragma unroll
         for (int s = 0; s < EB; s++) {
-                int e = bt + s * nbt;
-                st.pa[s] = -1;
-                st.pb[s] = -1;
-                if (e < nelem) {
-                        int k = e / NB;
-                        if (b0 + (uint32_t)(e - k * NB) < (uint32_t)nb) {
-                                st.pa[s] = arow[k];
-                                st.pb[s] = mb[e];   // -1 where b has no link with pair k
-                        }
-                }
+                const int e = bt + s * nbt;
+                const int ec = e < nelem ? e : nelem - 1;
+                const int k = ec / NB;
+                const int col = ec - k * NB;
+                const bool in = e < nelem && b0 + (uint32_t)col < (uint32_t)nb;
+                const int a = arow[k], m = mb[ec];   // m = -1 where b has no link with pair k
+                const bool ia = in && a >= 0, ib = in && m >= 0;
+                fl |= (Flags)((ia ? 1u : 0u) | ((uint32_t)(a & 1) << 1) | (ib ? 4u : 0u) |
+                              ((uint32_t)(m & 1) << 3)) << (4 * s);
+                oa[s] = ia ? ((uint32_t)(a >> 1) * (uint32_t)nb + b0 + col) * 8u : 0u;
+                ob[s] = ib ? (rowI + (uint32_t)(m >> 1)) * 8u : 0u;
         }
+        st.fl = fl;
 #pragma unroll
         for (int s = 0; s < EB; s++) {
-                int col = (bt + s * nbt) % NB;
-                uint32_t oa = ((uint32_t)(st.pa[s] >> 1) * (uint32_t)nb + b0 + col) * 8u;
-                uint32_t ob = (rowI + (uint32_t)(st.pb[s] >> 1)) * 8u;
-                st.va[s] = st.pa[s] >= 0 ? *reinterpret_cast<const double *>(cb + oa) : 0.0;
-                st.vb[s] = st.pb[s] >= 0 ? *reinterpret_cast<const double *>(cb + ob) : 0.0;
+                st.va[s] = *reinterpret_cast<const double *>(cb + oa[s]);
+                st.vb[s] = *reinterpret_cast<const double *>(cb + ob[s]);
         }
 }
 
@@ -121,9 +132,10 @@ __device__ __forceinline__ void fb_commit_c(FusedBuild<NBLD, MAXW> &st, int npai
         for (int s = 0; s < EB; s++) {
                 int e = bt + s * nbt;
                 if (e < nelem) {
-                        double v = ((st.pa[s] & 1) ? -st.va[s] : st.va[s]) +
-                                   ((st.pb[s] & 1) ? -st.vb[s] : st.vb[s]);
-                        bb[e] = (BT)v;
+                        const uint32_t f = (uint32_t)(st.fl >> (4 * s));
+                        const double va = (f & 1) ? ((f & 2) ? -st.va[s] : st.va[s]) : 0.0;
+                        const double vb = (f & 4) ? ((f & 8) ? -st.vb[s] : st.vb[s]) : 0.0;
+                        bb[e] = (BT)(va + vb);
                 }
         }
 }
@@ -139,11 +151,10 @@ __device__ __forceinline__ void fb_issue_maps(FusedBuild<NBLD, MAXW> &st, int np
         const int64_t b0 = j * FUSED_NB;
 #pragma unroll
         for (int s = 0; s < EM; s++) {
-                int t = bt + s * NBLD * 64;
-                if (t < 2 * npair) {
-                        int k = t >> 1, h = (t & 1) * 8;
-                        st.mv[s] = *reinterpret_cast<const v4i *>(pmapb + (int64_t)k * nbp + b0 + h);
-                }
+                // unconditional (clamped) load, as in fb_issue_c
+                const int t = bt + s * NBLD * 64, tc = t < 2 * npair ? t : 2 * npair - 1;
+                const int k = tc >> 1, h = (tc & 1) * 8;
+                st.mv[s] = *reinterpret_cast<const v4i *>(pmapb + (int64_t)k * nbp + b0 + h);
         }
 }
 

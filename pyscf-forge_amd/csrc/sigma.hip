@@ -312,7 +312,7 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
         BT *buf1 = buf0 + KPMAX * NB;
         int *arow = reinterpret_cast<int *>(buf1 + KPMAX * NB);   // pmapa[pq][I]
         int *aslot = arow + MAXP;                                  // alpha slot of pq, or -1
-        int16_t *maps = reinterpret_cast<int16_t *>(aslot + MAXP); // 3 x [MAXP][NB] beta maps
+        int16_t *maps = reinterpret_cast<int16_t *>(aslot + MAXP); // FUSED_NMAP x [MAXP][NB] beta maps
 
         const int tid = threadIdx.x, nthr = blockDim.x;
         const int wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
@@ -329,8 +329,9 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
         // (3) aslot + maps of chunks 0,1, (4) chunk 0 built, then one per chunk.
         if (!mfma_wave) {
                 // ---- builder wavefronts
+                constexpr int NBLD = Ty::NBLD;
                 const int bt = tid - mtiles * 64;
-                FusedBuild<Ty::NBLD> st;
+                FusedBuild<NBLD> s0, s1;   // c of even / odd chunks in flight
                 for (int64_t r = blockIdx.x; r < ta; r += gridDim.x) {
                         const int64_t I = ROWS ? (int64_t)rows[a0 + r] : a0 + r;
                         int64_t nl = nch;
@@ -342,35 +343,43 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                         }
                         auto chunk = [&](int64_t jj) -> int64_t { return ROWS ? (int64_t)cl[jj] : jj; };
                         __syncthreads();   // (1)
-                        for (int q = bt; q < npair; q += Ty::NBLD * 64) {
+                        for (int q = bt; q < npair; q += NBLD * 64) {
                                 arow[q] = pmapa[(int64_t)q * nstra + I];
                                 aslot[q] = -1;
                         }
                         __syncthreads();   // (2)
-                        for (int k = bt; k < nlinka; k += Ty::NBLD * 64)
+                        for (int k = bt; k < nlinka; k += NBLD * 64)
                                 aslot[link_pair(linka[(int64_t)k * nstra + I].info)] = k;
-                        fb_issue_maps(st, npair, nbp, chunk(0), bt, pmapb);
-                        fb_commit_maps(st, npair, bt, maps);
-                        if (nl > 1) {
-                                fb_issue_maps(st, npair, nbp, chunk(1), bt, pmapb);
-                                fb_commit_maps(st, npair, bt, maps + MAXP * NB);
+                        // chunk jj clamped to the row (loads past the last chunk
+                        // repeat it and are never committed)
+                        auto cc = [&](int64_t jj) -> int64_t { return chunk(jj < nl ? jj : nl - 1); };
+                        auto mbuf = [&](int64_t jj) -> int16_t * {
+                                return maps + ((jj < nl ? jj : nl - 1) % FUSED_NMAP) * MAXP * NB;
+                        };
+                        for (int64_t jj = 0; jj < FUSED_NMAP - 1 && jj < nl; jj++) {
+                                fb_issue_maps(s0, npair, nbp, chunk(jj), bt, pmapb);
+                                fb_commit_maps(s0, npair, bt, mbuf(jj));
                         }
                         __syncthreads();   // (3)
-                        fb_issue_c(st, npair, nb, I, chunk(0), bt, arow, maps, c);
-                        fb_commit_c(st, npair, bt, buf0);
+                        fb_issue_c(s0, npair, nb, I, cc(0), bt, arow, mbuf(0), c);
+                        fb_issue_c(s1, npair, nb, I, cc(1), bt, arow, mbuf(1), c);
+                        fb_commit_c(s0, npair, bt, buf0);
                         __syncthreads();   // (4)
-                        for (int64_t j = 0; j < nl; j++) {
-                                // both round trips in flight together: c for chunk j+1,
-                                // pair maps for chunk j+2
-                                if (j + 1 < nl)
-                                        fb_issue_c(st, npair, nb, I, chunk(j + 1), bt, arow,
-                                                   maps + ((j + 1) % 3) * MAXP * NB, c);
-                                if (j + 2 < nl)
-                                        fb_issue_maps(st, npair, nbp, chunk(j + 2), bt, pmapb);
-                                if (j + 1 < nl) fb_commit_c(st, npair, bt, ((j + 1) & 1) ? buf1 : buf0);
-                                if (j + 2 < nl)
-                                        fb_commit_maps(st, npair, bt, maps + ((j + 2) % 3) * MAXP * NB);
+                        // phase j: c of chunk j+2 issued into `issue` while the
+                        // values of chunk j+1 (issued one phase earlier) are
+                        // committed from `commit`; pair maps three chunks ahead
+                        auto phase = [&](int64_t j, FusedBuild<NBLD> &issue, FusedBuild<NBLD> &commit) {
+                                // the pair-map load first: its commit then leaves the
+                                // c loads of chunk j+2 in flight across the barrier
+                                fb_issue_maps(commit, npair, nbp, cc(j + 3), bt, pmapb);
+                                fb_issue_c(issue, npair, nb, I, cc(j + 2), bt, arow, mbuf(j + 2), c);
+                                if (j + 1 < nl) fb_commit_c(commit, npair, bt, ((j + 1) & 1) ? buf1 : buf0);
+                                if (j + 3 < nl) fb_commit_maps(commit, npair, bt, mbuf(j + 3));
                                 __syncthreads();
+                        };
+                        for (int64_t j = 0; j < nl; j += 2) {
+                                phase(j, s0, s1);
+                                if (j + 1 < nl) phase(j + 1, s1, s0);
                         }
                         double *dst = sigma + I * nb;
                         for (int64_t i = tid; i < nb; i += nthr) {
@@ -408,7 +417,7 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                 __syncthreads();   // (4)
                 for (int64_t j = 0; j < nl; j++) {
                         const BT *bb = (j & 1) ? buf1 : buf0;
-                        const int16_t *mb = maps + (j % 3) * MAXP * NB;
+                        const int16_t *mb = maps + (j % FUSED_NMAP) * MAXP * NB;
                         const typename Ty::Acc acc = mma_steps<PREC, KS, KS>(ah, al, bb, lk, li);
                         const int64_t b = (ROWS ? (int64_t)cl[j] : j) * NB + li;
                         if (b < nb) {
