@@ -20,7 +20,7 @@ constexpr int FUSED_KS = 34;      // k-steps held in registers (4-wide MFMA): np
 constexpr int FUSED_KS_BF16 = 5;  // k-steps held in registers (32-wide bf16 MFMA)
 constexpr int FUSED_NACC = 2;     // independent accumulator chains (FP64/FP32)
 constexpr int FUSED_MAXNB = 16384;  // sigma row kept in LDS; pmapb fits int16
-constexpr int FUSED_NMAP = 4;       // beta pair-map slices in LDS (chunks j .. j+3)
+constexpr int FUSED_NMAP = 3;       // beta pair-map slices in LDS (chunks j .. j+2)
 
 template <int PREC>
 struct FusedTypes;
@@ -71,24 +71,62 @@ constexpr size_t fused_lds_bytes(int64_t nb)
 // next-but-one chunk's beta pair-map slice in flight.
 template <int NBLD, int MAXW = FUSED_MAXW>
 struct FusedBuild {
-        static constexpr int EB = (16 * MAXW * FUSED_NB + 64 * NBLD - 1) / (64 * NBLD);
+        // tile rows: 16 per MFMA wavefront, at most npair <= 136 (norb <= 16)
+        static constexpr int ROWS = 16 * MAXW < 136 ? 16 * MAXW : 136;
+        static constexpr int EB = (ROWS * FUSED_NB + 64 * NBLD - 1) / (64 * NBLD);
         static constexpr int EM = (2 * 16 * MAXW + 64 * NBLD - 1) / (64 * NBLD);
         static_assert(EB <= 16, "element flags hold 16 elements");
         typedef typename std::conditional<(EB <= 8), uint32_t, uint64_t>::type Flags;
         Flags fl;             // element s: bit 4s alpha term, 4s+1 its sign, 4s+2 beta term, 4s+3 sign
         double va[EB], vb[EB];
         v4i mv[EM];           // 8 int16 map entries per 16-byte load
+        int mr[EB];           // beta pair-map entry of each element (int16 bits)
 };
+
+// Beta pair-map entries of this thread's elements of chunk j straight from
+// pmapb into registers (element (k, col) -> pmapb[k][b0 + col]; loads
+// clamped to the tile, no branches).
+template <int NBLD, int MAXW>
+__device__ __forceinline__ void fb_issue_mapv(FusedBuild<NBLD, MAXW> &st, int npair, int64_t nbp, int64_t j,
+                                              int bt, const int16_t *__restrict__ pmapb)
+{
+        constexpr int NB = FUSED_NB, EB = FusedBuild<NBLD, MAXW>::EB;
+        const int nbt = NBLD * 64, nelem = npair * NB;
+        const uint16_t *pm = reinterpret_cast<const uint16_t *>(pmapb) + j * NB;
+#pragma unroll
+        for (int s = 0; s < EB; s++) {
+                const int e = bt + s * nbt, ec = e < nelem ? e : nelem - 1;
+                const int k = ec / NB;
+                st.mr[s] = pm[(int64_t)k * nbp + (ec - k * NB)];
+        }
+}
+
+// The entries of fb_issue_mapv into the LDS pair-map slice of the chunk
+// (read by the scatter of the MFMA wavefronts).
+template <int NBLD, int MAXW>
+__device__ __forceinline__ void fb_store_mapv(const FusedBuild<NBLD, MAXW> &st, int npair, int bt,
+                                              int16_t *__restrict__ mb)
+{
+        constexpr int EB = FusedBuild<NBLD, MAXW>::EB;
+        const int nbt = NBLD * 64, nelem = npair * FUSED_NB;
+#pragma unroll
+        for (int s = 0; s < EB; s++) {
+                const int e = bt + s * nbt;
+                if (e < nelem) mb[e] = (int16_t)st.mr[s];
+        }
+}
 
 // Issue the c loads of chunk j (element (k, col) of the tile is
 //   sa c[Ja, b0+col] + sb c[I, Jb],  arow[k] -> (Ja, sa), mb[k][col] -> (Jb, sb)),
 // addressed with 32-bit byte offsets (the fused path needs c below 4 GiB).
-template <int NBLD, int MAXW>
+template <int NBLD, int MAXW, bool REGMAP = false>
 __device__ __forceinline__ void fb_issue_c(FusedBuild<NBLD, MAXW> &st, int npair, int64_t nb, int64_t I,
                                            int64_t j, int bt, const int *__restrict__ arow,
                                            const int16_t *__restrict__ mb,
                                            const double *__restrict__ c)
 {
+        // REGMAP: the beta map entries come from st.mr (fb_issue_mapv), not
+        // from the LDS slice mb
         constexpr int NB = FUSED_NB, EB = FusedBuild<NBLD, MAXW>::EB;
         const int nbt = NBLD * 64, nelem = npair * NB;
         const uint32_t b0 = (uint32_t)(j * NB);
@@ -108,7 +146,8 @@ __device__ __forceinline__ void fb_issue_c(FusedBuild<NBLD, MAXW> &st, int npair
                 const int k = ec / NB;
                 const int col = ec - k * NB;
                 const bool in = e < nelem && b0 + (uint32_t)col < (uint32_t)nb;
-                const int a = arow[k], m = mb[ec];   // m = -1 where b has no link with pair k
+                // m = -1 where b has no link with pair k
+                const int a = arow[k], m = REGMAP ? (int)(int16_t)st.mr[s] : (int)mb[ec];
                 const bool ia = in && a >= 0, ib = in && m >= 0;
                 fl |= (Flags)((ia ? 1u : 0u) | ((uint32_t)(a & 1) << 1) | (ib ? 4u : 0u) |
                               ((uint32_t)(m & 1) << 3)) << (4 * s);

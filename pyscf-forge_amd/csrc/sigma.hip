@@ -353,28 +353,28 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                         // chunk jj clamped to the row (loads past the last chunk
                         // repeat it and are never committed)
                         auto cc = [&](int64_t jj) -> int64_t { return chunk(jj < nl ? jj : nl - 1); };
-                        auto mbuf = [&](int64_t jj) -> int16_t * {
-                                return maps + ((jj < nl ? jj : nl - 1) % FUSED_NMAP) * MAXP * NB;
-                        };
-                        for (int64_t jj = 0; jj < FUSED_NMAP - 1 && jj < nl; jj++) {
-                                fb_issue_maps(s0, npair, nbp, chunk(jj), bt, pmapb);
-                                fb_commit_maps(s0, npair, bt, mbuf(jj));
-                        }
+                        auto mbuf = [&](int64_t jj) -> int16_t * { return maps + (jj % FUSED_NMAP) * MAXP * NB; };
+                        fb_issue_mapv(s0, npair, nbp, cc(0), bt, pmapb);
+                        fb_issue_mapv(s1, npair, nbp, cc(1), bt, pmapb);
                         __syncthreads();   // (3)
-                        fb_issue_c(s0, npair, nb, I, cc(0), bt, arow, mbuf(0), c);
-                        fb_issue_c(s1, npair, nb, I, cc(1), bt, arow, mbuf(1), c);
+                        fb_issue_c<NBLD, FUSED_MAXW, true>(s0, npair, nb, I, cc(0), bt, arow, nullptr, c);
+                        fb_issue_c<NBLD, FUSED_MAXW, true>(s1, npair, nb, I, cc(1), bt, arow, nullptr, c);
+                        fb_store_mapv(s0, npair, bt, mbuf(0));
+                        if (nl > 1) fb_store_mapv(s1, npair, bt, mbuf(1));
+                        fb_issue_mapv(s0, npair, nbp, cc(2), bt, pmapb);
                         fb_commit_c(s0, npair, bt, buf0);
                         __syncthreads();   // (4)
-                        // phase j: c of chunk j+2 issued into `issue` while the
-                        // values of chunk j+1 (issued one phase earlier) are
-                        // committed from `commit`; pair maps three chunks ahead
+                        // phase j: `issue` holds the map entries of chunk j+2 (loaded
+                        // one phase earlier) and issues its c loads; `commit` holds
+                        // the c values of chunk j+1 (issued one phase earlier),
+                        // committed now, and loads the map entries of chunk j+3.  The
+                        // loads of this phase stay in flight across the barrier.
                         auto phase = [&](int64_t j, FusedBuild<NBLD> &issue, FusedBuild<NBLD> &commit) {
-                                // the pair-map load first: its commit then leaves the
-                                // c loads of chunk j+2 in flight across the barrier
-                                fb_issue_maps(commit, npair, nbp, cc(j + 3), bt, pmapb);
-                                fb_issue_c(issue, npair, nb, I, cc(j + 2), bt, arow, mbuf(j + 2), c);
+                                fb_issue_mapv(commit, npair, nbp, cc(j + 3), bt, pmapb);
+                                fb_issue_c<NBLD, FUSED_MAXW, true>(issue, npair, nb, I, cc(j + 2), bt, arow,
+                                                                   nullptr, c);
                                 if (j + 1 < nl) fb_commit_c(commit, npair, bt, ((j + 1) & 1) ? buf1 : buf0);
-                                if (j + 3 < nl) fb_commit_maps(commit, npair, bt, mbuf(j + 3));
+                                if (j + 2 < nl) fb_store_mapv(issue, npair, bt, mbuf(j + 2));
                                 __syncthreads();
                         };
                         for (int64_t j = 0; j < nl; j += 2) {
