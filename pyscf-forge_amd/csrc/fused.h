@@ -300,7 +300,7 @@ struct fcis_rowset {
         int64_t nrows;
         const int32_t *d_rows;     // [nrows] alpha string of each position
         const int2 *d_rowcl;       // [nrows] (offset into d_clist, chunk count >= 1)
-        const uint16_t *d_clist;   // chunk indices, ascending per row
+        const uint16_t *d_clist;   // chunk indices, ascending per row; allocated to whole 32-bit words
         const int32_t *d_rpos;     // [nstra] position of each alpha string, -1 if absent
         int64_t ndets;
         const uint32_t *d_dets;    // [ndets] wanted determinants (I * nstrb + Ib)

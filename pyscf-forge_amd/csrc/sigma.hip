@@ -306,6 +306,7 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
         constexpr int KS = Ty::KS, KW = Ty::KW, NB = FUSED_NB;
         constexpr int KPMAX = KS * KW;
         constexpr int MAXP = 16 * FUSED_MAXW;
+        constexpr int NBLD = Ty::NBLD;
         extern __shared__ __align__(16) char smem[];
         double *srow = reinterpret_cast<double *>(smem);   // nbp doubles: 16-byte aligned end
         BT *buf0 = reinterpret_cast<BT *>(smem + (size_t)nbp * sizeof(double));
@@ -318,6 +319,14 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
         const int wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
         const bool mfma_wave = wave < mtiles;
         const int64_t nch = (nb + NB - 1) / NB;
+        // ROWS: entry c0 + jj of the chunk lists, read as its 32-bit word (the
+        // list is padded to whole words): a uniform address, so a scalar load
+        // that never makes the vector-memory counter wait for loads in flight
+        auto list_chunk = [&](int64_t c0, int64_t jj) -> int64_t {
+                const int64_t q = c0 + jj;
+                return (int64_t)((reinterpret_cast<const uint32_t *>(clist)[q >> 1] >> (16 * (int)(q & 1))) &
+                                 0xffffu);
+        };
 
         for (int e = npair * NB + tid; e < KPMAX * NB; e += nthr) {
                 buf0[e] = (BT)0;
@@ -329,19 +338,23 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
         // (3) aslot + maps of chunks 0,1, (4) chunk 0 built, then one per chunk.
         if (!mfma_wave) {
                 // ---- builder wavefronts
-                constexpr int NBLD = Ty::NBLD;
                 const int bt = tid - mtiles * 64;
                 FusedBuild<NBLD> s0, s1;   // c of even / odd chunks in flight
                 for (int64_t r = blockIdx.x; r < ta; r += gridDim.x) {
                         const int64_t I = ROWS ? (int64_t)rows[a0 + r] : a0 + r;
                         int64_t nl = nch;
-                        const uint16_t *cl = nullptr;
+                        int64_t c0 = 0;
                         if constexpr (ROWS) {
                                 const int2 e = rowcl[a0 + r];
-                                cl = clist + e.x;
+                                c0 = e.x;
                                 nl = e.y;
                         }
-                        auto chunk = [&](int64_t jj) -> int64_t { return ROWS ? (int64_t)cl[jj] : jj; };
+                        // the chunk index through readfirstlane: opaque to strength
+                        // reduction, which otherwise keeps one address register pair per
+                        // gathered element live across the chunk loop
+                        auto chunk = [&](int64_t jj) -> int64_t {
+                                return ROWS ? list_chunk(c0, jj) : (int64_t)__builtin_amdgcn_readfirstlane((int)jj);
+                        };
                         __syncthreads();   // (1)
                         for (int q = bt; q < npair; q += NBLD * 64) {
                                 arow[q] = pmapa[(int64_t)q * nstra + I];
@@ -357,11 +370,14 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                         fb_issue_mapv(s0, npair, nbp, cc(0), bt, pmapb);
                         fb_issue_mapv(s1, npair, nbp, cc(1), bt, pmapb);
                         __syncthreads();   // (3)
+                        // loads left in flight here, oldest first: maps of chunk 2, c of
+                        // chunk 1 -- the order the loop body leaves them in, so that the
+                        // loop head sees one order and every wait stays partial
                         fb_issue_c<NBLD, FUSED_MAXW, true>(s0, npair, nb, I, cc(0), bt, arow, nullptr, c);
-                        fb_issue_c<NBLD, FUSED_MAXW, true>(s1, npair, nb, I, cc(1), bt, arow, nullptr, c);
                         fb_store_mapv(s0, npair, bt, mbuf(0));
-                        if (nl > 1) fb_store_mapv(s1, npair, bt, mbuf(1));
                         fb_issue_mapv(s0, npair, nbp, cc(2), bt, pmapb);
+                        fb_issue_c<NBLD, FUSED_MAXW, true>(s1, npair, nb, I, cc(1), bt, arow, nullptr, c);
+                        fb_store_mapv(s1, npair, bt, mbuf(1));
                         fb_commit_c(s0, npair, bt, buf0);
                         __syncthreads();   // (4)
                         // phase j: `issue` holds the map entries of chunk j+2 (loaded
@@ -373,14 +389,21 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                                 fb_issue_mapv(commit, npair, nbp, cc(j + 3), bt, pmapb);
                                 fb_issue_c<NBLD, FUSED_MAXW, true>(issue, npair, nb, I, cc(j + 2), bt, arow,
                                                                    nullptr, c);
-                                if (j + 1 < nl) fb_commit_c(commit, npair, bt, ((j + 1) & 1) ? buf1 : buf0);
-                                if (j + 2 < nl) fb_store_mapv(issue, npair, bt, mbuf(j + 2));
+                                // unconditional: past the row's last chunk the commit and
+                                // the map store go to the B buffer / map slice that no
+                                // wavefront reads in this phase (clamped chunk data).
+                                // Every path to the loop head then has the same loads in
+                                // flight, and the commits wait for their own chunk only.
+                                fb_commit_c(commit, npair, bt, ((j + 1) & 1) ? buf1 : buf0);
+                                fb_store_mapv(issue, npair, bt, mbuf(j + 2));
                                 __syncthreads();
                         };
-                        for (int64_t j = 0; j < nl; j += 2) {
+                        int64_t j = 0;
+                        for (; j + 1 < nl; j += 2) {
                                 phase(j, s0, s1);
-                                if (j + 1 < nl) phase(j + 1, s1, s0);
+                                phase(j + 1, s1, s0);
                         }
+                        if (j < nl) phase(j, s0, s1);
                         double *dst = sigma + I * nb;
                         for (int64_t i = tid; i < nb; i += nthr) {
                                 double v = srow[i];
@@ -405,10 +428,10 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
         for (int64_t r = blockIdx.x; r < ta; r += gridDim.x) {
                 const int64_t I = ROWS ? (int64_t)rows[a0 + r] : a0 + r;
                 int64_t nl = nch;
-                const uint16_t *cl = nullptr;
+                int64_t c0 = 0;
                 if constexpr (ROWS) {
                         const int2 e = rowcl[a0 + r];
-                        cl = clist + e.x;
+                        c0 = e.x;
                         nl = e.y;
                 }
                 __syncthreads();   // (1)
@@ -419,7 +442,7 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                         const BT *bb = (j & 1) ? buf1 : buf0;
                         const int16_t *mb = maps + (j % FUSED_NMAP) * MAXP * NB;
                         const typename Ty::Acc acc = mma_steps<PREC, KS, KS>(ah, al, bb, lk, li);
-                        const int64_t b = (ROWS ? (int64_t)cl[j] : j) * NB + li;
+                        const int64_t b = (ROWS ? list_chunk(c0, j) : j) * NB + li;
                         if (b < nb) {
 #pragma unroll
                                 for (int q = 0; q < 4; q++) {
@@ -619,21 +642,27 @@ void launch_fused(const fcis_sigma_plan *p, hipStream_t st, int64_t a0, int64_t 
         const int32_t *rows = ROWS ? rs->d_rows : nullptr;
         const int2 *rowcl = ROWS ? rs->d_rowcl : nullptr;
         const uint16_t *clist = ROWS ? rs->d_clist : nullptr;
+#define FCIS_LAUNCH_FUSED(PR, OUT)                                                                  \
+        hipLaunchKernelGGL((k_fused_row<PR, ROWS>), fgrid, fblk, p->fused_lds, st, p->npair,    \
+                           p->mtiles, p->ksteps, s->nlinka, nb, na, a0, ta, lda, acc, s->d_linka,   \
+                           p->d_pmapa, p->d_pmapb16, p->nbp, ci0, p->d_hfrag, p->d_hfrag_lo,        \
+                           (OUT *)p->d_vt1, sig, rows, rowcl, clist)
         if (p->precision == FCIS_PREC_FP64)
-                hipLaunchKernelGGL((k_fused_row<FCIS_PREC_FP64, ROWS>), fgrid, fblk, p->fused_lds, st,
-                                   p->npair, p->mtiles, p->ksteps, s->nlinka, nb, na, a0, ta, lda, acc,
-                                   s->d_linka, p->d_pmapa, p->d_pmapb16, p->nbp, ci0, p->d_hfrag,
-                                   p->d_hfrag_lo, (double *)p->d_vt1, sig, rows, rowcl, clist);
+                FCIS_LAUNCH_FUSED(FCIS_PREC_FP64, double);
         else if (p->precision == FCIS_PREC_FP32)
-                hipLaunchKernelGGL((k_fused_row<FCIS_PREC_FP32, ROWS>), fgrid, fblk, p->fused_lds, st,
-                                   p->npair, p->mtiles, p->ksteps, s->nlinka, nb, na, a0, ta, lda, acc,
-                                   s->d_linka, p->d_pmapa, p->d_pmapb16, p->nbp, ci0, p->d_hfrag,
-                                   p->d_hfrag_lo, (float *)p->d_vt1, sig, rows, rowcl, clist);
+                FCIS_LAUNCH_FUSED(FCIS_PREC_FP32, float);
         else
-                hipLaunchKernelGGL((k_fused_row<FCIS_PREC_BF16X3, ROWS>), fgrid, fblk, p->fused_lds, st,
-                                   p->npair, p->mtiles, p->ksteps, s->nlinka, nb, na, a0, ta, lda, acc,
-                                   s->d_linka, p->d_pmapa, p->d_pmapb16, p->nbp, ci0, p->d_hfrag,
-                                   p->d_hfrag_lo, (float *)p->d_vt1, sig, rows, rowcl, clist);
+                FCIS_LAUNCH_FUSED(FCIS_PREC_BF16X3, float);
+#undef FCIS_LAUNCH_FUSED
+}
+
+// The fused row kernel of a precision, ROWS or not.
+template <bool ROWS>
+const void *fused_kernel(int precision)
+{
+        if (precision == FCIS_PREC_FP64) return (const void *)k_fused_row<FCIS_PREC_FP64, ROWS>;
+        if (precision == FCIS_PREC_FP32) return (const void *)k_fused_row<FCIS_PREC_FP32, ROWS>;
+        return (const void *)k_fused_row<FCIS_PREC_BF16X3, ROWS>;
 }
 
 // Alpha pull onto the wanted determinants of a row set, for the positions
@@ -765,12 +794,8 @@ int fcis::sigma_plan_create(fcis_sigma_plan **out, const fcis_space *space, cons
                 int dev = 0, ncu = 0, occ = 0;
                 FCIS_HIP(hipGetDevice(&dev));
                 FCIS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-                const void *fk = precision == FCIS_PREC_FP64 ? (const void *)k_fused_row<FCIS_PREC_FP64, false>
-                               : precision == FCIS_PREC_FP32 ? (const void *)k_fused_row<FCIS_PREC_FP32, false>
-                                                              : (const void *)k_fused_row<FCIS_PREC_BF16X3, false>;
-                const void *fkr = precision == FCIS_PREC_FP64 ? (const void *)k_fused_row<FCIS_PREC_FP64, true>
-                                : precision == FCIS_PREC_FP32 ? (const void *)k_fused_row<FCIS_PREC_FP32, true>
-                                                               : (const void *)k_fused_row<FCIS_PREC_BF16X3, true>;
+                const void *fk = fused_kernel<false>(precision);
+                const void *fkr = fused_kernel<true>(precision);
                 FCIS_HIP(hipFuncSetAttribute(fk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
                 FCIS_HIP(hipFuncSetAttribute(fkr, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
                 p->nbld = precision == FCIS_PREC_FP64 ? FusedTypes<FCIS_PREC_FP64>::NBLD
