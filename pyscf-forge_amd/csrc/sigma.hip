@@ -453,7 +453,7 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                                                 int e = mb[pq * NB + li];
                                                 if (e >= 0) atomicAdd(&srow[e >> 1], (e & 1) ? -v : v);
                                                 int k = aslot[pq];
-                                                if (k >= 0) walpha[(int64_t)k * ld + r * nb + b] = acc[q];
+                                                if (k >= 0) walpha[(int64_t)k * ld + r * nbp + b] = acc[q];
                                         }
                                 }
                         }
@@ -471,9 +471,11 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
 // Alpha pull for one alpha tile [a0, a0+ta): one thread per destination
 // determinant (J, Jb), adding sum_k s_k walpha[rslot_k][I_k - a0][Jb] over
 // the alpha links k of J whose target I_k lies in the tile; the reverse
-// slot rslot_k is the slot of the link I_k -> J in I_k's list.
+// slot rslot_k is the slot of the link I_k -> J in I_k's list.  walpha rows
+// are wrs (nb rounded up to whole 16-determinant chunks) apart, so that the
+// fused kernel's 16-wide stores fill whole 128-byte lines.
 template <typename T>
-__global__ __launch_bounds__(256) void k_alpha_pull(int64_t nb, int64_t nstra, int64_t a0,
+__global__ __launch_bounds__(256) void k_alpha_pull(int64_t nb, int64_t wrs, int64_t nstra, int64_t a0,
                                                     int64_t ta, int64_t ld, int nlinka,
                                                     const fcis_link *__restrict__ linka,
                                                     const T *__restrict__ walpha,
@@ -482,14 +484,43 @@ __global__ __launch_bounds__(256) void k_alpha_pull(int64_t nb, int64_t nstra, i
         const int64_t nchunk = (nb + 255) / 256;
         const int64_t J = blockIdx.x / nchunk;
         const int64_t Jb = (blockIdx.x - J * nchunk) * 256 + threadIdx.x;
+        // the links of J, all loaded at once (nlinka <= npair <= 16 * FUSED_MAXW
+        // on the fused path)
+        __shared__ fcis_link lks[16 * FUSED_MAXW];
+        for (int k = threadIdx.x; k < nlinka; k += 256) lks[k] = linka[(int64_t)k * nstra + J];
+        __syncthreads();
         if (Jb >= nb) return;
+        // links in groups of U with all U loads issued before any is used
+        // (links outside the tile read walpha[Jb] and add zero); the sum
+        // runs in link order either way
+        constexpr int U = 8;
         double acc = 0.0;
-        for (int k = 0; k < nlinka; k++) {
-                fcis_link L = linka[(int64_t)k * nstra + J];
-                int64_t I = L.addr;
+        int k = 0;
+        for (; k + U <= nlinka; k += U) {
+                T v[U];
+                bool in[U], neg[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                        const fcis_link L = lks[k + u];
+                        const int64_t I = L.addr;
+                        in[u] = I >= a0 && I < a0 + ta;
+                        neg[u] = link_neg(L.info);
+                        v[u] = walpha[in[u] ? (int64_t)link_rslot(L.info) * ld + (I - a0) * wrs + Jb : Jb];
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                        if (in[u]) {
+                                const double x = (double)v[u];
+                                acc += neg[u] ? -x : x;
+                        }
+                }
+        }
+        for (; k < nlinka; k++) {
+                const fcis_link L = lks[k];
+                const int64_t I = L.addr;
                 if (I < a0 || I >= a0 + ta) continue;
-                double v = (double)walpha[(int64_t)link_rslot(L.info) * ld + (I - a0) * nb + Jb];
-                acc += link_neg(L.info) ? -v : v;
+                const double x = (double)walpha[(int64_t)link_rslot(L.info) * ld + (I - a0) * wrs + Jb];
+                acc += link_neg(L.info) ? -x : x;
         }
         sigma[J * nb + Jb] += acc;
 }
@@ -672,7 +703,7 @@ const void *fused_kernel(int precision)
 // The last launch (out != nullptr) writes out[d] = sigma[m] + that sum.
 template <typename T>
 __global__ __launch_bounds__(256) void k_alpha_pull_rows(
-        int64_t ndets, const uint32_t *__restrict__ dets, int64_t nb, int64_t nstra, int64_t r0,
+        int64_t ndets, const uint32_t *__restrict__ dets, int64_t nb, int64_t wrs, int64_t nstra, int64_t r0,
         int64_t tr, int64_t ld, int nlinka, const fcis_link *__restrict__ linka,
         const int32_t *__restrict__ rpos, const T *__restrict__ walpha, double *__restrict__ sigma,
         double *__restrict__ out)
@@ -687,7 +718,7 @@ __global__ __launch_bounds__(256) void k_alpha_pull_rows(
                         const int64_t pos = rpos[L.addr];
                         if (pos < r0 || pos >= r0 + tr) continue;
                         const double v = (double)walpha[(int64_t)link_rslot(L.info) * ld +
-                                                        (pos - r0) * nb + Jb];
+                                                        (pos - r0) * wrs + Jb];
                         acc += link_neg(L.info) ? -v : v;
                 }
                 if (out)
@@ -725,7 +756,7 @@ int fcis::sigma_plan_execute_rows(fcis_sigma_plan *p, const fcis_rowset *rs, con
         for (int t = 0; t < ntiles; t++) {
                 const int64_t r0 = (int64_t)t * tile;
                 const int64_t tr = std::min<int64_t>(tile, rs->nrows - r0);
-                const int64_t lda = tr * nb;
+                const int64_t lda = tr * p->nbp;   // walpha rows nbp apart
                 double *o = t == ntiles - 1 ? out : nullptr;
                 if (p->timing) FCIS_HIP(hipEventRecord(p->ev[0], st));
                 if (p->timing) FCIS_HIP(hipEventRecord(p->ev[1], st));
@@ -733,11 +764,11 @@ int fcis::sigma_plan_execute_rows(fcis_sigma_plan *p, const fcis_rowset *rs, con
                 if (p->timing) FCIS_HIP(hipEventRecord(p->ev[2], st));
                 if (p->precision == FCIS_PREC_FP64)
                         hipLaunchKernelGGL(k_alpha_pull_rows<double>, dim3(pgrid), dim3(256), 0, st,
-                                           rs->ndets, rs->d_dets, nb, na, r0, tr, lda, s->nlinka,
+                                           rs->ndets, rs->d_dets, nb, p->nbp, na, r0, tr, lda, s->nlinka,
                                            s->d_linka, rs->d_rpos, (const double *)p->d_vt1, sig, o);
                 else
                         hipLaunchKernelGGL(k_alpha_pull_rows<float>, dim3(pgrid), dim3(256), 0, st,
-                                           rs->ndets, rs->d_dets, nb, na, r0, tr, lda, s->nlinka,
+                                           rs->ndets, rs->d_dets, nb, p->nbp, na, r0, tr, lda, s->nlinka,
                                            s->d_linka, rs->d_rpos, (const float *)p->d_vt1, sig, o);
                 FCIS_CHECK_LAUNCH();
                 if (p->timing) {
@@ -810,13 +841,14 @@ int fcis::sigma_plan_create(fcis_sigma_plan **out, const fcis_space *space, cons
         // workspace per alpha string: walpha (nlinka rows, fused path) or
         // t1 + vt1 (npair rows each)
         const uint64_t rows_per_alpha = p->fused ? (uint64_t)space->nlinka : 2ull * p->npair;
-        uint64_t per_alpha = rows_per_alpha * (uint64_t)nb * elem;
+        const int64_t wrs = p->fused ? p->nbp : nb;   // walpha rows padded to whole chunks
+        uint64_t per_alpha = rows_per_alpha * (uint64_t)wrs * elem;
         int64_t ta = per_alpha ? (int64_t)(workspace_bytes / per_alpha) : space->nstra;
         if (ta < 1) ta = 1;
         if (ta > space->nstra) ta = space->nstra;
         p->tile_a = ta;
         p->ntiles = (int)((space->nstra + ta - 1) / ta);
-        size_t buf = (size_t)(p->fused ? space->nlinka : p->npair) * ta * nb * elem;
+        size_t buf = (size_t)(p->fused ? space->nlinka : p->npair) * ta * wrs * elem;
         p->ws_bytes = p->fused ? buf : 2 * buf;
         const size_t nf = frag_count(p);
         size_t hbytes = precision == FCIS_PREC_BF16X3 ? nf * 16 : nf * elem;
@@ -918,7 +950,7 @@ int fcis_sigma_plan_execute(fcis_sigma_plan_t *p, const double *ci0, double *ci1
                 if (p->timing) FCIS_HIP(hipEventRecord(p->ev[0], st));
                 if (p->fused) {
                         const int acc = p->ntiles > 1 ? 1 : 0;
-                        const int64_t lda = ta * nb;   // walpha row stride
+                        const int64_t lda = ta * p->nbp;   // walpha slot stride (rows nbp apart)
                         if (t == 0 && acc)
                                 hipLaunchKernelGGL(k_zero, dim3((na * nb + 255) / 256), dim3(256), 0, st,
                                                    na * nb, ci1);
@@ -927,11 +959,11 @@ int fcis_sigma_plan_execute(fcis_sigma_plan_t *p, const double *ci0, double *ci1
                         if (p->timing) FCIS_HIP(hipEventRecord(p->ev[2], st));
                         const dim3 pgrid(na * ((nb + 255) / 256));
                         if (p->precision == FCIS_PREC_FP64)
-                                hipLaunchKernelGGL(k_alpha_pull<double>, pgrid, dim3(256), 0, st, nb, na, a0,
+                                hipLaunchKernelGGL(k_alpha_pull<double>, pgrid, dim3(256), 0, st, nb, p->nbp, na, a0,
                                                    ta, lda, s->nlinka, s->d_linka,
                                                    (const double *)p->d_vt1, ci1);
                         else
-                                hipLaunchKernelGGL(k_alpha_pull<float>, pgrid, dim3(256), 0, st, nb, na, a0,
+                                hipLaunchKernelGGL(k_alpha_pull<float>, pgrid, dim3(256), 0, st, nb, p->nbp, na, a0,
                                                    ta, lda, s->nlinka, s->d_linka,
                                                    (const float *)p->d_vt1, ci1);
                 } else if (p->precision == FCIS_PREC_FP64) {
