@@ -493,7 +493,7 @@ __global__ __launch_bounds__(256) void k_alpha_pull(int64_t nb, int64_t wrs, int
         // links in groups of U with all U loads issued before any is used
         // (links outside the tile read walpha[Jb] and add zero); the sum
         // runs in link order either way
-        constexpr int U = 8;
+        constexpr int U = 12;
         double acc = 0.0;
         int k = 0;
         for (; k + U <= nlinka; k += U) {
