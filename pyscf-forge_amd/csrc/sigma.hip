@@ -555,20 +555,60 @@ __global__ __launch_bounds__(256) void k_scatter(int64_t nb, int64_t nstra, int6
         const int64_t nchunk = (nb + 255) / 256;
         const int64_t J = blockIdx.x / nchunk;
         int64_t Jb = (blockIdx.x - J * nchunk) * 256 + threadIdx.x;
+        // the α links of J in LDS (nlinka entries of dynamic shared memory)
+        extern __shared__ fcis_link lks[];
+        for (int k = threadIdx.x; k < nlinka; k += 256) lks[k] = linka[(int64_t)k * nstra + J];
+        __syncthreads();
         if (Jb >= nb) return;
+        // U loads in flight per thread; every sum runs in link order
+        constexpr int U = 8;
         double acc = accumulate ? sigma[J * nb + Jb] : 0.0;
-        for (int k = 0; k < nlinka; k++) {
-                fcis_link L = linka[(int64_t)k * nstra + J];
-                int64_t I = L.addr;
+        int k = 0;
+        for (; k + U <= nlinka; k += U) {
+                T v[U];
+                bool in[U], neg[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                        const fcis_link L = lks[k + u];
+                        const int64_t I = L.addr;
+                        in[u] = I >= a0 && I < a0 + ta;
+                        neg[u] = link_neg(L.info);
+                        v[u] = vt1[in[u] ? (int64_t)link_pair(L.info) * ld + (I - a0) * nb + Jb : Jb];
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                        if (in[u]) {
+                                const double x = (double)v[u];
+                                acc += neg[u] ? -x : x;
+                        }
+                }
+        }
+        for (; k < nlinka; k++) {
+                const fcis_link L = lks[k];
+                const int64_t I = L.addr;
                 if (I < a0 || I >= a0 + ta) continue;
-                double x = (double)vt1[(int64_t)link_pair(L.info) * ld + (I - a0) * nb + Jb];
+                const double x = (double)vt1[(int64_t)link_pair(L.info) * ld + (I - a0) * nb + Jb];
                 acc += link_neg(L.info) ? -x : x;
         }
         if (J >= a0 && J < a0 + ta) {
                 const T *row = vt1 + (J - a0) * nb;
-                for (int k = 0; k < nlinkb; k++) {
-                        fcis_link L = linkb[(int64_t)k * nb + Jb];
-                        double x = (double)row[(int64_t)link_pair(L.info) * ld + L.addr];
+                k = 0;
+                for (; k + U <= nlinkb; k += U) {
+                        fcis_link L[U];
+                        T v[U];
+#pragma unroll
+                        for (int u = 0; u < U; u++) L[u] = linkb[(int64_t)(k + u) * nb + Jb];
+#pragma unroll
+                        for (int u = 0; u < U; u++) v[u] = row[(int64_t)link_pair(L[u].info) * ld + L[u].addr];
+#pragma unroll
+                        for (int u = 0; u < U; u++) {
+                                const double x = (double)v[u];
+                                acc += link_neg(L[u].info) ? -x : x;
+                        }
+                }
+                for (; k < nlinkb; k++) {
+                        const fcis_link L = linkb[(int64_t)k * nb + Jb];
+                        const double x = (double)row[(int64_t)link_pair(L.info) * ld + L.addr];
                         acc += link_neg(L.info) ? -x : x;
                 }
         }
@@ -712,8 +752,34 @@ __global__ __launch_bounds__(256) void k_alpha_pull_rows(
              d += (int64_t)gridDim.x * blockDim.x) {
                 const int64_t m = dets[d];
                 const int64_t J = m / nb, Jb = m - J * nb;
+                // U links, their positions and U walpha loads in flight at a
+                // time; the sum runs in link order
+                constexpr int U = 8;
                 double acc = 0.0;
-                for (int k = 0; k < nlinka; k++) {
+                int k = 0;
+                for (; k + U <= nlinka; k += U) {
+                        fcis_link L[U];
+                        int32_t pos[U];
+                        T v[U];
+#pragma unroll
+                        for (int u = 0; u < U; u++) L[u] = linka[(int64_t)(k + u) * nstra + J];
+#pragma unroll
+                        for (int u = 0; u < U; u++) pos[u] = rpos[L[u].addr];
+#pragma unroll
+                        for (int u = 0; u < U; u++) {
+                                const bool in = pos[u] >= r0 && pos[u] < r0 + tr;
+                                v[u] = walpha[in ? (int64_t)link_rslot(L[u].info) * ld + (pos[u] - r0) * wrs + Jb
+                                                 : Jb];
+                        }
+#pragma unroll
+                        for (int u = 0; u < U; u++) {
+                                if (pos[u] >= r0 && pos[u] < r0 + tr) {
+                                        const double x = (double)v[u];
+                                        acc += link_neg(L[u].info) ? -x : x;
+                                }
+                        }
+                }
+                for (; k < nlinka; k++) {
                         const fcis_link L = linka[(int64_t)k * nstra + J];
                         const int64_t pos = rpos[L.addr];
                         if (pos < r0 || pos >= r0 + tr) continue;
@@ -974,7 +1040,7 @@ int fcis_sigma_plan_execute(fcis_sigma_plan_t *p, const double *ci0, double *ci1
                         hipLaunchKernelGGL(k_contract_f64, ggrid, gblk, 0, st, p->npair, p->mtiles,
                                            p->ksteps, ndet, ld, (const double *)p->d_hfrag, t1, vt1);
                         if (p->timing) FCIS_HIP(hipEventRecord(p->ev[2], st));
-                        hipLaunchKernelGGL(k_scatter<double>, sgrid, dim3(256), 0, st, nb, na, a0, ta,
+                        hipLaunchKernelGGL(k_scatter<double>, sgrid, dim3(256), s->nlinka * sizeof(fcis_link), st, nb, na, a0, ta,
                                            ld, s->nlinka, s->nlinkb, t > 0 ? 1 : 0, s->d_linka,
                                            s->d_linkb, vt1, ci1);
                 } else {
@@ -992,7 +1058,7 @@ int fcis_sigma_plan_execute(fcis_sigma_plan_t *p, const double *ci0, double *ci1
                                                    (const bf16x8 *)p->d_hfrag,
                                                    (const bf16x8 *)p->d_hfrag_lo, t1, vt1);
                         if (p->timing) FCIS_HIP(hipEventRecord(p->ev[2], st));
-                        hipLaunchKernelGGL(k_scatter<float>, sgrid, dim3(256), 0, st, nb, na, a0, ta,
+                        hipLaunchKernelGGL(k_scatter<float>, sgrid, dim3(256), s->nlinka * sizeof(fcis_link), st, nb, na, a0, ta,
                                            ld, s->nlinka, s->nlinkb, t > 0 ? 1 : 0, s->d_linka,
                                            s->d_linkb, vt1, ci1);
                 }
