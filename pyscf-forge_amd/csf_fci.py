@@ -297,14 +297,6 @@ class CSFTransformer:
         return self._pack(csfvec, order, self.nsym, self.ncsf_full, self.unpack_dev)
 
     # -- determinant packing, projections, printing (csfstring.py:42-220) --
-    @property
-    def ndeta(self):
-        return math.comb(self.norb, self.neleca)
-
-    @property
-    def ndetb(self):
-        return math.comb(self.norb, self.nelecb)
-
     def _det_sym_index(self):
         """Device addresses of the determinants whose configuration has
         irrep wfnsym (XOR of orbsym over the singly occupied orbitals, the
