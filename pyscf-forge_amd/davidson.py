@@ -68,6 +68,12 @@ class DeviceOps:
         coef = numpy.ascontiguousarray(coef, dtype=numpy.float64).reshape(m, 1)
         self.ops.gemm([(self._rows(X, m, n), [y.data_ptr()], int(n), coef, float(beta))])
 
+    def combine(self, n, vecs, coef, y):
+        """y = sum_i coef[i] vecs[i] in one pass (y may not be one of vecs)."""
+        rows = numpy.array([v.data_ptr() for v in vecs], dtype=numpy.int64)
+        coef = numpy.ascontiguousarray(coef, dtype=numpy.float64).reshape(len(vecs), 1)
+        self.ops.gemm([(rows, [y.data_ptr()], int(n), coef, 0.0)])
+
 
 def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
              max_cycle=50, max_space=12, lindep=1e-14, verbose=False,
@@ -103,11 +109,17 @@ def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
         norm0 = math.sqrt(ops.dots(1, n, t, t)[0])
         if norm0 == 0.0:
             return False
+        norm = norm0
         for _ in range(2):
-            if space:
-                c = ops.dots(space, n, X, t)
-                ops.lincomb(space, n, X, -c, 1.0, t)
-        norm = math.sqrt(max(ops.dots(1, n, t, t)[0], 0.0))
+            if not space:
+                break
+            c = ops.dots(space, n, X, t)
+            ops.lincomb(space, n, X, -c, 1.0, t)
+            before, norm = norm, math.sqrt(max(ops.dots(1, n, t, t)[0], 0.0))
+            # a second pass only where the first cancelled much of t
+            # (Daniel-Gragg-Kaufman-Stewart: |t'| < |t| / sqrt(2))
+            if norm >= before * 0.7071067811865476:
+                break
         if norm * norm < lindep or norm < 1e-7 * norm0:
             return False
         ops.lincomb(1, n, t, numpy.array([1.0 / norm]), 0.0, X[space])
@@ -142,8 +154,7 @@ def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
         de = e - e_old
         new_vectors = []
         for k in range(nroots):
-            ops.lincomb(1, n, axr[k], numpy.array([1.0]), 0.0, r)
-            ops.lincomb(1, n, xr[k], numpy.array([-e[k]]), 1.0, r)
+            ops.combine(n, (axr[k], xr[k]), numpy.array([1.0, -e[k]]), r)
             rnorm[k] = math.sqrt(max(ops.dots(1, n, r, r)[0], 0.0))
             conv[k] = abs(de[k]) < tol and rnorm[k] < tol_residual
             if not conv[k]:
