@@ -109,7 +109,10 @@ __global__ __launch_bounds__(64 * (MAXW + batch_nbld<PREC>())) void k_batch_row(
         // (string, slot): (1) row start, (2) arow, (3) maps of chunks 0,1,
         // (4) chunk 0 built, one per chunk; (B) item done.
         if (wave >= MAXW) {
-                // ---- builder wavefronts
+                // ---- builder wavefronts (issue priority as in the fused row
+                // kernel: builders first for FP64/FP32, MFMA wavefronts first
+                // for bf16x3)
+                if constexpr (PREC != FCIS_PREC_BF16X3) __builtin_amdgcn_s_setprio(1);
                 const int bt = tid - MAXW * 64;
                 for (;;) {
                         __syncthreads();   // (A)
@@ -171,6 +174,7 @@ __global__ __launch_bounds__(64 * (MAXW + batch_nbld<PREC>())) void k_batch_row(
 
         // ---- MFMA wavefronts (waves >= the problem's mtiles only keep the
         // barrier schedule)
+        if constexpr (PREC == FCIS_PREC_BF16X3) __builtin_amdgcn_s_setprio(1);
         typename Ty::A ah[KS];
         typename Ty::A al[PREC == FCIS_PREC_BF16X3 ? KS : 1];
         int cur = -1;
