@@ -488,9 +488,12 @@ __global__ __launch_bounds__(256) void k_alpha_pull(int64_t nb, int64_t wrs, int
                                                     const T *__restrict__ walpha,
                                                     double *__restrict__ sigma)
 {
-        const int64_t nchunk = (nb + 255) / 256;
+        // two adjacent determinants per thread, one 2-wide load per link: the
+        // walpha rows and slots have even lengths, so a pair never crosses one
+        typedef typename std::conditional<std::is_same<T, double>::value, double2, float2>::type T2;
+        const int64_t nchunk = ((nb + 1) / 2 + 255) / 256;
         const int64_t J = blockIdx.x / nchunk;
-        const int64_t Jb = (blockIdx.x - J * nchunk) * 256 + threadIdx.x;
+        const int64_t Jb = 2 * ((blockIdx.x - J * nchunk) * 256 + threadIdx.x);
         // the links of J, all loaded at once (nlinka <= npair <= 16 * FUSED_MAXW
         // on the fused path)
         __shared__ fcis_link lks[16 * FUSED_MAXW];
@@ -498,13 +501,13 @@ __global__ __launch_bounds__(256) void k_alpha_pull(int64_t nb, int64_t wrs, int
         __syncthreads();
         if (Jb >= nb) return;
         // links in groups of U with all U loads issued before any is used
-        // (links outside the tile read walpha[Jb] and add zero); the sum
-        // runs in link order either way
+        // (links outside the tile read walpha[Jb] and add zero); the sums
+        // run in link order either way
         constexpr int U = 12;
-        double acc = 0.0;
+        double acc0 = 0.0, acc1 = 0.0;
         int k = 0;
         for (; k + U <= nlinka; k += U) {
-                T v[U];
+                T2 v[U];
                 bool in[U], neg[U];
 #pragma unroll
                 for (int u = 0; u < U; u++) {
@@ -512,13 +515,15 @@ __global__ __launch_bounds__(256) void k_alpha_pull(int64_t nb, int64_t wrs, int
                         const int64_t I = L.addr;
                         in[u] = I >= a0 && I < a0 + ta;
                         neg[u] = link_neg(L.info);
-                        v[u] = walpha[in[u] ? (int64_t)link_rslot(L.info) * ld + (I - a0) * wrs + Jb : Jb];
+                        v[u] = *reinterpret_cast<const T2 *>(
+                                walpha + (in[u] ? (int64_t)link_rslot(L.info) * ld + (I - a0) * wrs + Jb : Jb));
                 }
 #pragma unroll
                 for (int u = 0; u < U; u++) {
                         if (in[u]) {
-                                const double x = (double)v[u];
-                                acc += neg[u] ? -x : x;
+                                const double x0 = (double)v[u].x, x1 = (double)v[u].y;
+                                acc0 += neg[u] ? -x0 : x0;
+                                acc1 += neg[u] ? -x1 : x1;
                         }
                 }
         }
@@ -526,10 +531,14 @@ __global__ __launch_bounds__(256) void k_alpha_pull(int64_t nb, int64_t wrs, int
                 const fcis_link L = lks[k];
                 const int64_t I = L.addr;
                 if (I < a0 || I >= a0 + ta) continue;
-                const double x = (double)walpha[(int64_t)link_rslot(L.info) * ld + (I - a0) * wrs + Jb];
-                acc += link_neg(L.info) ? -x : x;
+                const T2 v = *reinterpret_cast<const T2 *>(walpha + (int64_t)link_rslot(L.info) * ld +
+                                                           (I - a0) * wrs + Jb);
+                const double x0 = (double)v.x, x1 = (double)v.y;
+                acc0 += link_neg(L.info) ? -x0 : x0;
+                acc1 += link_neg(L.info) ? -x1 : x1;
         }
-        sigma[J * nb + Jb] += acc;
+        sigma[J * nb + Jb] += acc0;
+        if (Jb + 1 < nb) sigma[J * nb + Jb + 1] += acc1;
 }
 
 __global__ void k_zero(int64_t n, double *__restrict__ x)
@@ -1030,7 +1039,7 @@ int fcis_sigma_plan_execute(fcis_sigma_plan_t *p, const double *ci0, double *ci1
                         if (p->timing) FCIS_HIP(hipEventRecord(p->ev[1], st));
                         launch_fused<false>(p, st, a0, ta, lda, acc, ci0, ci1, nullptr);
                         if (p->timing) FCIS_HIP(hipEventRecord(p->ev[2], st));
-                        const dim3 pgrid(na * ((nb + 255) / 256));
+                        const dim3 pgrid(na * (((nb + 1) / 2 + 255) / 256));   // two determinants per thread
                         if (p->precision == FCIS_PREC_FP64)
                                 hipLaunchKernelGGL(k_alpha_pull<double>, pgrid, dim3(256), 0, st, nb, p->nbp, na, a0,
                                                    ta, lda, s->nlinka, s->d_linka,
