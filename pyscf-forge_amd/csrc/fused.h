@@ -55,12 +55,27 @@ struct FusedTypes<FCIS_PREC_BF16X3> {
         static constexpr int NBLD = 7;
 };
 
+// bf16x3 fused kernel: the builders split each B element into bf16 high and
+// low parts once and store them column-major, [col][k] with this k stride
+// (padded against LDS bank conflicts), so that an MFMA wavefront reads its
+// 8 k-values of a column as one 16-byte load instead of converting every
+// element itself.
+constexpr int FUSED_KPS_BF16 = FUSED_KS_BF16 * 32 + 8;
+
+// Bytes of one B buffer of the fused row kernel.
+template <int PREC>
+constexpr size_t fused_bbytes()
+{
+        return PREC == FCIS_PREC_BF16X3
+                       ? 2 * sizeof(__bf16) * FUSED_NB * FUSED_KPS_BF16
+                       : sizeof(typename FusedTypes<PREC>::B) * FusedTypes<PREC>::KS * FusedTypes<PREC>::KW *
+                                 FUSED_NB;
+}
+
 template <int PREC>
 constexpr size_t fused_lds_bytes(int64_t nb)
 {
-        typedef typename FusedTypes<PREC>::B BT;
-        return (size_t)((nb + FUSED_NB - 1) / FUSED_NB * FUSED_NB) * sizeof(double) +
-               2 * sizeof(BT) * FusedTypes<PREC>::KS * FusedTypes<PREC>::KW * FUSED_NB +
+        return (size_t)((nb + FUSED_NB - 1) / FUSED_NB * FUSED_NB) * sizeof(double) + 2 * fused_bbytes<PREC>() +
                2 * sizeof(int) * 16 * FUSED_MAXW +
                FUSED_NMAP * sizeof(int16_t) * 16 * FUSED_MAXW * FUSED_NB;
 }
@@ -179,6 +194,31 @@ __device__ __forceinline__ void fb_commit_c(FusedBuild<NBLD, MAXW> &st, int npai
         }
 }
 
+// fb_commit_c for the bf16x3 kernel: element (k, col) split into bf16 high
+// and low parts, bh[col][k] and bl[col][k] (k stride FUSED_KPS_BF16).
+template <int NBLD, int MAXW>
+__device__ __forceinline__ void fb_commit_split(FusedBuild<NBLD, MAXW> &st, int npair, int bt,
+                                                __bf16 *__restrict__ bh)
+{
+        constexpr int EB = FusedBuild<NBLD, MAXW>::EB, NB = FUSED_NB, KP = FUSED_KPS_BF16;
+        __bf16 *bl = bh + NB * KP;
+        const int nbt = NBLD * 64, nelem = npair * NB;
+#pragma unroll
+        for (int s = 0; s < EB; s++) {
+                const int e = bt + s * nbt;
+                if (e < nelem) {
+                        const uint32_t f = (uint32_t)(st.fl >> (4 * s));
+                        const double va = (f & 1) ? ((f & 2) ? -st.va[s] : st.va[s]) : 0.0;
+                        const double vb = (f & 4) ? ((f & 8) ? -st.vb[s] : st.vb[s]) : 0.0;
+                        const float x = (float)(va + vb);
+                        const __bf16 hi = (__bf16)x;
+                        const int k = e / NB, col = e - k * NB;
+                        bh[col * KP + k] = hi;
+                        bl[col * KP + k] = (__bf16)(x - (float)hi);
+                }
+        }
+}
+
 // Beta pair-map slice of chunk j, pmapb[k][b0 .. b0+NB) for k < npair (16
 // int16 = two 16-byte loads per pair; pmapb rows are padded to a multiple of
 // FUSED_NB entries with -1, so no load crosses a row).
@@ -260,6 +300,27 @@ __device__ __forceinline__ typename FusedTypes<PREC>::Acc mma_steps(
                 }
                 return ac[0] + ac[1];
         }
+}
+
+// mma_steps for the bf16x3 fused kernel, whose B tile the builders stored
+// pre-split and column-major (fb_commit_split): same three products, same
+// accumulation order, no conversions on the MFMA wavefronts.
+template <int KS>
+__device__ __forceinline__ v4f mma_presplit(const bf16x8 (&ah)[KS], const bf16x8 (&al)[KS],
+                                            const __bf16 *bh, int lk, int li)
+{
+        constexpr int KP = FUSED_KPS_BF16;
+        const __bf16 *bl = bh + FUSED_NB * KP;
+        v4f a0c = {0, 0, 0, 0}, a1c = a0c, a2c = a0c;
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+                const bf16x8 h = *reinterpret_cast<const bf16x8 *>(bh + li * KP + 32 * ks + 8 * lk);
+                const bf16x8 l = *reinterpret_cast<const bf16x8 *>(bl + li * KP + 32 * ks + 8 * lk);
+                a0c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[ks], h, a0c, 0, 0, 0);
+                a1c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ks], l, a1c, 0, 0, 0);
+                a2c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ks], h, a2c, 0, 0, 0);
+        }
+        return (a0c + a1c) + a2c;
 }
 
 }  // namespace

@@ -309,9 +309,13 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
         constexpr int NBLD = Ty::NBLD;
         extern __shared__ __align__(16) char smem[];
         double *srow = reinterpret_cast<double *>(smem);   // nbp doubles: 16-byte aligned end
-        BT *buf0 = reinterpret_cast<BT *>(smem + (size_t)nbp * sizeof(double));
-        BT *buf1 = buf0 + KPMAX * NB;
-        int *arow = reinterpret_cast<int *>(buf1 + KPMAX * NB);   // pmapa[pq][I]
+        // B tiles: [k][col] in BT, or for bf16x3 bf16 high and low parts [col][k]
+        constexpr size_t BB = fused_bbytes<PREC>();
+        constexpr bool PRESPLIT = PREC == FCIS_PREC_BF16X3;
+        char *bbase = smem + (size_t)nbp * sizeof(double);
+        BT *buf0 = reinterpret_cast<BT *>(bbase);
+        BT *buf1 = reinterpret_cast<BT *>(bbase + BB);
+        int *arow = reinterpret_cast<int *>(bbase + 2 * BB);   // pmapa[pq][I]
         int *aslot = arow + MAXP;                                  // alpha slot of pq, or -1
         int16_t *maps = reinterpret_cast<int16_t *>(aslot + MAXP); // FUSED_NMAP x [MAXP][NB] beta maps
 
@@ -328,9 +332,14 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                                  0xffffu);
         };
 
-        for (int e = npair * NB + tid; e < KPMAX * NB; e += nthr) {
-                buf0[e] = (BT)0;
-                buf1[e] = (BT)0;
+        if constexpr (PRESPLIT) {
+                // rows k >= npair (and the padding) stay zero; commits write the rest
+                for (int e = tid; e < (int)(2 * BB / 4); e += nthr) reinterpret_cast<uint32_t *>(bbase)[e] = 0u;
+        } else {
+                for (int e = npair * NB + tid; e < KPMAX * NB; e += nthr) {
+                        buf0[e] = (BT)0;
+                        buf1[e] = (BT)0;
+                }
         }
         for (int64_t i = tid; i < nb; i += nthr) srow[i] = 0.0;
 
@@ -340,10 +349,8 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                 // ---- builder wavefronts
                 // Issue priority: a builder shares its SIMD with two or three MFMA
                 // wavefronts, and the last builder to commit sets the chunk time,
-                // so for FP64/FP32 the builders go first (σ 184 -> 173 ms at
-                // (16o,16e) FP64).  The bf16x3 MFMA wavefronts also split B on
-                // the VALU; there they go first instead.
-                if constexpr (PREC != FCIS_PREC_BF16X3) __builtin_amdgcn_s_setprio(1);
+                // so the builders go first (FP64 σ 184 -> 173 ms at (16o,16e)).
+                __builtin_amdgcn_s_setprio(1);
                 const int bt = tid - mtiles * 64;
                 FusedBuild<NBLD> s0, s1;   // c of even / odd chunks in flight
                 for (int64_t r = blockIdx.x; r < ta; r += gridDim.x) {
@@ -384,7 +391,10 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                         fb_issue_mapv(s0, npair, nbp, cc(2), bt, pmapb);
                         fb_issue_c<NBLD, FUSED_MAXW, true>(s1, npair, nb, I, cc(1), bt, arow, nullptr, c);
                         fb_store_mapv(s1, npair, bt, mbuf(1));
-                        fb_commit_c(s0, npair, bt, buf0);
+                        if constexpr (PRESPLIT)
+                                fb_commit_split(s0, npair, bt, reinterpret_cast<__bf16 *>(buf0));
+                        else
+                                fb_commit_c(s0, npair, bt, buf0);
                         __syncthreads();   // (4)
                         // phase j: `issue` holds the map entries of chunk j+2 (loaded
                         // one phase earlier) and issues its c loads; `commit` holds
@@ -400,7 +410,11 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                                 // wavefront reads in this phase (clamped chunk data).
                                 // Every path to the loop head then has the same loads in
                                 // flight, and the commits wait for their own chunk only.
-                                fb_commit_c(commit, npair, bt, ((j + 1) & 1) ? buf1 : buf0);
+                                if constexpr (PRESPLIT)
+                                        fb_commit_split(commit, npair, bt,
+                                                        reinterpret_cast<__bf16 *>(((j + 1) & 1) ? buf1 : buf0));
+                                else
+                                        fb_commit_c(commit, npair, bt, ((j + 1) & 1) ? buf1 : buf0);
                                 fb_store_mapv(issue, npair, bt, mbuf(j + 2));
                                 __syncthreads();
                         };
@@ -421,7 +435,6 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
         }
 
         // ---- MFMA wavefronts: this wavefront's 16-row tile of h in registers
-        if constexpr (PREC == FCIS_PREC_BF16X3) __builtin_amdgcn_s_setprio(1);
         typename Ty::A ah[KS];
         typename Ty::A al[PREC == FCIS_PREC_BF16X3 ? KS : 1];
 #pragma unroll
@@ -448,7 +461,11 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                 for (int64_t j = 0; j < nl; j++) {
                         const BT *bb = (j & 1) ? buf1 : buf0;
                         const int16_t *mb = maps + (j % FUSED_NMAP) * MAXP * NB;
-                        const typename Ty::Acc acc = mma_steps<PREC, KS, KS>(ah, al, bb, lk, li);
+                        typename Ty::Acc acc;
+                        if constexpr (PRESPLIT)
+                                acc = mma_presplit<KS>(ah, al, reinterpret_cast<const __bf16 *>(bb), lk, li);
+                        else
+                                acc = mma_steps<PREC, KS, KS>(ah, al, bb, lk, li);
                         const int64_t b = (ROWS ? list_chunk(c0, j) : j) * NB + li;
                         if (b < nb) {
 #pragma unroll
