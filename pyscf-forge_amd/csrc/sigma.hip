@@ -351,7 +351,12 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                 // wavefronts, and the last builder to commit sets the chunk time,
                 // so the builders go first (FP64 σ 184 -> 173 ms at (16o,16e)).
                 __builtin_amdgcn_s_setprio(1);
-                const int bt = tid - mtiles * 64;
+                // Waves w and w + 4 share a SIMD, so with 9 MFMA wavefronts the
+                // builder at wave 12 shares its SIMD with three of them; it takes
+                // the last builder slot, whose last element group is (partly) empty.
+                int bi = wave - mtiles;
+                if (mtiles == FUSED_MAXW && NBLD == 7) bi = bi == 3 ? 6 : bi == 6 ? 3 : bi;
+                const int bt = bi * 64 + lane;
                 FusedBuild<NBLD> s0, s1;   // c of even / odd chunks in flight
                 for (int64_t r = blockIdx.x; r < ta; r += gridDim.x) {
                         const int64_t I = ROWS ? (int64_t)rows[a0 + r] : a0 + r;
