@@ -357,7 +357,32 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                 int bi = wave - mtiles;
                 if (mtiles == FUSED_MAXW && NBLD == 7) bi = bi == 3 ? 6 : bi == 6 ? 3 : bi;
                 const int bt = bi * 64 + lane;
-                FusedBuild<NBLD> s0, s1;   // c of even / odd chunks in flight
+                // FP64: that last slot gathers nothing at all; the other six
+                // builders take six element groups each (FP64 σ 173 -> 167 ms at
+                // (16o,16e); for FP32 / bf16x3 the larger builder state spills
+                // and the seventh builder pays its way).  The idle wavefront keeps
+                // the barrier schedule and its share of the σ row write-out.
+                constexpr int NBA = PREC == FCIS_PREC_FP64 ? NBLD - 1 : NBLD;   // gathering builders
+                if (bi >= NBA) {
+                        for (int64_t r = blockIdx.x; r < ta; r += gridDim.x) {
+                                const int64_t I = ROWS ? (int64_t)rows[a0 + r] : a0 + r;
+                                int64_t nl = nch;
+                                if constexpr (ROWS) nl = rowcl[a0 + r].y;
+                                __syncthreads();   // (1)
+                                __syncthreads();   // (2)
+                                __syncthreads();   // (3)
+                                __syncthreads();   // (4)
+                                for (int64_t j = 0; j < nl; j++) __syncthreads();
+                                double *dst = sigma + I * nb;
+                                for (int64_t i = tid; i < nb; i += nthr) {
+                                        double v = srow[i];
+                                        dst[i] = accumulate ? dst[i] + v : v;
+                                        srow[i] = 0.0;
+                                }
+                        }
+                        return;
+                }
+                FusedBuild<NBA> s0, s1;   // c of even / odd chunks in flight
                 for (int64_t r = blockIdx.x; r < ta; r += gridDim.x) {
                         const int64_t I = ROWS ? (int64_t)rows[a0 + r] : a0 + r;
                         int64_t nl = nch;
@@ -374,12 +399,12 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                                 return ROWS ? list_chunk(c0, jj) : (int64_t)__builtin_amdgcn_readfirstlane((int)jj);
                         };
                         __syncthreads();   // (1)
-                        for (int q = bt; q < npair; q += NBLD * 64) {
+                        for (int q = bt; q < npair; q += NBA * 64) {
                                 arow[q] = pmapa[(int64_t)q * nstra + I];
                                 aslot[q] = -1;
                         }
                         __syncthreads();   // (2)
-                        for (int k = bt; k < nlinka; k += NBLD * 64)
+                        for (int k = bt; k < nlinka; k += NBA * 64)
                                 aslot[link_pair(linka[(int64_t)k * nstra + I].info)] = k;
                         // chunk jj clamped to the row (loads past the last chunk
                         // repeat it and are never committed)
@@ -391,10 +416,10 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                         // loads left in flight here, oldest first: maps of chunk 2, c of
                         // chunk 1 -- the order the loop body leaves them in, so that the
                         // loop head sees one order and every wait stays partial
-                        fb_issue_c<NBLD, FUSED_MAXW, true>(s0, npair, nb, I, cc(0), bt, arow, nullptr, c);
+                        fb_issue_c<NBA, FUSED_MAXW, true>(s0, npair, nb, I, cc(0), bt, arow, nullptr, c);
                         fb_store_mapv(s0, npair, bt, mbuf(0));
                         fb_issue_mapv(s0, npair, nbp, cc(2), bt, pmapb);
-                        fb_issue_c<NBLD, FUSED_MAXW, true>(s1, npair, nb, I, cc(1), bt, arow, nullptr, c);
+                        fb_issue_c<NBA, FUSED_MAXW, true>(s1, npair, nb, I, cc(1), bt, arow, nullptr, c);
                         fb_store_mapv(s1, npair, bt, mbuf(1));
                         if constexpr (PRESPLIT)
                                 fb_commit_split(s0, npair, bt, reinterpret_cast<__bf16 *>(buf0));
@@ -406,9 +431,9 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                         // the c values of chunk j+1 (issued one phase earlier),
                         // committed now, and loads the map entries of chunk j+3.  The
                         // loads of this phase stay in flight across the barrier.
-                        auto phase = [&](int64_t j, FusedBuild<NBLD> &issue, FusedBuild<NBLD> &commit) {
+                        auto phase = [&](int64_t j, FusedBuild<NBA> &issue, FusedBuild<NBA> &commit) {
                                 fb_issue_mapv(commit, npair, nbp, cc(j + 3), bt, pmapb);
-                                fb_issue_c<NBLD, FUSED_MAXW, true>(issue, npair, nb, I, cc(j + 2), bt, arow,
+                                fb_issue_c<NBA, FUSED_MAXW, true>(issue, npair, nb, I, cc(j + 2), bt, arow,
                                                                    nullptr, c);
                                 // unconditional: past the row's last chunk the commit and
                                 // the map store go to the B buffer / map slice that no
