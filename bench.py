@@ -318,8 +318,9 @@ def main():
                     help="run a 4 GiB device copy first (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one MI355X: this build has no multi-GPU path "
-              "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one MI355X: there is no multi-GPU path "
+              "(the sigma plan's alpha tiles are the unit a sharded version would "
+              "split across GPUs); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import importlib

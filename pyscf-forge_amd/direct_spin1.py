@@ -1,8 +1,8 @@
 """Drop-in direct-CI solver on MI355X with pyscf's ``direct_spin1`` API.
 
 Module functions and ``FCISolver`` keep the signatures of
-pyscf.fci.direct_spin1 as the reference's solvers call them (SURVEYed
-boundary: ``absorb_h1e(h1e, eri, norb, nelec, fac=1)`` csf.py:518,
+pyscf.fci.direct_spin1 as the reference's solvers call them (the call
+sites: ``absorb_h1e(h1e, eri, norb, nelec, fac=1)`` csf.py:518,
 ``contract_2e(eri, fcivec, norb, nelec, link_index=None)`` fci_gas.py:919 /
 csf.py:525, ``make_hdiag`` fci_gas.py:904, ``kernel`` fci_gas.py:1150,
 ``make_rdm1(s)/make_rdm12(s)/trans_rdm12(s)`` fci_gas.py:1401-1483,
