@@ -20,7 +20,7 @@ constexpr int FUSED_KS = 34;      // k-steps held in registers (4-wide MFMA): np
 constexpr int FUSED_KS_BF16 = 5;  // k-steps held in registers (32-wide bf16 MFMA)
 constexpr int FUSED_NACC = 2;     // independent accumulator chains (FP64/FP32)
 constexpr int FUSED_MAXNB = 16384;  // sigma row kept in LDS; pmapb fits int16
-constexpr int FUSED_NMAP = 3;       // beta pair-map slices in LDS (chunks j .. j+2)
+constexpr int FUSED_NMAP = 4;       // beta pair-map slices in LDS (chunks j .. j+3)
 
 template <int PREC>
 struct FusedTypes;

@@ -363,16 +363,70 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                 // and the seventh builder pays its way).  The idle wavefront keeps
                 // the barrier schedule and its share of the σ row write-out.
                 constexpr int NBA = PREC == FCIS_PREC_FP64 ? NBLD - 1 : NBLD;   // gathering builders
+                constexpr bool MAPWAVE = NBA < NBLD;   // FP64: that wavefront loads the pair maps
                 if (bi >= NBA) {
+                        // pair-map wavefront: the beta pair-map slice of chunk j + 3
+                        // into LDS slice (j + 3) % 4 during phase j (loaded one phase
+                        // earlier, 16-byte loads), for the builders' gathers of phase
+                        // j + 1 and the MFMA scatter of phase j + 3
+                        constexpr int EMW = (2 * MAXP + 63) / 64;
+                        v4i m0[EMW], m1[EMW];
                         for (int64_t r = blockIdx.x; r < ta; r += gridDim.x) {
                                 const int64_t I = ROWS ? (int64_t)rows[a0 + r] : a0 + r;
-                                int64_t nl = nch;
-                                if constexpr (ROWS) nl = rowcl[a0 + r].y;
+                                int64_t nl = nch, c0 = 0;
+                                if constexpr (ROWS) {
+                                        const int2 e = rowcl[a0 + r];
+                                        c0 = e.x;
+                                        nl = e.y;
+                                }
+                                auto cc = [&](int64_t jj) -> int64_t {
+                                        const int64_t q = jj < nl ? jj : nl - 1;
+                                        return ROWS ? list_chunk(c0, q) : (int64_t)__builtin_amdgcn_readfirstlane((int)q);
+                                };
+                                auto issue = [&](v4i(&m)[EMW], int64_t jj) {
+                                        const int64_t b0 = cc(jj) * NB;
+#pragma unroll
+                                        for (int s = 0; s < EMW; s++) {
+                                                const int t = lane + s * 64, tc = t < 2 * npair ? t : 2 * npair - 1;
+                                                m[s] = *reinterpret_cast<const v4i *>(pmapb + (int64_t)(tc >> 1) * nbp + b0 +
+                                                                                      (tc & 1) * 8);
+                                        }
+                                };
+                                auto store = [&](const v4i(&m)[EMW], int64_t jj) {
+                                        int16_t *mb = maps + (jj % FUSED_NMAP) * MAXP * NB;
+#pragma unroll
+                                        for (int s = 0; s < EMW; s++) {
+                                                const int t = lane + s * 64;
+                                                if (t < 2 * npair)
+                                                        *reinterpret_cast<v4i *>(mb + (t >> 1) * NB + (t & 1) * 8) = m[s];
+                                        }
+                                };
                                 __syncthreads();   // (1)
                                 __syncthreads();   // (2)
+                                if constexpr (MAPWAVE) {
+                                        issue(m0, 0);
+                                        store(m0, 0);
+                                        issue(m0, 1);
+                                        store(m0, 1);
+                                        issue(m0, 2);
+                                        store(m0, 2);
+                                        issue(m0, 3);
+                                }
                                 __syncthreads();   // (3)
                                 __syncthreads();   // (4)
-                                for (int64_t j = 0; j < nl; j++) __syncthreads();
+                                auto mphase = [&](int64_t j, v4i(&ld_)[EMW], v4i(&st_)[EMW]) {
+                                        if constexpr (MAPWAVE) {
+                                                issue(ld_, j + 4);
+                                                store(st_, j + 3);
+                                        }
+                                        __syncthreads();
+                                };
+                                int64_t j = 0;
+                                for (; j + 1 < nl; j += 2) {
+                                        mphase(j, m1, m0);
+                                        mphase(j + 1, m0, m1);
+                                }
+                                if (j < nl) mphase(j, m1, m0);
                                 double *dst = sigma + I * nb;
                                 for (int64_t i = tid; i < nb; i += nthr) {
                                         double v = srow[i];
@@ -410,17 +464,24 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                         // repeat it and are never committed)
                         auto cc = [&](int64_t jj) -> int64_t { return chunk(jj < nl ? jj : nl - 1); };
                         auto mbuf = [&](int64_t jj) -> int16_t * { return maps + (jj % FUSED_NMAP) * MAXP * NB; };
-                        fb_issue_mapv(s0, npair, nbp, cc(0), bt, pmapb);
-                        fb_issue_mapv(s1, npair, nbp, cc(1), bt, pmapb);
+                        if constexpr (!MAPWAVE) {
+                                fb_issue_mapv(s0, npair, nbp, cc(0), bt, pmapb);
+                                fb_issue_mapv(s1, npair, nbp, cc(1), bt, pmapb);
+                        }
                         __syncthreads();   // (3)
                         // loads left in flight here, oldest first: maps of chunk 2, c of
                         // chunk 1 -- the order the loop body leaves them in, so that the
                         // loop head sees one order and every wait stays partial
-                        fb_issue_c<NBA, FUSED_MAXW, true>(s0, npair, nb, I, cc(0), bt, arow, nullptr, c);
-                        fb_store_mapv(s0, npair, bt, mbuf(0));
-                        fb_issue_mapv(s0, npair, nbp, cc(2), bt, pmapb);
-                        fb_issue_c<NBA, FUSED_MAXW, true>(s1, npair, nb, I, cc(1), bt, arow, nullptr, c);
-                        fb_store_mapv(s1, npair, bt, mbuf(1));
+                        if constexpr (MAPWAVE) {
+                                fb_issue_c<NBA, FUSED_MAXW, false>(s0, npair, nb, I, cc(0), bt, arow, mbuf(0), c);
+                                fb_issue_c<NBA, FUSED_MAXW, false>(s1, npair, nb, I, cc(1), bt, arow, mbuf(1), c);
+                        } else {
+                                fb_issue_c<NBA, FUSED_MAXW, true>(s0, npair, nb, I, cc(0), bt, arow, nullptr, c);
+                                fb_store_mapv(s0, npair, bt, mbuf(0));
+                                fb_issue_mapv(s0, npair, nbp, cc(2), bt, pmapb);
+                                fb_issue_c<NBA, FUSED_MAXW, true>(s1, npair, nb, I, cc(1), bt, arow, nullptr, c);
+                                fb_store_mapv(s1, npair, bt, mbuf(1));
+                        }
                         if constexpr (PRESPLIT)
                                 fb_commit_split(s0, npair, bt, reinterpret_cast<__bf16 *>(buf0));
                         else
@@ -432,9 +493,14 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                         // committed now, and loads the map entries of chunk j+3.  The
                         // loads of this phase stay in flight across the barrier.
                         auto phase = [&](int64_t j, FusedBuild<NBA> &issue, FusedBuild<NBA> &commit) {
-                                fb_issue_mapv(commit, npair, nbp, cc(j + 3), bt, pmapb);
-                                fb_issue_c<NBA, FUSED_MAXW, true>(issue, npair, nb, I, cc(j + 2), bt, arow,
-                                                                   nullptr, c);
+                                if constexpr (MAPWAVE) {
+                                        fb_issue_c<NBA, FUSED_MAXW, false>(issue, npair, nb, I, cc(j + 2), bt, arow,
+                                                                            mbuf(j + 2), c);
+                                } else {
+                                        fb_issue_mapv(commit, npair, nbp, cc(j + 3), bt, pmapb);
+                                        fb_issue_c<NBA, FUSED_MAXW, true>(issue, npair, nb, I, cc(j + 2), bt, arow,
+                                                                           nullptr, c);
+                                }
                                 // unconditional: past the row's last chunk the commit and
                                 // the map store go to the B buffer / map slice that no
                                 // wavefront reads in this phase (clamped chunk data).
@@ -445,7 +511,7 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                                                         reinterpret_cast<__bf16 *>(((j + 1) & 1) ? buf1 : buf0));
                                 else
                                         fb_commit_c(commit, npair, bt, ((j + 1) & 1) ? buf1 : buf0);
-                                fb_store_mapv(issue, npair, bt, mbuf(j + 2));
+                                if constexpr (!MAPWAVE) fb_store_mapv(issue, npair, bt, mbuf(j + 2));
                                 __syncthreads();
                         };
                         int64_t j = 0;
