@@ -357,13 +357,16 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                 int bi = wave - mtiles;
                 if (mtiles == FUSED_MAXW && NBLD == 7) bi = bi == 3 ? 6 : bi == 6 ? 3 : bi;
                 const int bt = bi * 64 + lane;
-                // FP64: that last slot gathers nothing at all; the other six
-                // builders take six element groups each (FP64 σ 173 -> 167 ms at
-                // (16o,16e); for FP32 / bf16x3 the larger builder state spills
-                // and the seventh builder pays its way).  The idle wavefront keeps
-                // the barrier schedule and its share of the σ row write-out.
-                constexpr int NBA = PREC == FCIS_PREC_FP64 ? NBLD - 1 : NBLD;   // gathering builders
-                constexpr bool MAPWAVE = NBA < NBLD;   // FP64: that wavefront loads the pair maps
+                // FP64 / FP32: that last slot gathers no c values; it loads the
+                // beta pair maps of all builders into LDS instead (16-byte loads),
+                // and the other six builders take six element groups each and
+                // read their map entries from LDS (FP64 σ 173 -> 160 ms, FP32
+                // 117 -> 106 ms at (16o,16e)).  bf16x3 keeps seven gathering
+                // builders with register-held map entries (1% faster there).
+                // The map wavefront also keeps the barrier schedule and does its
+                // share of the σ row write-out.
+                constexpr int NBA = PREC == FCIS_PREC_BF16X3 ? NBLD : NBLD - 1;   // gathering builders
+                constexpr bool MAPWAVE = NBA < NBLD;
                 if (bi >= NBA) {
                         // pair-map wavefront: the beta pair-map slice of chunk j + 3
                         // into LDS slice (j + 3) % 4 during phase j (loaded one phase
