@@ -55,6 +55,8 @@ def _load():
                                                                      ctypes.c_int64]
         lib.orc_make_hdiag.argtypes = [ctypes.c_int] * 3 + [_dp, _dp, _dp]
         lib.orc_trans_rdm12s.argtypes = [ctypes.c_int] * 3 + [_dp] * 7
+        lib.orc_sigma_rows.argtypes = [ctypes.c_int] * 3 + [_dp] * 3 + [
+            ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), _dp]
         _lib = lib
     return _lib
 
@@ -153,6 +155,25 @@ def contract_2e_sample(h2e, fcivec, norb, nelec, ncols):
     c = _c64(fcivec).reshape(-1)
     out = numpy.empty_like(c)
     _load().orc_contract_2e_sample(norb, na, nb, _d(h2e), _d(c), _d(out), int(ncols))
+    return out
+
+
+def sigma_rows(h1e, eri, ci, norb, nelec, dets):
+    """(H c)[d] for the determinant indices ``dets`` (alpha * nstrb + beta)
+    by the Slater-Condon rules on the un-absorbed integrals (fci_oracle.c
+    orc_sigma_rows): spot checks of sigma where no full CPU sigma fits."""
+    na, nb = _unpack_nelec(nelec)
+    h1e = _c64(h1e, (norb, norb))
+    eri = pair_matrix(eri, norb)
+    c = _c64(ci).reshape(-1)
+    if c.size != num_strings(norb, na) * num_strings(norb, nb):
+        raise ValueError("CI vector size mismatch")
+    dets = numpy.ascontiguousarray(dets, dtype=numpy.int64).reshape(-1)
+    if dets.size and (dets.min() < 0 or dets.max() >= c.size):
+        raise ValueError("determinant index out of range")
+    out = numpy.empty(dets.size)
+    _load().orc_sigma_rows(norb, na, nb, _d(h1e), _d(eri), _d(c), dets.size,
+                           dets.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), _d(out))
     return out
 
 

@@ -20,6 +20,7 @@
  *   contract_2e            libfci_gas golden vectors, brute-force Slater-Condon
  *   make_hdiag             libfci_gas golden vectors, diag of brute-force H
  *   trans_rdm12s           libfci_gas golden vectors, E = sum h dm1 + 1/2 eri dm2
+ *   sigma_rows             brute-force H rows, libfci_gas golden sigma entries
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -575,5 +576,198 @@ int orc_trans_rdm12s(int n, int na, int nb, const double *bra,
         free(acc);
         free(la);
         free(lb);
+        return 0;
+}
+
+/* ------------------------------------------------------------------------
+ * Rows of H by the Slater-Condon rules, for spot checks of sigma on spaces
+ * too large for any full CPU sigma: out[i] = sum_j <D_i|H|D_j> c[j] over
+ * the determinants j connected to D_i = dets[i] (CAS index Ia*nstrb + Ib),
+ * from the UN-absorbed integrals, h1e (n,n) and eri as the (npair,npair)
+ * 4-fold pair matrix:
+ *   H = sum_pq h_pq E_pq + 1/2 sum_pqrs (pq|rs) (E_pq E_rs - delta_qr E_ps).
+ * Independent of link tables and of absorb_h1e: every matrix element is the
+ * textbook Slater-Condon expression, and every phase is obtained by applying
+ * the annihilation / creation operators one at a time to the bit string
+ * (a sign (-1)^{#occupied below the orbital} per operator, the convention of
+ * pyscf's cistring used by direct_spin1, whose sigma the reference calls at
+ * pyscf/csf_fci/csf.py:449).
+ *   diagonal      sum_i h_ii + sum_{i<j same spin} [(ii|jj) - (ij|ji)]
+ *                 + sum_{i alpha, j beta} (ii|jj)
+ *   single q->p   phase * [h_pq + sum_{j occ, same spin} ((pq|jj) - (pj|jq))
+ *                 + sum_{j occ, other spin} (pq|jj)]
+ *   double, same spin q1<q2 -> p1<p2
+ *                 phase(a+_p1 a+_p2 a_q2 a_q1) * [(p1q1|p2q2) - (p1q2|p2q1)]
+ *   double, alpha q->p and beta s->r
+ *                 phase_alpha * phase_beta * (pq|rs)
+ * Pinned by tests/test_oracle.py against the from-scratch brute-force H and
+ * against the libfci_gas golden sigma vectors. */
+
+static int64_t g_binom[ORC_MAXORB + 1][ORC_MAXORB + 1];
+
+static void init_binom(void)
+{
+        for (int p = 0; p <= ORC_MAXORB; p++) {
+                for (int k = 0; k <= ORC_MAXORB; k++) {
+                        g_binom[p][k] = binom64(p, k);
+                }
+        }
+}
+
+static int64_t rank_str(uint64_t s)
+{
+        int64_t addr = 0;
+        int i = 1;
+        while (s) {
+                int p = __builtin_ctzll(s);
+                addr += g_binom[p][i++];
+                s &= s - 1;
+        }
+        return addr;
+}
+
+/* Apply a_q (cre = 0) or a_q^+ (cre = 1) to *s; returns the sign, 0 if the
+ * result vanishes. */
+static int apply_op(uint64_t *s, int q, int cre)
+{
+        uint64_t bit = 1ULL << q;
+        if (cre ? (*s & bit) != 0 : (*s & bit) == 0) {
+                return 0;
+        }
+        int sg = (popc(*s & (bit - 1)) & 1) ? -1 : 1;
+        *s ^= bit;
+        return sg;
+}
+
+static double eri_at(const double *eri, int np, int p, int q, int r, int s)
+{
+        return eri[(int64_t)tril(p, q) * np + tril(r, s)];
+}
+
+/* Same-spin singles and doubles of string s (other-spin string t fixed)
+ * contribute out += sum elem * c[addr]; `alpha` selects which index moves. */
+static double same_spin_part(int n, int np, const double *h1e, const double *eri,
+                             uint64_t s, uint64_t t, int alpha, int64_t fixed,
+                             int64_t nsb, const double *ci)
+{
+        double acc = 0.0;
+        int occ[ORC_MAXORB], vir[ORC_MAXORB], no = 0, nv = 0;
+        for (int i = 0; i < n; i++) {
+                if ((s >> i) & 1) {
+                        occ[no++] = i;
+                } else {
+                        vir[nv++] = i;
+                }
+        }
+        int tocc[ORC_MAXORB], nt = 0;
+        for (int i = 0; i < n; i++) {
+                if ((t >> i) & 1) {
+                        tocc[nt++] = i;
+                }
+        }
+        /* singles */
+        for (int a = 0; a < no; a++) {
+                int q = occ[a];
+                for (int b = 0; b < nv; b++) {
+                        int p = vir[b];
+                        uint64_t s1 = s;
+                        int sg = apply_op(&s1, q, 0);
+                        sg *= apply_op(&s1, p, 1);
+                        double v = h1e[p * n + q];
+                        for (int j = 0; j < no; j++) {
+                                int o = occ[j];
+                                v += eri_at(eri, np, p, q, o, o) - eri_at(eri, np, p, o, o, q);
+                        }
+                        for (int j = 0; j < nt; j++) {
+                                v += eri_at(eri, np, p, q, tocc[j], tocc[j]);
+                        }
+                        int64_t a1 = rank_str(s1);
+                        int64_t idx = alpha ? a1 * nsb + fixed : fixed * nsb + a1;
+                        acc += sg * v * ci[idx];
+                }
+        }
+        /* doubles */
+        for (int a1 = 0; a1 < no; a1++) {
+                for (int a2 = a1 + 1; a2 < no; a2++) {
+                        int q1 = occ[a1], q2 = occ[a2];
+                        for (int b1 = 0; b1 < nv; b1++) {
+                                for (int b2 = b1 + 1; b2 < nv; b2++) {
+                                        int p1 = vir[b1], p2 = vir[b2];
+                                        uint64_t s1 = s;
+                                        int sg = apply_op(&s1, q1, 0);
+                                        sg *= apply_op(&s1, q2, 0);
+                                        sg *= apply_op(&s1, p2, 1);
+                                        sg *= apply_op(&s1, p1, 1);
+                                        double v = eri_at(eri, np, p1, q1, p2, q2) -
+                                                   eri_at(eri, np, p1, q2, p2, q1);
+                                        int64_t ad = rank_str(s1);
+                                        int64_t idx = alpha ? ad * nsb + fixed : fixed * nsb + ad;
+                                        acc += sg * v * ci[idx];
+                                }
+                        }
+                }
+        }
+        return acc;
+}
+
+int orc_sigma_rows(int n, int na, int nb, const double *h1e, const double *eri,
+                   const double *ci, int64_t m, const int64_t *dets, double *out)
+{
+        if (n < 0 || n > ORC_MAXORB || na < 0 || nb < 0 || na > n || nb > n) {
+                return -1;
+        }
+        init_binom();
+        const int np = n * (n + 1) / 2;
+        const int64_t nsa = binom64(n, na), nsb = binom64(n, nb);
+#pragma omp parallel for schedule(dynamic)
+        for (int64_t i = 0; i < m; i++) {
+                const int64_t d = dets[i];
+                const int64_t Ia = d / nsb, Ib = d - Ia * nsb;
+                if (Ia < 0 || Ia >= nsa) {
+                        out[i] = NAN;
+                        continue;
+                }
+                const uint64_t sa = orc_addr2str(n, na, Ia), sb = orc_addr2str(n, nb, Ib);
+                /* diagonal */
+                double e0 = 0.0;
+                for (int p = 0; p < n; p++) {
+                        int oa = (sa >> p) & 1, ob = (sb >> p) & 1;
+                        e0 += (oa + ob) * h1e[p * n + p];
+                        for (int q = 0; q < p; q++) {
+                                double J = eri_at(eri, np, p, p, q, q), K = eri_at(eri, np, p, q, q, p);
+                                if (oa && ((sa >> q) & 1)) e0 += J - K;
+                                if (ob && ((sb >> q) & 1)) e0 += J - K;
+                        }
+                        for (int q = 0; q < n; q++) {
+                                if (oa && ((sb >> q) & 1)) e0 += eri_at(eri, np, p, p, q, q);
+                        }
+                }
+                double acc = e0 * ci[d];
+                acc += same_spin_part(n, np, h1e, eri, sa, sb, 1, Ib, nsb, ci);
+                acc += same_spin_part(n, np, h1e, eri, sb, sa, 0, Ia, nsb, ci);
+                /* alpha-beta doubles */
+                for (int q = 0; q < n; q++) {
+                        if (!((sa >> q) & 1)) continue;
+                        for (int p = 0; p < n; p++) {
+                                if (p == q || ((sa >> p) & 1)) continue;
+                                uint64_t ta = sa;
+                                int sga = apply_op(&ta, q, 0);
+                                sga *= apply_op(&ta, p, 1);
+                                const int64_t ja = rank_str(ta);
+                                for (int s = 0; s < n; s++) {
+                                        if (!((sb >> s) & 1)) continue;
+                                        for (int r = 0; r < n; r++) {
+                                                if (r == s || ((sb >> r) & 1)) continue;
+                                                uint64_t tb = sb;
+                                                int sgb = apply_op(&tb, s, 0);
+                                                sgb *= apply_op(&tb, r, 1);
+                                                acc += sga * sgb * eri_at(eri, np, p, q, r, s) *
+                                                       ci[ja * nsb + rank_str(tb)];
+                                        }
+                                }
+                        }
+                }
+                out[i] = acc;
+        }
         return 0;
 }

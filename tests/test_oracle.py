@@ -150,3 +150,52 @@ def test_lowest_eigenvalue_matches_dense():
     w = numpy.linalg.eigvalsh(brute.hamiltonian(h1e, eri, n, ne))
     e, v = oracle.lowest_eigenvalues(h1e, eri, n, ne, nroots=2)
     numpy.testing.assert_allclose(e, w[:2], atol=1e-10)
+
+
+# --- Slater-Condon rows (orc_sigma_rows): the spot-check oracle for spaces
+# too large for a full CPU sigma.  Pinned against the brute-force H (which
+# builds H from explicit operator products, no Slater-Condon rules) and
+# against the reference's libfci_gas sigma vectors.
+
+@pytest.mark.parametrize("n,ne,seed", [(4, (2, 2), 0), (5, (3, 1), 1), (6, (3, 3), 2),
+                                       (6, (0, 2), 3), (5, (2, 0), 4), (6, (4, 2), 5)])
+def test_sigma_rows_vs_brute_force(n, ne, seed):
+    h1e, eri, c = syn.make_problem(n, ne, seed=seed)
+    H = brute.hamiltonian(h1e, eri, n, ne)
+    dets = numpy.arange(H.shape[0])
+    got = oracle.sigma_rows(h1e, eri, c, n, ne, dets)
+    numpy.testing.assert_allclose(got, H @ c.reshape(-1), rtol=0, atol=1e-13)
+    # unit vectors give whole rows of H
+    for j in (0, H.shape[0] // 2, H.shape[0] - 1):
+        e = numpy.zeros(H.shape[0])
+        e[j] = 1.0
+        numpy.testing.assert_allclose(oracle.sigma_rows(h1e, eri, e, n, ne, dets), H[:, j],
+                                      rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("name", golden_cases())
+def test_sigma_rows_vs_golden(name):
+    g = load_golden(name)
+    n, ne = g["norb"], g["nelec"]
+    ci = g["ci"].reshape(-1)
+    rng = numpy.random.default_rng(11)
+    dets = numpy.unique(numpy.concatenate([[0, ci.size - 1],
+                                           rng.integers(0, ci.size, min(ci.size, 200))]))
+    got = oracle.sigma_rows(g["h1e"], g["eri"], ci, n, ne, dets)
+    ref = g["sigma"].reshape(-1)[dets]
+    numpy.testing.assert_allclose(got, ref, rtol=0, atol=1e-13 * numpy.abs(g["sigma"]).max())
+
+
+def test_sigma_rows_vs_contract_2e_12_orbitals():
+    n, ne = 12, (6, 5)
+    h1e, eri, c = syn.make_problem(n, ne, seed=9)
+    ref = oracle.contract_2e(oracle.absorb_h1e(h1e, eri, n, ne, 0.5), c, n, ne).reshape(-1)
+    dets = numpy.random.default_rng(3).integers(0, ref.size, 300)
+    got = oracle.sigma_rows(h1e, eri, c, n, ne, dets)
+    numpy.testing.assert_allclose(got, ref[dets], rtol=0, atol=1e-13 * numpy.abs(ref).max())
+
+
+def test_sigma_rows_rejects_bad_index():
+    h1e, eri, c = syn.make_problem(4, (2, 2), seed=0)
+    with pytest.raises(ValueError):
+        oracle.sigma_rows(h1e, eri, c, 4, (2, 2), [c.size])
