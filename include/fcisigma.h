@@ -157,6 +157,23 @@ int fcis_trans_rdm12s(const fcis_space_t *space, const double *d_bra,
                       const double *d_ket, double *dm1a, double *dm1b,
                       double *dm2aa, double *dm2ab, double *dm2bb,
                       void *stream);
+/* fcis_trans_rdm12s without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving or optimisation loops, harnesses that
+ * forbid hipMalloc).  Same arguments, conventions and outputs; its
+ * intermediates (the T arrays of an alpha tile, per-chunk partial products,
+ * the Gram blocks and the assembled RDMs) live in one fixed per-device
+ * scratch array of 128 MiB that the library holds on every GPU it runs on
+ * until the process ends, tiled over alpha strings to fit.  Spaces whose
+ * single alpha string needs more than that (nstr_beta * (norb^2 + 1)
+ * roughly above 4 M, e.g. norb = 18 with 9 beta electrons) are rejected with
+ * FCIS_ERR_INVALID.  Calls on one GPU share the array: calls that overlap
+ * on that GPU (other streams or threads) race.  The call waits for its
+ * stream before returning (host outputs). */
+int fcis_trans_rdm12s_noalloc(const fcis_space_t *space, const double *d_bra,
+                              const double *d_ket, double *dm1a, double *dm1b,
+                              double *dm2aa, double *dm2ab, double *dm2bb,
+                              void *stream);
 
 /*
  * Hamiltonian block over m selected determinants (pspace): d_h0[i][j] =

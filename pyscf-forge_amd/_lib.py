@@ -54,6 +54,7 @@ SIGNATURES = {
     "fcis_sigma_plan_get_timing": (ctypes.c_int, [_vp, _dp, _i64p]),
     "fcis_make_hdiag": (ctypes.c_int, [_vp, _dp, _dp, _vp, _vp]),
     "fcis_trans_rdm12s": (ctypes.c_int, [_vp, _vp, _vp, _dp, _dp, _dp, _dp, _dp, _vp]),
+    "fcis_trans_rdm12s_noalloc": (ctypes.c_int, [_vp, _vp, _vp, _dp, _dp, _dp, _dp, _dp, _vp]),
     "fcis_pspace": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int, _vp, _vp, _vp]),
     "fcis_pspace_uhf": (ctypes.c_int, [_vp, _dp, _dp, _dp, ctypes.c_int, _vp, _vp, _vp]),
     "fcis_contract_1e_ab": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),

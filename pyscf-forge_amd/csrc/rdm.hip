@@ -12,6 +12,7 @@
 // the same quantities path by path (fci_rdm_gas.c:318-604, :1334-1465).
 #include <stdlib.h>
 #include <string.h>
+#include <mutex>
 #include <vector>
 
 #include "common.h"
@@ -102,6 +103,140 @@ __global__ void k_rdm_reduce(int64_t nelem, int nchunk, int accumulate,
         out[e] = s;
 }
 
+
+// Buffers of one RDM evaluation: T (four [nd][wp] arrays for the bra and
+// ket, alpha and beta), part (per-chunk partial products) and G (the three
+// [wp][wp] Gram blocks aa, ab, bb), for alpha tiles of ta strings.
+struct RdmBuffers {
+        double *T, *part, *G;
+        int64_t ta;
+};
+
+struct RdmShape {
+        int n, wp, nblk;
+        int64_t nb, na;
+        size_t W2;
+};
+
+RdmShape rdm_shape(const fcis_space *s)
+{
+        RdmShape r;
+        r.n = s->norb;
+        r.wp = (r.n * r.n + 1 + 15) / 16 * 16;
+        r.nblk = r.wp / 16;
+        r.nb = s->nstrb;
+        r.na = s->nstra;
+        r.W2 = (size_t)r.wp * r.wp;
+        return r;
+}
+
+// Bytes of T and part for a tile of ta alpha strings.
+size_t rdm_tile_bytes(const RdmShape &r, int64_t ta)
+{
+        const int64_t nd = ta * r.nb;
+        const int64_t nchunk = (nd + RDM_KCHUNK - 1) / RDM_KCHUNK;
+        return 4 * (size_t)nd * r.wp * 8 + (size_t)nchunk * r.W2 * 8;
+}
+
+// G = the three Gram blocks of bra and ket over all alpha tiles (launches
+// only; stream-ordered).
+int rdm_gram(const fcis_space *s, const RdmShape &r, const RdmBuffers &b, const double *d_bra,
+             const double *d_ket, hipStream_t st)
+{
+        const int64_t ndt = b.ta * r.nb;
+        double *Tab = b.T, *Tbb = b.T + (size_t)ndt * r.wp;                          // bra alpha / beta
+        double *Tak = b.T + 2 * (size_t)ndt * r.wp, *Tbk = b.T + 3 * (size_t)ndt * r.wp;  // ket
+        for (int64_t a0 = 0; a0 < r.na; a0 += b.ta) {
+                int64_t t = r.na - a0 < b.ta ? r.na - a0 : b.ta;
+                int64_t nd = t * r.nb;
+                int64_t nchunk = (nd + RDM_KCHUNK - 1) / RDM_KCHUNK;
+                hipLaunchKernelGGL(k_rdm_build_t, dim3((nd + 255) / 256), dim3(256), 0, st, r.n, r.nb,
+                                   r.na, a0, nd, r.wp, s->nlinka, s->nlinkb, s->d_linka, s->d_linkb,
+                                   d_bra, Tab, Tbb);
+                hipLaunchKernelGGL(k_rdm_build_t, dim3((nd + 255) / 256), dim3(256), 0, st, r.n, r.nb,
+                                   r.na, a0, nd, r.wp, s->nlinka, s->nlinkb, s->d_linka, s->d_linkb,
+                                   d_ket, Tak, Tbk);
+                const double *As[3] = {Tab, Tab, Tbb};
+                const double *Bs[3] = {Tak, Tbk, Tbk};
+                for (int g = 0; g < 3; g++) {
+                        hipLaunchKernelGGL(k_rdm_gemm, dim3(nchunk, (r.nblk * r.nblk + 3) / 4),
+                                           dim3(256), 0, st, nd, r.wp, r.nblk, As[g], Bs[g], b.part);
+                        hipLaunchKernelGGL(k_rdm_reduce, dim3((r.W2 + 255) / 256), dim3(256), 0, st,
+                                           (int64_t)r.W2, (int)nchunk, a0 > 0 ? 1 : 0, b.part,
+                                           b.G + g * r.W2);
+                }
+                if (hipGetLastError() != hipSuccess) {
+                        fcis::set_error("trans_rdm12s: kernel launch failed");
+                        return FCIS_ERR_DEVICE;
+                }
+        }
+        return FCIS_SUCCESS;
+}
+
+// The RDMs in their final layout from G, on the device:
+//   out = [dm1a n^2][dm1b n^2][dm2aa n^4][dm2ab n^4][dm2bb n^4]
+//   dm1^s[p][q] = G^ss[n^2][(q,p)],
+//   dm2^st[p][q][r][t] = G^st[(q,p)][(r,t)] - delta_st delta_qr dm1^s[t][p].
+__global__ void k_rdm_assemble(int n, int wp, const double *__restrict__ G, double *__restrict__ out)
+{
+        const int n2 = n * n;
+        const int64_t n4 = (int64_t)n2 * n2, W2 = (int64_t)wp * wp;
+        const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (e >= 2 * n2 + 3 * n4) return;
+        if (e < 2 * n2) {
+                const int sp = (int)(e / n2), idx = (int)(e - (int64_t)sp * n2);
+                const int p = idx / n, q = idx - p * n;
+                out[e] = G[(sp ? 2 : 0) * W2 + (int64_t)n2 * wp + q * n + p];
+                return;
+        }
+        const int64_t f = e - 2 * n2;
+        const int g = (int)(f / n4);
+        const int64_t idx = f - g * n4;
+        const int p = (int)(idx / ((int64_t)n * n2)), q = (int)(idx / n2 % n), r = (int)(idx / n % n),
+                  t = (int)(idx % n);
+        double v = G[g * W2 + (int64_t)(q * n + p) * wp + r * n + t];
+        if (g != 1 && q == r) v -= G[g * W2 + (int64_t)n2 * wp + p * n + t];
+        out[e] = v;
+}
+
+// Scratch of the non-allocating RDM entry point, one copy per device:
+// G, the per-chunk partial products and the T arrays of an alpha tile, then
+// the assembled RDMs.  The library holds it on every GPU it runs on until
+// the process ends.
+constexpr size_t RDM_SCRATCH_BYTES = (size_t)128 << 20;
+static __device__ __align__(256) char scratch[RDM_SCRATCH_BYTES];
+
+// Device address of `scratch` per device, from hipGetSymbolAddress on the
+// first call on that device (with that device current).
+constexpr int RDM_MAXDEV = 64;
+void *g_scratch_ptr[RDM_MAXDEV];
+std::mutex g_scratch_mu;
+
+int scratch_ptr(char **out)
+{
+        int dev = 0;
+        FCIS_HIP(hipGetDevice(&dev));
+        if (dev < 0 || dev >= RDM_MAXDEV) {
+                fcis::set_error("fcis_trans_rdm12s_noalloc: device %d out of range", dev);
+                return FCIS_ERR_INVALID;
+        }
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (g_scratch_ptr[dev] == nullptr) {
+                void *p = nullptr;
+                hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+                if (e != hipSuccess || p == nullptr) {
+                        fcis::set_error("fcis_trans_rdm12s_noalloc: hipGetSymbolAddress: %s",
+                                        hipGetErrorString(e));
+                        return FCIS_ERR_DEVICE;
+                }
+                g_scratch_ptr[dev] = p;
+        }
+        *out = static_cast<char *>(g_scratch_ptr[dev]);
+        return FCIS_SUCCESS;
+}
+
+size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
 }  // namespace
 
 extern "C" int fcis_trans_rdm12s(const fcis_space_t *s, const double *d_bra, const double *d_ket,
@@ -113,17 +248,15 @@ extern "C" int fcis_trans_rdm12s(const fcis_space_t *s, const double *d_bra, con
                 return FCIS_ERR_INVALID;
         }
         hipStream_t st = (hipStream_t)stream;
-        const int n = s->norb, n2 = n * n;
-        const int wp = (n2 + 1 + 15) / 16 * 16;
-        const int nblk = wp / 16;
-        const int64_t nb = s->nstrb, na = s->nstra;
-        const size_t W2 = (size_t)wp * wp;
+        const RdmShape r = rdm_shape(s);
+        const int n = r.n, n2 = n * n, wp = r.wp;
+        const size_t W2 = r.W2;
         // alpha tile sized so that four T arrays stay within ~2 GiB
-        int64_t per_alpha = 4 * nb * wp * 8;
-        int64_t ta = per_alpha ? ((int64_t)2 << 30) / per_alpha : na;
+        int64_t per_alpha = 4 * r.nb * wp * 8;
+        int64_t ta = per_alpha ? ((int64_t)2 << 30) / per_alpha : r.na;
         if (ta < 1) ta = 1;
-        if (ta > na) ta = na;
-        int64_t ndt = ta * nb;
+        if (ta > r.na) ta = r.na;
+        int64_t ndt = ta * r.nb;
         int64_t maxchunk = (ndt + RDM_KCHUNK - 1) / RDM_KCHUNK;
         double *T = nullptr, *part = nullptr, *G = nullptr;
         size_t tbytes = 4 * (size_t)ndt * wp * 8;
@@ -135,33 +268,7 @@ extern "C" int fcis_trans_rdm12s(const fcis_space_t *s, const double *d_bra, con
                 fcis::set_error("fcis_trans_rdm12s: device allocation failed");
                 return FCIS_ERR_MEMORY;
         }
-        double *Tab = T, *Tbb = T + (size_t)ndt * wp;        // bra alpha / beta
-        double *Tak = T + 2 * (size_t)ndt * wp, *Tbk = T + 3 * (size_t)ndt * wp;  // ket
-        int ret = FCIS_SUCCESS;
-        for (int64_t a0 = 0; a0 < na && ret == FCIS_SUCCESS; a0 += ta) {
-                int64_t t = na - a0 < ta ? na - a0 : ta;
-                int64_t nd = t * nb;
-                int64_t nchunk = (nd + RDM_KCHUNK - 1) / RDM_KCHUNK;
-                hipLaunchKernelGGL(k_rdm_build_t, dim3((nd + 255) / 256), dim3(256), 0, st, n, nb,
-                                   na, a0, nd, wp, s->nlinka, s->nlinkb, s->d_linka, s->d_linkb,
-                                   d_bra, Tab, Tbb);
-                hipLaunchKernelGGL(k_rdm_build_t, dim3((nd + 255) / 256), dim3(256), 0, st, n, nb,
-                                   na, a0, nd, wp, s->nlinka, s->nlinkb, s->d_linka, s->d_linkb,
-                                   d_ket, Tak, Tbk);
-                const double *As[3] = {Tab, Tab, Tbb};
-                const double *Bs[3] = {Tak, Tbk, Tbk};
-                for (int g = 0; g < 3; g++) {
-                        hipLaunchKernelGGL(k_rdm_gemm, dim3(nchunk, (nblk * nblk + 3) / 4),
-                                           dim3(256), 0, st, nd, wp, nblk, As[g], Bs[g], part);
-                        hipLaunchKernelGGL(k_rdm_reduce, dim3((W2 + 255) / 256), dim3(256), 0, st,
-                                           (int64_t)W2, (int)nchunk, a0 > 0 ? 1 : 0, part,
-                                           G + g * W2);
-                }
-                if (hipGetLastError() != hipSuccess) {
-                        fcis::set_error("fcis_trans_rdm12s: kernel launch failed");
-                        ret = FCIS_ERR_DEVICE;
-                }
-        }
+        int ret = rdm_gram(s, r, RdmBuffers{T, part, G, ta}, d_bra, d_ket, st);
         std::vector<double> h(3 * W2);
         if (ret == FCIS_SUCCESS &&
             (hipMemcpyAsync(h.data(), G, 3 * W2 * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -189,13 +296,68 @@ extern "C" int fcis_trans_rdm12s(const fcis_space_t *s, const double *d_bra, con
                 if (o == nullptr) continue;
                 for (int p = 0; p < n; p++)
                         for (int q = 0; q < n; q++)
-                                for (int r = 0; r < n; r++)
+                                for (int r2 = 0; r2 < n; r2++)
                                         for (int t = 0; t < n; t++) {
-                                                double v = Gs[g][(size_t)(q * n + p) * wp + r * n + t];
-                                                if (g != 1 && q == r)
+                                                double v = Gs[g][(size_t)(q * n + p) * wp + r2 * n + t];
+                                                if (g != 1 && q == r2)
                                                         v -= (g == 0 ? d1a : d1b)[t * n + p];
-                                                o[(((size_t)p * n + q) * n + r) * n + t] = v;
+                                                o[(((size_t)p * n + q) * n + r2) * n + t] = v;
                                         }
         }
+        return FCIS_SUCCESS;
+}
+
+extern "C" int fcis_trans_rdm12s_noalloc(const fcis_space_t *s, const double *d_bra,
+                                         const double *d_ket, double *dm1a, double *dm1b,
+                                         double *dm2aa, double *dm2ab, double *dm2bb, void *stream)
+{
+        if (s == nullptr || d_bra == nullptr || d_ket == nullptr) {
+                fcis::set_error("fcis_trans_rdm12s_noalloc: null argument");
+                return FCIS_ERR_INVALID;
+        }
+        hipStream_t st = (hipStream_t)stream;
+        const RdmShape r = rdm_shape(s);
+        const int n2 = r.n * r.n;
+        const int64_t n4 = (int64_t)n2 * n2;
+        // scratch layout: G | assembled RDMs | part | T
+        const size_t gbytes = align256(3 * r.W2 * 8);
+        const size_t obytes = align256((size_t)(2 * n2 + 3 * n4) * 8);
+        const size_t fixed = gbytes + obytes;
+        int64_t ta = r.na;
+        while (ta > 1 && fixed + rdm_tile_bytes(r, ta) + 512 > RDM_SCRATCH_BYTES) ta = (ta + 1) / 2;
+        if (r.na > 0 && fixed + rdm_tile_bytes(r, ta) + 512 > RDM_SCRATCH_BYTES) {
+                fcis::set_error("fcis_trans_rdm12s_noalloc: one alpha string of this space needs %zu "
+                                "bytes of scratch, above the fixed %zu (norb %d, %lld beta strings); "
+                                "use fcis_trans_rdm12s",
+                                fixed + rdm_tile_bytes(r, 1), RDM_SCRATCH_BYTES, r.n, (long long)r.nb);
+                return FCIS_ERR_INVALID;
+        }
+        char *base = nullptr;
+        int ret = scratch_ptr(&base);
+        if (ret != FCIS_SUCCESS) return ret;
+        double *G = reinterpret_cast<double *>(base);
+        double *rd = reinterpret_cast<double *>(base + gbytes);
+        const int64_t ndt = ta * r.nb;
+        const int64_t nchunk = (ndt + RDM_KCHUNK - 1) / RDM_KCHUNK;
+        double *part = reinterpret_cast<double *>(base + fixed);
+        double *T = reinterpret_cast<double *>(base + fixed + align256((size_t)nchunk * r.W2 * 8));
+        if (r.na > 0) {
+                ret = rdm_gram(s, r, RdmBuffers{T, part, G, ta}, d_bra, d_ket, st);
+                if (ret != FCIS_SUCCESS) return ret;
+        } else {
+                hipLaunchKernelGGL(k_rdm_reduce, dim3((3 * r.W2 + 255) / 256), dim3(256), 0, st,
+                                   (int64_t)(3 * r.W2), 0, 0, G, G);
+        }
+        const int64_t ntot = 2 * n2 + 3 * n4;
+        hipLaunchKernelGGL(k_rdm_assemble, dim3((ntot + 255) / 256), dim3(256), 0, st, r.n, r.wp, G, rd);
+        FCIS_CHECK_LAUNCH();
+        double *outs[5] = {dm1a, dm1b, dm2aa, dm2ab, dm2bb};
+        for (int k = 0; k < 5; k++) {
+                if (outs[k] == nullptr) continue;
+                const size_t off = k < 2 ? (size_t)k * n2 : 2 * (size_t)n2 + (size_t)(k - 2) * n4;
+                const size_t cnt = k < 2 ? (size_t)n2 : (size_t)n4;
+                FCIS_HIP(hipMemcpyAsync(outs[k], rd + off, cnt * 8, hipMemcpyDeviceToHost, st));
+        }
+        FCIS_HIP(hipStreamSynchronize(st));
         return FCIS_SUCCESS;
 }
