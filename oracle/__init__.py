@@ -177,6 +177,63 @@ def sigma_rows(h1e, eri, ci, norb, nelec, dets):
     return out
 
 
+_MASK64 = (1 << 64) - 1
+_GOLDEN64 = 0x9E3779B97F4A7C15
+
+
+def splitmix64_mix(z):
+    """The SplitMix64 output function (Vigna's splitmix64.c; Stafford's
+    variant 13), on Python ints."""
+    z &= _MASK64
+    z ^= z >> 30
+    z = (z * 0xBF58476D1CE4E5B9) & _MASK64
+    z ^= z >> 27
+    z = (z * 0x94D049BB133111EB) & _MASK64
+    return z ^ (z >> 31)
+
+
+def init_guess(hdiag, nroots, noise_norm=1e-3, addrs=None):
+    """Initial vectors of the reference's GAS/CAS solver: a unit vector at
+    each of the nroots lowest diagonal entries, in numpy.argpartition order
+    (_lowest_hdiag_addresses, pyscf/mcscf/fci_gas.py:46-58; the whole range
+    when nroots >= size), each plus a global sign noise of total norm
+    noise_norm, sign +1 where bit 0 of mix(i + (root + 1) * golden) is set
+    (_add_distributed_noise :61-88, get_init_guess :881-902).  ``addrs``
+    replaces the selected addresses (to check the noise alone)."""
+    hdiag = numpy.asarray(hdiag, dtype=numpy.float64).reshape(-1)
+    n = hdiag.size
+    if addrs is not None:
+        addrs = numpy.asarray(addrs, dtype=numpy.int64)
+    elif n <= nroots:
+        addrs = numpy.arange(n)
+    else:
+        addrs = numpy.argpartition(hdiag, nroots - 1)[:nroots]
+    scale = noise_norm / numpy.sqrt(float(n))
+    out = []
+    idx = numpy.arange(n, dtype=numpy.uint64)
+    for root, a in enumerate(addrs):
+        z = idx + numpy.uint64(((root + 1) * _GOLDEN64) & _MASK64)
+        z ^= z >> numpy.uint64(30)
+        z *= numpy.uint64(0xBF58476D1CE4E5B9)
+        z ^= z >> numpy.uint64(27)
+        z *= numpy.uint64(0x94D049BB133111EB)
+        z ^= z >> numpy.uint64(31)
+        g = numpy.zeros(n)
+        g[int(a)] = 1.0
+        g += scale * numpy.where(z & numpy.uint64(1), 1.0, -1.0)
+        out.append(g)
+    return addrs, out
+
+
+def precond(dx, hdiag, e, level_shift):
+    """The reference's diagonal preconditioner (pyscf/mcscf/fci_gas.py:
+    1337-1342): dx / (hdiag - e), every denominator of magnitude below
+    level_shift replaced by +level_shift."""
+    d = numpy.asarray(hdiag, dtype=numpy.float64) - e
+    d = numpy.where(numpy.abs(d) < level_shift, level_shift, d)
+    return numpy.asarray(dx, dtype=numpy.float64) / d
+
+
 def make_hdiag(h1e, eri, norb, nelec):
     na, nb = _unpack_nelec(nelec)
     h1e = _c64(h1e, (norb, norb))

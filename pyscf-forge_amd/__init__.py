@@ -14,7 +14,7 @@ from .direct_spin1 import (  # noqa: F401
     FCISolver, SigmaPlan, Space, SpinPlan, absorb_h1e, contract_2e, contract_ss, energy,
     gen_linkstr_index,
     get_space, make_hdiag, make_rdm1, make_rdm1s, make_rdm12, make_rdm12s,
-    pair_matrix, pspace, spin_square_from_rdm12s, trans_rdm1, trans_rdm1s, trans_rdm12,
+    pair_matrix, pspace, release_plans, spin_square_from_rdm12s, trans_rdm1, trans_rdm1s, trans_rdm12,
     trans_rdm12s)
 
 # drop-in alias: code written against pyscf.fci.direct_spin1 can use

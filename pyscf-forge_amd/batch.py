@@ -191,7 +191,7 @@ def solve_batch(problems, nroots=4, tol=1e-10, tol_residual=None, max_cycle=100,
         hd = HD[int(hoff[p]):int(hoff[p] + nd[p])]
         _lib.check(lib.fcis_make_hdiag(sp.handle, _lib.dptr(h1), _lib.dptr(er), _lib.vptr(hd),
                                        stream), "fcis_make_hdiag")
-        addrs = torch.topk(hd, int(kk[p]), largest=False).indices.cpu().numpy()
+        addrs = _ds.lowest_addresses(hd, int(kk[p]))
         for r, a in enumerate(addrs):
             _lib.check(lib.fcis_init_guess(int(nd[p]), int(a), r, _ds.INIT_NOISE_NORM,
                                            ctypes.c_void_p(int(tbase[p] + stride[p] * r)), stream),

@@ -780,7 +780,7 @@ class FCISolver(_ds.FCISolver):
         import torch
         n = hdiag.numel()
         k = min(nroots, n)
-        addrs = torch.topk(hdiag, k, largest=False).indices
+        addrs = _ds.lowest_addresses(hdiag, k)
         out = []
         for a in addrs.tolist():
             g = torch.zeros(n, dtype=torch.float64, device=hdiag.device)
@@ -1030,13 +1030,13 @@ def kernel(fci, h1e, eri, norb, nelec, smult=None, idx_sym=None, ci0=None, trans
 
 
 def csf_solver(mol=None, smult=None, symm=None):
-    """CAS CSF solver factory (pyscf.csf_fci.csf_solver, csf_fci/__init__.py:6-12).
-    One class serves both cases: point-group symmetry switches on when the
-    solver is given ``orbsym`` (attribute or kernel kwarg); ``symm`` is
-    accepted for signature parity and taken from ``mol.symmetry`` when
-    unset, as the reference does."""
+    """CAS CSF solver factory (pyscf.csf_fci.csf_solver, csf_fci/__init__.py:6-12):
+    the point-group solver (csf_symm.FCISolver) when ``symm`` is set or,
+    unset, when ``mol.symmetry`` is, else the plain CSF solver (which also
+    packs by irrep when given ``orbsym``)."""
     if mol is not None and symm is None:
         symm = getattr(mol, "symmetry", None)
-    sol = FCISolver(mol, smult=smult)
-    sol.symm = bool(symm)
-    return sol
+    if symm:
+        from . import csf_symm
+        return csf_symm.FCISolver(mol, smult=smult)
+    return FCISolver(mol, smult=smult)

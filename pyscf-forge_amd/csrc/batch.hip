@@ -393,7 +393,7 @@ int build_items(fcis_batch *b, int rows_per_item)
                 (void)hipFree(b->d_items);
                 b->d_items = nullptr;
                 b->items_cap = 0;
-                if (hipMalloc(&b->d_items, sizeof(BatchItem) * sorted.size()) != hipSuccess) {
+                if (fcis::dev_malloc(&b->d_items, sizeof(BatchItem) * sorted.size()) != hipSuccess) {
                         fcis::set_error("fcis_batch: item table allocation failed");
                         return FCIS_ERR_MEMORY;
                 }
@@ -504,13 +504,13 @@ int batch_init(fcis_batch *b, int nprob, const fcis_space *const *spaces, const 
         }
         b->total = off;
         b->nslots = aoff;
-        if (hipMalloc(&b->d_pmapa, 4 * (size_t)npa + 16) != hipSuccess ||
-            hipMalloc(&b->d_pmapb, 2 * (size_t)npb + 16) != hipSuccess ||
-            hipMalloc(&b->d_hfrag, fb * (size_t)nhf + 16) != hipSuccess ||
-            hipMalloc(&b->d_hfrag_lo, precision == FCIS_PREC_BF16X3 ? fb * (size_t)nhf + 16 : 16) != hipSuccess ||
-            hipMalloc(&b->d_probs, sizeof(BatchProb) * nprob) != hipSuccess ||
-            hipMalloc(&b->d_counter, 256) != hipSuccess ||
-            hipMalloc(&b->d_active, (size_t)b->nslots + 16) != hipSuccess) {
+        if (fcis::dev_malloc(&b->d_pmapa, 4 * (size_t)npa + 16) != hipSuccess ||
+            fcis::dev_malloc(&b->d_pmapb, 2 * (size_t)npb + 16) != hipSuccess ||
+            fcis::dev_malloc(&b->d_hfrag, fb * (size_t)nhf + 16) != hipSuccess ||
+            fcis::dev_malloc(&b->d_hfrag_lo, precision == FCIS_PREC_BF16X3 ? fb * (size_t)nhf + 16 : 16) != hipSuccess ||
+            fcis::dev_malloc(&b->d_probs, sizeof(BatchProb) * nprob) != hipSuccess ||
+            fcis::dev_malloc(&b->d_counter, 256) != hipSuccess ||
+            fcis::dev_malloc(&b->d_active, (size_t)b->nslots + 16) != hipSuccess) {
                 fcis::set_error("fcis_batch_create: device allocation failed");
                 return FCIS_ERR_MEMORY;
         }

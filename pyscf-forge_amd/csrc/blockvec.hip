@@ -194,7 +194,7 @@ int stage_reserve(Stage &s, size_t bytes)
         s.host = s.dev = nullptr;
         s.cap = 0;
         if (hipHostMalloc((void **)&s.host, cap) != hipSuccess ||
-            hipMalloc((void **)&s.dev, cap) != hipSuccess) {
+            fcis::dev_malloc((void **)&s.dev, cap) != hipSuccess) {
                 fcis::set_error("block vector ops: staging allocation of %zu bytes failed", cap);
                 return FCIS_ERR_MEMORY;
         }

@@ -34,6 +34,17 @@ void set_error(const char *fmt, ...);
                 }                                                             \
         } while (0)
 
+// hipMalloc that leaves no error state behind when it fails: HIP keeps the
+// failure as the thread's last error, which the next hipGetLastError (the
+// launch check after the next kernel) would otherwise report.
+template <typename T>
+static inline hipError_t dev_malloc(T **p, size_t bytes)
+{
+        hipError_t e = hipMalloc(p, bytes);
+        if (e != hipSuccess) (void)hipGetLastError();
+        return e;
+}
+
 static inline int64_t binom_host(int n, int k)
 {
         if (k < 0 || k > n) return 0;

@@ -534,9 +534,9 @@ int fcis_csf_create(fcis_csf_t **out, const fcis_space_t *s, int smult, void *st
         c->d_kex = nullptr;
         c->d_sym = nullptr;
         c->nsym = 0;
-        if (hipMalloc(&c->d_mask, 4 * (size_t)off + 4) != hipSuccess ||
-            hipMalloc(&c->d_U, 8 * (Uall.size() + 1)) != hipSuccess ||
-            hipMalloc(&c->d_kex, 8 * (size_t)norb * norb + 8) != hipSuccess) {
+        if (fcis::dev_malloc(&c->d_mask, 4 * (size_t)off + 4) != hipSuccess ||
+            fcis::dev_malloc(&c->d_U, 8 * (Uall.size() + 1)) != hipSuccess ||
+            fcis::dev_malloc(&c->d_kex, 8 * (size_t)norb * norb + 8) != hipSuccess) {
                 fcis_csf_free(c);
                 fcis::set_error("fcis_csf_create: device allocation failed");
                 return FCIS_ERR_MEMORY;
@@ -593,7 +593,7 @@ int fcis_csf_symmetry(fcis_csf_t *c, const int *orbsym, int wfnsym, int64_t *nsy
         }
         c->nsym = (int64_t)idx.size();
         if (nsym) *nsym = c->nsym;
-        if (hipMalloc(&c->d_sym, 4 * idx.size() + 4) != hipSuccess) {
+        if (fcis::dev_malloc(&c->d_sym, 4 * idx.size() + 4) != hipSuccess) {
                 c->d_sym = nullptr;
                 c->nsym = c->ncsf;
                 fcis::set_error("fcis_csf_symmetry: device allocation failed");

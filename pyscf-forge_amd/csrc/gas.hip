@@ -298,10 +298,10 @@ int build_rowset(fcis_gas_space *g, const std::vector<std::vector<uint8_t>> &poo
                 tot += scnt[asec[I]];
         }
         g->nchunk_rows = tot;
-        if (hipMalloc(&g->d_rows, 4 * (size_t)nrows + 4) != hipSuccess ||
-            hipMalloc(&g->d_rpos, 4 * (size_t)na + 4) != hipSuccess ||
-            hipMalloc(&g->d_rowcl, 8 * (size_t)nrows + 8) != hipSuccess ||
-            hipMalloc(&g->d_clist, 2 * clist.size() + 2) != hipSuccess) {
+        if (fcis::dev_malloc(&g->d_rows, 4 * (size_t)nrows + 4) != hipSuccess ||
+            fcis::dev_malloc(&g->d_rpos, 4 * (size_t)na + 4) != hipSuccess ||
+            fcis::dev_malloc(&g->d_rowcl, 8 * (size_t)nrows + 8) != hipSuccess ||
+            fcis::dev_malloc(&g->d_clist, 2 * clist.size() + 2) != hipSuccess) {
                 fcis::set_error("fcis_gas_space_create: device allocation failed");
                 return FCIS_ERR_MEMORY;
         }
@@ -550,9 +550,9 @@ int fcis_gas_space_create(fcis_gas_space_t **out, const fcis_space_t *cas, int n
         int64_t *d_tab = nullptr;
         uint32_t *d_sec = nullptr;
         int st_ok = FCIS_SUCCESS;
-        if (hipMalloc(&g->d_map, 4 * (size_t)g->ndet + 4) != hipSuccess ||
-            hipMalloc(&d_tab, 8 * tab.size()) != hipSuccess ||
-            hipMalloc(&d_sec, 4 * secaddr.size() + 4) != hipSuccess) {
+        if (fcis::dev_malloc(&g->d_map, 4 * (size_t)g->ndet + 4) != hipSuccess ||
+            fcis::dev_malloc(&d_tab, 8 * tab.size()) != hipSuccess ||
+            fcis::dev_malloc(&d_sec, 4 * secaddr.size() + 4) != hipSuccess) {
                 fcis::set_error("fcis_gas_space_create: device allocation failed");
                 st_ok = FCIS_ERR_MEMORY;
         } else if (hipMemcpyAsync(d_tab, tab.data(), 8 * tab.size(), hipMemcpyHostToDevice, st) !=
@@ -744,7 +744,7 @@ int fcis_gas_make_hdiag(const fcis_gas_space_t *g, const double *h1e, const doub
         }
         hipStream_t st = (hipStream_t)stream;
         Scratch tmp(st);
-        FCIS_HIP(hipMalloc(&tmp.p, 8 * (size_t)g->ncas));
+        FCIS_HIP(fcis::dev_malloc(&tmp.p, 8 * (size_t)g->ncas));
         int r = fcis_make_hdiag(g->cas, h1e, eri, tmp.p, stream);
         if (r != FCIS_SUCCESS) return r;
         return project(g, tmp.p, d_hdiag, 1, st);
@@ -760,7 +760,7 @@ int fcis_gas_trans_rdm12s(const fcis_gas_space_t *g, const double *d_bra, const 
         }
         hipStream_t st = (hipStream_t)stream;
         Scratch tmp(st);
-        FCIS_HIP(hipMalloc(&tmp.p, 16 * (size_t)g->ncas));
+        FCIS_HIP(fcis::dev_malloc(&tmp.p, 16 * (size_t)g->ncas));
         int r = embed(g, d_bra, tmp.p, 1, st);
         if (r == FCIS_SUCCESS) r = embed(g, d_ket, tmp.p + g->ncas, 1, st);
         if (r == FCIS_SUCCESS) {
@@ -791,7 +791,7 @@ int fcis_gas_sigma_plan_create(fcis_gas_sigma_plan_t **out, const fcis_gas_space
         p->gas = g;
         p->cas_plan = nullptr;
         p->d_a = p->d_b = nullptr;
-        if (hipMalloc(&p->d_a, 16 * (size_t)g->ncas) != hipSuccess) {
+        if (fcis::dev_malloc(&p->d_a, 16 * (size_t)g->ncas) != hipSuccess) {
                 fcis_gas_sigma_plan_free(p);
                 fcis::set_error("fcis_gas_sigma_plan_create: device allocation failed");
                 return FCIS_ERR_MEMORY;

@@ -121,7 +121,7 @@ int scratch(size_t bytes, double **out)
                 if (s.d) (void)hipFree(s.d);
                 s.d = nullptr;
                 s.bytes = 0;
-                FCIS_HIP(hipMalloc(&s.d, bytes));
+                FCIS_HIP(fcis::dev_malloc(&s.d, bytes));
                 s.bytes = bytes;
         }
         *out = s.d;
@@ -174,9 +174,9 @@ int fcis_make_hdiag(const fcis_space_t *s, const double *h1e, const double *eri,
         hipStream_t st = (hipStream_t)stream;
         int64_t ndet = s->nstra * s->nstrb;
         int ret = FCIS_SUCCESS;
-        if (hipMalloc(&d_ea, 8 * (ea.size() + 1)) != hipSuccess ||
-            hipMalloc(&d_eb, 8 * (eb.size() + 1)) != hipSuccess ||
-            hipMalloc(&d_jb, 8 * (jb.size() + 1)) != hipSuccess) {
+        if (fcis::dev_malloc(&d_ea, 8 * (ea.size() + 1)) != hipSuccess ||
+            fcis::dev_malloc(&d_eb, 8 * (eb.size() + 1)) != hipSuccess ||
+            fcis::dev_malloc(&d_jb, 8 * (jb.size() + 1)) != hipSuccess) {
                 fcis::set_error("fcis_make_hdiag: device allocation failed");
                 ret = FCIS_ERR_MEMORY;
         } else if (hipMemcpyAsync(d_ea, ea.data(), 8 * ea.size(), hipMemcpyHostToDevice, st) != hipSuccess ||

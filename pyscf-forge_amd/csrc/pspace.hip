@@ -146,7 +146,7 @@ int fcis_pspace_uhf(const fcis_space_t *s, const double *h1a, const double *h1b,
         const int n = s->norb, np = s->npair;
         double *d_h = nullptr;
         const size_t nh = (size_t)n * n, ne = (size_t)np * np;
-        FCIS_HIP(hipMalloc(&d_h, 8 * (2 * nh + ne)));
+        FCIS_HIP(fcis::dev_malloc(&d_h, 8 * (2 * nh + ne)));
         int ret = FCIS_SUCCESS;
         if (hipMemcpyAsync(d_h, h1a, 8 * nh, hipMemcpyHostToDevice, st) != hipSuccess ||
             hipMemcpyAsync(d_h + nh, h1b, 8 * nh, hipMemcpyHostToDevice, st) != hipSuccess ||

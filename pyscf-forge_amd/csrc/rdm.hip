@@ -260,9 +260,9 @@ extern "C" int fcis_trans_rdm12s(const fcis_space_t *s, const double *d_bra, con
         int64_t maxchunk = (ndt + RDM_KCHUNK - 1) / RDM_KCHUNK;
         double *T = nullptr, *part = nullptr, *G = nullptr;
         size_t tbytes = 4 * (size_t)ndt * wp * 8;
-        if (hipMalloc(&T, tbytes ? tbytes : 8) != hipSuccess ||
-            hipMalloc(&part, maxchunk * W2 * 8 + 8) != hipSuccess ||
-            hipMalloc(&G, 3 * W2 * 8) != hipSuccess) {
+        if (fcis::dev_malloc(&T, tbytes ? tbytes : 8) != hipSuccess ||
+            fcis::dev_malloc(&part, maxchunk * W2 * 8 + 8) != hipSuccess ||
+            fcis::dev_malloc(&G, 3 * W2 * 8) != hipSuccess) {
                 (void)hipFree(T);
                 (void)hipFree(part);
                 fcis::set_error("fcis_trans_rdm12s: device allocation failed");

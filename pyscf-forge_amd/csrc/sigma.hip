@@ -1051,13 +1051,13 @@ int fcis::sigma_plan_create(fcis_sigma_plan **out, const fcis_space *space, cons
         const size_t nf = frag_count(p);
         size_t hbytes = precision == FCIS_PREC_BF16X3 ? nf * 16 : nf * elem;
         size_t pa = (size_t)p->npair * space->nstra * 4, pb = (size_t)p->npair * nb * 4;
-        if (hipMalloc(&p->d_hfrag, hbytes) != hipSuccess ||
-            hipMalloc(&p->d_hfrag_lo, precision == FCIS_PREC_BF16X3 ? hbytes : 16) != hipSuccess ||
-            hipMalloc(&p->d_pmapa, pa ? pa : 4) != hipSuccess ||
-            hipMalloc(&p->d_pmapb, pb ? pb : 4) != hipSuccess ||
-            hipMalloc(&p->d_pmapb16, p->fused ? (size_t)p->npair * p->nbp * 2 + 16 : 16) != hipSuccess ||
-            hipMalloc(&p->d_t1, (buf && !p->fused) ? buf : 8) != hipSuccess ||
-            hipMalloc(&p->d_vt1, buf ? buf : 8) != hipSuccess) {
+        if (fcis::dev_malloc(&p->d_hfrag, hbytes) != hipSuccess ||
+            fcis::dev_malloc(&p->d_hfrag_lo, precision == FCIS_PREC_BF16X3 ? hbytes : 16) != hipSuccess ||
+            fcis::dev_malloc(&p->d_pmapa, pa ? pa : 4) != hipSuccess ||
+            fcis::dev_malloc(&p->d_pmapb, pb ? pb : 4) != hipSuccess ||
+            fcis::dev_malloc(&p->d_pmapb16, p->fused ? (size_t)p->npair * p->nbp * 2 + 16 : 16) != hipSuccess ||
+            fcis::dev_malloc(&p->d_t1, (buf && !p->fused) ? buf : 8) != hipSuccess ||
+            fcis::dev_malloc(&p->d_vt1, buf ? buf : 8) != hipSuccess) {
                 fcis_sigma_plan_free(p);
                 fcis::set_error("fcis_sigma_plan_create: device allocation of %zu bytes failed",
                                 2 * buf + 2 * hbytes + pa + pb);

@@ -184,7 +184,7 @@ int fcis_spin_plan_create(fcis_spin_plan_t **out, const fcis_space_t *s, void *s
         p->space = s;
         p->d_pmapb = nullptr;
         const int64_t n = (int64_t)s->npair * s->nstrb;
-        if (hipMalloc(&p->d_pmapb, 4 * (size_t)n + 4) != hipSuccess) {
+        if (fcis::dev_malloc(&p->d_pmapb, 4 * (size_t)n + 4) != hipSuccess) {
                 fcis_spin_plan_free(p);
                 fcis::set_error("fcis_spin_plan_create: device allocation failed");
                 return FCIS_ERR_MEMORY;

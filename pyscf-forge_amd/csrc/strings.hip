@@ -263,10 +263,10 @@ int fcis_space_create(fcis_space_t **out, int norb, int na, int nb, void *stream
         (void)hipGetDevice(&s->device);
         size_t la = sizeof(fcis_link) * (size_t)s->nstra * (s->nlinka ? s->nlinka : 1);
         size_t lb = sizeof(fcis_link) * (size_t)s->nstrb * (s->nlinkb ? s->nlinkb : 1);
-        if (hipMalloc(&s->d_linka, la) != hipSuccess ||
-            hipMalloc(&s->d_linkb, lb) != hipSuccess ||
-            hipMalloc(&s->d_stra, 8 * (size_t)s->nstra) != hipSuccess ||
-            hipMalloc(&s->d_strb, 8 * (size_t)s->nstrb) != hipSuccess) {
+        if (fcis::dev_malloc(&s->d_linka, la) != hipSuccess ||
+            fcis::dev_malloc(&s->d_linkb, lb) != hipSuccess ||
+            fcis::dev_malloc(&s->d_stra, 8 * (size_t)s->nstra) != hipSuccess ||
+            fcis::dev_malloc(&s->d_strb, 8 * (size_t)s->nstrb) != hipSuccess) {
                 fcis_space_free(s);
                 fcis::set_error("fcis_space_create: device allocation failed");
                 return FCIS_ERR_MEMORY;
@@ -354,11 +354,11 @@ int fcis_space_create_ordered(fcis_space_t **out, int norb, int na, int nb,
         hipStream_t st = (hipStream_t)stream;
         size_t la = sizeof(fcis_link) * (size_t)s->nstra * (s->nlinka ? s->nlinka : 1);
         size_t lb = sizeof(fcis_link) * (size_t)s->nstrb * (s->nlinkb ? s->nlinkb : 1);
-        if (hipMalloc(&s->d_linka, la) != hipSuccess || hipMalloc(&s->d_linkb, lb) != hipSuccess ||
-            hipMalloc(&s->d_stra, 8 * (size_t)s->nstra) != hipSuccess ||
-            hipMalloc(&s->d_strb, 8 * (size_t)s->nstrb) != hipSuccess ||
-            hipMalloc(&s->d_perma, 4 * (size_t)s->nstra) != hipSuccess ||
-            hipMalloc(&s->d_permb, 4 * (size_t)s->nstrb) != hipSuccess) {
+        if (fcis::dev_malloc(&s->d_linka, la) != hipSuccess || fcis::dev_malloc(&s->d_linkb, lb) != hipSuccess ||
+            fcis::dev_malloc(&s->d_stra, 8 * (size_t)s->nstra) != hipSuccess ||
+            fcis::dev_malloc(&s->d_strb, 8 * (size_t)s->nstrb) != hipSuccess ||
+            fcis::dev_malloc(&s->d_perma, 4 * (size_t)s->nstra) != hipSuccess ||
+            fcis::dev_malloc(&s->d_permb, 4 * (size_t)s->nstrb) != hipSuccess) {
                 fcis_space_free(s);
                 fcis::set_error("fcis_space_create_ordered: device allocation failed");
                 return FCIS_ERR_MEMORY;

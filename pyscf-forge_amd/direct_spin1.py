@@ -240,14 +240,58 @@ def _default_workspace():
     return int(free * 0.45)
 
 
+# Plans of the module-level contract_2e, reused across calls with the same
+# space, precision and absorbed Hamiltonian (the reference's solvers call
+# contract_2e once per Davidson vector with unchanged integrals, and its
+# GasContractPlan is built once per Hamiltonian, fci_gas.py:570-650).  A plan
+# holds device tables and the sigma workspace, so at most PLAN_CACHE_SIZE are
+# kept, and a plan whose workspace exceeds PLAN_CACHE_FRACTION of the device
+# memory is freed after its call; release_plans() frees them all.
+PLAN_CACHE_SIZE = 2
+PLAN_CACHE_FRACTION = 0.25
+_PLANS = {}            # key -> SigmaPlan, dict order = recency
+
+
+def release_plans():
+    """Free the plans cached by contract_2e."""
+    while _PLANS:
+        _PLANS.pop(next(iter(_PLANS))).close()
+
+
+def _cached_plan(sp, eri, precision, workspace_bytes):
+    import hashlib
+    h = pair_matrix(eri, sp.norb)
+    key = (sp.norb, sp.nelec, str(precision), int(workspace_bytes),
+           hashlib.sha256(h.tobytes()).hexdigest())
+    plan = _PLANS.pop(key, None)
+    if plan is None:
+        while len(_PLANS) >= PLAN_CACHE_SIZE:
+            _PLANS.pop(next(iter(_PLANS))).close()
+        try:
+            plan = SigmaPlan(sp, h, precision, workspace_bytes or _default_workspace())
+        except RuntimeError:
+            # the cached plans' workspaces may be what is missing
+            release_plans()
+            plan = SigmaPlan(sp, h, precision, workspace_bytes or _default_workspace())
+    _PLANS[key] = plan
+    return plan
+
+
 def contract_2e(eri, fcivec, norb, nelec, link_index=None, precision="fp64", **kwargs):
-    """sigma = H c for the absorbed pair matrix ``eri`` (direct_spin1.contract_2e)."""
+    """sigma = H c for the absorbed pair matrix ``eri`` (direct_spin1.contract_2e).
+
+    Host (numpy) vectors are copied to the device and back; device tensors
+    stay there.  The plan for (space, precision, eri) is kept for the next
+    call (release_plans frees it)."""
     sp = get_space(norb, nelec)
     c, host = _to_device(fcivec, sp.ndet)
     import torch
     out = torch.empty_like(c)
-    with SigmaPlan(sp, eri, precision, kwargs.get("workspace_bytes", 0) or _default_workspace()) as plan:
-        plan(c, out)
+    plan = _cached_plan(sp, eri, precision, int(kwargs.get("workspace_bytes", 0) or 0))
+    plan(c, out)
+    if plan.workspace_bytes > PLAN_CACHE_FRACTION * torch.cuda.mem_get_info()[1]:
+        for key in [k for k, v in _PLANS.items() if v is plan]:
+            _PLANS.pop(key).close()
     shape = numpy.shape(fcivec) if host else tuple(fcivec.shape)
     if host:
         return out.cpu().numpy().reshape(shape)
@@ -473,6 +517,29 @@ def spin_square_from_rdm12s(dm1s, dm2s, nelec):
 
 # ---------------------------------------------------------------- solver
 
+def lowest_addresses(hd, count):
+    """Addresses of the ``count`` lowest entries of a device diagonal, in the
+    order of the reference's _lowest_hdiag_addresses (pyscf/mcscf/fci_gas.py:
+    46-58): the whole range 0..n-1 when count >= n, else
+    numpy.argpartition(hd, count - 1)[:count], which lists them by ascending
+    value whenever the count + 1 lowest values are distinct.  Ties (the
+    alpha <-> beta mirror determinants of spaces with na == nb) are broken
+    here by the lower address; numpy's own tie order depends on the
+    selection routine its build dispatches to.  Host int64 array."""
+    import torch
+    n = hd.numel()
+    count = int(count)
+    if count >= n:
+        return numpy.arange(n, dtype=numpy.int64)
+    # the count-th lowest value, then every entry at or below it (ties at
+    # the boundary included) ordered by (value, address) on the host
+    thr = torch.topk(hd, count, largest=False, sorted=True).values[-1]
+    cand = torch.nonzero(hd <= thr).reshape(-1)
+    val = hd[cand].cpu().numpy()
+    cand = cand.cpu().numpy()
+    return cand[numpy.lexsort((cand, val))[:count]].astype(numpy.int64)
+
+
 class FCISolver:
     """CAS FCI solver with pyscf's fcisolver protocol (kernel, contract_2e,
     absorb_h1e, make_hdiag, make_rdm1(s)/make_rdm12(s)/trans_rdm*, energy,
@@ -567,8 +634,7 @@ class FCISolver:
         hd = hdiag if isinstance(hdiag, torch.Tensor) else torch.as_tensor(
             numpy.asarray(hdiag, dtype=numpy.float64)).cuda()
         n = hd.numel()
-        k = min(nroots, n)
-        addrs = torch.topk(hd, k, largest=False).indices.cpu().numpy()
+        addrs = lowest_addresses(hd, nroots)
         lib = _lib.load()
         out = []
         for root, a in enumerate(addrs):

@@ -440,20 +440,13 @@ class GasSpinPlan:
         out = self.contract_dev(c, torch.empty_like(c))
         return out.cpu().numpy().reshape(shape) if host else out.reshape(shape)
 
-    def penalty(self, vector, parameters):
-        """shift * (S^2 - target) x, or shift * (S^2 - target)^2 x for a
-        target above the minimum (_spin_penalty_action, fci_gas.py:242-252),
-        with both S^2 applications on the device.  numpy in, numpy out."""
+    def penalty(self, vector, penalty):
+        """P x for a SpinPenalty P, both S^2 applications on the device;
+        numpy in, numpy out (device tensors stay on the device)."""
         import torch
-        shift, target, minimum, _ = parameters
         x, host, shape = _vector_in(vector, self.ndet)
-        d = self.contract_dev(x, torch.empty_like(x))
-        d.sub_(x, alpha=target)
-        if target >= minimum + 0.1:
-            d2 = self.contract_dev(d, torch.empty_like(d))
-            d = d2.sub_(d, alpha=target)
-        d.mul_(shift)
-        return d.cpu().numpy().reshape(shape) if host else d.reshape(shape)
+        y = penalty.apply(lambda v: self.contract_dev(v, torch.empty_like(v)), x)
+        return y.cpu().numpy().reshape(shape) if host else y.reshape(shape)
 
     def diagonal_dev(self):
         import torch
@@ -497,52 +490,73 @@ class GasSpinPlan:
         return x.cpu().numpy().reshape(shape) if host else x.reshape(shape)
 
 
-def _spin_penalty_parameters(solver, norb, nelec):
-    """(shift, target, minimum, allowed S(S+1)) of the solver's spin
-    penalty, or None (fci_gas.py:211-239)."""
-    if getattr(solver, "ss_penalty", None) is None:
-        return None
-    shift = float(solver.ss_penalty)
-    if not numpy.isfinite(shift) or shift < 0.0:
-        raise ValueError("spin penalty shift must be finite and nonnegative")
-    na, nb = _ds._unpack_nelec(nelec)
-    sz = 0.5 * abs(na - nb)
-    minimum = sz * (sz + 1.0)
-    tv = getattr(solver, "ss_value", None)
-    target = minimum if tv is None else float(tv)
-    if not numpy.isfinite(target):
-        raise ValueError("target S^2 must be finite")
-    if target < minimum - 1e-10:
-        raise ValueError("target S^2 is below the minimum allowed by Nalpha-Nbeta")
-    ne = na + nb
-    smax = 0.5 * min(ne, 2 * int(norb) - ne)
-    spins = numpy.arange(sz, smax + 0.5, 1.0)
-    eig = spins * (spins + 1.0)
-    if not numpy.any(numpy.abs(eig - target) < 1e-10):
-        raise ValueError("target S^2 is not an allowed S(S+1) value for this space")
-    return shift, target, minimum, tuple(float(x) for x in eig)
+class SpinPenalty:
+    """The GAS solver's spin penalty P = shift * (S^2 - t), or
+    shift * (S^2 - t)^2 when t lies above the lowest S(S+1) the electron
+    counts allow (the linear form would reward spins below t).  The
+    penalty the reference's fixed-orbital solver and Newton driver add
+    (fci_gas.py:211-263).
 
+    With m = |N_alpha - N_beta| / 2 and at most min(N, 2 norb - N) unpaired
+    electrons, S runs over m, m + 1, ..., min(N, 2 norb - N) / 2, and t must
+    be one of those S(S+1) (default: the lowest, m(m+1))."""
 
-def _spin_penalty_action(contract_ss, vector, parameters):
-    """The spin penalty without H (fci_gas.py:242-252): shift * (S^2 - t) x,
-    squared for a target above the minimum; ``contract_ss`` maps a vector
-    to S^2 times it."""
-    shift, target, minimum, _ = parameters
-    delta = numpy.asarray(contract_ss(vector)) - target * vector
-    if target >= minimum + 0.1:
-        delta = numpy.asarray(contract_ss(delta)) - target * delta
-    return shift * delta
+    __slots__ = ("shift", "target", "spectrum", "quadratic")
 
+    def __init__(self, shift, target, spectrum):
+        self.shift = float(shift)
+        self.target = float(target)
+        self.spectrum = tuple(float(x) for x in spectrum)
+        self.quadratic = self.target > self.spectrum[0] + 1e-10
 
-def _spin_penalty_diagonal(spin_diagonal, parameters):
-    """Preconditioner diagonal of the penalty (fci_gas.py:255-263):
-    shift * (diag(S^2) - t), squared elementwise for the quadratic form (an
-    approximation of diag((S^2 - t)^2), as in the reference)."""
-    shift, target, minimum, _ = parameters
-    delta = spin_diagonal - target
-    if target >= minimum + 0.1:
-        delta = delta * delta
-    return shift * delta
+    @classmethod
+    def of(cls, solver, norb, nelec):
+        """The penalty a solver carries (``ss_penalty`` / ``ss_value``, as set
+        by fix_spin_), or None."""
+        shift = getattr(solver, "ss_penalty", None)
+        if shift is None:
+            return None
+        shift = float(shift)
+        if not (numpy.isfinite(shift) and shift >= 0.0):
+            raise ValueError("spin penalty shift must be finite and nonnegative")
+        na, nb = _ds._unpack_nelec(nelec)
+        unpaired_max = min(na + nb, 2 * int(norb) - na - nb)
+        s_values = 0.5 * numpy.arange(abs(na - nb), unpaired_max + 1, 2)
+        spectrum = s_values * (s_values + 1.0)
+        want = getattr(solver, "ss_value", None)
+        target = spectrum[0] if want is None else float(want)
+        if not numpy.isfinite(target):
+            raise ValueError("target S^2 must be finite")
+        if target < spectrum[0] - 1e-10:
+            raise ValueError("target S^2 is below the minimum allowed by Nalpha-Nbeta")
+        if numpy.min(numpy.abs(spectrum - target)) >= 1e-10:
+            raise ValueError("target S^2 is not an allowed S(S+1) value for this space")
+        return cls(shift, target, spectrum)
+
+    def active(self):
+        return self.shift != 0.0
+
+    def apply(self, s2, x):
+        """P x for a vector x and ``s2`` mapping a vector to S^2 times it
+        (arrays in, array out; x is not modified)."""
+        y = s2(x) - self.target * x
+        if self.quadratic:
+            y = s2(y) - self.target * y
+        return self.shift * y
+
+    def diagonal(self, s2_diagonal):
+        """Preconditioner diagonal: shift * (diag(S^2) - t), squared
+        entrywise in the quadratic case (an approximation of the diagonal of
+        (S^2 - t)^2, whose exact value needs the off-diagonal S^2 row)."""
+        d = s2_diagonal - self.target
+        return self.shift * (d * d if self.quadratic else d)
+
+    def value(self, x, s2x):
+        """<x|P|x> for a normalised x with s2x = S^2 x (device or host)."""
+        if self.quadratic:
+            d = s2x - self.target * x
+            return self.shift * float((d * d).sum())
+        return self.shift * (float((x * s2x).sum()) - self.target)
 
 
 def fix_spin_(fciobj, shift=0.2, ss=None):
@@ -556,25 +570,22 @@ def fix_spin_(fciobj, shift=0.2, ss=None):
 
 
 class _PenaltyBasis:
-    """GAS basis whose H application adds shift * (S^2 - target)[^2]
-    (_spin_penalty_action, fci_gas.py:242-252)."""
+    """GAS basis whose H application adds the spin penalty P (SpinPenalty)."""
 
     def __init__(self, gas, spin, params):
         self.gas, self.spin, self.params = gas, spin, params
         self.n = gas.ndet
 
     def penalty(self, x, out, t1, t2, ops):
-        """out += shift * delta(x), using two scratch vectors."""
-        shift, target, minimum, _ = self.params
-        n = self.n
+        """out += P x with two scratch vectors (no allocation per call)."""
+        pen, n = self.params, self.n
         self.spin.contract_dev(x, t1)
-        ops.lincomb(1, n, x, numpy.array([-target]), 1.0, t1)
-        d = t1
-        if target >= minimum + 0.1:
+        ops.lincomb(1, n, x, numpy.array([-pen.target]), 1.0, t1)
+        if pen.quadratic:
             self.spin.contract_dev(t1, t2)
-            ops.lincomb(1, n, t1, numpy.array([-target]), 1.0, t2)
-            d = t2
-        ops.lincomb(1, n, d, numpy.array([shift]), 1.0, out)
+            ops.lincomb(1, n, t1, numpy.array([-pen.target]), 1.0, t2)
+            t1 = t2
+        ops.lincomb(1, n, t1, numpy.array([pen.shift]), 1.0, out)
 
     def hop(self, plan):
         import torch
@@ -861,7 +872,7 @@ class FCISolver(_ds.FCISolver):
             raise ValueError("pspace_size must be nonnegative")
         nelec = _ds._unpack_nelec(nelec, self.spin)
         self.norb, self.nelec = int(norb), nelec
-        penalty = _spin_penalty_parameters(self, norb, nelec)
+        penalty = SpinPenalty.of(self, norb, nelec)
         self.e_spin_penalty = self.e_physical = self.spin_penalty_method = None
         gas = self._space(norb, nelec)
         if nroots > gas.ndet:
@@ -886,11 +897,8 @@ class FCISolver(_ds.FCISolver):
             addr = torch.arange(gas.ndet, device="cuda")
             H = self._h_block(gas, h1e, eri, addr)
             if full_spin_pspace:
-                shift, target, minimum, _ = penalty
-                d = spin.matrix() - target * numpy.eye(gas.ndet)
-                if target >= minimum + 0.1:
-                    d = spin.matrix() @ d - target * d
-                H = H + shift * d
+                S2 = spin.matrix()
+                H = H + penalty.apply(lambda v: S2 @ v, numpy.eye(gas.ndet))
                 self.spin_penalty_method = "exact-small-space"
             w, v = numpy.linalg.eigh(H)
             e = w[:nroots]
@@ -901,11 +909,7 @@ class FCISolver(_ds.FCISolver):
             if penalty is not None:
                 ci0 = self._penalty_guesses(gas, spin, penalty, hdiag, ci0, nroots, lindep)
                 basis = _PenaltyBasis(gas, spin, penalty)
-                shift, target, minimum, _ = penalty
-                sd = spin.diagonal_dev() - target
-                if target >= minimum + 0.1:
-                    sd = sd * sd
-                pre_diag = hdiag + shift * sd
+                pre_diag = hdiag + penalty.diagonal(spin.diagonal_dev())
                 self.spin_penalty_method = "projected-plus-global-davidson"
             elif ci0 is not None:
                 ci0 = self._guess_list(ci0, nroots, gas.ndet)
@@ -918,17 +922,10 @@ class FCISolver(_ds.FCISolver):
             self.stats = stats
         e = numpy.asarray(e, dtype=numpy.float64).reshape(-1) + float(ecore)
         if penalty is not None:
-            shift, target, minimum, _ = penalty
-            pen = []
-            for x in c[:nroots]:
-                x = x.reshape(-1)
-                sx = spin.contract_dev(x, torch.empty_like(x))
-                if target < minimum + 0.1:
-                    pen.append(shift * (float(torch.dot(x, sx)) - target))
-                else:
-                    dlt = sx - target * x
-                    pen.append(shift * float(torch.dot(dlt, dlt)))
-            pen = numpy.asarray(pen)
+            pen = numpy.array([penalty.value(x.reshape(-1),
+                                             spin.contract_dev(x.reshape(-1),
+                                                               torch.empty_like(x.reshape(-1))))
+                               for x in c[:nroots]])
             phys = e[:nroots] - pen
             self.e_spin_penalty = float(pen[0]) if nroots == 1 else pen
             self.e_physical = float(phys[0]) if nroots == 1 else phys
@@ -949,7 +946,7 @@ class FCISolver(_ds.FCISolver):
         orthonormalised (nroots of them), then the unprojected guesses of
         the first roots as global probes."""
         import torch
-        shift, target, minimum, eig = penalty
+        target, eig = penalty.target, penalty.spectrum
         count = min(gas.ndet, max(4 * nroots, nroots + 4))
         default = self._init_guess_dev(hdiag, count)
         if ci0 is None:

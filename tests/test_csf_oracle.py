@@ -107,3 +107,22 @@ def test_hdiag_csf_vs_brute_force(norb, na, nb, smult):
     ref = numpy.einsum("kd,de,ke->k", U, H, U)
     mine = ocsf.hdiag_csf(h1e, eri, norb, na, nb, smult)
     numpy.testing.assert_allclose(mine, ref, atol=1e-12)
+
+
+def test_irrep_names_of_the_d2h_subgroups():
+    from conftest import package
+    cs = package().csf_symm
+    assert cs.irrep_name2id("D2h", "B3u") == 7 and cs.irrep_name2id("D2h", "ag") == 0
+    assert cs.irrep_name2id("Dooh", "B1u") == 5          # read as D2h
+    assert cs.irrep_name2id("Coov", "A2") == 1           # read as C2v
+    assert cs.irrep_name2id("Cs", 'A"') == 1
+    for g, table in cs.IRREP_IDS.items():
+        assert sorted(table.values()) == list(range(len(table)))
+        for name, i in table.items():
+            assert cs.irrep_id2name(g, i) == name
+    # the product of two irreps is the XOR of their ids: B1 x B2 = A2 in C2v
+    assert cs.irrep_name2id("C2v", "B1") ^ cs.irrep_name2id("C2v", "B2") == cs.irrep_name2id("C2v", "A2")
+    with pytest.raises(ValueError):
+        cs.irrep_name2id("C2v", "Eg")
+    with pytest.raises(ValueError):
+        cs.irrep_name2id("Oh", "A1g")

@@ -352,3 +352,28 @@ def test_supports_match_unit_transforms(sym):
         numpy.testing.assert_array_equal(got_a[order], nz)
         numpy.testing.assert_allclose(got_c[order], d[nz], rtol=0, atol=1e-15)
     t.close()
+
+
+def test_csf_symm_solver_irrep_names_and_restored_attributes():
+    """csf_symm.FCISolver: an irrep name with the molecule's group gives the
+    irrep id's spectrum; kernel kwargs apply to that call only; the guess
+    needs orbsym and wfnsym; csf_solver(symm=True) returns this solver."""
+    from types import SimpleNamespace
+    cs = pkg.csf_symm
+    n, na, nb, sm = 6, 3, 3, 1
+    h1, g = _symm_problem(ORBSYM6, 11)
+    mol = SimpleNamespace(groupname="C2v", symmetry=True)
+    sol = pkg.csf_fci.csf_solver(mol, smult=sm)
+    assert isinstance(sol, cs.FCISolver)
+    for name in ("A1", "B2"):
+        w = cs.irrep_name2id("C2v", name)
+        want, _ = _levels(h1, g, n, na, nb, sm, ORBSYM6, w)
+        e, ci = sol.kernel(h1, g, n, (na, nb), orbsym=ORBSYM6, wfnsym=name, nroots=2)
+        numpy.testing.assert_allclose(e, want[:2], atol=1e-9)
+        assert sol.orbsym is None and sol.wfnsym is None
+        irr = _det_irreps(n, na, nb, ORBSYM6)
+        assert abs(ci[0].reshape(-1)[irr != w]).max() < 1e-12
+    with pytest.raises(AssertionError):
+        sol.get_init_guess(n, (na, nb), 1, numpy.zeros(10))
+    with pytest.raises(NotImplementedError):
+        sol.kernel(h1, g, n, (na, nb), orbsym=ORBSYM6, wfnsym=12)

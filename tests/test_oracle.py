@@ -199,3 +199,44 @@ def test_sigma_rows_rejects_bad_index():
     h1e, eri, c = syn.make_problem(4, (2, 2), seed=0)
     with pytest.raises(ValueError):
         oracle.sigma_rows(h1e, eri, c, 4, (2, 2), [c.size])
+
+
+# --- initial guess and preconditioner of the reference's solver
+
+def test_splitmix64_known_answers():
+    """The first four outputs of SplitMix64 from state 0 (Vigna's
+    splitmix64.c): state k*golden mixed; the init-guess noise of root r at
+    index 0 is mix((r+1)*golden), i.e. these values."""
+    want = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F, 0xF88BB8A8724C81EC]
+    got = [oracle.splitmix64_mix(k * 0x9E3779B97F4A7C15) for k in (1, 2, 3, 4)]
+    assert got == want
+    _, g = oracle.init_guess(numpy.arange(5.0), 4, noise_norm=1.0)
+    for r, w in enumerate(want):
+        assert numpy.sign(g[r][0] - (1.0 if r == 0 else 0.0)) == (1.0 if w & 1 else -1.0)
+
+
+def test_init_guess_oracle_structure():
+    rng = numpy.random.default_rng(4)
+    hd = rng.normal(size=1000)
+    addrs, g = oracle.init_guess(hd, 3)
+    assert list(addrs) == list(numpy.argsort(hd)[:3])
+    for r, (a, v) in enumerate(zip(addrs, g)):
+        noise = v.copy()
+        noise[a] -= 1.0
+        assert abs(numpy.linalg.norm(noise) - 1e-3) < 1e-15
+        # (1 + x) - 1 recovers x only to the rounding of 1
+        assert numpy.allclose(numpy.abs(noise), 1e-3 / numpy.sqrt(1000.0), rtol=0, atol=2e-16)
+    # the noise patterns of different roots are independent (not shifts)
+    s0, s1 = numpy.sign(g[0]), numpy.sign(g[1])
+    assert 0.3 < numpy.mean(s0[10:] == s1[10:]) < 0.7
+    # whole range when the space is not larger than nroots
+    a2, _ = oracle.init_guess([3.0, 1.0], 2)
+    assert list(a2) == [0, 1]
+
+
+def test_precond_oracle_level_shift():
+    hd = numpy.array([1.0, 0.5 + 1e-4, 0.5 - 1e-4, -2.0, 0.5 + 1e-3])
+    dx = numpy.ones(5)
+    out = oracle.precond(dx, hd, 0.5, 1e-3)
+    # small denominators of either sign become +level_shift
+    numpy.testing.assert_allclose(out, [2.0, 1e3, 1e3, -0.4, 1.0 / (1e-3)], rtol=1e-12)
