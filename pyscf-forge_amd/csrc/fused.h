@@ -252,6 +252,147 @@ __device__ __forceinline__ void fb_commit_maps(FusedBuild<NBLD, MAXW> &st, int n
         }
 }
 
+// ---- Builders of the single-problem fused kernel: c read through a raw
+// buffer resource, so that absent links load zero instead of being selected
+// away.  A raw buffer load at a byte offset >= num_records returns 0; the
+// builders give every absent link the offset FUSED_OOB, which stays out of
+// range for any c of the fused path (below 4 GiB - 1 MiB) even after a
+// column offset (< 128 KiB) is added.
+constexpr uint32_t FUSED_OOB = 0xFFF00000u;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t fused_c_rsrc(const double *c, uint32_t bytes)
+{
+        return __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(c), (short)0, (int)bytes, 0x00020000);
+}
+
+__device__ __forceinline__ double fused_c_load(__amdgpu_buffer_rsrc_t rc, uint32_t off)
+{
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rc, off, 0, 0));
+}
+
+// -x for bit = 1, x for bit = 0 (sign flip without selects)
+__device__ __forceinline__ double fused_flip(double x, uint32_t bit)
+{
+        return __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, x) ^ ((uint64_t)bit << 63));
+}
+
+// Builder state of one chunk: c values in flight and their two signs.
+template <int NBLD, int MAXW = FUSED_MAXW>
+struct FusedGather {
+        // tile rows: 16 per MFMA wavefront, at most npair <= 136 (norb <= 16)
+        static constexpr int ROWS = 16 * MAXW < 136 ? 16 * MAXW : 136;
+        static constexpr int EB = (ROWS * FUSED_NB + 64 * NBLD - 1) / (64 * NBLD);
+        static_assert(EB <= 16, "sign flags hold 16 elements");
+        typedef uint32_t Flags;
+        Flags sg;             // element s: bit 2s alpha sign, 2s+1 beta sign
+        double va[EB], vb[EB];
+        int mr[EB];           // beta pair-map entries (register-held maps)
+};
+
+// Issue the c loads of chunk j: element e = (k, col), k = e / 16, of the
+// tile is sa c[Ja, b0 + col] + sb c[I, Jb]: abyte[k] = byte offset of row Ja
+// with sa in bit 0 (FUSED_OOB if I has no alpha link with pair k, also for
+// k >= npair), mb[e] or mr = (Jb << 1 | sb) or -1.  Elements past the tile
+// or past the row's end load values that are never used (or zeros).
+template <int NBLD, int MAXW, bool REGMAP>
+__device__ __forceinline__ void fg_issue(FusedGather<NBLD, MAXW> &st, uint32_t rowI_byte, int64_t j, int bt,
+                                         const uint32_t *__restrict__ abyte, const int16_t *__restrict__ mb,
+                                         __amdgpu_buffer_rsrc_t rc)
+{
+        constexpr int NB = FUSED_NB, EB = FusedGather<NBLD, MAXW>::EB;
+        constexpr int nbt = NBLD * 64;
+        const uint32_t cbyte = (uint32_t)(j * NB) * 8u;
+        typedef typename FusedGather<NBLD, MAXW>::Flags Flags;
+        Flags sg = 0;
+        uint32_t oa[EB], ob[EB];
+#pragma unroll
+        for (int s = 0; s < EB; s++) {
+                const int e = bt + s * nbt;
+                const uint32_t av = abyte[e >> 4];
+                const int m = REGMAP ? (int)(int16_t)st.mr[s] : (int)mb[e];
+                oa[s] = (av & ~7u) + cbyte + (uint32_t)(e & 15) * 8u;
+                ob[s] = m >= 0 ? rowI_byte + (uint32_t)(m >> 1) * 8u : FUSED_OOB;
+                sg |= (Flags)((av & 1u) | ((uint32_t)(m & 1) << 1)) << (2 * s);
+        }
+        st.sg = sg;
+#pragma unroll
+        for (int s = 0; s < EB; s++) {
+                st.va[s] = fused_c_load(rc, oa[s]);
+                st.vb[s] = fused_c_load(rc, ob[s]);
+        }
+}
+
+template <int NBLD, int MAXW, typename BT>
+__device__ __forceinline__ void fg_commit(const FusedGather<NBLD, MAXW> &st, int npair, int bt, BT *bb)
+{
+        constexpr int EB = FusedGather<NBLD, MAXW>::EB;
+        constexpr int nbt = NBLD * 64;
+        const int nelem = npair * FUSED_NB;
+#pragma unroll
+        for (int s = 0; s < EB; s++) {
+                const int e = bt + s * nbt;
+                if (e < nelem) {
+                        const uint32_t f = (uint32_t)(st.sg >> (2 * s));
+                        bb[e] = (BT)(fused_flip(st.va[s], f & 1u) + fused_flip(st.vb[s], (f >> 1) & 1u));
+                }
+        }
+}
+
+// fg_commit for the bf16x3 kernel (column-major high / low parts, as
+// fb_commit_split).
+template <int NBLD, int MAXW>
+__device__ __forceinline__ void fg_commit_split(const FusedGather<NBLD, MAXW> &st, int npair, int bt,
+                                                __bf16 *__restrict__ bh)
+{
+        constexpr int EB = FusedGather<NBLD, MAXW>::EB, NB = FUSED_NB, KP = FUSED_KPS_BF16;
+        constexpr int nbt = NBLD * 64;
+        __bf16 *bl = bh + NB * KP;
+        const int nelem = npair * NB;
+#pragma unroll
+        for (int s = 0; s < EB; s++) {
+                const int e = bt + s * nbt;
+                if (e < nelem) {
+                        const uint32_t f = (uint32_t)(st.sg >> (2 * s));
+                        const float x = (float)(fused_flip(st.va[s], f & 1u) + fused_flip(st.vb[s], (f >> 1) & 1u));
+                        const __bf16 hi = (__bf16)x;
+                        const int k = e >> 4, col = e & 15;
+                        bh[col * KP + k] = hi;
+                        bl[col * KP + k] = (__bf16)(x - (float)hi);
+                }
+        }
+}
+
+// Register-held beta pair-map entries of chunk j (bf16x3): element e ->
+// pmapb[e / 16][b0 + e % 16] (rows past npair read row npair - 1; never used).
+template <int NBLD, int MAXW>
+__device__ __forceinline__ void fg_issue_mapv(FusedGather<NBLD, MAXW> &st, int npair, int64_t nbp, int64_t j,
+                                              int bt, const int16_t *__restrict__ pmapb)
+{
+        constexpr int NB = FUSED_NB, EB = FusedGather<NBLD, MAXW>::EB;
+        constexpr int nbt = NBLD * 64;
+        const uint16_t *pm = reinterpret_cast<const uint16_t *>(pmapb) + j * NB;
+#pragma unroll
+        for (int s = 0; s < EB; s++) {
+                const int e = bt + s * nbt;
+                const int k = (e >> 4) < npair ? (e >> 4) : npair - 1;
+                st.mr[s] = pm[(int64_t)k * nbp + (e & 15)];
+        }
+}
+
+template <int NBLD, int MAXW>
+__device__ __forceinline__ void fg_store_mapv(const FusedGather<NBLD, MAXW> &st, int npair, int bt,
+                                              int16_t *__restrict__ mb)
+{
+        constexpr int EB = FusedGather<NBLD, MAXW>::EB;
+        constexpr int nbt = NBLD * 64;
+        const int nelem = npair * FUSED_NB;
+#pragma unroll
+        for (int s = 0; s < EB; s++) {
+                const int e = bt + s * nbt;
+                if (e < nelem) mb[e] = (int16_t)st.mr[s];
+        }
+}
+
 // One chunk of the contraction on MFMA: the 16 x 16 block
 //   acc[i][j] = sum_k h[16 w + i][k] t1[k][j]
 // over exactly KSP k-steps (4 k per step; 32 for bf16x3), without
@@ -345,6 +486,7 @@ struct fcis_sigma_plan {
         int16_t *d_pmapb16;          // beta pair map as int16, rows padded to nbp (fused path)
         int64_t nbp;
         int nbld;                    // builder wavefronts of the fused kernel
+        int32_t *d_order;            // alpha-string processing order (single-tile fused plans) or null
         int timing;
         hipEvent_t ev[4];
         double ms[3];

@@ -23,6 +23,7 @@
 // too, fci_contract_gas.c:55-71).
 #include <stdlib.h>
 #include <string.h>
+#include <string>
 #include <vector>
 
 #include "common.h"
@@ -290,7 +291,10 @@ __global__ __launch_bounds__(256) void k_contract_bf16x3(int npair, int mtiles, 
 //
 // ROWS: restricted rows (fcis_rowset).  Position a0 + r processes alpha
 // string rows[a0 + r] over its listed beta chunks only; walpha is indexed by
-// position.  Otherwise position a0 + r is alpha string a0 + r, all chunks.
+// position.  Otherwise position r of the tile processes alpha string
+// a0 + rows[r] (a processing order of the tile's strings, see
+// fused_row_order) or a0 + r when rows is null, all chunks; walpha is
+// indexed by string.
 template <int PREC, bool ROWS>
 __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_fused_row(
         int npair, int mtiles, int ksteps, int nlinka, int64_t nb, int64_t nstra, int64_t a0,
@@ -315,11 +319,14 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
         char *bbase = smem + (size_t)nbp * sizeof(double);
         BT *buf0 = reinterpret_cast<BT *>(bbase);
         BT *buf1 = reinterpret_cast<BT *>(bbase + BB);
-        int *arow = reinterpret_cast<int *>(bbase + 2 * BB);   // pmapa[pq][I]
-        int *aslot = arow + MAXP;                                  // alpha slot of pq, or -1
+        // abyte[pq]: byte offset of c row Ja of I's alpha link with pair pq,
+        // its sign in bit 0, or FUSED_OOB (pq >= npair too)
+        uint32_t *abyte = reinterpret_cast<uint32_t *>(bbase + 2 * BB);
+        int *aslot = reinterpret_cast<int *>(abyte + MAXP);        // alpha slot of pq, or -1
         int16_t *maps = reinterpret_cast<int16_t *>(aslot + MAXP); // FUSED_NMAP x [MAXP][NB] beta maps
 
         const int tid = threadIdx.x, nthr = blockDim.x;
+        const __amdgpu_buffer_rsrc_t rc = fused_c_rsrc(c, (uint32_t)(nstra * nb * 8));
         const int wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
         const bool mfma_wave = wave < mtiles;
         const int64_t nch = (nb + NB - 1) / NB;
@@ -375,7 +382,7 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                         constexpr int EMW = (2 * MAXP + 63) / 64;
                         v4i m0[EMW], m1[EMW];
                         for (int64_t r = blockIdx.x; r < ta; r += gridDim.x) {
-                                const int64_t I = ROWS ? (int64_t)rows[a0 + r] : a0 + r;
+                                const int64_t I = ROWS ? (int64_t)rows[a0 + r] : a0 + (rows != nullptr ? (int64_t)rows[r] : r);
                                 int64_t nl = nch, c0 = 0;
                                 if constexpr (ROWS) {
                                         const int2 e = rowcl[a0 + r];
@@ -439,9 +446,9 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                         }
                         return;
                 }
-                FusedBuild<NBA> s0, s1;   // c of even / odd chunks in flight
+                FusedGather<NBA> s0, s1;   // c of even / odd chunks in flight
                 for (int64_t r = blockIdx.x; r < ta; r += gridDim.x) {
-                        const int64_t I = ROWS ? (int64_t)rows[a0 + r] : a0 + r;
+                        const int64_t I = ROWS ? (int64_t)rows[a0 + r] : a0 + (rows != nullptr ? (int64_t)rows[r] : r);
                         int64_t nl = nch;
                         int64_t c0 = 0;
                         if constexpr (ROWS) {
@@ -456,10 +463,13 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                                 return ROWS ? list_chunk(c0, jj) : (int64_t)__builtin_amdgcn_readfirstlane((int)jj);
                         };
                         __syncthreads();   // (1)
-                        for (int q = bt; q < npair; q += NBA * 64) {
-                                arow[q] = pmapa[(int64_t)q * nstra + I];
+                        for (int q = bt; q < MAXP; q += NBA * 64) {
+                                const int a = q < npair ? pmapa[(int64_t)q * nstra + I] : -1;
+                                abyte[q] = a >= 0 ? (uint32_t)(a >> 1) * (uint32_t)nb * 8u | (uint32_t)(a & 1)
+                                                  : FUSED_OOB;
                                 aslot[q] = -1;
                         }
+                        const uint32_t rowI_byte = (uint32_t)(I * nb) * 8u;
                         __syncthreads();   // (2)
                         for (int k = bt; k < nlinka; k += NBA * 64)
                                 aslot[link_pair(linka[(int64_t)k * nstra + I].info)] = k;
@@ -468,41 +478,41 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                         auto cc = [&](int64_t jj) -> int64_t { return chunk(jj < nl ? jj : nl - 1); };
                         auto mbuf = [&](int64_t jj) -> int16_t * { return maps + (jj % FUSED_NMAP) * MAXP * NB; };
                         if constexpr (!MAPWAVE) {
-                                fb_issue_mapv(s0, npair, nbp, cc(0), bt, pmapb);
-                                fb_issue_mapv(s1, npair, nbp, cc(1), bt, pmapb);
+                                fg_issue_mapv(s0, npair, nbp, cc(0), bt, pmapb);
+                                fg_issue_mapv(s1, npair, nbp, cc(1), bt, pmapb);
                         }
                         __syncthreads();   // (3)
                         // loads left in flight here, oldest first: maps of chunk 2, c of
                         // chunk 1 -- the order the loop body leaves them in, so that the
                         // loop head sees one order and every wait stays partial
                         if constexpr (MAPWAVE) {
-                                fb_issue_c<NBA, FUSED_MAXW, false>(s0, npair, nb, I, cc(0), bt, arow, mbuf(0), c);
-                                fb_issue_c<NBA, FUSED_MAXW, false>(s1, npair, nb, I, cc(1), bt, arow, mbuf(1), c);
+                                fg_issue<NBA, FUSED_MAXW, false>(s0, rowI_byte, cc(0), bt, abyte, mbuf(0), rc);
+                                fg_issue<NBA, FUSED_MAXW, false>(s1, rowI_byte, cc(1), bt, abyte, mbuf(1), rc);
                         } else {
-                                fb_issue_c<NBA, FUSED_MAXW, true>(s0, npair, nb, I, cc(0), bt, arow, nullptr, c);
-                                fb_store_mapv(s0, npair, bt, mbuf(0));
-                                fb_issue_mapv(s0, npair, nbp, cc(2), bt, pmapb);
-                                fb_issue_c<NBA, FUSED_MAXW, true>(s1, npair, nb, I, cc(1), bt, arow, nullptr, c);
-                                fb_store_mapv(s1, npair, bt, mbuf(1));
+                                fg_issue<NBA, FUSED_MAXW, true>(s0, rowI_byte, cc(0), bt, abyte, nullptr, rc);
+                                fg_store_mapv(s0, npair, bt, mbuf(0));
+                                fg_issue_mapv(s0, npair, nbp, cc(2), bt, pmapb);
+                                fg_issue<NBA, FUSED_MAXW, true>(s1, rowI_byte, cc(1), bt, abyte, nullptr, rc);
+                                fg_store_mapv(s1, npair, bt, mbuf(1));
                         }
                         if constexpr (PRESPLIT)
-                                fb_commit_split(s0, npair, bt, reinterpret_cast<__bf16 *>(buf0));
+                                fg_commit_split(s0, npair, bt, reinterpret_cast<__bf16 *>(buf0));
                         else
-                                fb_commit_c(s0, npair, bt, buf0);
+                                fg_commit(s0, npair, bt, buf0);
                         __syncthreads();   // (4)
                         // phase j: `issue` holds the map entries of chunk j+2 (loaded
                         // one phase earlier) and issues its c loads; `commit` holds
                         // the c values of chunk j+1 (issued one phase earlier),
                         // committed now, and loads the map entries of chunk j+3.  The
                         // loads of this phase stay in flight across the barrier.
-                        auto phase = [&](int64_t j, FusedBuild<NBA> &issue, FusedBuild<NBA> &commit) {
+                        auto phase = [&](int64_t j, FusedGather<NBA> &issue, FusedGather<NBA> &commit) {
                                 if constexpr (MAPWAVE) {
-                                        fb_issue_c<NBA, FUSED_MAXW, false>(issue, npair, nb, I, cc(j + 2), bt, arow,
-                                                                            mbuf(j + 2), c);
+                                        fg_issue<NBA, FUSED_MAXW, false>(issue, rowI_byte, cc(j + 2), bt, abyte,
+                                                                          mbuf(j + 2), rc);
                                 } else {
-                                        fb_issue_mapv(commit, npair, nbp, cc(j + 3), bt, pmapb);
-                                        fb_issue_c<NBA, FUSED_MAXW, true>(issue, npair, nb, I, cc(j + 2), bt, arow,
-                                                                           nullptr, c);
+                                        fg_issue_mapv(commit, npair, nbp, cc(j + 3), bt, pmapb);
+                                        fg_issue<NBA, FUSED_MAXW, true>(issue, rowI_byte, cc(j + 2), bt, abyte,
+                                                                         nullptr, rc);
                                 }
                                 // unconditional: past the row's last chunk the commit and
                                 // the map store go to the B buffer / map slice that no
@@ -510,11 +520,11 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                                 // Every path to the loop head then has the same loads in
                                 // flight, and the commits wait for their own chunk only.
                                 if constexpr (PRESPLIT)
-                                        fb_commit_split(commit, npair, bt,
+                                        fg_commit_split(commit, npair, bt,
                                                         reinterpret_cast<__bf16 *>(((j + 1) & 1) ? buf1 : buf0));
                                 else
-                                        fb_commit_c(commit, npair, bt, ((j + 1) & 1) ? buf1 : buf0);
-                                if constexpr (!MAPWAVE) fb_store_mapv(issue, npair, bt, mbuf(j + 2));
+                                        fg_commit(commit, npair, bt, ((j + 1) & 1) ? buf1 : buf0);
+                                if constexpr (!MAPWAVE) fg_store_mapv(issue, npair, bt, mbuf(j + 2));
                                 __syncthreads();
                         };
                         int64_t j = 0;
@@ -545,7 +555,7 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                         al[ks] = ks < ksteps ? ((const typename Ty::A *)hfrag_lo_v)[o] : typename Ty::A{};
         }
         for (int64_t r = blockIdx.x; r < ta; r += gridDim.x) {
-                const int64_t I = ROWS ? (int64_t)rows[a0 + r] : a0 + r;
+                const int64_t I = ROWS ? (int64_t)rows[a0 + r] : a0 + (rows != nullptr ? (int64_t)rows[r] : r);
                 int64_t nl = nch;
                 int64_t c0 = 0;
                 if constexpr (ROWS) {
@@ -557,9 +567,26 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                 __syncthreads();   // (2)
                 __syncthreads();   // (3)
                 __syncthreads();   // (4)
+                // this lane's four output rows (f64 MFMA: D row lk + 4q; f32/bf16
+                // MFMA: D row 4lk + q) and their alpha slots, for the whole row
+                int slot[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                        const int pq = wave * 16 + (PREC == FCIS_PREC_FP64 ? lk + 4 * q : 4 * lk + q);
+                        slot[q] = pq < npair ? aslot[pq] : -1;
+                }
+                typename Ty::Out *wrow = walpha + (ROWS ? r : I - a0) * nbp;
                 for (int64_t j = 0; j < nl; j++) {
                         const BT *bb = (j & 1) ? buf1 : buf0;
                         const int16_t *mb = maps + (j % FUSED_NMAP) * MAXP * NB;
+                        // the beta-scatter map entries, read before the MFMA chain so
+                        // that the scatter does not wait on LDS
+                        int em[4];
+#pragma unroll
+                        for (int q = 0; q < 4; q++) {
+                                const int pq = wave * 16 + (PREC == FCIS_PREC_FP64 ? lk + 4 * q : 4 * lk + q);
+                                em[q] = pq < npair ? (int)mb[pq * NB + li] : -1;
+                        }
                         typename Ty::Acc acc;
                         if constexpr (PRESPLIT)
                                 acc = mma_presplit<KS>(ah, al, reinterpret_cast<const __bf16 *>(bb), lk, li);
@@ -569,15 +596,9 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                         if (b < nb) {
 #pragma unroll
                                 for (int q = 0; q < 4; q++) {
-                                        // f64 MFMA: D row lk + 4q; f32/bf16 MFMA: D row 4lk + q
-                                        int pq = wave * 16 + (PREC == FCIS_PREC_FP64 ? lk + 4 * q : 4 * lk + q);
-                                        if (pq < npair) {
-                                                double v = (double)acc[q];
-                                                int e = mb[pq * NB + li];
-                                                if (e >= 0) atomicAdd(&srow[e >> 1], (e & 1) ? -v : v);
-                                                int k = aslot[pq];
-                                                if (k >= 0) walpha[(int64_t)k * ld + r * nbp + b] = acc[q];
-                                        }
+                                        const double v = (double)acc[q];
+                                        if (em[q] >= 0) atomicAdd(&srow[em[q] >> 1], fused_flip(v, em[q] & 1));
+                                        if (slot[q] >= 0) wrow[(int64_t)slot[q] * ld + b] = acc[q];
                                 }
                         }
                         __syncthreads();
@@ -833,6 +854,112 @@ int upload_h(fcis_sigma_plan *p, const double *h2e)
 
 namespace {
 
+// Processing order of the alpha strings of a single-tile fused plan.  The
+// persistent workgroups take positions r = blockIdx.x + k * grid, and
+// workgroups are handed to the 8 XCDs round robin, so the 32 workgroups an
+// XCD runs together in round k hold positions with one residue mod 8.  A
+// row gathers c from its alpha neighbours (strings one excitation away), and
+// rows that run together on an XCD share that XCD's L2, so each XCD gets a
+// block of strings with few distinct neighbours:
+//   "greedy"   blocks grown from the lowest unplaced string by repeatedly
+//              adding the string with most neighbours already in the
+//              block's neighbourhood (about 26 distinct neighbours per row
+//              at (16o,8e) against 72 unshared),
+//   "xcd"      consecutive colex ranks per XCD (about 30),
+//   "colex"    no reordering (position r = string r; about 44 per row).
+// FCIS_ROW_ORDER selects one (default greedy).
+std::vector<int32_t> greedy_blocks(const fcis_space *s, int block)
+{
+        const int64_t na = s->nstra;
+        const int n = s->norb;
+        // neighbour lists (every string one alpha excitation away, itself
+        // included) from the strings
+        std::vector<int32_t> nb_off(na + 1, 0), nb;
+        nb.reserve((size_t)na * (s->nlinka ? s->nlinka : 1));
+        for (int64_t a = 0; a < na; a++) {
+                const uint64_t str = fcis::space_string(s, 0, a);
+                nb.push_back((int32_t)a);
+                for (int q = 0; q < n; q++) {
+                        if (!((str >> q) & 1)) continue;
+                        for (int pp = 0; pp < n; pp++) {
+                                if ((str >> pp) & 1) continue;
+                                nb.push_back((int32_t)fcis::space_addr(s, 0, (str & ~(1ull << q)) | (1ull << pp)));
+                        }
+                }
+                nb_off[a + 1] = (int32_t)nb.size();
+        }
+        std::vector<int32_t> order;
+        order.reserve(na);
+        std::vector<char> placed(na, 0), in_u(na, 0);
+        std::vector<int32_t> cnt(na, 0), touched, ulist;
+        int64_t next = 0;
+        while ((int64_t)order.size() < na) {
+                while (placed[next]) next++;
+                auto add = [&](int64_t x) {
+                        placed[x] = 1;
+                        order.push_back((int32_t)x);
+                        for (int32_t i = nb_off[x]; i < nb_off[x + 1]; i++) {
+                                const int32_t y = nb[i];
+                                if (in_u[y]) continue;
+                                in_u[y] = 1;
+                                ulist.push_back(y);
+                                for (int32_t k = nb_off[y]; k < nb_off[y + 1]; k++) {
+                                        if (cnt[nb[k]]++ == 0) touched.push_back(nb[k]);
+                                }
+                        }
+                };
+                add(next);
+                for (int b = 1; b < block && (int64_t)order.size() < na; b++) {
+                        int64_t best = -1;
+                        for (int32_t z : touched)
+                                if (!placed[z] && (best < 0 || cnt[z] > cnt[best] || (cnt[z] == cnt[best] && z < best)))
+                                        best = z;
+                        if (best < 0) {
+                                while (placed[next]) next++;
+                                best = next;
+                        }
+                        add(best);
+                }
+                for (int32_t z : touched) cnt[z] = 0;
+                for (int32_t y : ulist) in_u[y] = 0;
+                touched.clear();
+                ulist.clear();
+        }
+        return order;
+}
+
+int build_row_order(fcis_sigma_plan *p)
+{
+        const char *env = getenv("FCIS_ROW_ORDER");
+        const std::string mode = env ? env : "greedy";
+        const int64_t na = p->space->nstra;
+        const int grid = p->fused_grid, nx = 8;
+        if (mode == "colex" || grid % nx != 0 || na < grid) return FCIS_SUCCESS;
+        const int per = grid / nx;
+        std::vector<int32_t> order;
+        if (mode == "greedy") {
+                order = greedy_blocks(p->space, per);
+        } else {
+                order.resize(na);
+                for (int64_t a = 0; a < na; a++) order[a] = (int32_t)a;
+        }
+        // position r = k * grid + w runs on workgroup w (XCD w % 8) in round
+        // k: full rounds give XCD x the entries x * per .. x * per + per - 1
+        // of that round's slice of the order
+        std::vector<int32_t> pos(na);
+        for (int64_t r = 0; r < na; r++) {
+                const int64_t k = r / grid, w = r - k * grid;
+                const bool full = (k + 1) * grid <= na;
+                pos[r] = full ? order[k * grid + (w % nx) * per + w / nx] : order[r];
+        }
+        if (fcis::dev_malloc(&p->d_order, 4 * (size_t)na) != hipSuccess) {
+                fcis::set_error("fcis_sigma_plan_create: device allocation failed (row order)");
+                return FCIS_ERR_MEMORY;
+        }
+        FCIS_HIP(hipMemcpy(p->d_order, pos.data(), 4 * (size_t)na, hipMemcpyHostToDevice));
+        return FCIS_SUCCESS;
+}
+
 // One launch of the fused row kernel over positions [a0, a0 + ta) (alpha
 // strings, or the row set's positions when ROWS), walpha in p->d_vt1.
 template <bool ROWS>
@@ -842,7 +969,7 @@ void launch_fused(const fcis_sigma_plan *p, hipStream_t st, int64_t a0, int64_t 
         const fcis_space *s = p->space;
         const int64_t nb = s->nstrb, na = s->nstra;
         const dim3 fgrid(p->fused_grid), fblk(64 * (p->mtiles + p->nbld));
-        const int32_t *rows = ROWS ? rs->d_rows : nullptr;
+        const int32_t *rows = ROWS ? rs->d_rows : p->d_order;
         const int2 *rowcl = ROWS ? rs->d_rowcl : nullptr;
         const uint16_t *clist = ROWS ? rs->d_clist : nullptr;
 #define FCIS_LAUNCH_FUSED(PR, OUT)                                                                  \
@@ -1088,6 +1215,13 @@ int fcis::sigma_plan_create(fcis_sigma_plan **out, const fcis_space *space, cons
                                    p->nbp, p->d_pmapb, p->d_pmapb16);
                 FCIS_CHECK_LAUNCH();
         }
+        if (p->fused && p->ntiles == 1) {
+                int st2 = build_row_order(p);
+                if (st2 != FCIS_SUCCESS) {
+                        fcis_sigma_plan_free(p);
+                        return st2;
+                }
+        }
         FCIS_HIP(hipDeviceSynchronize());
         *out = p;
         return FCIS_SUCCESS;
@@ -1247,6 +1381,7 @@ void fcis_sigma_plan_free(fcis_sigma_plan_t *p)
         (void)hipFree(p->d_pmapb16);
         (void)hipFree(p->d_t1);
         (void)hipFree(p->d_vt1);
+        (void)hipFree(p->d_order);
         delete p;
 }
 
