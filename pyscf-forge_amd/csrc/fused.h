@@ -15,7 +15,9 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 namespace {
 
 constexpr int FUSED_NB = 16;      // determinants per chunk (one MFMA column block)
-constexpr int FUSED_MAXW = 9;     // wavefronts per workgroup (norb <= 16)
+constexpr int FUSED_MAXW = 9;     // 16-row tiles of npair (norb <= 16): B tile and LDS sizing
+constexpr int FUSED_MTILES = 8;   // MFMA wavefronts of the single-problem fused kernel
+constexpr int FUSED_NBLD = 8;     // its builder wavefronts (7 gathering + 1 pair-map / diagonal)
 constexpr int FUSED_KS = 34;      // k-steps held in registers (4-wide MFMA): npair <= 136
 constexpr int FUSED_KS_BF16 = 5;  // k-steps held in registers (32-wide bf16 MFMA)
 constexpr int FUSED_NACC = 2;     // independent accumulator chains (FP64/FP32)
@@ -286,6 +288,7 @@ struct FusedGather {
         typedef uint32_t Flags;
         Flags sg;             // element s: bit 2s alpha sign, 2s+1 beta sign
         double va[EB], vb[EB];
+        double vf[EB];        // fBeta[k][b] of each element (diagonal term)
         int mr[EB];           // beta pair-map entries (register-held maps)
 };
 
@@ -294,10 +297,14 @@ struct FusedGather {
 // with sa in bit 0 (FUSED_OOB if I has no alpha link with pair k, also for
 // k >= npair), mb[e] or mr = (Jb << 1 | sb) or -1.  Elements past the tile
 // or past the row's end load values that are never used (or zeros).
-template <int NBLD, int MAXW, bool REGMAP>
+//
+// With DIAG, also the fBeta[k][b0 + col] of every element (rows nbp apart,
+// through the buffer resource fr: rows past npair read zero) for the
+// diagonal term.
+template <int NBLD, int MAXW, bool REGMAP, bool DIAG = false>
 __device__ __forceinline__ void fg_issue(FusedGather<NBLD, MAXW> &st, uint32_t rowI_byte, int64_t j, int bt,
                                          const uint32_t *__restrict__ abyte, const int16_t *__restrict__ mb,
-                                         __amdgpu_buffer_rsrc_t rc)
+                                         __amdgpu_buffer_rsrc_t rc, __amdgpu_buffer_rsrc_t fr, int64_t nbp)
 {
         constexpr int NB = FUSED_NB, EB = FusedGather<NBLD, MAXW>::EB;
         constexpr int nbt = NBLD * 64;
@@ -320,46 +327,64 @@ __device__ __forceinline__ void fg_issue(FusedGather<NBLD, MAXW> &st, uint32_t r
                 st.va[s] = fused_c_load(rc, oa[s]);
                 st.vb[s] = fused_c_load(rc, ob[s]);
         }
+        if constexpr (DIAG) {
+                // element s of this thread: row (bt >> 4) + s * nbt / 16, column bt & 15
+                static_assert(nbt % 16 == 0, "a thread keeps its column");
+                const uint32_t voff = (uint32_t)(((int64_t)(bt >> 4) * nbp + j * NB + (bt & 15)) * 8);
+#pragma unroll
+                for (int s = 0; s < EB; s++)
+                        st.vf[s] = fused_c_load(fr, voff + (uint32_t)(s * (nbt / 16) * nbp * 8));
+        }
 }
 
-template <int NBLD, int MAXW, typename BT>
-__device__ __forceinline__ void fg_commit(const FusedGather<NBLD, MAXW> &st, int npair, int bt, BT *bb)
+// Commit the tile elements; with DIAG, return sum_s fBeta * t1 over this
+// thread's elements (its column's part of the diagonal term).
+template <int NBLD, int MAXW, bool DIAG = false, typename BT>
+__device__ __forceinline__ double fg_commit(const FusedGather<NBLD, MAXW> &st, int npair, int bt, BT *bb)
 {
         constexpr int EB = FusedGather<NBLD, MAXW>::EB;
         constexpr int nbt = NBLD * 64;
         const int nelem = npair * FUSED_NB;
+        double d = 0.0;
 #pragma unroll
         for (int s = 0; s < EB; s++) {
                 const int e = bt + s * nbt;
                 if (e < nelem) {
                         const uint32_t f = (uint32_t)(st.sg >> (2 * s));
-                        bb[e] = (BT)(fused_flip(st.va[s], f & 1u) + fused_flip(st.vb[s], (f >> 1) & 1u));
+                        const double t = fused_flip(st.va[s], f & 1u) + fused_flip(st.vb[s], (f >> 1) & 1u);
+                        bb[e] = (BT)t;
+                        if constexpr (DIAG) d += st.vf[s] * t;
                 }
         }
+        return d;
 }
 
 // fg_commit for the bf16x3 kernel (column-major high / low parts, as
 // fb_commit_split).
-template <int NBLD, int MAXW>
-__device__ __forceinline__ void fg_commit_split(const FusedGather<NBLD, MAXW> &st, int npair, int bt,
-                                                __bf16 *__restrict__ bh)
+template <int NBLD, int MAXW, bool DIAG = false>
+__device__ __forceinline__ double fg_commit_split(const FusedGather<NBLD, MAXW> &st, int npair, int bt,
+                                                  __bf16 *__restrict__ bh)
 {
         constexpr int EB = FusedGather<NBLD, MAXW>::EB, NB = FUSED_NB, KP = FUSED_KPS_BF16;
         constexpr int nbt = NBLD * 64;
         __bf16 *bl = bh + NB * KP;
         const int nelem = npair * NB;
+        double d = 0.0;
 #pragma unroll
         for (int s = 0; s < EB; s++) {
                 const int e = bt + s * nbt;
                 if (e < nelem) {
                         const uint32_t f = (uint32_t)(st.sg >> (2 * s));
-                        const float x = (float)(fused_flip(st.va[s], f & 1u) + fused_flip(st.vb[s], (f >> 1) & 1u));
+                        const double t = fused_flip(st.va[s], f & 1u) + fused_flip(st.vb[s], (f >> 1) & 1u);
+                        const float x = (float)t;
                         const __bf16 hi = (__bf16)x;
                         const int k = e >> 4, col = e & 15;
                         bh[col * KP + k] = hi;
                         bl[col * KP + k] = (__bf16)(x - (float)hi);
+                        if constexpr (DIAG) d += st.vf[s] * t;
                 }
         }
+        return d;
 }
 
 // Register-held beta pair-map entries of chunk j (bf16x3): element e ->
@@ -486,6 +511,13 @@ struct fcis_sigma_plan {
         int16_t *d_pmapb16;          // beta pair map as int16, rows padded to nbp (fused path)
         int64_t nbp;
         int nbld;                    // builder wavefronts of the fused kernel
+        // fused kernel's h: rows = the off-diagonal pairs (mpair), then row
+        // mrow_f for the alpha occupations' diagonal term, zero padding
+        int mtiles_f, mrow_f;
+        void *d_hfrag_f, *d_hfrag_f_lo;
+        int16_t *d_mpair;            // [16 * FUSED_MTILES] pair of each fused row
+        double *d_falpha;            // [nstr_a][npair]: sum_{p in I} h[pp][k]
+        double *d_fbeta;             // [npair][nbp]:    sum_{p in b} h[pp][k]
         int32_t *d_order;            // alpha-string processing order (single-tile fused plans) or null
         int timing;
         hipEvent_t ev[4];

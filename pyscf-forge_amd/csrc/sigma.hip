@@ -23,6 +23,7 @@
 // too, fci_contract_gas.c:55-71).
 #include <stdlib.h>
 #include <string.h>
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -296,12 +297,14 @@ __global__ __launch_bounds__(256) void k_contract_bf16x3(int npair, int mtiles, 
 // fused_row_order) or a0 + r when rows is null, all chunks; walpha is
 // indexed by string.
 template <int PREC, bool ROWS>
-__global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_fused_row(
-        int npair, int mtiles, int ksteps, int nlinka, int64_t nb, int64_t nstra, int64_t a0,
+__global__ __launch_bounds__(64 * (FUSED_MTILES + FUSED_NBLD)) void k_fused_row(
+        int npair, int mtiles, int mrow_f, int na, int nlinka, int64_t nb, int64_t nstra, int64_t a0,
         int64_t ta, int64_t ld, int accumulate, const fcis_link *__restrict__ linka,
         const int32_t *__restrict__ pmapa, const int16_t *__restrict__ pmapb, int64_t nbp,
         const double *__restrict__ c, const void *__restrict__ hfrag_v,
-        const void *__restrict__ hfrag_lo_v, typename FusedTypes<PREC>::Out *__restrict__ walpha,
+        const void *__restrict__ hfrag_lo_v, const int16_t *__restrict__ mpair,
+        const double *__restrict__ falpha, const double *__restrict__ fbeta,
+        typename FusedTypes<PREC>::Out *__restrict__ walpha,
         double *__restrict__ sigma, const int32_t *__restrict__ rows,
         const int2 *__restrict__ rowcl, const uint16_t *__restrict__ clist)
 {
@@ -310,7 +313,7 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
         constexpr int KS = Ty::KS, KW = Ty::KW, NB = FUSED_NB;
         constexpr int KPMAX = KS * KW;
         constexpr int MAXP = 16 * FUSED_MAXW;
-        constexpr int NBLD = Ty::NBLD;
+        constexpr int NBLD = FUSED_NBLD;
         extern __shared__ __align__(16) char smem[];
         double *srow = reinterpret_cast<double *>(smem);   // nbp doubles: 16-byte aligned end
         // B tiles: [k][col] in BT, or for bf16x3 bf16 high and low parts [col][k]
@@ -322,7 +325,7 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
         // abyte[pq]: byte offset of c row Ja of I's alpha link with pair pq,
         // its sign in bit 0, or FUSED_OOB (pq >= npair too)
         uint32_t *abyte = reinterpret_cast<uint32_t *>(bbase + 2 * BB);
-        int *aslot = reinterpret_cast<int *>(abyte + MAXP);        // alpha slot of pq, or -1
+        int *aslot = reinterpret_cast<int *>(abyte + MAXP);        // walpha slot of pq, or -1
         int16_t *maps = reinterpret_cast<int16_t *>(aslot + MAXP); // FUSED_NMAP x [MAXP][NB] beta maps
 
         const int tid = threadIdx.x, nthr = blockDim.x;
@@ -330,6 +333,7 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
         const int wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
         const bool mfma_wave = wave < mtiles;
         const int64_t nch = (nb + NB - 1) / NB;
+        const int nelem = npair * NB;
         // ROWS: entry c0 + jj of the chunk lists, read as its 32-bit word (the
         // list is padded to whole words): a uniform address, so a scalar load
         // that never makes the vector-memory counter wait for loads in flight
@@ -338,51 +342,41 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                 return (int64_t)((reinterpret_cast<const uint32_t *>(clist)[q >> 1] >> (16 * (int)(q & 1))) &
                                  0xffffu);
         };
+        auto row_of = [&](int64_t r) -> int64_t {
+                return ROWS ? (int64_t)rows[a0 + r] : a0 + (rows != nullptr ? (int64_t)rows[r] : r);
+        };
 
         if constexpr (PRESPLIT) {
                 // rows k >= npair (and the padding) stay zero; commits write the rest
                 for (int e = tid; e < (int)(2 * BB / 4); e += nthr) reinterpret_cast<uint32_t *>(bbase)[e] = 0u;
         } else {
-                for (int e = npair * NB + tid; e < KPMAX * NB; e += nthr) {
+                for (int e = nelem + tid; e < KPMAX * NB; e += nthr) {
                         buf0[e] = (BT)0;
                         buf1[e] = (BT)0;
                 }
         }
         for (int64_t i = tid; i < nb; i += nthr) srow[i] = 0.0;
 
-        // Barrier schedule per row (both roles): (1) row start, (2) arow,
-        // (3) aslot + maps of chunks 0,1, (4) chunk 0 built, then one per chunk.
+        // Barrier schedule per row (every role): (1) row start, (2) abyte,
+        // (3) aslot + maps of chunks 0..2, (4) chunk 0 built, then one per chunk.
         if (!mfma_wave) {
-                // ---- builder wavefronts
-                // Issue priority: a builder shares its SIMD with two or three MFMA
-                // wavefronts, and the last builder to commit sets the chunk time,
-                // so the builders go first (FP64 σ 184 -> 173 ms at (16o,16e)).
+                // ---- builder wavefronts: seven gather c into the B tiles, the
+                // eighth loads the beta pair maps and forms the diagonal term.
+                // Builders go first at issue (their commits set the chunk time
+                // when they straggle).
                 __builtin_amdgcn_s_setprio(1);
-                // Waves w and w + 4 share a SIMD, so with 9 MFMA wavefronts the
-                // builder at wave 12 shares its SIMD with three of them; it takes
-                // the last builder slot, whose last element group is (partly) empty.
-                int bi = wave - mtiles;
-                if (mtiles == FUSED_MAXW && NBLD == 7) bi = bi == 3 ? 6 : bi == 6 ? 3 : bi;
-                const int bt = bi * 64 + lane;
-                // FP64 / FP32: that last slot gathers no c values; it loads the
-                // beta pair maps of all builders into LDS instead (16-byte loads),
-                // and the other six builders take six element groups each and
-                // read their map entries from LDS (FP64 σ 173 -> 160 ms, FP32
-                // 117 -> 106 ms at (16o,16e)).  bf16x3 keeps seven gathering
-                // builders with register-held map entries (1% faster there).
-                // The map wavefront also keeps the barrier schedule and does its
-                // share of the σ row write-out.
-                constexpr int NBA = PREC == FCIS_PREC_BF16X3 ? NBLD : NBLD - 1;   // gathering builders
-                constexpr bool MAPWAVE = NBA < NBLD;
-                if (bi >= NBA) {
-                        // pair-map wavefront: the beta pair-map slice of chunk j + 3
-                        // into LDS slice (j + 3) % 4 during phase j (loaded one phase
-                        // earlier, 16-byte loads), for the builders' gathers of phase
-                        // j + 1 and the MFMA scatter of phase j + 3
+                const int bi = wave - mtiles;
+                constexpr int NBA = NBLD - 1;   // gathering builders
+                if (bi == NBA) {
+                        // pair-map wavefront.  Phase j: the beta pair-map slice of
+                        // chunk j + 4 is loaded (16-byte loads) and that of chunk
+                        // j + 3, loaded one phase earlier, stored to LDS slice
+                        // (j + 3) % 4 (for the builders' gathers of phase j + 1 and
+                        // the MFMA scatter of phase j + 3).
                         constexpr int EMW = (2 * MAXP + 63) / 64;
                         v4i m0[EMW], m1[EMW];
                         for (int64_t r = blockIdx.x; r < ta; r += gridDim.x) {
-                                const int64_t I = ROWS ? (int64_t)rows[a0 + r] : a0 + (rows != nullptr ? (int64_t)rows[r] : r);
+                                const int64_t I = row_of(r);
                                 int64_t nl = nch, c0 = 0;
                                 if constexpr (ROWS) {
                                         const int2 e = rowcl[a0 + r];
@@ -413,22 +407,18 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                                 };
                                 __syncthreads();   // (1)
                                 __syncthreads();   // (2)
-                                if constexpr (MAPWAVE) {
-                                        issue(m0, 0);
-                                        store(m0, 0);
-                                        issue(m0, 1);
-                                        store(m0, 1);
-                                        issue(m0, 2);
-                                        store(m0, 2);
-                                        issue(m0, 3);
-                                }
+                                issue(m0, 0);
+                                store(m0, 0);
+                                issue(m0, 1);
+                                store(m0, 1);
+                                issue(m0, 2);
+                                store(m0, 2);
+                                issue(m0, 3);
                                 __syncthreads();   // (3)
                                 __syncthreads();   // (4)
                                 auto mphase = [&](int64_t j, v4i(&ld_)[EMW], v4i(&st_)[EMW]) {
-                                        if constexpr (MAPWAVE) {
-                                                issue(ld_, j + 4);
-                                                store(st_, j + 3);
-                                        }
+                                        issue(ld_, j + 4);
+                                        store(st_, j + 3);
                                         __syncthreads();
                                 };
                                 int64_t j = 0;
@@ -446,9 +436,22 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                         }
                         return;
                 }
+                const int bt = bi * 64 + lane;
+                // the beta occupations' part of the diagonal term,
+                //   d[b] = sum_k fBeta[k][b] t1[k][b],
+                // formed at commit from the tile elements in registers (a thread
+                // keeps one column; lanes of a column summed, then one LDS
+                // atomic per column and wavefront)
+                const __amdgpu_buffer_rsrc_t fbr = __builtin_amdgcn_make_buffer_rsrc(
+                        const_cast<double *>(fbeta), (short)0, (int)(npair * nbp * 8), 0x00020000);
+                auto add_diag = [&](double d, int64_t b0) {
+                        d += __shfl_xor(d, 16);
+                        d += __shfl_xor(d, 32);
+                        if (lane < 16 && b0 + (bt & 15) < nb) atomicAdd(&srow[b0 + (bt & 15)], d);
+                };
                 FusedGather<NBA> s0, s1;   // c of even / odd chunks in flight
                 for (int64_t r = blockIdx.x; r < ta; r += gridDim.x) {
-                        const int64_t I = ROWS ? (int64_t)rows[a0 + r] : a0 + (rows != nullptr ? (int64_t)rows[r] : r);
+                        const int64_t I = row_of(r);
                         int64_t nl = nch;
                         int64_t c0 = 0;
                         if constexpr (ROWS) {
@@ -471,60 +474,46 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                         }
                         const uint32_t rowI_byte = (uint32_t)(I * nb) * 8u;
                         __syncthreads();   // (2)
-                        for (int k = bt; k < nlinka; k += NBA * 64)
-                                aslot[link_pair(linka[(int64_t)k * nstra + I].info)] = k;
+                        // walpha slots: the off-diagonal alpha links (slots na ..);
+                        // the diagonal ones act within row I (the diagonal term)
+                        for (int k = na + bt; k < nlinka; k += NBA * 64)
+                                aslot[link_pair(linka[(int64_t)k * nstra + I].info)] = k - na;
                         // chunk jj clamped to the row (loads past the last chunk
                         // repeat it and are never committed)
                         auto cc = [&](int64_t jj) -> int64_t { return chunk(jj < nl ? jj : nl - 1); };
                         auto mbuf = [&](int64_t jj) -> int16_t * { return maps + (jj % FUSED_NMAP) * MAXP * NB; };
-                        if constexpr (!MAPWAVE) {
-                                fg_issue_mapv(s0, npair, nbp, cc(0), bt, pmapb);
-                                fg_issue_mapv(s1, npair, nbp, cc(1), bt, pmapb);
-                        }
                         __syncthreads();   // (3)
-                        // loads left in flight here, oldest first: maps of chunk 2, c of
-                        // chunk 1 -- the order the loop body leaves them in, so that the
-                        // loop head sees one order and every wait stays partial
-                        if constexpr (MAPWAVE) {
-                                fg_issue<NBA, FUSED_MAXW, false>(s0, rowI_byte, cc(0), bt, abyte, mbuf(0), rc);
-                                fg_issue<NBA, FUSED_MAXW, false>(s1, rowI_byte, cc(1), bt, abyte, mbuf(1), rc);
-                        } else {
-                                fg_issue<NBA, FUSED_MAXW, true>(s0, rowI_byte, cc(0), bt, abyte, nullptr, rc);
-                                fg_store_mapv(s0, npair, bt, mbuf(0));
-                                fg_issue_mapv(s0, npair, nbp, cc(2), bt, pmapb);
-                                fg_issue<NBA, FUSED_MAXW, true>(s1, rowI_byte, cc(1), bt, abyte, nullptr, rc);
-                                fg_store_mapv(s1, npair, bt, mbuf(1));
-                        }
+                        // loads left in flight here, oldest first: c of chunk 1 -- the
+                        // order the loop body leaves them in, so that the loop head sees
+                        // one order and every wait stays partial
+                        fg_issue<NBA, FUSED_MAXW, false, true>(s0, rowI_byte, cc(0), bt, abyte, mbuf(0), rc, fbr, nbp);
+                        fg_issue<NBA, FUSED_MAXW, false, true>(s1, rowI_byte, cc(1), bt, abyte, mbuf(1), rc, fbr, nbp);
                         if constexpr (PRESPLIT)
-                                fg_commit_split(s0, npair, bt, reinterpret_cast<__bf16 *>(buf0));
+                                add_diag(fg_commit_split<NBA, FUSED_MAXW, true>(s0, npair, bt,
+                                                                                reinterpret_cast<__bf16 *>(buf0)),
+                                         cc(0) * NB);
                         else
-                                fg_commit(s0, npair, bt, buf0);
+                                add_diag(fg_commit<NBA, FUSED_MAXW, true>(s0, npair, bt, buf0), cc(0) * NB);
                         __syncthreads();   // (4)
-                        // phase j: `issue` holds the map entries of chunk j+2 (loaded
-                        // one phase earlier) and issues its c loads; `commit` holds
-                        // the c values of chunk j+1 (issued one phase earlier),
-                        // committed now, and loads the map entries of chunk j+3.  The
-                        // loads of this phase stay in flight across the barrier.
+                        // phase j: `issue` issues the c loads of chunk j+2 (its map
+                        // slice is in LDS); `commit` holds the c values of chunk j+1
+                        // (issued one phase earlier), committed now.  The loads of
+                        // this phase stay in flight across the barrier.
                         auto phase = [&](int64_t j, FusedGather<NBA> &issue, FusedGather<NBA> &commit) {
-                                if constexpr (MAPWAVE) {
-                                        fg_issue<NBA, FUSED_MAXW, false>(issue, rowI_byte, cc(j + 2), bt, abyte,
-                                                                          mbuf(j + 2), rc);
-                                } else {
-                                        fg_issue_mapv(commit, npair, nbp, cc(j + 3), bt, pmapb);
-                                        fg_issue<NBA, FUSED_MAXW, true>(issue, rowI_byte, cc(j + 2), bt, abyte,
-                                                                         nullptr, rc);
-                                }
-                                // unconditional: past the row's last chunk the commit and
-                                // the map store go to the B buffer / map slice that no
-                                // wavefront reads in this phase (clamped chunk data).
-                                // Every path to the loop head then has the same loads in
-                                // flight, and the commits wait for their own chunk only.
+                                fg_issue<NBA, FUSED_MAXW, false, true>(issue, rowI_byte, cc(j + 2), bt, abyte,
+                                                                        mbuf(j + 2), rc, fbr, nbp);
+                                // unconditional: past the row's last chunk the commit goes
+                                // to the B buffer that no wavefront reads in this phase
+                                // (clamped chunk data); its diagonal term is dropped
+                                double d;
                                 if constexpr (PRESPLIT)
-                                        fg_commit_split(commit, npair, bt,
-                                                        reinterpret_cast<__bf16 *>(((j + 1) & 1) ? buf1 : buf0));
+                                        d = fg_commit_split<NBA, FUSED_MAXW, true>(
+                                                commit, npair, bt,
+                                                reinterpret_cast<__bf16 *>(((j + 1) & 1) ? buf1 : buf0));
                                 else
-                                        fg_commit(commit, npair, bt, ((j + 1) & 1) ? buf1 : buf0);
-                                if constexpr (!MAPWAVE) fg_store_mapv(issue, npair, bt, mbuf(j + 2));
+                                        d = fg_commit<NBA, FUSED_MAXW, true>(commit, npair, bt,
+                                                                             ((j + 1) & 1) ? buf1 : buf0);
+                                if (j + 1 < nl) add_diag(d, cc(j + 1) * NB);
                                 __syncthreads();
                         };
                         int64_t j = 0;
@@ -543,19 +532,34 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                 return;
         }
 
-        // ---- MFMA wavefronts: this wavefront's 16-row tile of h in registers
+        // ---- MFMA wavefronts: this wavefront's 16-row tile of the fused h
+        // (rows: the off-diagonal pairs mpair[m], then row mrow_f whose A
+        // entries are fAlpha(I), set per alpha string) in registers
         typename Ty::A ah[KS];
         typename Ty::A al[PREC == FCIS_PREC_BF16X3 ? KS : 1];
 #pragma unroll
         for (int ks = 0; ks < KS; ks++) {
                 // zero past the problem's k-steps: mma_steps runs all KS
                 const size_t o = ((size_t)ks * mtiles + wave) * 64 + lane;
-                ah[ks] = ks < ksteps ? ((const typename Ty::A *)hfrag_v)[o] : typename Ty::A{};
+                const int kstep = PREC == FCIS_PREC_BF16X3 ? 32 : 4;
+                const bool in = ks * kstep < npair;
+                ah[ks] = in ? ((const typename Ty::A *)hfrag_v)[o] : typename Ty::A{};
                 if constexpr (PREC == FCIS_PREC_BF16X3)
-                        al[ks] = ks < ksteps ? ((const typename Ty::A *)hfrag_lo_v)[o] : typename Ty::A{};
+                        al[ks] = in ? ((const typename Ty::A *)hfrag_lo_v)[o] : typename Ty::A{};
         }
+        // this lane's four output rows (f64 MFMA: D row lk + 4q; f32/bf16 MFMA:
+        // D row 4lk + q) and their pairs (-1: the diagonal row or padding)
+        int qpair[4];
+        bool qdiag[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+                const int m = wave * 16 + (PREC == FCIS_PREC_FP64 ? lk + 4 * q : 4 * lk + q);
+                qpair[q] = m < mrow_f ? (int)mpair[m] : -1;
+                qdiag[q] = m == mrow_f;
+        }
+        const bool frow = wave == mrow_f / 16 && li == mrow_f % 16;   // holds fAlpha(I)
         for (int64_t r = blockIdx.x; r < ta; r += gridDim.x) {
-                const int64_t I = ROWS ? (int64_t)rows[a0 + r] : a0 + (rows != nullptr ? (int64_t)rows[r] : r);
+                const int64_t I = row_of(r);
                 int64_t nl = nch;
                 int64_t c0 = 0;
                 if constexpr (ROWS) {
@@ -564,35 +568,51 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
                         nl = e.y;
                 }
                 __syncthreads();   // (1)
+                if (frow) {
+                        // A row mrow_f = fAlpha(I): A[i][k] at lane i + 16 (k % 4) for
+                        // k-step k / 4 (f64 / f32), lane i + 16 (k / 8 % 4) element k % 8
+                        // for k-step k / 32 (bf16)
+                        const double *fa = falpha + I * npair;
+#pragma unroll
+                        for (int ks = 0; ks < KS; ks++) {
+                                if constexpr (PREC == FCIS_PREC_BF16X3) {
+#pragma unroll
+                                        for (int e = 0; e < 8; e++) {
+                                                const int k = 32 * ks + 8 * lk + e;
+                                                const float x = k < npair ? (float)fa[k] : 0.0f;
+                                                const __bf16 hi = (__bf16)x;
+                                                ah[ks][e] = hi;
+                                                al[ks][e] = (__bf16)(x - (float)hi);
+                                        }
+                                } else {
+                                        const int k = 4 * ks + lk;
+                                        ah[ks] = k < npair ? (typename Ty::A)fa[k] : (typename Ty::A)0;
+                                }
+                        }
+                }
                 __syncthreads();   // (2)
                 __syncthreads();   // (3)
-                __syncthreads();   // (4)
-                // this lane's four output rows (f64 MFMA: D row lk + 4q; f32/bf16
-                // MFMA: D row 4lk + q) and their alpha slots, for the whole row
                 int slot[4];
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                        const int pq = wave * 16 + (PREC == FCIS_PREC_FP64 ? lk + 4 * q : 4 * lk + q);
-                        slot[q] = pq < npair ? aslot[pq] : -1;
-                }
+                for (int q = 0; q < 4; q++) slot[q] = qpair[q] >= 0 ? aslot[qpair[q]] : -1;
+                __syncthreads();   // (4)
                 typename Ty::Out *wrow = walpha + (ROWS ? r : I - a0) * nbp;
                 for (int64_t j = 0; j < nl; j++) {
                         const BT *bb = (j & 1) ? buf1 : buf0;
                         const int16_t *mb = maps + (j % FUSED_NMAP) * MAXP * NB;
-                        // the beta-scatter map entries, read before the MFMA chain so
-                        // that the scatter does not wait on LDS
+                        const int64_t b = (ROWS ? list_chunk(c0, j) : j) * NB + li;
+                        // the beta-scatter targets, read before the MFMA chain so that
+                        // the scatter does not wait on LDS (the diagonal row adds to
+                        // its own determinant)
                         int em[4];
 #pragma unroll
-                        for (int q = 0; q < 4; q++) {
-                                const int pq = wave * 16 + (PREC == FCIS_PREC_FP64 ? lk + 4 * q : 4 * lk + q);
-                                em[q] = pq < npair ? (int)mb[pq * NB + li] : -1;
-                        }
+                        for (int q = 0; q < 4; q++)
+                                em[q] = qpair[q] >= 0 ? (int)mb[qpair[q] * NB + li] : (qdiag[q] ? (int)(b << 1) : -1);
                         typename Ty::Acc acc;
                         if constexpr (PRESPLIT)
                                 acc = mma_presplit<KS>(ah, al, reinterpret_cast<const __bf16 *>(bb), lk, li);
                         else
                                 acc = mma_steps<PREC, KS, KS>(ah, al, bb, lk, li);
-                        const int64_t b = (ROWS ? list_chunk(c0, j) : j) * NB + li;
                         if (b < nb) {
 #pragma unroll
                                 for (int q = 0; q < 4; q++) {
@@ -613,14 +633,15 @@ __global__ __launch_bounds__(64 * (FUSED_MAXW + FusedTypes<PREC>::NBLD)) void k_
 }
 
 // Alpha pull for one alpha tile [a0, a0+ta): one thread per destination
-// determinant (J, Jb), adding sum_k s_k walpha[rslot_k][I_k - a0][Jb] over
-// the alpha links k of J whose target I_k lies in the tile; the reverse
-// slot rslot_k is the slot of the link I_k -> J in I_k's list.  walpha rows
+// determinant (J, Jb), adding sum_k s_k walpha[rslot_k - kfirst][I_k - a0][Jb]
+// over the off-diagonal alpha links k >= kfirst (= na; the diagonal links
+// come first) of J whose target I_k lies in the tile; the reverse slot
+// rslot_k is the slot of the link I_k -> J in I_k's list.  walpha rows
 // are wrs (nb rounded up to whole 16-determinant chunks) apart, so that the
 // fused kernel's 16-wide stores fill whole 128-byte lines.
 template <typename T>
 __global__ __launch_bounds__(256) void k_alpha_pull(int64_t nb, int64_t wrs, int64_t nstra, int64_t a0,
-                                                    int64_t ta, int64_t ld, int nlinka,
+                                                    int64_t ta, int64_t ld, int kfirst, int nlinka,
                                                     const fcis_link *__restrict__ linka,
                                                     const T *__restrict__ walpha,
                                                     double *__restrict__ sigma)
@@ -642,7 +663,7 @@ __global__ __launch_bounds__(256) void k_alpha_pull(int64_t nb, int64_t wrs, int
         // run in link order either way
         constexpr int U = 12;
         double acc0 = 0.0, acc1 = 0.0;
-        int k = 0;
+        int k = kfirst;
         for (; k + U <= nlinka; k += U) {
                 T2 v[U];
                 bool in[U], neg[U];
@@ -653,7 +674,8 @@ __global__ __launch_bounds__(256) void k_alpha_pull(int64_t nb, int64_t wrs, int
                         in[u] = I >= a0 && I < a0 + ta;
                         neg[u] = link_neg(L.info);
                         v[u] = *reinterpret_cast<const T2 *>(
-                                walpha + (in[u] ? (int64_t)link_rslot(L.info) * ld + (I - a0) * wrs + Jb : Jb));
+                                walpha + (in[u] ? (int64_t)(link_rslot(L.info) - kfirst) * ld + (I - a0) * wrs + Jb
+                                                : Jb));
                 }
 #pragma unroll
                 for (int u = 0; u < U; u++) {
@@ -668,7 +690,7 @@ __global__ __launch_bounds__(256) void k_alpha_pull(int64_t nb, int64_t wrs, int
                 const fcis_link L = lks[k];
                 const int64_t I = L.addr;
                 if (I < a0 || I >= a0 + ta) continue;
-                const T2 v = *reinterpret_cast<const T2 *>(walpha + (int64_t)link_rslot(L.info) * ld +
+                const T2 v = *reinterpret_cast<const T2 *>(walpha + (int64_t)(link_rslot(L.info) - kfirst) * ld +
                                                            (I - a0) * wrs + Jb);
                 const double x0 = (double)v.x, x1 = (double)v.y;
                 acc0 += link_neg(L.info) ? -x0 : x0;
@@ -806,47 +828,99 @@ size_t frag_count(const fcis_sigma_plan *p)
         return (size_t)p->ksteps * p->mtiles * 64;
 }
 
-int upload_h(fcis_sigma_plan *p, const double *h2e)
+// A fragments of the MFMA tiles whose rows are the pairs rowpair[m] (-1: a
+// zero row), laid out so that each fragment load is one contiguous read:
+//   FP64 / FP32: frag[(ks * mtiles + mt) * 64 + lane]
+//                  = h[rowpair[mt*16 + lane%16]][4 ks + lane/16]
+//   bf16x3:      hi / lo[((ks * mtiles + mt) * 64 + lane) * 8 + e]
+//                  = h[rowpair[mt*16 + lane%16]][32 ks + 8 (lane/16) + e]
+int upload_frags(const fcis_sigma_plan *p, const double *h2e, const std::vector<int> &rowpair, int mtiles,
+                 void *dst, void *dst_lo)
 {
         const int np = p->npair;
-        p->h.assign(h2e, h2e + (size_t)np * np);
-        const size_t nf = frag_count(p);
+        const size_t nf = (size_t)p->ksteps * mtiles * 64;
+        auto hval = [&](int m, int k) -> double {
+                const int row = m < (int)rowpair.size() ? rowpair[m] : -1;
+                return (row >= 0 && k < np) ? h2e[(size_t)row * np + k] : 0.0;
+        };
         if (p->precision == FCIS_PREC_BF16X3) {
-                // 8 consecutive k per lane; k-step of 32
                 std::vector<uint16_t> hi(nf * 8), lo(nf * 8);
                 for (int ks = 0; ks < p->ksteps; ks++)
-                        for (int mt = 0; mt < p->mtiles; mt++)
+                        for (int mt = 0; mt < mtiles; mt++)
                                 for (int l = 0; l < 64; l++)
                                         for (int e = 0; e < 8; e++) {
-                                                int i = mt * 16 + (l & 15);
-                                                int k = 32 * ks + 8 * (l >> 4) + e;
-                                                float x = (i < np && k < np)
-                                                                  ? (float)h2e[(size_t)i * np + k]
-                                                                  : 0.0f;
-                                                uint16_t xh = bf16_bits(x);
-                                                size_t o = (((size_t)ks * p->mtiles + mt) * 64 + l) * 8 + e;
+                                                const float x = (float)hval(mt * 16 + (l & 15), 32 * ks + 8 * (l >> 4) + e);
+                                                const uint16_t xh = bf16_bits(x);
+                                                const size_t o = (((size_t)ks * mtiles + mt) * 64 + l) * 8 + e;
                                                 hi[o] = xh;
                                                 lo[o] = bf16_bits(x - bf16_value(xh));
                                         }
-                FCIS_HIP(hipMemcpy(p->d_hfrag, hi.data(), hi.size() * 2, hipMemcpyHostToDevice));
-                FCIS_HIP(hipMemcpy(p->d_hfrag_lo, lo.data(), lo.size() * 2, hipMemcpyHostToDevice));
+                FCIS_HIP(hipMemcpy(dst, hi.data(), hi.size() * 2, hipMemcpyHostToDevice));
+                FCIS_HIP(hipMemcpy(dst_lo, lo.data(), lo.size() * 2, hipMemcpyHostToDevice));
                 return FCIS_SUCCESS;
         }
         std::vector<double> frag(nf, 0.0);
         for (int ks = 0; ks < p->ksteps; ks++)
-                for (int mt = 0; mt < p->mtiles; mt++)
-                        for (int l = 0; l < 64; l++) {
-                                int i = mt * 16 + (l & 15), k = 4 * ks + (l >> 4);
-                                if (i < np && k < np)
-                                        frag[((size_t)ks * p->mtiles + mt) * 64 + l] =
-                                                h2e[(size_t)i * np + k];
-                        }
+                for (int mt = 0; mt < mtiles; mt++)
+                        for (int l = 0; l < 64; l++)
+                                frag[((size_t)ks * mtiles + mt) * 64 + l] = hval(mt * 16 + (l & 15), 4 * ks + (l >> 4));
         if (p->precision == FCIS_PREC_FP64) {
-                FCIS_HIP(hipMemcpy(p->d_hfrag, frag.data(), nf * 8, hipMemcpyHostToDevice));
+                FCIS_HIP(hipMemcpy(dst, frag.data(), nf * 8, hipMemcpyHostToDevice));
         } else {
                 std::vector<float> f(frag.begin(), frag.end());
-                FCIS_HIP(hipMemcpy(p->d_hfrag, f.data(), nf * 4, hipMemcpyHostToDevice));
+                FCIS_HIP(hipMemcpy(dst, f.data(), nf * 4, hipMemcpyHostToDevice));
         }
+        return FCIS_SUCCESS;
+}
+
+// The off-diagonal pairs in pair order: the rows of the fused kernel's h.
+std::vector<int> offdiag_pairs(int norb)
+{
+        std::vector<int> out;
+        for (int pp = 0; pp < norb; pp++)
+                for (int q = 0; q < pp; q++) out.push_back(fcis::pair_index(pp, q));
+        std::sort(out.begin(), out.end());
+        return out;
+}
+
+int upload_h(fcis_sigma_plan *p, const double *h2e)
+{
+        const int np = p->npair;
+        p->h.assign(h2e, h2e + (size_t)np * np);
+        std::vector<int> natural(p->mtiles * 16);
+        for (int m = 0; m < (int)natural.size(); m++) natural[m] = m < np ? m : -1;
+        int st = upload_frags(p, h2e, natural, p->mtiles, p->d_hfrag, p->d_hfrag_lo);
+        if (st != FCIS_SUCCESS || !p->fused) return st;
+        // fused kernel: the off-diagonal rows; the diagonal pairs pp act on the
+        // determinant itself, sigma[D] += sum_p occ_p(D) vt1[pp][D] =
+        // sum_k (fAlpha[I][k] + fBeta[k][Ib]) t1[k][D], fAlpha / fBeta the
+        // sums of h[pp][k] over the occupied orbitals of the alpha / beta string
+        std::vector<int> rows = offdiag_pairs(p->space->norb);
+        st = upload_frags(p, h2e, rows, p->mtiles_f, p->d_hfrag_f, p->d_hfrag_f_lo);
+        if (st != FCIS_SUCCESS) return st;
+        const fcis_space *s = p->space;
+        const int n = s->norb;
+        std::vector<double> fa((size_t)s->nstra * np), fb((size_t)np * p->nbp, 0.0);
+        for (int64_t a = 0; a < s->nstra; a++) {
+                const uint64_t str = fcis::space_string(s, 0, a);
+                for (int k = 0; k < np; k++) {
+                        double v = 0.0;
+                        for (int q = 0; q < n; q++)
+                                if ((str >> q) & 1) v += h2e[(size_t)fcis::pair_index(q, q) * np + k];
+                        fa[(size_t)a * np + k] = v;
+                }
+        }
+        for (int64_t b = 0; b < s->nstrb; b++) {
+                const uint64_t str = fcis::space_string(s, 1, b);
+                for (int k = 0; k < np; k++) {
+                        double v = 0.0;
+                        for (int q = 0; q < n; q++)
+                                if ((str >> q) & 1) v += h2e[(size_t)fcis::pair_index(q, q) * np + k];
+                        fb[(size_t)k * p->nbp + b] = v;
+                }
+        }
+        FCIS_HIP(hipMemcpy(p->d_falpha, fa.data(), 8 * fa.size(), hipMemcpyHostToDevice));
+        FCIS_HIP(hipMemcpy(p->d_fbeta, fb.data(), 8 * fb.size(), hipMemcpyHostToDevice));
         return FCIS_SUCCESS;
 }
 
@@ -968,15 +1042,16 @@ void launch_fused(const fcis_sigma_plan *p, hipStream_t st, int64_t a0, int64_t 
 {
         const fcis_space *s = p->space;
         const int64_t nb = s->nstrb, na = s->nstra;
-        const dim3 fgrid(p->fused_grid), fblk(64 * (p->mtiles + p->nbld));
+        const dim3 fgrid(p->fused_grid), fblk(64 * (p->mtiles_f + p->nbld));
         const int32_t *rows = ROWS ? rs->d_rows : p->d_order;
         const int2 *rowcl = ROWS ? rs->d_rowcl : nullptr;
         const uint16_t *clist = ROWS ? rs->d_clist : nullptr;
-#define FCIS_LAUNCH_FUSED(PR, OUT)                                                                  \
-        hipLaunchKernelGGL((k_fused_row<PR, ROWS>), fgrid, fblk, p->fused_lds, st, p->npair,    \
-                           p->mtiles, p->ksteps, s->nlinka, nb, na, a0, ta, lda, acc, s->d_linka,   \
-                           p->d_pmapa, p->d_pmapb16, p->nbp, ci0, p->d_hfrag, p->d_hfrag_lo,        \
-                           (OUT *)p->d_vt1, sig, rows, rowcl, clist)
+#define FCIS_LAUNCH_FUSED(PR, OUT)                                                                    \
+        hipLaunchKernelGGL((k_fused_row<PR, ROWS>), fgrid, fblk, p->fused_lds, st, p->npair,      \
+                           p->mtiles_f, p->mrow_f, s->na, s->nlinka, nb, na, a0, ta, lda, acc,        \
+                           s->d_linka, p->d_pmapa, p->d_pmapb16, p->nbp, ci0, p->d_hfrag_f,           \
+                           p->d_hfrag_f_lo, p->d_mpair, p->d_falpha, p->d_fbeta, (OUT *)p->d_vt1,     \
+                           sig, rows, rowcl, clist)
         if (p->precision == FCIS_PREC_FP64)
                 FCIS_LAUNCH_FUSED(FCIS_PREC_FP64, double);
         else if (p->precision == FCIS_PREC_FP32)
@@ -1003,7 +1078,7 @@ const void *fused_kernel(int precision)
 template <typename T>
 __global__ __launch_bounds__(256) void k_alpha_pull_rows(
         int64_t ndets, const uint32_t *__restrict__ dets, int64_t nb, int64_t wrs, int64_t nstra, int64_t r0,
-        int64_t tr, int64_t ld, int nlinka, const fcis_link *__restrict__ linka,
+        int64_t tr, int64_t ld, int kfirst, int nlinka, const fcis_link *__restrict__ linka,
         const int32_t *__restrict__ rpos, const T *__restrict__ walpha, double *__restrict__ sigma,
         double *__restrict__ out)
 {
@@ -1015,7 +1090,7 @@ __global__ __launch_bounds__(256) void k_alpha_pull_rows(
                 // time; the sum runs in link order
                 constexpr int U = 8;
                 double acc = 0.0;
-                int k = 0;
+                int k = kfirst;
                 for (; k + U <= nlinka; k += U) {
                         fcis_link L[U];
                         int32_t pos[U];
@@ -1027,7 +1102,8 @@ __global__ __launch_bounds__(256) void k_alpha_pull_rows(
 #pragma unroll
                         for (int u = 0; u < U; u++) {
                                 const bool in = pos[u] >= r0 && pos[u] < r0 + tr;
-                                v[u] = walpha[in ? (int64_t)link_rslot(L[u].info) * ld + (pos[u] - r0) * wrs + Jb
+                                v[u] = walpha[in ? (int64_t)(link_rslot(L[u].info) - kfirst) * ld +
+                                                           (pos[u] - r0) * wrs + Jb
                                                  : Jb];
                         }
 #pragma unroll
@@ -1042,7 +1118,7 @@ __global__ __launch_bounds__(256) void k_alpha_pull_rows(
                         const fcis_link L = linka[(int64_t)k * nstra + J];
                         const int64_t pos = rpos[L.addr];
                         if (pos < r0 || pos >= r0 + tr) continue;
-                        const double v = (double)walpha[(int64_t)link_rslot(L.info) * ld +
+                        const double v = (double)walpha[(int64_t)(link_rslot(L.info) - kfirst) * ld +
                                                         (pos - r0) * wrs + Jb];
                         acc += link_neg(L.info) ? -v : v;
                 }
@@ -1089,11 +1165,11 @@ int fcis::sigma_plan_execute_rows(fcis_sigma_plan *p, const fcis_rowset *rs, con
                 if (p->timing) FCIS_HIP(hipEventRecord(p->ev[2], st));
                 if (p->precision == FCIS_PREC_FP64)
                         hipLaunchKernelGGL(k_alpha_pull_rows<double>, dim3(pgrid), dim3(256), 0, st,
-                                           rs->ndets, rs->d_dets, nb, p->nbp, na, r0, tr, lda, s->nlinka,
+                                           rs->ndets, rs->d_dets, nb, p->nbp, na, r0, tr, lda, s->na, s->nlinka,
                                            s->d_linka, rs->d_rpos, (const double *)p->d_vt1, sig, o);
                 else
                         hipLaunchKernelGGL(k_alpha_pull_rows<float>, dim3(pgrid), dim3(256), 0, st,
-                                           rs->ndets, rs->d_dets, nb, p->nbp, na, r0, tr, lda, s->nlinka,
+                                           rs->ndets, rs->d_dets, nb, p->nbp, na, r0, tr, lda, s->na, s->nlinka,
                                            s->d_linka, rs->d_rpos, (const float *)p->d_vt1, sig, o);
                 FCIS_CHECK_LAUNCH();
                 if (p->timing) {
@@ -1141,10 +1217,13 @@ int fcis::sigma_plan_create(fcis_sigma_plan **out, const fcis_space *space, cons
         const size_t lds = precision == FCIS_PREC_FP64 ? fused_lds_bytes<FCIS_PREC_FP64>(nb)
                          : precision == FCIS_PREC_FP32 ? fused_lds_bytes<FCIS_PREC_FP32>(nb)
                                                         : fused_lds_bytes<FCIS_PREC_BF16X3>(nb);
-        p->fused = p->mtiles <= FUSED_MAXW &&
+        // fused rows: the npair - norb off-diagonal pairs and the diagonal row
+        p->mrow_f = space->npair - space->norb;
+        p->mtiles_f = (p->mrow_f + 1 + 15) / 16;
+        p->fused = p->mtiles <= FUSED_MAXW && p->mtiles_f <= FUSED_MTILES &&
                    p->ksteps <= (precision == FCIS_PREC_BF16X3 ? FUSED_KS_BF16 : FUSED_KS) &&
                    nb < FUSED_MAXNB && space->nlinka <= 255 && lds <= 160 * 1024 &&
-                   (uint64_t)space->nstra * (uint64_t)nb * 8ull < (1ull << 32) &&
+                   (uint64_t)space->nstra * (uint64_t)nb * 8ull < (uint64_t)FUSED_OOB &&
                    !(honour_env && getenv("FCIS_DETERMINISTIC"));
         if (p->fused) {
                 int dev = 0, ncu = 0, occ = 0;
@@ -1154,18 +1233,17 @@ int fcis::sigma_plan_create(fcis_sigma_plan **out, const fcis_space *space, cons
                 const void *fkr = fused_kernel<true>(precision);
                 FCIS_HIP(hipFuncSetAttribute(fk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
                 FCIS_HIP(hipFuncSetAttribute(fkr, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-                p->nbld = precision == FCIS_PREC_FP64 ? FusedTypes<FCIS_PREC_FP64>::NBLD
-                        : precision == FCIS_PREC_FP32 ? FusedTypes<FCIS_PREC_FP32>::NBLD
-                                                       : FusedTypes<FCIS_PREC_BF16X3>::NBLD;
+                p->nbld = FUSED_NBLD;
                 p->nbp = (nb + FUSED_NB - 1) / FUSED_NB * FUSED_NB;
-                FCIS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fk, 64 * (p->mtiles + p->nbld), lds));
+                FCIS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fk, 64 * (p->mtiles_f + p->nbld), lds));
                 if (occ < 1) occ = 1;
                 p->fused_grid = ncu * occ;
                 p->fused_lds = lds;
         }
-        // workspace per alpha string: walpha (nlinka rows, fused path) or
-        // t1 + vt1 (npair rows each)
-        const uint64_t rows_per_alpha = p->fused ? (uint64_t)space->nlinka : 2ull * p->npair;
+        // workspace per alpha string: walpha (the nlinka - na off-diagonal
+        // alpha links, fused path) or t1 + vt1 (npair rows each)
+        const int nwslot = space->nlinka - space->na;
+        const uint64_t rows_per_alpha = p->fused ? (uint64_t)nwslot : 2ull * p->npair;
         const int64_t wrs = p->fused ? p->nbp : nb;   // walpha rows padded to whole chunks
         uint64_t per_alpha = rows_per_alpha * (uint64_t)wrs * elem;
         int64_t ta = per_alpha ? (int64_t)(workspace_bytes / per_alpha) : space->nstra;
@@ -1173,10 +1251,12 @@ int fcis::sigma_plan_create(fcis_sigma_plan **out, const fcis_space *space, cons
         if (ta > space->nstra) ta = space->nstra;
         p->tile_a = ta;
         p->ntiles = (int)((space->nstra + ta - 1) / ta);
-        size_t buf = (size_t)(p->fused ? space->nlinka : p->npair) * ta * wrs * elem;
+        size_t buf = (size_t)(p->fused ? nwslot : p->npair) * ta * wrs * elem;
         p->ws_bytes = p->fused ? buf : 2 * buf;
         const size_t nf = frag_count(p);
         size_t hbytes = precision == FCIS_PREC_BF16X3 ? nf * 16 : nf * elem;
+        const size_t nff = (size_t)p->ksteps * p->mtiles_f * 64;
+        const size_t hfbytes = precision == FCIS_PREC_BF16X3 ? nff * 16 : nff * elem;
         size_t pa = (size_t)p->npair * space->nstra * 4, pb = (size_t)p->npair * nb * 4;
         if (fcis::dev_malloc(&p->d_hfrag, hbytes) != hipSuccess ||
             fcis::dev_malloc(&p->d_hfrag_lo, precision == FCIS_PREC_BF16X3 ? hbytes : 16) != hipSuccess ||
@@ -1184,11 +1264,23 @@ int fcis::sigma_plan_create(fcis_sigma_plan **out, const fcis_space *space, cons
             fcis::dev_malloc(&p->d_pmapb, pb ? pb : 4) != hipSuccess ||
             fcis::dev_malloc(&p->d_pmapb16, p->fused ? (size_t)p->npair * p->nbp * 2 + 16 : 16) != hipSuccess ||
             fcis::dev_malloc(&p->d_t1, (buf && !p->fused) ? buf : 8) != hipSuccess ||
-            fcis::dev_malloc(&p->d_vt1, buf ? buf : 8) != hipSuccess) {
+            fcis::dev_malloc(&p->d_vt1, buf ? buf : 8) != hipSuccess ||
+            (p->fused && (fcis::dev_malloc(&p->d_hfrag_f, hfbytes) != hipSuccess ||
+                          fcis::dev_malloc(&p->d_hfrag_f_lo, precision == FCIS_PREC_BF16X3 ? hfbytes : 16) !=
+                                  hipSuccess ||
+                          fcis::dev_malloc(&p->d_mpair, 2 * 16 * FUSED_MTILES) != hipSuccess ||
+                          fcis::dev_malloc(&p->d_falpha, 8 * (size_t)space->nstra * p->npair) != hipSuccess ||
+                          fcis::dev_malloc(&p->d_fbeta, 8 * (size_t)p->npair * p->nbp) != hipSuccess))) {
                 fcis_sigma_plan_free(p);
                 fcis::set_error("fcis_sigma_plan_create: device allocation of %zu bytes failed",
                                 2 * buf + 2 * hbytes + pa + pb);
                 return FCIS_ERR_MEMORY;
+        }
+        if (p->fused) {
+                std::vector<int> rows = offdiag_pairs(space->norb);
+                std::vector<int16_t> mp(16 * FUSED_MTILES, -1);
+                for (size_t m = 0; m < rows.size(); m++) mp[m] = (int16_t)rows[m];
+                FCIS_HIP(hipMemcpy(p->d_mpair, mp.data(), 2 * mp.size(), hipMemcpyHostToDevice));
         }
         int st = upload_h(p, h2e);
         if (st != FCIS_SUCCESS) {
@@ -1292,11 +1384,11 @@ int fcis_sigma_plan_execute(fcis_sigma_plan_t *p, const double *ci0, double *ci1
                         const dim3 pgrid(na * (((nb + 1) / 2 + 255) / 256));   // two determinants per thread
                         if (p->precision == FCIS_PREC_FP64)
                                 hipLaunchKernelGGL(k_alpha_pull<double>, pgrid, dim3(256), 0, st, nb, p->nbp, na, a0,
-                                                   ta, lda, s->nlinka, s->d_linka,
+                                                   ta, lda, s->na, s->nlinka, s->d_linka,
                                                    (const double *)p->d_vt1, ci1);
                         else
                                 hipLaunchKernelGGL(k_alpha_pull<float>, pgrid, dim3(256), 0, st, nb, p->nbp, na, a0,
-                                                   ta, lda, s->nlinka, s->d_linka,
+                                                   ta, lda, s->na, s->nlinka, s->d_linka,
                                                    (const float *)p->d_vt1, ci1);
                 } else if (p->precision == FCIS_PREC_FP64) {
                         double *t1 = (double *)p->d_t1, *vt1 = (double *)p->d_vt1;
@@ -1382,6 +1474,11 @@ void fcis_sigma_plan_free(fcis_sigma_plan_t *p)
         (void)hipFree(p->d_t1);
         (void)hipFree(p->d_vt1);
         (void)hipFree(p->d_order);
+        (void)hipFree(p->d_hfrag_f);
+        (void)hipFree(p->d_hfrag_f_lo);
+        (void)hipFree(p->d_mpair);
+        (void)hipFree(p->d_falpha);
+        (void)hipFree(p->d_fbeta);
         delete p;
 }
 
