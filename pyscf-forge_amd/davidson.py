@@ -77,16 +77,39 @@ class DeviceOps:
 
 def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
              max_cycle=50, max_space=12, lindep=1e-14, verbose=False,
-             callback=None, ops=None):
+             callback=None, ops=None, hop_check=None, tol_low=None,
+             tol_residual_low=None, check_every=6):
     """Lowest ``nroots`` eigenpairs.  x0: list of device vectors (guesses).
 
     Returns DavidsonResult with e (numpy), x / ax (device tensors of shape
     (nroots, n)), per-root convergence flags, cycle and H-application counts.
+
+    Mixed precision: with ``hop_check`` (the exact operator; ``hop`` is then
+    an approximate, cheaper one) every new subspace vector is applied with
+    ``hop``.  The first stage runs to (tol_low, tol_residual_low).  Then, and
+    whenever the iterations meet (tol, tol_residual) or ``check_every``
+    iterations have passed, the Ritz vectors are applied with ``hop_check``:
+    they are accepted if their exact residuals and the change of their exact
+    Rayleigh quotients from the iteration's Ritz values meet (tol,
+    tol_residual); otherwise the subspace restarts from them with their exact
+    images (plus the previous Ritz directions) and goes on.  The subspace is
+    kept through the switch: later corrections enter the Ritz vectors with
+    coefficients of the size of the remaining error, so the approximate
+    images perturb the residual only in proportion to that error, and the
+    solve needs about two exact applications per root.  ``nhop`` counts
+    ``hop``, ``nhop_check`` ``hop_check``.
     """
     ops = ops if ops is not None else DeviceOps()
     n = int(x0[0].reshape(-1).shape[0])
     if tol_residual is None:
         tol_residual = math.sqrt(tol)
+    mixed = hop_check is not None
+    if mixed:
+        stage_tol = tol if tol_low is None else max(tol, tol_low)
+        stage_res = tol_residual if tol_residual_low is None else max(tol_residual, tol_residual_low)
+    else:
+        stage_tol, stage_res = tol, tol_residual
+    refining = False
     cap = int(max_space) + (nroots - 1) * 3
     cap = max(cap, nroots + len(x0))
     cap = min(cap, 64, n)
@@ -101,6 +124,12 @@ def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
     heff = numpy.zeros((cap, cap))
     space = 0
     nhop = 0
+    nhop_check = 0
+    last_check = 0
+    # X[:nexact] carry exact images (mixed precision, after the first check):
+    # heff[a, b] = X_a . AX_b, and the coupling of an exact vector i with an
+    # approximate j is taken from the exact image, heff[j, i]
+    nexact = 0
 
     def add_vector(t):
         """Orthonormalise t against X[:space] (Gram-Schmidt, twice) and
@@ -128,8 +157,25 @@ def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
         col = ops.dots(space + 1, n, X, AX[space])
         heff[:space + 1, space] = col
         heff[space, :space + 1] = col
+        if nexact:
+            heff[space, :nexact] = ops.dots(nexact, n, AX, X[space])
         space += 1
         return True
+
+    def subspace_matrix():
+        h = 0.5 * (heff[:space, :space] + heff[:space, :space].T)
+        if nexact:
+            h[nexact:, :nexact] = heff[nexact:space, :nexact]
+            h[:nexact, nexact:] = heff[nexact:space, :nexact].T
+        return h
+
+    def residual_norms(e):
+        """|AX v_k - e_k X v_k| of the Ritz pairs in xr / axr."""
+        out = numpy.empty(nroots)
+        for k in range(nroots):
+            ops.combine(n, (axr[k], xr[k]), numpy.array([1.0, -e[k]]), r)
+            out[k] = math.sqrt(max(ops.dots(1, n, r, r)[0], 0.0))
+        return out
 
     for g in x0:
         t = ops.clone(g)
@@ -144,7 +190,7 @@ def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
     cycles = 0
     for icyc in range(max_cycle):
         cycles = icyc + 1
-        h = 0.5 * (heff[:space, :space] + heff[:space, :space].T)
+        h = subspace_matrix()
         w, v = numpy.linalg.eigh(h)
         e = w[:nroots].copy()
         v = v[:, :nroots]
@@ -152,30 +198,47 @@ def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
             ops.lincomb(space, n, X, v[:, k], 0.0, xr[k])
             ops.lincomb(space, n, AX, v[:, k], 0.0, axr[k])
         de = e - e_old
+        rnorm[:] = residual_norms(e)
+        conv = (numpy.abs(de) < stage_tol) & (rnorm < stage_res)
+        checked = False
+        if mixed and (conv.all() or (refining and icyc - last_check >= check_every)
+                      or (not refining and icyc + 1 >= max_cycle // 2)):
+            # exact images of the Ritz vectors: the acceptance test, and the
+            # anchor of the subspace from here on
+            for k in range(nroots):
+                hop_check(xr[k], axr[k])
+            nhop_check += nroots
+            e_chk = numpy.array([ops.dots(1, n, xr[k], axr[k])[0] for k in range(nroots)])
+            rnorm[:] = residual_norms(e_chk)
+            conv = (numpy.abs(e_chk - e) < tol) & (rnorm < tol_residual) if refining \
+                else numpy.zeros(nroots, dtype=bool)
+            de = e_chk - e_old
+            e = e_chk
+            refining, checked, last_check = True, True, icyc
+            stage_tol, stage_res = tol, tol_residual
         new_vectors = []
         for k in range(nroots):
-            ops.combine(n, (axr[k], xr[k]), numpy.array([1.0, -e[k]]), r)
-            rnorm[k] = math.sqrt(max(ops.dots(1, n, r, r)[0], 0.0))
-            conv[k] = abs(de[k]) < tol and rnorm[k] < tol_residual
             if not conv[k]:
+                ops.combine(n, (axr[k], xr[k]), numpy.array([1.0, -e[k]]), r)
                 t = ops.empty((n,), x0[0])
                 precond(r, e[k], t, xr[k])
                 new_vectors.append(t)
         if verbose:
-            print("davidson %d  space %d  e %s  de %s  |r| %s" % (
-                icyc, space, e, de, rnorm))
+            print("davidson %d  space %d  e %s  de %s  |r| %s%s" % (
+                icyc, space, e, de, rnorm, "  (checked)" if checked else ""))
         if callback is not None:
             callback(icyc, e, rnorm)
         e_old = e
-        if conv.all():
+        if conv.all() and (checked or not mixed):
             break
-        if space + len(new_vectors) > cap:
+        if checked or space + len(new_vectors) > cap:
             # Thick restart: keep the current Ritz vectors and the part of the
             # previous iteration's Ritz vectors orthogonal to them (the
             # locally optimal direction), formed in subspace coordinates so
-            # their images come from AX without new H applications.
+            # their images come from AX without new H applications.  After a
+            # check the Ritz vectors carry their exact images.
             keep = v.copy()
-            if v_prev is not None:
+            if v_prev is not None and v_prev.shape[0] <= space:
                 prev = numpy.zeros((space, v_prev.shape[1]))
                 prev[:v_prev.shape[0]] = v_prev
                 for k in range(prev.shape[1]):
@@ -196,7 +259,15 @@ def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
                 ops.copy(AX[nroots + k], axp[k])
             space = nk
             heff[:] = 0.0
-            heff[:nk, :nk] = 0.5 * (hk + hk.T)
+            if checked:
+                for j in range(nk):
+                    heff[:nk, j] = ops.dots(nk, n, X, AX[j])
+            else:
+                heff[:nk, :nk] = 0.5 * (hk + hk.T)
+            # after a check the kept Ritz vectors are the exact ones (later
+            # restarts keep Ritz vectors whose approximate part is of the
+            # size of the remaining error)
+            nexact = nroots if refining else 0
             v = numpy.eye(nk)[:, :nroots]
         v_prev = v
         added = 0
@@ -205,7 +276,9 @@ def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
                 added += 1
         if added == 0:
             break
-    return DavidsonResult(conv.copy(), e_old, xr, axr, cycles, nhop, rnorm.copy())
+    res = DavidsonResult(conv.copy(), e_old, xr, axr, cycles, nhop, rnorm.copy())
+    res.nhop_check = nhop_check
+    return res
 
 
 def make_pspace_precond(hdiag, pw, pv, addr, level_shift=1e-3, ops=None):

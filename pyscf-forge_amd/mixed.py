@@ -4,11 +4,13 @@ Mixed precision (precision "fp32" or "bf16x3"): the Davidson iterations use
 a sigma whose t1 contraction runs in FP32 MFMA or in a 3-term bf16 split
 (FP32 accumulation), while residuals, orthogonalisation and the subspace
 eigenproblem stay FP64.  When the low-precision residual reaches its noise
-floor the solve is continued from the current Ritz vectors with the FP64
-sigma (iterative refinement) until the FP64 criteria of the reference's
-Davidson call (tol on the energy change, sqrt(tol) on the residual,
-fci_gas.py:1344-1350) hold, so the returned energy and vectors satisfy the
-same convergence test as a plain FP64 solve.
+floor, the Ritz vectors are applied with the FP64 sigma and the same
+subspace goes on from them (iterative refinement inside one Davidson run,
+davidson.davidson with hop_check): new corrections are still applied in low
+precision, and every acceptance is decided on FP64 images against the FP64
+criteria of the reference's Davidson call (tol on the energy change,
+sqrt(tol) on the residual, fci_gas.py:1344-1350), so the returned energy and
+vectors satisfy the same convergence test as a plain FP64 solve.
 """
 
 import time
@@ -75,34 +77,40 @@ def solve(solver, sp, h2e, hdiag, nroots, ci0, precision, tol=1e-10, tol_residua
     free = torch.cuda.mem_get_info()[0]
     need = _davidson_bytes(n, nroots, max_space)
     if basis is not None:
-        need += 2 * sp.ndet * 8
+        # determinant-space buffers of each plan's basis operator
+        need += (2 if precision == "fp64" else 4) * sp.ndet * 8
     ws = int(solver.workspace_bytes) or max(int((free - need) * 0.85), 1 << 26)
     stats = {"precision": precision}
     verbose = bool(verbose and verbose > 4)
 
-    def run(prec, guesses, **kw):
+    if precision == "fp64":
         t0 = time.perf_counter()
-        with SigmaPlan(sp, h2e, prec, ws) as plan:
+        with SigmaPlan(sp, h2e, "fp64", ws) as plan:
             hop = plan if basis is None else basis.hop(plan)
             res = _dv.davidson(hop, precond, guesses, nroots=nroots, lindep=lindep,
-                               max_space=max_space, verbose=verbose, **kw)
-        torch.cuda.synchronize()
-        return res, time.perf_counter() - t0, plan
-
-    if precision == "fp64":
-        res, dt, plan = run("fp64", guesses, tol=tol, tol_residual=tol_residual,
-                            max_cycle=max_cycle)
-        stats.update(time=dt, nhop=res.nhop, cycles=res.cycles, ntiles=plan.ntiles)
+                               max_space=max_space, verbose=verbose, tol=tol,
+                               tol_residual=tol_residual, max_cycle=max_cycle)
+            torch.cuda.synchronize()
+            stats.update(time=time.perf_counter() - t0, nhop=res.nhop, cycles=res.cycles,
+                         ntiles=plan.ntiles)
         return res.e, list(res.x), res.converged, stats
 
     if precision not in LOW_STAGE:
         raise ValueError("unknown precision %r" % (precision,))
     lo = LOW_STAGE[precision]
-    res_lo, dt_lo, plan = run(precision, guesses, max_cycle=max_cycle, **lo)
-    stats.update(time_low=dt_lo, nhop_low=res_lo.nhop, cycles_low=res_lo.cycles,
-                 e_low=res_lo.e.tolist(), rnorm_low=res_lo.rnorm.tolist())
-    res, dt, plan = run("fp64", list(res_lo.x), tol=tol, tol_residual=tol_residual,
-                        max_cycle=max_cycle)
-    stats.update(time_refine=dt, nhop_refine=res.nhop, cycles_refine=res.cycles,
-                 time=dt_lo + dt, ntiles=plan.ntiles)
+    t0 = time.perf_counter()
+    # both plans live through the solve; the FP64 one (twice the walpha
+    # bytes) gets two thirds of the workspace budget
+    with SigmaPlan(sp, h2e, "fp64", max(ws * 2 // 3, 1 << 26)) as plan64, \
+            SigmaPlan(sp, h2e, precision, max(ws // 3, 1 << 26)) as plan_lo:
+        hop_lo = plan_lo if basis is None else basis.hop(plan_lo)
+        hop64 = plan64 if basis is None else basis.hop(plan64)
+        res = _dv.davidson(hop_lo, precond, guesses, nroots=nroots, lindep=lindep,
+                           max_space=max_space, verbose=verbose, tol=tol,
+                           tol_residual=tol_residual, max_cycle=max_cycle, hop_check=hop64,
+                           tol_low=lo["tol"], tol_residual_low=lo["tol_residual"])
+        torch.cuda.synchronize()
+        stats.update(time=time.perf_counter() - t0, nhop_low=res.nhop,
+                     nhop_fp64=res.nhop_check, cycles=res.cycles,
+                     ntiles=plan64.ntiles, ntiles_low=plan_lo.ntiles)
     return res.e, list(res.x), res.converged, stats

@@ -107,3 +107,61 @@ def test_davidson_linearly_dependent_guesses():
     with pytest.raises(RuntimeError):
         dav.davidson(hop, lambda r, e, out, x0: out.__setitem__(Ellipsis, r), [g, 2.0 * g],
                      nroots=2, ops=NumpyOps())
+
+
+def _rounded(x, bits):
+    """x with its mantissa rounded to ``bits`` bits (a nonlinear,
+    input-dependent error like that of a low-precision sigma)."""
+    m, ex = numpy.frexp(x)
+    return numpy.ldexp(numpy.round(m * 2.0 ** bits) / 2.0 ** bits, ex)
+
+
+@pytest.mark.parametrize("n,nroots,bits,seed", [(300, 1, 16, 11), (400, 3, 16, 12),
+                                                 (300, 2, 22, 13)])
+def test_mixed_refinement_keeps_the_subspace(n, nroots, bits, seed):
+    """Mixed precision inside one run: the cheap operator rounds its input
+    and its output to ``bits`` mantissa bits (16: about the bf16x3 sigma's
+    error, 22: about FP32's); the exact one is checked only at the
+    acceptance points.  The result meets the exact criteria (eigenvalues of
+    eigh to 1e-9, exact residual below sqrt(tol)) with a few exact
+    applications per root, fewer than a restart from the low-precision Ritz
+    vectors needs."""
+    H = _problem(n, seed)
+    hdiag = numpy.diag(H).copy()
+    nexact = [0]
+
+    def hop_lo(x, out):
+        out[...] = _rounded(H @ _rounded(x, bits), bits)
+
+    def hop(x, out):
+        nexact[0] += 1
+        out[...] = H @ x
+
+    def precond(r, e, out, x0):
+        dd = hdiag - e
+        dd[numpy.abs(dd) < 1e-8] = 1e-8
+        out[...] = r / dd
+
+    guesses = []
+    for k in numpy.argsort(hdiag)[:nroots]:
+        g = numpy.zeros(n)
+        g[k] = 1.0
+        guesses.append(g)
+    tol = 1e-12
+    res = dav.davidson(hop_lo, precond, guesses, nroots=nroots, tol=tol, max_cycle=200,
+                       max_space=12, ops=NumpyOps(), hop_check=hop, tol_low=1e-6,
+                       tol_residual_low=5e-4)
+    w = numpy.linalg.eigh(H)[0][:nroots]
+    assert res.converged.all()
+    numpy.testing.assert_allclose(res.e, w, rtol=0, atol=1e-9)
+    X = numpy.asarray(res.x).reshape(nroots, n)
+    r = X @ H.T - res.e[:, None] * X
+    assert numpy.linalg.norm(r, axis=1).max() < 1e-6
+    assert res.nhop_check == nexact[0]
+    # the restart alternative: low-precision Ritz vectors into a plain exact solve
+    lo = dav.davidson(hop_lo, precond, guesses, nroots=nroots, tol=1e-6, tol_residual=5e-4,
+                      max_cycle=200, max_space=12, ops=NumpyOps())
+    ref = dav.davidson(lambda x, out: out.__setitem__(Ellipsis, H @ x), precond, list(lo.x),
+                       nroots=nroots, tol=tol, max_cycle=200, max_space=12, ops=NumpyOps())
+    assert res.nhop_check <= 3 * nroots
+    assert res.nhop_check < ref.nhop

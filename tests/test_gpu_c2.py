@@ -181,10 +181,10 @@ def test_c2_mixed_precision_davidson_matches_fp64(c2):
         sm.max_cycle = 100
         em, cm = sm.kernel(h1e, eri, N, NE)
         print("C2 %s-mixed Davidson: E = %.12f, dE = %.2e, low %d + FP64 %d sigma"
-              % (prec, em, em - e64, sm.stats["nhop_low"], sm.stats["nhop_refine"]))
+              % (prec, em, em - e64, sm.stats["nhop_low"], sm.stats["nhop_fp64"]))
         assert sm.converged
         assert abs(em - e64) < 1e-8
-        assert sm.stats["nhop_low"] > 0 and sm.stats["nhop_refine"] > 0
+        assert sm.stats["nhop_low"] > 0 and sm.stats["nhop_fp64"] > 0
         ov = abs(float(numpy.dot(numpy.asarray(cm).reshape(-1), s64.ci.reshape(-1))))
         assert abs(ov - 1.0) < 1e-5
         del cm, sm

@@ -185,7 +185,7 @@ def test_mixed_precision_davidson_matches_fp64(precision):
     assert sm.converged and s64.converged
     assert abs(e64 - w[0]) < 1e-9
     assert abs(em - w[0]) < 1e-8
-    assert sm.stats["nhop_low"] > 0 and sm.stats["nhop_refine"] > 0
+    assert sm.stats["nhop_low"] > 0 and sm.stats["nhop_fp64"] > 0
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
