@@ -28,6 +28,7 @@ template <int PREC>
 struct FusedTypes;
 template <>
 struct FusedTypes<FCIS_PREC_FP64> {
+        typedef double G;    // element type the builders gather from c
         typedef double B;
         typedef double A;
         typedef v4d Acc;
@@ -38,6 +39,7 @@ struct FusedTypes<FCIS_PREC_FP64> {
 };
 template <>
 struct FusedTypes<FCIS_PREC_FP32> {
+        typedef float G;    // element type the builders gather from c
         typedef float B;
         typedef float A;
         typedef v4f Acc;
@@ -48,6 +50,7 @@ struct FusedTypes<FCIS_PREC_FP32> {
 };
 template <>
 struct FusedTypes<FCIS_PREC_BF16X3> {
+        typedef float G;    // element type the builders gather from c
         typedef float B;
         typedef bf16x8 A;
         typedef v4f Acc;
@@ -259,17 +262,25 @@ __device__ __forceinline__ void fb_commit_maps(FusedBuild<NBLD, MAXW> &st, int n
 // away.  A raw buffer load at a byte offset >= num_records returns 0; the
 // builders give every absent link the offset FUSED_OOB, which stays out of
 // range for any c of the fused path (below 4 GiB - 1 MiB) even after a
-// column offset (< 128 KiB) is added.
+// column offset (< 128 KiB) is added.  The FP64 kernel gathers c itself; the
+// FP32 and bf16x3 kernels, whose tile elements are rounded to float anyway,
+// gather a float copy of c (half the bytes per lane through the vector
+// memory pipeline, the builders' bound).
 constexpr uint32_t FUSED_OOB = 0xFFF00000u;
 
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t fused_c_rsrc(const double *c, uint32_t bytes)
+template <typename V>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t fused_c_rsrc(const V *c, uint32_t bytes)
 {
-        return __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(c), (short)0, (int)bytes, 0x00020000);
+        return __builtin_amdgcn_make_buffer_rsrc(const_cast<V *>(c), (short)0, (int)bytes, 0x00020000);
 }
 
-__device__ __forceinline__ double fused_c_load(__amdgpu_buffer_rsrc_t rc, uint32_t off)
+template <typename V>
+__device__ __forceinline__ V fused_c_load(__amdgpu_buffer_rsrc_t rc, uint32_t off)
 {
-        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rc, off, 0, 0));
+        if constexpr (sizeof(V) == 8)
+                return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(rc, off, 0, 0));
+        else
+                return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b32(rc, off, 0, 0));
 }
 
 // -x for bit = 1, x for bit = 0 (sign flip without selects)
@@ -278,8 +289,14 @@ __device__ __forceinline__ double fused_flip(double x, uint32_t bit)
         return __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, x) ^ ((uint64_t)bit << 63));
 }
 
-// Builder state of one chunk: c values in flight and their two signs.
-template <int NBLD, int MAXW = FUSED_MAXW>
+__device__ __forceinline__ float fused_flip(float x, uint32_t bit)
+{
+        return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, x) ^ (bit << 31));
+}
+
+// Builder state of one chunk: c values in flight (element type V) and their
+// two signs.
+template <int NBLD, int MAXW, typename V>
 struct FusedGather {
         // tile rows: 16 per MFMA wavefront, at most npair <= 136 (norb <= 16)
         static constexpr int ROWS = 16 * MAXW < 136 ? 16 * MAXW : 136;
@@ -287,95 +304,95 @@ struct FusedGather {
         static_assert(EB <= 16, "sign flags hold 16 elements");
         typedef uint32_t Flags;
         Flags sg;             // element s: bit 2s alpha sign, 2s+1 beta sign
-        double va[EB], vb[EB];
-        double vf[EB];        // fBeta[k][b] of each element (diagonal term)
-        int mr[EB];           // beta pair-map entries (register-held maps)
+        V va[EB], vb[EB];
+        V vf[EB];             // fBeta[k][b] of each element (diagonal term)
 };
 
-// Issue the c loads of chunk j: element e = (k, col), k = e / 16, of the
-// tile is sa c[Ja, b0 + col] + sb c[I, Jb]: abyte[k] = byte offset of row Ja
-// with sa in bit 0 (FUSED_OOB if I has no alpha link with pair k, also for
-// k >= npair), mb[e] or mr = (Jb << 1 | sb) or -1.  Elements past the tile
-// or past the row's end load values that are never used (or zeros).
-//
-// With DIAG, also the fBeta[k][b0 + col] of every element (rows nbp apart,
-// through the buffer resource fr: rows past npair read zero) for the
-// diagonal term.
-template <int NBLD, int MAXW, bool REGMAP, bool DIAG = false>
-__device__ __forceinline__ void fg_issue(FusedGather<NBLD, MAXW> &st, uint32_t rowI_byte, int64_t j, int bt,
+// Issue the loads of chunk j: element e = bt + s * NBLD * 64 = (k, col),
+// k = e / 16, of the tile is sa c[Ja, b0 + col] + sb c[I, Jb], with
+// abyte[k] = byte offset of c row Ja (sa in bit 0; FUSED_OOB if I has no
+// alpha link with pair k, also for k >= npair) and mb[e] = Jb << 1 | sb or
+// -1 (the chunk's beta pair-map slice).  With DIAG, also fBeta[k][b0 + col]
+// of every element (rows nbp apart, through fr: rows past npair read zero)
+// for the diagonal term.  Elements past the tile or past the row's end load
+// values that are never used (or zeros).
+template <int NBLD, int MAXW, bool DIAG, typename V>
+__device__ __forceinline__ void fg_issue(FusedGather<NBLD, MAXW, V> &st, uint32_t rowI_byte, int64_t j, int bt,
                                          const uint32_t *__restrict__ abyte, const int16_t *__restrict__ mb,
                                          __amdgpu_buffer_rsrc_t rc, __amdgpu_buffer_rsrc_t fr, int64_t nbp)
 {
-        constexpr int NB = FUSED_NB, EB = FusedGather<NBLD, MAXW>::EB;
+        constexpr int NB = FUSED_NB, EB = FusedGather<NBLD, MAXW, V>::EB;
         constexpr int nbt = NBLD * 64;
-        const uint32_t cbyte = (uint32_t)(j * NB) * 8u;
-        typedef typename FusedGather<NBLD, MAXW>::Flags Flags;
+        constexpr uint32_t W = sizeof(V);
+        static_assert(EB * nbt <= 16 * MAXW * FUSED_NB, "elements stay in the map slice");
+        const uint32_t cbyte = (uint32_t)(j * NB) * W;
+        typedef typename FusedGather<NBLD, MAXW, V>::Flags Flags;
         Flags sg = 0;
         uint32_t oa[EB], ob[EB];
 #pragma unroll
         for (int s = 0; s < EB; s++) {
                 const int e = bt + s * nbt;
                 const uint32_t av = abyte[e >> 4];
-                const int m = REGMAP ? (int)(int16_t)st.mr[s] : (int)mb[e];
-                oa[s] = (av & ~7u) + cbyte + (uint32_t)(e & 15) * 8u;
-                ob[s] = m >= 0 ? rowI_byte + (uint32_t)(m >> 1) * 8u : FUSED_OOB;
+                const int m = (int)mb[e];
+                oa[s] = (av & ~1u) + cbyte + (uint32_t)(e & 15) * W;
+                ob[s] = m >= 0 ? rowI_byte + (uint32_t)(m >> 1) * W : FUSED_OOB;
                 sg |= (Flags)((av & 1u) | ((uint32_t)(m & 1) << 1)) << (2 * s);
         }
         st.sg = sg;
 #pragma unroll
         for (int s = 0; s < EB; s++) {
-                st.va[s] = fused_c_load(rc, oa[s]);
-                st.vb[s] = fused_c_load(rc, ob[s]);
+                st.va[s] = fused_c_load<V>(rc, oa[s]);
+                st.vb[s] = fused_c_load<V>(rc, ob[s]);
         }
         if constexpr (DIAG) {
                 // element s of this thread: row (bt >> 4) + s * nbt / 16, column bt & 15
                 static_assert(nbt % 16 == 0, "a thread keeps its column");
-                const uint32_t voff = (uint32_t)(((int64_t)(bt >> 4) * nbp + j * NB + (bt & 15)) * 8);
+                const uint32_t voff = (uint32_t)(((int64_t)(bt >> 4) * nbp + j * NB + (bt & 15)) * W);
 #pragma unroll
                 for (int s = 0; s < EB; s++)
-                        st.vf[s] = fused_c_load(fr, voff + (uint32_t)(s * (nbt / 16) * nbp * 8));
+                        st.vf[s] = fused_c_load<V>(fr, voff + (uint32_t)(s * (nbt / 16) * nbp * W));
         }
 }
 
-// Commit the tile elements; with DIAG, return sum_s fBeta * t1 over this
-// thread's elements (its column's part of the diagonal term).
-template <int NBLD, int MAXW, bool DIAG = false, typename BT>
-__device__ __forceinline__ double fg_commit(const FusedGather<NBLD, MAXW> &st, int npair, int bt, BT *bb)
+// Commit the tile elements, [k][col] in BT; with DIAG, return sum_s fBeta *
+// t1 over this thread's elements (its column's part of the diagonal term).
+template <int NBLD, int MAXW, bool DIAG = false, typename V, typename BT>
+__device__ __forceinline__ double fg_commit(const FusedGather<NBLD, MAXW, V> &st, int npair, int bt, BT *bb)
 {
-        constexpr int EB = FusedGather<NBLD, MAXW>::EB;
+        constexpr int EB = FusedGather<NBLD, MAXW, V>::EB;
         constexpr int nbt = NBLD * 64;
         const int nelem = npair * FUSED_NB;
-        double d = 0.0;
+        V d = 0;
 #pragma unroll
         for (int s = 0; s < EB; s++) {
                 const int e = bt + s * nbt;
                 if (e < nelem) {
                         const uint32_t f = (uint32_t)(st.sg >> (2 * s));
-                        const double t = fused_flip(st.va[s], f & 1u) + fused_flip(st.vb[s], (f >> 1) & 1u);
+                        const V t = fused_flip(st.va[s], f & 1u) + fused_flip(st.vb[s], (f >> 1) & 1u);
                         bb[e] = (BT)t;
                         if constexpr (DIAG) d += st.vf[s] * t;
                 }
         }
-        return d;
+        return (double)d;
 }
 
 // fg_commit for the bf16x3 kernel (column-major high / low parts, as
 // fb_commit_split).
-template <int NBLD, int MAXW, bool DIAG = false>
-__device__ __forceinline__ double fg_commit_split(const FusedGather<NBLD, MAXW> &st, int npair, int bt,
+template <int NBLD, int MAXW, bool DIAG = false, typename V>
+__device__ __forceinline__ double fg_commit_split(const FusedGather<NBLD, MAXW, V> &st, int npair, int bt,
                                                   __bf16 *__restrict__ bh)
 {
-        constexpr int EB = FusedGather<NBLD, MAXW>::EB, NB = FUSED_NB, KP = FUSED_KPS_BF16;
+        constexpr int EB = FusedGather<NBLD, MAXW, V>::EB, NB = FUSED_NB, KP = FUSED_KPS_BF16;
         constexpr int nbt = NBLD * 64;
         __bf16 *bl = bh + NB * KP;
         const int nelem = npair * NB;
-        double d = 0.0;
+        V d = 0;
 #pragma unroll
         for (int s = 0; s < EB; s++) {
                 const int e = bt + s * nbt;
                 if (e < nelem) {
                         const uint32_t f = (uint32_t)(st.sg >> (2 * s));
-                        const double t = fused_flip(st.va[s], f & 1u) + fused_flip(st.vb[s], (f >> 1) & 1u);
+                        const V t = fused_flip(st.va[s], f & 1u) + fused_flip(st.vb[s], (f >> 1) & 1u);
                         const float x = (float)t;
                         const __bf16 hi = (__bf16)x;
                         const int k = e >> 4, col = e & 15;
@@ -384,38 +401,7 @@ __device__ __forceinline__ double fg_commit_split(const FusedGather<NBLD, MAXW> 
                         if constexpr (DIAG) d += st.vf[s] * t;
                 }
         }
-        return d;
-}
-
-// Register-held beta pair-map entries of chunk j (bf16x3): element e ->
-// pmapb[e / 16][b0 + e % 16] (rows past npair read row npair - 1; never used).
-template <int NBLD, int MAXW>
-__device__ __forceinline__ void fg_issue_mapv(FusedGather<NBLD, MAXW> &st, int npair, int64_t nbp, int64_t j,
-                                              int bt, const int16_t *__restrict__ pmapb)
-{
-        constexpr int NB = FUSED_NB, EB = FusedGather<NBLD, MAXW>::EB;
-        constexpr int nbt = NBLD * 64;
-        const uint16_t *pm = reinterpret_cast<const uint16_t *>(pmapb) + j * NB;
-#pragma unroll
-        for (int s = 0; s < EB; s++) {
-                const int e = bt + s * nbt;
-                const int k = (e >> 4) < npair ? (e >> 4) : npair - 1;
-                st.mr[s] = pm[(int64_t)k * nbp + (e & 15)];
-        }
-}
-
-template <int NBLD, int MAXW>
-__device__ __forceinline__ void fg_store_mapv(const FusedGather<NBLD, MAXW> &st, int npair, int bt,
-                                              int16_t *__restrict__ mb)
-{
-        constexpr int EB = FusedGather<NBLD, MAXW>::EB;
-        constexpr int nbt = NBLD * 64;
-        const int nelem = npair * FUSED_NB;
-#pragma unroll
-        for (int s = 0; s < EB; s++) {
-                const int e = bt + s * nbt;
-                if (e < nelem) mb[e] = (int16_t)st.mr[s];
-        }
+        return (double)d;
 }
 
 // One chunk of the contraction on MFMA: the 16 x 16 block
@@ -517,7 +503,9 @@ struct fcis_sigma_plan {
         void *d_hfrag_f, *d_hfrag_f_lo;
         int16_t *d_mpair;            // [16 * FUSED_MTILES] pair of each fused row
         double *d_falpha;            // [nstr_a][npair]: sum_{p in I} h[pp][k]
-        double *d_fbeta;             // [npair][nbp]:    sum_{p in b} h[pp][k]
+        void *d_fbeta;               // [npair][nbp]:    sum_{p in b} h[pp][k] (double, float for
+                                     // the FP32 / bf16x3 kernels)
+        float *d_cg;                 // float copy of c gathered by the FP32 / bf16x3 fused kernels
         int32_t *d_order;            // alpha-string processing order (single-tile fused plans) or null
         int timing;
         hipEvent_t ev[4];

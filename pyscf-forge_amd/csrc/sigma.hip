@@ -301,15 +301,16 @@ __global__ __launch_bounds__(64 * (FUSED_MTILES + FUSED_NBLD)) void k_fused_row(
         int npair, int mtiles, int mrow_f, int na, int nlinka, int64_t nb, int64_t nstra, int64_t a0,
         int64_t ta, int64_t ld, int accumulate, const fcis_link *__restrict__ linka,
         const int32_t *__restrict__ pmapa, const int16_t *__restrict__ pmapb, int64_t nbp,
-        const double *__restrict__ c, const void *__restrict__ hfrag_v,
+        const typename FusedTypes<PREC>::G *__restrict__ c, const void *__restrict__ hfrag_v,
         const void *__restrict__ hfrag_lo_v, const int16_t *__restrict__ mpair,
-        const double *__restrict__ falpha, const double *__restrict__ fbeta,
+        const double *__restrict__ falpha, const typename FusedTypes<PREC>::G *__restrict__ fbeta,
         typename FusedTypes<PREC>::Out *__restrict__ walpha,
         double *__restrict__ sigma, const int32_t *__restrict__ rows,
         const int2 *__restrict__ rowcl, const uint16_t *__restrict__ clist)
 {
         typedef FusedTypes<PREC> Ty;
         typedef typename Ty::B BT;
+        typedef typename Ty::G G;
         constexpr int KS = Ty::KS, KW = Ty::KW, NB = FUSED_NB;
         constexpr int KPMAX = KS * KW;
         constexpr int MAXP = 16 * FUSED_MAXW;
@@ -329,7 +330,7 @@ __global__ __launch_bounds__(64 * (FUSED_MTILES + FUSED_NBLD)) void k_fused_row(
         int16_t *maps = reinterpret_cast<int16_t *>(aslot + MAXP); // FUSED_NMAP x [MAXP][NB] beta maps
 
         const int tid = threadIdx.x, nthr = blockDim.x;
-        const __amdgpu_buffer_rsrc_t rc = fused_c_rsrc(c, (uint32_t)(nstra * nb * 8));
+        const __amdgpu_buffer_rsrc_t rc = fused_c_rsrc(c, (uint32_t)(nstra * nb * (int64_t)sizeof(G)));
         const int wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
         const bool mfma_wave = wave < mtiles;
         const int64_t nch = (nb + NB - 1) / NB;
@@ -442,14 +443,14 @@ __global__ __launch_bounds__(64 * (FUSED_MTILES + FUSED_NBLD)) void k_fused_row(
                 // formed at commit from the tile elements in registers (a thread
                 // keeps one column; lanes of a column summed, then one LDS
                 // atomic per column and wavefront)
-                const __amdgpu_buffer_rsrc_t fbr = __builtin_amdgcn_make_buffer_rsrc(
-                        const_cast<double *>(fbeta), (short)0, (int)(npair * nbp * 8), 0x00020000);
+                const __amdgpu_buffer_rsrc_t fbr = fused_c_rsrc(fbeta, (uint32_t)(npair * nbp * (int64_t)sizeof(G)));
                 auto add_diag = [&](double d, int64_t b0) {
                         d += __shfl_xor(d, 16);
                         d += __shfl_xor(d, 32);
                         if (lane < 16 && b0 + (bt & 15) < nb) atomicAdd(&srow[b0 + (bt & 15)], d);
                 };
-                FusedGather<NBA> s0, s1;   // c of even / odd chunks in flight
+                typedef FusedGather<NBA, FUSED_MAXW, G> Gather;
+                Gather s0, s1;   // c of even / odd chunks in flight
                 for (int64_t r = blockIdx.x; r < ta; r += gridDim.x) {
                         const int64_t I = row_of(r);
                         int64_t nl = nch;
@@ -468,11 +469,12 @@ __global__ __launch_bounds__(64 * (FUSED_MTILES + FUSED_NBLD)) void k_fused_row(
                         __syncthreads();   // (1)
                         for (int q = bt; q < MAXP; q += NBA * 64) {
                                 const int a = q < npair ? pmapa[(int64_t)q * nstra + I] : -1;
-                                abyte[q] = a >= 0 ? (uint32_t)(a >> 1) * (uint32_t)nb * 8u | (uint32_t)(a & 1)
+                                abyte[q] = a >= 0 ? (uint32_t)(a >> 1) * (uint32_t)nb * (uint32_t)sizeof(G) |
+                                                            (uint32_t)(a & 1)
                                                   : FUSED_OOB;
                                 aslot[q] = -1;
                         }
-                        const uint32_t rowI_byte = (uint32_t)(I * nb) * 8u;
+                        const uint32_t rowI_byte = (uint32_t)(I * nb) * (uint32_t)sizeof(G);
                         __syncthreads();   // (2)
                         // walpha slots: the off-diagonal alpha links (slots na ..);
                         // the diagonal ones act within row I (the diagonal term)
@@ -486,8 +488,8 @@ __global__ __launch_bounds__(64 * (FUSED_MTILES + FUSED_NBLD)) void k_fused_row(
                         // loads left in flight here, oldest first: c of chunk 1 -- the
                         // order the loop body leaves them in, so that the loop head sees
                         // one order and every wait stays partial
-                        fg_issue<NBA, FUSED_MAXW, false, true>(s0, rowI_byte, cc(0), bt, abyte, mbuf(0), rc, fbr, nbp);
-                        fg_issue<NBA, FUSED_MAXW, false, true>(s1, rowI_byte, cc(1), bt, abyte, mbuf(1), rc, fbr, nbp);
+                        fg_issue<NBA, FUSED_MAXW, true>(s0, rowI_byte, cc(0), bt, abyte, mbuf(0), rc, fbr, nbp);
+                        fg_issue<NBA, FUSED_MAXW, true>(s1, rowI_byte, cc(1), bt, abyte, mbuf(1), rc, fbr, nbp);
                         if constexpr (PRESPLIT)
                                 add_diag(fg_commit_split<NBA, FUSED_MAXW, true>(s0, npair, bt,
                                                                                 reinterpret_cast<__bf16 *>(buf0)),
@@ -499,9 +501,9 @@ __global__ __launch_bounds__(64 * (FUSED_MTILES + FUSED_NBLD)) void k_fused_row(
                         // slice is in LDS); `commit` holds the c values of chunk j+1
                         // (issued one phase earlier), committed now.  The loads of
                         // this phase stay in flight across the barrier.
-                        auto phase = [&](int64_t j, FusedGather<NBA> &issue, FusedGather<NBA> &commit) {
-                                fg_issue<NBA, FUSED_MAXW, false, true>(issue, rowI_byte, cc(j + 2), bt, abyte,
-                                                                        mbuf(j + 2), rc, fbr, nbp);
+                        auto phase = [&](int64_t j, Gather &issue, Gather &commit) {
+                                fg_issue<NBA, FUSED_MAXW, true>(issue, rowI_byte, cc(j + 2), bt, abyte, mbuf(j + 2), rc,
+                                                                fbr, nbp);
                                 // unconditional: past the row's last chunk the commit goes
                                 // to the B buffer that no wavefront reads in this phase
                                 // (clamped chunk data); its diagonal term is dropped
@@ -920,7 +922,12 @@ int upload_h(fcis_sigma_plan *p, const double *h2e)
                 }
         }
         FCIS_HIP(hipMemcpy(p->d_falpha, fa.data(), 8 * fa.size(), hipMemcpyHostToDevice));
-        FCIS_HIP(hipMemcpy(p->d_fbeta, fb.data(), 8 * fb.size(), hipMemcpyHostToDevice));
+        if (p->precision == FCIS_PREC_FP64) {
+                FCIS_HIP(hipMemcpy(p->d_fbeta, fb.data(), 8 * fb.size(), hipMemcpyHostToDevice));
+        } else {
+                std::vector<float> fb32(fb.begin(), fb.end());
+                FCIS_HIP(hipMemcpy(p->d_fbeta, fb32.data(), 4 * fb32.size(), hipMemcpyHostToDevice));
+        }
         return FCIS_SUCCESS;
 }
 
@@ -1046,19 +1053,36 @@ void launch_fused(const fcis_sigma_plan *p, hipStream_t st, int64_t a0, int64_t 
         const int32_t *rows = ROWS ? rs->d_rows : p->d_order;
         const int2 *rowcl = ROWS ? rs->d_rowcl : nullptr;
         const uint16_t *clist = ROWS ? rs->d_clist : nullptr;
-#define FCIS_LAUNCH_FUSED(PR, OUT)                                                                    \
+        // the FP32 / bf16x3 kernels gather the float copy of c (p->d_cg)
+#define FCIS_LAUNCH_FUSED(PR, OUT, CG)                                                                \
         hipLaunchKernelGGL((k_fused_row<PR, ROWS>), fgrid, fblk, p->fused_lds, st, p->npair,      \
                            p->mtiles_f, p->mrow_f, s->na, s->nlinka, nb, na, a0, ta, lda, acc,        \
-                           s->d_linka, p->d_pmapa, p->d_pmapb16, p->nbp, ci0, p->d_hfrag_f,           \
-                           p->d_hfrag_f_lo, p->d_mpair, p->d_falpha, p->d_fbeta, (OUT *)p->d_vt1,     \
-                           sig, rows, rowcl, clist)
+                           s->d_linka, p->d_pmapa, p->d_pmapb16, p->nbp, CG, p->d_hfrag_f,            \
+                           p->d_hfrag_f_lo, p->d_mpair, p->d_falpha,                                  \
+                           (const FusedTypes<PR>::G *)p->d_fbeta, (OUT *)p->d_vt1, sig, rows, rowcl,  \
+                           clist)
         if (p->precision == FCIS_PREC_FP64)
-                FCIS_LAUNCH_FUSED(FCIS_PREC_FP64, double);
+                FCIS_LAUNCH_FUSED(FCIS_PREC_FP64, double, ci0);
         else if (p->precision == FCIS_PREC_FP32)
-                FCIS_LAUNCH_FUSED(FCIS_PREC_FP32, float);
+                FCIS_LAUNCH_FUSED(FCIS_PREC_FP32, float, p->d_cg);
         else
-                FCIS_LAUNCH_FUSED(FCIS_PREC_BF16X3, float);
+                FCIS_LAUNCH_FUSED(FCIS_PREC_BF16X3, float, p->d_cg);
 #undef FCIS_LAUNCH_FUSED
+}
+
+// The float copy of c that the FP32 / bf16x3 fused kernels gather.
+__global__ __launch_bounds__(256) void k_to_float(int64_t n, const double *__restrict__ x, float *__restrict__ y)
+{
+        const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) y[i] = (float)x[i];
+}
+
+void fused_gather_copy(const fcis_sigma_plan *p, hipStream_t st, const double *ci0)
+{
+        if (!p->fused || p->precision == FCIS_PREC_FP64) return;
+        const int64_t n = p->space->nstra * p->space->nstrb;
+        const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_to_float, dim3(blocks), dim3(256), 0, st, n, ci0, p->d_cg);
 }
 
 // The fused row kernel of a precision, ROWS or not.
@@ -1151,6 +1175,7 @@ int fcis::sigma_plan_execute_rows(fcis_sigma_plan *p, const fcis_rowset *rs, con
         const int64_t tile = p->tile_a;
         const int ntiles = (int)((rs->nrows + tile - 1) / tile);
         const int acc = ntiles > 1 ? 1 : 0;
+        fused_gather_copy(p, st, ci0);
         if (acc)
                 hipLaunchKernelGGL(k_zero, dim3((na * nb + 255) / 256), dim3(256), 0, st, na * nb, sig);
         const int64_t pgrid = std::min<int64_t>((rs->ndets + 255) / 256, 16384);
@@ -1270,7 +1295,10 @@ int fcis::sigma_plan_create(fcis_sigma_plan **out, const fcis_space *space, cons
                                   hipSuccess ||
                           fcis::dev_malloc(&p->d_mpair, 2 * 16 * FUSED_MTILES) != hipSuccess ||
                           fcis::dev_malloc(&p->d_falpha, 8 * (size_t)space->nstra * p->npair) != hipSuccess ||
-                          fcis::dev_malloc(&p->d_fbeta, 8 * (size_t)p->npair * p->nbp) != hipSuccess))) {
+                          fcis::dev_malloc(&p->d_fbeta, elem * (size_t)p->npair * p->nbp) != hipSuccess ||
+                          fcis::dev_malloc(&p->d_cg, precision == FCIS_PREC_FP64
+                                                             ? 16
+                                                             : 4 * (size_t)space->nstra * nb) != hipSuccess))) {
                 fcis_sigma_plan_free(p);
                 fcis::set_error("fcis_sigma_plan_create: device allocation of %zu bytes failed",
                                 2 * buf + 2 * hbytes + pa + pb);
@@ -1364,6 +1392,7 @@ int fcis_sigma_plan_execute(fcis_sigma_plan_t *p, const double *ci0, double *ci1
         const int64_t nb = s->nstrb, na = s->nstra;
         if (na * nb == 0) return FCIS_SUCCESS;
         const dim3 gblk(64 * GEMM_WAVES);
+        fused_gather_copy(p, st, ci0);
         for (int t = 0; t < p->ntiles; t++) {
                 const int64_t a0 = (int64_t)t * p->tile_a;
                 const int64_t ta = na - a0 < p->tile_a ? na - a0 : p->tile_a;
@@ -1479,6 +1508,7 @@ void fcis_sigma_plan_free(fcis_sigma_plan_t *p)
         (void)hipFree(p->d_mpair);
         (void)hipFree(p->d_falpha);
         (void)hipFree(p->d_fbeta);
+        (void)hipFree(p->d_cg);
         delete p;
 }
 
