@@ -92,43 +92,303 @@ def kernel_roofline(plan, sp, prec, calls, sync):
     return out
 
 
-def cpu_baseline(pkg, budget_s=12.0):
+class HostOps:
+    """The Davidson driver's vector interface on host arrays (the CPU
+    reference leg runs the same algorithm over the reference's sigma)."""
+
+    def empty(self, shape, like):
+        import numpy
+        return numpy.empty(shape)
+
+    def clone(self, v):
+        import numpy
+        return numpy.array(v, dtype=numpy.float64).reshape(-1)
+
+    def copy(self, dst, src):
+        dst[...] = src
+
+    def dots(self, m, n, X, v):
+        import numpy
+        return numpy.atleast_2d(X)[:m].reshape(m, n) @ v.reshape(-1)
+
+    def lincomb(self, m, n, X, coef, beta, y):
+        import numpy
+        Xm = numpy.atleast_2d(X)[:m].reshape(m, n)
+        y[...] = (beta * y.reshape(-1) if beta != 0.0 else 0.0) + numpy.asarray(coef).reshape(m) @ Xm
+
+    def combine(self, n, vecs, coef, y):
+        y[...] = sum(c * v.reshape(-1) for c, v in zip(coef, vecs))
+
+
+def rel(a, b):
+    import numpy
+    a = numpy.asarray(a).reshape(-1)
+    b = numpy.asarray(b).reshape(-1)
+    return float(numpy.linalg.norm(a - b) / numpy.linalg.norm(b))
+
+
+def cpu_baseline(pkg, gpu, sync):
     """The reference's own libfci_gas (oracle/_ref, built from the reference
-    sources) on the host cores: full FP64 sigma of the (14o,14e) problem,
-    repeated for ~budget_s, scaled to (16o,16e) sigma/s by the algorithmic
-    work ratio F_sigma(14o)/F_sigma(16o)."""
+    sources) on the host cores, beside the GPU results of the same inputs:
+      * headline: one full FP64 sigma of (16o,16e) on the GPU run's c
+        (timed; its difference from the GPU FP64 sigma);
+      * C0 (8o,8e): sigma difference, and the ground-state energy of ARPACK
+        over the reference sigma against the GPU Davidson;
+      * C1 (14o,14e): sigma time and difference, and 12 Davidson iterations
+        (the same algorithm, host vectors) over the reference sigma;
+      * C4 (12o,12e): the reference RDMs of the GPU ground state against the
+        GPU RDM kernel."""
     import numpy
     import oracle
     if not oracle.ref_available():
         return None
     ref = oracle.RefLib()
+    cores = int(os.environ.get("OMP_NUM_THREADS", 0)) or len(os.sched_getaffinity(0))
+    out = {"unit": "sigma/s", "cores": cores, "kind": "reference", "configs": {}}
+
+    hd = gpu.get("headline")
+    if hd is not None:
+        n, ne = hd["norb"], hd["nelec"]
+        c = hd["c"]
+        y = numpy.empty_like(c)
+        plan = ref.plan(hd["h2e"], n, ne)
+        t0 = time.perf_counter()
+        plan(c, y)
+        dt = time.perf_counter() - t0
+        runs = [dt]
+        if dt < 10.0:
+            t0 = time.perf_counter()
+            plan(c, y)
+            runs.append(time.perf_counter() - t0)
+        plan.close()
+        dt = min(runs)
+        out["value"] = 1.0 / dt
+        out["sample"] = ("libfci_gas (reference C, OpenMP on %d threads, -O3 -march=x86-64-v3) FP64 "
+                         "sigma of CAS(16o,16e) [165.6M dets] on the GPU run's c: %d run(s), "
+                         "best %.2f s" % (cores, len(runs), dt))
+        out["sigma_s_16o"] = dt
+        out["gpu_fp64_sigma_rel_diff"] = rel(hd["sigma64"], y)
+        log("CPU reference sigma (16o,16e): %.2f s on %d threads, GPU FP64 rel diff %.2e" % (
+            dt, cores, out["gpu_fp64_sigma_rel_diff"]))
+        del y
+
+    c0 = gpu.get("c0")
+    if c0 is not None:
+        import scipy.sparse.linalg as sla
+        n, ne = c0["norb"], c0["nelec"]
+        y = ref.contract_2e(c0["h2e"], c0["c"], n, ne)
+        nd = c0["c"].size
+        op = sla.LinearOperator((nd, nd), dtype=numpy.float64,
+                                matvec=lambda x: ref.contract_2e(c0["h2e"], x, n, ne).reshape(-1))
+        w = sla.eigsh(op, k=1, which="SA", tol=1e-13)[0]
+        out["configs"]["c0"] = {"sigma_rel_diff_gpu_fp64": rel(c0["sigma64"], y),
+                                "energy_ref_arpack": float(w[0]),
+                                "dE_gpu_fp64_vs_ref": abs(c0["e64"] - float(w[0]))}
+        log("C0 vs reference: %s" % out["configs"]["c0"])
+
+    c1 = gpu.get("c1")
+    if c1 is not None:
+        n, ne = c1["norb"], c1["nelec"]
+        c = c1["c"]
+        y = numpy.empty_like(c)
+        plan = ref.plan(c1["h2e"], n, ne)
+        plan(c, y)
+        t0 = time.perf_counter()
+        plan(c, y)
+        dt = time.perf_counter() - t0
+        rec = {"sigma_s": dt, "sigma_rel_diff_gpu_fp64": rel(c1["sigma64"], y)}
+        hdiag = ref.make_hdiag(c1["h1e"], c1["eri"], n, ne).reshape(-1)
+        # the reference solver's guess (fci_gas.py:46-88) and diagonal
+        # preconditioner (:1337-1342), as the GPU run uses them
+        _, guess = oracle.init_guess(hdiag, 1)
+
+        def hop(x, o):
+            plan(numpy.ascontiguousarray(x), o)
+
+        def precond(r, e, o, x0):
+            o[...] = oracle.precond(r, hdiag, e, 1e-3)
+
+        t0 = time.perf_counter()
+        res = pkg.davidson.davidson(hop, precond, list(guess), nroots=1, tol=1e-10, max_cycle=12,
+                                    max_space=12, ops=HostOps())
+        rec["davidson12_s"] = time.perf_counter() - t0
+        rec["davidson12_energy"] = float(res.e[0])
+        rec["davidson12_dE_vs_gpu"] = abs(float(res.e[0]) - c1["e12"])
+        plan.close()
+        out["configs"]["c1"] = rec
+        log("C1 vs reference: %s" % rec)
+
+    c4 = gpu.get("c4")
+    if c4 is not None:
+        n, ne = c4["norb"], c4["nelec"]
+        (a, b), (aa, ab, bb) = ref.trans_rdm12s(c4["ci"], c4["ci"], n, ne)
+        dm1 = a + b
+        dm2 = aa + bb + ab + ab.transpose(2, 3, 0, 1)
+        out["configs"]["c4"] = {"rdm1_max_abs_diff": float(numpy.abs(dm1 - c4["dm1"]).max()),
+                                "rdm2_max_abs_diff": float(numpy.abs(dm2 - c4["dm2"]).max())}
+        log("C4 vs reference: %s" % out["configs"]["c4"])
+    return out
+
+
+def sigma_precisions(pkg, sp, h2e, c, steps, warmup, sync, with_roofline=False,
+                     precisions=("fp64", "fp32", "bf16x3")):
+    """FP64 / FP32 / bf16x3 sigma of c: time per sigma and relative error
+    against the FP64 sigma (FP64 first when it is among the precisions);
+    returns (records, FP64 sigma or None, kernel rooflines)."""
+    import torch
+    out = torch.empty_like(c)
+    npair = sp.norb * (sp.norb + 1) // 2
+    F = 2.0 * npair * npair * sp.ndet
+    recs, kern, s64 = {}, {}, None
+    order = [p for p in ("fp64", "fp32", "bf16x3") if p in precisions]
+    for prec in order:
+        free = torch.cuda.mem_get_info()[0]
+        plan = pkg.SigmaPlan(sp, h2e, prec, int(free * 0.85))
+        for _ in range(warmup):
+            plan(c, out)
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            plan(c, out)
+        sync()
+        dt = (time.perf_counter() - t0) / steps
+        if with_roofline:
+            plan._bench_in, plan._bench_out = c, out
+            kern[prec] = kernel_roofline(plan, sp, prec, 2, sync)
+            kern[prec]["ntiles"] = plan.ntiles
+        if prec == "fp64":
+            s64 = out.clone()
+        recs[prec] = {"ms": dt * 1e3, "sigma_per_s": 1.0 / dt, "TFLOP/s": F / dt / 1e12,
+                      "rel_err_vs_fp64": 0.0 if prec == "fp64" else None if s64 is None else
+                      float(torch.linalg.vector_norm(out - s64) / torch.linalg.vector_norm(s64))}
+        plan.close()
+        del plan
+    del out
+    return recs, s64, kern
+
+
+def davidson_precisions(pkg, sync, h1e, eri, n, ne, max_cycle, **opts):
+    """FP64 and FP32 / bf16x3-mixed Davidson ground states: time, energy,
+    difference from the FP64 energy."""
+    import torch
+    out, e64 = {}, None
+    for prec in ("fp64", "fp32", "bf16x3"):
+        s = pkg.FCISolver()
+        s.precision = prec
+        s.max_cycle = max_cycle
+        for k, v in opts.items():
+            setattr(s, k, v)
+        sync()
+        t0 = time.perf_counter()
+        e, ci = s.kernel(h1e, eri, n, ne)
+        sync()
+        dt = time.perf_counter() - t0
+        e64 = e if prec == "fp64" else e64
+        out[prec] = {"seconds": dt, "energy": e, "converged": bool(s.converged),
+                     "dE_vs_fp64": abs(e - e64),
+                     "stats": {k: v for k, v in getattr(s, "stats", {}).items()
+                               if isinstance(v, (int, float, str))}}
+        del ci, s
+        torch.cuda.empty_cache()
+    return out
+
+
+def config0(pkg, sync):
+    """C0: CAS(8o,8e), (4,4), 4900 dets, c from the synthetic recipe (seed 0)."""
+    import torch
+    n, ne = 8, (4, 4)
+    h1e, eri, c = pkg.synthetic.make_problem(n, ne, seed=0)
+    h2e = pkg.absorb_h1e(h1e, eri, n, ne, 0.5)
+    sp = pkg.get_space(n, ne)
+    cd = torch.from_numpy(c.reshape(-1).copy()).cuda()
+    recs, s64, _ = sigma_precisions(pkg, sp, h2e, cd, 5, 1, sync)
+    dav = davidson_precisions(pkg, sync, h1e, eri, n, ne, 100, davidson_only=True)
+    out = {"workload": "CAS(8o,8e) nelec (4,4), 4900 dets, synthetic integrals and c, seed 0",
+           "sigma": recs, "davidson": dav}
+    log("C0: sigma %s, Davidson dE %s" % (
+        {p: (round(r["ms"], 3), r["rel_err_vs_fp64"]) for p, r in recs.items()},
+        {p: r["dE_vs_fp64"] for p, r in dav.items()}))
+    host = {"norb": n, "nelec": ne, "h2e": h2e, "c": c.reshape(-1).copy(),
+            "sigma64": s64.cpu().numpy(), "e64": dav["fp64"]["energy"]}
+    return out, host
+
+
+def config1(pkg, sync):
+    """C1: CAS(14o,14e), (7,7), 11.8M dets: FP64 / FP32 / bf16x3 sigma, 12
+    FP64 Davidson iterations (nroots 1), and the converged FP64 and mixed
+    ground states."""
+    import torch
     n, ne = 14, (7, 7)
     h1e, eri = pkg.synthetic.make_problem(n, ne, seed=0, with_ci=False)
-    h2e = oracle.absorb_h1e(h1e, eri, n, ne, 0.5)
-    nstr = math.comb(n, 7)
-    c = numpy.random.default_rng(0).standard_normal(nstr * nstr)
-    c /= numpy.linalg.norm(c)
-    out = numpy.empty_like(c)
-    plan = ref.plan(h2e, n, ne)
-    plan(c, out)
+    h2e = pkg.absorb_h1e(h1e, eri, n, ne, 0.5)
+    sp = pkg.get_space(n, ne)
+    g = torch.Generator(device="cuda").manual_seed(0)
+    c = torch.randn(sp.ndet, dtype=torch.float64, device="cuda", generator=g)
+    c /= torch.linalg.vector_norm(c)
+    recs, s64, _ = sigma_precisions(pkg, sp, h2e, c, 5, 1, sync)
+    # 12 iterations from the lowest-diagonal determinant, diagonal
+    # preconditioner: the same algorithm the CPU reference leg runs
+    s = pkg.FCISolver()
+    sync()
     t0 = time.perf_counter()
-    reps = 0
-    while True:
-        plan(c, out)
-        reps += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    dt = (time.perf_counter() - t0) / reps
-    plan.close()
-    f1 = 2.0 * (n * (n + 1) // 2) ** 2 * nstr * nstr
-    f2 = 2.0 * 136 ** 2 * math.comb(16, 8) ** 2
-    cores = int(os.environ.get("OMP_NUM_THREADS", 0)) or len(os.sched_getaffinity(0))
-    return {"value": (1.0 / dt) * f1 / f2, "unit": "sigma/s", "cores": cores,
-            "kind": "reference",
-            "sample": "libfci_gas (reference C, OpenMP, -O3 -march=x86-64-v3) FP64 sigma of "
-                      "(14o,14e) [11.8M dets], %d runs of %.2f s, scaled to (16o,16e) by "
-                      "F_sigma ratio %.4f" % (reps, dt, f1 / f2),
-            "sigma_s_14o": dt}
+    e12, _ = s.kernel(h1e, eri, n, ne, max_cycle=12, pspace_size=0)
+    sync()
+    d12 = time.perf_counter() - t0
+    dav = davidson_precisions(pkg, sync, h1e, eri, n, ne, 100)
+    out = {"workload": "CAS(14o,14e) nelec (7,7), 11,778,624 dets, synthetic integrals seed 0",
+           "sigma": recs, "davidson12_fp64_s": d12, "davidson12_energy": float(e12),
+           "davidson": dav}
+    log("C1: sigma %s, 12 iterations %.3f s, Davidson %s" % (
+        {p: (round(r["ms"], 2), r["rel_err_vs_fp64"]) for p, r in recs.items()}, d12,
+        {p: (round(r["seconds"], 2), r["dE_vs_fp64"]) for p, r in dav.items()}))
+    host = {"norb": n, "nelec": ne, "h1e": h1e, "eri": eri, "h2e": h2e, "c": c.cpu().numpy(),
+            "sigma64": s64.cpu().numpy(), "e12": float(e12)}
+    del c, s64
+    torch.cuda.empty_cache()
+    return out, host
+
+
+def config4(pkg, sync):
+    """C4: CAS(12o,12e), (6,6), 853,776 dets: E(h1e, eri) = c^T H c through
+    the autograd Function at the FP64 ground state, backward = RDM kernel;
+    gradient against a central finite difference along one random
+    symmetric perturbation (step 1e-5), FP32 integrals too."""
+    import numpy
+    import torch
+    n, ne = 12, (6, 6)
+    h1e, eri = pkg.synthetic.make_problem(n, ne, seed=0, with_ci=False)
+    s = pkg.FCISolver()
+    e, ci = s.kernel(h1e, eri, n, ne)
+    c = torch.from_numpy(numpy.ascontiguousarray(ci).reshape(-1)).cuda()
+    out = {"workload": "CAS(12o,12e) nelec (6,6), 853,776 dets, synthetic integrals seed 0",
+           "energy": float(e)}
+    for dt_name, dtype in (("fp64", torch.float64), ("fp32", torch.float32)):
+        h = torch.tensor(h1e, dtype=dtype, requires_grad=True)
+        g = torch.tensor(eri, dtype=dtype, requires_grad=True)
+        sync()
+        t0 = time.perf_counter()
+        E = pkg.autograd.ci_energy(h, g, c, n, ne)
+        sync()
+        t1 = time.perf_counter()
+        E.backward()
+        sync()
+        t2 = time.perf_counter()
+        rng = numpy.random.default_rng(5)
+        dh = rng.normal(size=h1e.shape)
+        dh = 0.5 * (dh + dh.T)
+        step = 1e-5
+        ep = float(pkg.autograd.ci_energy(torch.tensor(h1e + step * dh), torch.tensor(eri), c, n, ne))
+        em = float(pkg.autograd.ci_energy(torch.tensor(h1e - step * dh), torch.tensor(eri), c, n, ne))
+        fd = (ep - em) / (2 * step)
+        an = float((h.grad.double() * torch.tensor(dh)).sum())
+        out[dt_name] = {"forward_s": t1 - t0, "backward_rdm_s": t2 - t1,
+                        "E_minus_solver": abs(float(E.detach()) - float(e)),
+                        "dE_dh_fd_rel_diff": abs(an - fd) / max(abs(fd), 1e-30)}
+    dm1, dm2 = pkg.make_rdm12(ci, n, ne)
+    log("C4: %s" % out)
+    host = {"norb": n, "nelec": ne, "ci": numpy.ascontiguousarray(ci).reshape(-1),
+            "dm1": numpy.asarray(dm1), "dm2": numpy.asarray(dm2)}
+    return out, host
 
 
 def batch_solve(pkg, sync):
@@ -144,7 +404,8 @@ def batch_solve(pkg, sync):
         h1e, eri = pkg.synthetic.make_problem(n, ne, seed=s, with_ci=False)
         probs.append((h1e, eri, n, ne))
         nd = math.comb(n, ne[0]) * math.comb(n, ne[1])
-        F += 4 * 2.0 * (n * (n + 1) // 2) ** 2 * nd
+        # four roots and the solver's guard root per problem
+        F += 5 * 2.0 * (n * (n + 1) // 2) ** 2 * nd
     sync()
     t0 = time.perf_counter()
     res = pkg.batch.solve_batch(probs, nroots=4, max_cycle=300)
@@ -166,8 +427,19 @@ def batch_solve(pkg, sync):
         "sigma_all_slots_TFLOP/s": tf,
         "sigma_frac_of_fp64_peak": None if tf is None else tf / PEAK["fp64"],
     }
+    fx = os.path.join(ROOT, "tests", "golden", "batch", "manifest_eigs.npz")
+    if os.path.exists(fx):
+        # lowest eigenvalues of every problem (dense eigh / Lanczos, committed data)
+        with numpy.load(fx) as f:
+            exact = f["e"]
+        if numpy.isfinite(exact).all():
+            d = numpy.array([numpy.abs(numpy.asarray(e) - exact[p, :len(e)]).max()
+                             for p, e in enumerate(res.e)])
+            out["dE_vs_exact_max"] = float(d.max())
+            out["problems_off_exact_1e-6"] = int((d > 1e-6).sum())
     log("batch: %d problems in %.2f s (%.2f problems/s), %d converged, sigma(all slots) %.2f ms = "
-        "%.1f TFLOP/s" % (len(probs), dt, len(probs) / dt, nconv, st["tuned_ms"], tf or 0.0))
+        "%.1f TFLOP/s, max |dE| vs exact %s" % (len(probs), dt, len(probs) / dt, nconv,
+                                                st["tuned_ms"], tf or 0.0, out.get("dE_vs_exact_max")))
     return out
 
 
@@ -314,6 +586,8 @@ def main():
                     help="skip the CSF-basis (singlet) Davidson at the same size")
     ap.add_argument("--no-gas", action="store_true",
                     help="skip the restricted (RAS) space inside the same CAS")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the CAS(8o,8e), CAS(14o,14e) and CAS(12o,12e) autograd sections")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 4 GiB device copy first (HBM counter calibration)")
     args = ap.parse_args()
@@ -349,41 +623,17 @@ def main():
     g = torch.Generator(device="cuda").manual_seed(0)
     c = torch.randn(sp.ndet, dtype=torch.float64, device="cuda", generator=g)
     c /= torch.linalg.vector_norm(c)
-    out = torch.empty_like(c)
-    npair = n * (n + 1) // 2
-    F = 2.0 * npair * npair * sp.ndet
-
-    result = {}
-    kern = {}
-    sigma64 = None
     precs = [p for p in args.precisions.split(",") if p]
-    for prec in precs:
-        free = torch.cuda.mem_get_info()[0]
-        plan = pkg.SigmaPlan(sp, h2e, prec, int(free * 0.85))
-        plan._bench_in, plan._bench_out = c, out
-        for _ in range(args.warmup):
-            plan(c, out)
-        sync()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            plan(c, out)
-        sync()
-        dt = (time.perf_counter() - t0) / args.steps
-        kern[prec] = kernel_roofline(plan, sp, prec, 2, sync)
-        kern[prec]["ntiles"] = plan.ntiles
-        if prec == "fp64":
-            sigma64 = out.clone()
-            err = 0.0
-        else:
-            err = float(torch.linalg.vector_norm(out - sigma64) / torch.linalg.vector_norm(sigma64)) \
-                if sigma64 is not None else None
-        result[prec] = {"ms": dt * 1e3, "sigma_per_s": 1.0 / dt,
-                        "TFLOP/s": F / dt / 1e12, "rel_err_vs_fp64": err}
+    result, sigma64, kern = sigma_precisions(pkg, sp, h2e, c, args.steps, args.warmup, sync,
+                                             with_roofline=True, precisions=precs)
+    for prec, r in result.items():
         log("%s sigma %.2f ms (%.2f TFLOP/s eff.), contraction %.2f TFLOP/s, err %s" % (
-            prec, dt * 1e3, F / dt / 1e12, kern[prec]["contract"]["TFLOP/s"], err))
-        plan.close()
-        del plan
-    del out, c, sigma64
+            prec, r["ms"], r["TFLOP/s"], kern[prec]["contract"]["TFLOP/s"], r["rel_err_vs_fp64"]))
+    gpu_host = {}
+    if not args.no_cpu and sigma64 is not None:
+        gpu_host["headline"] = {"norb": n, "nelec": ne, "h2e": h2e, "c": c.cpu().numpy(),
+                                "sigma64": sigma64.cpu().numpy()}
+    del c, sigma64
     torch.cuda.empty_cache()
 
     dav = None
@@ -410,6 +660,12 @@ def main():
             del ci, solver
             torch.cuda.empty_cache()
 
+    configs = {}
+    if not args.no_configs:
+        configs["c0"], gpu_host["c0"] = config0(pkg, sync)
+        configs["c1"], gpu_host["c1"] = config1(pkg, sync)
+        configs["c4"], gpu_host["c4"] = config4(pkg, sync)
+
     batch = None
     if not args.no_batch:
         batch = batch_solve(pkg, sync)
@@ -426,7 +682,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (reference libfci_gas on host cores)")
-        cpu = cpu_baseline(pkg)
+        cpu = cpu_baseline(pkg, gpu_host, sync)
+    del gpu_host
 
     main_k = kern["fp64"]["contract"] if "fp64" in kern else None
     main_name = None
@@ -467,6 +724,7 @@ def main():
         "batch": batch,
         "csf": csf,
         "gas": gas,
+        "configs": configs,
     }
     print(json.dumps(line), flush=True)
 
