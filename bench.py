@@ -45,9 +45,10 @@ def kernel_roofline(plan, sp, prec, calls, sync):
     """Per-kernel averages from the plan's HIP-event timers, with the
     algorithmic work per launch:
       contraction  F = 2 npair^2 dets                          (MFMA bound)
-      fused path   reads c (8 B/det), writes walpha (s nlinka B/det) and
-                   sigma (8 B/det); alpha pull reads walpha and sigma and
-                   writes sigma: (s nlinka + 16) B/det          (HBM bound)
+      fused path   reads c (8 B/det), writes walpha (s (nlinka - nalpha)
+                   B/det: the off-diagonal alpha links) and sigma (8 B/det);
+                   alpha pull reads walpha and sigma and writes sigma:
+                   (s (nlinka - nalpha) + 16) B/det              (HBM bound)
       3-kernel     gather  8 + s npair B/det, scatter s npair + 8 B/det
     with s = 8 (FP64) or 4 (FP32 / bf16x3 intermediates)."""
     plan.set_timing(True)
@@ -65,8 +66,9 @@ def kernel_roofline(plan, sp, prec, calls, sync):
     flop = 2.0 * npair * npair * dets
     out = {"launches_per_sigma": per_sigma_launch.tolist(), "fused": bool(plan.fused)}
     if plan.fused:
-        fused_bytes = (16.0 + s * sp.nlinka) * dets
-        pull_bytes = (s * sp.nlinka + 16.0) * dets
+        nws = sp.nlinka - sp.nelec[0]
+        fused_bytes = (16.0 + s * nws) * dets
+        pull_bytes = (s * nws + 16.0) * dets
         out["k_fused_row"] = {"ms": avg[1], "TFLOP/s": flop / avg[1] / 1e9,
                               "frac_mfma": flop / avg[1] / 1e9 / PEAK[prec],
                               "flop_per_launch": flop, "dets_per_launch": dets,
