@@ -121,6 +121,12 @@ class HostOps:
     def combine(self, n, vecs, coef, y):
         y[...] = sum(c * v.reshape(-1) for c, v in zip(coef, vecs))
 
+    def combine_many(self, n, tasks):
+        import numpy
+        for vecs, outs, coef in tasks:
+            for j, o in enumerate(outs):
+                o[...] = sum(c * v.reshape(-1) for c, v in zip(numpy.asarray(coef)[:, j], vecs))
+
 
 def rel(a, b):
     import numpy

@@ -74,6 +74,18 @@ class DeviceOps:
         coef = numpy.ascontiguousarray(coef, dtype=numpy.float64).reshape(len(vecs), 1)
         self.ops.gemm([(rows, [y.data_ptr()], int(n), coef, 0.0)])
 
+    def combine_many(self, n, tasks):
+        """For each (vecs, outs, coef): outs[j] = sum_i coef[i, j] vecs[i]
+        (outputs distinct from the inputs), all tasks in one launch that
+        streams each task's inputs once."""
+        blocks = []
+        for vecs, outs, coef in tasks:
+            rows = numpy.array([v.data_ptr() for v in vecs], dtype=numpy.int64)
+            cols = [o.data_ptr() for o in outs]
+            c = numpy.ascontiguousarray(coef, dtype=numpy.float64).reshape(len(vecs), len(outs))
+            blocks.append((rows, cols, int(n), c, 0.0))
+        self.ops.gemm(blocks)
+
 
 def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
              max_cycle=50, max_space=12, lindep=1e-14, verbose=False,
@@ -119,7 +131,7 @@ def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
     axr = ops.empty((nroots, n), x0[0])
     xp = ops.empty((nroots, n), x0[0])      # previous Ritz directions (restart)
     axp = ops.empty((nroots, n), x0[0])
-    r = ops.empty((n,), x0[0])
+    R = ops.empty((nroots, n), x0[0])      # residuals of the Ritz pairs
     v_prev = None
     heff = numpy.zeros((cap, cap))
     space = 0
@@ -169,12 +181,33 @@ def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
             h[:nexact, nexact:] = heff[nexact:space, :nexact].T
         return h
 
-    def residual_norms(e):
-        """|AX v_k - e_k X v_k| of the Ritz pairs in xr / axr."""
+    def ritz(v, e):
+        """xr_k = X v_k, axr_k = AX v_k and R_k = axr_k - e_k xr_k, one pass
+        over X and AX for all roots (combine_many) where the ops have it."""
+        if getattr(ops, "combine_many", None) is not None:
+            vecs = [X[i] for i in range(space)] + [AX[i] for i in range(space)]
+            tasks = []
+            for k in range(nroots):
+                c = numpy.zeros((2 * space, 3))
+                c[:space, 0] = v[:, k]
+                c[space:, 1] = v[:, k]
+                c[:space, 2] = -e[k] * v[:, k]
+                c[space:, 2] = v[:, k]
+                tasks.append((vecs, (xr[k], axr[k], R[k]), c))
+            ops.combine_many(n, tasks)
+        else:
+            for k in range(nroots):
+                ops.lincomb(space, n, X, v[:, k], 0.0, xr[k])
+                ops.lincomb(space, n, AX, v[:, k], 0.0, axr[k])
+                ops.combine(n, (axr[k], xr[k]), numpy.array([1.0, -e[k]]), R[k])
+
+    def residual_norms(e, recompute):
+        """|R_k| (R_k = axr_k - e_k xr_k formed again when ``recompute``)."""
         out = numpy.empty(nroots)
         for k in range(nroots):
-            ops.combine(n, (axr[k], xr[k]), numpy.array([1.0, -e[k]]), r)
-            out[k] = math.sqrt(max(ops.dots(1, n, r, r)[0], 0.0))
+            if recompute:
+                ops.combine(n, (axr[k], xr[k]), numpy.array([1.0, -e[k]]), R[k])
+            out[k] = math.sqrt(max(ops.dots(1, n, R[k], R[k])[0], 0.0))
         return out
 
     for g in x0:
@@ -194,11 +227,9 @@ def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
         w, v = numpy.linalg.eigh(h)
         e = w[:nroots].copy()
         v = v[:, :nroots]
-        for k in range(nroots):
-            ops.lincomb(space, n, X, v[:, k], 0.0, xr[k])
-            ops.lincomb(space, n, AX, v[:, k], 0.0, axr[k])
+        ritz(v, e)
         de = e - e_old
-        rnorm[:] = residual_norms(e)
+        rnorm[:] = residual_norms(e, False)
         conv = (numpy.abs(de) < stage_tol) & (rnorm < stage_res)
         checked = False
         if mixed and (conv.all() or (refining and icyc - last_check >= check_every)
@@ -209,7 +240,7 @@ def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
                 hop_check(xr[k], axr[k])
             nhop_check += nroots
             e_chk = numpy.array([ops.dots(1, n, xr[k], axr[k])[0] for k in range(nroots)])
-            rnorm[:] = residual_norms(e_chk)
+            rnorm[:] = residual_norms(e_chk, True)
             conv = (numpy.abs(e_chk - e) < tol) & (rnorm < tol_residual) if refining \
                 else numpy.zeros(nroots, dtype=bool)
             de = e_chk - e_old
@@ -219,9 +250,8 @@ def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
         new_vectors = []
         for k in range(nroots):
             if not conv[k]:
-                ops.combine(n, (axr[k], xr[k]), numpy.array([1.0, -e[k]]), r)
                 t = ops.empty((n,), x0[0])
-                precond(r, e[k], t, xr[k])
+                precond(R[k], e[k], t, xr[k])
                 new_vectors.append(t)
         if verbose:
             print("davidson %d  space %d  e %s  de %s  |r| %s%s" % (

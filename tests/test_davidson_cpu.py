@@ -37,6 +37,18 @@ class NumpyOps:
     def combine(self, n, vecs, coef, y):
         y[...] = sum(c * v.reshape(-1) for c, v in zip(coef, vecs))
 
+    def combine_many(self, n, tasks):
+        for vecs, outs, coef in tasks:
+            V = numpy.stack([v.reshape(-1) for v in vecs])
+            for j, o in enumerate(outs):
+                o[...] = numpy.asarray(coef)[:, j] @ V
+
+
+class PlainOps(NumpyOps):
+    """The same without the one-pass Ritz/residual combination (the
+    per-root fallback of the driver)."""
+    combine_many = None
+
 
 def _problem(n, seed, spread=10.0):
     rng = numpy.random.default_rng(seed)
@@ -46,7 +58,7 @@ def _problem(n, seed, spread=10.0):
     return H
 
 
-def _run(H, nroots, max_space, tol=1e-12):
+def _run(H, nroots, max_space, tol=1e-12, ops=None):
     n = H.shape[0]
     hdiag = numpy.diag(H).copy()
 
@@ -64,14 +76,15 @@ def _run(H, nroots, max_space, tol=1e-12):
         g[k] = 1.0
         guesses.append(g)
     return dav.davidson(hop, precond, guesses, nroots=nroots, tol=tol, max_cycle=200,
-                        max_space=max_space, ops=NumpyOps())
+                        max_space=max_space, ops=ops if ops is not None else NumpyOps())
 
 
-@pytest.mark.parametrize("n,nroots,max_space,seed", [(200, 1, 12, 1), (300, 3, 12, 2),
-                                                      (150, 4, 6, 3), (400, 2, 8, 4)])
-def test_davidson_eigenvalues_vs_eigh(n, nroots, max_space, seed):
+@pytest.mark.parametrize("n,nroots,max_space,seed,ops", [(200, 1, 12, 1, NumpyOps), (300, 3, 12, 2, NumpyOps),
+                                                          (150, 4, 6, 3, NumpyOps), (400, 2, 8, 4, NumpyOps),
+                                                          (300, 3, 12, 2, PlainOps)])
+def test_davidson_eigenvalues_vs_eigh(n, nroots, max_space, seed, ops):
     H = _problem(n, seed)
-    res = _run(H, nroots, max_space)
+    res = _run(H, nroots, max_space, ops=ops())
     w = numpy.linalg.eigh(H)[0][:nroots]
     assert res.converged.all()
     numpy.testing.assert_allclose(res.e, w, rtol=0, atol=1e-9)
