@@ -121,6 +121,10 @@ class HostOps:
     def combine(self, n, vecs, coef, y):
         y[...] = sum(c * v.reshape(-1) for c, v in zip(coef, vecs))
 
+    def dots_list(self, n, vecs, v):
+        import numpy
+        return numpy.array([float(u.reshape(-1) @ v.reshape(-1)) for u in vecs])
+
     def combine_many(self, n, tasks):
         import numpy
         for vecs, outs, coef in tasks:

@@ -74,6 +74,11 @@ class DeviceOps:
         coef = numpy.ascontiguousarray(coef, dtype=numpy.float64).reshape(len(vecs), 1)
         self.ops.gemm([(rows, [y.data_ptr()], int(n), coef, 0.0)])
 
+    def dots_list(self, n, vecs, v):
+        """[u . v for u in vecs] in one launch (vecs anywhere in memory)."""
+        rows = numpy.array([u.data_ptr() for u in vecs], dtype=numpy.int64)
+        return self.ops.gram([(rows, [v.data_ptr()], int(n))])[0][:, 0].copy()
+
     def combine_many(self, n, tasks):
         """For each (vecs, outs, coef): outs[j] = sum_i coef[i, j] vecs[i]
         (outputs distinct from the inputs), all tasks in one launch that
@@ -143,27 +148,59 @@ def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
     # approximate j is taken from the exact image, heff[j, i]
     nexact = 0
 
-    def add_vector(t):
-        """Orthonormalise t against X[:space] (Gram-Schmidt, twice) and
-        append it with its image.  Returns False if t is linearly dependent."""
-        nonlocal space, nhop
-        norm0 = math.sqrt(ops.dots(1, n, t, t)[0])
-        if norm0 == 0.0:
-            return False
-        norm = norm0
-        for _ in range(2):
+    fused_gs = getattr(ops, "dots_list", None) is not None and \
+        getattr(ops, "combine_many", None) is not None
+
+    def project(t, norm, passes):
+        """The remaining classical Gram-Schmidt passes on t (two in all, the
+        second only where the first cancelled much of t: Daniel-Gragg-
+        Kaufman-Stewart, |t'| < |t| / sqrt(2)); returns |t|."""
+        for _ in range(passes, 2):
             if not space:
                 break
             c = ops.dots(space, n, X, t)
             ops.lincomb(space, n, X, -c, 1.0, t)
             before, norm = norm, math.sqrt(max(ops.dots(1, n, t, t)[0], 0.0))
-            # a second pass only where the first cancelled much of t
-            # (Daniel-Gragg-Kaufman-Stewart: |t'| < |t| / sqrt(2))
             if norm >= before * 0.7071067811865476:
                 break
-        if norm * norm < lindep or norm < 1e-7 * norm0:
-            return False
-        ops.lincomb(1, n, t, numpy.array([1.0 / norm]), 0.0, X[space])
+        return norm
+
+    def add_vector(t):
+        """Orthonormalise t against X[:space] (Gram-Schmidt, twice) and
+        append it with its image.  Returns False if t is linearly dependent."""
+        nonlocal space, nhop
+        written = False
+        if fused_gs and space:
+            # X^T t and |t|^2 in one pass.  Where one projection suffices
+            # (|t - X c|^2 = |t|^2 - |c|^2 >= |t|^2 / 2, the test of the
+            # second pass) the normalised projection is written in one more
+            # pass; otherwise the first projection is applied and the second
+            # pass follows as usual.
+            g = ops.dots_list(n, [X[i] for i in range(space)] + [t], t)
+            c, tt = g[:space], float(g[space])
+            if tt <= 0.0:
+                return False
+            norm0 = math.sqrt(tt)
+            rest = tt - float(c @ c)
+            if rest >= 0.5 * tt:
+                norm = math.sqrt(rest)
+                if norm * norm < lindep or norm < 1e-7 * norm0:
+                    return False
+                coef = numpy.concatenate([-c, [1.0]]) / norm
+                ops.combine_many(n, [([X[i] for i in range(space)] + [t], (X[space],), coef[:, None])])
+                written = True
+            else:
+                ops.lincomb(space, n, X, -c, 1.0, t)
+                norm = project(t, math.sqrt(max(ops.dots(1, n, t, t)[0], 0.0)), 1)
+        else:
+            norm0 = math.sqrt(ops.dots(1, n, t, t)[0])
+            if norm0 == 0.0:
+                return False
+            norm = project(t, norm0, 0)
+        if not written:
+            if norm * norm < lindep or norm < 1e-7 * norm0:
+                return False
+            ops.lincomb(1, n, t, numpy.array([1.0 / norm]), 0.0, X[space])
         hop(X[space], AX[space])
         nhop += 1
         col = ops.dots(space + 1, n, X, AX[space])
@@ -329,6 +366,7 @@ def make_pspace_precond(hdiag, pw, pv, addr, level_shift=1e-3, ops=None):
     pv = numpy.asarray(pv, dtype=numpy.float64)
     addr = addr.to(device=hdiag.device, dtype=torch.int64)
     d1 = torch.empty_like(hdiag)
+    blocks = getattr(ops, "ops", None)      # DeviceOps: its BlockOps (precond lists)
 
     def diag_solve(v, e, out):
         _lib.check(lib.fcis_precond(n, _lib.vptr(hdiag), float(e), 1e-8, _lib.vptr(v),
@@ -336,13 +374,23 @@ def make_pspace_precond(hdiag, pw, pv, addr, level_shift=1e-3, ops=None):
 
     def precond(r, e, out, x0):
         es = float(e) - level_shift
-        diag_solve(x0, es, d1)
-        diag_solve(r, es, out)
+        if blocks is not None:
+            # both diagonal solves in one launch, both dots in another
+            hp = hdiag.data_ptr()
+            blocks.precond([(hp, x0.data_ptr(), d1.data_ptr(), n, es),
+                            (hp, r.data_ptr(), out.data_ptr(), n, es)], 1e-8)
+        else:
+            diag_solve(x0, es, d1)
+            diag_solve(r, es, out)
         g = torch.stack([x0[addr], r[addr], d1[addr], out[addr]]).cpu().numpy()
         xa, ra, d1a, d2a = g
         inv = (pv / (pw - es)) @ pv.T
-        x0_h0r = ops.dots(1, n, x0, out)[0] - xa @ d2a + xa @ (inv @ ra)
-        x0_h0x0 = ops.dots(1, n, x0, d1)[0] - xa @ d1a + xa @ (inv @ xa)
+        if getattr(ops, "dots_list", None) is not None:
+            x0_out, x0_d1 = ops.dots_list(n, [out, d1], x0)
+        else:
+            x0_out, x0_d1 = ops.dots(1, n, x0, out)[0], ops.dots(1, n, x0, d1)[0]
+        x0_h0r = x0_out - xa @ d2a + xa @ (inv @ ra)
+        x0_h0x0 = x0_d1 - xa @ d1a + xa @ (inv @ xa)
         e1 = x0_h0r / x0_h0x0
         ops.lincomb(1, n, d1, numpy.array([-e1]), 1.0, out)
     return precond

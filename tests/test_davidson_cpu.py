@@ -37,6 +37,9 @@ class NumpyOps:
     def combine(self, n, vecs, coef, y):
         y[...] = sum(c * v.reshape(-1) for c, v in zip(coef, vecs))
 
+    def dots_list(self, n, vecs, v):
+        return numpy.array([float(u.reshape(-1) @ v.reshape(-1)) for u in vecs])
+
     def combine_many(self, n, tasks):
         for vecs, outs, coef in tasks:
             V = numpy.stack([v.reshape(-1) for v in vecs])
@@ -45,9 +48,10 @@ class NumpyOps:
 
 
 class PlainOps(NumpyOps):
-    """The same without the one-pass Ritz/residual combination (the
-    per-root fallback of the driver)."""
+    """The same without the one-pass Ritz/residual combination and the
+    one-pass Gram-Schmidt (the driver's fallbacks)."""
     combine_many = None
+    dots_list = None
 
 
 def _problem(n, seed, spread=10.0):
