@@ -67,6 +67,13 @@ struct FusedTypes<FCIS_PREC_BF16X3> {
 // element itself.
 constexpr int FUSED_KPS_BF16 = FUSED_KS_BF16 * 32 + 8;
 
+// The bf16x3 fused kernel's diagonal tile: A fragments (bf16 high, then low
+// parts, [k-step][lane]) of the 16 rows h[pp][k], p < norb, kept in LDS.
+constexpr size_t fused_dtile_bytes()
+{
+        return 2 * sizeof(bf16x8) * FUSED_KS_BF16 * 64;
+}
+
 // Bytes of one B buffer of the fused row kernel.
 template <int PREC>
 constexpr size_t fused_bbytes()
@@ -82,7 +89,8 @@ constexpr size_t fused_lds_bytes(int64_t nb)
 {
         return (size_t)((nb + FUSED_NB - 1) / FUSED_NB * FUSED_NB) * sizeof(double) + 2 * fused_bbytes<PREC>() +
                2 * sizeof(int) * 16 * FUSED_MAXW +
-               FUSED_NMAP * sizeof(int16_t) * 16 * FUSED_MAXW * FUSED_NB;
+               FUSED_NMAP * sizeof(int16_t) * 16 * FUSED_MAXW * FUSED_NB +
+               (PREC == FCIS_PREC_BF16X3 ? fused_dtile_bytes() : 0);
 }
 
 // Builder wavefronts: gather the B tile of chunk j+1 (pair maps, then c)
@@ -454,6 +462,28 @@ __device__ __forceinline__ typename FusedTypes<PREC>::Acc mma_steps(
         }
 }
 
+// One 16 x 16 bf16x3 block whose A fragments come from LDS (af: KS high
+// parts then KS low parts, [k-step][lane]) against the pre-split B tile, the
+// same three products and accumulation order as mma_presplit.
+template <int KS>
+__device__ __forceinline__ v4f mma_presplit_lds(const bf16x8 *__restrict__ af, const __bf16 *bh, int lane,
+                                                int lk, int li)
+{
+        constexpr int KP = FUSED_KPS_BF16;
+        const __bf16 *bl = bh + FUSED_NB * KP;
+        v4f a0c = {0, 0, 0, 0}, a1c = a0c, a2c = a0c;
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+                const bf16x8 ahk = af[ks * 64 + lane], alk = af[(KS + ks) * 64 + lane];
+                const bf16x8 h = *reinterpret_cast<const bf16x8 *>(bh + li * KP + 32 * ks + 8 * lk);
+                const bf16x8 l = *reinterpret_cast<const bf16x8 *>(bl + li * KP + 32 * ks + 8 * lk);
+                a0c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alk, h, a0c, 0, 0, 0);
+                a1c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahk, l, a1c, 0, 0, 0);
+                a2c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahk, h, a2c, 0, 0, 0);
+        }
+        return (a0c + a1c) + a2c;
+}
+
 // mma_steps for the bf16x3 fused kernel, whose B tile the builders stored
 // pre-split and column-major (fb_commit_split): same three products, same
 // accumulation order, no conversions on the MFMA wavefronts.
@@ -501,6 +531,7 @@ struct fcis_sigma_plan {
         // mrow_f for the alpha occupations' diagonal term, zero padding
         int mtiles_f, mrow_f;
         void *d_hfrag_f, *d_hfrag_f_lo;
+        void *d_dtile;               // bf16x3: A fragments of the diagonal pairs' rows (hi, lo)
         int16_t *d_mpair;            // [16 * FUSED_MTILES] pair of each fused row
         double *d_falpha;            // [nstr_a][npair]: sum_{p in I} h[pp][k]
         void *d_fbeta;               // [npair][nbp]:    sum_{p in b} h[pp][k] (double, float for

@@ -306,7 +306,8 @@ __global__ __launch_bounds__(64 * (FUSED_MTILES + FUSED_NBLD)) void k_fused_row(
         const double *__restrict__ falpha, const typename FusedTypes<PREC>::G *__restrict__ fbeta,
         typename FusedTypes<PREC>::Out *__restrict__ walpha,
         double *__restrict__ sigma, const int32_t *__restrict__ rows,
-        const int2 *__restrict__ rowcl, const uint16_t *__restrict__ clist)
+        const int2 *__restrict__ rowcl, const uint16_t *__restrict__ clist,
+        const bf16x8 *__restrict__ dtile, const uint64_t *__restrict__ strb)
 {
         typedef FusedTypes<PREC> Ty;
         typedef typename Ty::B BT;
@@ -328,6 +329,14 @@ __global__ __launch_bounds__(64 * (FUSED_MTILES + FUSED_NBLD)) void k_fused_row(
         uint32_t *abyte = reinterpret_cast<uint32_t *>(bbase + 2 * BB);
         int *aslot = reinterpret_cast<int *>(abyte + MAXP);        // walpha slot of pq, or -1
         int16_t *maps = reinterpret_cast<int16_t *>(aslot + MAXP); // FUSED_NMAP x [MAXP][NB] beta maps
+        // bf16x3: the beta-occupation part of the diagonal pairs runs on the
+        // matrix pipe (which has room in this kernel): the pair-map wavefront
+        // multiplies the 16 rows h[pp][k] (A fragments from LDS) into each
+        // chunk's tile and adds sum_{p in b} (h[pp] t1)[p][b] to sigma row I;
+        // the other kernels form sum_k fBeta[k][b] t1[k][b] in the builders
+        constexpr bool DTILE = PREC == FCIS_PREC_BF16X3;
+        constexpr bool BDIAG = !DTILE;
+        bf16x8 *dlds = reinterpret_cast<bf16x8 *>(maps + FUSED_NMAP * MAXP * NB);
 
         const int tid = threadIdx.x, nthr = blockDim.x;
         const __amdgpu_buffer_rsrc_t rc = fused_c_rsrc(c, (uint32_t)(nstra * nb * (int64_t)sizeof(G)));
@@ -357,6 +366,8 @@ __global__ __launch_bounds__(64 * (FUSED_MTILES + FUSED_NBLD)) void k_fused_row(
                 }
         }
         for (int64_t i = tid; i < nb; i += nthr) srow[i] = 0.0;
+        if constexpr (DTILE)
+                for (int e = tid; e < 2 * KS * 64; e += nthr) dlds[e] = dtile[e];
 
         // Barrier schedule per row (every role): (1) row start, (2) abyte,
         // (3) aslot + maps of chunks 0..2, (4) chunk 0 built, then one per chunk.
@@ -417,9 +428,28 @@ __global__ __launch_bounds__(64 * (FUSED_MTILES + FUSED_NBLD)) void k_fused_row(
                                 issue(m0, 3);
                                 __syncthreads();   // (3)
                                 __syncthreads();   // (4)
+                                // bf16x3: the diagonal tile of chunk j (in B buffer j & 1
+                                // this phase), rows p = 4 lk + q, column li
+                                auto diag_tile = [&](int64_t j) {
+                                        if constexpr (DTILE) {
+                                                const int64_t b = cc(j) * NB + li;
+                                                const uint64_t occ = strb[b < nb ? b : nb - 1];
+                                                const v4f dv = mma_presplit_lds<KS>(
+                                                        dlds, reinterpret_cast<const __bf16 *>((j & 1) ? buf1 : buf0), lane,
+                                                        lk, li);
+                                                float part = 0.0f;
+#pragma unroll
+                                                for (int q = 0; q < 4; q++)
+                                                        part += ((occ >> (4 * lk + q)) & 1u) ? dv[q] : 0.0f;
+                                                part += __shfl_xor(part, 16);
+                                                part += __shfl_xor(part, 32);
+                                                if (lk == 0 && b < nb) atomicAdd(&srow[b], (double)part);
+                                        }
+                                };
                                 auto mphase = [&](int64_t j, v4i(&ld_)[EMW], v4i(&st_)[EMW]) {
                                         issue(ld_, j + 4);
                                         store(st_, j + 3);
+                                        diag_tile(j);
                                         __syncthreads();
                                 };
                                 int64_t j = 0;
@@ -488,34 +518,32 @@ __global__ __launch_bounds__(64 * (FUSED_MTILES + FUSED_NBLD)) void k_fused_row(
                         // loads left in flight here, oldest first: c of chunk 1 -- the
                         // order the loop body leaves them in, so that the loop head sees
                         // one order and every wait stays partial
-                        fg_issue<NBA, FUSED_MAXW, true>(s0, rowI_byte, cc(0), bt, abyte, mbuf(0), rc, fbr, nbp);
-                        fg_issue<NBA, FUSED_MAXW, true>(s1, rowI_byte, cc(1), bt, abyte, mbuf(1), rc, fbr, nbp);
+                        fg_issue<NBA, FUSED_MAXW, BDIAG>(s0, rowI_byte, cc(0), bt, abyte, mbuf(0), rc, fbr, nbp);
+                        fg_issue<NBA, FUSED_MAXW, BDIAG>(s1, rowI_byte, cc(1), bt, abyte, mbuf(1), rc, fbr, nbp);
                         if constexpr (PRESPLIT)
-                                add_diag(fg_commit_split<NBA, FUSED_MAXW, true>(s0, npair, bt,
-                                                                                reinterpret_cast<__bf16 *>(buf0)),
-                                         cc(0) * NB);
+                                fg_commit_split<NBA, FUSED_MAXW, BDIAG>(s0, npair, bt, reinterpret_cast<__bf16 *>(buf0));
                         else
-                                add_diag(fg_commit<NBA, FUSED_MAXW, true>(s0, npair, bt, buf0), cc(0) * NB);
+                                add_diag(fg_commit<NBA, FUSED_MAXW, BDIAG>(s0, npair, bt, buf0), cc(0) * NB);
                         __syncthreads();   // (4)
                         // phase j: `issue` issues the c loads of chunk j+2 (its map
                         // slice is in LDS); `commit` holds the c values of chunk j+1
                         // (issued one phase earlier), committed now.  The loads of
                         // this phase stay in flight across the barrier.
                         auto phase = [&](int64_t j, Gather &issue, Gather &commit) {
-                                fg_issue<NBA, FUSED_MAXW, true>(issue, rowI_byte, cc(j + 2), bt, abyte, mbuf(j + 2), rc,
-                                                                fbr, nbp);
+                                fg_issue<NBA, FUSED_MAXW, BDIAG>(issue, rowI_byte, cc(j + 2), bt, abyte, mbuf(j + 2), rc,
+                                                                 fbr, nbp);
                                 // unconditional: past the row's last chunk the commit goes
                                 // to the B buffer that no wavefront reads in this phase
                                 // (clamped chunk data); its diagonal term is dropped
-                                double d;
-                                if constexpr (PRESPLIT)
-                                        d = fg_commit_split<NBA, FUSED_MAXW, true>(
+                                if constexpr (PRESPLIT) {
+                                        fg_commit_split<NBA, FUSED_MAXW, BDIAG>(
                                                 commit, npair, bt,
                                                 reinterpret_cast<__bf16 *>(((j + 1) & 1) ? buf1 : buf0));
-                                else
-                                        d = fg_commit<NBA, FUSED_MAXW, true>(commit, npair, bt,
-                                                                             ((j + 1) & 1) ? buf1 : buf0);
-                                if (j + 1 < nl) add_diag(d, cc(j + 1) * NB);
+                                } else {
+                                        const double d = fg_commit<NBA, FUSED_MAXW, BDIAG>(commit, npair, bt,
+                                                                                           ((j + 1) & 1) ? buf1 : buf0);
+                                        if (j + 1 < nl) add_diag(d, cc(j + 1) * NB);
+                                }
                                 __syncthreads();
                         };
                         int64_t j = 0;
@@ -902,6 +930,16 @@ int upload_h(fcis_sigma_plan *p, const double *h2e)
         if (st != FCIS_SUCCESS) return st;
         const fcis_space *s = p->space;
         const int n = s->norb;
+        if (p->precision == FCIS_PREC_BF16X3) {
+                // the diagonal pairs' rows for the bf16x3 kernel's diagonal tile
+                // (k-steps past the problem's stay zero)
+                std::vector<int> diag(n);
+                for (int q = 0; q < n; q++) diag[q] = fcis::pair_index(q, q);
+                char *d = static_cast<char *>(p->d_dtile);
+                FCIS_HIP(hipMemset(d, 0, fused_dtile_bytes()));
+                st = upload_frags(p, h2e, diag, 1, d, d + fused_dtile_bytes() / 2);
+                if (st != FCIS_SUCCESS) return st;
+        }
         std::vector<double> fa((size_t)s->nstra * np), fb((size_t)np * p->nbp, 0.0);
         for (int64_t a = 0; a < s->nstra; a++) {
                 const uint64_t str = fcis::space_string(s, 0, a);
@@ -1060,7 +1098,7 @@ void launch_fused(const fcis_sigma_plan *p, hipStream_t st, int64_t a0, int64_t 
                            s->d_linka, p->d_pmapa, p->d_pmapb16, p->nbp, CG, p->d_hfrag_f,            \
                            p->d_hfrag_f_lo, p->d_mpair, p->d_falpha,                                  \
                            (const FusedTypes<PR>::G *)p->d_fbeta, (OUT *)p->d_vt1, sig, rows, rowcl,  \
-                           clist)
+                           clist, (const bf16x8 *)p->d_dtile, s->d_strb)
         if (p->precision == FCIS_PREC_FP64)
                 FCIS_LAUNCH_FUSED(FCIS_PREC_FP64, double, ci0);
         else if (p->precision == FCIS_PREC_FP32)
@@ -1298,7 +1336,8 @@ int fcis::sigma_plan_create(fcis_sigma_plan **out, const fcis_space *space, cons
                           fcis::dev_malloc(&p->d_fbeta, elem * (size_t)p->npair * p->nbp) != hipSuccess ||
                           fcis::dev_malloc(&p->d_cg, precision == FCIS_PREC_FP64
                                                              ? 16
-                                                             : 4 * (size_t)space->nstra * nb) != hipSuccess))) {
+                                                             : 4 * (size_t)space->nstra * nb) != hipSuccess ||
+                          fcis::dev_malloc(&p->d_dtile, fused_dtile_bytes()) != hipSuccess))) {
                 fcis_sigma_plan_free(p);
                 fcis::set_error("fcis_sigma_plan_create: device allocation of %zu bytes failed",
                                 2 * buf + 2 * hbytes + pa + pb);
@@ -1509,6 +1548,7 @@ void fcis_sigma_plan_free(fcis_sigma_plan_t *p)
         (void)hipFree(p->d_falpha);
         (void)hipFree(p->d_fbeta);
         (void)hipFree(p->d_cg);
+        (void)hipFree(p->d_dtile);
         delete p;
 }
 
