@@ -135,20 +135,23 @@ def _prepare(problems):
     return spaces, h2es, ints
 
 
-def solve_batch(problems, nroots=4, tol=1e-10, tol_residual=None, max_cycle=100, max_space=20,
+def solve_batch(problems, nroots=4, tol=1e-10, tol_residual=None, max_cycle=100, max_space=12,
                 lindep=1e-14, level_shift=1e-3, tune=True, use_graph=True, verbose=False,
-                nguard=0):
+                nguard=None):
     """Lowest ``nroots`` states of every problem (h1e, eri, norb, nelec).
 
     Davidson per problem as direct_spin1.FCISolver.kernel (initial guess of
     fci_gas.py:881-902, preconditioner :1337-1342, convergence |dE| < tol
-    and |r| < tol_residual = sqrt(tol), thick restart), all problems advancing
-    together; nguard extra roots are iterated but not required to converge.
-    The default subspace (max_space 20 + 3 per extra root) is larger than
-    the single solver's 12: with four roots a 21-vector space restarts
-    every third iteration, which on the 64-problem synthetic set made one
-    problem's fourth root converge to the fifth state.
+    and |r| < tol_residual = sqrt(tol), thick restart, the reference's
+    max_space 12 + 3 per extra root), all problems advancing together.
+    ``nguard`` extra roots (default: one when nroots > 1) are iterated
+    beside the requested ones but not required to converge: their
+    correction vectors keep the subspace from settling on the state above a
+    root it has not yet found (without the guard, one problem of the
+    64-problem synthetic set converges its fourth root to the fifth state).
     Returns a BatchResult (energies without ecore)."""
+    if nguard is None:
+        nguard = 1 if int(nroots) > 1 else 0
     import torch
     t_setup = time.perf_counter()
     lib = _lib.load()
