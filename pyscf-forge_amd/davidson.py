@@ -74,6 +74,12 @@ class DeviceOps:
         coef = numpy.ascontiguousarray(coef, dtype=numpy.float64).reshape(len(vecs), 1)
         self.ops.gemm([(rows, [y.data_ptr()], int(n), coef, 0.0)])
 
+    def rotate(self, X, m, n, W):
+        """X[:m] <- (X[:m] as rows) transformed by W: row j = sum_i W[i, j] X_i,
+        in place (one launch; m <= 16)."""
+        rows = self._rows(X, m, n)
+        self.ops.gemm([(rows, rows, int(n), numpy.ascontiguousarray(W, dtype=numpy.float64), 0.0)])
+
     def dots_list(self, n, vecs, v):
         """[u . v for u in vecs] in one launch (vecs anywhere in memory)."""
         rows = numpy.array([u.data_ptr() for u in vecs], dtype=numpy.int64)
@@ -261,9 +267,9 @@ def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
     for icyc in range(max_cycle):
         cycles = icyc + 1
         h = subspace_matrix()
-        w, v = numpy.linalg.eigh(h)
-        e = w[:nroots].copy()
-        v = v[:, :nroots]
+        w_all, v_all = numpy.linalg.eigh(h)
+        e = w_all[:nroots].copy()
+        v = v_all[:, :nroots]
         ritz(v, e)
         de = e - e_old
         rnorm[:] = residual_norms(e, False)
@@ -298,7 +304,22 @@ def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
         e_old = e
         if conv.all() and (checked or not mixed):
             break
-        if checked or space + len(new_vectors) > cap:
+        if (checked and space + len(new_vectors) <= cap and space <= 16
+                and getattr(ops, "rotate", None) is not None):
+            # After a check with room left: rotate the whole subspace to the
+            # Ritz basis (nothing is dropped) and give the Ritz vectors their
+            # exact images; the low-precision block becomes diag(w)
+            ops.rotate(X, space, n, v_all)
+            ops.rotate(AX, space, n, v_all)
+            ops.copy(X[:nroots], xr)
+            ops.copy(AX[:nroots], axr)
+            heff[:] = 0.0
+            heff[:space, :space] = numpy.diag(w_all)
+            for i in range(nroots):
+                heff[:space, i] = ops.dots(space, n, X, AX[i])
+            nexact = nroots
+            v = numpy.eye(space)[:, :nroots]
+        elif checked or space + len(new_vectors) > cap:
             # Thick restart: keep the current Ritz vectors and the part of the
             # previous iteration's Ritz vectors orthogonal to them (the
             # locally optimal direction), formed in subspace coordinates so
