@@ -256,7 +256,10 @@ int fcis_csf_det2csf(const fcis_csf_t *csf, const double *d_det, double *d_csf,
                      int nvec, void *stream);
 /* nvec CSF vectors -> determinant vectors; determinants of spin blocks with
  * no CSF of this multiplicity are set to zero (transform_civec_csf2det
- * csfstring.py:416-487 / vec_csf2det(normalize=False) :54-60). */
+ * csfstring.py:416-487 / vec_csf2det(normalize=False) :54-60).
+ * Both transforms run one GEMM per spin block on a CSD-ordered work array
+ * the transformer owns (grown to nvec vectors on first use, nvec <= 65535):
+ * calls on one transformer must not overlap. */
 int fcis_csf_csf2det(const fcis_csf_t *csf, const double *d_csf, double *d_det,
                      int nvec, void *stream);
 /* Diagonal of H in the CSF basis (make_hdiag_csf csf.py:70-144 ->

@@ -21,6 +21,8 @@
 #include <new>
 #include <vector>
 
+#include <rocblas/rocblas.h>
+
 #include "common.h"
 
 namespace {
@@ -129,45 +131,34 @@ __global__ void k_csd_mask(CsfTable t, int64_t ndet_all, uint32_t *__restrict__ 
         mask[pos] = (uint32_t)(crank(astr) * (uint64_t)t.nstrb + crank(bstr));
 }
 
-__global__ void k_det2csf(CsfTable t, int nvec, int64_t ndet_all, int64_t ncsf_all,
-                          const uint32_t *__restrict__ mask, const double *__restrict__ U,
-                          const double *__restrict__ det, double *__restrict__ csf)
+// det <-> CSF as one GEMM per spin block over the determinant vector in
+// CSD order: a block is an (nconf x ndet) row-major matrix of
+// determinants (configuration-major, spin determinants within), its U an
+// (ndet x ncsf) row-major matrix, its CSFs an (nconf x ncsf) row-major
+// matrix, so CSF = D U and D = CSF U^T (rocBLAS, column-major views).  The
+// two kernels below move determinants between address order and CSD order
+// (mask[pos] = address of CSD position pos; positions of blocks without
+// CSFs receive zero).
+__global__ void k_csd_gather(int64_t ndet_all, const uint32_t *__restrict__ mask,
+                             const double *__restrict__ det, double *__restrict__ dcsd)
 {
-        int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-        if (gid >= ncsf_all * nvec) return;
-        int v = (int)(gid / ncsf_all);
-        int64_t i = gid - (int64_t)v * ncsf_all;
-        const CsfBlock &B = t.b[find_block_csf(t, i)];
-        int64_t li = i - B.coff;
-        int64_t conf = li / B.ncsf;
-        int k = (int)(li - conf * B.ncsf);
-        const uint32_t *m = mask + B.off + conf * B.ndet;
-        const double *u = U + B.uoff + k;
-        const double *c = det + (int64_t)v * ndet_all;
-        double acc = 0.0;
-        for (int d = 0; d < B.ndet; d++) acc += u[(int64_t)d * B.ncsf] * c[m[d]];
-        csf[gid] = acc;
+        const int v = blockIdx.y;
+        const double *src = det + (int64_t)v * ndet_all;
+        double *dst = dcsd + (int64_t)v * ndet_all;
+        const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+        for (int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; pos < ndet_all; pos += stride)
+                dst[pos] = src[mask[pos]];
 }
 
-__global__ void k_csf2det(CsfTable t, int nvec, int64_t ndet_all, int64_t ncsf_all,
-                          const uint32_t *__restrict__ mask, const double *__restrict__ U,
-                          const double *__restrict__ csf, double *__restrict__ det)
+__global__ void k_csd_scatter(CsfTable t, int64_t ndet_all, const uint32_t *__restrict__ mask,
+                              const double *__restrict__ dcsd, double *__restrict__ det)
 {
-        int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-        if (gid >= ndet_all * nvec) return;
-        int v = (int)(gid / ndet_all);
-        int64_t pos = gid - (int64_t)v * ndet_all;
-        const CsfBlock &B = t.b[find_block_csd(t, pos)];
-        double acc = 0.0;
-        if (B.ncsf > 0) {
-                int64_t local = pos - B.off;
-                int64_t conf = local / B.ndet;
-                int d = (int)(local - conf * B.ndet);
-                const double *u = U + B.uoff + (int64_t)d * B.ncsf;
-                const double *x = csf + (int64_t)v * ncsf_all + B.coff + conf * B.ncsf;
-                for (int k = 0; k < B.ncsf; k++) acc += u[k] * x[k];
-        }
-        det[(int64_t)v * ndet_all + mask[pos]] = acc;
+        const int v = blockIdx.y;
+        const double *src = dcsd + (int64_t)v * ndet_all;
+        double *dst = det + (int64_t)v * ndet_all;
+        const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+        for (int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; pos < ndet_all; pos += stride)
+                dst[mask[pos]] = t.b[find_block_csd(t, pos)].ncsf > 0 ? src[pos] : 0.0;
 }
 
 // Determinant supports of m CSFs (full-space indices): for CSF i the
@@ -356,6 +347,9 @@ struct fcis_csf {
         int64_t ndet, ncsf;
         uint32_t *d_mask;
         double *d_U;
+        rocblas_handle blas;      // the per-block GEMMs of det2csf / csf2det
+        double *d_work;           // determinant vectors in CSD order (det2csf / csf2det)
+        int64_t work_n;           // its capacity in doubles
         double *d_kex;
         // point-group packing: the CSFs whose configuration has irrep wfnsym
         int64_t nsym;
@@ -440,8 +434,7 @@ int conf_irrep(const CsfTable &t, const CsfBlock &B, int64_t dconf, int64_t scon
 
 static int csf_upload(fcis_csf *c, const std::vector<double> &Uall, hipStream_t st)
 {
-        if (!Uall.empty())
-                FCIS_HIP(hipMemcpyAsync(c->d_U, Uall.data(), 8 * Uall.size(), hipMemcpyHostToDevice, st));
+        if (!Uall.empty()) FCIS_HIP(hipMemcpy(c->d_U, Uall.data(), 8 * Uall.size(), hipMemcpyHostToDevice));
         hipLaunchKernelGGL(k_csd_mask, dim3((c->ndet + 255) / 256), dim3(256), 0, st, c->t, c->ndet,
                            c->d_mask);
         FCIS_CHECK_LAUNCH();
@@ -531,15 +524,25 @@ int fcis_csf_create(fcis_csf_t **out, const fcis_space_t *s, int smult, void *st
         }
         c->d_mask = nullptr;
         c->d_U = nullptr;
+        c->blas = nullptr;
+        c->d_work = nullptr;
+        c->work_n = 0;
         c->d_kex = nullptr;
         c->d_sym = nullptr;
         c->nsym = 0;
         if (fcis::dev_malloc(&c->d_mask, 4 * (size_t)off + 4) != hipSuccess ||
             fcis::dev_malloc(&c->d_U, 8 * (Uall.size() + 1)) != hipSuccess ||
+
             fcis::dev_malloc(&c->d_kex, 8 * (size_t)norb * norb + 8) != hipSuccess) {
                 fcis_csf_free(c);
                 fcis::set_error("fcis_csf_create: device allocation failed");
                 return FCIS_ERR_MEMORY;
+        }
+        if (rocblas_create_handle(&c->blas) != rocblas_status_success) {
+                c->blas = nullptr;
+                fcis_csf_free(c);
+                fcis::set_error("fcis_csf_create: rocBLAS handle creation failed");
+                return FCIS_ERR_DEVICE;
         }
         int st = csf_upload(c, Uall, (hipStream_t)stream);
         if (st != FCIS_SUCCESS) {
@@ -555,6 +558,8 @@ void fcis_csf_free(fcis_csf_t *c)
         if (c == nullptr) return;
         (void)hipFree(c->d_mask);
         (void)hipFree(c->d_U);
+        (void)hipFree(c->d_work);
+        if (c->blas) rocblas_destroy_handle(c->blas);
         (void)hipFree(c->d_kex);
         if (c->d_sym) (void)hipFree(c->d_sym);
         delete c;
@@ -718,29 +723,90 @@ int fcis_csf_info(const fcis_csf_t *c, int64_t *ncsf, int64_t *ndet)
         return FCIS_SUCCESS;
 }
 
+// The CSD-order work array for nvec vectors (grown on demand).
+static int csf_work(fcis_csf_t *c, int nvec)
+{
+        const int64_t need = c->ndet * (int64_t)nvec;
+        if (need <= c->work_n) return FCIS_SUCCESS;
+        (void)hipFree(c->d_work);
+        c->d_work = nullptr;
+        c->work_n = 0;
+        if (fcis::dev_malloc(&c->d_work, 8 * (size_t)need) != hipSuccess) {
+                fcis::set_error("CSF transform: device allocation of %zu bytes failed", 8 * (size_t)need);
+                return FCIS_ERR_MEMORY;
+        }
+        c->work_n = need;
+        return FCIS_SUCCESS;
+}
+
+// CSF = D U (det2csf) or D = CSF U^T (csf2det) for every block with CSFs,
+// all nvec vectors in one strided-batched call per block.
+static int csf_gemms(fcis_csf_t *c, bool to_csf, const double *d_csf_in, double *d_csf_out, int nvec,
+                     hipStream_t st)
+{
+        if (rocblas_set_stream(c->blas, st) != rocblas_status_success) {
+                fcis::set_error("CSF transform: rocblas_set_stream failed");
+                return FCIS_ERR_DEVICE;
+        }
+        const double one = 1.0, zero = 0.0;
+        for (int b = 0; b < c->t.nblk; b++) {
+                const CsfBlock &B = c->t.b[b];
+                if (B.ncsf == 0 || B.nconf == 0) continue;
+                const double *U = c->d_U + B.uoff;
+                double *D = c->d_work + B.off;
+                rocblas_status r;
+                // column-major views: U is (ncsf x ndet), D (ndet x nconf) and the
+                // CSFs (ncsf x nconf); det2csf: CSF = U D, csf2det: D = U^T CSF
+                if (to_csf)
+                        r = rocblas_dgemm_strided_batched(c->blas, rocblas_operation_none, rocblas_operation_none,
+                                                          B.ncsf, (rocblas_int)B.nconf, B.ndet, &one, U, B.ncsf, 0,
+                                                          D, B.ndet, c->ndet, &zero, d_csf_out + B.coff, B.ncsf,
+                                                          c->ncsf, nvec);
+                else
+                        r = rocblas_dgemm_strided_batched(c->blas, rocblas_operation_transpose,
+                                                          rocblas_operation_none, B.ndet, (rocblas_int)B.nconf, B.ncsf,
+                                                          &one, U, B.ncsf, 0, d_csf_in + B.coff, B.ncsf, c->ncsf,
+                                                          &zero, D, B.ndet, c->ndet, nvec);
+                if (r != rocblas_status_success) {
+                        fcis::set_error("CSF transform: rocblas_dgemm_strided_batched failed (status %d)", (int)r);
+                        return FCIS_ERR_DEVICE;
+                }
+        }
+        return FCIS_SUCCESS;
+}
+
 int fcis_csf_det2csf(const fcis_csf_t *c, const double *d_det, double *d_csf, int nvec, void *stream)
 {
-        if (c == nullptr || d_det == nullptr || (d_csf == nullptr && c->ncsf > 0) || nvec < 1) {
+        if (c == nullptr || d_det == nullptr || (d_csf == nullptr && c->ncsf > 0) || nvec < 1 || nvec > 65535) {
                 fcis::set_error("fcis_csf_det2csf: invalid argument");
                 return FCIS_ERR_INVALID;
         }
-        int64_t n = c->ncsf * nvec;
-        if (n == 0) return FCIS_SUCCESS;
-        hipLaunchKernelGGL(k_det2csf, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, c->t,
-                           nvec, c->ndet, c->ncsf, c->d_mask, c->d_U, d_det, d_csf);
+        if (c->ncsf == 0) return FCIS_SUCCESS;
+        fcis_csf_t *cm = const_cast<fcis_csf_t *>(c);
+        int r = csf_work(cm, nvec);
+        if (r != FCIS_SUCCESS) return r;
+        hipStream_t st = (hipStream_t)stream;
+        const unsigned g = (unsigned)std::min<int64_t>((c->ndet + 255) / 256, 16384);
+        hipLaunchKernelGGL(k_csd_gather, dim3(g, nvec), dim3(256), 0, st, c->ndet, c->d_mask, d_det, c->d_work);
         FCIS_CHECK_LAUNCH();
-        return FCIS_SUCCESS;
+        return csf_gemms(cm, true, nullptr, d_csf, nvec, st);
 }
 
 int fcis_csf_csf2det(const fcis_csf_t *c, const double *d_csf, double *d_det, int nvec, void *stream)
 {
-        if (c == nullptr || d_det == nullptr || (d_csf == nullptr && c->ncsf > 0) || nvec < 1) {
+        if (c == nullptr || d_det == nullptr || (d_csf == nullptr && c->ncsf > 0) || nvec < 1 || nvec > 65535) {
                 fcis::set_error("fcis_csf_csf2det: invalid argument");
                 return FCIS_ERR_INVALID;
         }
-        int64_t n = c->ndet * nvec;
-        hipLaunchKernelGGL(k_csf2det, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, c->t,
-                           nvec, c->ndet, c->ncsf, c->d_mask, c->d_U, d_csf, d_det);
+        fcis_csf_t *cm = const_cast<fcis_csf_t *>(c);
+        int r = csf_work(cm, nvec);
+        if (r != FCIS_SUCCESS) return r;
+        hipStream_t st = (hipStream_t)stream;
+        r = csf_gemms(cm, false, d_csf, nullptr, nvec, st);
+        if (r != FCIS_SUCCESS) return r;
+        const unsigned g = (unsigned)std::min<int64_t>((c->ndet + 255) / 256, 16384);
+        hipLaunchKernelGGL(k_csd_scatter, dim3(g, nvec), dim3(256), 0, st, c->t, c->ndet, c->d_mask, c->d_work,
+                           d_det);
         FCIS_CHECK_LAUNCH();
         return FCIS_SUCCESS;
 }
