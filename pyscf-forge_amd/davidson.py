@@ -101,11 +101,22 @@ class DeviceOps:
 def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
              max_cycle=50, max_space=12, lindep=1e-14, verbose=False,
              callback=None, ops=None, hop_check=None, tol_low=None,
-             tol_residual_low=None, check_every=6):
+             tol_residual_low=None, check_every=6, follow_state=False):
     """Lowest ``nroots`` eigenpairs.  x0: list of device vectors (guesses).
 
     Returns DavidsonResult with e (numpy), x / ax (device tensors of shape
     (nroots, n)), per-root convergence flags, cycle and H-application counts.
+
+    State matching, as pyscf.lib.davidson1 does it: each root's energy
+    change is taken against the previous iteration's Ritz value of the state
+    it overlaps most (the subspace eigenvectors may swap order between
+    iterations).  ``follow_state`` (the reference passes it at
+    fci_gas.py:1350 and csf.py:489) adds davidson1's guard against a state
+    flip: when the largest residual norm exceeds 1 and grew more than three
+    times over the previous iteration, with more than nroots + 2 vectors in
+    the subspace, the subspace restarts from the previous iteration's Ritz
+    vectors (formed from X and AX, no new H applications) and the new
+    directions are dropped; ``nrestore`` counts these restarts.
 
     Mixed precision: with ``hop_check`` (the exact operator; ``hop`` is then
     an approximate, cheaper one) every new subspace vector is applied with
@@ -260,9 +271,31 @@ def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
     if space < nroots:
         raise RuntimeError("Davidson: initial guesses are linearly dependent")
 
+    def restore(vp):
+        """Restart the subspace from the vectors X vp (orthonormal columns
+        vp in subspace coordinates, padded to the current space) with the
+        images AX vp: one pass over X and AX, no H applications."""
+        nonlocal space, nexact
+        k = vp.shape[1]
+        for j in range(k):
+            ops.lincomb(space, n, X, vp[:, j], 0.0, xp[j])
+            ops.lincomb(space, n, AX, vp[:, j], 0.0, axp[j])
+        hk = vp.T @ subspace_matrix() @ vp
+        for j in range(k):
+            ops.copy(X[j], xp[j])
+            ops.copy(AX[j], axp[j])
+        space = k
+        heff[:] = 0.0
+        heff[:k, :k] = 0.5 * (hk + hk.T)
+        # combinations of exact and approximate images are neither: the
+        # next check anchors exact ones again
+        nexact = 0
+
     e_old = numpy.full(nroots, numpy.inf)
     conv = numpy.zeros(nroots, dtype=bool)
     rnorm = numpy.full(nroots, numpy.inf)
+    rmax_prev = numpy.inf
+    nrestore = 0
     cycles = 0
     for icyc in range(max_cycle):
         cycles = icyc + 1
@@ -271,8 +304,29 @@ def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
         e = w_all[:nroots].copy()
         v = v_all[:, :nroots]
         ritz(v, e)
-        de = e - e_old
+        e_ref = e_old
+        if v_prev is not None and numpy.isfinite(e_old).all():
+            # compare each root with the previous Ritz value of the state it
+            # overlaps most (v_prev: previous Ritz vectors in the current
+            # subspace coordinates, new rows zero)
+            head = v_prev.shape[0]
+            e_ref = e_old[numpy.argmax(numpy.abs(v[:head].T @ v_prev), axis=1)]
+        de = e - e_ref
         rnorm[:] = residual_norms(e, False)
+        rmax = float(rnorm.max())
+        if (follow_state and v_prev is not None and rmax > 1.0 and rmax > 3.0 * rmax_prev
+                and space > nroots + 2
+                and not ((numpy.abs(de) < stage_tol) & (rnorm < stage_res)).all()):
+            vp = numpy.zeros((space, v_prev.shape[1]))
+            vp[:v_prev.shape[0]] = v_prev
+            restore(vp)
+            nrestore += 1
+            v_prev = numpy.eye(space)[:, :nroots]
+            if verbose:
+                print("davidson %d  large |r| %.3g: restart from the previous Ritz vectors"
+                      % (icyc, rmax))
+            continue
+        rmax_prev = rmax
         conv = (numpy.abs(de) < stage_tol) & (rnorm < stage_res)
         checked = False
         if mixed and (conv.all() or (refining and icyc - last_check >= check_every)
@@ -366,6 +420,7 @@ def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
             break
     res = DavidsonResult(conv.copy(), e_old, xr, axr, cycles, nhop, rnorm.copy())
     res.nhop_check = nhop_check
+    res.nrestore = nrestore
     return res
 
 

@@ -918,7 +918,8 @@ class FCISolver(_ds.FCISolver):
                                       tol=tol, tol_residual=tol_residual,
                                       max_cycle=max_cycle, max_space=max_space,
                                       lindep=lindep, level_shift=level_shift,
-                                      verbose=verbose, basis=basis)
+                                      verbose=verbose, basis=basis,
+                                      follow_state=penalty is None)
             self.stats = stats
         e = numpy.asarray(e, dtype=numpy.float64).reshape(-1) + float(ecore)
         if penalty is not None:

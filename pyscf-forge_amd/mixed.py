@@ -35,12 +35,15 @@ def _davidson_bytes(ndet, nroots, max_space):
 
 def solve(solver, sp, h2e, hdiag, nroots, ci0, precision, tol=1e-10, tol_residual=None,
           max_cycle=50, max_space=12, lindep=1e-14, level_shift=1e-3, verbose=0,
-          basis=None, precond=None):
+          basis=None, precond=None, follow_state=True):
     """Davidson solve in the determinant basis, or, with ``basis`` (an object
     with ``n`` and ``hop(plan)``, e.g. a CSF transformer), in that basis with
     H x = basis.hop(plan)(x, out); hdiag and ci0 are then in that basis.
     ``precond(r, e, out, x0)`` replaces the diagonal preconditioner
-    r / (hdiag - e) (|hdiag - e| < level_shift -> level_shift)."""
+    r / (hdiag - e) (|hdiag - e| < level_shift -> level_shift).
+    ``follow_state`` is davidson1's large-residual restart, which the
+    reference's solvers switch on (fci_gas.py:1350 without a spin penalty,
+    csf.py:489)."""
     import torch
     from .direct_spin1 import SigmaPlan
 
@@ -89,10 +92,11 @@ def solve(solver, sp, h2e, hdiag, nroots, ci0, precision, tol=1e-10, tol_residua
             hop = plan if basis is None else basis.hop(plan)
             res = _dv.davidson(hop, precond, guesses, nroots=nroots, lindep=lindep,
                                max_space=max_space, verbose=verbose, tol=tol,
-                               tol_residual=tol_residual, max_cycle=max_cycle)
+                               tol_residual=tol_residual, max_cycle=max_cycle,
+                               follow_state=follow_state)
             torch.cuda.synchronize()
             stats.update(time=time.perf_counter() - t0, nhop=res.nhop, cycles=res.cycles,
-                         ntiles=plan.ntiles)
+                         ntiles=plan.ntiles, nrestore=res.nrestore)
         return res.e, list(res.x), res.converged, stats
 
     if precision not in LOW_STAGE:
@@ -108,9 +112,10 @@ def solve(solver, sp, h2e, hdiag, nroots, ci0, precision, tol=1e-10, tol_residua
         res = _dv.davidson(hop_lo, precond, guesses, nroots=nroots, lindep=lindep,
                            max_space=max_space, verbose=verbose, tol=tol,
                            tol_residual=tol_residual, max_cycle=max_cycle, hop_check=hop64,
-                           tol_low=lo["tol"], tol_residual_low=lo["tol_residual"])
+                           tol_low=lo["tol"], tol_residual_low=lo["tol_residual"],
+                           follow_state=follow_state)
         torch.cuda.synchronize()
         stats.update(time=time.perf_counter() - t0, nhop_low=res.nhop,
-                     nhop_fp64=res.nhop_check, cycles=res.cycles,
+                     nhop_fp64=res.nhop_check, cycles=res.cycles, nrestore=res.nrestore,
                      ntiles=plan64.ntiles, ntiles_low=plan_lo.ntiles)
     return res.e, list(res.x), res.converged, stats

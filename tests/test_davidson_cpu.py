@@ -187,3 +187,58 @@ def test_mixed_refinement_keeps_the_subspace(n, nroots, bits, seed):
                        nroots=nroots, tol=tol, max_cycle=200, max_space=12, ops=NumpyOps())
     assert res.nhop_check <= 4 * nroots
     assert res.nhop_check < ref.nhop
+
+
+@pytest.mark.parametrize("n,nroots,seed", [(300, 1, 21), (300, 3, 22)])
+def test_follow_state_restores_after_residual_jump(n, nroots, seed):
+    """davidson1's follow_state guard (the reference switches it on,
+    fci_gas.py:1350, csf.py:489): when the largest residual norm exceeds 1
+    and grew more than threefold, the subspace restarts from the previous
+    Ritz vectors.  Here one H application returns a grossly wrong image
+    (a planted fault of the operator); with the guard the solve drops that
+    direction and still reaches the eigenvalues of eigh, and on a clean
+    operator the guard never fires and changes nothing."""
+    H = _problem(n, seed)
+    hdiag = numpy.diag(H).copy()
+    calls = [0]
+
+    def hop_bad(x, out):
+        calls[0] += 1
+        out[...] = H @ x
+        if calls[0] == nroots + 3:
+            out[...] += 50.0 * numpy.sign(x)
+
+    def hop(x, out):
+        out[...] = H @ x
+
+    def precond(r, e, out, x0):
+        dd = hdiag - e
+        dd[numpy.abs(dd) < 1e-8] = 1e-8
+        out[...] = r / dd
+
+    guesses = []
+    for k in numpy.argsort(hdiag)[:nroots]:
+        g = numpy.zeros(n)
+        g[k] = 1.0
+        guesses.append(g)
+    w = numpy.linalg.eigh(H)[0][:nroots]
+    res = dav.davidson(hop_bad, precond, guesses, nroots=nroots, tol=1e-12, max_cycle=200,
+                       max_space=12, ops=NumpyOps(), follow_state=True)
+    assert res.nrestore >= 1
+    assert res.converged.all()
+    numpy.testing.assert_allclose(res.e, w, rtol=0, atol=1e-9)
+    X = numpy.asarray(res.x).reshape(nroots, n)
+    assert numpy.linalg.norm(X @ H.T - res.e[:, None] * X, axis=1).max() < 1e-5
+    # without the guard the wrong image stays in the subspace
+    calls[0] = 0
+    off = dav.davidson(hop_bad, precond, guesses, nroots=nroots, tol=1e-12, max_cycle=60,
+                       max_space=12, ops=NumpyOps())
+    assert off.nrestore == 0 and not off.converged.all()
+
+    clean = dav.davidson(hop, precond, guesses, nroots=nroots, tol=1e-12, max_cycle=200,
+                         max_space=12, ops=NumpyOps(), follow_state=True)
+    plain = dav.davidson(hop, precond, guesses, nroots=nroots, tol=1e-12, max_cycle=200,
+                         max_space=12, ops=NumpyOps())
+    assert clean.nrestore == 0
+    assert clean.cycles == plain.cycles
+    numpy.testing.assert_allclose(clean.e, w, rtol=0, atol=1e-9)
