@@ -495,6 +495,39 @@ int fcis_gas_contract_ss(const fcis_gas_space_t *gas, const fcis_spin_plan_t *sp
 int fcis_gas_rows_info(const fcis_gas_space_t *gas, int64_t *nrows, int64_t *nchunks,
                        int64_t *nchunks_cas);
 
+/* Sector-native space of the block set D (arguments as fcis_gas_space_create,
+ * with the electron counts na, nb instead of a CAS space): the same sector
+ * ids, block offsets and sector string addresses, and no CAS embedding.
+ * sum(norb[]) <= 63, norb[k] <= 31, ngas <= 15, as libfci_gas allows
+ * (fci_gas.h:24-27); fewer than 2^31 strings per spin in the sectors one
+ * move from D.  fcis_gas_sigma_plan_* (FP64 or FP32), fcis_gas_make_hdiag,
+ * fcis_gas_trans_rdm12s, fcis_gas_pspace and the descriptor queries accept
+ * it; the CAS-embedding entry points (embed, project, det_index,
+ * sigma_execute, contract_ss, rows_info) reject it.  Replaces
+ * gas_space_from_blocks (fci_gas.h:220, fci_string_gas.c:1683) and with it
+ * the sector-addressed contraction, diagonal and RDM plans of libfci_gas
+ * (fci_contract_gas.c:2388-2454, fci_hdiag_gas.c:678-709,
+ * fci_rdm_gas.c:2207-2243). */
+int fcis_gas_space_create_native(fcis_gas_space_t **out, int ngas, const int *norb, int na,
+                                 int nb, int nblock, const int *block_occ, void *stream);
+/* native = 1 for a sector-native space; nclosure = determinants within one
+ * excitation of D (the t1 columns of its sigma), maxrow = the longest
+ * closure row (the fewest t1 columns a workspace must hold), nstra / nstrb =
+ * strings of the sectors one move from D per spin (zeros for a CAS-embedded
+ * space). */
+int fcis_gas_native_info(const fcis_gas_space_t *gas, int *native, int64_t *nclosure,
+                         int64_t *maxrow, int64_t *nstra, int64_t *nstrb);
+/* Alpha and beta strings of the GAS determinants addr[i], i < n (host
+ * arrays; gas_addr2str_sector over the block layout, fci_string_gas.c:462-489). */
+int fcis_gas_det_strings(const fcis_gas_space_t *gas, int64_t n, const int64_t *addr,
+                         uint64_t *stra, uint64_t *strb);
+/* H block over the m GAS determinants addr[] (host int64) into device d_h0
+ * (m x m), Slater-Condon rules on the un-absorbed h1e and pair matrix, for
+ * either kind of space (the reference's small-space builder,
+ * fci_gas.py:955-988, 1130-1148). */
+int fcis_gas_pspace(const fcis_gas_space_t *gas, const double *h1e, const double *eri, int m,
+                    const int64_t *addr, double *d_h0, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -57,6 +57,9 @@ def _load():
         lib.orc_trans_rdm12s.argtypes = [ctypes.c_int] * 3 + [_dp] * 7
         lib.orc_sigma_rows.argtypes = [ctypes.c_int] * 3 + [_dp] * 3 + [
             ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), _dp]
+        lib.orc_det_sigma_rows.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int64, _up, _up, _dp,
+                                           ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), _dp]
+        lib.orc_det_hamiltonian.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int64, _up, _up, _dp]
         _lib = lib
     return _lib
 
@@ -175,6 +178,44 @@ def sigma_rows(h1e, eri, ci, norb, nelec, dets):
     _load().orc_sigma_rows(norb, na, nb, _d(h1e), _d(eri), _d(c), dets.size,
                            dets.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), _d(out))
     return out
+
+
+def det_sigma_rows(h1e, eri, ci, norb, stra, strb, rows):
+    """(H c)[rows] for the determinants (stra[j], strb[j]) of an explicit
+    list (up to 64 orbitals), H restricted to the list, by the Slater-Condon
+    rules on the un-absorbed integrals (fci_oracle.c orc_det_sigma_rows):
+    for a GAS determinant list this is the restricted sigma P H P c."""
+    h1e = _c64(h1e, (norb, norb))
+    eri = pair_matrix(eri, norb)
+    c = _c64(ci).reshape(-1)
+    sa = numpy.ascontiguousarray(stra, dtype=numpy.uint64).reshape(-1)
+    sb = numpy.ascontiguousarray(strb, dtype=numpy.uint64).reshape(-1)
+    if not (c.size == sa.size == sb.size):
+        raise ValueError("determinant list and CI vector differ in size")
+    rows = numpy.ascontiguousarray(rows, dtype=numpy.int64).reshape(-1)
+    out = numpy.empty(rows.size)
+    st = _load().orc_det_sigma_rows(int(norb), _d(h1e), _d(eri), c.size,
+                                    sa.ctypes.data_as(_up), sb.ctypes.data_as(_up), _d(c),
+                                    rows.size, rows.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                    _d(out))
+    if st != 0:
+        raise ValueError("orc_det_sigma_rows failed (%d)" % st)
+    return out
+
+
+def det_hamiltonian(h1e, eri, norb, stra, strb):
+    """Dense H over an explicit determinant list (orc_det_hamiltonian; the
+    rules of det_sigma_rows), for exact restricted spectra of small lists."""
+    h1e = _c64(h1e, (norb, norb))
+    eri = pair_matrix(eri, norb)
+    sa = numpy.ascontiguousarray(stra, dtype=numpy.uint64).reshape(-1)
+    sb = numpy.ascontiguousarray(strb, dtype=numpy.uint64).reshape(-1)
+    H = numpy.empty((sa.size, sa.size))
+    st = _load().orc_det_hamiltonian(int(norb), _d(h1e), _d(eri), sa.size,
+                                     sa.ctypes.data_as(_up), sb.ctypes.data_as(_up), _d(H))
+    if st != 0:
+        raise ValueError("orc_det_hamiltonian failed (%d)" % st)
+    return H
 
 
 _MASK64 = (1 << 64) - 1

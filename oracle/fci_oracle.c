@@ -771,3 +771,232 @@ int orc_sigma_rows(int n, int na, int nb, const double *h1e, const double *eri,
         }
         return 0;
 }
+
+/* Slater-Condon rows over an explicit determinant list.
+ *
+ * out[i] = sum_J <I|H|J> c[J] for I = (stra[rows[i]], strb[rows[i]]) and J
+ * over the m determinants (stra[j], strb[j]) of the list (any order, up to
+ * 64 orbitals), by the same rules as orc_sigma_rows; the determinants
+ * connected to I (singles and doubles of either spin, alpha-beta doubles)
+ * are enumerated from I's strings and looked up in an open-addressing hash
+ * of the list, so H restricted to the list is applied exactly: for a GAS
+ * space this is P H P, the restricted Hamiltonian of the reference's
+ * libfci_gas (pyscf/mcscf/test/test_gasci.py:522-575).  Pinned by
+ * tests/test_oracle.py against orc_sigma_rows on full CAS lists and against
+ * the libfci_gas GAS golden sigma vectors. */
+
+typedef struct {
+        uint64_t *ka, *kb;
+        int64_t *val;
+        int64_t mask;
+} dethash_t;
+
+static uint64_t dh_mix(uint64_t a, uint64_t b)
+{
+        uint64_t z = a * 0x9E3779B97F4A7C15ULL ^ (b + 0x632BE59BD9B4E019ULL + (a << 6) + (a >> 2));
+        z ^= z >> 31;
+        z *= 0xBF58476D1CE4E5B9ULL;
+        z ^= z >> 29;
+        return z;
+}
+
+static int dh_init(dethash_t *h, int64_t m, const uint64_t *sa, const uint64_t *sb)
+{
+        int64_t cap = 16;
+        while (cap < 2 * m) cap <<= 1;
+        h->mask = cap - 1;
+        h->ka = malloc(sizeof(uint64_t) * cap);
+        h->kb = malloc(sizeof(uint64_t) * cap);
+        h->val = malloc(sizeof(int64_t) * cap);
+        if (!h->ka || !h->kb || !h->val) return -1;
+        for (int64_t i = 0; i < cap; i++) h->val[i] = -1;
+        for (int64_t j = 0; j < m; j++) {
+                int64_t k = (int64_t)(dh_mix(sa[j], sb[j]) & (uint64_t)h->mask);
+                while (h->val[k] >= 0) k = (k + 1) & h->mask;
+                h->ka[k] = sa[j];
+                h->kb[k] = sb[j];
+                h->val[k] = j;
+        }
+        return 0;
+}
+
+static int64_t dh_find(const dethash_t *h, uint64_t a, uint64_t b)
+{
+        int64_t k = (int64_t)(dh_mix(a, b) & (uint64_t)h->mask);
+        while (h->val[k] >= 0) {
+                if (h->ka[k] == a && h->kb[k] == b) return h->val[k];
+                k = (k + 1) & h->mask;
+        }
+        return -1;
+}
+
+static void dh_free(dethash_t *h)
+{
+        free(h->ka);
+        free(h->kb);
+        free(h->val);
+}
+
+/* Visitor of the nonzero <I|H|J> of a row: f(ctx, j, value) for every
+ * determinant j of the list connected to I (I itself included). */
+typedef void (*row_visit_t)(void *ctx, int64_t j, double value);
+
+static void visit(const dethash_t *h, uint64_t a, uint64_t b, double v, row_visit_t f, void *ctx)
+{
+        const int64_t j = dh_find(h, a, b);
+        if (j >= 0) f(ctx, j, v);
+}
+
+/* Same-spin singles and doubles of string s (the moving spin; t the other
+ * spin's string); `alpha` says which of the pair (s, t) is alpha. */
+static void same_spin_list(int n, int np, const double *h1e, const double *eri, uint64_t s,
+                           uint64_t t, int alpha, const dethash_t *hs, row_visit_t f, void *ctx)
+{
+        int occ[64], vir[64], no = 0, nv = 0, tocc[64], nt = 0;
+        for (int i = 0; i < n; i++) {
+                if ((s >> i) & 1) occ[no++] = i;
+                else vir[nv++] = i;
+                if ((t >> i) & 1) tocc[nt++] = i;
+        }
+        for (int a = 0; a < no; a++) {
+                const int q = occ[a];
+                for (int b = 0; b < nv; b++) {
+                        const int p = vir[b];
+                        uint64_t s1 = s;
+                        int sg = apply_op(&s1, q, 0);
+                        sg *= apply_op(&s1, p, 1);
+                        if ((alpha ? dh_find(hs, s1, t) : dh_find(hs, t, s1)) < 0) continue;
+                        double v = h1e[p * n + q];
+                        for (int j = 0; j < no; j++) {
+                                const int o = occ[j];
+                                v += eri_at(eri, np, p, q, o, o) - eri_at(eri, np, p, o, o, q);
+                        }
+                        for (int j = 0; j < nt; j++) v += eri_at(eri, np, p, q, tocc[j], tocc[j]);
+                        if (alpha) visit(hs, s1, t, sg * v, f, ctx);
+                        else visit(hs, t, s1, sg * v, f, ctx);
+                }
+        }
+        for (int a1 = 0; a1 < no; a1++)
+                for (int a2 = a1 + 1; a2 < no; a2++)
+                        for (int b1 = 0; b1 < nv; b1++)
+                                for (int b2 = b1 + 1; b2 < nv; b2++) {
+                                        const int q1 = occ[a1], q2 = occ[a2], p1 = vir[b1], p2 = vir[b2];
+                                        uint64_t s1 = s;
+                                        int sg = apply_op(&s1, q1, 0);
+                                        sg *= apply_op(&s1, q2, 0);
+                                        sg *= apply_op(&s1, p2, 1);
+                                        sg *= apply_op(&s1, p1, 1);
+                                        const double v = sg * (eri_at(eri, np, p1, q1, p2, q2) -
+                                                               eri_at(eri, np, p1, q2, p2, q1));
+                                        if (alpha) visit(hs, s1, t, v, f, ctx);
+                                        else visit(hs, t, s1, v, f, ctx);
+                                }
+}
+
+static void det_row(int n, const double *h1e, const double *eri, uint64_t sa, uint64_t sb,
+                    int64_t self, const dethash_t *hs, row_visit_t f, void *ctx)
+{
+        const int np = n * (n + 1) / 2;
+        double e0 = 0.0;
+        for (int p = 0; p < n; p++) {
+                const int oa = (sa >> p) & 1, ob = (sb >> p) & 1;
+                e0 += (oa + ob) * h1e[p * n + p];
+                for (int q = 0; q < p; q++) {
+                        const double J = eri_at(eri, np, p, p, q, q), K = eri_at(eri, np, p, q, q, p);
+                        if (oa && ((sa >> q) & 1)) e0 += J - K;
+                        if (ob && ((sb >> q) & 1)) e0 += J - K;
+                }
+                for (int q = 0; q < n; q++)
+                        if (oa && ((sb >> q) & 1)) e0 += eri_at(eri, np, p, p, q, q);
+        }
+        f(ctx, self, e0);
+        same_spin_list(n, np, h1e, eri, sa, sb, 1, hs, f, ctx);
+        same_spin_list(n, np, h1e, eri, sb, sa, 0, hs, f, ctx);
+        for (int q = 0; q < n; q++) {
+                if (!((sa >> q) & 1)) continue;
+                for (int p = 0; p < n; p++) {
+                        if (p == q || ((sa >> p) & 1)) continue;
+                        uint64_t ta = sa;
+                        int sga = apply_op(&ta, q, 0);
+                        sga *= apply_op(&ta, p, 1);
+                        for (int s = 0; s < n; s++) {
+                                if (!((sb >> s) & 1)) continue;
+                                for (int r = 0; r < n; r++) {
+                                        if (r == s || ((sb >> r) & 1)) continue;
+                                        uint64_t tb = sb;
+                                        int sgb = apply_op(&tb, s, 0);
+                                        sgb *= apply_op(&tb, r, 1);
+                                        visit(hs, ta, tb, sga * sgb * eri_at(eri, np, p, q, r, s), f, ctx);
+                                }
+                        }
+                }
+        }
+}
+
+typedef struct {
+        const double *c;
+        double acc;
+} dot_ctx_t;
+
+static void dot_visit(void *ctx, int64_t j, double v)
+{
+        dot_ctx_t *d = (dot_ctx_t *)ctx;
+        d->acc += v * d->c[j];
+}
+
+typedef struct {
+        double *row;
+} dense_ctx_t;
+
+static void dense_visit(void *ctx, int64_t j, double v)
+{
+        ((dense_ctx_t *)ctx)->row[j] += v;
+}
+
+int orc_det_sigma_rows(int n, const double *h1e, const double *eri, int64_t m,
+                       const uint64_t *stra, const uint64_t *strb, const double *c, int64_t nrow,
+                       const int64_t *rows, double *out)
+{
+        if (n < 0 || n > 64 || m < 0 || nrow < 0) return -1;
+        dethash_t hs;
+        if (dh_init(&hs, m, stra, strb) != 0) {
+                dh_free(&hs);
+                return -2;
+        }
+        int bad = 0;
+#pragma omp parallel for schedule(dynamic)
+        for (int64_t i = 0; i < nrow; i++) {
+                const int64_t d = rows[i];
+                if (d < 0 || d >= m) {
+                        out[i] = NAN;
+                        bad = 1;
+                        continue;
+                }
+                dot_ctx_t ctx = {c, 0.0};
+                det_row(n, h1e, eri, stra[d], strb[d], d, &hs, dot_visit, &ctx);
+                out[i] = ctx.acc;
+        }
+        dh_free(&hs);
+        return bad ? -1 : 0;
+}
+
+/* Dense H (m x m, row-major) over the determinant list, by the rules of
+ * orc_det_sigma_rows (small lists: exact restricted spectra). */
+int orc_det_hamiltonian(int n, const double *h1e, const double *eri, int64_t m,
+                        const uint64_t *stra, const uint64_t *strb, double *H)
+{
+        if (n < 0 || n > 64 || m < 0) return -1;
+        dethash_t hs;
+        if (dh_init(&hs, m, stra, strb) != 0) {
+                dh_free(&hs);
+                return -2;
+        }
+        memset(H, 0, sizeof(double) * (size_t)m * (size_t)m);
+#pragma omp parallel for schedule(dynamic)
+        for (int64_t i = 0; i < m; i++) {
+                dense_ctx_t ctx = {H + (size_t)i * (size_t)m};
+                det_row(n, h1e, eri, stra[i], strb[i], i, &hs, dense_visit, &ctx);
+        }
+        dh_free(&hs);
+        return 0;
+}

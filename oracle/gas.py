@@ -28,6 +28,7 @@ multi-space, multi-block cases and with the fixtures it wrote under
 tests/golden/, and the restricted eigenvalues with a dense projected H.
 """
 
+import functools
 import itertools
 
 import numpy
@@ -35,6 +36,7 @@ import numpy
 from . import contract_2e, make_hdiag, trans_rdm12s, num_strings
 
 
+@functools.lru_cache(maxsize=None)
 def _local_strings(norb, nocc):
     """Bit strings of nocc electrons in norb orbitals in combinatorial
     (colex) address order."""
@@ -113,12 +115,25 @@ class GasLayout:
             occ = self.sectors[sid]
             per = [_local_strings(n, k) << st
                    for n, k, st in zip(self.norb, occ, self.start)]
-            out = numpy.zeros(self.sector_nstr[sid], dtype=numpy.int64)
             # row-major over spaces, the last space fastest
-            for i, combo in enumerate(itertools.product(*per)):
-                out[i] = sum(int(x) for x in combo)
+            out = numpy.zeros(1, dtype=numpy.int64)
+            for x in per:
+                out = (out[:, None] + x[None, :]).reshape(-1)
             self._strings[sid] = out
         return self._strings[sid]
+
+    def det_strings(self):
+        """(alpha strings, beta strings) of every GAS determinant, in the
+        space's order (uint64)."""
+        sa = numpy.empty(self.ndet, dtype=numpy.uint64)
+        sb = numpy.empty(self.ndet, dtype=numpy.uint64)
+        for d in self.block_desc:
+            a = self.sector_strings(d["sa"]).astype(numpy.uint64)
+            b = self.sector_strings(d["sb"]).astype(numpy.uint64)
+            o, na, nb = d["offset"], d["na"], d["nb"]
+            sa[o:o + na * nb] = numpy.repeat(a, nb)
+            sb[o:o + na * nb] = numpy.tile(b, na)
+        return sa, sb
 
     def cas_index(self):
         """(ndet,) flat (nstr_alpha, nstr_beta) C-order CAS index of every

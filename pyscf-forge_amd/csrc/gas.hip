@@ -19,51 +19,16 @@
 // (fci_gas.py gas2fci :141-182), runs the CAS sigma and projects back
 // (fci2gas :185-208).  d_map[d] is the CAS determinant index of GAS
 // determinant d; it is built on the device from per-sector CAS string
-// addresses.
+// addresses.  A sector-native space (cas == null, gas_native.hip) has the
+// same layout and no CAS: its sigma, diagonal and RDMs run on the GAS
+// determinants and their single-excitation closure directly.
 #include <stdlib.h>
 #include <string.h>
 #include <algorithm>
 #include <new>
 #include <vector>
 
-#include "common.h"
-#include "fused.h"
-
-#define GAS_MAXNGAS 15
-
-struct fcis_gas_space {
-        const fcis_space *cas;
-        int ngas, na, nb;
-        int norb[GAS_MAXNGAS], start[GAS_MAXNGAS];
-        int nsector;
-        std::vector<uint8_t> occ;       // [nsector][ngas]
-        std::vector<int64_t> stride;    // [nsector][ngas]
-        std::vector<int64_t> nstr;      // [nsector]
-        int nblock;
-        std::vector<int64_t> boff;      // [nblock]
-        std::vector<int> bsa, bsb;      // [nblock]
-        int64_t ndet;
-        int64_t ncas;
-        uint32_t *d_map;                // [ndet] CAS determinant index
-        // Restricted rows of the fused CAS sigma (fcis_rowset): every alpha
-        // string whose sector pairs with some beta sector in S1, the GAS
-        // blocks and their single-excitation neighbours (sa', sb), (sa, sb')
-        // with sa' (sb') one inter-subspace move from sa (sb) or equal, each
-        // with the FUSED_NB-wide beta chunks holding strings of those beta
-        // sectors.  t1 and vt1 vanish outside S1 when c lives on the GAS
-        // determinants, so sigma on them is exact.
-        int32_t *d_rows, *d_rpos;
-        int2 *d_rowcl;
-        uint16_t *d_clist;
-        fcis_rowset rs;
-        int64_t nchunk_rows;            // sum of chunk counts over rows
-};
-
-struct fcis_gas_sigma_plan {
-        const fcis_gas_space *gas;
-        fcis_sigma_plan_t *cas_plan;
-        double *d_a, *d_b;              // CAS-sized embed / sigma buffers
-};
+#include "gas.h"
 
 namespace {
 
@@ -96,18 +61,6 @@ int64_t rank_colex(uint64_t s)
         return a;
 }
 
-uint64_t sector_string(const fcis_gas_space *g, int s, int64_t addr)
-{
-        uint64_t str = 0;
-        const uint8_t *occ = &g->occ[(size_t)s * g->ngas];
-        const int64_t *st = &g->stride[(size_t)s * g->ngas];
-        for (int k = 0; k < g->ngas; k++) {
-                int64_t q = addr / st[k];
-                addr -= q * st[k];
-                str |= unrank_local(g->norb[k], occ[k], q) << g->start[k];
-        }
-        return str;
-}
 
 // Grid-stride: d_map[d] for every GAS determinant.  Block b of a
 // determinant is found by binary search over the block offsets.
@@ -176,6 +129,12 @@ __global__ void k_gas_det_index(const uint32_t *__restrict__ map, int64_t ndet, 
         }
 }
 
+int no_cas(const char *who)
+{
+        fcis::set_error("%s: the space is sector-native (no CAS embedding)", who);
+        return FCIS_ERR_INVALID;
+}
+
 unsigned grid_for(int64_t n)
 {
         int64_t g = (n + GAS_THREADS - 1) / GAS_THREADS;
@@ -233,31 +192,151 @@ int sector_of(const fcis_gas_space *g, const std::vector<std::vector<uint8_t>> &
         return (it != pool.end() && *it == o) ? (int)(it - pool.begin()) : -1;
 }
 
+}  // namespace
+
+uint64_t fcis::gas_sector_string(const fcis_gas_space *g, int s, int64_t addr)
+{
+        uint64_t str = 0;
+        const uint8_t *occ = &g->occ[(size_t)s * g->ngas];
+        const int64_t *st = &g->stride[(size_t)s * g->ngas];
+        for (int k = 0; k < g->ngas; k++) {
+                int64_t q = addr / st[k];
+                addr -= q * st[k];
+                str |= unrank_local(g->norb[k], occ[k], q) << g->start[k];
+        }
+        return str;
+}
+
+std::vector<int> fcis::gas_moves(const fcis_gas_space *g, const SectorPool &pool, int s)
+{
+        std::vector<int> out{s};
+        for (int r = 0; r < g->ngas; r++) {
+                for (int t = 0; t < g->ngas; t++) {
+                        if (r == t || pool[s][r] == 0 || pool[s][t] >= g->norb[t]) continue;
+                        std::vector<uint8_t> o = pool[s];
+                        o[r]--;
+                        o[t]++;
+                        auto it = std::lower_bound(pool.begin(), pool.end(), o);
+                        if (it != pool.end() && *it == o) out.push_back((int)(it - pool.begin()));
+                }
+        }
+        return out;
+}
+
+int fcis::gas_layout(fcis_gas_space *g, int ngas, const int *norb, int na, int nb, int nblock,
+                     const int *block_occ, SectorPool &pool, const char *who)
+{
+        g->ngas = ngas;
+        g->na = na;
+        g->nb = nb;
+        int tot = 0;
+        for (int k = 0; k < ngas; k++) {
+                if (norb[k] < 1 || norb[k] > GAS_MAXLOCAL) {
+                        fcis::set_error("%s: subspace %d has %d orbitals", who, k, norb[k]);
+                        return FCIS_ERR_INVALID;
+                }
+                g->norb[k] = norb[k];
+                g->start[k] = tot;
+                tot += norb[k];
+        }
+        if (tot > GAS_MAXORB) {
+                fcis::set_error("%s: %d orbitals (at most %d)", who, tot, GAS_MAXORB);
+                return FCIS_ERR_INVALID;
+        }
+        // blocks: valid occupations, electron counts, strictly increasing
+        const int w = 2 * ngas;
+        std::vector<uint8_t> rows((size_t)nblock * w);
+        for (int b = 0; b < nblock; b++) {
+                int sa = 0, sb = 0;
+                for (int k = 0; k < w; k++) {
+                        int v = block_occ[(size_t)b * w + k];
+                        if (v < 0 || v > norb[k % ngas]) {
+                                fcis::set_error("%s: block %d occupation out of range", who, b);
+                                return FCIS_ERR_INVALID;
+                        }
+                        rows[(size_t)b * w + k] = (uint8_t)v;
+                        (k < ngas ? sa : sb) += v;
+                }
+                if (sa != na || sb != nb) {
+                        fcis::set_error("%s: block %d holds (%d, %d) electrons, the space (%d, %d)",
+                                        who, b, sa, sb, na, nb);
+                        return FCIS_ERR_INVALID;
+                }
+                if (b > 0 && lex_cmp(&rows[(size_t)(b - 1) * w], &rows[(size_t)b * w], w) >= 0) {
+                        fcis::set_error("%s: blocks must be unique and sorted (block %d)", who, b);
+                        return FCIS_ERR_INVALID;
+                }
+        }
+        // sector pool: block sectors and their single inter-subspace moves
+        pool.clear();
+        std::vector<uint8_t> t(ngas);
+        for (int b = 0; b < nblock; b++) {
+                for (int spin = 0; spin < 2; spin++) {
+                        const uint8_t *o = &rows[(size_t)b * w + spin * ngas];
+                        pool.emplace_back(o, o + ngas);
+                        for (int r = 0; r < ngas; r++) {
+                                for (int s = 0; s < ngas; s++) {
+                                        if (r == s || o[r] == 0 || o[s] >= norb[s]) continue;
+                                        std::copy(o, o + ngas, t.begin());
+                                        t[r]--;
+                                        t[s]++;
+                                        pool.push_back(t);
+                                }
+                        }
+                }
+        }
+        std::sort(pool.begin(), pool.end());
+        pool.erase(std::unique(pool.begin(), pool.end()), pool.end());
+        if (pool.size() > 65534) {
+                fcis::set_error("%s: more than 65534 sectors", who);
+                return FCIS_ERR_INVALID;
+        }
+        g->nsector = (int)pool.size();
+        g->occ.resize((size_t)g->nsector * ngas);
+        g->stride.resize((size_t)g->nsector * ngas);
+        g->nstr.resize(g->nsector);
+        for (int s = 0; s < g->nsector; s++) {
+                int64_t acc = 1;
+                for (int k = ngas - 1; k >= 0; k--) {
+                        g->occ[(size_t)s * ngas + k] = pool[s][k];
+                        g->stride[(size_t)s * ngas + k] = acc;
+                        acc *= fcis::binom_host(norb[k], pool[s][k]);
+                }
+                g->nstr[s] = acc;
+        }
+        auto find = [&](const uint8_t *o) {
+                std::vector<uint8_t> key(o, o + ngas);
+                return (int)(std::lower_bound(pool.begin(), pool.end(), key) - pool.begin());
+        };
+        g->nblock = nblock;
+        g->boff.resize(nblock);
+        g->bsa.resize(nblock);
+        g->bsb.resize(nblock);
+        int64_t off = 0;
+        for (int b = 0; b < nblock; b++) {
+                g->bsa[b] = find(&rows[(size_t)b * w]);
+                g->bsb[b] = find(&rows[(size_t)b * w + ngas]);
+                g->boff[b] = off;
+                off += g->nstr[g->bsa[b]] * g->nstr[g->bsb[b]];
+        }
+        g->ndet = off;
+        return FCIS_SUCCESS;
+}
+
+namespace {
+
 // Tables of fcis_rowset for the fused CAS sigma restricted to S1 (see the
 // struct); rows sorted by chunk count, longest first, for the persistent
 // grid's round-robin.
 int build_rowset(fcis_gas_space *g, const std::vector<std::vector<uint8_t>> &pool,
                  const std::vector<uint8_t> &blk, hipStream_t st)
 {
-        const int ngas = g->ngas, ns = g->nsector;
+        const int ns = g->nsector;
         const int64_t na = g->cas->nstra, nb = g->cas->nstrb;
         const int64_t nch = (nb + FUSED_NB - 1) / FUSED_NB;
         // S1 sector pairs
         std::vector<std::vector<char>> allow(ns, std::vector<char>(ns, 0));
-        auto moves = [&](int s) {
-                std::vector<int> out{s};
-                for (int r = 0; r < ngas; r++) {
-                        for (int t = 0; t < ngas; t++) {
-                                if (r == t || pool[s][r] == 0 || pool[s][t] >= g->norb[t]) continue;
-                                std::vector<uint8_t> o = pool[s];
-                                o[r]--;
-                                o[t]++;
-                                auto it = std::lower_bound(pool.begin(), pool.end(), o);
-                                if (it != pool.end() && *it == o) out.push_back((int)(it - pool.begin()));
-                        }
-                }
-                return out;
-        };
+        auto moves = [&](int s) { return fcis::gas_moves(g, pool, s); };
         for (int b = 0; b < g->nblock; b++) {
                 (void)blk;
                 const int sa = g->bsa[b], sb = g->bsb[b];
@@ -389,6 +468,7 @@ void fcis_gas_space_free(fcis_gas_space_t *g)
         if (g->d_rpos) (void)hipFree(g->d_rpos);
         if (g->d_rowcl) (void)hipFree(g->d_rowcl);
         if (g->d_clist) (void)hipFree(g->d_clist);
+        if (g->nat) fcis::gas_native_free(g->nat);
         delete g;
 }
 
@@ -405,25 +485,15 @@ int fcis_gas_space_create(fcis_gas_space_t **out, const fcis_space_t *cas, int n
         fcis_gas_space *g = new (std::nothrow) fcis_gas_space();
         if (g == nullptr) return FCIS_ERR_MEMORY;
         g->cas = cas;
-        g->ngas = ngas;
-        g->na = cas->na;
-        g->nb = cas->nb;
-        g->d_map = nullptr;
-        g->d_rows = g->d_rpos = nullptr;
-        g->d_rowcl = nullptr;
-        g->d_clist = nullptr;
-        int tot = 0;
-        for (int k = 0; k < ngas; k++) {
-                if (norb[k] < 1 || norb[k] > 31) {
-                        fcis_gas_space_free(g);
-                        fcis::set_error("fcis_gas_space_create: subspace %d has %d orbitals", k,
-                                        norb[k]);
-                        return FCIS_ERR_INVALID;
-                }
-                g->norb[k] = norb[k];
-                g->start[k] = tot;
-                tot += norb[k];
+        fcis::SectorPool pool;
+        int r0 = fcis::gas_layout(g, ngas, norb, cas->na, cas->nb, nblock, block_occ, pool,
+                                  "fcis_gas_space_create");
+        if (r0 != FCIS_SUCCESS) {
+                fcis_gas_space_free(g);
+                return r0;
         }
+        int tot = 0;
+        for (int k = 0; k < ngas; k++) tot += norb[k];
         if (tot != cas->norb) {
                 fcis_gas_space_free(g);
                 fcis::set_error("fcis_gas_space_create: subspaces hold %d orbitals, the CAS "
@@ -436,94 +506,12 @@ int fcis_gas_space_create(fcis_gas_space_t **out, const fcis_space_t *cas, int n
                 fcis::set_error("fcis_gas_space_create: CAS space of 2^32 or more determinants");
                 return FCIS_ERR_INVALID;
         }
-        // blocks: valid occupations, electron counts, strictly increasing
-        const int w = 2 * ngas;
-        std::vector<uint8_t> rows((size_t)nblock * w);
-        for (int b = 0; b < nblock; b++) {
-                int sa = 0, sb = 0;
-                for (int k = 0; k < w; k++) {
-                        int v = block_occ[(size_t)b * w + k];
-                        if (v < 0 || v > norb[k % ngas]) {
-                                fcis_gas_space_free(g);
-                                fcis::set_error("fcis_gas_space_create: block %d occupation "
-                                                "out of range", b);
-                                return FCIS_ERR_INVALID;
-                        }
-                        rows[(size_t)b * w + k] = (uint8_t)v;
-                        (k < ngas ? sa : sb) += v;
-                }
-                if (sa != g->na || sb != g->nb) {
-                        fcis_gas_space_free(g);
-                        fcis::set_error("fcis_gas_space_create: block %d holds (%d, %d) "
-                                        "electrons, the space (%d, %d)", b, sa, sb, g->na, g->nb);
-                        return FCIS_ERR_INVALID;
-                }
-                if (b > 0 && lex_cmp(&rows[(size_t)(b - 1) * w], &rows[(size_t)b * w], w) >= 0) {
-                        fcis_gas_space_free(g);
-                        fcis::set_error("fcis_gas_space_create: blocks must be unique and "
-                                        "sorted (block %d)", b);
-                        return FCIS_ERR_INVALID;
-                }
-        }
-        // sector pool: block sectors and their single inter-subspace moves
-        std::vector<std::vector<uint8_t>> pool;
-        std::vector<uint8_t> t(ngas);
-        for (int b = 0; b < nblock; b++) {
-                for (int spin = 0; spin < 2; spin++) {
-                        const uint8_t *o = &rows[(size_t)b * w + spin * ngas];
-                        pool.emplace_back(o, o + ngas);
-                        for (int r = 0; r < ngas; r++) {
-                                for (int s = 0; s < ngas; s++) {
-                                        if (r == s || o[r] == 0 || o[s] >= norb[s]) continue;
-                                        std::copy(o, o + ngas, t.begin());
-                                        t[r]--;
-                                        t[s]++;
-                                        pool.push_back(t);
-                                }
-                        }
-                }
-        }
-        std::sort(pool.begin(), pool.end());
-        pool.erase(std::unique(pool.begin(), pool.end()), pool.end());
-        if (pool.size() > 65534) {
-                fcis_gas_space_free(g);
-                fcis::set_error("fcis_gas_space_create: more than 65534 sectors");
-                return FCIS_ERR_INVALID;
-        }
-        g->nsector = (int)pool.size();
-        g->occ.resize((size_t)g->nsector * ngas);
-        g->stride.resize((size_t)g->nsector * ngas);
-        g->nstr.resize(g->nsector);
-        for (int s = 0; s < g->nsector; s++) {
-                int64_t acc = 1;
-                for (int k = ngas - 1; k >= 0; k--) {
-                        g->occ[(size_t)s * ngas + k] = pool[s][k];
-                        g->stride[(size_t)s * ngas + k] = acc;
-                        acc *= fcis::binom_host(norb[k], pool[s][k]);
-                }
-                g->nstr[s] = acc;
-        }
-        auto find = [&](const uint8_t *o) {
-                std::vector<uint8_t> key(o, o + ngas);
-                return (int)(std::lower_bound(pool.begin(), pool.end(), key) - pool.begin());
-        };
-        g->nblock = nblock;
-        g->boff.resize(nblock);
-        g->bsa.resize(nblock);
-        g->bsb.resize(nblock);
-        int64_t off = 0;
-        for (int b = 0; b < nblock; b++) {
-                g->bsa[b] = find(&rows[(size_t)b * w]);
-                g->bsb[b] = find(&rows[(size_t)b * w + ngas]);
-                g->boff[b] = off;
-                off += g->nstr[g->bsa[b]] * g->nstr[g->bsb[b]];
-        }
-        g->ndet = off;
         if (g->ndet >= ((int64_t)1 << 32)) {
                 fcis_gas_space_free(g);
                 fcis::set_error("fcis_gas_space_create: 2^32 or more determinants");
                 return FCIS_ERR_INVALID;
         }
+        std::vector<uint8_t> rows;      // unused by the CAS row tables
         // CAS string addresses (in the CAS space's own order) of every
         // sector used by a block, per spin
         std::vector<int64_t> base[2] = {std::vector<int64_t>(g->nsector, -1),
@@ -536,7 +524,7 @@ int fcis_gas_space_create(fcis_gas_space_t **out, const fcis_space_t *cas, int n
                         base[spin][s] = (int64_t)secaddr.size();
                         for (int64_t a = 0; a < g->nstr[s]; a++) {
                                 secaddr.push_back(
-                                        (uint32_t)fcis::space_addr(cas, spin, sector_string(g, s, a)));
+                                        (uint32_t)fcis::space_addr(cas, spin, fcis::gas_sector_string(g, s, a)));
                         }
                 }
         }
@@ -674,7 +662,7 @@ int fcis_gas_addr2str(const fcis_gas_space_t *g, int sector, int64_t addr, uint6
                 fcis::set_error("fcis_gas_addr2str: invalid sector or address");
                 return FCIS_ERR_INVALID;
         }
-        *str = sector_string(g, sector, addr);
+        *str = fcis::gas_sector_string(g, sector, addr);
         return FCIS_SUCCESS;
 }
 
@@ -708,6 +696,7 @@ int fcis_gas_det_index(const fcis_gas_space_t *g, int64_t n, const int64_t *d_ad
                 fcis::set_error("fcis_gas_det_index: invalid argument");
                 return FCIS_ERR_INVALID;
         }
+        if (g->cas == nullptr) return no_cas("fcis_gas_det_index");
         if (n == 0) return FCIS_SUCCESS;
         hipLaunchKernelGGL(k_gas_det_index, dim3(grid_for(n)), dim3(GAS_THREADS), 0,
                            (hipStream_t)stream, g->d_map, g->ndet, n, d_addr, d_cas_addr);
@@ -722,6 +711,7 @@ int fcis_gas_embed(const fcis_gas_space_t *g, const double *d_gas, double *d_cas
                 fcis::set_error("fcis_gas_embed: invalid argument");
                 return FCIS_ERR_INVALID;
         }
+        if (g->cas == nullptr) return no_cas("fcis_gas_embed");
         return embed(g, d_gas, d_cas, nvec, (hipStream_t)stream);
 }
 
@@ -732,6 +722,7 @@ int fcis_gas_project(const fcis_gas_space_t *g, const double *d_cas, double *d_g
                 fcis::set_error("fcis_gas_project: invalid argument");
                 return FCIS_ERR_INVALID;
         }
+        if (g->cas == nullptr) return no_cas("fcis_gas_project");
         return project(g, d_cas, d_gas, nvec, (hipStream_t)stream);
 }
 
@@ -743,6 +734,7 @@ int fcis_gas_make_hdiag(const fcis_gas_space_t *g, const double *h1e, const doub
                 return FCIS_ERR_INVALID;
         }
         hipStream_t st = (hipStream_t)stream;
+        if (g->cas == nullptr) return fcis::gas_native_hdiag(g, h1e, eri, d_hdiag, st);
         Scratch tmp(st);
         FCIS_HIP(fcis::dev_malloc(&tmp.p, 8 * (size_t)g->ncas));
         int r = fcis_make_hdiag(g->cas, h1e, eri, tmp.p, stream);
@@ -759,6 +751,8 @@ int fcis_gas_trans_rdm12s(const fcis_gas_space_t *g, const double *d_bra, const 
                 return FCIS_ERR_INVALID;
         }
         hipStream_t st = (hipStream_t)stream;
+        if (g->cas == nullptr)
+                return fcis::gas_native_rdm(g, d_bra, d_ket, dm1a, dm1b, dm2aa, dm2ab, dm2bb, st);
         Scratch tmp(st);
         FCIS_HIP(fcis::dev_malloc(&tmp.p, 16 * (size_t)g->ncas));
         int r = embed(g, d_bra, tmp.p, 1, st);
@@ -775,6 +769,7 @@ void fcis_gas_sigma_plan_free(fcis_gas_sigma_plan_t *p)
         if (p == nullptr) return;
         if (p->cas_plan) fcis_sigma_plan_free(p->cas_plan);
         if (p->d_a) (void)hipFree(p->d_a);
+        if (p->nat) fcis::gas_native_plan_free(p->nat);
         delete p;
 }
 
@@ -791,6 +786,16 @@ int fcis_gas_sigma_plan_create(fcis_gas_sigma_plan_t **out, const fcis_gas_space
         p->gas = g;
         p->cas_plan = nullptr;
         p->d_a = p->d_b = nullptr;
+        p->nat = nullptr;
+        if (g->cas == nullptr) {
+                int r = fcis::gas_native_plan_create(&p->nat, g, h2e, precision, workspace_bytes);
+                if (r != FCIS_SUCCESS) {
+                        fcis_gas_sigma_plan_free(p);
+                        return r;
+                }
+                *out = p;
+                return FCIS_SUCCESS;
+        }
         if (fcis::dev_malloc(&p->d_a, 16 * (size_t)g->ncas) != hipSuccess) {
                 fcis_gas_sigma_plan_free(p);
                 fcis::set_error("fcis_gas_sigma_plan_create: device allocation failed");
@@ -815,6 +820,7 @@ int fcis_gas_sigma_execute(const fcis_gas_space_t *g, fcis_sigma_plan_t *cas_pla
                                 "arrays and a plan on the space's CAS space are required)");
                 return FCIS_ERR_INVALID;
         }
+        if (g->cas == nullptr) return no_cas("fcis_gas_sigma_execute");
         hipStream_t st = (hipStream_t)stream;
         double *a = d_work, *b = d_work + g->ncas;
         int r = embed(g, d_ci0, a, 1, st);
@@ -833,6 +839,7 @@ int fcis_gas_contract_ss(const fcis_gas_space_t *g, const fcis_spin_plan_t *spin
                 fcis::set_error("fcis_gas_contract_ss: invalid argument");
                 return FCIS_ERR_INVALID;
         }
+        if (g->cas == nullptr) return no_cas("fcis_gas_contract_ss");
         hipStream_t st = (hipStream_t)stream;
         double *a = d_work, *b = d_work + g->ncas;
         int r = embed(g, d_ci0, a, 1, st);
@@ -848,6 +855,7 @@ int fcis_gas_rows_info(const fcis_gas_space_t *g, int64_t *nrows, int64_t *nchun
                 fcis::set_error("fcis_gas_rows_info: null space");
                 return FCIS_ERR_INVALID;
         }
+        if (g->cas == nullptr) return no_cas("fcis_gas_rows_info");
         if (nrows) *nrows = g->rs.nrows;
         if (nchunks) *nchunks = g->nchunk_rows;
         if (nchunks_cas) *nchunks_cas = g->cas->nstra * ((g->cas->nstrb + FUSED_NB - 1) / FUSED_NB);
@@ -862,7 +870,103 @@ int fcis_gas_sigma_plan_execute(fcis_gas_sigma_plan_t *p, const double *d_ci0, d
                                 "output must be distinct device arrays)");
                 return FCIS_ERR_INVALID;
         }
+        if (p->nat) return fcis::gas_native_plan_execute(p->nat, d_ci0, d_ci1, (hipStream_t)stream);
         return fcis_gas_sigma_execute(p->gas, p->cas_plan, d_ci0, d_ci1, p->d_a, stream);
+}
+
+int fcis_gas_space_create_native(fcis_gas_space_t **out, int ngas, const int *norb, int na,
+                                 int nb, int nblock, const int *block_occ, void *stream)
+{
+        if (out == nullptr || norb == nullptr || block_occ == nullptr || ngas < 1 ||
+            ngas > GAS_MAXNGAS || nblock < 1 || na < 0 || nb < 0) {
+                fcis::set_error("fcis_gas_space_create_native: invalid argument");
+                return FCIS_ERR_INVALID;
+        }
+        *out = nullptr;
+        fcis_gas_space *g = new (std::nothrow) fcis_gas_space();
+        if (g == nullptr) return FCIS_ERR_MEMORY;
+        g->cas = nullptr;
+        fcis::SectorPool pool;
+        int r = fcis::gas_layout(g, ngas, norb, na, nb, nblock, block_occ, pool,
+                                 "fcis_gas_space_create_native");
+        if (r == FCIS_SUCCESS) r = fcis::gas_native_create(g, pool, (hipStream_t)stream);
+        if (r != FCIS_SUCCESS) {
+                fcis_gas_space_free(g);
+                return r;
+        }
+        *out = g;
+        return FCIS_SUCCESS;
+}
+
+int fcis_gas_det_strings(const fcis_gas_space_t *g, int64_t n, const int64_t *addr,
+                         uint64_t *stra, uint64_t *strb)
+{
+        if (g == nullptr || n < 0 || (n > 0 && (addr == nullptr || stra == nullptr ||
+                                                strb == nullptr))) {
+                fcis::set_error("fcis_gas_det_strings: invalid argument");
+                return FCIS_ERR_INVALID;
+        }
+        for (int64_t i = 0; i < n; i++) {
+                const int64_t d = addr[i];
+                if (d < 0 || d >= g->ndet) {
+                        fcis::set_error("fcis_gas_det_strings: determinant %lld outside the space",
+                                        (long long)d);
+                        return FCIS_ERR_INVALID;
+                }
+                const int b = (int)(std::upper_bound(g->boff.begin(), g->boff.end(), d) -
+                                    g->boff.begin()) - 1;
+                const int64_t nbs = g->nstr[g->bsb[b]];
+                const int64_t ia = (d - g->boff[b]) / nbs, ib = d - g->boff[b] - ia * nbs;
+                stra[i] = fcis::gas_sector_string(g, g->bsa[b], ia);
+                strb[i] = fcis::gas_sector_string(g, g->bsb[b], ib);
+        }
+        return FCIS_SUCCESS;
+}
+
+int fcis_gas_pspace(const fcis_gas_space_t *g, const double *h1e, const double *eri, int m,
+                    const int64_t *addr, double *d_h0, void *stream)
+{
+        if (g == nullptr || h1e == nullptr || eri == nullptr || m < 0 ||
+            (m > 0 && (addr == nullptr || d_h0 == nullptr))) {
+                fcis::set_error("fcis_gas_pspace: invalid argument");
+                return FCIS_ERR_INVALID;
+        }
+        if ((int64_t)m * m > ((int64_t)1 << 31)) {
+                fcis::set_error("fcis_gas_pspace: %d determinants exceed the 46340 limit", m);
+                return FCIS_ERR_INVALID;
+        }
+        if (m == 0) return FCIS_SUCCESS;
+        std::vector<uint64_t> s(2 * (size_t)m);
+        int r = fcis_gas_det_strings(g, m, addr, s.data(), s.data() + m);
+        if (r != FCIS_SUCCESS) return r;
+        hipStream_t st = (hipStream_t)stream;
+        uint64_t *d_s = nullptr;
+        if (fcis::dev_malloc(&d_s, 16 * (size_t)m) != hipSuccess) {
+                fcis::set_error("fcis_gas_pspace: device allocation failed");
+                return FCIS_ERR_MEMORY;
+        }
+        int n = 0;
+        for (int k = 0; k < g->ngas; k++) n += g->norb[k];
+        if (hipMemcpyAsync(d_s, s.data(), 16 * (size_t)m, hipMemcpyHostToDevice, st) != hipSuccess) {
+                fcis::set_error("fcis_gas_pspace: upload failed");
+                r = FCIS_ERR_DEVICE;
+        } else {
+                r = fcis::pspace_strings(n, m, d_s, d_s + m, h1e, eri, d_h0, st);
+        }
+        (void)hipStreamSynchronize(st);
+        (void)hipFree(d_s);
+        return r;
+}
+
+int fcis_gas_native_info(const fcis_gas_space_t *g, int *native, int64_t *nclosure,
+                         int64_t *maxrow, int64_t *nstra, int64_t *nstrb)
+{
+        if (g == nullptr) {
+                fcis::set_error("fcis_gas_native_info: null space");
+                return FCIS_ERR_INVALID;
+        }
+        if (native) *native = g->cas == nullptr;
+        return fcis::gas_native_info(g, nclosure, maxrow, nstra, nstrb);
 }
 
 }  // extern "C"

@@ -1,0 +1,1043 @@
+// Sector-native restricted (GAS) spaces: sigma, diagonal, RDMs and
+// H blocks on the GAS determinants without a CAS embedding, so that the
+// orbital limit is the reference's (63 in all subspaces, 31 per subspace,
+// pyscf/lib/mcscf/fci_gas.h:24-27) and device memory scales with the GAS
+// space, not with the CAS space of sum(norb) orbitals.
+//
+// The restricted Hamiltonian is P H P (the identity the reference's test
+// checks libfci_gas against, pyscf/mcscf/test/test_gasci.py:522-575).  With
+// the absorbed pair matrix h and the symmetric pair operators
+// F_pq = E_pq + E_qp (p != q), F_pp = E_pp, E = E^alpha + E^beta,
+//   sigma_I = sum_{pq,rs} h[rs,pq] sum_K <I|F_rs|K> <K|F_pq|c>,
+// where K runs over every determinant one excitation (or none) from a GAS
+// determinant: the closure S1 = D plus the blocks (sa', sb) and (sa, sb')
+// with sa' (sb') one inter-subspace move from a block's alpha (beta) sector
+// (the reference's sector pool, fci_string_gas.c:588-791).  The device path
+// is the t1 / GEMM / pull scheme of the CAS three-kernel sigma on S1:
+//   t1[pq, K]  = sum over the links (pq, J, s) of K's alpha and beta
+//                strings with J in D: s c[J]                     (gather)
+//   vt1        = h . t1                                          (rocBLAS GEMM)
+//   sigma[I]   = sum over the links (pq, K, s) of I's strings: s vt1[pq, K]
+//                                                                (pull)
+// Every sigma element is produced by one thread (destination-owned, no
+// atomics, bitwise reproducible).  S1 is laid out row-major over alpha
+// strings: the strings of the alpha sectors of S1 in sector order are the
+// rows, a row holds the strings of its sector's S1 beta sectors one after
+// the other, and t1 / vt1 are tiled over whole rows to fit the workspace.
+//
+// String tables per spin: every string of the spin's S1 sectors, in sector
+// order, gets a global index g (sector base + address in the sector); the
+// pair map pmap[pq][g] is the global index of a_p^+ a_q |g> (or a_q^+ a_p,
+// whichever is nonzero; the string itself for p == q in the string) with
+// the sign in bit 31, or -1 when the result is zero or leaves the spin's
+// S1 sectors.  The link lists hold the same entries compactly, k(n-k+1)
+// slots per string of k electrons (diagonal (q,q) first for each occupied
+// q, then (p,q) for every empty p): lk[slot][g] as in pmap and lkp[slot][g]
+// the pair index, -1 where the result leaves the S1 sectors.  The pull
+// walks them (npair may exceed the link count many times over at large
+// orbital counts); the gather writes every t1 row and reads the pair map.
+// RDMs use the same closure with ordered pairs:
+//   D^s_v[d n + c, K] = <K| a_d^+ a_c |v> (spin s),
+//   <bra| a_p^+ a_q a_r^+ a_s |ket> = sum_K D_bra[q n + p, K] D_ket[r n + s, K],
+//   <bra| a_p^+ a_q |ket>           = sum_{K in D} bra_K D_ket[p n + q, K],
+// assembled into dm1[p,q] = <q^+ p> and dm2[p,q,r,s] = <p^+ r^+ s q>
+// (fci_gas.h:255-268) with rocBLAS GEMMs over K.
+#include <stdlib.h>
+#include <string.h>
+#include <algorithm>
+#include <map>
+#include <new>
+#include <set>
+#include <vector>
+
+#include <rocblas/rocblas.h>
+
+#include "gas.h"
+
+namespace {
+
+constexpr int NT = 256;
+
+__constant__ fcis::BinomTable c_binom = fcis::make_binom_table();
+
+struct SubInfo {
+        int ngas;
+        int norb[GAS_MAXNGAS], start[GAS_MAXNGAS];
+        int8_t sub_of[64];              // subspace of each orbital
+};
+
+// Device view of the native tables (kernel argument).
+struct NatView {
+        int n, npair, nsecA, nsecB, nblock;
+        int64_t nstrA, nstrB, ndet;
+        const uint64_t *strA, *strB;
+        const uint16_t *ssecA, *ssecB;
+        const uint32_t *saddrA, *saddrB;
+        const int32_t *pmapA, *pmapB;
+        const int32_t *lkA, *lkB;       // [nlink][nstr] target | sign << 31, or -1
+        const int16_t *lkpA, *lkpB;     // [nlink][nstr] pair index
+        int nlinkA, nlinkB;
+        const int64_t *sbaseA, *sbaseB;
+        const int64_t *cbase, *rowlen;
+        const int32_t *segptr, *segsec;
+        const int64_t *segoff;
+        const int32_t *dblk;
+        const int64_t *s1bo;
+        const int64_t *boff;
+        const int32_t *bsa, *bsb;
+        const int64_t *bnb;
+};
+
+__device__ __forceinline__ int upper_index(const int64_t *__restrict__ a, int n, int64_t x)
+{
+        int lo = 0, hi = n - 1;
+        while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (a[mid] <= x) lo = mid;
+                else hi = mid - 1;
+        }
+        return lo;
+}
+
+// Position of S1 column col: row r (global alpha string), its local alpha
+// sector A and address a, the local beta sector B of the column and the
+// beta address b.
+struct S1Pos {
+        int64_t r, a, b;
+        int A, B;
+};
+
+__device__ __forceinline__ S1Pos s1_locate(const NatView &v, int64_t col)
+{
+        S1Pos p;
+        p.A = upper_index(v.cbase, v.nsecA, col);
+        int64_t j = col - v.cbase[p.A];
+        const int64_t L = v.rowlen[p.A];
+        p.a = j / L;
+        j -= p.a * L;
+        const int s0 = v.segptr[p.A], s1 = v.segptr[p.A + 1];
+        const int seg = s0 + upper_index(v.segoff + s0, s1 - s0, j);
+        p.B = v.segsec[seg];
+        p.b = j - v.segoff[seg];
+        p.r = v.sbaseA[p.A] + p.a;
+        return p;
+}
+
+struct DPos {
+        int64_t a, b;
+        int A, B, blk;
+};
+
+__device__ __forceinline__ DPos d_locate(const NatView &v, int64_t d)
+{
+        DPos q;
+        q.blk = upper_index(v.boff, v.nblock, d);
+        const int64_t nb = v.bnb[q.blk], off = d - v.boff[q.blk];
+        q.a = off / nb;
+        q.b = off - q.a * nb;
+        q.A = v.bsa[q.blk];
+        q.B = v.bsb[q.blk];
+        return q;
+}
+
+// Index of GAS determinant (alpha local sector A, address a; beta local
+// sector B, address b), or -1 when (A, B) is not a block of D.
+__device__ __forceinline__ int64_t d_index(const NatView &v, int A, int64_t a, int B, int64_t b)
+{
+        const int blk = v.dblk[(int64_t)A * v.nsecB + B];
+        return blk < 0 ? -1 : v.boff[blk] + a * v.bnb[blk] + b;
+}
+
+__device__ __forceinline__ uint64_t unrank_dev(int n, int k, int64_t addr)
+{
+        uint64_t s = 0;
+        int e = k;
+        for (int p = n - 1; p >= 0 && e > 0; p--) {
+                const int64_t c = c_binom.v[p][e];
+                if (c <= addr) {
+                        s |= 1ull << p;
+                        addr -= c;
+                        e--;
+                }
+        }
+        return s;
+}
+
+__device__ __forceinline__ int64_t rank_dev(uint64_t s)
+{
+        int64_t a = 0;
+        int i = 1;
+        while (s) {
+                const int p = __ffsll((long long)s) - 1;
+                a += c_binom.v[p][i++];
+                s &= s - 1;
+        }
+        return a;
+}
+
+// Strings of one spin: g -> (string, local sector, address in sector).
+__global__ __launch_bounds__(NT) void k_nat_strings(SubInfo si, int nsec, int64_t nstr,
+                                                    const int64_t *__restrict__ sbase,
+                                                    const uint8_t *__restrict__ occ,
+                                                    const int64_t *__restrict__ stride,
+                                                    uint64_t *__restrict__ str,
+                                                    uint16_t *__restrict__ ssec,
+                                                    uint32_t *__restrict__ saddr)
+{
+        const int64_t g = (int64_t)blockIdx.x * NT + threadIdx.x;
+        if (g >= nstr) return;
+        const int s = upper_index(sbase, nsec, g);
+        int64_t addr = g - sbase[s];
+        saddr[g] = (uint32_t)addr;
+        ssec[g] = (uint16_t)s;
+        uint64_t x = 0;
+        for (int k = 0; k < si.ngas; k++) {
+                const int64_t st = stride[(int64_t)s * si.ngas + k];
+                const int64_t q = addr / st;
+                addr -= q * st;
+                x |= unrank_dev(si.norb[k], occ[(int64_t)s * si.ngas + k], q) << si.start[k];
+        }
+        str[g] = x;
+}
+
+// Pair map of one spin (see the file comment).  mv[(s * ngas + r) * ngas + t]
+// is the local sector of sector s with one electron moved from subspace r
+// to subspace t, or -1.
+__global__ __launch_bounds__(NT) void k_nat_pmap(SubInfo si, int n, int64_t nstr,
+                                                 const uint64_t *__restrict__ str,
+                                                 const uint16_t *__restrict__ ssec,
+                                                 const int64_t *__restrict__ sbase,
+                                                 const int64_t *__restrict__ stride,
+                                                 const int32_t *__restrict__ mv,
+                                                 int32_t *__restrict__ pmap,
+                                                 int32_t *__restrict__ lk,
+                                                 int16_t *__restrict__ lkp)
+{
+        const int64_t g = (int64_t)blockIdx.x * NT + threadIdx.x;
+        if (g >= nstr) return;
+        const uint64_t S = str[g];
+        const int s = ssec[g];
+        const int ng = si.ngas;
+        int slot = 0;
+        for (uint64_t occ = S; occ; occ &= occ - 1) {
+                const int q = __ffsll((long long)occ) - 1;
+                pmap[(int64_t)fcis::pair_index(q, q) * nstr + g] = (int32_t)g;
+                lk[(int64_t)slot * nstr + g] = (int32_t)g;
+                lkp[(int64_t)slot * nstr + g] = (int16_t)fcis::pair_index(q, q);
+                slot++;
+                const int kq = si.sub_of[q];
+                for (int p = 0; p < n; p++) {
+                        if ((S >> p) & 1) continue;
+                        const int kp = si.sub_of[p];
+                        const int ts = kq == kp ? s : mv[((int64_t)s * ng + kq) * ng + kp];
+                        const int this_slot = slot++;
+                        lkp[(int64_t)this_slot * nstr + g] = (int16_t)fcis::pair_index(p, q);
+                        if (ts < 0) {
+                                lk[(int64_t)this_slot * nstr + g] = -1;
+                                continue;
+                        }
+                        const uint64_t T = (S & ~(1ull << q)) | (1ull << p);
+                        int64_t addr = 0;
+                        for (int k = 0; k < ng; k++) {
+                                const uint64_t sub = (T >> si.start[k]) & ((1ull << si.norb[k]) - 1);
+                                addr += rank_dev(sub) * stride[(int64_t)ts * ng + k];
+                        }
+                        const int lo = p < q ? p : q, hi = p < q ? q : p;
+                        const uint64_t m = hi > lo + 1 ? ((1ull << hi) - (1ull << (lo + 1))) : 0ull;
+                        const uint32_t neg = (uint32_t)(__popcll(S & m) & 1);
+                        const uint32_t t = (uint32_t)(sbase[ts] + addr) | (neg << 31);
+                        pmap[(int64_t)fcis::pair_index(p, q) * nstr + g] = (int32_t)t;
+                        lk[(int64_t)this_slot * nstr + g] = (int32_t)t;
+                }
+        }
+}
+
+__device__ __forceinline__ int32_t pm_index(int32_t e) { return e & 0x7fffffff; }
+
+// t1[pq][k] for the S1 columns col0 + k, k < ncol.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_nat_gather(NatView v, int64_t col0, int64_t ncol, int64_t ld,
+                                                   const double *__restrict__ c, T *__restrict__ t1)
+{
+        const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+        if (k >= ncol) return;
+        const S1Pos p = s1_locate(v, col0 + k);
+        const int64_t gb = v.sbaseB[p.B] + p.b;
+        const int32_t *dA = v.dblk + (int64_t)p.A * v.nsecB;
+        for (int pq = 0; pq < v.npair; pq++) {
+                double x = 0.0;
+                const int32_t ea = v.pmapA[(int64_t)pq * v.nstrA + p.r];
+                if (ea != -1) {
+                        const int32_t g = pm_index(ea);
+                        const int blk = v.dblk[(int64_t)v.ssecA[g] * v.nsecB + p.B];
+                        if (blk >= 0) {
+                                const double y = c[v.boff[blk] + (int64_t)v.saddrA[g] * v.bnb[blk] + p.b];
+                                x = ea < 0 ? -y : y;
+                        }
+                }
+                const int32_t eb = v.pmapB[(int64_t)pq * v.nstrB + gb];
+                if (eb != -1) {
+                        const int32_t g = pm_index(eb);
+                        const int blk = dA[v.ssecB[g]];
+                        if (blk >= 0) {
+                                const double y = c[v.boff[blk] + p.a * v.bnb[blk] + v.saddrB[g]];
+                                x += eb < 0 ? -y : y;
+                        }
+                }
+                t1[(int64_t)pq * ld + k] = (T)x;
+        }
+}
+
+// sigma[d] (+)= pulls from vt1 of the S1 rows [r0, r1) (columns from col0).
+template <typename T>
+__global__ __launch_bounds__(NT) void k_nat_pull(NatView v, int64_t r0, int64_t r1, int64_t col0,
+                                                 int64_t ld, int accumulate,
+                                                 const T *__restrict__ vt1, double *__restrict__ sigma)
+{
+        const int64_t d = (int64_t)blockIdx.x * NT + threadIdx.x;
+        if (d >= v.ndet) return;
+        const DPos q = d_locate(v, d);
+        const int64_t r = v.sbaseA[q.A] + q.a, gb = v.sbaseB[q.B] + q.b;
+        double acc = accumulate ? sigma[d] : 0.0;
+        for (int k = 0; k < v.nlinkA; k++) {
+                const int32_t ea = v.lkA[(int64_t)k * v.nstrA + r];
+                if (ea == -1) continue;
+                const int64_t g = pm_index(ea);
+                if (g < r0 || g >= r1) continue;
+                const int A2 = v.ssecA[g];
+                const int64_t bo = v.s1bo[(int64_t)A2 * v.nsecB + q.B];
+                if (bo < 0) continue;
+                const int pq = v.lkpA[(int64_t)k * v.nstrA + r];
+                const int64_t col = v.cbase[A2] + (int64_t)v.saddrA[g] * v.rowlen[A2] + bo + q.b;
+                const double x = (double)vt1[(int64_t)pq * ld + (col - col0)];
+                acc += ea < 0 ? -x : x;
+        }
+        if (r >= r0 && r < r1) {
+                const int64_t rowc = v.cbase[q.A] + q.a * v.rowlen[q.A] - col0;
+                const int64_t *bo = v.s1bo + (int64_t)q.A * v.nsecB;
+                for (int k = 0; k < v.nlinkB; k++) {
+                        const int32_t eb = v.lkB[(int64_t)k * v.nstrB + gb];
+                        if (eb == -1) continue;
+                        const int32_t g = pm_index(eb);
+                        const int64_t o = bo[v.ssecB[g]];
+                        if (o < 0) continue;
+                        const int pq = v.lkpB[(int64_t)k * v.nstrB + gb];
+                        const double x = (double)vt1[(int64_t)pq * ld + rowc + o + v.saddrB[g]];
+                        acc += eb < 0 ? -x : x;
+                }
+        }
+        sigma[d] = acc;
+}
+
+// Diagonal of H: sum_p h_pp over both strings, (J - K) over same-spin
+// pairs, J over opposite-spin pairs (fci_hdiag_gas.c:144-164, :678-709),
+// with J[p][q] = (pp|qq), K[p][q] = (pq|qp) from the un-absorbed integrals.
+__global__ __launch_bounds__(NT) void k_nat_hdiag(NatView v, const double *__restrict__ hjk,
+                                                  double *__restrict__ out)
+{
+        const int64_t d = (int64_t)blockIdx.x * NT + threadIdx.x;
+        if (d >= v.ndet) return;
+        const DPos q = d_locate(v, d);
+        const uint64_t sa = v.strA[v.sbaseA[q.A] + q.a], sb = v.strB[v.sbaseB[q.B] + q.b];
+        const int n = v.n;
+        const double *h = hjk, *J = hjk + n, *K = hjk + n + n * n;
+        double e = 0.0;
+        for (uint64_t x = sa; x; x &= x - 1) {
+                const int p = __ffsll((long long)x) - 1;
+                e += h[p];
+                for (uint64_t y = sa & ((1ull << p) - 1); y; y &= y - 1) {
+                        const int r = __ffsll((long long)y) - 1;
+                        e += J[p * n + r] - K[p * n + r];
+                }
+                for (uint64_t y = sb; y; y &= y - 1) e += J[p * n + __ffsll((long long)y) - 1];
+        }
+        for (uint64_t x = sb; x; x &= x - 1) {
+                const int p = __ffsll((long long)x) - 1;
+                e += h[p];
+                for (uint64_t y = sb & ((1ull << p) - 1); y; y &= y - 1) {
+                        const int r = __ffsll((long long)y) - 1;
+                        e += J[p * n + r] - K[p * n + r];
+                }
+        }
+        out[d] = e;
+}
+
+// D^SPIN_x[d n + c][k] = <K| a_d^+ a_c |x> for the S1 columns col0 + k.
+template <int SPIN>
+__global__ __launch_bounds__(NT) void k_nat_rdm_gather(NatView v, int64_t col0, int64_t ncol, int64_t ld,
+                                                       const double *__restrict__ x,
+                                                       double *__restrict__ D)
+{
+        const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+        if (k >= ncol) return;
+        const S1Pos p = s1_locate(v, col0 + k);
+        const int64_t gb = v.sbaseB[p.B] + p.b;
+        const uint64_t S = SPIN == 0 ? v.strA[p.r] : v.strB[gb];
+        const int64_t self = d_index(v, p.A, p.a, p.B, p.b);
+        const double xs = self >= 0 ? x[self] : 0.0;
+        const int n = v.n;
+        for (int dd = 0; dd < n; dd++) {
+                const bool occd = (S >> dd) & 1;
+                for (int cc = 0; cc < n; cc++) {
+                        double val = 0.0;
+                        if (occd) {
+                                if (cc == dd) {
+                                        val = xs;
+                                } else if (!((S >> cc) & 1)) {
+                                        const int pq = fcis::pair_index(cc, dd);
+                                        const int32_t e = SPIN == 0 ? v.pmapA[(int64_t)pq * v.nstrA + p.r]
+                                                                    : v.pmapB[(int64_t)pq * v.nstrB + gb];
+                                        if (e != -1) {
+                                                const int32_t g = pm_index(e);
+                                                const int64_t j = SPIN == 0
+                                                        ? d_index(v, v.ssecA[g], v.saddrA[g], p.B, p.b)
+                                                        : d_index(v, p.A, p.a, v.ssecB[g], v.saddrB[g]);
+                                                if (j >= 0) val = e < 0 ? -x[j] : x[j];
+                                        }
+                                }
+                        }
+                        D[(int64_t)(dd * n + cc) * ld + k] = val;
+                }
+        }
+}
+
+// e[k] = x at S1 column col0 + k if that determinant is in D, else 0.
+__global__ __launch_bounds__(NT) void k_nat_s1_embed(NatView v, int64_t col0, int64_t ncol,
+                                                     const double *__restrict__ x, double *__restrict__ e)
+{
+        const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+        if (k >= ncol) return;
+        const S1Pos p = s1_locate(v, col0 + k);
+        const int64_t j = d_index(v, p.A, p.a, p.B, p.b);
+        e[k] = j >= 0 ? x[j] : 0.0;
+}
+
+__global__ void k_nat_fill(int32_t *__restrict__ x, int64_t n, int32_t val)
+{
+        for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
+                x[i] = val;
+}
+
+unsigned blocks_for(int64_t n)
+{
+        return (unsigned)std::max<int64_t>(1, (n + NT - 1) / NT);
+}
+
+}  // namespace
+
+struct gas_native {
+        int n = 0, npair = 0;
+        int nsec[2] = {0, 0};
+        std::vector<int> glob[2];       // local sector -> pool sector
+        std::vector<int64_t> sbase[2];  // [nsec + 1] first global string index
+        int64_t nstr[2] = {0, 0};
+        std::vector<int64_t> rowlen;    // [nsecA] S1 row length
+        std::vector<int64_t> cbase;     // [nsecA + 1] first S1 column of the sector's rows
+        int64_t n1 = 0;                 // S1 size
+        int64_t maxrow = 0;
+        SubInfo si{};
+        NatView v{};
+        std::vector<void *> dev;        // device allocations
+};
+
+struct gas_native_plan {
+        const fcis_gas_space *g = nullptr;
+        int precision = FCIS_PREC_FP64;
+        rocblas_handle blas = nullptr;
+        void *d_h = nullptr;            // npair x npair, double or float
+        void *d_t1 = nullptr, *d_vt1 = nullptr;
+        int64_t ld = 0;                 // columns per tile
+        std::vector<int64_t> r0, c0;    // tile row and column starts (+ ends)
+};
+
+namespace {
+
+template <typename T>
+int dev_upload(gas_native *nt, T **out, const std::vector<T> &h, hipStream_t st)
+{
+        *out = nullptr;
+        if (fcis::dev_malloc(out, sizeof(T) * std::max<size_t>(h.size(), 1)) != hipSuccess) {
+                fcis::set_error("sector-native GAS space: device allocation failed");
+                return FCIS_ERR_MEMORY;
+        }
+        nt->dev.push_back(*out);
+        if (!h.empty())
+                FCIS_HIP(hipMemcpyAsync(*out, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, st));
+        return FCIS_SUCCESS;
+}
+
+template <typename T>
+int dev_alloc(gas_native *nt, T **out, size_t count)
+{
+        *out = nullptr;
+        if (fcis::dev_malloc(out, sizeof(T) * std::max<size_t>(count, 1)) != hipSuccess) {
+                fcis::set_error("sector-native GAS space: device allocation of %zu bytes failed",
+                                sizeof(T) * count);
+                return FCIS_ERR_MEMORY;
+        }
+        nt->dev.push_back(*out);
+        return FCIS_SUCCESS;
+}
+
+#define NAT_TRY(x)                                \
+        do {                                      \
+                int r_ = (x);                     \
+                if (r_ != FCIS_SUCCESS) return r_; \
+        } while (0)
+
+int build_native(fcis_gas_space *g, gas_native *nt, const fcis::SectorPool &pool, hipStream_t st)
+{
+        const int ng = g->ngas;
+        int n = 0;
+        for (int k = 0; k < ng; k++) n += g->norb[k];
+        nt->n = n;
+        nt->npair = n * (n + 1) / 2;
+        nt->si.ngas = ng;
+        for (int k = 0; k < ng; k++) {
+                nt->si.norb[k] = g->norb[k];
+                nt->si.start[k] = g->start[k];
+                for (int p = g->start[k]; p < g->start[k] + g->norb[k]; p++) nt->si.sub_of[p] = (int8_t)k;
+        }
+        // S1 sector pairs (pool ids)
+        std::set<std::pair<int, int>> s1;
+        for (int b = 0; b < g->nblock; b++) {
+                const int sa = g->bsa[b], sb = g->bsb[b];
+                for (int x : fcis::gas_moves(g, pool, sa)) s1.insert({x, sb});
+                for (int y : fcis::gas_moves(g, pool, sb)) s1.insert({sa, y});
+        }
+        std::vector<int> loc[2] = {std::vector<int>(g->nsector, -1), std::vector<int>(g->nsector, -1)};
+        {
+                std::set<int> sa, sb;
+                for (auto &e : s1) {
+                        sa.insert(e.first);
+                        sb.insert(e.second);
+                }
+                nt->glob[0].assign(sa.begin(), sa.end());
+                nt->glob[1].assign(sb.begin(), sb.end());
+        }
+        for (int spin = 0; spin < 2; spin++) {
+                nt->nsec[spin] = (int)nt->glob[spin].size();
+                nt->sbase[spin].assign(nt->nsec[spin] + 1, 0);
+                for (int i = 0; i < nt->nsec[spin]; i++) {
+                        loc[spin][nt->glob[spin][i]] = i;
+                        nt->sbase[spin][i + 1] = nt->sbase[spin][i] + g->nstr[nt->glob[spin][i]];
+                }
+                nt->nstr[spin] = nt->sbase[spin][nt->nsec[spin]];
+                if (nt->nstr[spin] >= 0x7fffffffLL) {
+                        fcis::set_error("sector-native GAS space: 2^31 or more %s strings",
+                                        spin ? "beta" : "alpha");
+                        return FCIS_ERR_INVALID;
+                }
+        }
+        const int nsa = nt->nsec[0], nsb = nt->nsec[1];
+        if ((int64_t)nsa * nsb > ((int64_t)1 << 24)) {
+                fcis::set_error("sector-native GAS space: %d x %d sector pairs exceed 2^24", nsa, nsb);
+                return FCIS_ERR_INVALID;
+        }
+        // S1 rows: per alpha sector the beta sectors of S1, in order
+        std::vector<int32_t> segptr(nsa + 1, 0), segsec;
+        std::vector<int64_t> segoff, s1bo((size_t)nsa * nsb, -1);
+        nt->rowlen.assign(nsa, 0);
+        nt->cbase.assign(nsa + 1, 0);
+        {
+                std::vector<std::vector<int>> lists(nsa);
+                for (auto &e : s1) lists[loc[0][e.first]].push_back(loc[1][e.second]);
+                for (int A = 0; A < nsa; A++) {
+                        std::sort(lists[A].begin(), lists[A].end());
+                        segptr[A] = (int32_t)segsec.size();
+                        int64_t off = 0;
+                        for (int B : lists[A]) {
+                                segsec.push_back(B);
+                                segoff.push_back(off);
+                                s1bo[(size_t)A * nsb + B] = off;
+                                off += g->nstr[nt->glob[1][B]];
+                        }
+                        nt->rowlen[A] = off;
+                        nt->maxrow = std::max(nt->maxrow, off);
+                        nt->cbase[A + 1] = nt->cbase[A] + off * g->nstr[nt->glob[0][A]];
+                }
+                segptr[nsa] = (int32_t)segsec.size();
+        }
+        nt->n1 = nt->cbase[nsa];
+        // D blocks on local sectors
+        std::vector<int32_t> dblk((size_t)nsa * nsb, -1), bsa(g->nblock), bsb(g->nblock);
+        std::vector<int64_t> boff(g->nblock + 1), bnb(g->nblock);
+        for (int b = 0; b < g->nblock; b++) {
+                bsa[b] = loc[0][g->bsa[b]];
+                bsb[b] = loc[1][g->bsb[b]];
+                dblk[(size_t)bsa[b] * nsb + bsb[b]] = b;
+                boff[b] = g->boff[b];
+                bnb[b] = g->nstr[g->bsb[b]];
+        }
+        boff[g->nblock] = g->ndet;
+        // per spin: sector occupations, strides and the move table
+        NatView &v = nt->v;
+        v.n = n;
+        v.npair = nt->npair;
+        v.nsecA = nsa;
+        v.nsecB = nsb;
+        v.nblock = g->nblock;
+        v.nstrA = nt->nstr[0];
+        v.nstrB = nt->nstr[1];
+        v.ndet = g->ndet;
+        for (int spin = 0; spin < 2; spin++) {
+                const int ns = nt->nsec[spin];
+                std::vector<uint8_t> occ((size_t)ns * ng);
+                std::vector<int64_t> stride((size_t)ns * ng);
+                std::vector<int32_t> mv((size_t)ns * ng * ng, -1);
+                for (int i = 0; i < ns; i++) {
+                        const int s = nt->glob[spin][i];
+                        for (int k = 0; k < ng; k++) {
+                                occ[(size_t)i * ng + k] = g->occ[(size_t)s * ng + k];
+                                stride[(size_t)i * ng + k] = g->stride[(size_t)s * ng + k];
+                        }
+                        for (int r = 0; r < ng; r++)
+                                for (int t = 0; t < ng; t++) {
+                                        if (r == t || pool[s][r] == 0 || pool[s][t] >= g->norb[t]) continue;
+                                        std::vector<uint8_t> o = pool[s];
+                                        o[r]--;
+                                        o[t]++;
+                                        auto it = std::lower_bound(pool.begin(), pool.end(), o);
+                                        if (it != pool.end() && *it == o)
+                                                mv[((size_t)i * ng + r) * ng + t] = loc[spin][(int)(it - pool.begin())];
+                                }
+                }
+                const int64_t ns_str = nt->nstr[spin];
+                int64_t *d_sbase;
+                uint8_t *d_occ;
+                int64_t *d_stride;
+                int32_t *d_mv;
+                uint64_t *d_str;
+                uint16_t *d_ssec;
+                uint32_t *d_saddr;
+                int32_t *d_pmap, *d_lk;
+                int16_t *d_lkp;
+                const int ke = spin ? g->nb : g->na;
+                const int nlink = ke * (n - ke + 1);
+                NAT_TRY(dev_upload(nt, &d_sbase, nt->sbase[spin], st));
+                NAT_TRY(dev_upload(nt, &d_occ, occ, st));
+                NAT_TRY(dev_upload(nt, &d_stride, stride, st));
+                NAT_TRY(dev_upload(nt, &d_mv, mv, st));
+                NAT_TRY(dev_alloc(nt, &d_str, ns_str));
+                NAT_TRY(dev_alloc(nt, &d_ssec, ns_str));
+                NAT_TRY(dev_alloc(nt, &d_saddr, ns_str));
+                NAT_TRY(dev_alloc(nt, &d_pmap, (size_t)nt->npair * ns_str));
+                NAT_TRY(dev_alloc(nt, &d_lk, (size_t)nlink * ns_str));
+                NAT_TRY(dev_alloc(nt, &d_lkp, (size_t)nlink * ns_str));
+                hipLaunchKernelGGL(k_nat_strings, dim3(blocks_for(ns_str)), dim3(NT), 0, st, nt->si, ns,
+                                   ns_str, d_sbase, d_occ, d_stride, d_str, d_ssec, d_saddr);
+                FCIS_CHECK_LAUNCH();
+                hipLaunchKernelGGL(k_nat_fill, dim3(4096), dim3(NT), 0, st, d_pmap,
+                                   (int64_t)nt->npair * ns_str, (int32_t)-1);
+                FCIS_CHECK_LAUNCH();
+                hipLaunchKernelGGL(k_nat_pmap, dim3(blocks_for(ns_str)), dim3(NT), 0, st, nt->si, n, ns_str,
+                                   d_str, d_ssec, d_sbase, d_stride, d_mv, d_pmap, d_lk, d_lkp);
+                FCIS_CHECK_LAUNCH();
+                if (spin == 0) {
+                        v.strA = d_str;
+                        v.ssecA = d_ssec;
+                        v.saddrA = d_saddr;
+                        v.pmapA = d_pmap;
+                        v.lkA = d_lk;
+                        v.lkpA = d_lkp;
+                        v.nlinkA = nlink;
+                        v.sbaseA = d_sbase;
+                } else {
+                        v.strB = d_str;
+                        v.ssecB = d_ssec;
+                        v.saddrB = d_saddr;
+                        v.pmapB = d_pmap;
+                        v.lkB = d_lk;
+                        v.lkpB = d_lkp;
+                        v.nlinkB = nlink;
+                        v.sbaseB = d_sbase;
+                }
+        }
+        int64_t *d_cbase, *d_rowlen, *d_segoff, *d_s1bo, *d_boff, *d_bnb;
+        int32_t *d_segptr, *d_segsec, *d_dblk, *d_bsa, *d_bsb;
+        NAT_TRY(dev_upload(nt, &d_cbase, nt->cbase, st));
+        NAT_TRY(dev_upload(nt, &d_rowlen, nt->rowlen, st));
+        NAT_TRY(dev_upload(nt, &d_segptr, segptr, st));
+        NAT_TRY(dev_upload(nt, &d_segsec, segsec, st));
+        NAT_TRY(dev_upload(nt, &d_segoff, segoff, st));
+        NAT_TRY(dev_upload(nt, &d_s1bo, s1bo, st));
+        NAT_TRY(dev_upload(nt, &d_dblk, dblk, st));
+        NAT_TRY(dev_upload(nt, &d_boff, boff, st));
+        NAT_TRY(dev_upload(nt, &d_bsa, bsa, st));
+        NAT_TRY(dev_upload(nt, &d_bsb, bsb, st));
+        NAT_TRY(dev_upload(nt, &d_bnb, bnb, st));
+        v.cbase = d_cbase;
+        v.rowlen = d_rowlen;
+        v.segptr = d_segptr;
+        v.segsec = d_segsec;
+        v.segoff = d_segoff;
+        v.s1bo = d_s1bo;
+        v.dblk = d_dblk;
+        v.boff = d_boff;
+        v.bsa = d_bsa;
+        v.bsb = d_bsb;
+        v.bnb = d_bnb;
+        FCIS_HIP(hipStreamSynchronize(st));
+        return FCIS_SUCCESS;
+}
+
+// Row tiles: consecutive S1 rows whose columns fit `cap`.
+void make_tiles(const gas_native *nt, int64_t cap, std::vector<int64_t> &r0, std::vector<int64_t> &c0)
+{
+        r0.clear();
+        c0.clear();
+        r0.push_back(0);
+        c0.push_back(0);
+        int64_t cur = 0;
+        for (int A = 0; A < nt->nsec[0]; A++) {
+                const int64_t L = nt->rowlen[A];
+                const int64_t nrow = nt->sbase[0][A + 1] - nt->sbase[0][A];
+                int64_t done = 0;
+                while (done < nrow) {
+                        const int64_t room = (cap - (nt->cbase[A] + done * L - c0.back())) / L;
+                        if (room <= 0) {
+                                r0.push_back(nt->sbase[0][A] + done);
+                                c0.push_back(nt->cbase[A] + done * L);
+                                continue;
+                        }
+                        done += std::min(room, nrow - done);
+                }
+                cur = nt->cbase[A + 1];
+        }
+        (void)cur;
+        r0.push_back(nt->nstr[0]);
+        c0.push_back(nt->n1);
+}
+
+// Column capacity of a tile for `per_col` bytes per column within `budget`.
+int64_t tile_columns(const gas_native *nt, uint64_t budget, uint64_t per_col)
+{
+        int64_t cols = (int64_t)(budget / per_col);
+        return std::min(cols, nt->n1);
+}
+
+struct Blas {
+        rocblas_handle h = nullptr;
+        ~Blas()
+        {
+                if (h) rocblas_destroy_handle(h);
+        }
+};
+
+struct DevBuf {
+        double *p = nullptr;
+        hipStream_t st = nullptr;
+        ~DevBuf()
+        {
+                if (p) {
+                        (void)hipStreamSynchronize(st);
+                        (void)hipFree(p);
+                }
+        }
+};
+
+}  // namespace
+
+int fcis::gas_native_create(fcis_gas_space *g, const fcis::SectorPool &pool, hipStream_t st)
+{
+        gas_native *nt = new (std::nothrow) gas_native();
+        if (nt == nullptr) return FCIS_ERR_MEMORY;
+        g->nat = nt;
+        g->ncas = 0;
+        g->d_map = nullptr;
+        g->d_rows = g->d_rpos = nullptr;
+        g->d_rowcl = nullptr;
+        g->d_clist = nullptr;
+        return build_native(g, nt, pool, st);
+}
+
+void fcis::gas_native_free(gas_native *nt)
+{
+        if (nt == nullptr) return;
+        for (void *p : nt->dev) (void)hipFree(p);
+        delete nt;
+}
+
+int fcis::gas_native_info(const fcis_gas_space *g, int64_t *nclosure, int64_t *maxrow, int64_t *nstra,
+                          int64_t *nstrb)
+{
+        const gas_native *nt = g->nat;
+        if (nclosure) *nclosure = nt ? nt->n1 : 0;
+        if (maxrow) *maxrow = nt ? nt->maxrow : 0;
+        if (nstra) *nstra = nt ? nt->nstr[0] : 0;
+        if (nstrb) *nstrb = nt ? nt->nstr[1] : 0;
+        return FCIS_SUCCESS;
+}
+
+int fcis::gas_native_hdiag(const fcis_gas_space *g, const double *h1e, const double *eri,
+                           double *d_hdiag, hipStream_t st)
+{
+        const gas_native *nt = g->nat;
+        const int n = nt->n, np = nt->npair;
+        std::vector<double> hjk((size_t)n + 2 * (size_t)n * n);
+        for (int p = 0; p < n; p++) {
+                hjk[p] = h1e[p * n + p];
+                for (int q = 0; q < n; q++) {
+                        hjk[n + p * n + q] = eri[(size_t)fcis::pair_index(p, p) * np + fcis::pair_index(q, q)];
+                        hjk[n + n * n + p * n + q] =
+                                eri[(size_t)fcis::pair_index(p, q) * np + fcis::pair_index(q, p)];
+                }
+        }
+        DevBuf d;
+        d.st = st;
+        if (fcis::dev_malloc(&d.p, 8 * hjk.size()) != hipSuccess) {
+                fcis::set_error("fcis_gas_make_hdiag: device allocation failed");
+                return FCIS_ERR_MEMORY;
+        }
+        FCIS_HIP(hipMemcpyAsync(d.p, hjk.data(), 8 * hjk.size(), hipMemcpyHostToDevice, st));
+        if (g->ndet > 0) {
+                hipLaunchKernelGGL(k_nat_hdiag, dim3(blocks_for(g->ndet)), dim3(NT), 0, st, nt->v, d.p,
+                                   d_hdiag);
+                FCIS_CHECK_LAUNCH();
+        }
+        return FCIS_SUCCESS;
+}
+
+void fcis::gas_native_plan_free(gas_native_plan *p)
+{
+        if (p == nullptr) return;
+        if (p->blas) rocblas_destroy_handle(p->blas);
+        if (p->d_h) (void)hipFree(p->d_h);
+        if (p->d_t1) (void)hipFree(p->d_t1);
+        delete p;
+}
+
+int fcis::gas_native_plan_create(gas_native_plan **out, const fcis_gas_space *g, const double *h2e,
+                                 int precision, uint64_t workspace_bytes)
+{
+        *out = nullptr;
+        if (precision != FCIS_PREC_FP64 && precision != FCIS_PREC_FP32) {
+                fcis::set_error("fcis_gas_sigma_plan_create: a sector-native space computes in FP64 "
+                                "or FP32");
+                return FCIS_ERR_INVALID;
+        }
+        const gas_native *nt = g->nat;
+        gas_native_plan *p = new (std::nothrow) gas_native_plan();
+        if (p == nullptr) return FCIS_ERR_MEMORY;
+        p->g = g;
+        p->precision = precision;
+        const size_t es = precision == FCIS_PREC_FP64 ? 8 : 4;
+        const int np = nt->npair;
+        if (workspace_bytes == 0) workspace_bytes = (uint64_t)8 << 30;
+        int64_t cols = tile_columns(nt, workspace_bytes, 2 * es * (uint64_t)np);
+        if (cols < nt->maxrow) {
+                gas_native_plan_free(p);
+                fcis::set_error("fcis_gas_sigma_plan_create: a workspace of %llu bytes holds %lld "
+                                "columns, one closure row has %lld",
+                                (unsigned long long)workspace_bytes, (long long)cols,
+                                (long long)nt->maxrow);
+                return FCIS_ERR_MEMORY;
+        }
+        cols = std::max<int64_t>(cols, 1);
+        p->ld = cols;
+        make_tiles(nt, cols, p->r0, p->c0);
+        if (fcis::dev_malloc(&p->d_t1, 2 * es * (size_t)np * cols) != hipSuccess ||
+            fcis::dev_malloc(&p->d_h, es * (size_t)np * np) != hipSuccess) {
+                gas_native_plan_free(p);
+                fcis::set_error("fcis_gas_sigma_plan_create: device allocation failed");
+                return FCIS_ERR_MEMORY;
+        }
+        p->d_vt1 = static_cast<char *>(p->d_t1) + es * (size_t)np * cols;
+        if (precision == FCIS_PREC_FP64) {
+                if (hipMemcpy(p->d_h, h2e, 8 * (size_t)np * np, hipMemcpyHostToDevice) != hipSuccess) {
+                        gas_native_plan_free(p);
+                        fcis::set_error("fcis_gas_sigma_plan_create: upload failed");
+                        return FCIS_ERR_DEVICE;
+                }
+        } else {
+                std::vector<float> hf(h2e, h2e + (size_t)np * np);
+                if (hipMemcpy(p->d_h, hf.data(), 4 * hf.size(), hipMemcpyHostToDevice) != hipSuccess) {
+                        gas_native_plan_free(p);
+                        fcis::set_error("fcis_gas_sigma_plan_create: upload failed");
+                        return FCIS_ERR_DEVICE;
+                }
+        }
+        if (rocblas_create_handle(&p->blas) != rocblas_status_success) {
+                p->blas = nullptr;
+                gas_native_plan_free(p);
+                fcis::set_error("fcis_gas_sigma_plan_create: rocblas_create_handle failed");
+                return FCIS_ERR_DEVICE;
+        }
+        *out = p;
+        return FCIS_SUCCESS;
+}
+
+namespace {
+
+template <typename T>
+int native_execute(gas_native_plan *p, const double *ci0, double *ci1, hipStream_t st)
+{
+        const fcis_gas_space *g = p->g;
+        const gas_native *nt = g->nat;
+        const int np = nt->npair;
+        T *t1 = static_cast<T *>(p->d_t1), *vt1 = static_cast<T *>(p->d_vt1);
+        const T one = 1, zero = 0;
+        if (rocblas_set_stream(p->blas, st) != rocblas_status_success) {
+                fcis::set_error("fcis_gas_sigma_plan_execute: rocblas_set_stream failed");
+                return FCIS_ERR_DEVICE;
+        }
+        const int ntile = (int)p->r0.size() - 1;
+        for (int t = 0; t < ntile; t++) {
+                const int64_t col0 = p->c0[t], ncol = p->c0[t + 1] - col0;
+                if (ncol > 0) {
+                        hipLaunchKernelGGL(k_nat_gather<T>, dim3(blocks_for(ncol)), dim3(NT), 0, st, nt->v,
+                                           col0, ncol, p->ld, ci0, t1);
+                        FCIS_CHECK_LAUNCH();
+                        rocblas_status r;
+                        if (sizeof(T) == 8)
+                                r = rocblas_dgemm(p->blas, rocblas_operation_none, rocblas_operation_none,
+                                                  (rocblas_int)ncol, np, np, (const double *)&one,
+                                                  (const double *)t1, (rocblas_int)p->ld,
+                                                  (const double *)p->d_h, np, (const double *)&zero,
+                                                  (double *)vt1, (rocblas_int)p->ld);
+                        else
+                                r = rocblas_sgemm(p->blas, rocblas_operation_none, rocblas_operation_none,
+                                                  (rocblas_int)ncol, np, np, (const float *)&one,
+                                                  (const float *)t1, (rocblas_int)p->ld,
+                                                  (const float *)p->d_h, np, (const float *)&zero,
+                                                  (float *)vt1, (rocblas_int)p->ld);
+                        if (r != rocblas_status_success) {
+                                fcis::set_error("fcis_gas_sigma_plan_execute: rocBLAS GEMM failed (status %d)",
+                                                (int)r);
+                                return FCIS_ERR_DEVICE;
+                        }
+                }
+                if (g->ndet > 0) {
+                        hipLaunchKernelGGL(k_nat_pull<T>, dim3(blocks_for(g->ndet)), dim3(NT), 0, st, nt->v,
+                                           p->r0[t], p->r0[t + 1], col0, p->ld, t > 0 ? 1 : 0, vt1, ci1);
+                        FCIS_CHECK_LAUNCH();
+                }
+        }
+        return FCIS_SUCCESS;
+}
+
+}  // namespace
+
+int fcis::gas_native_plan_execute(gas_native_plan *p, const double *d_ci0, double *d_ci1, hipStream_t st)
+{
+        if (p->ld > INT32_MAX) {
+                fcis::set_error("fcis_gas_sigma_plan_execute: tile of more than 2^31 columns");
+                return FCIS_ERR_INVALID;
+        }
+        return p->precision == FCIS_PREC_FP64 ? native_execute<double>(p, d_ci0, d_ci1, st)
+                                              : native_execute<float>(p, d_ci0, d_ci1, st);
+}
+
+int fcis::gas_native_rdm(const fcis_gas_space *g, const double *d_bra, const double *d_ket,
+                         double *dm1a, double *dm1b, double *dm2aa, double *dm2ab, double *dm2bb,
+                         hipStream_t st)
+{
+        const gas_native *nt = g->nat;
+        const int n = nt->n, n2 = n * n;
+        const bool two = dm2aa != nullptr || dm2ab != nullptr || dm2bb != nullptr;
+        const bool same = d_bra == d_ket;
+        // device: y[2][n2] (dm1 rows), M[3][n2][n2] (aa, ab, bb), then the
+        // tile: D matrices (ket a, ket b, bra a, bra b) and the embedded bra
+        const int nmat = two ? (same ? 2 : 4) : 2;
+        const size_t fixed = 2 * (size_t)n2 + (two ? 3 * (size_t)n2 * n2 : 0);
+        size_t freeb = 0, totb = 0;
+        FCIS_HIP(hipMemGetInfo(&freeb, &totb));
+        const uint64_t budget = std::min<uint64_t>((uint64_t)freeb / 2, (uint64_t)16 << 30);
+        const uint64_t per_col = 8 * ((uint64_t)nmat * n2 + 1);
+        int64_t cols = tile_columns(nt, budget > 8 * fixed ? budget - 8 * fixed : 0, per_col);
+        if (cols < nt->maxrow) cols = nt->maxrow;
+        cols = std::max<int64_t>(cols, 1);
+        if (cols > INT32_MAX) cols = INT32_MAX;
+        std::vector<int64_t> r0, c0;
+        make_tiles(nt, cols, r0, c0);
+        DevBuf buf;
+        buf.st = st;
+        if (fcis::dev_malloc(&buf.p, 8 * (fixed + (size_t)cols * ((size_t)nmat * n2 + 1))) != hipSuccess) {
+                fcis::set_error("fcis_gas_trans_rdm12s: device allocation failed");
+                return FCIS_ERR_MEMORY;
+        }
+        double *y = buf.p, *M = y + 2 * (size_t)n2;
+        double *Dk[2], *Db[2], *e;
+        double *tile = buf.p + fixed;
+        Dk[0] = tile;
+        Dk[1] = tile + (size_t)n2 * cols;
+        if (two && !same) {
+                Db[0] = tile + 2 * (size_t)n2 * cols;
+                Db[1] = tile + 3 * (size_t)n2 * cols;
+        } else {
+                Db[0] = Dk[0];
+                Db[1] = Dk[1];
+        }
+        e = tile + (size_t)nmat * n2 * cols;
+        FCIS_HIP(hipMemsetAsync(buf.p, 0, 8 * fixed, st));
+        Blas blas;
+        if (rocblas_create_handle(&blas.h) != rocblas_status_success) {
+                blas.h = nullptr;
+                fcis::set_error("fcis_gas_trans_rdm12s: rocblas_create_handle failed");
+                return FCIS_ERR_DEVICE;
+        }
+        if (rocblas_set_stream(blas.h, st) != rocblas_status_success) {
+                fcis::set_error("fcis_gas_trans_rdm12s: rocblas_set_stream failed");
+                return FCIS_ERR_DEVICE;
+        }
+        const double one = 1.0;
+        for (size_t t = 0; t + 1 < r0.size(); t++) {
+                const int64_t col0 = c0[t], ncol = c0[t + 1] - col0;
+                if (ncol <= 0) continue;
+                const unsigned gb = blocks_for(ncol);
+                hipLaunchKernelGGL(k_nat_rdm_gather<0>, dim3(gb), dim3(NT), 0, st, nt->v, col0, ncol, cols,
+                                   d_ket, Dk[0]);
+                hipLaunchKernelGGL(k_nat_rdm_gather<1>, dim3(gb), dim3(NT), 0, st, nt->v, col0, ncol, cols,
+                                   d_ket, Dk[1]);
+                if (two && !same) {
+                        hipLaunchKernelGGL(k_nat_rdm_gather<0>, dim3(gb), dim3(NT), 0, st, nt->v, col0, ncol,
+                                           cols, d_bra, Db[0]);
+                        hipLaunchKernelGGL(k_nat_rdm_gather<1>, dim3(gb), dim3(NT), 0, st, nt->v, col0, ncol,
+                                           cols, d_bra, Db[1]);
+                }
+                hipLaunchKernelGGL(k_nat_s1_embed, dim3(gb), dim3(NT), 0, st, nt->v, col0, ncol, d_bra, e);
+                FCIS_CHECK_LAUNCH();
+                rocblas_status r = rocblas_status_success;
+                for (int s = 0; s < 2 && r == rocblas_status_success; s++)
+                        r = rocblas_dgemv(blas.h, rocblas_operation_transpose, (rocblas_int)ncol, n2, &one,
+                                          Dk[s], (rocblas_int)cols, e, 1, &one, y + (size_t)s * n2, 1);
+                if (two) {
+                        const double *A[3] = {Db[0], Db[0], Db[1]}, *B[3] = {Dk[0], Dk[1], Dk[1]};
+                        for (int m = 0; m < 3 && r == rocblas_status_success; m++)
+                                r = rocblas_dgemm(blas.h, rocblas_operation_transpose, rocblas_operation_none,
+                                                  n2, n2, (rocblas_int)ncol, &one, A[m], (rocblas_int)cols, B[m],
+                                                  (rocblas_int)cols, &one, M + (size_t)m * n2 * n2, n2);
+                }
+                if (r != rocblas_status_success) {
+                        fcis::set_error("fcis_gas_trans_rdm12s: rocBLAS failed (status %d)", (int)r);
+                        return FCIS_ERR_DEVICE;
+                }
+        }
+        std::vector<double> hy(2 * (size_t)n2), hM(two ? 3 * (size_t)n2 * n2 : 0);
+        FCIS_HIP(hipMemcpyAsync(hy.data(), y, 8 * hy.size(), hipMemcpyDeviceToHost, st));
+        if (two) FCIS_HIP(hipMemcpyAsync(hM.data(), M, 8 * hM.size(), hipMemcpyDeviceToHost, st));
+        FCIS_HIP(hipStreamSynchronize(st));
+        // y[s][d n + c] = <bra| a_d^+ a_c |ket>;  M[m] column-major:
+        // M(i, j) = hM[j n2 + i] = sum_K D_bra[i, K] D_ket[j, K]
+        double *dm1[2] = {dm1a, dm1b};
+        for (int s = 0; s < 2; s++) {
+                if (dm1[s] == nullptr) continue;
+                for (int p = 0; p < n; p++)
+                        for (int q = 0; q < n; q++) dm1[s][p * n + q] = hy[(size_t)s * n2 + q * n + p];
+        }
+        double *dm2[3] = {dm2aa, dm2ab, dm2bb};
+        for (int m = 0; m < 3; m++) {
+                if (dm2[m] == nullptr) continue;
+                const double *Mm = hM.data() + (size_t)m * n2 * n2;
+                const double *ys = hy.data() + (size_t)(m == 2 ? 1 : 0) * n2;
+                for (int p = 0; p < n; p++)
+                        for (int q = 0; q < n; q++)
+                                for (int r = 0; r < n; r++)
+                                        for (int s = 0; s < n; s++) {
+                                                double v = Mm[(size_t)(r * n + s) * n2 + (q * n + p)];
+                                                if (m != 1 && q == r) v -= ys[p * n + s];
+                                                dm2[m][((size_t)(p * n + q) * n + r) * n + s] = v;
+                                        }
+        }
+        return FCIS_SUCCESS;
+}

@@ -21,7 +21,7 @@
 // h_b for beta; the UHF pspace of csf.py:305-320); fcis_pspace is h_a = h_b.
 #include <vector>
 
-#include "common.h"
+#include "gas.h"
 
 namespace {
 
@@ -126,6 +126,39 @@ __global__ __launch_bounds__(256) void k_pspace(int n, int np, int m, const int6
 }
 
 }  // namespace
+
+int fcis::pspace_strings(int n, int m, const uint64_t *d_stra, const uint64_t *d_strb,
+                         const double *h1e, const double *eri, double *d_h0, hipStream_t st)
+{
+        // determinant i is (stra[i], strb[i]): address i * m + i over an
+        // "alpha" list and a "beta" list of m strings each
+        const int np = n * (n + 1) / 2;
+        const size_t nh = (size_t)n * n, ne = (size_t)np * np;
+        std::vector<int64_t> addr(m);
+        for (int i = 0; i < m; i++) addr[i] = (int64_t)i * m + i;
+        double *d_h = nullptr;
+        FCIS_HIP(fcis::dev_malloc(&d_h, 8 * (nh + ne) + 8 * (size_t)m));
+        int64_t *d_addr = reinterpret_cast<int64_t *>(d_h + nh + ne);
+        int ret = FCIS_SUCCESS;
+        if (hipMemcpyAsync(d_h, h1e, 8 * nh, hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(d_h + nh, eri, 8 * ne, hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(d_addr, addr.data(), 8 * (size_t)m, hipMemcpyHostToDevice, st) != hipSuccess) {
+                fcis::set_error("pspace: upload failed");
+                ret = FCIS_ERR_DEVICE;
+        } else {
+                const int64_t nt = (int64_t)m * m;
+                hipLaunchKernelGGL(k_pspace, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, n, np, m,
+                                   d_addr, (int64_t)m, d_stra, d_strb, d_h, d_h, d_h + nh, d_h0);
+                hipError_t e = hipGetLastError();
+                if (e == hipSuccess) e = hipStreamSynchronize(st);
+                if (e != hipSuccess) {
+                        fcis::set_error("pspace: %s", hipGetErrorString(e));
+                        ret = FCIS_ERR_DEVICE;
+                }
+        }
+        (void)hipFree(d_h);
+        return ret;
+}
 
 extern "C" {
 

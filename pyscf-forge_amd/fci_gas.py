@@ -7,9 +7,12 @@ determinants (the identity the reference's test checks libfci_gas against,
 pyscf/mcscf/test/test_gasci.py:522-575): every sigma embeds the GAS vector
 in the CAS array on the device (gas2fci), runs the CAS sigma kernels and
 projects back (fci2gas); the diagonal and the RDMs come from the CAS kernels
-the same way.  Cost per sigma is therefore that of the CAS sigma of
-sum(gas_orbs) orbitals, and the space is limited to CAS spaces of fewer
-than 2^32 determinants that fit in device memory.
+the same way.  Up to 16 orbitals the fused CAS kernel runs only over the
+determinants within one excitation of the space.  Above that (and on
+request, ``native=True``) the space is sector-native: the sigma, diagonal,
+RDMs and H blocks run on the GAS determinants and their single-excitation
+closure with no CAS array, up to the reference's 63 orbitals
+(pyscf/lib/mcscf/fci_gas.h:24-27).
 """
 
 import ctypes
@@ -25,6 +28,13 @@ from . import direct_spin1 as _ds
 GAS_PSPACE_MATVEC_MAX = 64      # fci_gas.py:38
 GAS_SPIN_PSPACE_MATVEC_MAX = 400    # fci_gas.py:39
 GAS_INIT_NOISE_NORM = 1e-3      # fci_gas.py:40
+# sum(gas_orbs) from which spaces are sector-native by default (the CAS
+# embedding's restricted fused kernel covers up to 16 orbitals)
+NATIVE_MIN_ORBITALS = 17
+
+
+def _auto_native(norb):
+    return sum(norb) >= NATIVE_MIN_ORBITALS
 
 
 def _pair(eri, norb):
@@ -68,12 +78,15 @@ def _sector_major_space(norb, nelec):
 class GasSpace:
     """Device GAS space for subspace sizes ``norb`` and block set ``blocks``
     (GasSpace, fci_gas.py:420-567).  Usable as a context manager; close()
-    releases the device maps.  The kernels run on a CAS string space of
-    sum(norb) orbitals in sector-major address order (``cas``), so that
-    each sector is one address range; gas2fci / fci2gas use a second map to
-    the colex (pyscf) CAS layout."""
+    releases the device maps.  A CAS-embedded space (``native=False``) runs
+    its kernels on a CAS string space of sum(norb) orbitals in sector-major
+    address order (``cas``), so that each sector is one address range; a
+    sector-native space (``native=True``; the default from
+    NATIVE_MIN_ORBITALS orbitals) has no CAS space (``cas`` is None) and
+    plans of its own (``make_plan``).  gas2fci / fci2gas use a map to the
+    colex (pyscf) CAS layout, built when first needed."""
 
-    def __init__(self, norb, nelec, blocks=None, lib=None, compress_links=False):
+    def __init__(self, norb, nelec, blocks=None, lib=None, compress_links=False, native=None):
         self.norb = tuple(int(x) for x in norb)
         self.nelec = _ds._unpack_nelec(nelec)
         if blocks is None:
@@ -83,11 +96,16 @@ class GasSpace:
         self.blocks = addons_gas.normalize_blocks(blocks, len(self.norb))
         self.limits = addons_gas.check_kernel_limits(self.norb, self.nelec, self.blocks)
         self.lib = lib or _lib.load()
-        self.cas = _sector_major_space(self.norb, self.nelec)
+        self.native = _auto_native(self.norb) if native is None else bool(native)
         self._compressed = bool(compress_links)
-        self.handle = self._create(self.cas)
-        self._colex_cas = _ds.get_space(sum(self.norb), self.nelec)
-        self._colex = self.handle if self._colex_cas is self.cas else None
+        if self.native:
+            self.cas = None
+            self.handle = self._create_native()
+        else:
+            self.cas = _sector_major_space(self.norb, self.nelec)
+            self.handle = self._create(self.cas)
+        self._colex_cas_ = None
+        self._colex = None
         nd, nb, ns = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()
         _lib.check(self.lib.fcis_gas_space_info(self.handle, ctypes.byref(nd), ctypes.byref(nb),
                                                 ctypes.byref(ns)), "fcis_gas_space_info")
@@ -105,12 +123,70 @@ class GasSpace:
             "fcis_gas_space_create")
         return h
 
+    def _create_native(self):
+        norb_a = numpy.ascontiguousarray(self.norb, dtype=numpy.int32)
+        occ = numpy.ascontiguousarray(self.blocks.reshape(-1), dtype=numpy.int32)
+        h = ctypes.c_void_p()
+        ip = ctypes.POINTER(ctypes.c_int)
+        _lib.check(self.lib.fcis_gas_space_create_native(
+            ctypes.byref(h), len(self.norb), norb_a.ctypes.data_as(ip), int(self.nelec[0]),
+            int(self.nelec[1]), int(self.blocks.shape[0]), occ.ctypes.data_as(ip),
+            _lib.stream_ptr()), "fcis_gas_space_create_native")
+        return h
+
+    @property
+    def _colex_cas(self):
+        """The colex CAS space of sum(norb) orbitals (gas2fci / fci2gas)."""
+        if self._colex_cas_ is None:
+            self._colex_cas_ = _ds.get_space(sum(self.norb), self.nelec)
+        return self._colex_cas_
+
     def _colex_handle(self):
         """GAS space on the colex CAS space (for gas2fci / fci2gas)."""
         self._live()
         if self._colex is None:
-            self._colex = self._create(self._colex_cas)
+            if self.cas is not None and self._colex_cas is self.cas:
+                self._colex = self.handle
+            else:
+                self._colex = self._create(self._colex_cas)
         return self._colex
+
+    def native_info(self):
+        """(determinants within one excitation of the space, the longest
+        closure row, alpha strings, beta strings) of a sector-native space's
+        tables; zeros otherwise."""
+        nat, n1, mr = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int64()
+        na, nb = ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(self.lib.fcis_gas_native_info(self._live(), ctypes.byref(nat), ctypes.byref(n1),
+                                                 ctypes.byref(mr), ctypes.byref(na),
+                                                 ctypes.byref(nb)), "fcis_gas_native_info")
+        return n1.value, mr.value, na.value, nb.value
+
+    def make_plan(self, h2e, precision="fp64", workspace_bytes=0):
+        """Sigma plan of a sector-native space (the Davidson driver's hook);
+        None for a CAS-embedded space, whose solves run SigmaPlans on
+        ``cas`` through ``hop``."""
+        if not self.native:
+            return None
+        return GasContractPlan(self, h2e, precision, workspace_bytes)
+
+    def work_bytes(self, precision="fp64"):
+        """Device bytes a Davidson solve needs beside its vectors and plans:
+        the CAS embedding buffers (none for a sector-native space)."""
+        if self.native:
+            return 0
+        return (2 if precision == "fp64" else 4) * self.cas.ndet * 8
+
+    def det_strings(self, addr):
+        """(alpha strings, beta strings) of the GAS determinants ``addr``."""
+        a = numpy.ascontiguousarray(addr, dtype=numpy.int64).reshape(-1)
+        sa = numpy.empty(a.size, dtype=numpy.uint64)
+        sb = numpy.empty(a.size, dtype=numpy.uint64)
+        _lib.check(self.lib.fcis_gas_det_strings(
+            self._live(), a.size, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+            sa.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+            sb.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))), "fcis_gas_det_strings")
+        return sa, sb
 
     def embed_colex(self, gas_vec, cas_out, nvec=1):
         """cas_out[v] <- gas2fci(gas_vec[v]) in pyscf's (colex) CAS layout."""
@@ -229,8 +305,11 @@ class GasSpace:
         return out
 
     def hop(self, plan):
-        """x (GAS) -> P H P x through a CAS sigma plan on ``self.cas``."""
+        """x (GAS) -> P H P x through a CAS sigma plan on ``self.cas`` (a
+        sector-native space's own plan applies directly)."""
         import torch
+        if self.native:
+            return plan
         work = torch.empty(2 * self.cas.ndet, dtype=torch.float64, device="cuda")
 
         def apply(x, out):
@@ -290,11 +369,12 @@ def _givens_factors(w):
     return rots, numpy.where(numpy.diag(w) < 0, -1.0, 1.0)
 
 
-def _cached_space(norb, nelec, blocks):
-    key = (tuple(norb), tuple(nelec), numpy.asarray(blocks, dtype=numpy.int32).tobytes())
+def _cached_space(norb, nelec, blocks, native=None):
+    native = _auto_native(tuple(norb)) if native is None else bool(native)
+    key = (tuple(norb), tuple(nelec), numpy.asarray(blocks, dtype=numpy.int32).tobytes(), native)
     g = _GAS_SPACES.get(key)
     if g is None:
-        g = _GAS_SPACES[key] = GasSpace(norb, nelec, blocks)
+        g = _GAS_SPACES[key] = GasSpace(norb, nelec, blocks, native=native)
     return g
 
 
@@ -368,6 +448,10 @@ class GasContractPlan:
     def nelec(self):
         return self.gas.nelec
 
+    def __call__(self, c, out):
+        """out <- P H P c (the Davidson driver's operator)."""
+        return self.execute(c, out)
+
     def execute(self, c, out):
         """out <- P H P c for device tensors."""
         if self._plan is None or self.gas is None:
@@ -416,6 +500,10 @@ class GasSpinPlan:
 
     def __init__(self, gas):
         import torch
+        if gas.cas is None:
+            raise NotImplementedError(
+                "S^2 contraction runs through the CAS embedding; this GAS space is "
+                "sector-native (spin_square works through the RDMs)")
         self.gas = gas
         self.ndet = gas.ndet
         na, nb = gas.nelec
@@ -665,7 +753,10 @@ class FCISolver(_ds.FCISolver):
     ``gas_orbs``, ``gas_restr`` and ``gas_restr_type`` (the four forms of
     addons_gas.normalize_gas_spec); CI vectors are 1-D in GAS block order.
     ``precision`` selects the Davidson sigma arithmetic as in the CAS
-    solver (FP64 refinement for "fp32" / "bf16x3")."""
+    solver (FP64 refinement for "fp32" / "bf16x3"; a sector-native space
+    computes in fp64 or fp32).  ``native`` selects the space kind: None
+    (default) embeds spaces of up to 16 orbitals in the CAS space and makes
+    larger ones sector-native, True / False force one kind."""
 
     def __init__(self, mol=None, gas_orbs=None, gas_restr=None,
                  gas_restr_type=addons_gas.GAS_RESTR_SPIN_SUPERGROUP, lib=None):
@@ -677,6 +768,7 @@ class FCISolver(_ds.FCISolver):
         self.e_spin_penalty = None
         self.e_physical = None
         self.spin_penalty_method = None
+        self.native = None
 
     # -- spaces ----------------------------------------------------------
     def _space_spec(self, norb, nelec):
@@ -697,11 +789,12 @@ class FCISolver(_ds.FCISolver):
     def make_space(self, norb, nelec, compress_links=False):
         """An owned GasSpace for this solver's restriction."""
         orbs, nelec, D = self._space_spec(norb, nelec)
-        return GasSpace(orbs, nelec, D, lib=self.lib, compress_links=compress_links)
+        return GasSpace(orbs, nelec, D, lib=self.lib, compress_links=compress_links,
+                        native=getattr(self, "native", None))
 
     def _space(self, norb, nelec):
         orbs, nelec, D = self._space_spec(norb, nelec)
-        return _cached_space(orbs, nelec, D)
+        return _cached_space(orbs, nelec, D, getattr(self, "native", None))
 
     def make_rdm_plan(self, norb, nelec):
         return GasRDMPlan(self, norb, nelec)
@@ -802,15 +895,19 @@ class FCISolver(_ds.FCISolver):
         return addr.cpu().numpy(), self._h_block(gas, h1e, eri, addr)
 
     def _h_block(self, gas, h1e, eri, addr):
-        """H over the GAS determinants ``addr`` (device int64): their CAS
-        indices, then the Slater-Condon block (fcis_pspace)."""
+        """H over the GAS determinants ``addr``: the Slater-Condon block of
+        their strings (fcis_gas_pspace, either kind of space)."""
         import torch
-        addr = addr.to(device="cuda", dtype=torch.int64).contiguous()
-        cas_addr = torch.empty_like(addr)
-        _lib.check(self.lib.fcis_gas_det_index(gas._live(), int(addr.numel()), _lib.vptr(addr),
-                                               _lib.vptr(cas_addr), _lib.stream_ptr()),
-                   "fcis_gas_det_index")
-        return _ds._pspace_dev(gas.cas, h1e, eri, cas_addr).cpu().numpy()
+        a = numpy.ascontiguousarray(addr.cpu().numpy() if hasattr(addr, "cpu") else addr,
+                                    dtype=numpy.int64).reshape(-1)
+        n = gas.norb_total
+        h1e = numpy.ascontiguousarray(h1e, dtype=numpy.float64).reshape(n, n)
+        eri = _pair(eri, n)
+        out = torch.empty((a.size, a.size), dtype=torch.float64, device="cuda")
+        _lib.check(self.lib.fcis_gas_pspace(gas._live(), _lib.dptr(h1e), _lib.dptr(eri), a.size,
+                                            a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                            _lib.vptr(out), _lib.stream_ptr()), "fcis_gas_pspace")
+        return out.cpu().numpy()
 
     def gen_linkstr(self, norb, nelec, tril=True, spin=None):
         raise NotImplementedError(
@@ -914,6 +1011,8 @@ class FCISolver(_ds.FCISolver):
             elif ci0 is not None:
                 ci0 = self._guess_list(ci0, nroots, gas.ndet)
             from .mixed import solve
+            if gas.native and precision == "bf16x3":
+                raise ValueError("a sector-native GAS space computes sigma in fp64 or fp32")
             e, c, conv, stats = solve(self, gas.cas, h2e, pre_diag, nroots, ci0, precision,
                                       tol=tol, tol_residual=tol_residual,
                                       max_cycle=max_cycle, max_space=max_space,

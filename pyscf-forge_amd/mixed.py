@@ -33,6 +33,16 @@ def _davidson_bytes(ndet, nroots, max_space):
     return (2 * cap + 2 * nroots + 2 + nroots) * ndet * 8
 
 
+def _plan(sp, basis, h2e, precision, ws):
+    """The sigma plan of a solve: the basis's own (``basis.make_plan``, a
+    space without a CAS string space) or a SigmaPlan on ``sp``."""
+    plan = basis.make_plan(h2e, precision, ws) if hasattr(basis, "make_plan") else None
+    if plan is not None:
+        return plan
+    from .direct_spin1 import SigmaPlan
+    return SigmaPlan(sp, h2e, precision, ws)
+
+
 def solve(solver, sp, h2e, hdiag, nroots, ci0, precision, tol=1e-10, tol_residual=None,
           max_cycle=50, max_space=12, lindep=1e-14, level_shift=1e-3, verbose=0,
           basis=None, precond=None, follow_state=True):
@@ -45,7 +55,6 @@ def solve(solver, sp, h2e, hdiag, nroots, ci0, precision, tol=1e-10, tol_residua
     reference's solvers switch on (fci_gas.py:1350 without a spin penalty,
     csf.py:489)."""
     import torch
-    from .direct_spin1 import SigmaPlan
 
     lib = _lib.load()
     n = sp.ndet if basis is None else int(basis.n)
@@ -81,14 +90,15 @@ def solve(solver, sp, h2e, hdiag, nroots, ci0, precision, tol=1e-10, tol_residua
     need = _davidson_bytes(n, nroots, max_space)
     if basis is not None:
         # determinant-space buffers of each plan's basis operator
-        need += (2 if precision == "fp64" else 4) * sp.ndet * 8
+        need += int(basis.work_bytes(precision)) if hasattr(basis, "work_bytes") else \
+            (2 if precision == "fp64" else 4) * sp.ndet * 8
     ws = int(solver.workspace_bytes) or max(int((free - need) * 0.85), 1 << 26)
     stats = {"precision": precision}
     verbose = bool(verbose and verbose > 4)
 
     if precision == "fp64":
         t0 = time.perf_counter()
-        with SigmaPlan(sp, h2e, "fp64", ws) as plan:
+        with _plan(sp, basis, h2e, "fp64", ws) as plan:
             hop = plan if basis is None else basis.hop(plan)
             res = _dv.davidson(hop, precond, guesses, nroots=nroots, lindep=lindep,
                                max_space=max_space, verbose=verbose, tol=tol,
@@ -96,7 +106,7 @@ def solve(solver, sp, h2e, hdiag, nroots, ci0, precision, tol=1e-10, tol_residua
                                follow_state=follow_state)
             torch.cuda.synchronize()
             stats.update(time=time.perf_counter() - t0, nhop=res.nhop, cycles=res.cycles,
-                         ntiles=plan.ntiles, nrestore=res.nrestore)
+                         ntiles=getattr(plan, "ntiles", None), nrestore=res.nrestore)
         return res.e, list(res.x), res.converged, stats
 
     if precision not in LOW_STAGE:
@@ -105,8 +115,8 @@ def solve(solver, sp, h2e, hdiag, nroots, ci0, precision, tol=1e-10, tol_residua
     t0 = time.perf_counter()
     # both plans live through the solve; the FP64 one (twice the walpha
     # bytes) gets two thirds of the workspace budget
-    with SigmaPlan(sp, h2e, "fp64", max(ws * 2 // 3, 1 << 26)) as plan64, \
-            SigmaPlan(sp, h2e, precision, max(ws // 3, 1 << 26)) as plan_lo:
+    with _plan(sp, basis, h2e, "fp64", max(ws * 2 // 3, 1 << 26)) as plan64, \
+            _plan(sp, basis, h2e, precision, max(ws // 3, 1 << 26)) as plan_lo:
         hop_lo = plan_lo if basis is None else basis.hop(plan_lo)
         hop64 = plan64 if basis is None else basis.hop(plan64)
         res = _dv.davidson(hop_lo, precond, guesses, nroots=nroots, lindep=lindep,
@@ -117,5 +127,6 @@ def solve(solver, sp, h2e, hdiag, nroots, ci0, precision, tol=1e-10, tol_residua
         torch.cuda.synchronize()
         stats.update(time=time.perf_counter() - t0, nhop_low=res.nhop,
                      nhop_fp64=res.nhop_check, cycles=res.cycles, nrestore=res.nrestore,
-                     ntiles=plan64.ntiles, ntiles_low=plan_lo.ntiles)
+                     ntiles=getattr(plan64, "ntiles", None),
+                     ntiles_low=getattr(plan_lo, "ntiles", None))
     return res.e, list(res.x), res.converged, stats

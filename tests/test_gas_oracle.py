@@ -188,3 +188,72 @@ def test_front_end_errors():
         ag.normalize_gas_restr((2, 2), (2, 1), [[2, 0, 1, 1]], "spin-supergroup")
     with pytest.raises(ValueError):
         ag.normalize_gas_spec((1, 2, 1), (2, 2), {"max_holes": 1}, "ras")
+
+
+# --- Slater-Condon rows over an explicit determinant list
+# (orc_det_sigma_rows): the spot-check oracle of sector-native GAS spaces,
+# whose orbital counts leave no CAS array.  Pinned against the reference's
+# libfci_gas restricted sigma (every row of every fixture) and against the
+# CAS Slater-Condon rows when the list is the whole CAS space.
+
+@pytest.mark.parametrize("name", CASES)
+def test_det_sigma_rows_vs_libfci_gas(name):
+    d = load(name)
+    L = og.GasLayout(d["gas_orbs"], d["nelec"], d["blocks"])
+    sa, sb = L.det_strings()
+    got = oracle.det_sigma_rows(d["h1e"], d["eri"], d["ci"], L.norb_total, sa, sb,
+                                numpy.arange(L.ndet))
+    assert numpy.abs(got - d["sigma"]).max() < 1e-12 * max(1.0, numpy.abs(d["sigma"]).max())
+
+
+@pytest.mark.parametrize("n,ne,seed", [(6, (3, 3), 0), (7, (4, 2), 1), (9, (5, 4), 2)])
+def test_det_sigma_rows_full_cas_equals_sigma_rows(n, ne, seed):
+    from conftest import package as _pk
+    syn = _pk().synthetic
+    h1e, eri, c = syn.make_problem(n, ne, seed=seed)
+    sa0 = numpy.array([oracle.addr2str(n, ne[0], a) for a in range(oracle.num_strings(n, ne[0]))],
+                      dtype=numpy.uint64)
+    sb0 = numpy.array([oracle.addr2str(n, ne[1], b) for b in range(oracle.num_strings(n, ne[1]))],
+                      dtype=numpy.uint64)
+    sa, sb = numpy.repeat(sa0, sb0.size), numpy.tile(sb0, sa0.size)
+    # a shuffled list: the result does not depend on the list order
+    perm = numpy.random.default_rng(seed).permutation(sa.size)
+    rows = numpy.arange(0, sa.size, 7)
+    got = oracle.det_sigma_rows(h1e, eri, c.reshape(-1)[perm], n, sa[perm], sb[perm],
+                                numpy.argsort(perm)[rows])
+    ref = oracle.sigma_rows(h1e, eri, c, n, ne, rows)
+    numpy.testing.assert_allclose(got, ref, rtol=0, atol=1e-13 * numpy.abs(ref).max())
+
+
+def test_det_sigma_rows_planted_sign_fails():
+    """A wrong sign of one coefficient changes the rows that couple to it:
+    the check distinguishes the restricted H from a perturbed one."""
+    d = load("gas_ras_n10")
+    L = og.GasLayout(d["gas_orbs"], d["nelec"], d["blocks"])
+    sa, sb = L.det_strings()
+    c = d["ci"].copy()
+    k = int(numpy.argmax(numpy.abs(c)))
+    c[k] = -c[k]
+    got = oracle.det_sigma_rows(d["h1e"], d["eri"], c, L.norb_total, sa, sb, numpy.arange(L.ndet))
+    assert numpy.abs(got - d["sigma"]).max() > 1e-6
+
+
+@pytest.mark.parametrize("n,ne,seed", [(4, (2, 2), 0), (5, (3, 2), 1), (6, (2, 3), 2)])
+def test_det_hamiltonian_full_cas_equals_brute_force(n, ne, seed):
+    from conftest import package as _pk
+    h1e, eri, _ = _pk().synthetic.make_problem(n, ne, seed=seed)
+    sa0 = numpy.array([oracle.addr2str(n, ne[0], a) for a in range(oracle.num_strings(n, ne[0]))],
+                      dtype=numpy.uint64)
+    sb0 = numpy.array([oracle.addr2str(n, ne[1], b) for b in range(oracle.num_strings(n, ne[1]))],
+                      dtype=numpy.uint64)
+    H = oracle.det_hamiltonian(h1e, eri, n, numpy.repeat(sa0, sb0.size), numpy.tile(sb0, sa0.size))
+    numpy.testing.assert_allclose(H, brute.hamiltonian(h1e, eri, n, ne), rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("name", [c for c in CASES if c.endswith(("n3", "n4"))])
+def test_det_hamiltonian_gas_spectrum_vs_libfci_gas(name):
+    d = load(name)
+    L = og.GasLayout(d["gas_orbs"], d["nelec"], d["blocks"])
+    sa, sb = L.det_strings()
+    w = numpy.linalg.eigvalsh(oracle.det_hamiltonian(d["h1e"], d["eri"], L.norb_total, sa, sb))
+    numpy.testing.assert_allclose(w[:4], d["evals"][:min(4, w.size)], rtol=0, atol=1e-11)

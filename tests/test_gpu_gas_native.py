@@ -1,0 +1,273 @@
+"""GPU parity of sector-native GAS spaces (no CAS embedding; up to the
+reference's 63 orbitals, pyscf/lib/mcscf/fci_gas.h:24-27).
+
+Checkers, all independent of the product path:
+  * the reference's libfci_gas fixtures (tests/golden/gas/): layout,
+    sigma, diagonal and RDMs of seven spaces;
+  * the oracle's Slater-Condon H over an explicit determinant list
+    (oracle.det_sigma_rows / det_hamiltonian, pinned in
+    tests/test_gas_oracle.py against libfci_gas and the brute-force H):
+    sampled sigma rows, dense restricted spectra, diagonals and H blocks at
+    24, 40 and 62 orbitals, where no CAS array of 32-bit indices exists;
+  * the CAS-embedded path on a 12-orbital space (sigma, diagonal, RDMs);
+  * mathematics: hermiticity, linearity, E = h.dm1 + 1/2 eri.dm2,
+    tr dm1 = N, <S^2> of a spin eigenstate.
+
+Bars: FP64 sigma 1e-12 of max|sigma| (destination-owned pull, no atomics);
+diagonal and RDMs 1e-12; energies 1e-9 (Davidson tolerance 1e-10 on the
+energy change); FP32 sigma relative error below 1e-5."""
+
+import glob
+import json
+import os
+
+import numpy
+import pytest
+
+import oracle
+from oracle import gas as og
+from conftest import GOLDEN, has_gpu, package
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not has_gpu(), reason="needs a HIP device")]
+
+pkg = package()
+fg = pkg.fci_gas
+ag = pkg.addons_gas
+syn = pkg.synthetic
+
+CASES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "gas", "*.npz")))
+
+
+def load(name):
+    with numpy.load(os.path.join(GOLDEN, "gas", name + ".npz")) as f:
+        d = {k: f[k] for k in f.files}
+    d["gas_orbs"] = tuple(int(x) for x in d["gas_orbs"])
+    d["nelec"] = tuple(int(x) for x in d["nelec"])
+    d["restr_type"] = str(d["restr_type"])
+    d["restr"] = json.loads(str(d["restr"]))
+    return d
+
+
+def _dev(x):
+    import torch
+    return torch.as_tensor(numpy.ascontiguousarray(x, dtype=numpy.float64)).cuda()
+
+
+def _sigma(gas, h2e, c, precision="fp64", workspace=0):
+    import torch
+    with fg.GasContractPlan(gas, h2e, precision, workspace) as p:
+        x = _dev(c)
+        out = torch.empty_like(x)
+        p.execute(x, out)
+        return out.cpu().numpy()
+
+
+def _rdm12s(gas, bra, ket):
+    plan = fg.GasRDMPlan(None, None, None, gas=gas)
+    return plan.make_rdm12s(bra, ket)
+
+
+def _space(orbs, nelec, restr, kind, native=True):
+    o, D = ag.normalize_gas_spec(orbs, nelec, restr, kind)
+    return fg.GasSpace(o, nelec, D, native=native), og.GasLayout(o, nelec, D)
+
+
+# --- the reference's libfci_gas fixtures ------------------------------------
+
+@pytest.mark.parametrize("name", CASES)
+def test_native_matches_libfci_gas(name):
+    d = load(name)
+    print("case", name, flush=True)
+    n = sum(d["gas_orbs"])
+    with fg.GasSpace(d["gas_orbs"], d["nelec"], d["blocks"], native=True) as gas:
+        assert gas.native and gas.cas is None
+        desc = list(gas.block_descriptors())
+        for k in ("offset", "sa", "sb", "na", "nb"):
+            assert [b[k] for b in desc] == d["block_" + k].tolist(), k
+        occ, nstr = gas.sector_occupations()
+        numpy.testing.assert_array_equal(occ, d["sector_occ"])
+        numpy.testing.assert_array_equal(nstr, d["sector_nstr"])
+        scale = max(1.0, numpy.abs(d["sigma"]).max())
+        sig = _sigma(gas, d["h2e"], d["ci"])
+        assert numpy.abs(sig - d["sigma"]).max() < 1e-12 * scale
+        # a workspace of a few closure rows: many tiles, same sigma
+        n1, maxrow, _, _ = gas.native_info()
+        tiny = 2 * 8 * (n * (n + 1) // 2) * max(maxrow, n1 // 7)
+        sig_t = _sigma(gas, d["h2e"], d["ci"], workspace=tiny)
+        assert numpy.abs(sig_t - sig).max() < 1e-13 * scale
+        hd = fg.FCISolver(gas_orbs=d["gas_orbs"], gas_restr=d["restr"],
+                          gas_restr_type=d["restr_type"])._hdiag_dev(d["h1e"], d["eri"], n,
+                                                                      d["nelec"], gas)
+        numpy.testing.assert_allclose(hd.cpu().numpy(), d["hdiag"], rtol=0, atol=1e-12)
+        (a, b), (aa, ab, bb) = _rdm12s(gas, d["ci"], d["ket"])
+        for got, key in zip((a, b, aa, ab, bb), ("dm1a", "dm1b", "dm2aa", "dm2ab", "dm2bb")):
+            numpy.testing.assert_allclose(got, d[key], rtol=0, atol=1e-12, err_msg=key)
+        (a, b), _ = _rdm12s(gas, d["ket"], d["ket"])
+        numpy.testing.assert_allclose(numpy.trace(a), d["nelec"][0] * numpy.dot(d["ket"], d["ket"]),
+                                      rtol=1e-12)
+
+
+def test_native_vs_cas_embedded_12_orbitals():
+    """RAS(2,8,2) with 2 holes / 2 particles, (6,6): the sector-native
+    sigma, diagonal and transition RDMs equal the CAS-embedded ones (fused
+    CAS kernel over the restricted rows)."""
+    orbs, ne = (2, 8, 2), (6, 6)
+    o, D = ag.normalize_gas_spec(orbs, ne, dict(max_holes=2, max_particles=2), "ras")
+    h1e, eri = syn.make_problem(12, ne, seed=5, with_ci=False)
+    h2e = fg.absorb_h1e(h1e, eri, 12, ne, 0.5)
+    with fg.GasSpace(o, ne, D, native=True) as nat, fg.GasSpace(o, ne, D, native=False) as emb:
+        assert nat.ndet == emb.ndet
+        rng = numpy.random.default_rng(1)
+        c = rng.normal(size=nat.ndet)
+        c /= numpy.linalg.norm(c)
+        k = rng.normal(size=nat.ndet)
+        s_nat, s_emb = _sigma(nat, h2e, c), _sigma(emb, h2e, c)
+        assert numpy.abs(s_nat - s_emb).max() < 1e-12 * numpy.abs(s_emb).max()
+        sol = fg.FCISolver(gas_orbs=orbs)
+        numpy.testing.assert_allclose(sol._hdiag_dev(h1e, eri, 12, ne, nat).cpu().numpy(),
+                                      sol._hdiag_dev(h1e, eri, 12, ne, emb).cpu().numpy(),
+                                      rtol=0, atol=1e-12)
+        r_nat, r_emb = _rdm12s(nat, c, k), _rdm12s(emb, c, k)
+        for x, y in zip(r_nat[0] + r_nat[1], r_emb[0] + r_emb[1]):
+            numpy.testing.assert_allclose(x, y, rtol=0, atol=1e-12)
+        # FP32 contraction against FP64
+        s32 = _sigma(nat, h2e, c, "fp32")
+        err = numpy.linalg.norm(s32 - s_nat) / numpy.linalg.norm(s_nat)
+        assert 1e-12 < err < 1e-5
+
+
+# --- beyond the CAS embedding ----------------------------------------------
+
+def test_native_62_orbitals_dense():
+    """62 orbitals in four subspaces (1, 3, 28, 30), (2,2) electrons,
+    cumulative occupations: strings use bits up to 61.  Against the
+    oracle's dense restricted H: sigma, diagonal, H block, the solver's
+    two lowest energies, and the ground state's RDM energy identity."""
+    orbs, ne = (1, 3, 28, 30), (2, 2)
+    restr = [[1, 2], [3, 4], [3, 4], [4, 4]]
+    gas, L = _space(orbs, ne, restr, "cumulative-occ")
+    with gas:
+        n = 62
+        h1e, eri = syn.make_problem(n, ne, seed=7, with_ci=False)
+        sa, sb = L.det_strings()
+        assert int(max(sa.max(), sb.max())).bit_length() > 32
+        H = oracle.det_hamiltonian(h1e, eri, n, sa, sb)
+        c = numpy.random.default_rng(2).normal(size=L.ndet)
+        h2e = fg.absorb_h1e(h1e, eri, n, ne, 0.5)
+        ref = H @ c
+        got = _sigma(gas, h2e, c)
+        assert numpy.abs(got - ref).max() < 1e-12 * numpy.abs(ref).max()
+        sol = fg.FCISolver(gas_orbs=orbs, gas_restr=restr, gas_restr_type="cumulative-occ")
+        numpy.testing.assert_allclose(sol.make_hdiag(h1e, eri, n, ne), numpy.diag(H),
+                                      rtol=0, atol=1e-12 * numpy.abs(numpy.diag(H)).max())
+        pick = numpy.sort(numpy.random.default_rng(3).choice(L.ndet, 60, replace=False))
+        Hb = sol._h_block(gas, h1e, eri, pick)
+        numpy.testing.assert_allclose(Hb, H[numpy.ix_(pick, pick)], rtol=0,
+                                      atol=1e-12 * numpy.abs(H).max())
+        w = numpy.linalg.eigvalsh(H)
+        sol.conv_tol = 1e-11
+        e, ci = sol.kernel(h1e, eri, n, ne, nroots=2, max_cycle=300)
+        print("stats", sol.stats, flush=True)
+        assert sol.converged.all() and gas.native
+        numpy.testing.assert_allclose(e, w[:2], rtol=0, atol=1e-8)
+        dm1, dm2 = sol.make_rdm12(ci[0], n, ne)
+        e_rdm = numpy.einsum("pq,pq", h1e, dm1) + 0.5 * numpy.einsum("pqrs,pqrs", _eri4(eri, n), dm2)
+        assert abs(e_rdm - w[0]) < 1e-8
+        assert abs(numpy.trace(dm1) - sum(ne)) < 1e-10
+
+
+def _eri4(eri, n):
+    """(n,n,n,n) integrals from the 4-fold pair matrix."""
+    idx = numpy.zeros((n, n), dtype=numpy.int64)
+    for p in range(n):
+        for q in range(n):
+            hi, lo = max(p, q), min(p, q)
+            idx[p, q] = hi * (hi + 1) // 2 + lo
+    return eri[idx[:, :, None, None], idx[None, None, :, :]]
+
+
+def test_native_40_orbitals_rows():
+    """40 orbitals in (2, 6, 12, 20), (4,4) electrons, 51,345 determinants
+    whose closure is 52 times larger: sigma against the oracle's
+    Slater-Condon rows at extreme and random determinants, the diagonal at
+    sampled determinants, hermiticity and linearity over the whole space,
+    FP32 error bound."""
+    orbs, ne = (2, 6, 12, 20), (4, 4)
+    restr = [[3, 4], [7, 8], [7, 8], [8, 8]]
+    gas, L = _space(orbs, ne, restr, "cumulative-occ")
+    with gas:
+        n = 40
+        h1e, eri = syn.make_problem(n, ne, seed=8, with_ci=False)
+        h2e = fg.absorb_h1e(h1e, eri, n, ne, 0.5)
+        rng = numpy.random.default_rng(4)
+        c = rng.normal(size=L.ndet)
+        c /= numpy.linalg.norm(c)
+        print("ndet", L.ndet, "closure", gas.native_info()[0], flush=True)
+        sig = _sigma(gas, h2e, c)
+        sa, sb = L.det_strings()
+        rows = numpy.unique(numpy.concatenate([[0, L.ndet - 1, int(numpy.argmax(numpy.abs(sig)))],
+                                               rng.integers(0, L.ndet, 250)]))
+        ref = oracle.det_sigma_rows(h1e, eri, c, n, sa, sb, rows)
+        assert numpy.abs(sig[rows] - ref).max() < 1e-12 * numpy.abs(sig).max()
+        # hermiticity and linearity
+        y = rng.normal(size=L.ndet)
+        sy = _sigma(gas, h2e, y)
+        assert abs(y @ sig - c @ sy) < 1e-12 * numpy.linalg.norm(sig) * numpy.linalg.norm(y)
+        s2 = _sigma(gas, h2e, 2.0 * c - 0.5 * y)
+        assert numpy.abs(s2 - (2.0 * sig - 0.5 * sy)).max() < 1e-12 * numpy.abs(s2).max()
+        # diagonal at sampled determinants: rows of unit vectors
+        sol = fg.FCISolver(gas_orbs=orbs, gas_restr=restr, gas_restr_type="cumulative-occ")
+        hd = sol.make_hdiag(h1e, eri, n, ne)
+        for dd in rows[:12]:
+            e = numpy.zeros(L.ndet)
+            e[dd] = 1.0
+            ref_d = oracle.det_sigma_rows(h1e, eri, e, n, sa, sb, [dd])[0]
+            assert abs(hd[dd] - ref_d) < 1e-12 * abs(ref_d)
+        s32 = _sigma(gas, h2e, c, "fp32")
+        err = numpy.linalg.norm(s32 - sig) / numpy.linalg.norm(sig)
+        assert err < 1e-5
+
+
+def test_native_solver_24_orbitals_mixed_and_spin():
+    """RAS(4,6,14), 2 holes / 2 particles, (5,5): 315,616 determinants,
+    solved by the default solver (sector-native above 16 orbitals).  The
+    FP32-mixed solve agrees with the FP64 one to 1e-8 Ha; sigma rows of the
+    ground state against the oracle; the RAS space is spin complete, so the
+    ground state's <S^2> from its RDMs is S(S+1)."""
+    orbs, ne = (4, 6, 14), (5, 5)
+    restr = dict(max_holes=2, max_particles=2)
+    n = 24
+    h1e, eri = syn.make_problem(n, ne, seed=9, with_ci=False)
+    sol = fg.FCISolver(gas_orbs=orbs, gas_restr=restr, gas_restr_type="ras")
+    e64, c64 = sol.kernel(h1e, eri, n, ne, max_cycle=300)
+    print("fp64", sol.stats, flush=True)
+    assert sol.converged and sol._space(n, ne).native
+    sol32 = fg.FCISolver(gas_orbs=orbs, gas_restr=restr, gas_restr_type="ras")
+    sol32.precision = "fp32"
+    e32, c32 = sol32.kernel(h1e, eri, n, ne, max_cycle=300)
+    print("fp32", sol32.stats, flush=True)
+    assert sol32.converged and abs(e32 - e64) < 1e-8
+    assert sol32.stats["nhop_fp64"] >= 1
+    o, D = ag.normalize_gas_spec(orbs, ne, restr, "ras")
+    L = og.GasLayout(o, ne, D)
+    sa, sb = L.det_strings()
+    rows = numpy.random.default_rng(5).integers(0, L.ndet, 100)
+    hc = oracle.det_sigma_rows(h1e, eri, c64, n, sa, sb, rows)
+    assert numpy.abs(hc - e64 * c64[rows]).max() < 1e-5
+    ss, _ = sol.spin_square(c64, n, ne)
+    assert min(abs(ss - S * (S + 1)) for S in range(4)) < 1e-6
+
+
+def test_native_rejects_cas_only_operations():
+    orbs, ne = (2, 8, 2), (6, 6)
+    gas, _ = _space(orbs, ne, dict(max_holes=1, max_particles=1), "ras")
+    with gas:
+        import torch
+        x = torch.zeros(gas.ndet, dtype=torch.float64, device="cuda")
+        with pytest.raises(NotImplementedError):
+            fg.GasSpinPlan(gas)
+        with pytest.raises(RuntimeError):
+            fg.GasContractPlan(gas, numpy.zeros((78, 78)), "bf16x3")
+        with pytest.raises(RuntimeError):
+            gas.embed_dev(x, torch.empty(10, dtype=torch.float64, device="cuda"))
