@@ -271,3 +271,42 @@ def test_native_rejects_cas_only_operations():
             fg.GasContractPlan(gas, numpy.zeros((78, 78)), "bf16x3")
         with pytest.raises(RuntimeError):
             gas.embed_dev(x, torch.empty(10, dtype=torch.float64, device="cuda"))
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference libfci_gas build absent")
+@pytest.mark.parametrize("orbs,ne,restr,kind", [
+    ((4, 6, 14), (5, 5), dict(max_holes=2, max_particles=2), "ras"),
+    ((2, 6, 12, 20), (4, 4), [[3, 4], [7, 8], [7, 8], [8, 8]], "cumulative-occ"),
+    ((1, 3, 28, 30), (2, 2), [[1, 2], [3, 4], [3, 4], [4, 4]], "cumulative-occ"),
+])
+def test_native_vs_reference_libfci_gas_large(orbs, ne, restr, kind):
+    """The reference's own libfci_gas (compiled from its sources into
+    oracle/_ref) on the same spaces at 24, 40 and 62 orbitals: layout,
+    whole sigma vectors, diagonal and transition RDMs."""
+    n = sum(orbs)
+    o, D = ag.normalize_gas_spec(orbs, ne, restr, kind)
+    R = oracle.RefLib().gas(o, ne, D)
+    h1e, eri = syn.make_problem(n, ne, seed=11, with_ci=False)
+    h2e = fg.absorb_h1e(h1e, eri, n, ne, 0.5)
+    with fg.GasSpace(o, ne, D, native=True) as gas:
+        print("case", orbs, ne, "ndet", gas.ndet, flush=True)
+        assert gas.ndet == R.ndet and gas.nsector == R.nsector
+        desc = list(gas.block_descriptors())
+        assert [(b["offset"], b["sa"], b["sb"], b["na"], b["nb"]) for b in desc] == \
+            [(b["offset"], b["sa"], b["sb"], b["na"], b["nb"]) for b in R.block_desc]
+        rng = numpy.random.default_rng(6)
+        c = rng.normal(size=gas.ndet)
+        c /= numpy.linalg.norm(c)
+        k = rng.normal(size=gas.ndet)
+        k /= numpy.linalg.norm(k)
+        ref = R.contract_2e(h2e, c)
+        got = _sigma(gas, h2e, c)
+        assert numpy.abs(got - ref).max() < 1e-12 * numpy.abs(ref).max()
+        sol = fg.FCISolver(gas_orbs=orbs)
+        hd = sol._hdiag_dev(h1e, eri, n, ne, gas).cpu().numpy()
+        hr = R.make_hdiag(h1e, eri)
+        assert numpy.abs(hd - hr).max() < 1e-12 * numpy.abs(hr).max()
+        (a, b), (aa, ab, bb) = _rdm12s(gas, c, k)
+        (ra, rb), (raa, rab, rbb) = R.trans_rdm12s(c, k)
+        for x, y in zip((a, b, aa, ab, bb), (ra, rb, raa, rab, rbb)):
+            assert numpy.abs(x - y).max() < 1e-12
