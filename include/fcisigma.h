@@ -487,7 +487,9 @@ int fcis_gas_sigma_execute(const fcis_gas_space_t *gas, fcis_sigma_plan_t *cas_p
                            const double *d_ci0, double *d_ci1, double *d_work, void *stream);
 /* d_ci1 = P S^2 P d_ci0 on GAS vectors with a spin plan on the space's CAS
  * space (exact S^2 when D is spin complete, as the reference requires,
- * fci_gas.py:993-1001), d_work 2 * nstr_a * nstr_b doubles. */
+ * fci_gas.py:993-1001), d_work 2 * nstr_a * nstr_b doubles.  A sector-native
+ * space applies S^2 = S_z(S_z+1) + N_b - sum_pq E^a_pq E^b_qp on its own
+ * tables; spin and d_work are then ignored and may be NULL. */
 int fcis_gas_contract_ss(const fcis_gas_space_t *gas, const fcis_spin_plan_t *spin,
                          const double *d_ci0, double *d_ci1, double *d_work, void *stream);
 /* Restricted work of the fused path: alpha rows, beta chunks over those
@@ -510,6 +512,9 @@ int fcis_gas_rows_info(const fcis_gas_space_t *gas, int64_t *nrows, int64_t *nch
  * fci_rdm_gas.c:2207-2243). */
 int fcis_gas_space_create_native(fcis_gas_space_t **out, int ngas, const int *norb, int na,
                                  int nb, int nblock, const int *block_occ, void *stream);
+/* Diagonal of S^2 on the determinants of a sector-native space (device
+ * d_out, ndet doubles; _GasSpinPlan.diagonal_vector, fci_gas.py:374-398). */
+int fcis_gas_ss_diag(const fcis_gas_space_t *gas, double *d_out, void *stream);
 /* native = 1 for a sector-native space; nclosure = determinants within one
  * excitation of D (the t1 columns of its sigma), maxrow = the longest
  * closure row (the fewest t1 columns a workspace must hold), nstra / nstrb =

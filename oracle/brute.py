@@ -112,3 +112,39 @@ def spin_square_matrix(n, nelec):
                     row = ia[a2] * len(sb) + ib[b2]
                     S2[row, col] += s1 * s2 * s3 * s4
     return S2
+
+
+def det_spin_square_matrix(stra, strb, nelec):
+    """S^2 over an explicit determinant list (stra[i], strb[i]), restricted
+    to the list (P S^2 P), by the ladder construction of spin_square_matrix
+    (S_- S_+ + S_z (S_z + 1), alpha spin-orbitals before beta ones); strings
+    of up to 64 orbitals.  Small lists (dense output)."""
+    na, nb = nelec
+    sa = [int(x) for x in numpy.asarray(stra, dtype=numpy.uint64)]
+    sb = [int(x) for x in numpy.asarray(strb, dtype=numpy.uint64)]
+    index = {(a, b): i for i, (a, b) in enumerate(zip(sa, sb))}
+    N = len(sa)
+    S2 = numpy.zeros((N, N))
+    sz = 0.5 * (na - nb)
+    pc = lambda x: bin(x).count("1")
+    for col, (a, b) in enumerate(zip(sa, sb)):
+        S2[col, col] += sz * (sz + 1)
+        bb = b & ~a
+        while bb:
+            p = (bb & -bb).bit_length() - 1
+            bb &= bb - 1
+            s1 = (-1) ** (na + pc(b & ((1 << p) - 1)))
+            b1 = b ^ (1 << p)
+            s2 = (-1) ** pc(a & ((1 << p) - 1))
+            a1 = a | (1 << p)
+            aa = a1 & ~b1
+            while aa:
+                q = (aa & -aa).bit_length() - 1
+                aa &= aa - 1
+                s3 = (-1) ** pc(a1 & ((1 << q) - 1))
+                a2 = a1 ^ (1 << q)
+                s4 = (-1) ** (na + pc(b1 & ((1 << q) - 1)))
+                row = index.get((a2, b1 | (1 << q)))
+                if row is not None:
+                    S2[row, col] += s1 * s2 * s3 * s4
+    return S2

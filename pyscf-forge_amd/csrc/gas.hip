@@ -834,12 +834,14 @@ int fcis_gas_sigma_execute(const fcis_gas_space_t *g, fcis_sigma_plan_t *cas_pla
 int fcis_gas_contract_ss(const fcis_gas_space_t *g, const fcis_spin_plan_t *spin,
                          const double *d_ci0, double *d_ci1, double *d_work, void *stream)
 {
+        if (g != nullptr && g->cas == nullptr && d_ci0 != nullptr && d_ci1 != nullptr &&
+            d_ci0 != d_ci1)
+                return fcis::gas_native_ss(g, d_ci0, d_ci1, (hipStream_t)stream);
         if (g == nullptr || spin == nullptr || d_ci0 == nullptr || d_ci1 == nullptr ||
             d_work == nullptr || d_ci0 == d_ci1) {
                 fcis::set_error("fcis_gas_contract_ss: invalid argument");
                 return FCIS_ERR_INVALID;
         }
-        if (g->cas == nullptr) return no_cas("fcis_gas_contract_ss");
         hipStream_t st = (hipStream_t)stream;
         double *a = d_work, *b = d_work + g->ncas;
         int r = embed(g, d_ci0, a, 1, st);
@@ -956,6 +958,20 @@ int fcis_gas_pspace(const fcis_gas_space_t *g, const double *h1e, const double *
         (void)hipStreamSynchronize(st);
         (void)hipFree(d_s);
         return r;
+}
+
+int fcis_gas_ss_diag(const fcis_gas_space_t *g, double *d_out, void *stream)
+{
+        if (g == nullptr || d_out == nullptr) {
+                fcis::set_error("fcis_gas_ss_diag: invalid argument");
+                return FCIS_ERR_INVALID;
+        }
+        if (g->cas != nullptr) {
+                fcis::set_error("fcis_gas_ss_diag: a CAS-embedded space takes the diagonal of its "
+                                "CAS spin plan (fcis_spin_plan_diag) projected");
+                return FCIS_ERR_INVALID;
+        }
+        return fcis::gas_native_ss(g, nullptr, d_out, (hipStream_t)stream);
 }
 
 int fcis_gas_native_info(const fcis_gas_space_t *g, int *native, int64_t *nclosure,

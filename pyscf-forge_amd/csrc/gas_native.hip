@@ -412,6 +412,46 @@ __global__ __launch_bounds__(NT) void k_nat_s1_embed(NatView v, int64_t col0, in
         e[k] = j >= 0 ? x[j] : 0.0;
 }
 
+// S^2 = S_z (S_z + 1) + N_beta - sum_pq E^alpha_pq E^beta_qp on the GAS
+// determinants (P S^2 P; exact when D is spin complete).  For determinant I
+// the double sum is |I_a & I_b| c_I (p = q, doubly occupied) plus, for every
+// alpha-only orbital p and beta-only orbital q, the spin exchange
+// s_a s_b c_J with J_a = a_q^+ a_p I_a (sign s_a), J_b = a_p^+ a_q I_b (s_b),
+// both from the pair maps.  shift = S_z (S_z + 1) + N_beta.
+__global__ __launch_bounds__(NT) void k_nat_ss(NatView v, double shift, const double *__restrict__ c,
+                                               double *__restrict__ out)
+{
+        const int64_t d = (int64_t)blockIdx.x * NT + threadIdx.x;
+        if (d >= v.ndet) return;
+        const DPos q = d_locate(v, d);
+        const int64_t r = v.sbaseA[q.A] + q.a, gb = v.sbaseB[q.B] + q.b;
+        const uint64_t sa = v.strA[r], sb = v.strB[gb];
+        double acc = (shift - (double)__popcll(sa & sb)) * c[d];
+        for (uint64_t oa = sa & ~sb; oa; oa &= oa - 1) {
+                const int p = __ffsll((long long)oa) - 1;
+                for (uint64_t ob = sb & ~sa; ob; ob &= ob - 1) {
+                        const int pq = fcis::pair_index(p, __ffsll((long long)ob) - 1);
+                        const int32_t ea = v.pmapA[(int64_t)pq * v.nstrA + r];
+                        const int32_t eb = v.pmapB[(int64_t)pq * v.nstrB + gb];
+                        if (ea == -1 || eb == -1) continue;
+                        const int32_t ja = pm_index(ea), jb = pm_index(eb);
+                        const int64_t j = d_index(v, v.ssecA[ja], v.saddrA[ja], v.ssecB[jb], v.saddrB[jb]);
+                        if (j < 0) continue;
+                        acc -= ((ea < 0) != (eb < 0)) ? -c[j] : c[j];
+                }
+        }
+        out[d] = acc;
+}
+
+__global__ __launch_bounds__(NT) void k_nat_ss_diag(NatView v, double shift, double *__restrict__ out)
+{
+        const int64_t d = (int64_t)blockIdx.x * NT + threadIdx.x;
+        if (d >= v.ndet) return;
+        const DPos q = d_locate(v, d);
+        const uint64_t sa = v.strA[v.sbaseA[q.A] + q.a], sb = v.strB[v.sbaseB[q.B] + q.b];
+        out[d] = shift - (double)__popcll(sa & sb);
+}
+
 __global__ void k_nat_fill(int32_t *__restrict__ x, int64_t n, int32_t val)
 {
         for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
@@ -793,6 +833,21 @@ int fcis::gas_native_hdiag(const fcis_gas_space *g, const double *h1e, const dou
         if (g->ndet > 0) {
                 hipLaunchKernelGGL(k_nat_hdiag, dim3(blocks_for(g->ndet)), dim3(NT), 0, st, nt->v, d.p,
                                    d_hdiag);
+                FCIS_CHECK_LAUNCH();
+        }
+        return FCIS_SUCCESS;
+}
+
+int fcis::gas_native_ss(const fcis_gas_space *g, const double *d_ci0, double *d_ci1, hipStream_t st)
+{
+        const double sz = 0.5 * (g->na - g->nb);
+        if (g->ndet > 0) {
+                if (d_ci0 != nullptr)
+                        hipLaunchKernelGGL(k_nat_ss, dim3(blocks_for(g->ndet)), dim3(NT), 0, st, g->nat->v,
+                                           sz * (sz + 1.0) + g->nb, d_ci0, d_ci1);
+                else
+                        hipLaunchKernelGGL(k_nat_ss_diag, dim3(blocks_for(g->ndet)), dim3(NT), 0, st,
+                                           g->nat->v, sz * (sz + 1.0) + g->nb, d_ci1);
                 FCIS_CHECK_LAUNCH();
         }
         return FCIS_SUCCESS;

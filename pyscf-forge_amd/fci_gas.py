@@ -494,32 +494,32 @@ class GasContractPlan:
 
 class GasSpinPlan:
     """S^2 on GAS vectors (_GasSpinPlan, fci_gas.py:266-419): the CAS S^2
-    kernel between embedding and projection, exact when the block set is
-    spin complete (checked by the callers, as in the reference).  Holds the
-    space, which must stay open."""
+    kernel between embedding and projection, or on a sector-native space
+    S_z(S_z+1) + N_beta - sum_pq E^a_pq E^b_qp on the space's own tables;
+    exact when the block set is spin complete (checked by the callers, as in
+    the reference).  Holds the space, which must stay open."""
 
     def __init__(self, gas):
         import torch
-        if gas.cas is None:
-            raise NotImplementedError(
-                "S^2 contraction runs through the CAS embedding; this GAS space is "
-                "sector-native (spin_square works through the RDMs)")
         self.gas = gas
         self.ndet = gas.ndet
         na, nb = gas.nelec
         sz = 0.5 * (na - nb)
         self.diagonal = sz * (sz + 1.0) + nb
-        self._plan = _ds.SpinPlan(gas.cas)
-        self._work = torch.empty(2 * gas.cas.ndet, dtype=torch.float64, device="cuda")
+        if gas.cas is None:
+            # sector-native: the library applies S^2 on the space's own tables
+            self._plan = self._work = None
+        else:
+            self._plan = _ds.SpinPlan(gas.cas)
+            self._work = torch.empty(2 * gas.cas.ndet, dtype=torch.float64, device="cuda")
 
     def contract_dev(self, x, out):
         """out <- S^2 x for device GAS vectors."""
         if x.data_ptr() == out.data_ptr():
             raise ValueError("S^2 input and output must not alias")
-        _lib.check(self.gas.lib.fcis_gas_contract_ss(self.gas._live(), self._plan.handle,
-                                                     _lib.vptr(x), _lib.vptr(out),
-                                                     _lib.vptr(self._work), _lib.stream_ptr()),
-                   "fcis_gas_contract_ss")
+        _lib.check(self.gas.lib.fcis_gas_contract_ss(
+            self.gas._live(), None if self._plan is None else self._plan.handle, _lib.vptr(x),
+            _lib.vptr(out), _lib.vptr(self._work), _lib.stream_ptr()), "fcis_gas_contract_ss")
         return out
 
     def contract(self, fcivec):
@@ -538,6 +538,11 @@ class GasSpinPlan:
 
     def diagonal_dev(self):
         import torch
+        if self._plan is None:
+            out = torch.empty(self.ndet, dtype=torch.float64, device="cuda")
+            _lib.check(self.gas.lib.fcis_gas_ss_diag(self.gas._live(), _lib.vptr(out),
+                                                     _lib.stream_ptr()), "fcis_gas_ss_diag")
+            return out
         full = torch.empty(self.gas.cas.ndet, dtype=torch.float64, device="cuda")
         self._plan.diag(full)
         out = torch.empty(self.ndet, dtype=torch.float64, device="cuda")
@@ -674,6 +679,12 @@ class _PenaltyBasis:
             ops.lincomb(1, n, t1, numpy.array([-pen.target]), 1.0, t2)
             t1 = t2
         ops.lincomb(1, n, t1, numpy.array([pen.shift]), 1.0, out)
+
+    def make_plan(self, h2e, precision="fp64", workspace_bytes=0):
+        return self.gas.make_plan(h2e, precision, workspace_bytes)
+
+    def work_bytes(self, precision="fp64"):
+        return self.gas.work_bytes(precision) + 2 * self.n * 8
 
     def hop(self, plan):
         import torch

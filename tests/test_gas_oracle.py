@@ -257,3 +257,27 @@ def test_det_hamiltonian_gas_spectrum_vs_libfci_gas(name):
     sa, sb = L.det_strings()
     w = numpy.linalg.eigvalsh(oracle.det_hamiltonian(d["h1e"], d["eri"], L.norb_total, sa, sb))
     numpy.testing.assert_allclose(w[:4], d["evals"][:min(4, w.size)], rtol=0, atol=1e-11)
+
+
+@pytest.mark.parametrize("n,ne", [(4, (2, 2)), (5, (3, 1)), (6, (3, 3))])
+def test_det_spin_square_full_cas_equals_spin_square_matrix(n, ne):
+    sa0 = numpy.array([oracle.addr2str(n, ne[0], a) for a in range(oracle.num_strings(n, ne[0]))],
+                      dtype=numpy.uint64)
+    sb0 = numpy.array([oracle.addr2str(n, ne[1], b) for b in range(oracle.num_strings(n, ne[1]))],
+                      dtype=numpy.uint64)
+    S = brute.det_spin_square_matrix(numpy.repeat(sa0, sb0.size), numpy.tile(sb0, sa0.size), ne)
+    numpy.testing.assert_allclose(S, brute.spin_square_matrix(n, ne), rtol=0, atol=1e-14)
+    w = numpy.linalg.eigvalsh(S)
+    s = numpy.sqrt(w + 0.25) - 0.5
+    assert numpy.abs(s * 2 - numpy.round(s * 2)).max() < 1e-10
+
+
+@pytest.mark.parametrize("name", ["gas_sg_n4", "gas_ras_n4", "gas_cum_n3", "gas_sg_n8"])
+def test_det_spin_square_gas_is_projection(name):
+    d = load(name)
+    L = og.GasLayout(d["gas_orbs"], d["nelec"], d["blocks"])
+    sa, sb = L.det_strings()
+    S = brute.det_spin_square_matrix(sa, sb, d["nelec"])
+    idx = L.cas_index()
+    ref = brute.spin_square_matrix(L.norb_total, d["nelec"])[numpy.ix_(idx, idx)]
+    numpy.testing.assert_allclose(S, ref, rtol=0, atol=1e-14)

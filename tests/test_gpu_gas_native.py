@@ -265,8 +265,6 @@ def test_native_rejects_cas_only_operations():
     with gas:
         import torch
         x = torch.zeros(gas.ndet, dtype=torch.float64, device="cuda")
-        with pytest.raises(NotImplementedError):
-            fg.GasSpinPlan(gas)
         with pytest.raises(RuntimeError):
             fg.GasContractPlan(gas, numpy.zeros((78, 78)), "bf16x3")
         with pytest.raises(RuntimeError):
@@ -310,3 +308,79 @@ def test_native_vs_reference_libfci_gas_large(orbs, ne, restr, kind):
         (ra, rb), (raa, rab, rbb) = R.trans_rdm12s(c, k)
         for x, y in zip((a, b, aa, ab, bb), (ra, rb, raa, rab, rbb)):
             assert numpy.abs(x - y).max() < 1e-12
+
+
+# --- S^2 and the spin penalty on sector-native spaces ----------------------
+
+@pytest.mark.parametrize("name", ["gas_sg_n4", "gas_ras_n4", "gas_cum_n3", "gas_sg_n8"])
+def test_native_contract_ss_vs_ladder_construction(name):
+    from oracle import brute
+    d = load(name)
+    L = og.GasLayout(d["gas_orbs"], d["nelec"], d["blocks"])
+    sa, sb = L.det_strings()
+    S = brute.det_spin_square_matrix(sa, sb, d["nelec"])
+    with fg.GasSpace(d["gas_orbs"], d["nelec"], d["blocks"], native=True) as gas:
+        plan = fg.GasSpinPlan(gas)
+        numpy.testing.assert_allclose(plan.contract(d["ci"]), S @ d["ci"], rtol=0, atol=1e-13)
+        numpy.testing.assert_allclose(plan.diagonal_vector(), numpy.diag(S), rtol=0, atol=1e-14)
+
+
+def test_native_spin_62_orbitals_penalty():
+    """62 orbitals, 1767 determinants: S^2 against the ladder construction
+    over the determinant list; the spin-penalised solver (fix_spin_) reaches
+    the lowest triplet and singlet of the dense restricted H within each
+    spin block."""
+    from oracle import brute
+    orbs, ne = (1, 3, 28, 30), (2, 2)
+    restr = [[1, 2], [3, 4], [3, 4], [4, 4]]
+    o, D = ag.normalize_gas_spec(orbs, ne, restr, "cumulative-occ")
+    assert ag.is_spin_complete(o, ne, D)
+    L = og.GasLayout(o, ne, D)
+    sa, sb = L.det_strings()
+    S = brute.det_spin_square_matrix(sa, sb, ne)
+    n = 62
+    h1e, eri = syn.make_problem(n, ne, seed=12, with_ci=False)
+    H = oracle.det_hamiltonian(h1e, eri, n, sa, sb)
+    with fg.GasSpace(o, ne, D, native=True) as gas:
+        x = numpy.random.default_rng(7).normal(size=L.ndet)
+        plan = fg.GasSpinPlan(gas)
+        numpy.testing.assert_allclose(plan.contract(x), S @ x, rtol=0, atol=1e-12)
+    w, v = numpy.linalg.eigh(S)
+    levels = {}
+    for ss in (0.0, 2.0):
+        P = v[:, abs(w - ss) < 1e-8]
+        levels[ss] = numpy.linalg.eigvalsh(P.T @ H @ P)[0]
+    # these integrals put the lowest triplet 24 Ha below the lowest singlet:
+    # the linear singlet penalty needs a shift above half that gap
+    for ss, shift in ((2.0, 1.0), (None, 40.0)):
+        sol = fg.fix_spin_(fg.FCISolver(gas_orbs=orbs, gas_restr=restr,
+                                        gas_restr_type="cumulative-occ"), shift, ss)
+        e, c = sol.kernel(h1e, eri, n, ne, max_cycle=600, max_space=24)
+        print("target", ss, sol.stats, flush=True)
+        assert sol.converged and sol._space(n, ne).native
+        ref = levels[0.0 if ss is None else ss]
+        assert abs(sol.e_physical - ref) < 1e-8, (ss, sol.e_physical, ref)
+
+
+def test_native_spin_24_orbitals_commutes_and_matches_rdm():
+    """RAS(4,6,14) (5,5): S^2 commutes with H on the (spin-complete) space,
+    and <x|S^2|x> equals the RDM formula of spin_square for a random x."""
+    import torch
+    orbs, ne = (4, 6, 14), (5, 5)
+    restr = dict(max_holes=2, max_particles=2)
+    n = 24
+    o, D = ag.normalize_gas_spec(orbs, ne, restr, "ras")
+    h1e, eri = syn.make_problem(n, ne, seed=13, with_ci=False)
+    h2e = fg.absorb_h1e(h1e, eri, n, ne, 0.5)
+    with fg.GasSpace(o, ne, D, native=True) as gas:
+        x = numpy.random.default_rng(8).normal(size=gas.ndet)
+        x /= numpy.linalg.norm(x)
+        plan = fg.GasSpinPlan(gas)
+        sx = plan.contract(x)
+        hx = _sigma(gas, h2e, x)
+        lhs = plan.contract(hx)
+        rhs = _sigma(gas, h2e, sx)
+        assert numpy.abs(lhs - rhs).max() < 1e-11 * numpy.abs(lhs).max()
+        ss_rdm, _ = fg.FCISolver(gas_orbs=orbs, gas_restr=restr,
+                                 gas_restr_type="ras").spin_square(x, n, ne)
+        assert abs(float(x @ sx) - ss_rdm) < 1e-10
