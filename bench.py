@@ -17,10 +17,12 @@ batched workload: 64 uneven CAS problems (norb 6..12, 4 roots each) solved
 together with one batched sigma launch per Davidson iteration, the
 singlet ground state in the CSF basis (csf2det, sigma, det2csf per step),
 and a restricted RAS(4,8,4) space inside the same CAS (restricted sigma in
-three precisions, the reference libfci_gas sigma on it, Davidson).
+three precisions, the reference libfci_gas sigma on it, Davidson), and a
+sector-native RAS(4,12,4) space of 20 orbitals (no CAS array: sigma in FP64
+and FP32 beside the reference libfci_gas on the same space).
 
     python bench.py [--steps K] [--warmup W] [--no-davidson] [--no-cpu] [--no-batch]
-                    [--no-csf] [--no-gas]
+                    [--no-csf] [--no-gas] [--no-gas-native] [--only-gas-native]
 """
 
 import argparse
@@ -582,6 +584,79 @@ def gas_solve(pkg, sync, h1e, eri, steps, with_cpu, max_cycle, davidson=True):
     return out
 
 
+def gas_native_sigma(pkg, sync, steps, with_cpu):
+    """Sector-native restricted sigma: RAS(4,12,4), (6,6) electrons, at most
+    2 holes in RAS1 and 2 electrons in RAS3 (20 orbitals, 6,989,020
+    determinants; the CAS of 20 orbitals, 2.4e8 determinants, is never
+    formed).  FP64 and FP32 sigma (t1 over the single-excitation closure,
+    GEMM, pull), and the reference's libfci_gas sigma on the same space on
+    the host cores."""
+    import numpy
+    import torch
+    fg, ag = pkg.fci_gas, pkg.addons_gas
+    orbs, ne, restr = (4, 12, 4), (6, 6), {"max_holes": 2, "max_particles": 2}
+    n = sum(orbs)
+    o, D = ag.normalize_gas_spec(orbs, ne, restr, "ras")
+    h1e, eri = pkg.synthetic.make_problem(n, ne, seed=0, with_ci=False)
+    h2e = fg.absorb_h1e(h1e, eri, n, ne, 0.5)
+    t0 = time.perf_counter()
+    gas = fg.GasSpace(o, ne, D, native=True)
+    sync()
+    n1, maxrow, nsa, nsb = gas.native_info()
+    out = {"workload": "RAS(4,12,4) (6,6) max_holes 2 max_particles 2, 20 orbitals, "
+                       "sector-native, synthetic integrals seed 0",
+           "ndet": gas.ndet, "closure": n1, "space_s": time.perf_counter() - t0}
+    g = torch.Generator(device="cuda").manual_seed(5)
+    x = torch.randn(gas.ndet, dtype=torch.float64, device="cuda", generator=g)
+    ref = None
+    npair = n * (n + 1) // 2
+    for prec in ("fp64", "fp32"):
+        with fg.GasContractPlan(gas, h2e, prec) as plan:
+            y = torch.empty_like(x)
+            plan.execute(x, y)
+            sync()
+            t1 = time.perf_counter()
+            for _ in range(steps):
+                plan.execute(x, y)
+            sync()
+            dt = (time.perf_counter() - t1) / steps
+        rec = {"ms": dt * 1e3, "sigma_per_s": 1.0 / dt,
+               "gemm_TFLOP/s": 2.0 * npair * npair * n1 / dt / 1e12}
+        if prec == "fp64":
+            ref = y.clone()
+        else:
+            rec["rel_err_vs_fp64"] = float((y - ref).norm() / ref.norm())
+        out[prec] = rec
+        log("native GAS sigma %s: %.2f ms" % (prec, dt * 1e3))
+    if with_cpu:
+        import oracle
+        if oracle.ref_available():
+            R = oracle.RefLib().gas(o, ne, D)
+            P = R.plan(h2e)
+            xc = x.cpu().numpy()
+            yc = numpy.empty_like(xc)
+            P(xc, yc)
+            t1 = time.perf_counter()
+            reps = 0
+            while reps < 2 or time.perf_counter() - t1 < 6.0:
+                P(xc, yc)
+                reps += 1
+            dt = (time.perf_counter() - t1) / reps
+            P.close()
+            R.close()
+            cores = int(os.environ.get("OMP_NUM_THREADS", 0)) or len(os.sched_getaffinity(0))
+            out["cpu_reference"] = {"sigma_s": dt, "cores": cores, "runs": reps,
+                                    "kind": "reference libfci_gas",
+                                    "max_rel_diff_vs_gpu_fp64":
+                                        float(numpy.abs(yc - ref.cpu().numpy()).max() /
+                                              numpy.abs(yc).max())}
+            log("native GAS sigma libfci_gas on %d cores: %.3f s" % (cores, dt))
+    del x, ref
+    gas.close()
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -598,6 +673,10 @@ def main():
                     help="skip the CSF-basis (singlet) Davidson at the same size")
     ap.add_argument("--no-gas", action="store_true",
                     help="skip the restricted (RAS) space inside the same CAS")
+    ap.add_argument("--no-gas-native", action="store_true",
+                    help="skip the sector-native RAS(4,12,4) space of 20 orbitals")
+    ap.add_argument("--only-gas-native", action="store_true",
+                    help="run only the sector-native GAS section (profiling)")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the CAS(8o,8e), CAS(14o,14e) and CAS(12o,12e) autograd sections")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -625,6 +704,11 @@ def main():
         log("copy calibration: 2 x 4 GiB device copies")
         del x, y
         torch.cuda.empty_cache()
+
+    if args.only_gas_native:
+        print(json.dumps({"gas_native": gas_native_sigma(pkg, sync, args.steps, not args.no_cpu)}),
+              flush=True)
+        return
 
     n = args.norb
     ne = (n // 2, n // 2)
@@ -691,6 +775,10 @@ def main():
         gas = gas_solve(pkg, sync, h1e, eri, args.steps, not args.no_cpu,
                         max(args.max_cycle, 400), not args.no_davidson)
 
+    gas_native = None
+    if not args.no_gas_native:
+        gas_native = gas_native_sigma(pkg, sync, args.steps, not args.no_cpu)
+
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (reference libfci_gas on host cores)")
@@ -736,6 +824,7 @@ def main():
         "batch": batch,
         "csf": csf,
         "gas": gas,
+        "gas_native": gas_native,
         "configs": configs,
     }
     print(json.dumps(line), flush=True)
