@@ -512,6 +512,12 @@ int fcis_gas_rows_info(const fcis_gas_space_t *gas, int64_t *nrows, int64_t *nch
  * fci_rdm_gas.c:2207-2243). */
 int fcis_gas_space_create_native(fcis_gas_space_t **out, int ngas, const int *norb, int na,
                                  int nb, int nblock, const int *block_occ, void *stream);
+/* Path of a restricted sigma plan: 0 / 1 the CAS plan's three-kernel /
+ * fused path (CAS-embedded space); on a sector-native space 2 = t1 over
+ * the single-excitation closure, GEMM and pull, 3 = the direct link-pair
+ * kernel (chosen by the estimated work, FP64 only; the environment
+ * variable FCIS_GAS_NATIVE_PATH=t1 / direct at plan creation forces one). */
+int fcis_gas_sigma_plan_path(const fcis_gas_sigma_plan_t *plan);
 /* Diagonal of S^2 on the determinants of a sector-native space (device
  * d_out, ndet doubles; _GasSpinPlan.diagonal_vector, fci_gas.py:374-398). */
 int fcis_gas_ss_diag(const fcis_gas_space_t *gas, double *d_out, void *stream);

@@ -118,6 +118,7 @@ SIGNATURES = {
     "fcis_gas_det_strings": (ctypes.c_int, [_vp, _i64, _i64p, _u64p, _u64p]),
     "fcis_gas_pspace": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int, _i64p, _vp, _vp]),
     "fcis_gas_ss_diag": (ctypes.c_int, [_vp, _vp, _vp]),
+    "fcis_gas_sigma_plan_path": (ctypes.c_int, [_vp]),
     "fcis_spin_plan_create": (ctypes.c_int, [ctypes.POINTER(_vp), _vp, _vp]),
     "fcis_spin_plan_contract": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "fcis_spin_plan_diag": (ctypes.c_int, [_vp, _vp, _vp]),

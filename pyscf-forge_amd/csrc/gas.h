@@ -79,6 +79,8 @@ int gas_native_plan_create(gas_native_plan **out, const fcis_gas_space *g, const
 int gas_native_plan_execute(gas_native_plan *p, const double *d_ci0, double *d_ci1,
                             hipStream_t st);
 void gas_native_plan_free(gas_native_plan *p);
+// 2: closure t1 / GEMM / pull, 3: direct link-pair kernel.
+int gas_native_plan_path(const gas_native_plan *p);
 // d_ci1 = P S^2 P d_ci0, or the diagonal of S^2 when d_ci0 is null.
 int gas_native_ss(const fcis_gas_space *g, const double *d_ci0, double *d_ci1, hipStream_t st);
 // S1 size and the string counts of the native tables (zeros for a

@@ -960,6 +960,13 @@ int fcis_gas_pspace(const fcis_gas_space_t *g, const double *h1e, const double *
         return r;
 }
 
+int fcis_gas_sigma_plan_path(const fcis_gas_sigma_plan_t *p)
+{
+        if (p == nullptr) return FCIS_ERR_INVALID;
+        if (p->nat) return fcis::gas_native_plan_path(p->nat);
+        return fcis_sigma_plan_path(p->cas_plan);
+}
+
 int fcis_gas_ss_diag(const fcis_gas_space_t *g, double *d_out, void *stream)
 {
         if (g == nullptr || d_out == nullptr) {

@@ -452,6 +452,98 @@ __global__ __launch_bounds__(NT) void k_nat_ss_diag(NatView v, double shift, dou
         out[d] = shift - (double)__popcll(sa & sb);
 }
 
+// Direct sigma (no closure): one wavefront per (block, alpha address, 64
+// beta addresses) of D, destination-owned.  With the link lists of the
+// strings (pair P, target, sign) and h symmetric-summed hs = h + h^T,
+//   sigma_I = sum over alpha links (P, K) of I_a, links (R, J) of K:
+//                 h[P][R] s s' c[J, I_b]                      (alpha-alpha)
+//           + the same over beta links                         (beta-beta)
+//           + sum over alpha links (P, J_a) of I_a and beta links (R, J_b)
+//             of I_b: hs[P][R] s s' c[J_a, J_b]                (alpha-beta)
+// (the t1 / GEMM form's sum over the closure, with K restricted to one spin
+// at a time; J must lie in D).  The alpha loops are uniform across the
+// wavefront (scalar loads of links and h), the c loads of the alpha-alpha
+// part coalesced; cost per determinant ~ (nlink_a + nlink_b)^2, against
+// npair^2 per closure column for the GEMM form.
+__global__ __launch_bounds__(NT) void k_nat_direct(NatView v, const int64_t *__restrict__ wbase,
+                                                   const double *__restrict__ h,
+                                                   const double *__restrict__ hs,
+                                                   const double *__restrict__ c, double *__restrict__ sigma)
+{
+        const int64_t w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (NT / 64) + (threadIdx.x >> 6))) +
+                          (int64_t)0;
+        const int lane = threadIdx.x & 63;
+        if (w >= wbase[v.nblock]) return;
+        const int blk = upper_index(wbase, v.nblock, w);
+        const int64_t nb = v.bnb[blk];
+        const int64_t nch = (nb + 63) >> 6;
+        const int64_t local = w - wbase[blk];
+        const int64_t a = local / nch;
+        const int64_t b = (local - a * nch) * 64 + lane;
+        const bool valid = b < nb;
+        const int64_t bl = valid ? b : nb - 1;
+        const int A = v.bsa[blk], B = v.bsb[blk];
+        const int64_t r = v.sbaseA[A] + a, gb = v.sbaseB[B] + bl;
+        const int npair = v.npair;
+        const int64_t nsa = v.nstrA, nsb = v.nstrB;
+        double acc = 0.0;
+        // alpha-alpha: uniform
+        for (int k1 = 0; k1 < v.nlinkA; k1++) {
+                const int32_t e1 = v.lkA[(int64_t)k1 * nsa + r];
+                if (e1 == -1) continue;
+                const int32_t K = pm_index(e1);
+                const double *hP = h + (int64_t)v.lkpA[(int64_t)k1 * nsa + r] * npair;
+                for (int k2 = 0; k2 < v.nlinkA; k2++) {
+                        const int32_t e2 = v.lkA[(int64_t)k2 * nsa + K];
+                        if (e2 == -1) continue;
+                        const int32_t J = pm_index(e2);
+                        const int blk2 = v.dblk[(int64_t)v.ssecA[J] * v.nsecB + B];
+                        if (blk2 < 0) continue;
+                        const double coef = hP[v.lkpA[(int64_t)k2 * nsa + K]];
+                        const double x = c[v.boff[blk2] + (int64_t)v.saddrA[J] * v.bnb[blk2] + bl];
+                        acc += ((e1 < 0) != (e2 < 0)) ? -coef * x : coef * x;
+                }
+        }
+        // beta-beta: per lane
+        const int32_t *dA = v.dblk + (int64_t)A * v.nsecB;
+        for (int k1 = 0; k1 < v.nlinkB; k1++) {
+                const int32_t e1 = v.lkB[(int64_t)k1 * nsb + gb];
+                if (e1 == -1) continue;
+                const int32_t K = pm_index(e1);
+                const double *hP = h + (int64_t)v.lkpB[(int64_t)k1 * nsb + gb] * npair;
+                for (int k2 = 0; k2 < v.nlinkB; k2++) {
+                        const int32_t e2 = v.lkB[(int64_t)k2 * nsb + K];
+                        if (e2 == -1) continue;
+                        const int32_t J = pm_index(e2);
+                        const int blk2 = dA[v.ssecB[J]];
+                        if (blk2 < 0) continue;
+                        const double coef = hP[v.lkpB[(int64_t)k2 * nsb + K]];
+                        const double x = c[v.boff[blk2] + a * v.bnb[blk2] + v.saddrB[J]];
+                        acc += ((e1 < 0) != (e2 < 0)) ? -coef * x : coef * x;
+                }
+        }
+        // alpha-beta: alpha links uniform, beta links per lane
+        for (int k1 = 0; k1 < v.nlinkA; k1++) {
+                const int32_t e1 = v.lkA[(int64_t)k1 * nsa + r];
+                if (e1 == -1) continue;
+                const int32_t Ja = pm_index(e1);
+                const int32_t *dA2 = v.dblk + (int64_t)v.ssecA[Ja] * v.nsecB;
+                const int64_t aJ = v.saddrA[Ja];
+                const double *hP = hs + (int64_t)v.lkpA[(int64_t)k1 * nsa + r] * npair;
+                for (int k2 = 0; k2 < v.nlinkB; k2++) {
+                        const int32_t e2 = v.lkB[(int64_t)k2 * nsb + gb];
+                        if (e2 == -1) continue;
+                        const int32_t Jb = pm_index(e2);
+                        const int blk2 = dA2[v.ssecB[Jb]];
+                        if (blk2 < 0) continue;
+                        const double coef = hP[v.lkpB[(int64_t)k2 * nsb + gb]];
+                        const double x = c[v.boff[blk2] + aJ * v.bnb[blk2] + v.saddrB[Jb]];
+                        acc += ((e1 < 0) != (e2 < 0)) ? -coef * x : coef * x;
+                }
+        }
+        if (valid) sigma[v.boff[blk] + a * nb + b] = acc;
+}
+
 __global__ void k_nat_fill(int32_t *__restrict__ x, int64_t n, int32_t val)
 {
         for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
@@ -483,11 +575,14 @@ struct gas_native {
 struct gas_native_plan {
         const fcis_gas_space *g = nullptr;
         int precision = FCIS_PREC_FP64;
+        int direct = 0;                 // 1: k_nat_direct, 0: t1 / GEMM / pull
         rocblas_handle blas = nullptr;
-        void *d_h = nullptr;            // npair x npair, double or float
+        void *d_h = nullptr;            // npair x npair, double or float (direct: h, then h + h^T)
         void *d_t1 = nullptr, *d_vt1 = nullptr;
         int64_t ld = 0;                 // columns per tile
         std::vector<int64_t> r0, c0;    // tile row and column starts (+ ends)
+        int64_t *d_wbase = nullptr;     // direct: wavefronts before each block (+ total)
+        int64_t nwave = 0;
 };
 
 namespace {
@@ -798,6 +893,11 @@ void fcis::gas_native_free(gas_native *nt)
         delete nt;
 }
 
+int fcis::gas_native_plan_path(const gas_native_plan *p)
+{
+        return p->direct ? 3 : 2;
+}
+
 int fcis::gas_native_info(const fcis_gas_space *g, int64_t *nclosure, int64_t *maxrow, int64_t *nstra,
                           int64_t *nstrb)
 {
@@ -859,6 +959,7 @@ void fcis::gas_native_plan_free(gas_native_plan *p)
         if (p->blas) rocblas_destroy_handle(p->blas);
         if (p->d_h) (void)hipFree(p->d_h);
         if (p->d_t1) (void)hipFree(p->d_t1);
+        if (p->d_wbase) (void)hipFree(p->d_wbase);
         delete p;
 }
 
@@ -878,6 +979,51 @@ int fcis::gas_native_plan_create(gas_native_plan **out, const fcis_gas_space *g,
         p->precision = precision;
         const size_t es = precision == FCIS_PREC_FP64 ? 8 : 4;
         const int np = nt->npair;
+        // Path: the direct form where its link-pair work per determinant,
+        // weighted by its measured rate (about 16 times slower per unit than
+        // the GEMM form's per closure column: irregular gathers against MFMA
+        // and streaming), is below the GEMM form's.  Measured crossover
+        // (FP64 sigma, ms, direct / GEMM): RAS 16 orbitals 149 / 54, 20
+        // orbitals 349 / 161, 24 orbitals 33 / 27, 40 orbitals 22 / 66 and
+        // 237 / 1131.  The direct form computes in FP64 for either
+        // precision.  FCIS_GAS_NATIVE_PATH=direct / t1 forces one.
+        {
+                const double la = nt->v.nlinkA, lb = nt->v.nlinkB;
+                const double direct_cost = (double)g->ndet * (la * la + lb * lb + 2.0 * la * lb) * 2.0;
+                const double t1_cost = (double)nt->n1 * ((double)np * np + 60.0 * np);
+                p->direct = 16.0 * direct_cost < t1_cost;
+                const char *env = getenv("FCIS_GAS_NATIVE_PATH");
+                if (env != nullptr && strcmp(env, "direct") == 0) p->direct = 1;
+                if (env != nullptr && strcmp(env, "t1") == 0) p->direct = 0;
+        }
+        if (p->direct) {
+                std::vector<int64_t> wb(g->nblock + 1, 0);
+                for (int b = 0; b < g->nblock; b++) {
+                        const int64_t nbs = g->nstr[g->bsb[b]];
+                        wb[b + 1] = wb[b] + g->nstr[g->bsa[b]] * ((nbs + 63) / 64);
+                }
+                p->nwave = wb[g->nblock];
+                std::vector<double> hh((size_t)2 * np * np);
+                for (int i = 0; i < np; i++)
+                        for (int j = 0; j < np; j++) {
+                                hh[(size_t)i * np + j] = h2e[(size_t)i * np + j];
+                                hh[(size_t)np * np + (size_t)i * np + j] = h2e[(size_t)i * np + j] + h2e[(size_t)j * np + i];
+                        }
+                if (fcis::dev_malloc(&p->d_h, 8 * hh.size()) != hipSuccess ||
+                    fcis::dev_malloc(&p->d_wbase, 8 * wb.size()) != hipSuccess) {
+                        gas_native_plan_free(p);
+                        fcis::set_error("fcis_gas_sigma_plan_create: device allocation failed");
+                        return FCIS_ERR_MEMORY;
+                }
+                if (hipMemcpy(p->d_h, hh.data(), 8 * hh.size(), hipMemcpyHostToDevice) != hipSuccess ||
+                    hipMemcpy(p->d_wbase, wb.data(), 8 * wb.size(), hipMemcpyHostToDevice) != hipSuccess) {
+                        gas_native_plan_free(p);
+                        fcis::set_error("fcis_gas_sigma_plan_create: upload failed");
+                        return FCIS_ERR_DEVICE;
+                }
+                *out = p;
+                return FCIS_SUCCESS;
+        }
         if (workspace_bytes == 0) workspace_bytes = (uint64_t)8 << 30;
         int64_t cols = tile_columns(nt, workspace_bytes, 2 * es * (uint64_t)np);
         if (cols < nt->maxrow) {
@@ -975,6 +1121,16 @@ int native_execute(gas_native_plan *p, const double *ci0, double *ci1, hipStream
 
 int fcis::gas_native_plan_execute(gas_native_plan *p, const double *d_ci0, double *d_ci1, hipStream_t st)
 {
+        if (p->direct) {
+                if (p->nwave > 0) {
+                        const int64_t np = p->g->nat->npair;
+                        const double *h = static_cast<const double *>(p->d_h);
+                        hipLaunchKernelGGL(k_nat_direct, dim3((unsigned)((p->nwave + NT / 64 - 1) / (NT / 64))),
+                                           dim3(NT), 0, st, p->g->nat->v, p->d_wbase, h, h + np * np, d_ci0, d_ci1);
+                        FCIS_CHECK_LAUNCH();
+                }
+                return FCIS_SUCCESS;
+        }
         if (p->ld > INT32_MAX) {
                 fcis::set_error("fcis_gas_sigma_plan_execute: tile of more than 2^31 columns");
                 return FCIS_ERR_INVALID;

@@ -452,6 +452,12 @@ class GasContractPlan:
         """out <- P H P c (the Davidson driver's operator)."""
         return self.execute(c, out)
 
+    @property
+    def path(self):
+        """"cas-three-kernel", "cas-fused", "native-t1" or "native-direct"."""
+        k = self.lib.fcis_gas_sigma_plan_path(self._plan)
+        return {0: "cas-three-kernel", 1: "cas-fused", 2: "native-t1", 3: "native-direct"}[k]
+
     def execute(self, c, out):
         """out <- P H P c for device tensors."""
         if self._plan is None or self.gas is None:

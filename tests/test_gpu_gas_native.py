@@ -384,3 +384,52 @@ def test_native_spin_24_orbitals_commutes_and_matches_rdm():
         ss_rdm, _ = fg.FCISolver(gas_orbs=orbs, gas_restr=restr,
                                  gas_restr_type="ras").spin_square(x, n, ne)
         assert abs(float(x @ sx) - ss_rdm) < 1e-10
+
+
+# --- the two sector-native sigma forms ---------------------------------------
+
+def _sigma_path(monkeypatch, gas, h2e, c, path):
+    monkeypatch.setenv("FCIS_GAS_NATIVE_PATH", path)
+    import torch
+    with fg.GasContractPlan(gas, h2e, "fp64") as p:
+        assert p.path == "native-" + path
+        x = _dev(c)
+        out = torch.empty_like(x)
+        p.execute(x, out)
+        return out.cpu().numpy()
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_native_direct_path_matches_libfci_gas(monkeypatch, name):
+    """The direct link-pair kernel on the reference's fixtures."""
+    d = load(name)
+    with fg.GasSpace(d["gas_orbs"], d["nelec"], d["blocks"], native=True) as gas:
+        sig = _sigma_path(monkeypatch, gas, d["h2e"], d["ci"], "direct")
+        scale = max(1.0, numpy.abs(d["sigma"]).max())
+        assert numpy.abs(sig - d["sigma"]).max() < 1e-12 * scale
+
+
+@pytest.mark.parametrize("orbs,ne,restr,kind", [
+    ((2, 8, 2), (6, 6), dict(max_holes=2, max_particles=2), "ras"),
+    ((4, 6, 14), (5, 5), dict(max_holes=2, max_particles=2), "ras"),
+    ((2, 6, 12, 20), (4, 4), [[3, 4], [7, 8], [7, 8], [8, 8]], "cumulative-occ"),
+    ((1, 3, 28, 30), (2, 2), [[1, 2], [3, 4], [3, 4], [4, 4]], "cumulative-occ"),
+])
+def test_native_direct_and_t1_paths_agree(monkeypatch, orbs, ne, restr, kind):
+    """Both sector-native sigma forms give the same restricted sigma (12 to
+    62 orbitals), and it matches the oracle's Slater-Condon rows."""
+    n = sum(orbs)
+    o, D = ag.normalize_gas_spec(orbs, ne, restr, kind)
+    L = og.GasLayout(o, ne, D)
+    h1e, eri = syn.make_problem(n, ne, seed=14, with_ci=False)
+    h2e = fg.absorb_h1e(h1e, eri, n, ne, 0.5)
+    with fg.GasSpace(o, ne, D, native=True) as gas:
+        c = numpy.random.default_rng(15).normal(size=gas.ndet)
+        s_t1 = _sigma_path(monkeypatch, gas, h2e, c, "t1")
+        s_dir = _sigma_path(monkeypatch, gas, h2e, c, "direct")
+        scale = numpy.abs(s_t1).max()
+        assert numpy.abs(s_dir - s_t1).max() < 1e-12 * scale
+        sa, sb = L.det_strings()
+        rows = numpy.random.default_rng(16).integers(0, gas.ndet, 80)
+        ref = oracle.det_sigma_rows(h1e, eri, c, n, sa, sb, rows)
+        assert numpy.abs(s_dir[rows] - ref).max() < 1e-12 * scale
