@@ -465,10 +465,14 @@ __global__ __launch_bounds__(NT) void k_nat_ss_diag(NatView v, double shift, dou
 // wavefront (scalar loads of links and h), the c loads of the alpha-alpha
 // part coalesced; cost per determinant ~ (nlink_a + nlink_b)^2, against
 // npair^2 per closure column for the GEMM form.
+// The beta-beta part runs in k_nat_direct_bb on the transposed vector (the
+// same uniform form with the spins' roles swapped); k_nat_direct adds its
+// result st (transposed blocks) to the alpha-alpha and alpha-beta parts.
 __global__ __launch_bounds__(NT) void k_nat_direct(NatView v, const int64_t *__restrict__ wbase,
                                                    const double *__restrict__ h,
                                                    const double *__restrict__ hs,
-                                                   const double *__restrict__ c, double *__restrict__ sigma)
+                                                   const double *__restrict__ c,
+                                                   const double *__restrict__ st, double *__restrict__ sigma)
 {
         const int64_t w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (NT / 64) + (threadIdx.x >> 6))) +
                           (int64_t)0;
@@ -504,24 +508,6 @@ __global__ __launch_bounds__(NT) void k_nat_direct(NatView v, const int64_t *__r
                         acc += ((e1 < 0) != (e2 < 0)) ? -coef * x : coef * x;
                 }
         }
-        // beta-beta: per lane
-        const int32_t *dA = v.dblk + (int64_t)A * v.nsecB;
-        for (int k1 = 0; k1 < v.nlinkB; k1++) {
-                const int32_t e1 = v.lkB[(int64_t)k1 * nsb + gb];
-                if (e1 == -1) continue;
-                const int32_t K = pm_index(e1);
-                const double *hP = h + (int64_t)v.lkpB[(int64_t)k1 * nsb + gb] * npair;
-                for (int k2 = 0; k2 < v.nlinkB; k2++) {
-                        const int32_t e2 = v.lkB[(int64_t)k2 * nsb + K];
-                        if (e2 == -1) continue;
-                        const int32_t J = pm_index(e2);
-                        const int blk2 = dA[v.ssecB[J]];
-                        if (blk2 < 0) continue;
-                        const double coef = hP[v.lkpB[(int64_t)k2 * nsb + K]];
-                        const double x = c[v.boff[blk2] + a * v.bnb[blk2] + v.saddrB[J]];
-                        acc += ((e1 < 0) != (e2 < 0)) ? -coef * x : coef * x;
-                }
-        }
         // alpha-beta: alpha links uniform, beta links per lane
         for (int k1 = 0; k1 < v.nlinkA; k1++) {
                 const int32_t e1 = v.lkA[(int64_t)k1 * nsa + r];
@@ -541,7 +527,69 @@ __global__ __launch_bounds__(NT) void k_nat_direct(NatView v, const int64_t *__r
                         acc += ((e1 < 0) != (e2 < 0)) ? -coef * x : coef * x;
                 }
         }
-        if (valid) sigma[v.boff[blk] + a * nb + b] = acc;
+        if (valid) {
+                const int64_t na = (v.boff[blk + 1] - v.boff[blk]) / nb;
+                sigma[v.boff[blk] + a * nb + b] = acc + st[v.boff[blk] + b * na + a];
+        }
+}
+
+// Beta-beta part of the direct sigma on transposed blocks (block b of D
+// stored beta-major, [nstr(sb)][nstr(sa)] at the same offset): one
+// wavefront per (block, beta address, 64 alpha addresses), the beta link
+// loops uniform and the loads of ct coalesced:
+//   st[I] = sum over beta links (P, K) of I_b, links (R, J) of K:
+//           h[P][R] s s' ct[I_a, J]
+__global__ __launch_bounds__(NT) void k_nat_direct_bb(NatView v, const int64_t *__restrict__ wbase,
+                                                      const double *__restrict__ h,
+                                                      const double *__restrict__ ct, double *__restrict__ st)
+{
+        const int64_t w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (NT / 64) + (threadIdx.x >> 6))) +
+                          (int64_t)0;
+        const int lane = threadIdx.x & 63;
+        if (w >= wbase[v.nblock]) return;
+        const int blk = upper_index(wbase, v.nblock, w);
+        const int64_t nb = v.bnb[blk];
+        const int64_t na = (v.boff[blk + 1] - v.boff[blk]) / nb;
+        const int64_t nch = (na + 63) >> 6;
+        const int64_t local = w - wbase[blk];
+        const int64_t b = local / nch;
+        const int64_t a = (local - b * nch) * 64 + lane;
+        const bool valid = a < na;
+        const int64_t al = valid ? a : na - 1;
+        const int A = v.bsa[blk], B = v.bsb[blk];
+        const int64_t gb = v.sbaseB[B] + b;
+        const int npair = v.npair;
+        const int64_t nsb = v.nstrB;
+        const int32_t *dA = v.dblk + (int64_t)A * v.nsecB;
+        double acc = 0.0;
+        for (int k1 = 0; k1 < v.nlinkB; k1++) {
+                const int32_t e1 = v.lkB[(int64_t)k1 * nsb + gb];
+                if (e1 == -1) continue;
+                const int32_t K = pm_index(e1);
+                const double *hP = h + (int64_t)v.lkpB[(int64_t)k1 * nsb + gb] * npair;
+                for (int k2 = 0; k2 < v.nlinkB; k2++) {
+                        const int32_t e2 = v.lkB[(int64_t)k2 * nsb + K];
+                        if (e2 == -1) continue;
+                        const int32_t J = pm_index(e2);
+                        const int blk2 = dA[v.ssecB[J]];
+                        if (blk2 < 0) continue;
+                        const double coef = hP[v.lkpB[(int64_t)k2 * nsb + K]];
+                        const double x = ct[v.boff[blk2] + (int64_t)v.saddrB[J] * na + al];
+                        acc += ((e1 < 0) != (e2 < 0)) ? -coef * x : coef * x;
+                }
+        }
+        if (valid) st[v.boff[blk] + b * na + a] = acc;
+}
+
+// ct[block b] = c[block b] transposed (alpha-major -> beta-major).
+__global__ __launch_bounds__(NT) void k_nat_transpose(NatView v, const double *__restrict__ c,
+                                                      double *__restrict__ ct)
+{
+        const int64_t d = (int64_t)blockIdx.x * NT + threadIdx.x;
+        if (d >= v.ndet) return;
+        const DPos q = d_locate(v, d);
+        const int64_t nb = v.bnb[q.blk], na = (v.boff[q.blk + 1] - v.boff[q.blk]) / nb;
+        ct[v.boff[q.blk] + q.b * na + q.a] = c[d];
 }
 
 __global__ void k_nat_fill(int32_t *__restrict__ x, int64_t n, int32_t val)
@@ -581,8 +629,9 @@ struct gas_native_plan {
         void *d_t1 = nullptr, *d_vt1 = nullptr;
         int64_t ld = 0;                 // columns per tile
         std::vector<int64_t> r0, c0;    // tile row and column starts (+ ends)
-        int64_t *d_wbase = nullptr;     // direct: wavefronts before each block (+ total)
-        int64_t nwave = 0;
+        int64_t *d_wbase = nullptr;     // direct: wavefronts before each block (+ total),
+        int64_t nwave = 0, nwave_bb = 0;   // then the same for the transposed beta-beta kernel
+        double *d_ct = nullptr;         // direct: transposed c and beta-beta part (2 ndet)
 };
 
 namespace {
@@ -960,6 +1009,7 @@ void fcis::gas_native_plan_free(gas_native_plan *p)
         if (p->d_h) (void)hipFree(p->d_h);
         if (p->d_t1) (void)hipFree(p->d_t1);
         if (p->d_wbase) (void)hipFree(p->d_wbase);
+        if (p->d_ct) (void)hipFree(p->d_ct);
         delete p;
 }
 
@@ -997,12 +1047,14 @@ int fcis::gas_native_plan_create(gas_native_plan **out, const fcis_gas_space *g,
                 if (env != nullptr && strcmp(env, "t1") == 0) p->direct = 0;
         }
         if (p->direct) {
-                std::vector<int64_t> wb(g->nblock + 1, 0);
+                std::vector<int64_t> wb(2 * (size_t)(g->nblock + 1), 0);
                 for (int b = 0; b < g->nblock; b++) {
-                        const int64_t nbs = g->nstr[g->bsb[b]];
-                        wb[b + 1] = wb[b] + g->nstr[g->bsa[b]] * ((nbs + 63) / 64);
+                        const int64_t nas = g->nstr[g->bsa[b]], nbs = g->nstr[g->bsb[b]];
+                        wb[b + 1] = wb[b] + nas * ((nbs + 63) / 64);
+                        wb[g->nblock + 1 + b + 1] = wb[g->nblock + 1 + b] + nbs * ((nas + 63) / 64);
                 }
                 p->nwave = wb[g->nblock];
+                p->nwave_bb = wb[2 * (size_t)g->nblock + 1];
                 std::vector<double> hh((size_t)2 * np * np);
                 for (int i = 0; i < np; i++)
                         for (int j = 0; j < np; j++) {
@@ -1010,7 +1062,8 @@ int fcis::gas_native_plan_create(gas_native_plan **out, const fcis_gas_space *g,
                                 hh[(size_t)np * np + (size_t)i * np + j] = h2e[(size_t)i * np + j] + h2e[(size_t)j * np + i];
                         }
                 if (fcis::dev_malloc(&p->d_h, 8 * hh.size()) != hipSuccess ||
-                    fcis::dev_malloc(&p->d_wbase, 8 * wb.size()) != hipSuccess) {
+                    fcis::dev_malloc(&p->d_wbase, 8 * wb.size()) != hipSuccess ||
+                    fcis::dev_malloc(&p->d_ct, 16 * (size_t)std::max<int64_t>(g->ndet, 1)) != hipSuccess) {
                         gas_native_plan_free(p);
                         fcis::set_error("fcis_gas_sigma_plan_create: device allocation failed");
                         return FCIS_ERR_MEMORY;
@@ -1123,10 +1176,19 @@ int fcis::gas_native_plan_execute(gas_native_plan *p, const double *d_ci0, doubl
 {
         if (p->direct) {
                 if (p->nwave > 0) {
-                        const int64_t np = p->g->nat->npair;
+                        const fcis_gas_space *g = p->g;
+                        const int64_t np = g->nat->npair;
                         const double *h = static_cast<const double *>(p->d_h);
+                        double *ct = p->d_ct, *sbb = p->d_ct + g->ndet;
+                        const int64_t *wbt = p->d_wbase + g->nblock + 1;
+                        hipLaunchKernelGGL(k_nat_transpose, dim3(blocks_for(g->ndet)), dim3(NT), 0, st, g->nat->v,
+                                           d_ci0, ct);
+                        hipLaunchKernelGGL(k_nat_direct_bb,
+                                           dim3((unsigned)((p->nwave_bb + NT / 64 - 1) / (NT / 64))), dim3(NT), 0,
+                                           st, g->nat->v, wbt, h, ct, sbb);
                         hipLaunchKernelGGL(k_nat_direct, dim3((unsigned)((p->nwave + NT / 64 - 1) / (NT / 64))),
-                                           dim3(NT), 0, st, p->g->nat->v, p->d_wbase, h, h + np * np, d_ci0, d_ci1);
+                                           dim3(NT), 0, st, g->nat->v, p->d_wbase, h, h + np * np, d_ci0, sbb,
+                                           d_ci1);
                         FCIS_CHECK_LAUNCH();
                 }
                 return FCIS_SUCCESS;

@@ -433,3 +433,38 @@ def test_native_direct_and_t1_paths_agree(monkeypatch, orbs, ne, restr, kind):
         rows = numpy.random.default_rng(16).integers(0, gas.ndet, 80)
         ref = oracle.det_sigma_rows(h1e, eri, c, n, sa, sb, rows)
         assert numpy.abs(s_dir[rows] - ref).max() < 1e-12 * scale
+
+
+def test_native_transform_ci_within_gas_24_orbitals():
+    """transform_ci_within_gas on a sector-native space (RAS(4,6,14) (5,5),
+    315,616 determinants): the rotated vector's RDMs (sector-native RDM
+    kernels) are the rotated RDMs, and the energy in the rotated integrals
+    (sector-native sigma) is unchanged."""
+    orbs, ne = (4, 6, 14), (5, 5)
+    restr = dict(max_holes=2, max_particles=2)
+    n = 24
+    h1e, eri = syn.make_problem(n, ne, seed=17, with_ci=False)
+    sol = fg.FCISolver(gas_orbs=orbs, gas_restr=restr, gas_restr_type="ras")
+    assert sol._space(n, ne).native
+    rng = numpy.random.default_rng(18)
+    u = numpy.zeros((n, n))
+    o = 0
+    for k in orbs:
+        q, _ = numpy.linalg.qr(rng.standard_normal((k, k)))
+        u[o:o + k, o:o + k] = q
+        o += k
+    c = rng.normal(size=sol._space(n, ne).ndet)
+    c /= numpy.linalg.norm(c)
+    c1 = sol.transform_ci_within_gas(c, n, ne, u)
+    assert abs(numpy.linalg.norm(c1) - 1.0) < 1e-12
+    (a0, b0), (aa0, ab0, bb0) = sol.make_rdm12s(c, n, ne)
+    (a1, b1), (aa1, ab1, bb1) = sol.make_rdm12s(c1, n, ne)
+    rot4 = lambda t: numpy.einsum("ijkl,ip,jq,kr,ls->pqrs", t, u, u, u, u, optimize=True)
+    numpy.testing.assert_allclose(a1, u.T @ a0 @ u, rtol=0, atol=1e-12)
+    numpy.testing.assert_allclose(b1, u.T @ b0 @ u, rtol=0, atol=1e-12)
+    for x1, x0 in ((aa1, aa0), (ab1, ab0), (bb1, bb0)):
+        numpy.testing.assert_allclose(x1, rot4(x0), rtol=0, atol=1e-12)
+    g = _eri4(eri, n)
+    e0 = sol.energy(h1e, g, c, n, ne)
+    e1 = sol.energy(u.T @ h1e @ u, rot4(g), c1, n, ne)
+    assert abs(e1 - e0) < 1e-10 * max(1.0, abs(e0))
