@@ -465,6 +465,42 @@ __global__ __launch_bounds__(NT) void k_nat_ss_diag(NatView v, double shift, dou
 // wavefront (scalar loads of links and h), the c loads of the alpha-alpha
 // part coalesced; cost per determinant ~ (nlink_a + nlink_b)^2, against
 // npair^2 per closure column for the GEMM form.
+// Inner link loop of the direct form, wavefront-uniform string K: sum over
+// the links (R, J) of K (slots of lk / lkp, ns strings) whose J pairs with
+// a block of D through dtab[ssec[J] * dstride] of h[P][R] s s' x[boff +
+// saddr[J] * rowlen + off] (rowlen the block's beta count, or rowlen_c when
+// given), U links per step with all their loads issued together.
+template <bool ROWLEN_BLOCK>
+__device__ __forceinline__ double uniform_links(const NatView &v, const int32_t *__restrict__ lk,
+                                                const int16_t *__restrict__ lkp, int64_t ns, int nl, int32_t K,
+                                                const uint16_t *__restrict__ ssec,
+                                                const uint32_t *__restrict__ saddr,
+                                                const int32_t *__restrict__ dtab, int64_t dstride,
+                                                const double *__restrict__ hP, const double *__restrict__ x,
+                                                int64_t off, int64_t rowlen_c, bool neg1)
+{
+        constexpr int U = 8;
+        double acc = 0.0;
+        for (int k2 = 0; k2 < nl; k2 += U) {
+                int32_t e[U], J[U], blk2[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) e[u] = k2 + u < nl ? lk[(int64_t)(k2 + u) * ns + K] : -1;
+#pragma unroll
+                for (int u = 0; u < U; u++) J[u] = e[u] == -1 ? 0 : pm_index(e[u]);
+#pragma unroll
+                for (int u = 0; u < U; u++) blk2[u] = e[u] == -1 ? -1 : dtab[(int64_t)ssec[J[u]] * dstride];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                        if (blk2[u] < 0) continue;
+                        const double coef = hP[lkp[(int64_t)(k2 + u) * ns + K]];
+                        const int64_t rl = ROWLEN_BLOCK ? v.bnb[blk2[u]] : rowlen_c;
+                        const double xv = x[v.boff[blk2[u]] + (int64_t)saddr[J[u]] * rl + off];
+                        acc += ((e[u] < 0) != neg1) ? -coef * xv : coef * xv;
+                }
+        }
+        return acc;
+}
+
 // The beta-beta part runs in k_nat_direct_bb on the transposed vector (the
 // same uniform form with the spins' roles swapped); k_nat_direct adds its
 // result st (transposed blocks) to the alpha-alpha and alpha-beta parts.
@@ -497,16 +533,8 @@ __global__ __launch_bounds__(NT) void k_nat_direct(NatView v, const int64_t *__r
                 if (e1 == -1) continue;
                 const int32_t K = pm_index(e1);
                 const double *hP = h + (int64_t)v.lkpA[(int64_t)k1 * nsa + r] * npair;
-                for (int k2 = 0; k2 < v.nlinkA; k2++) {
-                        const int32_t e2 = v.lkA[(int64_t)k2 * nsa + K];
-                        if (e2 == -1) continue;
-                        const int32_t J = pm_index(e2);
-                        const int blk2 = v.dblk[(int64_t)v.ssecA[J] * v.nsecB + B];
-                        if (blk2 < 0) continue;
-                        const double coef = hP[v.lkpA[(int64_t)k2 * nsa + K]];
-                        const double x = c[v.boff[blk2] + (int64_t)v.saddrA[J] * v.bnb[blk2] + bl];
-                        acc += ((e1 < 0) != (e2 < 0)) ? -coef * x : coef * x;
-                }
+                acc += uniform_links<true>(v, v.lkA, v.lkpA, nsa, v.nlinkA, K, v.ssecA, v.saddrA, v.dblk + B,
+                                           v.nsecB, hP, c, bl, 0, e1 < 0);
         }
         // alpha-beta: alpha links uniform, beta links per lane
         for (int k1 = 0; k1 < v.nlinkA; k1++) {
@@ -516,15 +544,23 @@ __global__ __launch_bounds__(NT) void k_nat_direct(NatView v, const int64_t *__r
                 const int32_t *dA2 = v.dblk + (int64_t)v.ssecA[Ja] * v.nsecB;
                 const int64_t aJ = v.saddrA[Ja];
                 const double *hP = hs + (int64_t)v.lkpA[(int64_t)k1 * nsa + r] * npair;
-                for (int k2 = 0; k2 < v.nlinkB; k2++) {
-                        const int32_t e2 = v.lkB[(int64_t)k2 * nsb + gb];
-                        if (e2 == -1) continue;
-                        const int32_t Jb = pm_index(e2);
-                        const int blk2 = dA2[v.ssecB[Jb]];
-                        if (blk2 < 0) continue;
-                        const double coef = hP[v.lkpB[(int64_t)k2 * nsb + gb]];
-                        const double x = c[v.boff[blk2] + aJ * v.bnb[blk2] + v.saddrB[Jb]];
-                        acc += ((e1 < 0) != (e2 < 0)) ? -coef * x : coef * x;
+                constexpr int U = 8;
+                for (int k2 = 0; k2 < v.nlinkB; k2 += U) {
+                        int32_t e2[U], Jb[U], blk2[U];
+#pragma unroll
+                        for (int u = 0; u < U; u++)
+                                e2[u] = k2 + u < v.nlinkB ? v.lkB[(int64_t)(k2 + u) * nsb + gb] : -1;
+#pragma unroll
+                        for (int u = 0; u < U; u++) Jb[u] = e2[u] == -1 ? 0 : pm_index(e2[u]);
+#pragma unroll
+                        for (int u = 0; u < U; u++) blk2[u] = e2[u] == -1 ? -1 : dA2[v.ssecB[Jb[u]]];
+#pragma unroll
+                        for (int u = 0; u < U; u++) {
+                                if (blk2[u] < 0) continue;
+                                const double coef = hP[v.lkpB[(int64_t)(k2 + u) * nsb + gb]];
+                                const double x = c[v.boff[blk2[u]] + aJ * v.bnb[blk2[u]] + v.saddrB[Jb[u]]];
+                                acc += ((e1 < 0) != (e2[u] < 0)) ? -coef * x : coef * x;
+                        }
                 }
         }
         if (valid) {
@@ -567,16 +603,8 @@ __global__ __launch_bounds__(NT) void k_nat_direct_bb(NatView v, const int64_t *
                 if (e1 == -1) continue;
                 const int32_t K = pm_index(e1);
                 const double *hP = h + (int64_t)v.lkpB[(int64_t)k1 * nsb + gb] * npair;
-                for (int k2 = 0; k2 < v.nlinkB; k2++) {
-                        const int32_t e2 = v.lkB[(int64_t)k2 * nsb + K];
-                        if (e2 == -1) continue;
-                        const int32_t J = pm_index(e2);
-                        const int blk2 = dA[v.ssecB[J]];
-                        if (blk2 < 0) continue;
-                        const double coef = hP[v.lkpB[(int64_t)k2 * nsb + K]];
-                        const double x = ct[v.boff[blk2] + (int64_t)v.saddrB[J] * na + al];
-                        acc += ((e1 < 0) != (e2 < 0)) ? -coef * x : coef * x;
-                }
+                acc += uniform_links<false>(v, v.lkB, v.lkpB, nsb, v.nlinkB, K, v.ssecB, v.saddrB, dA, 1, hP, ct,
+                                            al, na, e1 < 0);
         }
         if (valid) st[v.boff[blk] + b * na + a] = acc;
 }
