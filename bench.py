@@ -584,77 +584,91 @@ def gas_solve(pkg, sync, h1e, eri, steps, with_cpu, max_cycle, davidson=True):
     return out
 
 
+GAS_NATIVE_WORKLOADS = [
+    ("ras20", (4, 12, 4), (6, 6), {"max_holes": 2, "max_particles": 2}, "ras",
+     "RAS(4,12,4) (6,6) max_holes 2 max_particles 2, 20 orbitals"),
+    ("ras40", (2, 8, 30), (4, 4), {"max_holes": 1, "max_particles": 2}, "ras",
+     "RAS(2,8,30) (4,4) max_holes 1 max_particles 2, 40 orbitals"),
+    ("gas62", (2, 6, 24, 30), (4, 4), [[3, 4], [6, 8], [7, 8], [8, 8]], "cumulative-occ",
+     "GAS(2,6,24,30) (4,4) cumulative occupations [3,4] [6,8] [7,8] [8,8], 62 orbitals"),
+]
+
+
 def gas_native_sigma(pkg, sync, steps, with_cpu):
-    """Sector-native restricted sigma: RAS(4,12,4), (6,6) electrons, at most
-    2 holes in RAS1 and 2 electrons in RAS3 (20 orbitals, 6,989,020
-    determinants; the CAS of 20 orbitals, 2.4e8 determinants, is never
-    formed).  FP64 and FP32 sigma (t1 over the single-excitation closure,
-    GEMM, pull), and the reference's libfci_gas sigma on the same space on
-    the host cores."""
+    """Sector-native restricted sigma on three spaces whose CAS is never
+    formed: RAS(4,12,4) (6,6) in 20 orbitals (6,989,020 determinants; the
+    plan's cost model picks the GEMM form: t1 over the single-excitation
+    closure, GEMM, pull), RAS(2,8,30) (4,4) in 40 orbitals (1,543,896) and a
+    4-subspace GAS in 62 orbitals (1,352,121), both in the direct link-pair
+    form.  FP64 and FP32 plans, and the reference's libfci_gas sigma on the
+    same space on the host cores."""
     import numpy
     import torch
     fg, ag = pkg.fci_gas, pkg.addons_gas
-    orbs, ne, restr = (4, 12, 4), (6, 6), {"max_holes": 2, "max_particles": 2}
-    n = sum(orbs)
-    o, D = ag.normalize_gas_spec(orbs, ne, restr, "ras")
-    h1e, eri = pkg.synthetic.make_problem(n, ne, seed=0, with_ci=False)
-    h2e = fg.absorb_h1e(h1e, eri, n, ne, 0.5)
-    t0 = time.perf_counter()
-    gas = fg.GasSpace(o, ne, D, native=True)
-    sync()
-    n1, maxrow, nsa, nsb = gas.native_info()
-    out = {"workload": "RAS(4,12,4) (6,6) max_holes 2 max_particles 2, 20 orbitals, "
-                       "sector-native, synthetic integrals seed 0",
-           "ndet": gas.ndet, "closure": n1, "space_s": time.perf_counter() - t0}
-    g = torch.Generator(device="cuda").manual_seed(5)
-    x = torch.randn(gas.ndet, dtype=torch.float64, device="cuda", generator=g)
-    ref = None
-    npair = n * (n + 1) // 2
-    for prec in ("fp64", "fp32"):
-        with fg.GasContractPlan(gas, h2e, prec) as plan:
-            y = torch.empty_like(x)
-            plan.execute(x, y)
-            sync()
-            t1 = time.perf_counter()
-            for _ in range(steps):
+    res = {}
+    for key, orbs, ne, restr, kind, label in GAS_NATIVE_WORKLOADS:
+        n = sum(orbs)
+        o, D = ag.normalize_gas_spec(orbs, ne, restr, kind)
+        h1e, eri = pkg.synthetic.make_problem(n, ne, seed=0, with_ci=False)
+        h2e = fg.absorb_h1e(h1e, eri, n, ne, 0.5)
+        t0 = time.perf_counter()
+        gas = fg.GasSpace(o, ne, D, native=True)
+        sync()
+        n1, maxrow, nsa, nsb = gas.native_info()
+        out = {"workload": label + ", sector-native, synthetic integrals seed 0",
+               "ndet": gas.ndet, "closure": n1, "space_s": time.perf_counter() - t0}
+        g = torch.Generator(device="cuda").manual_seed(5)
+        x = torch.randn(gas.ndet, dtype=torch.float64, device="cuda", generator=g)
+        ref = None
+        npair = n * (n + 1) // 2
+        for prec in ("fp64", "fp32"):
+            with fg.GasContractPlan(gas, h2e, prec) as plan:
+                y = torch.empty_like(x)
                 plan.execute(x, y)
-            sync()
-            dt = (time.perf_counter() - t1) / steps
-        rec = {"ms": dt * 1e3, "sigma_per_s": 1.0 / dt,
-               "gemm_TFLOP/s": 2.0 * npair * npair * n1 / dt / 1e12}
-        if prec == "fp64":
-            ref = y.clone()
-        else:
-            rec["rel_err_vs_fp64"] = float((y - ref).norm() / ref.norm())
-        out[prec] = rec
-        log("native GAS sigma %s: %.2f ms" % (prec, dt * 1e3))
-    if with_cpu:
-        import oracle
-        if oracle.ref_available():
-            R = oracle.RefLib().gas(o, ne, D)
-            P = R.plan(h2e)
-            xc = x.cpu().numpy()
-            yc = numpy.empty_like(xc)
-            P(xc, yc)
-            t1 = time.perf_counter()
-            reps = 0
-            while reps < 2 or time.perf_counter() - t1 < 6.0:
+                sync()
+                t1 = time.perf_counter()
+                for _ in range(steps):
+                    plan.execute(x, y)
+                sync()
+                dt = (time.perf_counter() - t1) / steps
+                path = plan.path
+            rec = {"ms": dt * 1e3, "sigma_per_s": 1.0 / dt, "path": path}
+            if path == "native-t1":
+                rec["gemm_TFLOP/s"] = 2.0 * npair * npair * n1 / dt / 1e12
+            if prec == "fp64":
+                ref = y.clone()
+            else:
+                rec["rel_err_vs_fp64"] = float((y - ref).norm() / ref.norm())
+            out[prec] = rec
+            log("native GAS %s sigma %s (%s): %.2f ms" % (key, prec, path, dt * 1e3))
+        if with_cpu:
+            import oracle
+            if oracle.ref_available():
+                R = oracle.RefLib().gas(o, ne, D)
+                P = R.plan(h2e)
+                xc = x.cpu().numpy()
+                yc = numpy.empty_like(xc)
                 P(xc, yc)
-                reps += 1
-            dt = (time.perf_counter() - t1) / reps
-            P.close()
-            R.close()
-            cores = int(os.environ.get("OMP_NUM_THREADS", 0)) or len(os.sched_getaffinity(0))
-            out["cpu_reference"] = {"sigma_s": dt, "cores": cores, "runs": reps,
-                                    "kind": "reference libfci_gas",
-                                    "max_rel_diff_vs_gpu_fp64":
-                                        float(numpy.abs(yc - ref.cpu().numpy()).max() /
-                                              numpy.abs(yc).max())}
-            log("native GAS sigma libfci_gas on %d cores: %.3f s" % (cores, dt))
-    del x, ref
-    gas.close()
-    torch.cuda.empty_cache()
-    return out
+                t1 = time.perf_counter()
+                reps = 0
+                while reps < 2 or time.perf_counter() - t1 < 6.0:
+                    P(xc, yc)
+                    reps += 1
+                dt = (time.perf_counter() - t1) / reps
+                P.close()
+                R.close()
+                cores = int(os.environ.get("OMP_NUM_THREADS", 0)) or len(os.sched_getaffinity(0))
+                out["cpu_reference"] = {"sigma_s": dt, "cores": cores, "runs": reps,
+                                        "kind": "reference libfci_gas",
+                                        "max_rel_diff_vs_gpu_fp64":
+                                            float(numpy.abs(yc - ref.cpu().numpy()).max() /
+                                                  numpy.abs(yc).max())}
+                log("native GAS %s sigma libfci_gas on %d cores: %.3f s" % (key, cores, dt))
+        del x, ref
+        gas.close()
+        torch.cuda.empty_cache()
+        res[key] = out
+    return res
 
 
 def main():

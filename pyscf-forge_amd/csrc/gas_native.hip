@@ -42,6 +42,7 @@
 //   <bra| a_p^+ a_q |ket>           = sum_{K in D} bra_K D_ket[p n + q, K],
 // assembled into dm1[p,q] = <q^+ p> and dm2[p,q,r,s] = <p^+ r^+ s q>
 // (fci_gas.h:255-268) with rocBLAS GEMMs over K.
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 #include <algorithm>
@@ -620,6 +621,435 @@ __global__ __launch_bounds__(NT) void k_nat_transpose(NatView v, const double *_
         ct[v.boff[q.blk] + q.b * na + q.a] = c[d];
 }
 
+// ---- Operator form of the direct sigma ----
+// Same-spin part as a sparse one-spin operator: for the strings I of a
+// spin's D sectors (sectors that are that spin's sector of some block),
+//   A[I][J] = sum_{P,R} h[P][R] <I|F_P|K> <K|F_R|J>   (K over all strings)
+// over the strings J of the spin's D sectors at most two excitations from I
+// (I itself, single and double replacements), so that
+//   sigma_aa[I_a, I_b] = sum_J A_a[I_a][J] c[J, I_b]
+// (J's block with I_b's sector must be in D).  A row's entries are grouped
+// by J's sector in ascending order (segments); the number of strings of a
+// sector T within two excitations of a string of sector S depends only on
+// the two sectors' occupations, so every row of S has the same length and
+// the same segment boundaries.  Within a segment the entries follow the
+// enumeration order of the replacements of I: I, then (q, p) for the k
+// occupied q and n - k empty p, then pairs q < s of occupied and p < r of
+// empty orbitals.  The opposite-spin part walks link lists sorted the same
+// way (targets in the spin's D sectors only, diagonal slots first), so a
+// wavefront skips whole segments whose sector pair is not a block of D.
+constexpr int OP_MAXSEG = 1024;    // segments per source sector (LDS counters)
+
+struct OpView {
+        int ndsec;
+        int64_t nrows;                  // strings of the spin's D sectors
+        const int32_t *dsec;            // [ndsec] local sector
+        const int32_t *dindex;          // [nsec] position in dsec, or -1
+        const int64_t *rowbase;         // [ndsec + 1] first row of each D sector
+        const int64_t *opbase;          // [ndsec] first operator entry
+        const int32_t *oplen;           // [ndsec] row length
+        const int32_t *opsegptr;        // [ndsec + 1]
+        const int32_t *opsegT, *opsegoff;   // segment target sector and start in the row
+        const int16_t *opsegk;          // [ndsec][nsec] segment (relative) or -1
+        const int64_t *lsbase;          // [ndsec] first link entry
+        const int32_t *lslen;           // [ndsec] links per string
+        const int32_t *lssegptr;        // [ndsec + 1]
+        const int32_t *lssegT, *lssegoff;
+        const int16_t *lssegk;          // [ndsec][nsec]
+        const uint32_t *opaddr;         // target address within its sector
+        const uint64_t *ls;             // address | pair << 32 | sign << 63
+        const double *opcoef;           // A entries (per plan)
+};
+
+__device__ __forceinline__ int cre_des_neg(uint64_t S, int cre, int des)
+{
+        const int lo = cre < des ? cre : des, hi = cre < des ? des : cre;
+        const uint64_t m = hi > lo + 1 ? ((1ull << hi) - (1ull << (lo + 1))) : 0ull;
+        return __popcll(S & m) & 1;
+}
+
+__device__ __forceinline__ uint64_t lane_below_mask()
+{
+        const int lane = threadIdx.x & 63;
+        return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
+
+// (i, j), i < j, of combination index t over pairs ordered by j, then i.
+__device__ __forceinline__ void unpair(int64_t t, int &i, int &j)
+{
+        int jj = (int)((1.0 + sqrt(1.0 + 8.0 * (double)t)) * 0.5);
+        while ((int64_t)jj * (jj - 1) / 2 > t) jj--;
+        while ((int64_t)(jj + 1) * jj / 2 <= t) jj++;
+        j = jj;
+        i = (int)(t - (int64_t)jj * (jj - 1) / 2);
+}
+
+// Local sector of the string reached from sector S by moving one electron
+// from orbital q to orbital p (same sector inside one subspace), or -1.
+__device__ __forceinline__ int move_sector(const SubInfo &si, const int32_t *__restrict__ mv, int S, int q, int p)
+{
+        if (S < 0) return -1;
+        const int kq = si.sub_of[q], kp = si.sub_of[p];
+        return kq == kp ? S : mv[((int64_t)S * si.ngas + kq) * si.ngas + kp];
+}
+
+__device__ __forceinline__ int64_t sector_address(const SubInfo &si, const int64_t *__restrict__ stride, int T,
+                                                  uint64_t J)
+{
+        int64_t addr = 0;
+        for (int k = 0; k < si.ngas; k++) {
+                const uint64_t sub = (J >> si.start[k]) & ((1ull << si.norb[k]) - 1);
+                addr += rank_dev(sub) * stride[(int64_t)T * si.ngas + k];
+        }
+        return addr;
+}
+
+// Structure of one spin's operator rows (OPS) or sorted link lists (!OPS):
+// one wavefront per string of the spin's D sectors enumerates its
+// replacements, keeps those whose sector is a segment of its row and writes
+// them in enumeration order within each segment (ranks by ballot; per-
+// segment counters in LDS).  Link lists are row-major (ROWMAJOR: the links
+// of one string contiguous) or slot-major within each D sector (slot s of
+// string a of sector i at lsbase[i] + s * nstr + a).
+template <bool OPS>
+__global__ __launch_bounds__(NT) void k_op_struct(NatView v, OpView o, int spin, int rowmajor, SubInfo si,
+                                                  const int64_t *__restrict__ stride,
+                                                  const int32_t *__restrict__ mv, uint32_t *__restrict__ opaddr,
+                                                  uint64_t *__restrict__ ls, int *__restrict__ err)
+{
+        __shared__ int32_t cnt[NT / 64][OP_MAXSEG];
+        __shared__ int8_t orb[NT / 64][64];
+        const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const int64_t row = (int64_t)blockIdx.x * (NT / 64) + wv;
+        const bool active = row < o.nrows;
+        const int i = active ? upper_index(o.rowbase, o.ndsec, row) : 0;
+        const int64_t a = active ? row - o.rowbase[i] : 0;
+        const int S = o.dsec[i];
+        const int64_t *sbase = spin ? v.sbaseB : v.sbaseA;
+        const uint64_t I = active ? (spin ? v.strB : v.strA)[sbase[S] + a] : 0ull;
+        const int n = v.n, k = __popcll(I), nv = n - k;
+        const bool occb = lane < n && ((I >> lane) & 1);
+        const uint64_t mo = __ballot(occb), me = __ballot(lane < n && !occb);
+        if (occb) orb[wv][__popcll(mo & lane_below_mask())] = (int8_t)lane;
+        else if (lane < n) orb[wv][k + __popcll(me & lane_below_mask())] = (int8_t)lane;
+        const int sp0 = OPS ? o.opsegptr[i] : o.lssegptr[i];
+        const int nseg = (OPS ? o.opsegptr[i + 1] : o.lssegptr[i + 1]) - sp0;
+        for (int j = lane; j < nseg && j < OP_MAXSEG; j += 64) cnt[wv][j] = 0;
+        __syncthreads();
+        if (!active) return;
+        const int16_t *segk = (OPS ? o.opsegk : o.lssegk) + (int64_t)i * (spin ? v.nsecB : v.nsecA);
+        const int32_t *segoff = (OPS ? o.opsegoff : o.lssegoff) + sp0;
+        const int64_t nsS = o.rowbase[i + 1] - o.rowbase[i];
+        const int32_t rowlen = OPS ? o.oplen[i] : o.lslen[i];
+        const int64_t total = OPS ? 1 + (int64_t)k * nv + (int64_t)(k * (k - 1) / 2) * (nv * (nv - 1) / 2)
+                                  : (int64_t)k + (int64_t)k * nv;
+        const int64_t c2v = (int64_t)nv * (nv - 1) / 2;
+        for (int64_t x0 = 0; x0 < total; x0 += 64) {
+                const int64_t x = x0 + lane;
+                int seg = -1;
+                uint64_t val = 0;
+                if (x < total) {
+                        uint64_t J = I;
+                        int T = S;
+                        int cre = -1, des = -1;
+                        if (OPS) {
+                                if (x == 0) {
+                                } else if (x < 1 + (int64_t)k * nv) {
+                                        const int64_t y = x - 1;
+                                        des = orb[wv][y / nv];
+                                        cre = orb[wv][k + y % nv];
+                                        T = move_sector(si, mv, S, des, cre);
+                                        J = I ^ (1ull << des) ^ (1ull << cre);
+                                } else {
+                                        const int64_t y = x - 1 - (int64_t)k * nv;
+                                        int oi, oj, vi, vj;
+                                        unpair(y / c2v, oi, oj);
+                                        unpair(y % c2v, vi, vj);
+                                        const int q = orb[wv][oi], s2 = orb[wv][oj];
+                                        const int p = orb[wv][k + vi], r = orb[wv][k + vj];
+                                        T = move_sector(si, mv, move_sector(si, mv, S, q, p), s2, r);
+                                        J = I ^ (1ull << q) ^ (1ull << s2) ^ (1ull << p) ^ (1ull << r);
+                                }
+                        } else {
+                                if (x < k) {
+                                        des = cre = orb[wv][x];
+                                } else {
+                                        const int64_t y = x - k;
+                                        des = orb[wv][y / nv];
+                                        cre = orb[wv][k + y % nv];
+                                        T = move_sector(si, mv, S, des, cre);
+                                        J = I ^ (1ull << des) ^ (1ull << cre);
+                                }
+                        }
+                        if (T >= 0) seg = segk[T];
+                        if (seg >= 0) {
+                                const uint64_t addr = (uint64_t)sector_address(si, stride, T, J);
+                                val = OPS ? addr
+                                          : addr | ((uint64_t)fcis::pair_index(cre, des) << 32) |
+                                                    ((uint64_t)(cre != des ? cre_des_neg(I, cre, des) : 0) << 63);
+                        }
+                }
+                uint64_t pend = __ballot(seg >= 0);
+                while (pend) {
+                        const int l = __ffsll((long long)pend) - 1;
+                        const int k0 = __builtin_amdgcn_readlane(seg, l);
+                        const uint64_t m = __ballot(seg == k0);
+                        const int base = cnt[wv][k0];
+                        const int32_t send = k0 + 1 < nseg ? segoff[k0 + 1] : rowlen;
+                        if (seg == k0) {
+                                const int64_t slot = segoff[k0] + base + __popcll(m & lane_below_mask());
+                                if (slot >= send)
+                                        atomicAdd(err, 1);
+                                else if (OPS)
+                                        opaddr[o.opbase[i] + a * o.oplen[i] + slot] = (uint32_t)val;
+                                else
+                                        ls[o.lsbase[i] + (rowmajor ? a * o.lslen[i] + slot : slot * nsS + a)] = val;
+                        }
+                        __builtin_amdgcn_wave_barrier();
+                        if (lane == 0) cnt[wv][k0] = base + __popcll(m);
+                        __builtin_amdgcn_wave_barrier();
+                        pend &= ~m;
+                }
+        }
+        // every segment filled exactly
+        for (int j = lane; j < nseg; j += 64) {
+                const int32_t send = j + 1 < nseg ? segoff[j + 1] : rowlen;
+                if (cnt[wv][j] != send - segoff[j]) atomicAdd(err, 1);
+        }
+}
+
+// <I| sum_{P,R} h[P][R] F_P F_R |J> over one spin's strings (I and J at
+// most two replacements apart), summed over the paths I -> K -> J in a
+// fixed order.
+__device__ double op_element(uint64_t I, uint64_t J, int n, const double *__restrict__ h, int np)
+{
+        const uint64_t d = I ^ J;
+        const int nd = __popcll(d) >> 1;
+        const uint64_t full = n == 64 ? ~0ull : ((1ull << n) - 1);
+        double acc = 0.0;
+        if (nd == 0) {
+                for (uint64_t x = I; x; x &= x - 1) {
+                        const int p = __ffsll((long long)x) - 1;
+                        const double *hp = h + (int64_t)fcis::pair_index(p, p) * np;
+                        for (uint64_t y = I; y; y &= y - 1) {
+                                const int r = __ffsll((long long)y) - 1;
+                                acc += hp[fcis::pair_index(r, r)];
+                        }
+                }
+                for (uint64_t x = I; x; x &= x - 1) {
+                        const int q = __ffsll((long long)x) - 1;
+                        for (uint64_t y = ~I & full; y; y &= y - 1) {
+                                const int e = __ffsll((long long)y) - 1;
+                                const int P = fcis::pair_index(e, q);
+                                acc += h[(int64_t)P * np + P];
+                        }
+                }
+        } else if (nd == 1) {
+                const int q = __ffsll((long long)(I & d)) - 1, p = __ffsll((long long)(J & d)) - 1;
+                const int Ppq = fcis::pair_index(p, q);
+                const double sJ = cre_des_neg(I, p, q) ? -1.0 : 1.0;
+                double t = 0.0;
+                for (uint64_t x = I; x; x &= x - 1) {
+                        const int r = __ffsll((long long)x) - 1;
+                        t += h[(int64_t)fcis::pair_index(r, r) * np + Ppq];
+                }
+                for (uint64_t x = J; x; x &= x - 1) {
+                        const int r = __ffsll((long long)x) - 1;
+                        t += h[(int64_t)Ppq * np + fcis::pair_index(r, r)];
+                }
+                acc = sJ * t;
+                for (uint64_t y = ~I & full & ~(1ull << p); y; y &= y - 1) {
+                        const int e = __ffsll((long long)y) - 1;
+                        const uint64_t K = I ^ (1ull << q) ^ (1ull << e);
+                        const int ng = cre_des_neg(I, e, q) ^ cre_des_neg(K, p, e);
+                        const double v = h[(int64_t)fcis::pair_index(e, q) * np + fcis::pair_index(p, e)];
+                        acc += ng ? -v : v;
+                }
+                for (uint64_t y = I & ~(1ull << q); y; y &= y - 1) {
+                        const int e = __ffsll((long long)y) - 1;
+                        const uint64_t K = I ^ (1ull << e) ^ (1ull << p);
+                        const int ng = cre_des_neg(I, p, e) ^ cre_des_neg(K, e, q);
+                        const double v = h[(int64_t)fcis::pair_index(p, e) * np + fcis::pair_index(e, q)];
+                        acc += ng ? -v : v;
+                }
+        } else if (nd == 2) {
+                const uint64_t rm = I & d, ad = J & d;
+                const int q0 = __ffsll((long long)rm) - 1, q1 = 63 - __clzll((long long)rm);
+                const int p0 = __ffsll((long long)ad) - 1, p1 = 63 - __clzll((long long)ad);
+                const int qs[2] = {q0, q1}, ps[2] = {p0, p1};
+#pragma unroll
+                for (int u = 0; u < 2; u++)
+#pragma unroll
+                        for (int w = 0; w < 2; w++) {
+                                const int a1 = qs[u], b1 = ps[w], a2 = qs[1 - u], b2 = ps[1 - w];
+                                const uint64_t K = I ^ (1ull << a1) ^ (1ull << b1);
+                                const int ng = cre_des_neg(I, b1, a1) ^ cre_des_neg(K, b2, a2);
+                                const double v =
+                                        h[(int64_t)fcis::pair_index(b1, a1) * np + fcis::pair_index(b2, a2)];
+                                acc += ng ? -v : v;
+                        }
+        }
+        return acc;
+}
+
+// Operator entries of one spin for the integrals h: one wavefront per row.
+__global__ __launch_bounds__(NT) void k_op_coef(NatView v, OpView o, int spin, const double *__restrict__ h,
+                                                double *__restrict__ coef)
+{
+        const int64_t row = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (NT / 64) + (threadIdx.x >> 6))) +
+                            (int64_t)0;
+        const int lane = threadIdx.x & 63;
+        if (row >= o.nrows) return;
+        const int i = upper_index(o.rowbase, o.ndsec, row);
+        const int64_t a = row - o.rowbase[i];
+        const int64_t *sbase = spin ? v.sbaseB : v.sbaseA;
+        const uint64_t *str = spin ? v.strB : v.strA;
+        const uint64_t I = str[sbase[o.dsec[i]] + a];
+        const int64_t r0 = o.opbase[i] + a * o.oplen[i];
+        const int s0 = o.opsegptr[i], s1 = o.opsegptr[i + 1];
+        for (int sg = s0; sg < s1; sg++) {
+                const int64_t e0 = o.opsegoff[sg], e1 = sg + 1 < s1 ? o.opsegoff[sg + 1] : o.oplen[i];
+                const int64_t jb = sbase[o.opsegT[sg]];
+                for (int64_t e = e0 + lane; e < e1; e += 64)
+                        coef[r0 + e] = op_element(I, str[jb + o.opaddr[r0 + e]], v.n, h, v.npair);
+        }
+}
+
+// Same-spin part through the operator of one spin: the sum over J of a row
+// of A times x[base(J's block) + addr(J) * ld]; the block of J's sector T
+// with the other spin's sector is dblk[T * tstride + other] (alpha: T * nsecB
+// + B; beta: A * nsecB + T).
+__device__ __forceinline__ double op_row(const OpView &o, int i, int64_t row0, const NatView &v,
+                                         const int32_t *__restrict__ dtab, int64_t tstride,
+                                         const double *__restrict__ x, int64_t lane_off, int64_t ld)
+{
+        double acc = 0.0;
+        const int s0 = o.opsegptr[i], s1 = o.opsegptr[i + 1];
+        for (int sg = s0; sg < s1; sg++) {
+                const int blk2 = dtab[(int64_t)o.opsegT[sg] * tstride];
+                if (blk2 < 0) continue;
+                const int64_t e0 = row0 + o.opsegoff[sg], e1 = row0 + (sg + 1 < s1 ? o.opsegoff[sg + 1] : o.oplen[i]);
+                const double *xb = x + v.boff[blk2] + lane_off;
+                constexpr int U = 8;
+                int64_t e = e0;
+                for (; e + U <= e1; e += U) {
+                        double cf[U], xv[U];
+#pragma unroll
+                        for (int u = 0; u < U; u++) {
+                                cf[u] = o.opcoef[e + u];
+                                xv[u] = xb[(int64_t)o.opaddr[e + u] * ld];
+                        }
+#pragma unroll
+                        for (int u = 0; u < U; u++) acc += cf[u] * xv[u];
+                }
+                for (; e < e1; e++) acc += o.opcoef[e] * xb[(int64_t)o.opaddr[e] * ld];
+        }
+        return acc;
+}
+
+// Beta-beta part on the transposed vector (block b stored beta-major):
+// one wavefront per (block, beta address, 64 alpha addresses).
+__global__ __launch_bounds__(NT) void k_nat_op_bb(NatView v, OpView ob, const int64_t *__restrict__ wbase,
+                                                  const double *__restrict__ ct, double *__restrict__ st)
+{
+        const int64_t w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (NT / 64) + (threadIdx.x >> 6))) +
+                          (int64_t)0;
+        const int lane = threadIdx.x & 63;
+        if (w >= wbase[v.nblock]) return;
+        const int blk = upper_index(wbase, v.nblock, w);
+        const int64_t nb = v.bnb[blk];
+        const int64_t na = (v.boff[blk + 1] - v.boff[blk]) / nb;
+        const int64_t nch = (na + 63) >> 6;
+        const int64_t local = w - wbase[blk];
+        const int64_t b = local / nch;
+        const int64_t a = (local - b * nch) * 64 + lane;
+        const bool valid = a < na;
+        const int64_t al = valid ? a : na - 1;
+        const int A = v.bsa[blk], B = v.bsb[blk];
+        const int ib = ob.dindex[B];
+        const double acc = op_row(ob, ib, ob.opbase[ib] + b * ob.oplen[ib], v, v.dblk + (int64_t)A * v.nsecB, 1, ct,
+                                  al, na);
+        if (valid) st[v.boff[blk] + b * na + a] = acc;
+}
+
+// Direct sigma in the operator form: alpha-alpha through A_alpha on c,
+// alpha-beta over the sorted link lists (alpha links of I_a wavefront-
+// uniform, row-major; beta links of I_b per lane, slot-major, eight held in
+// registers while the alpha links of a segment pass), plus the beta-beta
+// part st (transposed blocks).  One wavefront per (block, alpha address,
+// 64 beta addresses).
+__global__ __launch_bounds__(NT) void k_nat_op_direct(NatView v, OpView oa, OpView ob,
+                                                      const int64_t *__restrict__ wbase,
+                                                      const double *__restrict__ hs, const double *__restrict__ c,
+                                                      const double *__restrict__ st, double *__restrict__ sigma)
+{
+        const int64_t w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (NT / 64) + (threadIdx.x >> 6))) +
+                          (int64_t)0;
+        const int lane = threadIdx.x & 63;
+        if (w >= wbase[v.nblock]) return;
+        const int blk = upper_index(wbase, v.nblock, w);
+        const int64_t nb = v.bnb[blk];
+        const int64_t nch = (nb + 63) >> 6;
+        const int64_t local = w - wbase[blk];
+        const int64_t a = local / nch;
+        const int64_t b = (local - a * nch) * 64 + lane;
+        const bool valid = b < nb;
+        const int64_t bl = valid ? b : nb - 1;
+        const int A = v.bsa[blk], B = v.bsb[blk];
+        const int ia = oa.dindex[A], ib = ob.dindex[B];
+        const int npair = v.npair;
+        // alpha-alpha
+        double acc = op_row(oa, ia, oa.opbase[ia] + a * oa.oplen[ia], v, v.dblk + B, v.nsecB, c, bl, nb);
+        // alpha-beta
+        const uint64_t *la = oa.ls + oa.lsbase[ia] + a * oa.lslen[ia];
+        const int64_t nsB = ob.rowbase[ib + 1] - ob.rowbase[ib];
+        const uint64_t *lb = ob.ls + ob.lsbase[ib] + bl;
+        const int ka0 = oa.lssegptr[ia], ka1 = oa.lssegptr[ia + 1];
+        const int kb0 = ob.lssegptr[ib], kb1 = ob.lssegptr[ib + 1];
+        for (int ka = ka0; ka < ka1; ka++) {
+                const int32_t *dA2 = v.dblk + (int64_t)oa.lssegT[ka] * v.nsecB;
+                const int sa0 = oa.lssegoff[ka], sa1 = ka + 1 < ka1 ? oa.lssegoff[ka + 1] : oa.lslen[ia];
+                for (int kb = kb0; kb < kb1; kb++) {
+                        const int blk2 = dA2[ob.lssegT[kb]];
+                        if (blk2 < 0) continue;
+                        const int64_t nb2 = v.bnb[blk2];
+                        const double *cb = c + v.boff[blk2];
+                        const int sb0 = ob.lssegoff[kb], sb1 = kb + 1 < kb1 ? ob.lssegoff[kb + 1] : ob.lslen[ib];
+                        constexpr int U = 8;
+                        for (int j0 = sb0; j0 < sb1; j0 += U) {
+                                int64_t bj[U];
+                                int rr[U];
+                                bool ng[U];
+#pragma unroll
+                                for (int u = 0; u < U; u++) {
+                                        const uint64_t e = j0 + u < sb1 ? lb[(int64_t)(j0 + u) * nsB] : 0ull;
+                                        bj[u] = (int64_t)(uint32_t)e;
+                                        rr[u] = (int)((e >> 32) & 0xffff);
+                                        ng[u] = (e >> 63) != 0;
+                                }
+                                const int nu = sb1 - j0 < U ? sb1 - j0 : U;
+                                for (int s = sa0; s < sa1; s++) {
+                                        const uint64_t e = la[s];
+                                        const double *hrow = hs + (int64_t)((e >> 32) & 0xffff) * npair;
+                                        const double *xrow = cb + (int64_t)(uint32_t)e * nb2;
+                                        const bool na_ = (e >> 63) != 0;
+#pragma unroll
+                                        for (int u = 0; u < U; u++) {
+                                                if (u < nu) {
+                                                        const double t = hrow[rr[u]] * xrow[bj[u]];
+                                                        acc += (na_ != ng[u]) ? -t : t;
+                                                }
+                                        }
+                                }
+                        }
+                }
+        }
+        if (valid) {
+                const int64_t na = (v.boff[blk + 1] - v.boff[blk]) / nb;
+                sigma[v.boff[blk] + a * nb + b] = acc + st[v.boff[blk] + b * na + a];
+        }
+}
+
 __global__ void k_nat_fill(int32_t *__restrict__ x, int64_t n, int32_t val)
 {
         for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
@@ -646,12 +1076,38 @@ struct gas_native {
         SubInfo si{};
         NatView v{};
         std::vector<void *> dev;        // device allocations
+        // per spin: local sector occupations, strides and moves (host and
+        // device), the D sectors (local ids) of the spin
+        std::vector<uint8_t> occ[2];
+        std::vector<int64_t> stride[2];
+        std::vector<int32_t> mv[2];
+        const int64_t *d_stride[2] = {nullptr, nullptr};
+        const int32_t *d_mv[2] = {nullptr, nullptr};
+        std::vector<int32_t> dsec[2];
+        // operator form: host segment tables (op_host), device structure
+        // (op_device: 0 not built, 1 built, -1 failed)
+        bool op_host_done = false;
+        int op_dev_state = 0;
+        struct OpHost {
+                std::vector<int32_t> dindex;
+                std::vector<int64_t> rowbase, opbase, lsbase;
+                std::vector<int32_t> oplen, opsegptr, opsegT, opsegoff;
+                std::vector<int32_t> lslen, lssegptr, lssegT, lssegoff;
+                std::vector<int16_t> opsegk, lssegk;
+                int64_t nnz = 0, nls = 0;
+                int maxseg = 0;
+        } oph[2];
+        OpView op[2] = {};              // device tables (opcoef unset)
+        double op_same = 0.0, op_cross = 0.0;   // estimated wavefront iterations per sigma
+        std::vector<int32_t> bsa_loc, bsb_loc, dblk_loc;
 };
 
 struct gas_native_plan {
         const fcis_gas_space *g = nullptr;
         int precision = FCIS_PREC_FP64;
-        int direct = 0;                 // 1: k_nat_direct, 0: t1 / GEMM / pull
+        int direct = 0;                 // 0: t1 / GEMM / pull, 1: link-pair loops, 2: operator form
+        double *d_opcoef = nullptr;     // operator form: A_alpha then A_beta entries
+        OpView oa{}, ob{};
         rocblas_handle blas = nullptr;
         void *d_h = nullptr;            // npair x npair, double or float (direct: h, then h + h^T)
         void *d_t1 = nullptr, *d_vt1 = nullptr;
@@ -782,6 +1238,14 @@ int build_native(fcis_gas_space *g, gas_native *nt, const fcis::SectorPool &pool
                 bnb[b] = g->nstr[g->bsb[b]];
         }
         boff[g->nblock] = g->ndet;
+        nt->bsa_loc = bsa;
+        nt->bsb_loc = bsb;
+        nt->dblk_loc = dblk;
+        {
+                std::set<int> da(bsa.begin(), bsa.end()), db(bsb.begin(), bsb.end());
+                nt->dsec[0].assign(da.begin(), da.end());
+                nt->dsec[1].assign(db.begin(), db.end());
+        }
         // per spin: sector occupations, strides and the move table
         NatView &v = nt->v;
         v.n = n;
@@ -830,6 +1294,11 @@ int build_native(fcis_gas_space *g, gas_native *nt, const fcis::SectorPool &pool
                 NAT_TRY(dev_upload(nt, &d_occ, occ, st));
                 NAT_TRY(dev_upload(nt, &d_stride, stride, st));
                 NAT_TRY(dev_upload(nt, &d_mv, mv, st));
+                nt->occ[spin] = occ;
+                nt->stride[spin] = stride;
+                nt->mv[spin] = mv;
+                nt->d_stride[spin] = d_stride;
+                nt->d_mv[spin] = d_mv;
                 NAT_TRY(dev_alloc(nt, &d_str, ns_str));
                 NAT_TRY(dev_alloc(nt, &d_ssec, ns_str));
                 NAT_TRY(dev_alloc(nt, &d_saddr, ns_str));
@@ -921,6 +1390,243 @@ void make_tiles(const gas_native *nt, int64_t cap, std::vector<int64_t> &r0, std
         c0.push_back(nt->n1);
 }
 
+// Local sector reached from local sector S (spin `spin`) by moving one
+// electron from subspace x to subspace y (S itself for x == y), or -1.
+int host_move(const gas_native *nt, int spin, int S, int x, int y)
+{
+        if (S < 0) return -1;
+        const int ng = nt->si.ngas;
+        return x == y ? S : nt->mv[spin][((size_t)S * ng + x) * ng + y];
+}
+
+// Segment tables of the operator form for both spins (see the device
+// comment): per D sector S the number of strings of each D sector T within
+// two replacements of a string of S (operator rows) and within one
+// (link lists, diagonal slots included), counted from the occupations:
+// o[x] electrons and w[x] = norb[x] - o[x] holes in subspace x.
+void op_host(const fcis_gas_space *g, gas_native *nt)
+{
+        if (nt->op_host_done) return;
+        const int ng = nt->si.ngas;
+        for (int spin = 0; spin < 2; spin++) {
+                auto &H = nt->oph[spin];
+                const int ns = nt->nsec[spin];
+                const std::vector<int32_t> &ds = nt->dsec[spin];
+                const int nd = (int)ds.size();
+                H.dindex.assign(ns, -1);
+                for (int i = 0; i < nd; i++) H.dindex[ds[i]] = i;
+                H.rowbase.assign(nd + 1, 0);
+                H.opbase.assign(nd, 0);
+                H.lsbase.assign(nd, 0);
+                H.oplen.assign(nd, 0);
+                H.lslen.assign(nd, 0);
+                H.opsegptr.assign(nd + 1, 0);
+                H.lssegptr.assign(nd + 1, 0);
+                H.opsegk.assign((size_t)nd * ns, -1);
+                H.lssegk.assign((size_t)nd * ns, -1);
+                H.opsegT.clear();
+                H.opsegoff.clear();
+                H.lssegT.clear();
+                H.lssegoff.clear();
+                H.nnz = H.nls = 0;
+                H.maxseg = 0;
+                for (int i = 0; i < nd; i++) {
+                        const int S = ds[i];
+                        const int64_t nstr = nt->sbase[spin][S + 1] - nt->sbase[spin][S];
+                        H.rowbase[i + 1] = H.rowbase[i] + nstr;
+                        std::vector<int64_t> o(ng), w(ng);
+                        int k = 0;
+                        for (int x = 0; x < ng; x++) {
+                                o[x] = nt->occ[spin][(size_t)S * ng + x];
+                                w[x] = nt->si.norb[x] - o[x];
+                                k += (int)o[x];
+                        }
+                        std::map<int, int64_t> cop, cls;
+                        cop[S] += 1;
+                        cls[S] += k;
+                        for (int x = 0; x < ng; x++)
+                                for (int y = 0; y < ng; y++) {
+                                        const int64_t m = o[x] * w[y];
+                                        const int T = host_move(nt, spin, S, x, y);
+                                        if (m == 0 || T < 0 || H.dindex[T] < 0) continue;
+                                        cop[T] += m;
+                                        cls[T] += m;
+                                }
+                        // doubles: removed electrons from x1 <= x2, added to y1 <= y2
+                        for (int x1 = 0; x1 < ng; x1++)
+                                for (int x2 = x1; x2 < ng; x2++) {
+                                        const int64_t mo = x1 == x2 ? o[x1] * (o[x1] - 1) / 2 : o[x1] * o[x2];
+                                        if (mo == 0) continue;
+                                        for (int y1 = 0; y1 < ng; y1++)
+                                                for (int y2 = y1; y2 < ng; y2++) {
+                                                        const int64_t mw = y1 == y2 ? w[y1] * (w[y1] - 1) / 2
+                                                                                    : w[y1] * w[y2];
+                                                        if (mw == 0) continue;
+                                                        const int T = host_move(nt, spin, host_move(nt, spin, S, x1, y1),
+                                                                                x2, y2);
+                                                        if (T < 0 || H.dindex[T] < 0) continue;
+                                                        cop[T] += mo * mw;
+                                                }
+                                }
+                        H.opsegptr[i] = (int32_t)H.opsegT.size();
+                        int64_t off = 0;
+                        for (auto &e : cop) {
+                                H.opsegk[(size_t)i * ns + e.first] = (int16_t)(H.opsegT.size() - H.opsegptr[i]);
+                                H.opsegT.push_back(e.first);
+                                H.opsegoff.push_back((int32_t)off);
+                                off += e.second;
+                        }
+                        H.oplen[i] = (int32_t)off;
+                        H.opbase[i] = H.nnz;
+                        H.nnz += nstr * off;
+                        H.maxseg = std::max<int>(H.maxseg, (int)cop.size());
+                        H.lssegptr[i] = (int32_t)H.lssegT.size();
+                        off = 0;
+                        for (auto &e : cls) {
+                                H.lssegk[(size_t)i * ns + e.first] = (int16_t)(H.lssegT.size() - H.lssegptr[i]);
+                                H.lssegT.push_back(e.first);
+                                H.lssegoff.push_back((int32_t)off);
+                                off += e.second;
+                        }
+                        H.lslen[i] = (int32_t)off;
+                        H.lsbase[i] = H.nls;
+                        H.nls += nstr * off;
+                        H.maxseg = std::max<int>(H.maxseg, (int)cls.size());
+                }
+                H.opsegptr[nd] = (int32_t)H.opsegT.size();
+                H.lssegptr[nd] = (int32_t)H.lssegT.size();
+        }
+        // work estimate: wavefront iterations of the sigma kernels, same-spin
+        // entries and opposite-spin link pairs
+        const auto &HA = nt->oph[0], &HB = nt->oph[1];
+        const int nsb = nt->nsec[1];
+        double same = 0.0, cross = 0.0;
+        auto seglen = [](const std::vector<int32_t> &off, int k, int k1, int32_t len) {
+                return (double)((k + 1 < k1 ? off[k + 1] : len) - off[k]);
+        };
+        for (int b = 0; b < g->nblock; b++) {
+                const int A = nt->bsa_loc[b], B = nt->bsb_loc[b];
+                const int ia = HA.dindex[A], ib = HB.dindex[B];
+                const double na = (double)(nt->sbase[0][A + 1] - nt->sbase[0][A]);
+                const double nb = (double)(nt->sbase[1][B + 1] - nt->sbase[1][B]);
+                double aa = 0.0, ab = 0.0, bb = 0.0;
+                for (int k = HA.opsegptr[ia]; k < HA.opsegptr[ia + 1]; k++)
+                        if (nt->dblk_loc[(size_t)HA.opsegT[k] * nsb + B] >= 0)
+                                aa += seglen(HA.opsegoff, k, HA.opsegptr[ia + 1], HA.oplen[ia]);
+                for (int k = HB.opsegptr[ib]; k < HB.opsegptr[ib + 1]; k++)
+                        if (nt->dblk_loc[(size_t)A * nsb + HB.opsegT[k]] >= 0)
+                                bb += seglen(HB.opsegoff, k, HB.opsegptr[ib + 1], HB.oplen[ib]);
+                for (int ka = HA.lssegptr[ia]; ka < HA.lssegptr[ia + 1]; ka++)
+                        for (int kb = HB.lssegptr[ib]; kb < HB.lssegptr[ib + 1]; kb++)
+                                if (nt->dblk_loc[(size_t)HA.lssegT[ka] * nsb + HB.lssegT[kb]] >= 0)
+                                        ab += seglen(HA.lssegoff, ka, HA.lssegptr[ia + 1], HA.lslen[ia]) *
+                                              seglen(HB.lssegoff, kb, HB.lssegptr[ib + 1], HB.lslen[ib]);
+                const double wa = na * std::ceil(nb / 64.0), wb = nb * std::ceil(na / 64.0);
+                same += wa * aa + wb * bb;
+                cross += wa * ab;
+        }
+        nt->op_same = same;
+        nt->op_cross = cross;
+        nt->op_host_done = true;
+}
+
+// Device tables of the operator form for both spins (structure only; the
+// entries of A come with each plan).  On failure the operator form is
+// marked unusable and the status returned.
+int op_device(const fcis_gas_space *g, gas_native *nt, hipStream_t st)
+{
+        if (nt->op_dev_state == 1) return FCIS_SUCCESS;
+        if (nt->op_dev_state < 0) {
+                fcis::set_error("sector-native GAS space: operator tables unavailable");
+                return FCIS_ERR_INVALID;
+        }
+        op_host(g, nt);
+        nt->op_dev_state = -1;
+        for (int spin = 0; spin < 2; spin++)
+                if (nt->oph[spin].maxseg > OP_MAXSEG) {
+                        fcis::set_error("sector-native GAS space: %d target sectors per sector exceed %d",
+                                        nt->oph[spin].maxseg, OP_MAXSEG);
+                        return FCIS_ERR_INVALID;
+                }
+        int *d_err;
+        NAT_TRY(dev_alloc(nt, &d_err, 1));
+        FCIS_HIP(hipMemsetAsync(d_err, 0, sizeof(int), st));
+        for (int spin = 0; spin < 2; spin++) {
+                const auto &H = nt->oph[spin];
+                OpView &o = nt->op[spin];
+                o = OpView{};
+                o.ndsec = (int)nt->dsec[spin].size();
+                o.nrows = H.rowbase[o.ndsec];
+                int32_t *p32;
+                int64_t *p64;
+                int16_t *p16;
+                NAT_TRY(dev_upload(nt, &p32, nt->dsec[spin], st));
+                o.dsec = p32;
+                NAT_TRY(dev_upload(nt, &p32, H.dindex, st));
+                o.dindex = p32;
+                NAT_TRY(dev_upload(nt, &p64, H.rowbase, st));
+                o.rowbase = p64;
+                NAT_TRY(dev_upload(nt, &p64, H.opbase, st));
+                o.opbase = p64;
+                NAT_TRY(dev_upload(nt, &p32, H.oplen, st));
+                o.oplen = p32;
+                NAT_TRY(dev_upload(nt, &p32, H.opsegptr, st));
+                o.opsegptr = p32;
+                NAT_TRY(dev_upload(nt, &p32, H.opsegT, st));
+                o.opsegT = p32;
+                NAT_TRY(dev_upload(nt, &p32, H.opsegoff, st));
+                o.opsegoff = p32;
+                NAT_TRY(dev_upload(nt, &p16, H.opsegk, st));
+                o.opsegk = p16;
+                NAT_TRY(dev_upload(nt, &p64, H.lsbase, st));
+                o.lsbase = p64;
+                NAT_TRY(dev_upload(nt, &p32, H.lslen, st));
+                o.lslen = p32;
+                NAT_TRY(dev_upload(nt, &p32, H.lssegptr, st));
+                o.lssegptr = p32;
+                NAT_TRY(dev_upload(nt, &p32, H.lssegT, st));
+                o.lssegT = p32;
+                NAT_TRY(dev_upload(nt, &p32, H.lssegoff, st));
+                o.lssegoff = p32;
+                NAT_TRY(dev_upload(nt, &p16, H.lssegk, st));
+                o.lssegk = p16;
+                uint32_t *addr;
+                uint64_t *ls;
+                NAT_TRY(dev_alloc(nt, &addr, (size_t)H.nnz));
+                NAT_TRY(dev_alloc(nt, &ls, (size_t)H.nls));
+                o.opaddr = addr;
+                o.ls = ls;
+                const dim3 grid((unsigned)((o.nrows + NT / 64 - 1) / (NT / 64)));
+                hipLaunchKernelGGL(k_op_struct<true>, grid, dim3(NT), 0, st, nt->v, o, spin, 0, nt->si,
+                                   nt->d_stride[spin], nt->d_mv[spin], addr, ls, d_err);
+                FCIS_CHECK_LAUNCH();
+                hipLaunchKernelGGL(k_op_struct<false>, grid, dim3(NT), 0, st, nt->v, o, spin, spin == 0 ? 1 : 0,
+                                   nt->si, nt->d_stride[spin], nt->d_mv[spin], addr, ls, d_err);
+                FCIS_CHECK_LAUNCH();
+        }
+        int herr = 0;
+        FCIS_HIP(hipMemcpyAsync(&herr, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
+        FCIS_HIP(hipStreamSynchronize(st));
+        if (herr != 0) {
+                fcis::set_error("sector-native GAS space: operator tables inconsistent (%d segments)", herr);
+                return FCIS_ERR_DEVICE;
+        }
+        nt->op_dev_state = 1;
+        return FCIS_SUCCESS;
+}
+
+// Bytes of the operator form's device tables (structure, both spins) and
+// of a plan's entries.
+uint64_t op_struct_bytes(const gas_native *nt)
+{
+        return (uint64_t)(nt->oph[0].nnz + nt->oph[1].nnz) * 4 + (uint64_t)(nt->oph[0].nls + nt->oph[1].nls) * 8;
+}
+
+uint64_t op_coef_bytes(const gas_native *nt)
+{
+        return (uint64_t)(nt->oph[0].nnz + nt->oph[1].nnz) * 8;
+}
+
 // Column capacity of a tile for `per_col` bytes per column within `budget`.
 int64_t tile_columns(const gas_native *nt, uint64_t budget, uint64_t per_col)
 {
@@ -972,7 +1678,7 @@ void fcis::gas_native_free(gas_native *nt)
 
 int fcis::gas_native_plan_path(const gas_native_plan *p)
 {
-        return p->direct ? 3 : 2;
+        return p->direct == 2 ? 3 : p->direct == 1 ? 4 : 2;
 }
 
 int fcis::gas_native_info(const fcis_gas_space *g, int64_t *nclosure, int64_t *maxrow, int64_t *nstra,
@@ -1038,6 +1744,7 @@ void fcis::gas_native_plan_free(gas_native_plan *p)
         if (p->d_t1) (void)hipFree(p->d_t1);
         if (p->d_wbase) (void)hipFree(p->d_wbase);
         if (p->d_ct) (void)hipFree(p->d_ct);
+        if (p->d_opcoef) (void)hipFree(p->d_opcoef);
         delete p;
 }
 
@@ -1057,22 +1764,48 @@ int fcis::gas_native_plan_create(gas_native_plan **out, const fcis_gas_space *g,
         p->precision = precision;
         const size_t es = precision == FCIS_PREC_FP64 ? 8 : 4;
         const int np = nt->npair;
-        // Path: the direct form where its link-pair work per determinant,
-        // weighted by its measured rate (about 16 times slower per unit than
-        // the GEMM form's per closure column: irregular gathers against MFMA
-        // and streaming), is below the GEMM form's.  Measured crossover
-        // (FP64 sigma, ms, direct / GEMM): RAS 16 orbitals 149 / 54, 20
-        // orbitals 349 / 161, 24 orbitals 33 / 27, 40 orbitals 22 / 66 and
-        // 237 / 1131.  The direct form computes in FP64 for either
-        // precision.  FCIS_GAS_NATIVE_PATH=direct / t1 forces one.
+        if (workspace_bytes == 0) workspace_bytes = (uint64_t)8 << 30;
+        // Path, by estimated time in ns (FP64 sigma; the direct forms compute
+        // in FP64 for either precision), each model fitted to measured sigma
+        // times at 16-62 orbitals: the t1 / GEMM / pull form by its GEMM,
+        // npair^2 + 60 npair per closure column at about 28 per ns; the
+        // link-pair form by its k(n-k+1)-squared slot visits per
+        // determinant; the operator form by its wavefront iterations (one per
+        // same-spin entry and one per opposite-spin link pair), about 0.08
+        // ns each, plus launches.  The operator form needs its tables (about
+        // 12 bytes per entry and 8 per sorted link) within the workspace.
+        // Measured (ms, operator / link-pair / t1): RAS(2,8,30) (4,4) 1.5M
+        // determinants 11.3 / 142 / 1134; GAS 62 orbitals 1.35M 18.1 / 368 /
+        // 10284; RAS(4,12,4) (6,6) 7.0M 26.6 / 251 / 164.
+        // FCIS_GAS_NATIVE_PATH=direct / links / t1 forces one.
         {
+                gas_native *nm = g->nat;
+                op_host(g, nm);
                 const double la = nt->v.nlinkA, lb = nt->v.nlinkB;
-                const double direct_cost = (double)g->ndet * (la * la + lb * lb + 2.0 * la * lb) * 2.0;
-                const double t1_cost = (double)nt->n1 * ((double)np * np + 60.0 * np);
-                p->direct = 16.0 * direct_cost < t1_cost;
+                const double t1_ns = (double)nt->n1 * ((double)np * np + 60.0 * np) * 1.0e-3 / 28.0;
+                const double links_ns = (double)g->ndet * (la * la + lb * lb + 2.0 * la * lb) * 2.0 * 16.0 * 1.0e-3 / 28.0;
+                const double op_ns = 0.08 * (nm->op_same + nm->op_cross) + 5.0e4;
+                const bool op_fits = op_struct_bytes(nm) + op_coef_bytes(nm) <= workspace_bytes &&
+                                     nm->oph[0].maxseg <= OP_MAXSEG && nm->oph[1].maxseg <= OP_MAXSEG &&
+                                     nm->op_dev_state >= 0;
+                p->direct = 0;
+                double best = t1_ns;
+                if (links_ns < best) {
+                        p->direct = 1;
+                        best = links_ns;
+                }
+                if (op_fits && op_ns < best) p->direct = 2;
                 const char *env = getenv("FCIS_GAS_NATIVE_PATH");
-                if (env != nullptr && strcmp(env, "direct") == 0) p->direct = 1;
+                if (env != nullptr && strcmp(env, "direct") == 0) p->direct = 2;
+                if (env != nullptr && strcmp(env, "links") == 0) p->direct = 1;
                 if (env != nullptr && strcmp(env, "t1") == 0) p->direct = 0;
+                if (p->direct == 2) {
+                        const int r = op_device(g, nm, nullptr);
+                        if (r != FCIS_SUCCESS) {
+                                gas_native_plan_free(p);
+                                return r;
+                        }
+                }
         }
         if (p->direct) {
                 std::vector<int64_t> wb(2 * (size_t)(g->nblock + 1), 0);
@@ -1102,10 +1835,34 @@ int fcis::gas_native_plan_create(gas_native_plan **out, const fcis_gas_space *g,
                         fcis::set_error("fcis_gas_sigma_plan_create: upload failed");
                         return FCIS_ERR_DEVICE;
                 }
+                if (p->direct == 2) {
+                        const gas_native *nm = g->nat;
+                        const int64_t nza = nm->oph[0].nnz, nzb = nm->oph[1].nnz;
+                        if (fcis::dev_malloc(&p->d_opcoef, 8 * (size_t)std::max<int64_t>(nza + nzb, 1)) != hipSuccess) {
+                                gas_native_plan_free(p);
+                                fcis::set_error("fcis_gas_sigma_plan_create: device allocation of the operator "
+                                                "entries failed");
+                                return FCIS_ERR_MEMORY;
+                        }
+                        p->oa = nm->op[0];
+                        p->ob = nm->op[1];
+                        p->oa.opcoef = p->d_opcoef;
+                        p->ob.opcoef = p->d_opcoef + nza;
+                        for (int spin = 0; spin < 2; spin++) {
+                                const OpView &o = spin ? p->ob : p->oa;
+                                hipLaunchKernelGGL(k_op_coef, dim3((unsigned)((o.nrows + NT / 64 - 1) / (NT / 64))),
+                                                   dim3(NT), 0, nullptr, nm->v, o, spin,
+                                                   static_cast<const double *>(p->d_h), const_cast<double *>(o.opcoef));
+                        }
+                        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(nullptr) != hipSuccess) {
+                                gas_native_plan_free(p);
+                                fcis::set_error("fcis_gas_sigma_plan_create: operator entries failed");
+                                return FCIS_ERR_DEVICE;
+                        }
+                }
                 *out = p;
                 return FCIS_SUCCESS;
         }
-        if (workspace_bytes == 0) workspace_bytes = (uint64_t)8 << 30;
         int64_t cols = tile_columns(nt, workspace_bytes, 2 * es * (uint64_t)np);
         if (cols < nt->maxrow) {
                 gas_native_plan_free(p);
@@ -1211,6 +1968,17 @@ int fcis::gas_native_plan_execute(gas_native_plan *p, const double *d_ci0, doubl
                         const int64_t *wbt = p->d_wbase + g->nblock + 1;
                         hipLaunchKernelGGL(k_nat_transpose, dim3(blocks_for(g->ndet)), dim3(NT), 0, st, g->nat->v,
                                            d_ci0, ct);
+                        if (p->direct == 2) {
+                                hipLaunchKernelGGL(k_nat_op_bb,
+                                                   dim3((unsigned)((p->nwave_bb + NT / 64 - 1) / (NT / 64))), dim3(NT),
+                                                   0, st, g->nat->v, p->ob, wbt, ct, sbb);
+                                hipLaunchKernelGGL(k_nat_op_direct,
+                                                   dim3((unsigned)((p->nwave + NT / 64 - 1) / (NT / 64))), dim3(NT), 0,
+                                                   st, g->nat->v, p->oa, p->ob, p->d_wbase, h + np * np, d_ci0, sbb,
+                                                   d_ci1);
+                                FCIS_CHECK_LAUNCH();
+                                return FCIS_SUCCESS;
+                        }
                         hipLaunchKernelGGL(k_nat_direct_bb,
                                            dim3((unsigned)((p->nwave_bb + NT / 64 - 1) / (NT / 64))), dim3(NT), 0,
                                            st, g->nat->v, wbt, h, ct, sbb);

@@ -454,9 +454,11 @@ class GasContractPlan:
 
     @property
     def path(self):
-        """"cas-three-kernel", "cas-fused", "native-t1" or "native-direct"."""
+        """"cas-three-kernel", "cas-fused", "native-t1" (closure GEMM),
+        "native-direct" (operator form) or "native-links" (link-pair loops)."""
         k = self.lib.fcis_gas_sigma_plan_path(self._plan)
-        return {0: "cas-three-kernel", 1: "cas-fused", 2: "native-t1", 3: "native-direct"}[k]
+        return {0: "cas-three-kernel", 1: "cas-fused", 2: "native-t1", 3: "native-direct",
+                4: "native-links"}[k]
 
     def execute(self, c, out):
         """out <- P H P c for device tensors."""

@@ -399,12 +399,14 @@ def _sigma_path(monkeypatch, gas, h2e, c, path):
         return out.cpu().numpy()
 
 
+@pytest.mark.parametrize("path", ["direct", "links"])
 @pytest.mark.parametrize("name", CASES)
-def test_native_direct_path_matches_libfci_gas(monkeypatch, name):
-    """The direct link-pair kernel on the reference's fixtures."""
+def test_native_direct_path_matches_libfci_gas(monkeypatch, name, path):
+    """The direct forms (operator form, link-pair loops) on the reference's
+    fixtures."""
     d = load(name)
     with fg.GasSpace(d["gas_orbs"], d["nelec"], d["blocks"], native=True) as gas:
-        sig = _sigma_path(monkeypatch, gas, d["h2e"], d["ci"], "direct")
+        sig = _sigma_path(monkeypatch, gas, d["h2e"], d["ci"], path)
         scale = max(1.0, numpy.abs(d["sigma"]).max())
         assert numpy.abs(sig - d["sigma"]).max() < 1e-12 * scale
 
@@ -416,8 +418,8 @@ def test_native_direct_path_matches_libfci_gas(monkeypatch, name):
     ((1, 3, 28, 30), (2, 2), [[1, 2], [3, 4], [3, 4], [4, 4]], "cumulative-occ"),
 ])
 def test_native_direct_and_t1_paths_agree(monkeypatch, orbs, ne, restr, kind):
-    """Both sector-native sigma forms give the same restricted sigma (12 to
-    62 orbitals), and it matches the oracle's Slater-Condon rows."""
+    """The three sector-native sigma forms give the same restricted sigma
+    (12 to 62 orbitals), and it matches the oracle's Slater-Condon rows."""
     n = sum(orbs)
     o, D = ag.normalize_gas_spec(orbs, ne, restr, kind)
     L = og.GasLayout(o, ne, D)
@@ -427,12 +429,39 @@ def test_native_direct_and_t1_paths_agree(monkeypatch, orbs, ne, restr, kind):
         c = numpy.random.default_rng(15).normal(size=gas.ndet)
         s_t1 = _sigma_path(monkeypatch, gas, h2e, c, "t1")
         s_dir = _sigma_path(monkeypatch, gas, h2e, c, "direct")
+        s_lnk = _sigma_path(monkeypatch, gas, h2e, c, "links")
         scale = numpy.abs(s_t1).max()
         assert numpy.abs(s_dir - s_t1).max() < 1e-12 * scale
+        assert numpy.abs(s_lnk - s_t1).max() < 1e-12 * scale
         sa, sb = L.det_strings()
         rows = numpy.random.default_rng(16).integers(0, gas.ndet, 80)
         ref = oracle.det_sigma_rows(h1e, eri, c, n, sa, sb, rows)
         assert numpy.abs(s_dir[rows] - ref).max() < 1e-12 * scale
+
+
+def test_native_operator_form_reproducible_and_chosen():
+    """The operator form is destination-owned with a fixed summation order:
+    two applications agree bitwise, and a second plan on the same space
+    (tables reused, entries recomputed) gives the same bits.  It is the
+    plan's own choice at 40 orbitals."""
+    import torch
+    orbs, ne, restr = (2, 8, 30), (4, 4), dict(max_holes=1, max_particles=2)
+    n = sum(orbs)
+    o, D = ag.normalize_gas_spec(orbs, ne, restr, "ras")
+    h1e, eri = syn.make_problem(n, ne, seed=21, with_ci=False)
+    h2e = fg.absorb_h1e(h1e, eri, n, ne, 0.5)
+    with fg.GasSpace(o, ne, D, native=True) as gas:
+        x = _dev(numpy.random.default_rng(22).normal(size=gas.ndet))
+        outs = []
+        for _ in range(2):
+            with fg.GasContractPlan(gas, h2e, "fp64") as p:
+                assert p.path == "native-direct"
+                for _ in range(2):
+                    y = torch.empty_like(x)
+                    p.execute(x, y)
+                    outs.append(y)
+        for y in outs[1:]:
+            assert torch.equal(y, outs[0])
 
 
 def test_native_transform_ci_within_gas_24_orbitals():
