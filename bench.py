@@ -596,12 +596,12 @@ GAS_NATIVE_WORKLOADS = [
 
 def gas_native_sigma(pkg, sync, steps, with_cpu):
     """Sector-native restricted sigma on three spaces whose CAS is never
-    formed: RAS(4,12,4) (6,6) in 20 orbitals (6,989,020 determinants; the
-    plan's cost model picks the GEMM form: t1 over the single-excitation
-    closure, GEMM, pull), RAS(2,8,30) (4,4) in 40 orbitals (1,543,896) and a
-    4-subspace GAS in 62 orbitals (1,352,121), both in the direct link-pair
-    form.  FP64 and FP32 plans, and the reference's libfci_gas sigma on the
-    same space on the host cores."""
+    formed: RAS(4,12,4) (6,6) in 20 orbitals (6,989,020 determinants),
+    RAS(2,8,30) (4,4) in 40 orbitals (1,543,896) and a 4-subspace GAS in 62
+    orbitals (1,352,121), each in the form the plan picks (the operator form
+    on all three), plus the GEMM form (t1 over the single-excitation
+    closure) on the 20-orbital space.  FP64 and FP32 plans, and the
+    reference's libfci_gas sigma on the same space on the host cores."""
     import numpy
     import torch
     fg, ag = pkg.fci_gas, pkg.addons_gas
@@ -621,7 +621,12 @@ def gas_native_sigma(pkg, sync, steps, with_cpu):
         x = torch.randn(gas.ndet, dtype=torch.float64, device="cuda", generator=g)
         ref = None
         npair = n * (n + 1) // 2
-        for prec in ("fp64", "fp32"):
+        runs = [("fp64", None), ("fp32", None)]
+        if key == "ras20":
+            runs.append(("fp64", "t1"))
+        for prec, force in runs:
+            if force:
+                os.environ["FCIS_GAS_NATIVE_PATH"] = force
             with fg.GasContractPlan(gas, h2e, prec) as plan:
                 y = torch.empty_like(x)
                 plan.execute(x, y)
@@ -632,14 +637,15 @@ def gas_native_sigma(pkg, sync, steps, with_cpu):
                 sync()
                 dt = (time.perf_counter() - t1) / steps
                 path = plan.path
+            os.environ.pop("FCIS_GAS_NATIVE_PATH", None)
             rec = {"ms": dt * 1e3, "sigma_per_s": 1.0 / dt, "path": path}
             if path == "native-t1":
                 rec["gemm_TFLOP/s"] = 2.0 * npair * npair * n1 / dt / 1e12
-            if prec == "fp64":
+            if ref is None:
                 ref = y.clone()
             else:
                 rec["rel_err_vs_fp64"] = float((y - ref).norm() / ref.norm())
-            out[prec] = rec
+            out[prec if force is None else prec + "_" + force] = rec
             log("native GAS %s sigma %s (%s): %.2f ms" % (key, prec, path, dt * 1e3))
         if with_cpu:
             import oracle

@@ -514,9 +514,13 @@ int fcis_gas_space_create_native(fcis_gas_space_t **out, int ngas, const int *no
                                  int nb, int nblock, const int *block_occ, void *stream);
 /* Path of a restricted sigma plan: 0 / 1 the CAS plan's three-kernel /
  * fused path (CAS-embedded space); on a sector-native space 2 = t1 over
- * the single-excitation closure, GEMM and pull, 3 = the direct link-pair
- * kernel (chosen by the estimated work, FP64 only; the environment
- * variable FCIS_GAS_NATIVE_PATH=t1 / direct at plan creation forces one). */
+ * the single-excitation closure, GEMM and pull, 3 = the operator form
+ * (same-spin one-spin operators with their two-replacement entries,
+ * opposite-spin sorted link lists; tables kept with the space, entries
+ * with the plan), 4 = the link-pair loops without tables.  Chosen by
+ * estimated time (3 and 4 compute in FP64 for either precision); the
+ * environment variable FCIS_GAS_NATIVE_PATH=t1 / direct / links at plan
+ * creation forces one. */
 int fcis_gas_sigma_plan_path(const fcis_gas_sigma_plan_t *plan);
 /* Diagonal of S^2 on the determinants of a sector-native space (device
  * d_out, ndet doubles; _GasSpinPlan.diagonal_vector, fci_gas.py:374-398). */

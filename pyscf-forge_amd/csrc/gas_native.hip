@@ -20,7 +20,11 @@
 //   sigma[I]   = sum over the links (pq, K, s) of I's strings: s vt1[pq, K]
 //                                                                (pull)
 // Every sigma element is produced by one thread (destination-owned, no
-// atomics, bitwise reproducible).  S1 is laid out row-major over alpha
+// atomics, bitwise reproducible).  Two direct forms avoid the closure where
+// npair^2 per closure column would far exceed the excitations (many
+// orbitals, few electrons): the operator form (one-spin operator matrices
+// for the same-spin parts, sorted link lists for the opposite-spin part;
+// described with its kernels below) and the link-pair loops.  S1 is laid out row-major over alpha
 // strings: the strings of the alpha sectors of S1 in sector order are the
 // rows, a row holds the strings of its sector's S1 beta sectors one after
 // the other, and t1 / vt1 are tiled over whole rows to fit the workspace.

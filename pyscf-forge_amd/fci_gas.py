@@ -10,8 +10,9 @@ projects back (fci2gas); the diagonal and the RDMs come from the CAS kernels
 the same way.  Up to 16 orbitals the fused CAS kernel runs only over the
 determinants within one excitation of the space.  Above that (and on
 request, ``native=True``) the space is sector-native: the sigma, diagonal,
-RDMs and H blocks run on the GAS determinants and their single-excitation
-closure with no CAS array, up to the reference's 63 orbitals
+RDMs and H blocks run on the GAS determinants (the sigma through one-spin
+operator matrices and sorted link lists, or the single-excitation closure)
+with no CAS array, up to the reference's 63 orbitals
 (pyscf/lib/mcscf/fci_gas.h:24-27).
 """
 
