@@ -749,27 +749,38 @@ static int csf_gemms(fcis_csf_t *c, bool to_csf, const double *d_csf_in, double 
                 return FCIS_ERR_DEVICE;
         }
         const double one = 1.0, zero = 0.0;
-        for (int b = 0; b < c->t.nblk; b++) {
-                const CsfBlock &B = c->t.b[b];
-                if (B.ncsf == 0 || B.nconf == 0) continue;
-                const double *U = c->d_U + B.uoff;
-                double *D = c->d_work + B.off;
-                rocblas_status r;
-                // column-major views: U is (ncsf x ndet), D (ndet x nconf) and the
-                // CSFs (ncsf x nconf); det2csf: CSF = U D, csf2det: D = U^T CSF
-                if (to_csf)
-                        r = rocblas_dgemm_strided_batched(c->blas, rocblas_operation_none, rocblas_operation_none,
-                                                          B.ncsf, (rocblas_int)B.nconf, B.ndet, &one, U, B.ncsf, 0,
-                                                          D, B.ndet, c->ndet, &zero, d_csf_out + B.coff, B.ncsf,
-                                                          c->ncsf, nvec);
-                else
-                        r = rocblas_dgemm_strided_batched(c->blas, rocblas_operation_transpose,
-                                                          rocblas_operation_none, B.ndet, (rocblas_int)B.nconf, B.ncsf,
-                                                          &one, U, B.ncsf, 0, d_csf_in + B.coff, B.ncsf, c->ncsf,
-                                                          &zero, D, B.ndet, c->ndet, nvec);
-                if (r != rocblas_status_success) {
-                        fcis::set_error("CSF transform: rocblas_dgemm_strided_batched failed (status %d)", (int)r);
-                        return FCIS_ERR_DEVICE;
+        // vectors per call: every operand within 2^32 elements of its base
+        // pointer (rocBLAS GEMMs whose operands spanned more returned wrong
+        // results without an error)
+        const int64_t span = std::max<int64_t>(std::max<int64_t>(c->ndet, c->ncsf), 1);
+        const int vchunk = (int)std::max<int64_t>(1, std::min<int64_t>(nvec, (int64_t)0xffffffffLL / span));
+        for (int v0 = 0; v0 < nvec; v0 += vchunk) {
+                const int nv = std::min(vchunk, nvec - v0);
+                double *work = c->d_work + (size_t)v0 * c->ndet;
+                for (int b = 0; b < c->t.nblk; b++) {
+                        const CsfBlock &B = c->t.b[b];
+                        if (B.ncsf == 0 || B.nconf == 0) continue;
+                        const double *U = c->d_U + B.uoff;
+                        double *D = work + B.off;
+                        rocblas_status r;
+                        // column-major views: U is (ncsf x ndet), D (ndet x nconf) and
+                        // the CSFs (ncsf x nconf); det2csf: CSF = U D, csf2det: D = U^T CSF
+                        if (to_csf)
+                                r = rocblas_dgemm_strided_batched(
+                                        c->blas, rocblas_operation_none, rocblas_operation_none, B.ncsf,
+                                        (rocblas_int)B.nconf, B.ndet, &one, U, B.ncsf, 0, D, B.ndet, c->ndet, &zero,
+                                        d_csf_out + (size_t)v0 * c->ncsf + B.coff, B.ncsf, c->ncsf, nv);
+                        else
+                                r = rocblas_dgemm_strided_batched(
+                                        c->blas, rocblas_operation_transpose, rocblas_operation_none, B.ndet,
+                                        (rocblas_int)B.nconf, B.ncsf, &one, U, B.ncsf, 0,
+                                        d_csf_in + (size_t)v0 * c->ncsf + B.coff, B.ncsf, c->ncsf, &zero, D, B.ndet,
+                                        c->ndet, nv);
+                        if (r != rocblas_status_success) {
+                                fcis::set_error("CSF transform: rocblas_dgemm_strided_batched failed (status %d)",
+                                                (int)r);
+                                return FCIS_ERR_DEVICE;
+                        }
                 }
         }
         return FCIS_SUCCESS;

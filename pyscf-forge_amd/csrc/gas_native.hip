@@ -62,6 +62,11 @@
 namespace {
 
 constexpr int NT = 256;
+// Elements a rocBLAS GEMM operand may span from its base pointer
+// ((columns - 1) * ld + rows): DGEMMs of npair 136 returned wrong results,
+// without an error, for ld = 47.4M and 63.2M (spans 6.4e9, 8.5e9) and were
+// exact for ld = 31.6M (span 4.29e9), so t1 tiles keep below 2^32.
+constexpr uint64_t GEMM_MAX_SPAN = 0xffffffffull;
 
 __constant__ fcis::BinomTable c_binom = fcis::make_binom_table();
 
@@ -1875,6 +1880,14 @@ int fcis::gas_native_plan_create(gas_native_plan **out, const fcis_gas_space *g,
                                 (unsigned long long)workspace_bytes, (long long)cols,
                                 (long long)nt->maxrow);
                 return FCIS_ERR_MEMORY;
+        }
+        cols = std::min<int64_t>(cols, (int64_t)(GEMM_MAX_SPAN / (uint64_t)np));
+        if (cols < nt->maxrow) {
+                gas_native_plan_free(p);
+                fcis::set_error("fcis_gas_sigma_plan_create: one closure row (%lld columns) exceeds the "
+                                "GEMM tile limit of %lld columns", (long long)nt->maxrow,
+                                (long long)(GEMM_MAX_SPAN / (uint64_t)np));
+                return FCIS_ERR_INVALID;
         }
         cols = std::max<int64_t>(cols, 1);
         p->ld = cols;

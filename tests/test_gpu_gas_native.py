@@ -439,6 +439,35 @@ def test_native_direct_and_t1_paths_agree(monkeypatch, orbs, ne, restr, kind):
         assert numpy.abs(s_dir[rows] - ref).max() < 1e-12 * scale
 
 
+def test_native_t1_tiles_within_gemm_span_105m_determinants(monkeypatch):
+    """GAS (8,8) (8,8), at least 4 electrons in the first 8 orbitals:
+    105,164,282 determinants, closure 141M columns.  A 96 GB workspace would
+    give t1 tiles whose GEMM operands span 6.4e9 elements, where rocBLAS
+    returned wrong sigma without an error; the plan keeps tiles below 2^32
+    elements and the t1 form agrees with the operator form."""
+    import torch
+    orbs, ne, restr = (8, 8), (8, 8), [[4, 8], [16, 16]]
+    n = sum(orbs)
+    o, D = ag.normalize_gas_spec(orbs, ne, restr, "cumulative-occ")
+    h1e, eri = syn.make_problem(n, ne, seed=0, with_ci=False)
+    h2e = fg.absorb_h1e(h1e, eri, n, ne, 0.5)
+    with fg.GasSpace(o, ne, D, native=True) as gas:
+        assert gas.ndet == 105164282
+        g = torch.Generator(device="cuda").manual_seed(1)
+        x = torch.randn(gas.ndet, dtype=torch.float64, device="cuda", generator=g)
+        monkeypatch.setenv("FCIS_GAS_NATIVE_PATH", "direct")
+        with fg.GasContractPlan(gas, h2e, "fp64") as p:
+            ref = torch.empty_like(x)
+            p.execute(x, ref)
+        monkeypatch.setenv("FCIS_GAS_NATIVE_PATH", "t1")
+        with fg.GasContractPlan(gas, h2e, "fp64", 96 << 30) as p:
+            assert p.path == "native-t1"
+            y = torch.empty_like(x)
+            p.execute(x, y)
+        scale = float(ref.abs().max())
+        assert float((y - ref).abs().max()) < 1e-12 * scale
+
+
 def test_native_operator_form_reproducible_and_chosen():
     """The operator form is destination-owned with a fixed summation order:
     two applications agree bitwise, and a second plan on the same space
