@@ -47,6 +47,16 @@ const char *fcis_last_error(void);
 /* Library / device version info: writes "gfx950 ..." into buf. */
 int fcis_device_info(char *buf, int buflen);
 
+/* Plan workspaces of 256 MiB and more stay allocated after their plan is
+ * freed, for the next plan of about the same size on that device (at most
+ * four blocks per process; freeing and re-allocating blocks of tens of GB
+ * costs up to about a second in the driver).  A failed allocation releases
+ * them and retries.  fcis_kept_memory: bytes kept on `device` (-1: all);
+ * fcis_release_kept_memory frees them.  No counterpart in the reference
+ * (memory management of this library). */
+int fcis_kept_memory(int device, uint64_t *bytes);
+int fcis_release_kept_memory(int device);
+
 /*
  * CAS string space (norb, na, nb): alpha/beta string addressing and link
  * tables, built ON THE DEVICE.  Replaces the string space and link-table

@@ -28,6 +28,8 @@ _u64p = ctypes.POINTER(ctypes.c_uint64)
 SIGNATURES = {
     "fcis_last_error": (ctypes.c_char_p, []),
     "fcis_device_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
+    "fcis_kept_memory": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]),
+    "fcis_release_kept_memory": (ctypes.c_int, [ctypes.c_int]),
     "fcis_space_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int,
                                          ctypes.c_int, ctypes.c_int, _vp]),
     "fcis_space_free": (None, [_vp]),

@@ -45,6 +45,24 @@ static inline hipError_t dev_malloc(T **p, size_t bytes)
         return e;
 }
 
+// Plan workspaces (devmem.hip): blocks of 256 MiB and more freed through
+// big_free stay allocated for the next big_alloc of about their size on the
+// same device (at most four per process); a failed allocation releases that
+// device's kept blocks and retries.  big_release(-1) frees every kept block.
+hipError_t big_alloc(void **p, size_t bytes);
+void big_free(void *p);
+size_t big_kept_bytes(int device);
+void big_release(int device);
+
+template <typename T>
+static inline hipError_t big_malloc(T **p, size_t bytes)
+{
+        void *q = nullptr;
+        const hipError_t e = big_alloc(&q, bytes);
+        *p = static_cast<T *>(q);
+        return e;
+}
+
 static inline int64_t binom_host(int n, int k)
 {
         if (k < 0 || k > n) return 0;

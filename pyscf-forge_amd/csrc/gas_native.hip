@@ -1750,7 +1750,7 @@ void fcis::gas_native_plan_free(gas_native_plan *p)
         if (p == nullptr) return;
         if (p->blas) rocblas_destroy_handle(p->blas);
         if (p->d_h) (void)hipFree(p->d_h);
-        if (p->d_t1) (void)hipFree(p->d_t1);
+        if (p->d_t1) fcis::big_free(p->d_t1);
         if (p->d_wbase) (void)hipFree(p->d_wbase);
         if (p->d_ct) (void)hipFree(p->d_ct);
         if (p->d_opcoef) (void)hipFree(p->d_opcoef);
@@ -1892,7 +1892,7 @@ int fcis::gas_native_plan_create(gas_native_plan **out, const fcis_gas_space *g,
         cols = std::max<int64_t>(cols, 1);
         p->ld = cols;
         make_tiles(nt, cols, p->r0, p->c0);
-        if (fcis::dev_malloc(&p->d_t1, 2 * es * (size_t)np * cols) != hipSuccess ||
+        if (fcis::big_malloc(&p->d_t1, 2 * es * (size_t)np * cols) != hipSuccess ||
             fcis::dev_malloc(&p->d_h, es * (size_t)np * np) != hipSuccess) {
                 gas_native_plan_free(p);
                 fcis::set_error("fcis_gas_sigma_plan_create: device allocation failed");

@@ -1326,15 +1326,15 @@ int fcis::sigma_plan_create(fcis_sigma_plan **out, const fcis_space *space, cons
             fcis::dev_malloc(&p->d_pmapa, pa ? pa : 4) != hipSuccess ||
             fcis::dev_malloc(&p->d_pmapb, pb ? pb : 4) != hipSuccess ||
             fcis::dev_malloc(&p->d_pmapb16, p->fused ? (size_t)p->npair * p->nbp * 2 + 16 : 16) != hipSuccess ||
-            fcis::dev_malloc(&p->d_t1, (buf && !p->fused) ? buf : 8) != hipSuccess ||
-            fcis::dev_malloc(&p->d_vt1, buf ? buf : 8) != hipSuccess ||
+            fcis::big_malloc(&p->d_t1, (buf && !p->fused) ? buf : 8) != hipSuccess ||
+            fcis::big_malloc(&p->d_vt1, buf ? buf : 8) != hipSuccess ||
             (p->fused && (fcis::dev_malloc(&p->d_hfrag_f, hfbytes) != hipSuccess ||
                           fcis::dev_malloc(&p->d_hfrag_f_lo, precision == FCIS_PREC_BF16X3 ? hfbytes : 16) !=
                                   hipSuccess ||
                           fcis::dev_malloc(&p->d_mpair, 2 * 16 * FUSED_MTILES) != hipSuccess ||
                           fcis::dev_malloc(&p->d_falpha, 8 * (size_t)space->nstra * p->npair) != hipSuccess ||
                           fcis::dev_malloc(&p->d_fbeta, elem * (size_t)p->npair * p->nbp) != hipSuccess ||
-                          fcis::dev_malloc(&p->d_cg, precision == FCIS_PREC_FP64
+                          fcis::big_malloc(&p->d_cg, precision == FCIS_PREC_FP64
                                                              ? 16
                                                              : 4 * (size_t)space->nstra * nb) != hipSuccess ||
                           fcis::dev_malloc(&p->d_dtile, fused_dtile_bytes()) != hipSuccess))) {
@@ -1539,15 +1539,15 @@ void fcis_sigma_plan_free(fcis_sigma_plan_t *p)
         (void)hipFree(p->d_pmapa);
         (void)hipFree(p->d_pmapb);
         (void)hipFree(p->d_pmapb16);
-        (void)hipFree(p->d_t1);
-        (void)hipFree(p->d_vt1);
+        fcis::big_free(p->d_t1);
+        fcis::big_free(p->d_vt1);
         (void)hipFree(p->d_order);
         (void)hipFree(p->d_hfrag_f);
         (void)hipFree(p->d_hfrag_f_lo);
         (void)hipFree(p->d_mpair);
         (void)hipFree(p->d_falpha);
         (void)hipFree(p->d_fbeta);
-        (void)hipFree(p->d_cg);
+        fcis::big_free(p->d_cg);
         (void)hipFree(p->d_dtile);
         delete p;
 }

@@ -234,10 +234,20 @@ def _to_device(x, ndet):
     return t, host
 
 
-def _default_workspace():
+def available_bytes():
+    """Free device memory plus the plan workspaces the library keeps for
+    reuse (fcis_kept_memory), which a new plan takes over."""
+    import ctypes
     import torch
     free, _ = torch.cuda.mem_get_info()
-    return int(free * 0.45)
+    kept = ctypes.c_uint64()
+    _lib.check(_lib.load().fcis_kept_memory(torch.cuda.current_device(), ctypes.byref(kept)),
+               "fcis_kept_memory")
+    return int(free) + int(kept.value)
+
+
+def _default_workspace():
+    return int(available_bytes() * 0.45)
 
 
 # Plans of the module-level contract_2e, reused across calls with the same
@@ -253,9 +263,11 @@ _PLANS = {}            # key -> SigmaPlan, dict order = recency
 
 
 def release_plans():
-    """Free the plans cached by contract_2e."""
+    """Free the plans cached by contract_2e and the plan workspaces the
+    library keeps for reuse."""
     while _PLANS:
         _PLANS.pop(next(iter(_PLANS))).close()
+    _lib.check(_lib.load().fcis_release_kept_memory(-1), "fcis_release_kept_memory")
 
 
 def _cached_plan(sp, eri, precision, workspace_bytes):

@@ -86,7 +86,8 @@ def solve(solver, sp, h2e, hdiag, nroots, ci0, precision, tol=1e-10, tol_residua
         if len(guesses) < nroots:
             guesses += solver._init_guess_dev(hdiag, nroots)[len(guesses):]
 
-    free = torch.cuda.mem_get_info()[0]
+    from .direct_spin1 import available_bytes
+    free = available_bytes()
     need = _davidson_bytes(n, nroots, max_space)
     if basis is not None:
         # determinant-space buffers of each plan's basis operator

@@ -280,3 +280,38 @@ def test_sigma_16_orbitals_bf16x3_fused_vs_fp64_three_kernel(monkeypatch):
             rel = float(torch.linalg.vector_norm(out - ref)) / nref
             print("bf16x3 fused (%d tiles) vs FP64: %.3e" % (plan.ntiles, rel))
             assert 1e-12 < rel < 2e-5
+
+
+def test_plan_workspace_kept_and_reused():
+    """A freed plan's workspace (256 MiB and more) stays allocated, the next
+    plan of that size takes it over and gives the same sigma, and
+    release_plans frees what is kept."""
+    import ctypes
+    import torch
+    lib = pkg._lib.load()
+    dev = torch.cuda.current_device()
+
+    def kept():
+        b = ctypes.c_uint64()
+        assert lib.fcis_kept_memory(dev, ctypes.byref(b)) == 0
+        return b.value
+
+    n, ne = 14, (7, 7)
+    h1e, eri = pkg.synthetic.make_problem(n, ne, seed=3, with_ci=False)
+    h2e = pkg.absorb_h1e(h1e, eri, n, ne, 0.5)
+    sp = pkg.get_space(n, ne)
+    c = torch.randn(sp.ndet, dtype=torch.float64, device="cuda",
+                    generator=torch.Generator(device="cuda").manual_seed(4))
+    pkg.direct_spin1.release_plans()
+    assert kept() == 0
+    ys = []
+    for _ in range(2):
+        with pkg.direct_spin1.SigmaPlan(sp, h2e, "fp64", 8 << 30) as p:
+            assert kept() == 0
+            y = torch.empty_like(c)
+            p(c, y)
+            ys.append(y)
+        assert kept() >= 1 << 28
+    assert torch.equal(ys[0], ys[1]) or float((ys[0] - ys[1]).abs().max()) < 1e-13 * float(ys[0].abs().max())
+    pkg.direct_spin1.release_plans()
+    assert kept() == 0
