@@ -136,6 +136,11 @@ struct fcis_space {
         uint64_t *h_stra, *h_strb;
         uint32_t *h_perma, *h_permb;
         uint32_t *d_perma, *d_permb;
+        // processing order of the alpha strings for fused plans with
+        // XCD blocks of order_block strings (host, malloc; built by the
+        // first such plan, reused by later ones)
+        int32_t *h_order;
+        int order_block;
 };
 
 namespace fcis {

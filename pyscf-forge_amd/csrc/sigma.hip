@@ -23,6 +23,7 @@
 // too, fci_contract_gas.c:55-71).
 #include <stdlib.h>
 #include <string.h>
+#include <mutex>
 #include <algorithm>
 #include <string>
 #include <vector>
@@ -1057,7 +1058,22 @@ int build_row_order(fcis_sigma_plan *p)
         const int per = grid / nx;
         std::vector<int32_t> order;
         if (mode == "greedy") {
-                order = greedy_blocks(p->space, per);
+                // depends on the space and the block size only: kept on the space
+                static std::mutex order_mu;
+                std::lock_guard<std::mutex> guard(order_mu);
+                fcis_space *sm = const_cast<fcis_space *>(p->space);
+                if (sm->h_order != nullptr && sm->order_block == per) {
+                        order.assign(sm->h_order, sm->h_order + na);
+                } else {
+                        order = greedy_blocks(p->space, per);
+                        int32_t *keep = (int32_t *)malloc(4 * (size_t)na);
+                        if (keep != nullptr) {
+                                memcpy(keep, order.data(), 4 * (size_t)na);
+                                free(sm->h_order);
+                                sm->h_order = keep;
+                                sm->order_block = per;
+                        }
+                }
         } else {
                 order.resize(na);
                 for (int64_t a = 0; a < na; a++) order[a] = (int32_t)a;

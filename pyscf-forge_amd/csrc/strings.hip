@@ -293,6 +293,7 @@ void fcis_space_free(fcis_space_t *s)
         free(s->h_strb);
         free(s->h_perma);
         free(s->h_permb);
+        free(s->h_order);
         free(s);
 }
 
