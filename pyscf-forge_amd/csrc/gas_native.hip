@@ -956,8 +956,44 @@ __device__ __forceinline__ double op_row(const OpView &o, int i, int64_t row0, c
         return acc;
 }
 
+// Wavefront geometry of the operator-form kernels on a block with `nr`
+// rows (the wavefront-uniform string) and `ncol` columns (the lane string):
+// ncol > 32: one row and 64 columns per wavefront; ncol <= 32: g = 64 / ncol
+// rows of all columns per wavefront (rows of one sector share their segment
+// structure, so the loops stay uniform; only the row's tables differ per
+// lane).  Wavefronts per block: op_waves.
+__host__ __device__ __forceinline__ int64_t op_waves(int64_t nr, int64_t ncol)
+{
+        return ncol > 32 ? nr * ((ncol + 63) / 64) : (nr + 64 / ncol - 1) / (64 / ncol);
+}
+
+struct OpLane {
+        int64_t row, col;     // clamped into the block
+        bool valid;
+};
+
+__device__ __forceinline__ OpLane op_lane(int64_t local, int lane, int64_t nr, int64_t ncol)
+{
+        OpLane q;
+        if (ncol > 32) {
+                const int64_t nch = (ncol + 63) >> 6;
+                q.row = local / nch;
+                const int64_t b = (local - q.row * nch) * 64 + lane;
+                q.valid = b < ncol;
+                q.col = q.valid ? b : ncol - 1;
+        } else {
+                const int g = 64 / (int)ncol;
+                const int r = lane / (int)ncol;
+                const int64_t a = local * g + r;
+                q.valid = r < g && a < nr;
+                q.row = q.valid ? a : nr - 1;
+                q.col = lane - r * ncol < ncol ? lane - r * ncol : ncol - 1;
+        }
+        return q;
+}
+
 // Beta-beta part on the transposed vector (block b stored beta-major):
-// one wavefront per (block, beta address, 64 alpha addresses).
+// wavefronts over (block, beta addresses, alpha addresses) as op_lane.
 __global__ __launch_bounds__(NT) void k_nat_op_bb(NatView v, OpView ob, const int64_t *__restrict__ wbase,
                                                   const double *__restrict__ ct, double *__restrict__ st)
 {
@@ -968,42 +1004,21 @@ __global__ __launch_bounds__(NT) void k_nat_op_bb(NatView v, OpView ob, const in
         const int blk = upper_index(wbase, v.nblock, w);
         const int64_t nb = v.bnb[blk];
         const int64_t na = (v.boff[blk + 1] - v.boff[blk]) / nb;
-        const int64_t nch = (na + 63) >> 6;
-        const int64_t local = w - wbase[blk];
-        const int64_t b = local / nch;
-        const int64_t a = (local - b * nch) * 64 + lane;
-        const bool valid = a < na;
-        const int64_t al = valid ? a : na - 1;
+        const OpLane q = op_lane(w - wbase[blk], lane, nb, na);
         const int A = v.bsa[blk], B = v.bsb[blk];
         const int ib = ob.dindex[B];
-        const double acc = op_row(ob, ib, ob.opbase[ib] + b * ob.oplen[ib], v, v.dblk + (int64_t)A * v.nsecB, 1, ct,
-                                  al, na);
-        if (valid) st[v.boff[blk] + b * na + a] = acc;
+        const double acc = op_row(ob, ib, ob.opbase[ib] + q.row * ob.oplen[ib], v, v.dblk + (int64_t)A * v.nsecB, 1,
+                                  ct, q.col, na);
+        if (q.valid) st[v.boff[blk] + q.row * na + q.col] = acc;
 }
 
-// Direct sigma in the operator form: alpha-alpha through A_alpha on c,
-// alpha-beta over the sorted link lists (alpha links of I_a wavefront-
-// uniform, row-major; beta links of I_b per lane, slot-major, eight held in
-// registers while the alpha links of a segment pass), plus the beta-beta
-// part st (transposed blocks).  One wavefront per (block, alpha address,
-// 64 beta addresses).
-__global__ __launch_bounds__(NT) void k_nat_op_direct(NatView v, OpView oa, OpView ob,
-                                                      const int64_t *__restrict__ wbase,
-                                                      const double *__restrict__ hs, const double *__restrict__ c,
-                                                      const double *__restrict__ st, double *__restrict__ sigma)
+// alpha-alpha and alpha-beta parts of the operator form at alpha address a
+// (wavefront-uniform when each wavefront holds one alpha string) and beta
+// address bl of block blk.
+__device__ __forceinline__ double op_direct_at(const NatView &v, const OpView &oa, const OpView &ob, int blk,
+                                               int64_t a, int64_t bl, int64_t nb, const double *__restrict__ hs,
+                                               const double *__restrict__ c)
 {
-        const int64_t w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (NT / 64) + (threadIdx.x >> 6))) +
-                          (int64_t)0;
-        const int lane = threadIdx.x & 63;
-        if (w >= wbase[v.nblock]) return;
-        const int blk = upper_index(wbase, v.nblock, w);
-        const int64_t nb = v.bnb[blk];
-        const int64_t nch = (nb + 63) >> 6;
-        const int64_t local = w - wbase[blk];
-        const int64_t a = local / nch;
-        const int64_t b = (local - a * nch) * 64 + lane;
-        const bool valid = b < nb;
-        const int64_t bl = valid ? b : nb - 1;
         const int A = v.bsa[blk], B = v.bsb[blk];
         const int ia = oa.dindex[A], ib = ob.dindex[B];
         const int npair = v.npair;
@@ -1053,10 +1068,39 @@ __global__ __launch_bounds__(NT) void k_nat_op_direct(NatView v, OpView oa, OpVi
                         }
                 }
         }
-        if (valid) {
-                const int64_t na = (v.boff[blk + 1] - v.boff[blk]) / nb;
-                sigma[v.boff[blk] + a * nb + b] = acc + st[v.boff[blk] + b * na + a];
+        return acc;
+}
+
+// Direct sigma in the operator form: alpha-alpha through A_alpha on c,
+// alpha-beta over the sorted link lists (alpha links of I_a row-major,
+// beta links of I_b per lane, slot-major, eight held in registers while the
+// alpha links of a segment pass), plus the beta-beta part st (transposed
+// blocks).  Wavefronts over (block, alpha addresses, beta addresses) as
+// op_lane: one alpha string per wavefront (uniform loads) where the block
+// has 64 beta strings or more, several otherwise.
+__global__ __launch_bounds__(NT) void k_nat_op_direct(NatView v, OpView oa, OpView ob,
+                                                      const int64_t *__restrict__ wbase,
+                                                      const double *__restrict__ hs, const double *__restrict__ c,
+                                                      const double *__restrict__ st, double *__restrict__ sigma)
+{
+        const int64_t w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (NT / 64) + (threadIdx.x >> 6))) +
+                          (int64_t)0;
+        const int lane = threadIdx.x & 63;
+        if (w >= wbase[v.nblock]) return;
+        const int blk = upper_index(wbase, v.nblock, w);
+        const int64_t nb = v.bnb[blk];
+        const int64_t na = (v.boff[blk + 1] - v.boff[blk]) / nb;
+        const int64_t local = w - wbase[blk];
+        const OpLane q = op_lane(local, lane, na, nb);
+        double acc;
+        if (nb > 32) {
+                // one alpha string per wavefront: its address is uniform
+                const int64_t a = __builtin_amdgcn_readfirstlane((int)q.row) + (int64_t)0;
+                acc = op_direct_at(v, oa, ob, blk, a, q.col, nb, hs, c);
+        } else {
+                acc = op_direct_at(v, oa, ob, blk, q.row, q.col, nb, hs, c);
         }
+        if (q.valid) sigma[v.boff[blk] + q.row * nb + q.col] = acc + st[v.boff[blk] + q.col * na + q.row];
 }
 
 __global__ void k_nat_fill(int32_t *__restrict__ x, int64_t n, int32_t val)
@@ -1820,8 +1864,10 @@ int fcis::gas_native_plan_create(gas_native_plan **out, const fcis_gas_space *g,
                 std::vector<int64_t> wb(2 * (size_t)(g->nblock + 1), 0);
                 for (int b = 0; b < g->nblock; b++) {
                         const int64_t nas = g->nstr[g->bsa[b]], nbs = g->nstr[g->bsb[b]];
-                        wb[b + 1] = wb[b] + nas * ((nbs + 63) / 64);
-                        wb[g->nblock + 1 + b + 1] = wb[g->nblock + 1 + b] + nbs * ((nas + 63) / 64);
+                        // operator form: several strings per wavefront on narrow blocks
+                        wb[b + 1] = wb[b] + (p->direct == 2 ? op_waves(nas, nbs) : nas * ((nbs + 63) / 64));
+                        wb[g->nblock + 1 + b + 1] = wb[g->nblock + 1 + b] +
+                                                    (p->direct == 2 ? op_waves(nbs, nas) : nbs * ((nas + 63) / 64));
                 }
                 p->nwave = wb[g->nblock];
                 p->nwave_bb = wb[2 * (size_t)g->nblock + 1];
