@@ -505,8 +505,9 @@ def gas_solve(pkg, sync, h1e, eri, steps, with_cpu, max_cycle, davidson=True):
     electrons, at most 2 holes in RAS1 and 2 electrons in RAS3 (4,870,404
     determinants, 2.9% of the CAS).  FP64 / FP32 / bf16x3 restricted sigma
     (fused kernel over the alpha rows and beta chunks within one excitation
-    of the space), the reference's libfci_gas sigma on the same space on the
-    host cores, and the FP64 and bf16x3-mixed Davidson ground states."""
+    of the space), the same space sector-native (operator form, FP64), the
+    reference's libfci_gas sigma on the same space on the host cores, and
+    the FP64 and bf16x3-mixed Davidson ground states."""
     import numpy
     import torch
     fg = pkg.fci_gas
@@ -541,6 +542,22 @@ def gas_solve(pkg, sync, h1e, eri, steps, with_cpu, max_cycle, davidson=True):
             rec["rel_err_vs_fp64"] = float((y - ref).norm() / ref.norm())
         out[prec] = rec
         log("GAS sigma %s: %.2f ms" % (prec, dt * 1e3))
+    # the same space sector-native (the operator form, FP64), beside the
+    # CAS-embedded kernels above
+    with fg.GasSpace(orbs, ne, gas.blocks, native=True) as gn:
+        with fg.GasContractPlan(gn, h2e, "fp64") as plan:
+            y = torch.empty_like(x)
+            plan.execute(x, y)
+            sync()
+            t1 = time.perf_counter()
+            for _ in range(steps):
+                plan.execute(x, y)
+            sync()
+            dt = (time.perf_counter() - t1) / steps
+            out["fp64_sector_native"] = {"ms": dt * 1e3, "sigma_per_s": 1.0 / dt, "path": plan.path,
+                                         "max_rel_diff_vs_cas_embedded":
+                                             float((y - ref).abs().max() / ref.abs().max())}
+        log("GAS sigma fp64 sector-native (%s): %.2f ms" % (plan.path, dt * 1e3))
     if with_cpu:
         import oracle
         if oracle.ref_available():
