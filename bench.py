@@ -554,10 +554,11 @@ def gas_solve(pkg, sync, h1e, eri, steps, with_cpu, max_cycle, davidson=True):
                 plan.execute(x, y)
             sync()
             dt = (time.perf_counter() - t1) / steps
-            out["fp64_sector_native"] = {"ms": dt * 1e3, "sigma_per_s": 1.0 / dt, "path": plan.path,
+            path = plan.path
+            out["fp64_sector_native"] = {"ms": dt * 1e3, "sigma_per_s": 1.0 / dt, "path": path,
                                          "max_rel_diff_vs_cas_embedded":
                                              float((y - ref).abs().max() / ref.abs().max())}
-        log("GAS sigma fp64 sector-native (%s): %.2f ms" % (plan.path, dt * 1e3))
+        log("GAS sigma fp64 sector-native (%s): %.2f ms" % (path, dt * 1e3))
     if with_cpu:
         import oracle
         if oracle.ref_available():

@@ -962,9 +962,9 @@ __device__ __forceinline__ double op_row(const OpView &o, int i, int64_t row0, c
 // rows of all columns per wavefront (rows of one sector share their segment
 // structure, so the loops stay uniform; only the row's tables differ per
 // lane).  Wavefronts per block: op_waves.
-__host__ __device__ __forceinline__ int64_t op_waves(int64_t nr, int64_t ncol)
+__host__ __device__ __forceinline__ int64_t op_waves(int64_t nr, int64_t ncol, int packmax)
 {
-        return ncol > 32 ? nr * ((ncol + 63) / 64) : (nr + 64 / ncol - 1) / (64 / ncol);
+        return ncol > packmax ? nr * ((ncol + 63) / 64) : (nr + 64 / ncol - 1) / (64 / ncol);
 }
 
 struct OpLane {
@@ -972,10 +972,10 @@ struct OpLane {
         bool valid;
 };
 
-__device__ __forceinline__ OpLane op_lane(int64_t local, int lane, int64_t nr, int64_t ncol)
+__device__ __forceinline__ OpLane op_lane(int64_t local, int lane, int64_t nr, int64_t ncol, int packmax)
 {
         OpLane q;
-        if (ncol > 32) {
+        if (ncol > packmax) {
                 const int64_t nch = (ncol + 63) >> 6;
                 q.row = local / nch;
                 const int64_t b = (local - q.row * nch) * 64 + lane;
@@ -995,7 +995,8 @@ __device__ __forceinline__ OpLane op_lane(int64_t local, int lane, int64_t nr, i
 // Beta-beta part on the transposed vector (block b stored beta-major):
 // wavefronts over (block, beta addresses, alpha addresses) as op_lane.
 __global__ __launch_bounds__(NT) void k_nat_op_bb(NatView v, OpView ob, const int64_t *__restrict__ wbase,
-                                                  const double *__restrict__ ct, double *__restrict__ st)
+                                                  int packmax, const double *__restrict__ ct,
+                                                  double *__restrict__ st)
 {
         const int64_t w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (NT / 64) + (threadIdx.x >> 6))) +
                           (int64_t)0;
@@ -1004,7 +1005,7 @@ __global__ __launch_bounds__(NT) void k_nat_op_bb(NatView v, OpView ob, const in
         const int blk = upper_index(wbase, v.nblock, w);
         const int64_t nb = v.bnb[blk];
         const int64_t na = (v.boff[blk + 1] - v.boff[blk]) / nb;
-        const OpLane q = op_lane(w - wbase[blk], lane, nb, na);
+        const OpLane q = op_lane(w - wbase[blk], lane, nb, na, packmax);
         const int A = v.bsa[blk], B = v.bsb[blk];
         const int ib = ob.dindex[B];
         const double acc = op_row(ob, ib, ob.opbase[ib] + q.row * ob.oplen[ib], v, v.dblk + (int64_t)A * v.nsecB, 1,
@@ -1079,7 +1080,7 @@ __device__ __forceinline__ double op_direct_at(const NatView &v, const OpView &o
 // op_lane: one alpha string per wavefront (uniform loads) where the block
 // has 64 beta strings or more, several otherwise.
 __global__ __launch_bounds__(NT) void k_nat_op_direct(NatView v, OpView oa, OpView ob,
-                                                      const int64_t *__restrict__ wbase,
+                                                      const int64_t *__restrict__ wbase, int packmax,
                                                       const double *__restrict__ hs, const double *__restrict__ c,
                                                       const double *__restrict__ st, double *__restrict__ sigma)
 {
@@ -1091,9 +1092,9 @@ __global__ __launch_bounds__(NT) void k_nat_op_direct(NatView v, OpView oa, OpVi
         const int64_t nb = v.bnb[blk];
         const int64_t na = (v.boff[blk + 1] - v.boff[blk]) / nb;
         const int64_t local = w - wbase[blk];
-        const OpLane q = op_lane(local, lane, na, nb);
+        const OpLane q = op_lane(local, lane, na, nb, packmax);
         double acc;
-        if (nb > 32) {
+        if (nb > packmax) {
                 // one alpha string per wavefront: its address is uniform
                 const int64_t a = __builtin_amdgcn_readfirstlane((int)q.row) + (int64_t)0;
                 acc = op_direct_at(v, oa, ob, blk, a, q.col, nb, hs, c);
@@ -1160,6 +1161,10 @@ struct gas_native_plan {
         int precision = FCIS_PREC_FP64;
         int direct = 0;                 // 0: t1 / GEMM / pull, 1: link-pair loops, 2: operator form
         double *d_opcoef = nullptr;     // operator form: A_alpha then A_beta entries
+        // operator form: several strings per wavefront on blocks of at most
+        // packmax columns (32: measured best of 32 / 16 / 8 / 4 / none on
+        // spaces of 16-62 orbitals)
+        int packmax = 32;
         OpView oa{}, ob{};
         rocblas_handle blas = nullptr;
         void *d_h = nullptr;            // npair x npair, double or float (direct: h, then h + h^T)
@@ -1865,9 +1870,9 @@ int fcis::gas_native_plan_create(gas_native_plan **out, const fcis_gas_space *g,
                 for (int b = 0; b < g->nblock; b++) {
                         const int64_t nas = g->nstr[g->bsa[b]], nbs = g->nstr[g->bsb[b]];
                         // operator form: several strings per wavefront on narrow blocks
-                        wb[b + 1] = wb[b] + (p->direct == 2 ? op_waves(nas, nbs) : nas * ((nbs + 63) / 64));
+                        wb[b + 1] = wb[b] + (p->direct == 2 ? op_waves(nas, nbs, p->packmax) : nas * ((nbs + 63) / 64));
                         wb[g->nblock + 1 + b + 1] = wb[g->nblock + 1 + b] +
-                                                    (p->direct == 2 ? op_waves(nbs, nas) : nbs * ((nas + 63) / 64));
+                                                    (p->direct == 2 ? op_waves(nbs, nas, p->packmax) : nbs * ((nas + 63) / 64));
                 }
                 p->nwave = wb[g->nblock];
                 p->nwave_bb = wb[2 * (size_t)g->nblock + 1];
@@ -2034,11 +2039,11 @@ int fcis::gas_native_plan_execute(gas_native_plan *p, const double *d_ci0, doubl
                         if (p->direct == 2) {
                                 hipLaunchKernelGGL(k_nat_op_bb,
                                                    dim3((unsigned)((p->nwave_bb + NT / 64 - 1) / (NT / 64))), dim3(NT),
-                                                   0, st, g->nat->v, p->ob, wbt, ct, sbb);
+                                                   0, st, g->nat->v, p->ob, wbt, p->packmax, ct, sbb);
                                 hipLaunchKernelGGL(k_nat_op_direct,
                                                    dim3((unsigned)((p->nwave + NT / 64 - 1) / (NT / 64))), dim3(NT), 0,
-                                                   st, g->nat->v, p->oa, p->ob, p->d_wbase, h + np * np, d_ci0, sbb,
-                                                   d_ci1);
+                                                   st, g->nat->v, p->oa, p->ob, p->d_wbase, p->packmax, h + np * np,
+                                                   d_ci0, sbb, d_ci1);
                                 FCIS_CHECK_LAUNCH();
                                 return FCIS_SUCCESS;
                         }

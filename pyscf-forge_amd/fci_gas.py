@@ -457,6 +457,8 @@ class GasContractPlan:
     def path(self):
         """"cas-three-kernel", "cas-fused", "native-t1" (closure GEMM),
         "native-direct" (operator form) or "native-links" (link-pair loops)."""
+        if self._plan is None:
+            raise RuntimeError("Hamiltonian contraction plan is closed")
         k = self.lib.fcis_gas_sigma_plan_path(self._plan)
         return {0: "cas-three-kernel", 1: "cas-fused", 2: "native-t1", 3: "native-direct",
                 4: "native-links"}[k]

@@ -18,9 +18,12 @@ pkg = package()
 
 
 def _free():
+    """Free device memory plus the plan workspaces the library keeps for
+    reuse (a failed allocation releases those, so only their sum is
+    invariant across the failure)."""
     import torch
     torch.cuda.synchronize()
-    return torch.cuda.mem_get_info()[0]
+    return pkg.direct_spin1.available_bytes()
 
 
 def _still_works():
@@ -35,6 +38,7 @@ def _still_works():
 def test_space_too_large_for_the_device_returns_memory_status():
     # 31 orbitals, 15 alpha electrons: 300,540,195 alpha strings x 255 links
     # x 8 B = 613 GB of link table, above the device's memory
+    pkg.direct_spin1.release_plans()
     before = _free()
     with pytest.raises(RuntimeError, match="status -2"):
         pkg.direct_spin1.Space(31, (15, 1))
@@ -47,6 +51,8 @@ def test_sigma_workspace_allocation_failure_frees_the_plan():
     h1e, eri = pkg.synthetic.make_problem(n, ne, seed=1, with_ci=False)
     h2e = pkg.absorb_h1e(h1e, eri, n, ne, 0.5)
     sp = pkg.direct_spin1.Space(n, ne)
+    # no workspace kept from earlier plans: the measurement sees this plan only
+    pkg.direct_spin1.release_plans()
     before = _free()
     # a 10 TB budget asks for one alpha tile of all 48,620 strings: t1 of 3.2 TB
     with pytest.raises(RuntimeError, match="status -2"):
