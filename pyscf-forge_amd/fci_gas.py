@@ -431,11 +431,18 @@ class GasContractPlan:
         self.gas = gas
         self.eri = _pair(eri, gas.norb_total)
         prec = _lib.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        self.precision = prec
         h = ctypes.c_void_p()
         _lib.check(self.lib.fcis_gas_sigma_plan_create(
             ctypes.byref(h), gas.handle, _lib.dptr(self.eri), prec,
             int(workspace_bytes) or _ds._default_workspace()), "fcis_gas_sigma_plan_create")
         self._plan = h
+
+    @property
+    def computes_fp64(self):
+        """True when the product is FP64 whatever the requested precision
+        (an FP64 plan, or the sector-native direct forms)."""
+        return self.precision == _lib.PRECISIONS["fp64"] or self.path in ("native-direct", "native-links")
 
     @property
     def ndet(self):

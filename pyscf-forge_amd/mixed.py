@@ -95,6 +95,7 @@ def solve(solver, sp, h2e, hdiag, nroots, ci0, precision, tol=1e-10, tol_residua
             (2 if precision == "fp64" else 4) * sp.ndet * 8
     ws = int(solver.workspace_bytes) or max(int((free - need) * 0.85), 1 << 26)
     stats = {"precision": precision}
+    verbose_level = verbose
     verbose = bool(verbose and verbose > 4)
 
     if precision == "fp64":
@@ -112,6 +113,22 @@ def solve(solver, sp, h2e, hdiag, nroots, ci0, precision, tol=1e-10, tol_residua
 
     if precision not in LOW_STAGE:
         raise ValueError("unknown precision %r" % (precision,))
+    if basis is not None and hasattr(basis, "make_plan"):
+        # a basis whose low-precision plan computes in FP64 anyway (the
+        # sector-native direct forms): the FP64 solve is the same product
+        # without the acceptance checks
+        probe = basis.make_plan(h2e, precision, max(ws // 3, 1 << 26))
+        exact = probe is not None and bool(getattr(probe, "computes_fp64", False))
+        if probe is not None:
+            probe.close()
+        if exact:
+            e, x, conv, st = solve(solver, sp, h2e, hdiag, nroots, ci0, "fp64", tol=tol,
+                                   tol_residual=tol_residual, max_cycle=max_cycle,
+                                   max_space=max_space, lindep=lindep, level_shift=level_shift,
+                                   verbose=verbose_level, basis=basis, precond=precond,
+                                   follow_state=follow_state)
+            st["precision_requested"] = precision
+            return e, x, conv, st
     lo = LOW_STAGE[precision]
     t0 = time.perf_counter()
     # both plans live through the solve; the FP64 one (twice the walpha

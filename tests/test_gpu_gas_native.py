@@ -229,12 +229,14 @@ def test_native_40_orbitals_rows():
         assert err < 1e-5
 
 
-def test_native_solver_24_orbitals_mixed_and_spin():
+def test_native_solver_24_orbitals_mixed_and_spin(monkeypatch):
     """RAS(4,6,14), 2 holes / 2 particles, (5,5): 315,616 determinants,
-    solved by the default solver (sector-native above 16 orbitals).  The
-    FP32-mixed solve agrees with the FP64 one to 1e-8 Ha; sigma rows of the
-    ground state against the oracle; the RAS space is spin complete, so the
-    ground state's <S^2> from its RDMs is S(S+1)."""
+    solved by the default solver (sector-native above 16 orbitals, operator
+    form).  An FP32-mixed solve through the closure GEMM form (FP32 product,
+    FP64 checks) agrees with the FP64 one to 1e-8 Ha; on the operator form,
+    which computes in FP64, an FP32 request runs the FP64 solve; sigma rows
+    of the ground state against the oracle; the RAS space is spin complete,
+    so the ground state's <S^2> from its RDMs is S(S+1)."""
     orbs, ne = (4, 6, 14), (5, 5)
     restr = dict(max_holes=2, max_particles=2)
     n = 24
@@ -243,12 +245,19 @@ def test_native_solver_24_orbitals_mixed_and_spin():
     e64, c64 = sol.kernel(h1e, eri, n, ne, max_cycle=300)
     print("fp64", sol.stats, flush=True)
     assert sol.converged and sol._space(n, ne).native
+    monkeypatch.setenv("FCIS_GAS_NATIVE_PATH", "t1")
     sol32 = fg.FCISolver(gas_orbs=orbs, gas_restr=restr, gas_restr_type="ras")
     sol32.precision = "fp32"
     e32, c32 = sol32.kernel(h1e, eri, n, ne, max_cycle=300)
     print("fp32", sol32.stats, flush=True)
     assert sol32.converged and abs(e32 - e64) < 1e-8
     assert sol32.stats["nhop_fp64"] >= 1
+    monkeypatch.delenv("FCIS_GAS_NATIVE_PATH")
+    solx = fg.FCISolver(gas_orbs=orbs, gas_restr=restr, gas_restr_type="ras")
+    solx.precision = "fp32"
+    ex, cx = solx.kernel(h1e, eri, n, ne, max_cycle=300)
+    assert solx.stats.get("precision_requested") == "fp32" and "nhop_fp64" not in solx.stats
+    assert solx.converged and abs(ex - e64) < 1e-10
     o, D = ag.normalize_gas_spec(orbs, ne, restr, "ras")
     L = og.GasLayout(o, ne, D)
     sa, sb = L.det_strings()
