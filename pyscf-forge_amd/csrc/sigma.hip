@@ -704,9 +704,13 @@ __global__ __launch_bounds__(256) void k_alpha_pull(int64_t nb, int64_t wrs, int
                         const int64_t I = L.addr;
                         in[u] = I >= a0 && I < a0 + ta;
                         neg[u] = link_neg(L.info);
-                        v[u] = *reinterpret_cast<const T2 *>(
+                        // each walpha element is read once: non-temporal loads
+                        typedef T NT2 __attribute__((ext_vector_type(2)));
+                        const NT2 w = __builtin_nontemporal_load(reinterpret_cast<const NT2 *>(
                                 walpha + (in[u] ? (int64_t)(link_rslot(L.info) - kfirst) * ld + (I - a0) * wrs + Jb
-                                                : Jb));
+                                                : Jb)));
+                        v[u].x = w.x;
+                        v[u].y = w.y;
                 }
 #pragma unroll
                 for (int u = 0; u < U; u++) {
