@@ -104,7 +104,11 @@ def test_rdm_vs_reference(name):
         numpy.testing.assert_allclose(mine, g[key], rtol=0, atol=1e-12)
 
 
-@pytest.mark.parametrize("n,ne,seed", [(4, (2, 2), 1), (5, (3, 2), 2), (6, (3, 3), 3), (6, (2, 1), 4), (5, (5, 0), 5)])
+@pytest.mark.parametrize("n,ne,seed", [(4, (2, 2), 1), (5, (3, 2), 2), (6, (3, 3), 3), (6, (2, 1), 4), (5, (5, 0), 5),
+                                       # degenerate spaces: one orbital, empty and full shells,
+                                       # one spin without electrons
+                                       (1, (1, 0), 6), (1, (1, 1), 7), (2, (1, 1), 8), (4, (0, 0), 9),
+                                       (4, (4, 4), 10), (6, (0, 3), 11), (5, (5, 2), 12), (7, (1, 1), 13)])
 def test_sigma_absorb_vs_brute_force(n, ne, seed):
     h1e, eri, c = syn.make_problem(n, ne, seed=seed)
     H = brute.hamiltonian(h1e, eri, n, ne)
