@@ -75,3 +75,42 @@ def test_diagonal_rdms_energy_degenerate_spaces(case):
         assert abs(e[1] - w[1]) < 1e-6
     ci0 = ci[0] if nroots > 1 else ci
     assert abs(pkg.energy(h1e, eri, ci0, n, ne) - w[0]) < 1e-10
+
+
+# The reference's degenerate GAS spaces (test_gasci.py:469-470): two
+# subspaces of 3 and 2 orbitals with no electrons, and with every orbital
+# doubly occupied.  One determinant each: sigma and the energy are its
+# diagonal element; an orbital rotation within the subspaces multiplies it
+# by det(u) per spin (det(u)^2 = 1), so the vector is unchanged.
+GAS_EDGE = [((0, 0), ((0, 0, 0, 0),)), ((5, 5), ((3, 2, 3, 2),))]
+
+
+@pytest.mark.parametrize("native", [False, True])
+@pytest.mark.parametrize("nelec,restr", GAS_EDGE, ids=["vacuum", "full"])
+def test_gas_degenerate_spaces(nelec, restr, native):
+    print("edge gas", nelec, "native" if native else "embedded", flush=True)
+    fg = pkg.fci_gas
+    n = 5
+    solver = fg.FCISolver(gas_orbs=(3, 2), gas_restr=restr, gas_restr_type="spin-supergroup")
+    solver.native = native
+    h1e, eri, _ = pkg.synthetic.make_problem(n, nelec, seed=5)
+    with solver.make_space(n, nelec) as gas:
+        assert gas.ndet == 1 and gas.native == native
+    c = numpy.ones(1)
+    h2e = solver.absorb_h1e(h1e, eri, n, nelec, 0.5)
+    ref = oracle.contract_2e(oracle.absorb_h1e(h1e, eri, n, nelec, 0.5), c.reshape(1, 1), n, nelec)
+    got = solver.contract_2e(h2e, c, n, nelec)
+    assert abs(float(numpy.asarray(got).reshape(-1)[0]) - float(ref.reshape(-1)[0])) < 1e-13 * max(
+        abs(float(ref.reshape(-1)[0])), 1.0)
+    hd = numpy.asarray(solver.make_hdiag(h1e, eri, n, nelec)).reshape(-1)
+    assert abs(hd[0] - oracle.make_hdiag(h1e, eri, n, nelec).reshape(-1)[0]) < 1e-12
+    e, ci = solver.kernel(h1e, eri, n, nelec)
+    assert abs(e - oracle.lowest_eigenvalues(h1e, eri, n, nelec)[0][0]) < 1e-10
+    rng = numpy.random.default_rng(17)
+    u = numpy.zeros((n, n))
+    u[:3, :3] = numpy.linalg.qr(rng.normal(size=(3, 3)))[0]
+    u[3:, 3:] = numpy.linalg.qr(rng.normal(size=(2, 2)))[0]
+    perm = numpy.eye(n)[:, [1, 0, 2, 3, 4]]
+    for rot in (u, perm):
+        out = solver.transform_ci_within_gas(c, n, nelec, rot)
+        numpy.testing.assert_allclose(numpy.asarray(out).reshape(-1), c, rtol=0, atol=1e-12)
