@@ -176,14 +176,44 @@ __global__ __launch_bounds__(LIST_THREADS) void k_precond_list(const BlockTask *
         }
 }
 
-// Pinned host staging + device table buffer shared by the list calls
-// (grown on demand; calls are serialised per process).
+// Pinned host staging + device table buffers of the list calls: a ring of
+// STAGE_SLOTS per device (grown on demand; calls are serialised per
+// process).  A call packs its tables into the next slot, copies them to the
+// device and launches on the caller's stream; gemm and preconditioner calls
+// then record the slot's event and return without waiting, so that the
+// host prepares the next call while the device runs this one.  A slot is
+// written again only after its event has completed (the copy and the
+// kernel that read it are done).  gram calls wait for their results.
+constexpr int STAGE_SLOTS = 8;
 struct Stage {
         char *host = nullptr, *dev = nullptr;
         size_t cap = 0;
+        hipEvent_t done = nullptr;   // recorded after the slot's last kernel
+        bool pending = false;
 };
-Stage g_stage[16];
+Stage g_stage[16][STAGE_SLOTS];
+int g_next[16];
 std::mutex g_mu;
+
+// The next slot of device dev, its previous use finished.
+int stage_acquire(int dev, Stage **out)
+{
+        Stage &S = g_stage[dev & 15][g_next[dev & 15]];
+        g_next[dev & 15] = (g_next[dev & 15] + 1) % STAGE_SLOTS;
+        if (S.done == nullptr) FCIS_HIP(hipEventCreateWithFlags(&S.done, hipEventDisableTiming));
+        if (S.pending) FCIS_HIP(hipEventSynchronize(S.done));
+        S.pending = false;
+        *out = &S;
+        return FCIS_SUCCESS;
+}
+
+// Mark the slot in use by the work just queued on stream st.
+int stage_release_async(Stage &S, hipStream_t st)
+{
+        FCIS_HIP(hipEventRecord(S.done, st));
+        S.pending = true;
+        return FCIS_SUCCESS;
+}
 
 int stage_reserve(Stage &s, size_t bytes)
 {
@@ -281,7 +311,10 @@ int fcis_gram_list(int nent, const fcis_block_entry *ent, int nxs, const double 
         int dev = 0;
         FCIS_HIP(hipGetDevice(&dev));
         std::lock_guard<std::mutex> lock(g_mu);
-        Stage &S = g_stage[dev & 15];
+        Stage *Sp = nullptr;
+        st = stage_acquire(dev, &Sp);
+        if (st != FCIS_SUCCESS) return st;
+        Stage &S = *Sp;
         Packer pk;
         const size_t o_t = pk.add(tasks.data(), tasks.size() * sizeof(BlockTask));
         const size_t o_e = pk.add(ent, nent * sizeof(fcis_block_entry));
@@ -336,7 +369,10 @@ int fcis_gemm_list(int nent, const fcis_block_entry *ent, int nxs, const double 
         int dev = 0;
         FCIS_HIP(hipGetDevice(&dev));
         std::lock_guard<std::mutex> lock(g_mu);
-        Stage &S = g_stage[dev & 15];
+        Stage *Sp = nullptr;
+        st = stage_acquire(dev, &Sp);
+        if (st != FCIS_SUCCESS) return st;
+        Stage &S = *Sp;
         Packer pk;
         const size_t o_t = pk.add(tasks.data(), tasks.size() * sizeof(BlockTask));
         const size_t o_e = pk.add(ent, nent * sizeof(fcis_block_entry));
@@ -353,8 +389,7 @@ int fcis_gemm_list(int nent, const fcis_block_entry *ent, int nxs, const double 
                            (const double *const *)(S.dev + o_x), (double *const *)(S.dev + o_y),
                            (const double *)(S.dev + o_c));
         FCIS_CHECK_LAUNCH();
-        FCIS_HIP(hipStreamSynchronize(s));
-        return FCIS_SUCCESS;
+        return stage_release_async(S, s);
 }
 
 int fcis_precond_list(int nent, const fcis_precond_entry *ent, double level_shift, void *stream)
@@ -377,11 +412,14 @@ int fcis_precond_list(int nent, const fcis_precond_entry *ent, double level_shif
         int dev = 0;
         FCIS_HIP(hipGetDevice(&dev));
         std::lock_guard<std::mutex> lock(g_mu);
-        Stage &S = g_stage[dev & 15];
+        Stage *Sp = nullptr;
+        int st = stage_acquire(dev, &Sp);
+        if (st != FCIS_SUCCESS) return st;
+        Stage &S = *Sp;
         Packer pk;
         const size_t o_t = pk.add(tasks.data(), tasks.size() * sizeof(BlockTask));
         const size_t o_e = pk.add(ent, nent * sizeof(fcis_precond_entry));
-        int st = stage_reserve(S, pk.total);
+        st = stage_reserve(S, pk.total);
         if (st != FCIS_SUCCESS) return st;
         hipStream_t s = (hipStream_t)stream;
         pk.copy(S.host);
@@ -390,8 +428,7 @@ int fcis_precond_list(int nent, const fcis_precond_entry *ent, double level_shif
                            (const BlockTask *)(S.dev + o_t), (const fcis_precond_entry *)(S.dev + o_e),
                            level_shift);
         FCIS_CHECK_LAUNCH();
-        FCIS_HIP(hipStreamSynchronize(s));
-        return FCIS_SUCCESS;
+        return stage_release_async(S, s);
 }
 
 }  // extern "C"
