@@ -101,7 +101,7 @@ class DeviceOps:
 def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
              max_cycle=50, max_space=12, lindep=1e-14, verbose=False,
              callback=None, ops=None, hop_check=None, tol_low=None,
-             tol_residual_low=None, check_every=6, follow_state=False):
+             tol_residual_low=None, check_every=6, check_max=20, follow_state=False):
     """Lowest ``nroots`` eigenpairs.  x0: list of device vectors (guesses).
 
     Returns DavidsonResult with e (numpy), x / ax (device tensors of shape
@@ -121,8 +121,11 @@ def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
     Mixed precision: with ``hop_check`` (the exact operator; ``hop`` is then
     an approximate, cheaper one) every new subspace vector is applied with
     ``hop``.  The first stage runs to (tol_low, tol_residual_low).  Then, and
-    whenever the iterations meet (tol, tol_residual) or ``check_every``
-    iterations have passed, the Ritz vectors are applied with ``hop_check``:
+    whenever the iterations meet (tol, tol_residual), or at least
+    ``check_every`` iterations have passed and the largest residual fell
+    less than twofold over the last three (a low-precision plateau), or
+    ``check_max`` iterations have passed, the Ritz vectors are applied with
+    ``hop_check``:
     they are accepted if their exact residuals and the change of their exact
     Rayleigh quotients from the iteration's Ritz values meet (tol,
     tol_residual); otherwise the subspace restarts from them with their exact
@@ -295,6 +298,7 @@ def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
     conv = numpy.zeros(nroots, dtype=bool)
     rnorm = numpy.full(nroots, numpy.inf)
     rmax_prev = numpy.inf
+    rhist = []       # largest residual of each iteration since the last check
     nrestore = 0
     cycles = 0
     for icyc in range(max_cycle):
@@ -329,7 +333,12 @@ def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
         rmax_prev = rmax
         conv = (numpy.abs(de) < stage_tol) & (rnorm < stage_res)
         checked = False
-        if mixed and (conv.all() or (refining and icyc - last_check >= check_every)
+        since = icyc - last_check
+        stalled = (refining and since >= check_every and len(rhist) >= 3
+                   and rmax > 0.5 * rhist[-3])
+        if refining:
+            rhist.append(rmax)
+        if mixed and (conv.all() or stalled or (refining and since >= check_max)
                       or (not refining and icyc + 1 >= max_cycle // 2)):
             # exact images of the Ritz vectors: the acceptance test, and the
             # anchor of the subspace from here on
@@ -344,6 +353,7 @@ def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
             e = e_chk
             refining, checked, last_check = True, True, icyc
             stage_tol, stage_res = tol, tol_residual
+            rhist = [float(rnorm.max())]
         new_vectors = []
         for k in range(nroots):
             if not conv[k]:
