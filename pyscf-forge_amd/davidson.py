@@ -123,7 +123,8 @@ def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
     ``hop``.  The first stage runs to (tol_low, tol_residual_low).  Then, and
     whenever the iterations meet (tol, tol_residual), or at least
     ``check_every`` iterations have passed and the largest residual fell
-    less than twofold over the last three (a low-precision plateau), or
+    by less than a quarter over the last three (a low-precision plateau;
+    a slow but steady decrease is not one), or
     ``check_max`` iterations have passed, the Ritz vectors are applied with
     ``hop_check``:
     they are accepted if their exact residuals and the change of their exact
@@ -335,7 +336,7 @@ def davidson(hop, precond, x0, nroots=1, tol=1e-10, tol_residual=None,
         checked = False
         since = icyc - last_check
         stalled = (refining and since >= check_every and len(rhist) >= 3
-                   and rmax > 0.5 * rhist[-3])
+                   and rmax > 0.8 * rhist[-3])
         if refining:
             rhist.append(rmax)
         if mixed and (conv.all() or stalled or (refining and since >= check_max)
