@@ -376,6 +376,12 @@ typedef struct {
         int64_t n;
         double e;
 } fcis_precond_entry;
+/* Ordering: every list call runs on `stream` after the work queued there
+ * before it.  fcis_gram_list returns when its results are in `out`;
+ * fcis_gemm_list and fcis_precond_list return once their work is queued
+ * (the host arrays they take are copied before they return), so the caller
+ * reads their outputs through work on the same stream or after
+ * synchronising it. */
 /* out[ooff + i*ny + j] = <X_i, Y_j> (out: HOST array of nout; partial sums
  * added in a fixed order). */
 int fcis_gram_list(int nent, const fcis_block_entry *ent, int nxs, const double *const *xs,
