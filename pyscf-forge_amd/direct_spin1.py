@@ -543,6 +543,9 @@ def lowest_addresses(hd, count):
     count = int(count)
     if count >= n:
         return numpy.arange(n, dtype=numpy.int64)
+    if count == 1:
+        # one reduction: argmin returns the first (lowest-address) minimum
+        return numpy.array([int(torch.argmin(hd))], dtype=numpy.int64)
     # the count-th lowest value, then every entry at or below it (ties at
     # the boundary included) ordered by (value, address) on the host
     thr = torch.topk(hd, count, largest=False, sorted=True).values[-1]

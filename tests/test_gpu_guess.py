@@ -99,3 +99,18 @@ def test_precond_matches_reference_rule():
     out2 = torch.empty_like(dxd)
     pkg.batch.BlockOps().precond([(hdd.data_ptr(), dxd.data_ptr(), out2.data_ptr(), n, e)], ls)
     numpy.testing.assert_array_equal(out2.cpu().numpy(), want)
+
+
+def test_lowest_address_single_root_ties():
+    """One root: the lowest value's lowest address (the rule the counts above
+    one follow), also when the minimum is tied."""
+    import torch
+    rng = numpy.random.default_rng(3)
+    hd = rng.normal(size=5000)
+    hd[3700] = hd[1300] = hd.min() - 1.0
+    d = torch.from_numpy(hd).cuda()
+    assert pkg.direct_spin1.lowest_addresses(d, 1).tolist() == [1300]
+    assert pkg.direct_spin1.lowest_addresses(d, 2).tolist() == [1300, 3700]
+    hd[1300] += 0.5
+    d = torch.from_numpy(hd).cuda()
+    assert pkg.direct_spin1.lowest_addresses(d, 1).tolist() == [3700]
