@@ -202,7 +202,10 @@ def solve_batch(problems, nroots=4, tol=1e-10, tol_residual=None, max_cycle=100,
     torch.cuda.synchronize()
     t_tune = time.perf_counter()
     if tune:
-        bs.tune(tin, tout)
+        # one timed launch per candidate schedule: a launch with every slot
+        # active takes ~24 ms at the 64-problem workload, long enough that one
+        # timing ranks the candidates (a second repetition cost 0.14 s per solve)
+        bs.tune(tin, tout, reps=1)
     bs.set_graph(use_graph)
     t_tune = time.perf_counter() - t_tune
     t_setup = time.perf_counter() - t_setup
