@@ -828,7 +828,7 @@ class FCISolver(_ds.FCISolver):
         PSPACE_MAX_DET determinants."""
         import torch
         npsp = min(int(npsp), t.ncsf_sym)
-        order = torch.topk(hdiag_csf, npsp, largest=False).indices
+        order = _ds.lowest_k(hdiag_csf, npsp)
         cnt = self._support_sizes(t, order)
         keep = int(numpy.searchsorted(numpy.cumsum(cnt), self.PSPACE_MAX_DET, side="right"))
         keep = max(1, keep)

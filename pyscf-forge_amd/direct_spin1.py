@@ -380,7 +380,7 @@ def pspace(h1e, eri, norb, nelec, hdiag=None, np=400):
     if hd.numel() != sp.ndet:
         raise ValueError("hdiag size does not match the determinant count")
     m = min(int(np), sp.ndet)
-    addr = torch.topk(hd, m, largest=False).indices
+    addr = lowest_k(hd, m)
     h0 = _pspace_dev(sp, h1e, eri, addr)
     return addr.cpu().numpy(), h0.cpu().numpy()
 
@@ -528,6 +528,31 @@ def spin_square_from_rdm12s(dm1s, dm2s, nelec):
 
 
 # ---------------------------------------------------------------- solver
+
+LOWEST_K_CHUNK = 16384
+
+
+def lowest_k(hd, k):
+    """Indices of the k lowest entries of a 1-D tensor, by ascending value
+    (torch.topk(hd, k, largest=False).indices).  On long vectors: the k lowest
+    entries lie in the k chunks of LOWEST_K_CHUNK entries with the lowest
+    minima (the k-th lowest chunk minimum bounds the k-th lowest entry), so
+    one min-reduction over the vector and a topk over k chunks (plus the
+    tail) replace a selection over all of it."""
+    import torch
+    n, k = hd.numel(), int(k)
+    C = LOWEST_K_CHUNK
+    m = n // C
+    if k < 1 or m < 4 * k:
+        return torch.topk(hd, k, largest=False).indices
+    body = hd[:m * C].view(m, C)
+    rows = torch.topk(body.amin(dim=1), k, largest=False).indices
+    cols = torch.arange(C, device=hd.device, dtype=rows.dtype)
+    cidx = torch.cat([(rows[:, None] * C + cols[None, :]).reshape(-1),
+                      torch.arange(m * C, n, device=hd.device, dtype=rows.dtype)])
+    sel = torch.topk(hd[cidx], k, largest=False).indices
+    return cidx[sel]
+
 
 def lowest_addresses(hd, count):
     """Addresses of the ``count`` lowest entries of a device diagonal, in the
@@ -703,7 +728,7 @@ class FCISolver:
             precond = None
             npsp = min(int(pspace_size), sp.ndet)
             if npsp > 0 and sp.ndet > PSPACE_MATVEC_MAX:
-                addr = torch.topk(hdiag, npsp, largest=False).indices
+                addr = lowest_k(hdiag, npsp)
                 pw, pv = numpy.linalg.eigh(_pspace_dev(sp, h1e, eri, addr).cpu().numpy())
                 precond = make_pspace_precond(hdiag, pw, pv, addr, level_shift)
             e, c, conv, stats = solve(self, sp, h2e, hdiag, nroots, ci0, precision,
